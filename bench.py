@@ -1,0 +1,137 @@
+"""Benchmark: Mpaths/s of the book-2 final scene at 1080x1080, 1024 spp (32x32 strata), depth 40, FP32, on one GPU.
+
+A step is one whole render of that image (1024 passes of 1080x1080 paths, 1.19 G paths) with the scene and the
+image's double sums resident in device memory. `value` is paths per second over the K timed steps (wall clock around
+them; every launch is waited for). Also reported:
+  - roofline: the dominant kernel's algorithmic bytes per launch (bytes per path, counted by the instrumented kernel
+    over 16 passes of the same image, times the paths of one launch) over its average launch time (HIP events on the
+    session's own stream, over the timed region), against 8 TB/s of HBM;
+  - cpu_baseline: the CPU restatement of the reference renderer (double, mt19937_64 per row, the reference's
+    algorithms) timed on this box's host cores on a bounded sample: the same scene at 1080x1080 with 16 spp;
+  - megakernel: the plain FP32 megakernel's Mpaths/s on the same workload, beside the integrator measured.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--integrator auto|megakernel|wavefront] [--spp N] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def measure(rt, scene, integrator, steps, warmup, seed=0):
+    """(Mpaths/s, ms per step, average launch seconds, launches per step, session) over `steps` timed renders"""
+    passes = scene.passes
+    paths_per_step = scene.width * scene.height * passes
+    s = rt.Session(scene, seed=seed, integrator=integrator, launch_seconds=0.1)
+    for _ in range(warmup):
+        s.set_camera(scene.desc.camera)
+        s.render_to(passes)
+    launches = 0
+    gpu_seconds = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.set_camera(scene.desc.camera)
+        r = s.render_to(passes)
+        launches += r.launches
+        gpu_seconds += r.gpu_seconds
+    elapsed = time.perf_counter() - t0
+    value = paths_per_step * steps / elapsed / 1e6
+    return value, 1000 * elapsed / steps, gpu_seconds / max(launches, 1), launches / steps, s
+
+
+def cpu_baseline(width: int, spp: int):
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    img, rays = oracle.render("final", width=width, spp=spp, seed=0, threads=threads, earth_width=2048)
+    seconds = time.perf_counter() - t0
+    paths = img.shape[0] * img.shape[1] * spp
+    return {"value": round(paths / seconds / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"final {img.shape[1]}x{img.shape[0]}, {spp} spp, depth 40, double, {seconds:.1f} s, "
+                      f"{rays / paths:.2f} rays/path"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--integrator", default="auto", choices=["auto", "megakernel", "wavefront"])
+    ap.add_argument("--spp", type=int, default=1024)
+    ap.add_argument("--width", type=int, default=1080)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-megakernel", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} asked)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import raytracing_amd as rt
+
+    scene = rt.build_scene("final", width=args.width, samples=args.spp)
+    print(f"final {scene.width}x{scene.height}, {scene.passes} spp; device {rt.query_device().name.decode()}",
+          file=sys.stderr, flush=True)
+    value, ms, avg_launch, launches_per_step, session = measure(rt, scene, args.integrator, args.steps, args.warmup)
+    kernel = session.kernel()
+    print(f"{args.integrator}: {value:.1f} Mpaths/s, {ms:.1f} ms/step, {launches_per_step:.1f} launches/step, "
+          f"kernel {kernel.name.decode()}", file=sys.stderr, flush=True)
+
+    # algorithmic bytes per path, counted over 16 passes of the same image by the instrumented megakernel
+    traffic = session.traffic(16)
+    bytes_per_path = traffic.bytes_per_path
+    paths_per_launch = scene.width * scene.height * scene.passes / launches_per_step
+    achieved = bytes_per_path * paths_per_launch / avg_launch / 1e9
+    events = {name: int(traffic.events[i]) for i, name in enumerate(rt.TRAFFIC_EVENTS)}
+    print(f"traffic: {bytes_per_path:.0f} B/path, {traffic.rays / traffic.paths:.2f} rays/path, {events}",
+          file=sys.stderr, flush=True)
+    session.close()
+
+    mega = None
+    if args.integrator != "megakernel" and not args.no_megakernel:
+        mv, mms, _, _, ms_ = measure(rt, scene, "megakernel", 1, 0)
+        ms_.close()
+        mega = round(mv, 2)
+        print(f"megakernel: {mv:.1f} Mpaths/s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(1080, 16)
+        print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "Mpaths/s (1080^2 final scene)",
+        "value": round(value, 2),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"final {scene.width}x{scene.height}, {scene.passes} spp (32x32 strata), depth 40, "
+                               f"FP32, mixture sampling, seeded 2048x1024 noise texture",
+                   "integrator": args.integrator, "kernel": kernel.name.decode(),
+                   "paths_per_step": scene.width * scene.height * scene.passes,
+                   "rays_per_path": round(traffic.rays / traffic.paths, 3),
+                   "megakernel_mpaths_s": mega},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_path": round(bytes_per_path, 1), "avg_launch_ms": round(1000 * avg_launch, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,484 @@
+/*
+ * raytracing_amd.h: the C ABI of the MI355X path tracer.
+ *
+ * The path tracer is reached through plain C: POD structs in double precision, pointers and sizes, integer status
+ * codes and a thread-local error message. No C++ or torch type crosses this boundary, so any host (C, C++, Python
+ * ctypes, a Go/Java/Node binding) can call it.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to DaniilK19/RayTracing):
+ *   - the scene tables mirror gpu::scene_desc and its parts      src/gpu/renderer.h:33-395
+ *   - rt_scene_build mirrors build_gpu_scene                      src/gpu_scene.ixx:1333-1445
+ *   - rt_render mirrors gpu::render                               src/gpu/renderer.h:526
+ *   - rt_session_* mirror class gpu::session                      src/gpu/renderer.h:585-673
+ *
+ * Errors. The reference throws std::runtime_error (GPU and scene errors), std::invalid_argument (options out of range)
+ * and std::logic_error (state misuse), renderer.h:513-526 and :588-601. Here each maps to a status code, and
+ * rt_last_error() returns the message of the last failure on the calling thread:
+ *   RT_OK 0, RT_ERROR_RUNTIME 1, RT_ERROR_INVALID_ARGUMENT 2, RT_ERROR_LOGIC 3.
+ * Every check that can fail without the GPU is made before any device memory is touched.
+ *
+ * Threading. A session belongs to one host thread at a time (renderer.h, session: no reentrancy). Different sessions
+ * may be used from different threads; each has its own HIP stream.
+ */
+#ifndef RAYTRACING_AMD_H
+#define RAYTRACING_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_ABI_VERSION 1
+
+enum rt_status
+{
+    RT_OK = 0,
+    RT_ERROR_RUNTIME = 1,
+    RT_ERROR_INVALID_ARGUMENT = 2,
+    RT_ERROR_LOGIC = 3,
+};
+
+/* A point, direction or color in double precision: gpu::vec3d, src/gpu/renderer.h:33-41 */
+typedef struct rt_vec3
+{
+    double x, y, z;
+} rt_vec3;
+
+/* --- Textures: gpu::texture_desc, image_desc, perlin_desc (src/gpu/renderer.h:45-90) ----------------------------- */
+
+enum rt_texture_kind
+{
+    RT_TEXTURE_SOLID = 0,
+    RT_TEXTURE_CHECKER = 1,
+    RT_TEXTURE_IMAGE = 2,
+    RT_TEXTURE_NOISE = 3,
+};
+
+typedef struct rt_texture_desc
+{
+    int32_t kind;      /* rt_texture_kind */
+    int32_t even;      /* checker: index of the texture of even cubes (smaller than this texture's own index) */
+    int32_t odd;       /* checker: index of the texture of odd cubes */
+    int32_t image;     /* image: index into rt_scene.images */
+    int32_t perlin;    /* noise: index into rt_scene.perlins */
+    int32_t reserved;
+    rt_vec3 rgb;       /* solid: the color */
+    double inv_scale;  /* checker: 1 / the side of its cubes */
+    double scale;      /* noise: the marble's stripe frequency along z */
+} rt_texture_desc;
+
+/* Linear colors, rows from the top: pixel (x, y) starts at rgb[3 * (y * width + x)] */
+typedef struct rt_image_desc
+{
+    int32_t width;
+    int32_t height;
+    const float* rgb;
+} rt_image_desc;
+
+#define RT_PERLIN_POINT_COUNT 256
+
+typedef struct rt_perlin_desc
+{
+    rt_vec3 random_vectors[RT_PERLIN_POINT_COUNT];
+    int32_t perm_x[RT_PERLIN_POINT_COUNT];
+    int32_t perm_y[RT_PERLIN_POINT_COUNT];
+    int32_t perm_z[RT_PERLIN_POINT_COUNT];
+} rt_perlin_desc;
+
+/* --- Materials: gpu::material_desc (src/gpu/renderer.h:94-111) --------------------------------------------------- */
+
+enum rt_material_kind
+{
+    RT_MATERIAL_LAMBERTIAN = 0,
+    RT_MATERIAL_METAL = 1,
+    RT_MATERIAL_DIELECTRIC = 2,
+    RT_MATERIAL_DIFFUSE_LIGHT = 3,
+    RT_MATERIAL_ISOTROPIC = 4,
+};
+
+typedef struct rt_material_desc
+{
+    int32_t kind;            /* rt_material_kind */
+    int32_t texture;         /* lambertian, diffuse_light, isotropic: index into rt_scene.textures */
+    rt_vec3 albedo;          /* metal */
+    double fuzz;             /* metal, at most 1 */
+    double refraction_index; /* dielectric */
+} rt_material_desc;
+
+/* --- Shapes in world coordinates: gpu::sphere_desc, quad_desc (src/gpu/renderer.h:115-139) ----------------------- */
+
+typedef struct rt_sphere_desc
+{
+    rt_vec3 center;       /* at time 0 */
+    double radius;
+    rt_vec3 displacement; /* the center moves by this much from time 0 to 1; exactly 0 for a sphere standing still */
+    double uv_cos;        /* the turn of rotate_y instances baked into it, for its texture coordinates */
+    double uv_sin;
+    int32_t material;
+    int32_t reserved;
+} rt_sphere_desc;
+
+typedef struct rt_quad_desc
+{
+    rt_vec3 q;
+    rt_vec3 u;
+    rt_vec3 v;
+    int32_t material;
+    int32_t reserved;
+} rt_quad_desc;
+
+/* --- Smoke and fog: gpu::medium_desc (src/gpu/renderer.h:146-161) ------------------------------------------------- */
+
+enum rt_boundary_shape
+{
+    RT_BOUNDARY_SPHERE = 0,
+    RT_BOUNDARY_CONVEX_QUADS = 1,
+};
+
+typedef struct rt_medium_desc
+{
+    int32_t boundary;       /* rt_boundary_shape */
+    int32_t material;       /* an isotropic material */
+    double neg_inv_density; /* -1 / density */
+    rt_sphere_desc sphere;  /* RT_BOUNDARY_SPHERE */
+    const rt_quad_desc* quads; /* RT_BOUNDARY_CONVEX_QUADS */
+    int64_t quad_count;
+} rt_medium_desc;
+
+/* --- The lights list the lights/mixture strategies aim at: gpu::light_desc (src/gpu/renderer.h:172-191) --------- */
+
+enum rt_light_shape
+{
+    RT_LIGHT_QUAD = 0,
+    RT_LIGHT_SPHERE = 1,
+};
+
+typedef struct rt_light_desc
+{
+    int32_t kind; /* rt_light_shape */
+    int32_t reserved;
+    rt_quad_desc quad;
+    rt_vec3 center;
+    double radius;
+    double weight; /* the chance of aiming at it, in (0, 1] */
+} rt_light_desc;
+
+/* --- The camera: gpu::camera_desc (src/gpu/renderer.h:195-232) ---------------------------------------------------- */
+
+enum rt_sampling_strategy
+{
+    RT_SAMPLING_UNIFORM = 0,
+    RT_SAMPLING_MATERIAL = 1,
+    RT_SAMPLING_LIGHTS = 2,
+    RT_SAMPLING_MIXTURE = 3,
+};
+
+typedef struct rt_camera_desc
+{
+    int32_t image_width;
+    int32_t image_height;
+    int32_t sqrt_spp;  /* the side of a pixel's grid of strata: a round is sqrt_spp^2 passes */
+    int32_t max_depth; /* at most this many rays per path */
+    rt_vec3 center;
+    rt_vec3 pixel00_loc;
+    rt_vec3 pixel_delta_u;
+    rt_vec3 pixel_delta_v;
+    int32_t defocus;   /* rays start on the lens disk */
+    int32_t sampling;  /* rt_sampling_strategy */
+    rt_vec3 defocus_disk_u;
+    rt_vec3 defocus_disk_v;
+    rt_vec3 sky_bottom;
+    rt_vec3 sky_top;
+} rt_camera_desc;
+
+/* --- Beyond the books: light combination, GGX side entries, environment (src/gpu/renderer.h:240-276) ------------- */
+
+enum rt_light_combination
+{
+    RT_COMBINE_BSDF_ONLY = 0,       /* -p material */
+    RT_COMBINE_LIGHTS_ONLY = 1,     /* -p nee */
+    RT_COMBINE_POWER_HEURISTIC = 2, /* -p mis */
+};
+
+enum rt_microfacet_kind
+{
+    RT_MICROFACET_NONE = 0,
+    RT_MICROFACET_CONDUCTOR = 1,
+    RT_MICROFACET_GLOSS = 2,
+};
+
+typedef struct rt_microfacet_desc
+{
+    int32_t kind; /* rt_microfacet_kind */
+    int32_t reserved;
+    double roughness; /* r in [0, 1]; the GGX width is r^2 */
+} rt_microfacet_desc;
+
+typedef struct rt_environment_desc
+{
+    rt_image_desc image; /* equirectangular, +y up, row 0 is +y; linear radiance */
+    double yaw_degrees;
+    double scale;
+} rt_environment_desc;
+
+/* --- Triangle meshes (beyond the reference, which renders triangles only through OptiX) -------------------------- */
+
+typedef struct rt_mesh_desc
+{
+    const float* positions;   /* 3 per vertex */
+    int64_t vertex_count;
+    const uint32_t* indices;  /* 3 per triangle, counter-clockwise seen from the front */
+    int64_t triangle_count;
+    int32_t material;         /* index into rt_scene.materials */
+    int32_t reserved;
+} rt_mesh_desc;
+
+typedef struct rt_mesh_instance_desc
+{
+    int32_t mesh;
+    int32_t reserved;
+    double object_to_world[12]; /* row-major 3x4 */
+} rt_mesh_instance_desc;
+
+/* --- The scene: gpu::scene_desc (src/gpu/renderer.h:381-395) ----------------------------------------------------- */
+
+typedef struct rt_scene
+{
+    const rt_sphere_desc* spheres;
+    int64_t sphere_count;
+    const rt_material_desc* materials;
+    int64_t material_count;
+    rt_camera_desc camera;
+    const rt_texture_desc* textures;
+    int64_t texture_count;
+    const rt_image_desc* images;
+    int64_t image_count;
+    const rt_perlin_desc* perlins;
+    int64_t perlin_count;
+    const rt_quad_desc* quads;
+    int64_t quad_count;
+    const rt_medium_desc* media;
+    int64_t medium_count;
+    const rt_light_desc* lights;
+    int64_t light_count;
+    /* quality_desc */
+    int32_t combination; /* rt_light_combination */
+    int32_t reserved;
+    const rt_microfacet_desc* microfacets; /* empty, or one per material */
+    int64_t microfacet_count;
+    const rt_environment_desc* environment; /* NULL: the camera's sky */
+    /* triangles */
+    const rt_mesh_desc* meshes;
+    int64_t mesh_count;
+    const rt_mesh_instance_desc* instances;
+    int64_t instance_count;
+} rt_scene;
+
+/* --- Built-in scenes: build_gpu_scene (src/gpu_scene.ixx:1303-1445) ----------------------------------------------
+ * The scenes of the books (materials, weekend, bouncing, checker, earth, perlin, quads, light, cornell, smoke, final,
+ * aluminum, glass) and the test scenes (furnace), built by this library's own C++ with the reference's mt19937_64 stream
+ * from its default seed, then flattened as gpu_scene.ixx does. The earth's image is a seeded noise image (the reference
+ * loads assets/earthmap.jpg): see rt_scene_overrides.earth_width. */
+
+typedef struct rt_scene_overrides
+{
+    int32_t image_width;       /* 0: the scene's own */
+    int32_t image_height;      /* 0: from the scene's aspect ratio */
+    int32_t samples_per_pixel; /* 0: the scene's own */
+    int32_t strata;            /* 0: samples rounded to one k x k grid; > 0: rounds of strata x strata */
+    int32_t sampling;          /* -1: the scene's own; else rt_sampling_strategy, or 4 nee, 5 mis */
+    int32_t max_depth;         /* 0: the scene's own */
+    int32_t earth_width;       /* the noise image standing in for earthmap.jpg: 0 means 2048 (x 1024) */
+    uint32_t earth_seed;       /* its generator's seed */
+} rt_scene_overrides;
+
+typedef struct rt_scene_handle rt_scene_handle;
+
+int rt_scene_build(const char* name, const rt_scene_overrides* overrides, rt_scene_handle** out);
+const rt_scene* rt_scene_get(const rt_scene_handle* handle);
+int64_t rt_scene_passes(const rt_scene_handle* handle); /* sqrt_spp^2, or the samples asked for with strata */
+void rt_scene_free(rt_scene_handle* handle);
+/* The names of the built-in scenes, separated by commas */
+const char* rt_scene_names(void);
+
+/* --- Rendering: gpu::render_options, session_options (src/gpu/renderer.h:405-454, 530-543) --------------------- */
+
+enum rt_precision
+{
+    RT_FLOAT32 = 0,
+    RT_FLOAT64 = 1,
+};
+
+enum rt_integrator
+{
+    RT_INTEGRATOR_AUTO = 0,       /* the wavefront integrator where it is available */
+    RT_INTEGRATOR_MEGAKERNEL = 1, /* one thread per pixel, the whole path in one kernel */
+    RT_INTEGRATOR_WAVEFRONT = 2,  /* per-bounce stages over queues of paths */
+};
+
+typedef struct rt_session_options
+{
+    uint64_t seed;
+    int32_t precision;      /* rt_precision */
+    int32_t integrator;     /* rt_integrator */
+    double launch_seconds;  /* how long one launch is planned to take: (0, 0.5]; 0 means 0.05 */
+    int32_t block_size;     /* megakernel threads per block: 0 for the tuned default */
+    int32_t wavefront_slots;/* wavefront path slots: 0 for the tuned default */
+} rt_session_options;
+
+typedef struct rt_render_report
+{
+    int64_t passes;              /* state after the call */
+    int32_t rows;
+    int32_t launches;
+    double gpu_seconds;
+    double longest_launch_seconds;
+} rt_render_report;
+
+typedef struct rt_kernel_info
+{
+    char name[32];
+    int32_t registers;       /* VGPRs per thread (architected + accumulation) */
+    int32_t local_bytes;     /* private (scratch) bytes per thread */
+    int32_t lds_bytes;       /* LDS per block */
+    int32_t threads_per_block;
+    int32_t blocks_per_cu;   /* as many as fit at once */
+    int32_t compute_units;
+} rt_kernel_info;
+
+typedef struct rt_session rt_session;
+
+/* gpu::session::session: validates, builds the BVH, uploads; the view is scene->camera */
+int rt_session_create(const rt_scene* scene, const rt_session_options* options, rt_session** out);
+void rt_session_destroy(rt_session* session);
+/* gpu::session::set_camera: a new view, starting from no samples */
+int rt_session_set_camera(rt_session* session, const rt_camera_desc* camera);
+/* gpu::session::render_to: render until the image holds `passes` complete passes (rows must be 0) */
+int rt_session_render_to(rt_session* session, int64_t passes, int32_t rows, rt_render_report* report);
+/* gpu::session::render_for: about `seconds` of GPU time, never past pass_limit */
+int rt_session_render_for(rt_session* session, double seconds, int64_t pass_limit, rt_render_report* report);
+int rt_session_state(const rt_session* session, int64_t* passes, int32_t* rows);
+/* gpu::session::pixels: sums / passes, W*H*3 doubles, rows from the top */
+int rt_session_pixels(rt_session* session, double* out, size_t count);
+/* The raw per-pixel double sums, W*H*3 (for exact checks of accumulation order) */
+int rt_session_sums(rt_session* session, double* out, size_t count);
+/* gpu::session::display: the books' bytes (sqrt, clamp 0.999, x256), RGBA, rows from the top */
+int rt_session_display_rgba8(rt_session* session, uint8_t* out, size_t count);
+/* gpu::session::rays, shadow_rays, kernel_seconds */
+int rt_session_counters(const rt_session* session, uint64_t* rays, uint64_t* shadow_rays, double* kernel_seconds);
+/* gpu::session::kernel */
+int rt_session_kernel_info(const rt_session* session, rt_kernel_info* info);
+
+/* gpu::render: a session rendered to `passes` (0: sqrt_spp^2), its pixels copied to `pixels` (W*H*3 doubles) */
+typedef struct rt_render_options
+{
+    rt_session_options session;
+    int64_t passes;
+} rt_render_options;
+
+typedef struct rt_render_stats
+{
+    double seconds;
+    double kernel_seconds;
+    uint64_t rays;
+    uint64_t shadow_rays;
+} rt_render_stats;
+
+int rt_render(const rt_scene* scene, const rt_render_options* options, double* pixels, size_t count,
+              rt_render_stats* stats);
+
+/* --- Parity probes (the reference's GpuTests drive closest_hit through test kernels, tests/test_scene.h) -------- */
+
+/* The session's BVH traversal on a batch of rays given on the host: origins and directions (3 floats each) and times,
+ * with each ray's `from` primitive and `inward` flag (src/gpu/bvh.cuh:576-706). Writes the closest primitive's global
+ * id (-1 for none), t, and whether it hit the front. Float32 sessions only. */
+int rt_session_closest_hit(rt_session* session, const float* origins, const float* directions, const float* times,
+                           const int32_t* from, const uint8_t* inward, int64_t count, int32_t* index, float* t,
+                           uint8_t* entering);
+
+/* The flat BVH this library builds for a scene in float32, on the host (no GPU needed), in the reference's depth-first
+ * node order (src/gpu/bvh.cuh:29-38,382-435): per node box_min[3], box_max[3] as floats, and index, count as ints;
+ * and the global ids of the scene's spheres and quads. Call with NULL buffers to learn the counts. */
+int rt_scene_bvh(const rt_scene* scene, float* node_boxes, int32_t* node_ints, int64_t* node_count,
+                 int32_t* sphere_ids, int32_t* quad_ids, int32_t* depth);
+
+/* The scene data the view's paths read, counted by an instrumented instance of the megakernel over passes
+ * [0, passes) of the view's first round, into scratch sums (the view's image and counters don't change). events[k] counts
+ * reads of one kind of record (in order: wide BVH nodes, spheres, moving spheres, quads, media, media boundary quads,
+ * hits, sphere frames, textures, texels, Perlin gradients, lights) and event_bytes[k] is that record's size; bytes_per_path
+ * is their weighted sum per path, plus the double sums' read and write of each pixel once per pass. This is the
+ * algorithmic traffic the roofline is measured with. No counterpart in the reference. */
+#define RT_TRAFFIC_EVENTS 12
+typedef struct rt_traffic
+{
+    uint64_t paths;
+    uint64_t rays;
+    uint64_t events[16];
+    double event_bytes[16];
+    double bytes_per_path;
+} rt_traffic;
+
+int rt_session_traffic(rt_session* session, int64_t passes, rt_traffic* traffic);
+
+/* --- Serving: progressive renders of mixed requests on one device (config 4) ------------------------------------- */
+
+typedef struct rt_server rt_server;
+
+typedef struct rt_server_options
+{
+    uint64_t memory_budget_bytes; /* device memory the server may hold for requests (0: 16 GiB) */
+    int32_t max_active;           /* requests rendering at once (0: 8) */
+    int32_t passes_per_step;      /* passes a request gets per scheduling step (0: 4) */
+} rt_server_options;
+
+typedef struct rt_request_status
+{
+    int64_t id;
+    int32_t state;         /* 0 queued, 1 active, 2 done, 3 failed */
+    int32_t reserved;
+    int64_t passes_done;
+    int64_t passes_total;
+    int64_t milestone;     /* the largest power-of-two pass count reached so far */
+    double submit_time;    /* seconds on the server's clock */
+    double start_time;
+    double milestone_time; /* when `milestone` was reached */
+    double finish_time;
+} rt_request_status;
+
+int rt_server_create(const rt_server_options* options, rt_server** out);
+void rt_server_destroy(rt_server* server);
+/* Admits a request: a scene (copied), its seed and its passes. Returns its id in *id. */
+int rt_server_submit(rt_server* server, const rt_scene* scene, uint64_t seed, int64_t passes, int64_t* id);
+/* One scheduling step: admits queued requests as memory and slots free up, and renders a step of every active one */
+int rt_server_step(rt_server* server, int32_t* active);
+int rt_server_status(const rt_server* server, int64_t id, rt_request_status* status);
+/* The image of a request's complete passes so far (W*H*3 doubles) */
+int rt_server_pixels(rt_server* server, int64_t id, double* out, size_t count);
+int rt_server_release(rt_server* server, int64_t id);
+
+/* --- Errors and the device ---------------------------------------------------------------------------------------- */
+
+const char* rt_last_error(void);
+int rt_abi_version(void);
+
+typedef struct rt_device_info
+{
+    char name[128];
+    char arch[64];
+    int32_t compute_units;
+    int32_t reserved;
+    double memory_gib;
+    int32_t runtime_version;
+    int32_t driver_version;
+} rt_device_info;
+
+/* gpu::query_device (src/gpu/renderer.h:516): describes device 0 and starts the runtime on it */
+int rt_query_device(rt_device_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

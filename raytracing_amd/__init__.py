@@ -1,0 +1,460 @@
+"""MI355X path tracer: Python bindings over the C ABI of ``librt_amd.so`` (include/raytracing_amd.h).
+
+The API mirrors the reference's renderer interface (DaniilK19/RayTracing ``src/gpu/renderer.h``):
+``build_scene`` is ``build_gpu_scene`` (``src/gpu_scene.ixx:1333-1445``), ``render`` is ``gpu::render``
+(``renderer.h:526``), ``Session`` is ``gpu::session`` (``renderer.h:585-673``). Errors raise the exception types the
+reference throws: ``RuntimeError`` (GPU and scene errors), ``ValueError`` (std::invalid_argument: options out of range)
+and ``RenderLogicError`` (std::logic_error: state misuse).
+
+The library is the only path: importing this package without the built ``librt_amd.so`` raises ``ImportError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "librt_amd.so")
+
+if not os.path.exists(LIBRARY_PATH):
+    raise ImportError(
+        f"{LIBRARY_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()' or make -C raytracing_amd)"
+    )
+_lib = C.CDLL(LIBRARY_PATH)
+
+
+class RenderLogicError(RuntimeError):
+    """std::logic_error of the reference: the session is used in a state that doesn't allow the call."""
+
+
+# --- the C structs (include/raytracing_amd.h) ------------------------------------------------------------------------
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("even", C.c_int32), ("odd", C.c_int32), ("image", C.c_int32),
+                ("perlin", C.c_int32), ("reserved", C.c_int32), ("rgb", Vec3), ("inv_scale", C.c_double),
+                ("scale", C.c_double)]
+
+
+class ImageDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb", C.POINTER(C.c_float))]
+
+
+class PerlinDesc(C.Structure):
+    _fields_ = [("random_vectors", Vec3 * 256), ("perm_x", C.c_int32 * 256), ("perm_y", C.c_int32 * 256),
+                ("perm_z", C.c_int32 * 256)]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("texture", C.c_int32), ("albedo", Vec3), ("fuzz", C.c_double),
+                ("refraction_index", C.c_double)]
+
+
+class SphereDesc(C.Structure):
+    _fields_ = [("center", Vec3), ("radius", C.c_double), ("displacement", Vec3), ("uv_cos", C.c_double),
+                ("uv_sin", C.c_double), ("material", C.c_int32), ("reserved", C.c_int32)]
+
+
+class QuadDesc(C.Structure):
+    _fields_ = [("q", Vec3), ("u", Vec3), ("v", Vec3), ("material", C.c_int32), ("reserved", C.c_int32)]
+
+
+class MediumDesc(C.Structure):
+    _fields_ = [("boundary", C.c_int32), ("material", C.c_int32), ("neg_inv_density", C.c_double),
+                ("sphere", SphereDesc), ("quads", C.POINTER(QuadDesc)), ("quad_count", C.c_int64)]
+
+
+class LightDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("quad", QuadDesc), ("center", Vec3),
+                ("radius", C.c_double), ("weight", C.c_double)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("image_width", C.c_int32), ("image_height", C.c_int32), ("sqrt_spp", C.c_int32),
+                ("max_depth", C.c_int32), ("center", Vec3), ("pixel00_loc", Vec3), ("pixel_delta_u", Vec3),
+                ("pixel_delta_v", Vec3), ("defocus", C.c_int32), ("sampling", C.c_int32), ("defocus_disk_u", Vec3),
+                ("defocus_disk_v", Vec3), ("sky_bottom", Vec3), ("sky_top", Vec3)]
+
+
+class MicrofacetDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("roughness", C.c_double)]
+
+
+class EnvironmentDesc(C.Structure):
+    _fields_ = [("image", ImageDesc), ("yaw_degrees", C.c_double), ("scale", C.c_double)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("vertex_count", C.c_int64),
+                ("indices", C.POINTER(C.c_uint32)), ("triangle_count", C.c_int64), ("material", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class MeshInstanceDesc(C.Structure):
+    _fields_ = [("mesh", C.c_int32), ("reserved", C.c_int32), ("object_to_world", C.c_double * 12)]
+
+
+class Scene(C.Structure):
+    _fields_ = [("spheres", C.POINTER(SphereDesc)), ("sphere_count", C.c_int64),
+                ("materials", C.POINTER(MaterialDesc)), ("material_count", C.c_int64),
+                ("camera", CameraDesc),
+                ("textures", C.POINTER(TextureDesc)), ("texture_count", C.c_int64),
+                ("images", C.POINTER(ImageDesc)), ("image_count", C.c_int64),
+                ("perlins", C.POINTER(PerlinDesc)), ("perlin_count", C.c_int64),
+                ("quads", C.POINTER(QuadDesc)), ("quad_count", C.c_int64),
+                ("media", C.POINTER(MediumDesc)), ("medium_count", C.c_int64),
+                ("lights", C.POINTER(LightDesc)), ("light_count", C.c_int64),
+                ("combination", C.c_int32), ("reserved", C.c_int32),
+                ("microfacets", C.POINTER(MicrofacetDesc)), ("microfacet_count", C.c_int64),
+                ("environment", C.POINTER(EnvironmentDesc)),
+                ("meshes", C.POINTER(MeshDesc)), ("mesh_count", C.c_int64),
+                ("instances", C.POINTER(MeshInstanceDesc)), ("instance_count", C.c_int64)]
+
+
+class SceneOverrides(C.Structure):
+    _fields_ = [("image_width", C.c_int32), ("image_height", C.c_int32), ("samples_per_pixel", C.c_int32),
+                ("strata", C.c_int32), ("sampling", C.c_int32), ("max_depth", C.c_int32),
+                ("earth_width", C.c_int32), ("earth_seed", C.c_uint32)]
+
+
+class SessionOptions(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("precision", C.c_int32), ("integrator", C.c_int32),
+                ("launch_seconds", C.c_double), ("block_size", C.c_int32), ("wavefront_slots", C.c_int32)]
+
+
+class RenderReport(C.Structure):
+    _fields_ = [("passes", C.c_int64), ("rows", C.c_int32), ("launches", C.c_int32), ("gpu_seconds", C.c_double),
+                ("longest_launch_seconds", C.c_double)]
+
+
+class KernelInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("registers", C.c_int32), ("local_bytes", C.c_int32),
+                ("lds_bytes", C.c_int32), ("threads_per_block", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("compute_units", C.c_int32)]
+
+
+class RenderOptions(C.Structure):
+    _fields_ = [("session", SessionOptions), ("passes", C.c_int64)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("kernel_seconds", C.c_double), ("rays", C.c_uint64),
+                ("shadow_rays", C.c_uint64)]
+
+
+class Traffic(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("events", C.c_uint64 * 16),
+                ("event_bytes", C.c_double * 16), ("bytes_per_path", C.c_double)]
+
+
+TRAFFIC_EVENTS = ["node", "sphere", "moving_sphere", "quad", "medium", "boundary_quad", "hit", "frame", "texture",
+                  "texel", "perlin", "light"]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("memory_budget_bytes", C.c_uint64), ("max_active", C.c_int32), ("passes_per_step", C.c_int32)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("id", C.c_int64), ("state", C.c_int32), ("reserved", C.c_int32), ("passes_done", C.c_int64),
+                ("passes_total", C.c_int64), ("milestone", C.c_int64), ("submit_time", C.c_double),
+                ("start_time", C.c_double), ("milestone_time", C.c_double), ("finish_time", C.c_double)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 128), ("arch", C.c_char * 64), ("compute_units", C.c_int32),
+                ("reserved", C.c_int32), ("memory_gib", C.c_double), ("runtime_version", C.c_int32),
+                ("driver_version", C.c_int32)]
+
+
+# Every function the header declares, with its signature
+_P = C.POINTER
+_SIGNATURES = {
+    "rt_last_error": (C.c_char_p, []),
+    "rt_abi_version": (C.c_int, []),
+    "rt_scene_names": (C.c_char_p, []),
+    "rt_scene_build": (C.c_int, [C.c_char_p, _P(SceneOverrides), _P(C.c_void_p)]),
+    "rt_scene_get": (_P(Scene), [C.c_void_p]),
+    "rt_scene_passes": (C.c_int64, [C.c_void_p]),
+    "rt_scene_free": (None, [C.c_void_p]),
+    "rt_session_create": (C.c_int, [_P(Scene), _P(SessionOptions), _P(C.c_void_p)]),
+    "rt_session_destroy": (None, [C.c_void_p]),
+    "rt_session_set_camera": (C.c_int, [C.c_void_p, _P(CameraDesc)]),
+    "rt_session_render_to": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _P(RenderReport)]),
+    "rt_session_render_for": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, _P(RenderReport)]),
+    "rt_session_state": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int32)]),
+    "rt_session_pixels": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_size_t]),
+    "rt_session_sums": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_size_t]),
+    "rt_session_display_rgba8": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_size_t]),
+    "rt_session_counters": (C.c_int, [C.c_void_p, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_double)]),
+    "rt_session_kernel_info": (C.c_int, [C.c_void_p, _P(KernelInfo)]),
+    "rt_render": (C.c_int, [_P(Scene), _P(RenderOptions), _P(C.c_double), C.c_size_t, _P(RenderStats)]),
+    "rt_session_closest_hit": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_int32),
+                                         _P(C.c_uint8), C.c_int64, _P(C.c_int32), _P(C.c_float), _P(C.c_uint8)]),
+    "rt_session_traffic": (C.c_int, [C.c_void_p, C.c_int64, _P(Traffic)]),
+    "rt_scene_bvh": (C.c_int, [_P(Scene), _P(C.c_float), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32),
+                               _P(C.c_int32), _P(C.c_int32)]),
+    "rt_server_create": (C.c_int, [_P(ServerOptions), _P(C.c_void_p)]),
+    "rt_server_destroy": (None, [C.c_void_p]),
+    "rt_server_submit": (C.c_int, [C.c_void_p, _P(Scene), C.c_uint64, C.c_int64, _P(C.c_int64)]),
+    "rt_server_step": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
+    "rt_server_status": (C.c_int, [C.c_void_p, C.c_int64, _P(RequestStatus)]),
+    "rt_server_pixels": (C.c_int, [C.c_void_p, C.c_int64, _P(C.c_double), C.c_size_t]),
+    "rt_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
+    "rt_query_device": (C.c_int, [_P(DeviceInfo)]),
+}
+for _name, (_res, _args) in _SIGNATURES.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+RT_OK, RT_ERROR_RUNTIME, RT_ERROR_INVALID_ARGUMENT, RT_ERROR_LOGIC = 0, 1, 2, 3
+SAMPLING = {"uniform": 0, "material": 1, "lights": 2, "mixture": 3, "nee": 4, "mis": 5}
+INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2}
+
+
+def library() -> C.CDLL:
+    return _lib
+
+
+def last_error() -> str:
+    return _lib.rt_last_error().decode()
+
+
+def _check(status: int) -> None:
+    if status == RT_OK:
+        return
+    message = last_error()
+    if status == RT_ERROR_INVALID_ARGUMENT:
+        raise ValueError(message)
+    if status == RT_ERROR_LOGIC:
+        raise RenderLogicError(message)
+    raise RuntimeError(message)
+
+
+def scene_names() -> list[str]:
+    return _lib.rt_scene_names().decode().split(",")
+
+
+class BuiltScene:
+    """A scene built by name and flattened (build_gpu_scene): ``desc`` is the rt_scene view over its tables."""
+
+    def __init__(self, name: str, width: int = 0, height: int = 0, samples: int = 0, strata: int = 0,
+                 sampling: Optional[str] = None, max_depth: int = 0, earth_width: int = 0, earth_seed: int = 0):
+        o = SceneOverrides(width, height, samples, strata, -1 if sampling is None else SAMPLING[sampling], max_depth,
+                           earth_width, earth_seed)
+        handle = C.c_void_p()
+        _check(_lib.rt_scene_build(name.encode(), C.byref(o), C.byref(handle)))
+        self._handle = handle
+        self.name = name
+        self.desc: Scene = _lib.rt_scene_get(handle).contents
+        self.passes: int = _lib.rt_scene_passes(handle)
+
+    @property
+    def width(self) -> int:
+        return self.desc.camera.image_width
+
+    @property
+    def height(self) -> int:
+        return self.desc.camera.image_height
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            _lib.rt_scene_free(self._handle)
+            self._handle = None
+
+
+def build_scene(name: str, **overrides) -> BuiltScene:
+    return BuiltScene(name, **overrides)
+
+
+def _session_options(seed=0, precision="float32", integrator="auto", launch_seconds=0.0, block_size=0,
+                     wavefront_slots=0) -> SessionOptions:
+    return SessionOptions(seed, 1 if precision in ("float64", "fp64") else 0, INTEGRATORS[integrator],
+                          launch_seconds, block_size, wavefront_slots)
+
+
+@dataclass
+class RenderResult:
+    """gpu::render_result (renderer.h:475-492)"""
+    pixels: np.ndarray  # (height, width, 3) float64, rows from the top
+    seconds: float
+    kernel_seconds: float
+    rays: int
+    shadow_rays: int
+
+
+def render(scene: BuiltScene, passes: int = 0, **options) -> RenderResult:
+    """gpu::render: a session rendered to ``passes`` (0: the scene's own samples)."""
+    desc = scene.desc
+    out = np.zeros((desc.camera.image_height, desc.camera.image_width, 3), dtype=np.float64)
+    ro = RenderOptions(_session_options(**options), passes if passes else scene.passes)
+    stats = RenderStats()
+    _check(_lib.rt_render(C.byref(desc), C.byref(ro), out.ctypes.data_as(_P(C.c_double)), out.size, C.byref(stats)))
+    return RenderResult(out, stats.seconds, stats.kernel_seconds, stats.rays, stats.shadow_rays)
+
+
+class Session:
+    """gpu::session: the scene stays on the GPU and the image of one view gains passes on demand."""
+
+    def __init__(self, scene, **options):
+        desc = scene.desc if isinstance(scene, BuiltScene) else scene
+        self._keep = scene
+        handle = C.c_void_p()
+        _check(_lib.rt_session_create(C.byref(desc), C.byref(_session_options(**options)), C.byref(handle)))
+        self._h = handle
+        self.camera = CameraDesc.from_buffer_copy(desc.camera)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.rt_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_camera(self, camera: CameraDesc) -> None:
+        _check(_lib.rt_session_set_camera(self._h, C.byref(camera)))
+        self.camera = CameraDesc.from_buffer_copy(camera)
+
+    def render_to(self, passes: int, rows: int = 0) -> RenderReport:
+        r = RenderReport()
+        _check(_lib.rt_session_render_to(self._h, passes, rows, C.byref(r)))
+        return r
+
+    def render_for(self, seconds: float, pass_limit: int) -> RenderReport:
+        r = RenderReport()
+        _check(_lib.rt_session_render_for(self._h, seconds, pass_limit, C.byref(r)))
+        return r
+
+    def state(self) -> tuple[int, int]:
+        p, r = C.c_int64(), C.c_int32()
+        _check(_lib.rt_session_state(self._h, C.byref(p), C.byref(r)))
+        return p.value, r.value
+
+    def _image(self, fn) -> np.ndarray:
+        out = np.zeros((self.camera.image_height, self.camera.image_width, 3), dtype=np.float64)
+        _check(fn(self._h, out.ctypes.data_as(_P(C.c_double)), out.size))
+        return out
+
+    def pixels(self) -> np.ndarray:
+        return self._image(_lib.rt_session_pixels)
+
+    def sums(self) -> np.ndarray:
+        return self._image(_lib.rt_session_sums)
+
+    def display(self) -> np.ndarray:
+        out = np.zeros((self.camera.image_height, self.camera.image_width, 4), dtype=np.uint8)
+        _check(_lib.rt_session_display_rgba8(self._h, out.ctypes.data_as(_P(C.c_uint8)), out.size))
+        return out
+
+    def counters(self) -> tuple[int, int, float]:
+        rays, shadow, seconds = C.c_uint64(), C.c_uint64(), C.c_double()
+        _check(_lib.rt_session_counters(self._h, C.byref(rays), C.byref(shadow), C.byref(seconds)))
+        return rays.value, shadow.value, seconds.value
+
+    def kernel(self) -> KernelInfo:
+        k = KernelInfo()
+        _check(_lib.rt_session_kernel_info(self._h, C.byref(k)))
+        return k
+
+    def traffic(self, passes: int) -> Traffic:
+        """The algorithmic scene traffic of `passes` passes, counted by the instrumented megakernel."""
+        t = Traffic()
+        _check(_lib.rt_session_traffic(self._h, passes, C.byref(t)))
+        return t
+
+    def closest_hit(self, origins, directions, times, from_ids, inward):
+        """The session's BVH traversal on given rays: (ids, t, entering)."""
+        o = np.ascontiguousarray(origins, dtype=np.float32)
+        d = np.ascontiguousarray(directions, dtype=np.float32)
+        tm = np.ascontiguousarray(times, dtype=np.float32)
+        f = np.ascontiguousarray(from_ids, dtype=np.int32)
+        inw = np.ascontiguousarray(inward, dtype=np.uint8)
+        n = len(tm)
+        ids = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        ent = np.zeros(n, np.uint8)
+        _check(_lib.rt_session_closest_hit(self._h, o.ctypes.data_as(_P(C.c_float)), d.ctypes.data_as(_P(C.c_float)),
+                                           tm.ctypes.data_as(_P(C.c_float)), f.ctypes.data_as(_P(C.c_int32)),
+                                           inw.ctypes.data_as(_P(C.c_uint8)), n, ids.ctypes.data_as(_P(C.c_int32)),
+                                           t.ctypes.data_as(_P(C.c_float)), ent.ctypes.data_as(_P(C.c_uint8))))
+        return ids, t, ent
+
+
+def scene_bvh(scene: BuiltScene):
+    """The float32 BVH this library builds for a scene, in the reference's node layout (host only)."""
+    count = C.c_int64()
+    depth = C.c_int32()
+    _check(_lib.rt_scene_bvh(C.byref(scene.desc), None, None, C.byref(count), None, None, C.byref(depth)))
+    boxes = np.zeros((count.value, 6), np.float32)
+    ints = np.zeros((count.value, 2), np.int32)
+    sphere_ids = np.zeros(scene.desc.sphere_count, np.int32)
+    quad_ids = np.zeros(scene.desc.quad_count, np.int32)
+    _check(_lib.rt_scene_bvh(C.byref(scene.desc), boxes.ctypes.data_as(_P(C.c_float)),
+                             ints.ctypes.data_as(_P(C.c_int32)), C.byref(count),
+                             sphere_ids.ctypes.data_as(_P(C.c_int32)), quad_ids.ctypes.data_as(_P(C.c_int32)),
+                             C.byref(depth)))
+    return boxes, ints, sphere_ids, quad_ids, depth.value
+
+
+def query_device() -> DeviceInfo:
+    info = DeviceInfo()
+    _check(_lib.rt_query_device(C.byref(info)))
+    return info
+
+
+class Server:
+    """Progressive renders of mixed requests on one device, admitted by a memory budget (continuous batching)."""
+
+    def __init__(self, memory_budget_bytes=0, max_active=0, passes_per_step=0):
+        h = C.c_void_p()
+        _check(_lib.rt_server_create(C.byref(ServerOptions(memory_budget_bytes, max_active, passes_per_step)),
+                                     C.byref(h)))
+        self._h = h
+        self._scenes = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.rt_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, scene: BuiltScene, seed: int, passes: int) -> int:
+        rid = C.c_int64()
+        _check(_lib.rt_server_submit(self._h, C.byref(scene.desc), seed, passes, C.byref(rid)))
+        self._scenes[rid.value] = (scene.width, scene.height)
+        return rid.value
+
+    def step(self) -> int:
+        active = C.c_int32()
+        _check(_lib.rt_server_step(self._h, C.byref(active)))
+        return active.value
+
+    def status(self, rid: int) -> RequestStatus:
+        st = RequestStatus()
+        _check(_lib.rt_server_status(self._h, rid, C.byref(st)))
+        return st
+
+    def pixels(self, rid: int) -> np.ndarray:
+        w, h = self._scenes[rid]
+        out = np.zeros((h, w, 3), np.float64)
+        _check(_lib.rt_server_pixels(self._h, rid, out.ctypes.data_as(_P(C.c_double)), out.size))
+        return out
+
+    def release(self, rid: int) -> None:
+        _check(_lib.rt_server_release(self._h, rid))

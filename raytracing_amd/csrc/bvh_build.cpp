@@ -1,0 +1,374 @@
+// The BVH builder (see bvh_build.hpp)
+#include "bvh_build.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <span>
+#include <stdexcept>
+
+namespace rt
+{
+namespace
+{
+struct v3d
+{
+    double x, y, z;
+};
+v3d cross(v3d u, v3d v) { return {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x}; }
+double dot(v3d u, v3d v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
+v3d scale(double s, v3d v) { return {s * v.x, s * v.y, s * v.z}; }
+v3d from(const rt_vec3& v) { return {v.x, v.y, v.z}; }
+template <typename Real>
+void store(Real* out, v3d v)
+{
+    out[0] = static_cast<Real>(v.x);
+    out[1] = static_cast<Real>(v.y);
+    out[2] = static_cast<Real>(v.z);
+}
+
+int longest_axis(const box_d& b)
+{
+    const double x = b.max[0] - b.min[0], y = b.max[1] - b.min[1], z = b.max[2] - b.min[2];
+    if (x > y)
+    {
+        return x > z ? 0 : 2;
+    }
+    return y > z ? 1 : 2;
+}
+
+// The box of a sphere in Real swept from one center to another, each bound moved out by one step of double
+// (src/gpu/bvh.cuh:147-176)
+template <typename Real>
+box_d sphere_box(const Real* a, const Real* b, Real radius)
+{
+    box_d box;
+    for (int k = 0; k < 3; ++k)
+    {
+        const double lo = std::min(static_cast<double>(a[k]), static_cast<double>(b[k]));
+        const double hi = std::max(static_cast<double>(a[k]), static_cast<double>(b[k]));
+        box.min[k] = std::nextafter(lo - static_cast<double>(radius), -INFINITY);
+        box.max[k] = std::nextafter(hi + static_cast<double>(radius), INFINITY);
+    }
+    return box;
+}
+
+bool stationary(const rt_sphere_desc& s)
+{
+    return s.displacement.x == 0 && s.displacement.y == 0 && s.displacement.z == 0;
+}
+
+template <typename Real>
+class builder
+{
+public:
+    builder(const rt_scene& scene, int max_leaf) : max_leaf_(max_leaf)
+    {
+        const auto n_spheres = static_cast<std::size_t>(scene.sphere_count);
+        for (std::size_t i = 0; i < n_spheres; ++i)
+        {
+            const rt_sphere_desc& s = scene.spheres[i];
+            if (!stationary(s))
+            {
+                continue;
+            }
+            dev::sphere_rec<Real> rec{};
+            store(rec.center, from(s.center));
+            rec.radius = static_cast<Real>(s.radius);
+            // onto_sphere's k and limit, in double from the sphere in Real (src/gpu/bvh.cuh:290-303)
+            const double x = rec.center[0], y = rec.center[1], z = rec.center[2], r = rec.radius;
+            const double c2 = x * x + y * y + z * z;
+            dev::sphere_frame<Real> frame{static_cast<Real>(s.uv_cos), static_cast<Real>(s.uv_sin),
+                                          static_cast<Real>(c2 - r * r), static_cast<Real>(2 * r * std::sqrt(c2))};
+            inputs_.push_back({dev::prim_sphere, static_cast<int>(spheres_.size()), static_cast<int>(i), s.material,
+                               frame, sphere_box<Real>(rec.center, rec.center, rec.radius)});
+            spheres_.push_back(rec);
+        }
+        out_.moving_base = static_cast<int32_t>(inputs_.size());
+        for (std::size_t i = 0; i < n_spheres; ++i)
+        {
+            const rt_sphere_desc& s = scene.spheres[i];
+            if (stationary(s))
+            {
+                continue;
+            }
+            dev::moving_sphere_rec<Real> rec{};
+            store(rec.center, from(s.center));
+            rec.radius = static_cast<Real>(s.radius);
+            store(rec.displacement, from(s.displacement));
+            Real end[3];
+            for (int k = 0; k < 3; ++k)
+            {
+                end[k] = rec.center[k] + rec.displacement[k]; // in Real, as the GPU computes the center at time 1
+            }
+            dev::sphere_frame<Real> frame{static_cast<Real>(s.uv_cos), static_cast<Real>(s.uv_sin), 0, 0};
+            inputs_.push_back({dev::prim_moving_sphere, static_cast<int>(moving_.size()), static_cast<int>(i),
+                               s.material, frame, sphere_box<Real>(rec.center, end, rec.radius)});
+            moving_.push_back(rec);
+        }
+        out_.quad_base = static_cast<int32_t>(inputs_.size());
+        for (int64_t i = 0; i < scene.quad_count; ++i)
+        {
+            inputs_.push_back({dev::prim_quad, static_cast<int>(quads_.size()), static_cast<int>(i),
+                               scene.quads[i].material, {}, quad_box(scene.quads[i])});
+            quads_.push_back(make_quad_rec<Real>(scene.quads[i]));
+        }
+        out_.sphere_ids.resize(n_spheres);
+        out_.quad_ids.resize(static_cast<std::size_t>(scene.quad_count));
+    }
+
+    bvh_arrays<Real> build() &&
+    {
+        out_.prim_materials.resize(inputs_.size());
+        out_.sphere_frames.resize(static_cast<std::size_t>(out_.quad_base));
+        std::vector<int> order(inputs_.size());
+        std::iota(order.begin(), order.end(), 0);
+        if (!order.empty())
+        {
+            add_subtree(order, 1);
+            pack_wide();
+        }
+        return std::move(out_);
+    }
+
+private:
+    struct input
+    {
+        int kind;
+        int index;
+        int source;
+        int material;
+        dev::sphere_frame<Real> frame;
+        box_d box;
+    };
+    int max_leaf_;
+    std::vector<input> inputs_;
+    std::vector<dev::sphere_rec<Real>> spheres_;
+    std::vector<dev::moving_sphere_rec<Real>> moving_;
+    std::vector<dev::quad_rec<Real>> quads_;
+    bvh_arrays<Real> out_;
+
+    int array_size(int kind) const
+    {
+        return kind == dev::prim_sphere ? static_cast<int>(out_.spheres.size())
+               : kind == dev::prim_moving_sphere ? static_cast<int>(out_.moving_spheres.size())
+                                                 : static_cast<int>(out_.quads.size());
+    }
+
+    void add_to_leaf(const input& in)
+    {
+        int id = 0;
+        switch (in.kind)
+        {
+        case dev::prim_sphere:
+            id = static_cast<int>(out_.spheres.size());
+            out_.spheres.push_back(spheres_[in.index]);
+            break;
+        case dev::prim_moving_sphere:
+            id = out_.moving_base + static_cast<int>(out_.moving_spheres.size());
+            out_.moving_spheres.push_back(moving_[in.index]);
+            break;
+        default:
+            id = out_.quad_base + static_cast<int>(out_.quads.size());
+            out_.quads.push_back(quads_[in.index]);
+            out_.prim_materials[id] = in.material;
+            out_.quad_ids[in.source] = id;
+            return;
+        }
+        out_.prim_materials[id] = in.material;
+        out_.sphere_frames[id] = in.frame;
+        out_.sphere_ids[in.source] = id;
+    }
+
+    // src/gpu/bvh.cuh:382-435, with a stable sort
+    int add_subtree(std::span<int> indices, int level)
+    {
+        if (level > 32)
+        {
+            throw std::runtime_error("the BVH is too deep for the traversal stack");
+        }
+        out_.depth = std::max(out_.depth, level);
+        box_d box;
+        for (const int i : indices)
+        {
+            box.enclose(inputs_[i].box);
+        }
+        const auto index = static_cast<int>(out_.nodes.size());
+        ref_node<Real> node{};
+        for (int k = 0; k < 3; ++k)
+        {
+            node.box_min[k] = round_down<Real>(box.min[k]);
+            node.box_max[k] = round_up<Real>(box.max[k]);
+        }
+        out_.nodes.push_back(node);
+        std::size_t middle = indices.size() / 2;
+        if (static_cast<int>(indices.size()) <= max_leaf_)
+        {
+            int kind = 3;
+            for (const int i : indices)
+            {
+                kind = std::min(kind, inputs_[i].kind);
+            }
+            const auto split = std::stable_partition(indices.begin(), indices.end(),
+                                                     [&](int i) { return inputs_[i].kind == kind; });
+            if (split == indices.end())
+            {
+                out_.nodes[index].index = array_size(kind);
+                out_.nodes[index].count = static_cast<int32_t>(indices.size()) | kind << 8;
+                for (const int i : indices)
+                {
+                    add_to_leaf(inputs_[i]);
+                }
+                return index;
+            }
+            middle = static_cast<std::size_t>(split - indices.begin());
+        }
+        else
+        {
+            const int axis = longest_axis(box);
+            std::stable_sort(indices.begin(), indices.end(),
+                             [&](int a, int b) { return inputs_[a].box.min[axis] < inputs_[b].box.min[axis]; });
+        }
+        add_subtree(indices.first(middle), level + 1);
+        out_.nodes[index].index = add_subtree(indices.subspan(middle), level + 1);
+        return index;
+    }
+
+    int32_t ref_of(int node, const std::vector<int32_t>& wide_index) const
+    {
+        const ref_node<Real>& n = out_.nodes[node];
+        if (n.count > 0)
+        {
+            const int count = n.count & 255;
+            if (n.index > dev::max_leaf_first || count > 16)
+            {
+                throw std::runtime_error("a BVH leaf beyond the packed reference's range");
+            }
+            return dev::make_leaf_ref(n.count >> 8, count, n.index);
+        }
+        return wide_index[node];
+    }
+
+    // Each inner node becomes a wide node holding its two children's boxes, numbered in depth-first order
+    void pack_wide()
+    {
+        std::vector<int32_t> wide_index(out_.nodes.size(), -1);
+        int32_t next = 0;
+        for (std::size_t i = 0; i < out_.nodes.size(); ++i)
+        {
+            if (out_.nodes[i].count == 0)
+            {
+                wide_index[i] = next++;
+            }
+        }
+        out_.wide.resize(static_cast<std::size_t>(next));
+        for (std::size_t i = 0; i < out_.nodes.size(); ++i)
+        {
+            const ref_node<Real>& n = out_.nodes[i];
+            if (n.count != 0)
+            {
+                continue;
+            }
+            const int first = static_cast<int>(i) + 1;
+            const int second = n.index;
+            dev::wide_node<Real> w{};
+            for (int k = 0; k < 3; ++k)
+            {
+                w.lo0[k] = out_.nodes[first].box_min[k];
+                w.hi0[k] = out_.nodes[first].box_max[k];
+                w.lo1[k] = out_.nodes[second].box_min[k];
+                w.hi1[k] = out_.nodes[second].box_max[k];
+            }
+            w.ref0 = ref_of(first, wide_index);
+            w.ref1 = ref_of(second, wide_index);
+            out_.wide[wide_index[i]] = w;
+        }
+        out_.root_ref = ref_of(0, wide_index);
+    }
+};
+}
+
+void box_d::enclose(const box_d& o)
+{
+    for (int a = 0; a < 3; ++a)
+    {
+        min[a] = std::min(min[a], o.min[a]);
+        max[a] = std::max(max[a], o.max[a]);
+    }
+}
+
+template <typename Real>
+Real round_down(double x)
+{
+    const auto r = static_cast<Real>(x);
+    return r <= x ? r : std::nextafter(r, -std::numeric_limits<Real>::infinity());
+}
+template <typename Real>
+Real round_up(double x)
+{
+    const auto r = static_cast<Real>(x);
+    return r >= x ? r : std::nextafter(r, std::numeric_limits<Real>::infinity());
+}
+
+box_d quad_box(const rt_quad_desc& d)
+{
+    const v3d q = from(d.q), u = from(d.u), v = from(d.v);
+    const v3d corners[4] = {q, {q.x + u.x, q.y + u.y, q.z + u.z}, {q.x + v.x, q.y + v.y, q.z + v.z},
+                            {q.x + u.x + v.x, q.y + u.y + v.y, q.z + u.z + v.z}};
+    box_d box;
+    double largest = 0;
+    for (const v3d& c : corners)
+    {
+        const double xyz[3] = {c.x, c.y, c.z};
+        for (int a = 0; a < 3; ++a)
+        {
+            box.min[a] = std::min(box.min[a], xyz[a]);
+            box.max[a] = std::max(box.max[a], xyz[a]);
+            largest = std::max(largest, std::fabs(xyz[a]));
+        }
+    }
+    const double margin = 1e-6 * (1 + largest);
+    for (int a = 0; a < 3; ++a)
+    {
+        box.min[a] -= margin;
+        box.max[a] += margin;
+        if (box.max[a] - box.min[a] < 0.0001)
+        {
+            box.min[a] -= 0.0001 / 2;
+            box.max[a] += 0.0001 / 2;
+        }
+    }
+    return box;
+}
+
+template <typename Real>
+dev::quad_rec<Real> make_quad_rec(const rt_quad_desc& d)
+{
+    const v3d n = cross(from(d.u), from(d.v));
+    const double nn = dot(n, n);
+    dev::quad_rec<Real> q{};
+    store(q.q, from(d.q));
+    store(q.normal, scale(1 / std::sqrt(nn), n));
+    const v3d w = scale(1 / nn, n);
+    store(q.alpha_axis, cross(from(d.v), w));
+    store(q.beta_axis, cross(w, from(d.u)));
+    return q;
+}
+
+int leaf_size_for(std::size_t shapes) { return shapes <= 8 ? 8 : 1; }
+
+template <typename Real>
+bvh_arrays<Real> build_bvh(const rt_scene& scene, int max_leaf_size)
+{
+    return builder<Real>(scene, max_leaf_size).build();
+}
+
+template bvh_arrays<float> build_bvh<float>(const rt_scene&, int);
+template bvh_arrays<double> build_bvh<double>(const rt_scene&, int);
+template dev::quad_rec<float> make_quad_rec<float>(const rt_quad_desc&);
+template dev::quad_rec<double> make_quad_rec<double>(const rt_quad_desc&);
+template float round_down<float>(double);
+template double round_down<double>(double);
+template float round_up<float>(double);
+template double round_up<double>(double);
+}

@@ -1,0 +1,407 @@
+// The C ABI (include/raytracing_amd.h): exceptions become status codes and a thread-local message
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/raytracing_amd.h"
+#include "bvh_build.hpp"
+#include "device_scene.hpp"
+#include "scene_build.hpp"
+#include "server.hpp"
+#include "session.hpp"
+
+struct rt_scene_handle
+{
+    rt::built_scene built;
+    rt_scene view{};
+};
+
+struct rt_session
+{
+    std::unique_ptr<rt::session> impl;
+};
+
+struct rt_server
+{
+    std::unique_ptr<rt::server> impl;
+};
+
+namespace
+{
+thread_local std::string last_error;
+
+template <typename F>
+int guarded(F&& f)
+{
+    try
+    {
+        f();
+        last_error.clear();
+        return RT_OK;
+    }
+    catch (const std::invalid_argument& e)
+    {
+        last_error = e.what();
+        return RT_ERROR_INVALID_ARGUMENT;
+    }
+    catch (const std::logic_error& e)
+    {
+        last_error = e.what();
+        return RT_ERROR_LOGIC;
+    }
+    catch (const std::exception& e)
+    {
+        last_error = e.what();
+        return RT_ERROR_RUNTIME;
+    }
+    catch (...)
+    {
+        last_error = "unknown error";
+        return RT_ERROR_RUNTIME;
+    }
+}
+
+void need(const void* p, const char* what)
+{
+    if (!p)
+    {
+        throw std::invalid_argument(std::string(what) + " must not be NULL");
+    }
+}
+}
+
+extern "C" {
+
+const char* rt_last_error(void) { return last_error.c_str(); }
+
+int rt_abi_version(void) { return RT_ABI_VERSION; }
+
+const char* rt_scene_names(void) { return rt::scene_names(); }
+
+int rt_scene_build(const char* name, const rt_scene_overrides* overrides, rt_scene_handle** out)
+{
+    return guarded([&] {
+        need(name, "name");
+        need(out, "out");
+        rt_scene_overrides o{};
+        o.sampling = -1;
+        if (overrides)
+        {
+            o = *overrides;
+        }
+        auto h = std::make_unique<rt_scene_handle>();
+        h->built = rt::build_scene(name, o);
+        h->view = h->built.tables.view();
+        *out = h.release();
+    });
+}
+
+const rt_scene* rt_scene_get(const rt_scene_handle* handle) { return handle ? &handle->view : nullptr; }
+
+int64_t rt_scene_passes(const rt_scene_handle* handle) { return handle ? handle->built.passes : 0; }
+
+void rt_scene_free(rt_scene_handle* handle) { delete handle; }
+
+int rt_session_create(const rt_scene* scene, const rt_session_options* options, rt_session** out)
+{
+    return guarded([&] {
+        need(scene, "scene");
+        need(out, "out");
+        rt_session_options o{};
+        if (options)
+        {
+            o = *options;
+        }
+        auto s = std::make_unique<rt_session>();
+        s->impl = rt::make_session(*scene, o);
+        *out = s.release();
+    });
+}
+
+void rt_session_destroy(rt_session* session) { delete session; }
+
+int rt_session_set_camera(rt_session* session, const rt_camera_desc* camera)
+{
+    return guarded([&] {
+        need(session, "session");
+        need(camera, "camera");
+        session->impl->set_camera(*camera);
+    });
+}
+
+int rt_session_render_to(rt_session* session, int64_t passes, int32_t rows, rt_render_report* report)
+{
+    return guarded([&] {
+        need(session, "session");
+        const rt_render_report r = session->impl->render_to(passes, rows);
+        if (report)
+        {
+            *report = r;
+        }
+    });
+}
+
+int rt_session_render_for(rt_session* session, double seconds, int64_t pass_limit, rt_render_report* report)
+{
+    return guarded([&] {
+        need(session, "session");
+        const rt_render_report r = session->impl->render_for(seconds, pass_limit);
+        if (report)
+        {
+            *report = r;
+        }
+    });
+}
+
+int rt_session_state(const rt_session* session, int64_t* passes, int32_t* rows)
+{
+    return guarded([&] {
+        need(session, "session");
+        const rt::accumulation a = session->impl->state();
+        if (passes)
+        {
+            *passes = a.passes;
+        }
+        if (rows)
+        {
+            *rows = a.rows;
+        }
+    });
+}
+
+int rt_session_pixels(rt_session* session, double* out, size_t count)
+{
+    return guarded([&] {
+        need(session, "session");
+        need(out, "out");
+        session->impl->pixels(out, count);
+    });
+}
+
+int rt_session_sums(rt_session* session, double* out, size_t count)
+{
+    return guarded([&] {
+        need(session, "session");
+        need(out, "out");
+        session->impl->sums(out, count);
+    });
+}
+
+int rt_session_display_rgba8(rt_session* session, uint8_t* out, size_t count)
+{
+    return guarded([&] {
+        need(session, "session");
+        need(out, "out");
+        session->impl->display(out, count);
+    });
+}
+
+int rt_session_counters(const rt_session* session, uint64_t* rays, uint64_t* shadow_rays, double* kernel_seconds)
+{
+    return guarded([&] {
+        need(session, "session");
+        session->impl->counters(rays, shadow_rays, kernel_seconds);
+    });
+}
+
+int rt_session_kernel_info(const rt_session* session, rt_kernel_info* info)
+{
+    return guarded([&] {
+        need(session, "session");
+        need(info, "info");
+        *info = session->impl->kernel();
+    });
+}
+
+int rt_render(const rt_scene* scene, const rt_render_options* options, double* pixels, size_t count,
+              rt_render_stats* stats)
+{
+    return guarded([&] {
+        need(scene, "scene");
+        need(pixels, "pixels");
+        const auto start = std::chrono::steady_clock::now();
+        rt_render_options o{};
+        if (options)
+        {
+            o = *options;
+        }
+        if (o.passes < 0)
+        {
+            throw std::invalid_argument("the number of passes must not be negative");
+        }
+        if (count < 3ull * static_cast<size_t>(scene->camera.image_width) * scene->camera.image_height)
+        {
+            throw std::invalid_argument("the output buffer is smaller than width x height x 3");
+        }
+        auto s = rt::make_session(*scene, o.session);
+        const int64_t passes =
+            o.passes > 0 ? o.passes : static_cast<int64_t>(scene->camera.sqrt_spp) * scene->camera.sqrt_spp;
+        s->render_to(passes, 0);
+        s->pixels(pixels, count);
+        if (stats)
+        {
+            s->counters(&stats->rays, &stats->shadow_rays, &stats->kernel_seconds);
+            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        }
+    });
+}
+
+int rt_session_closest_hit(rt_session* session, const float* origins, const float* directions, const float* times,
+                           const int32_t* from, const uint8_t* inward, int64_t count, int32_t* index, float* t,
+                           uint8_t* entering)
+{
+    return guarded([&] {
+        need(session, "session");
+        if (count > 0)
+        {
+            need(origins, "origins");
+            need(directions, "directions");
+            need(times, "times");
+            need(from, "from");
+            need(inward, "inward");
+            need(index, "index");
+            need(t, "t");
+            need(entering, "entering");
+        }
+        session->impl->closest_hit(origins, directions, times, from, inward, count, index, t, entering);
+    });
+}
+
+int rt_session_traffic(rt_session* session, int64_t passes, rt_traffic* traffic)
+{
+    return guarded([&] {
+        need(session, "session");
+        need(traffic, "traffic");
+        *traffic = session->impl->traffic(passes);
+    });
+}
+
+int rt_scene_bvh(const rt_scene* scene, float* node_boxes, int32_t* node_ints, int64_t* node_count,
+                 int32_t* sphere_ids, int32_t* quad_ids, int32_t* depth)
+{
+    return guarded([&] {
+        need(scene, "scene");
+        need(node_count, "node_count");
+        rt::validate_scene(*scene);
+        const auto tree = rt::build_bvh<float>(
+            *scene, rt::leaf_size_for(static_cast<size_t>(scene->sphere_count + scene->quad_count)));
+        *node_count = static_cast<int64_t>(tree.nodes.size());
+        if (depth)
+        {
+            *depth = tree.depth;
+        }
+        if (node_boxes && node_ints)
+        {
+            for (size_t i = 0; i < tree.nodes.size(); ++i)
+            {
+                const auto& n = tree.nodes[i];
+                std::memcpy(node_boxes + 6 * i, n.box_min, 3 * sizeof(float));
+                std::memcpy(node_boxes + 6 * i + 3, n.box_max, 3 * sizeof(float));
+                node_ints[2 * i] = n.index;
+                node_ints[2 * i + 1] = n.count;
+            }
+        }
+        if (sphere_ids)
+        {
+            std::memcpy(sphere_ids, tree.sphere_ids.data(), tree.sphere_ids.size() * sizeof(int32_t));
+        }
+        if (quad_ids)
+        {
+            std::memcpy(quad_ids, tree.quad_ids.data(), tree.quad_ids.size() * sizeof(int32_t));
+        }
+    });
+}
+
+int rt_server_create(const rt_server_options* options, rt_server** out)
+{
+    return guarded([&] {
+        need(out, "out");
+        rt_server_options o{};
+        if (options)
+        {
+            o = *options;
+        }
+        auto s = std::make_unique<rt_server>();
+        s->impl = std::make_unique<rt::server>(o);
+        *out = s.release();
+    });
+}
+
+void rt_server_destroy(rt_server* server) { delete server; }
+
+int rt_server_submit(rt_server* server, const rt_scene* scene, uint64_t seed, int64_t passes, int64_t* id)
+{
+    return guarded([&] {
+        need(server, "server");
+        need(scene, "scene");
+        need(id, "id");
+        *id = server->impl->submit(*scene, seed, passes);
+    });
+}
+
+int rt_server_step(rt_server* server, int32_t* active)
+{
+    return guarded([&] {
+        need(server, "server");
+        const int n = server->impl->step();
+        if (active)
+        {
+            *active = n;
+        }
+    });
+}
+
+int rt_server_status(const rt_server* server, int64_t id, rt_request_status* status)
+{
+    return guarded([&] {
+        need(server, "server");
+        need(status, "status");
+        *status = server->impl->status(id);
+    });
+}
+
+int rt_server_pixels(rt_server* server, int64_t id, double* out, size_t count)
+{
+    return guarded([&] {
+        need(server, "server");
+        need(out, "out");
+        server->impl->pixels(id, out, count);
+    });
+}
+
+int rt_server_release(rt_server* server, int64_t id)
+{
+    return guarded([&] {
+        need(server, "server");
+        server->impl->release(id);
+    });
+}
+
+int rt_query_device(rt_device_info* info)
+{
+    return guarded([&] {
+        need(info, "info");
+        hipDeviceProp_t p{};
+        rt::check(hipGetDeviceProperties(&p, 0), "reading the device's properties");
+        std::memset(info, 0, sizeof *info);
+        std::snprintf(info->name, sizeof info->name, "%s", p.name);
+        std::snprintf(info->arch, sizeof info->arch, "%s", p.gcnArchName);
+        info->compute_units = p.multiProcessorCount;
+        info->memory_gib = static_cast<double>(p.totalGlobalMem) / (1024.0 * 1024.0 * 1024.0);
+        int v = 0;
+        if (hipRuntimeGetVersion(&v) == hipSuccess)
+        {
+            info->runtime_version = v;
+        }
+        if (hipDriverGetVersion(&v) == hipSuccess)
+        {
+            info->driver_version = v;
+        }
+        rt::check(hipFree(nullptr), "starting the runtime");
+    });
+}
+}

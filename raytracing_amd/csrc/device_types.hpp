@@ -1,0 +1,254 @@
+// The scene as it lies in the GPU's memory: plain records shared by the host code that builds them (bvh_build.cpp,
+// device_scene.cpp) and the kernels that read them. Every record a hot loop reads is a whole number of 16-byte words,
+// read with dwordx4 loads.
+//
+// Layout choices (the reference's are in src/gpu/bvh.cuh:29-38, sphere.cuh:11-46, quad.cuh:19-30):
+//   - BVH: a node holds the boxes of BOTH its children (64 bytes in float: one cache-line-sized fetch per visited
+//     node), where the reference's 32-byte node holds its own box and the visit reads the two children's records, one
+//     of them elsewhere in memory. A child is an inner node's index, or a packed leaf reference (leaf_ref below). The
+//     traversal makes the same decisions in the same order as the reference's (the same boxes, the same slab test).
+//   - primitives keep the reference's global ids: spheres that stand still [0, moving_base), moving spheres
+//     [moving_base, quad_base), quads [quad_base, ...), and the media after the surfaces.
+#pragma once
+
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace rt::dev
+{
+// The kinds of primitives in a leaf
+enum prim_kind : int32_t
+{
+    prim_sphere = 0,
+    prim_moving_sphere = 1,
+    prim_quad = 2,
+    prim_triangle = 3,
+};
+
+// A leaf reference: bit 31 set, the kind in bits 28-29, the count minus 1 in bits 24-27 (at most 16 primitives), and
+// the first primitive's index in its kind's array in bits 0-23
+constexpr int32_t leaf_first_bits = 24;
+constexpr int32_t max_leaf_first = (1 << leaf_first_bits) - 1;
+__host__ __device__ constexpr int32_t make_leaf_ref(int32_t kind, int32_t count, int32_t first)
+{
+    return static_cast<int32_t>(0x80000000u | (static_cast<uint32_t>(kind) << 28) |
+                                (static_cast<uint32_t>(count - 1) << 24) | static_cast<uint32_t>(first));
+}
+__host__ __device__ constexpr int32_t leaf_kind(int32_t ref) { return (ref >> 28) & 3; }
+__host__ __device__ constexpr int32_t leaf_count(int32_t ref) { return ((ref >> 24) & 15) + 1; }
+__host__ __device__ constexpr int32_t leaf_first(int32_t ref) { return ref & max_leaf_first; }
+
+// An inner node: the boxes of its two children (in the reference's order: the first child, then the second) and their
+// references. 64 bytes in float.
+template <typename Real>
+struct alignas(16) wide_node
+{
+    Real lo0[3];
+    Real hi0[3];
+    Real lo1[3];
+    Real hi1[3];
+    int32_t ref0;
+    int32_t ref1;
+    int32_t unused[2];
+};
+static_assert(sizeof(wide_node<float>) == 64);
+
+template <typename Real>
+struct alignas(16) sphere_rec
+{
+    Real center[3];
+    Real radius;
+};
+
+template <typename Real>
+struct alignas(16) moving_sphere_rec
+{
+    Real center[3];
+    Real radius;
+    Real displacement[3];
+    Real unused;
+};
+
+// A quad as its hit test reads it: the corner, the unit normal and the dual axes of (u, v) (src/gpu/quad.cuh:19-43)
+template <typename Real>
+struct alignas(16) quad_rec
+{
+    Real q[3];
+    Real normal[3];
+    Real alpha_axis[3];
+    Real beta_axis[3];
+};
+static_assert(sizeof(quad_rec<float>) == 48);
+
+// A triangle (Moller-Trumbore): a vertex and two edges, with the material in the last word
+template <typename Real>
+struct alignas(16) triangle_rec
+{
+    Real v0[3];
+    Real e1[3];
+    Real e2[3];
+    Real unused[3];
+};
+
+// What only a sphere hit reads besides its material (src/gpu/sphere.cuh:36-46)
+template <typename Real>
+struct alignas(16) sphere_frame
+{
+    Real cos_theta;
+    Real sin_theta;
+    Real k;
+    Real limit;
+};
+
+enum material_flags : uint32_t
+{
+    needs_uv = 1,
+    needs_exact_point = 2,
+};
+
+// A material and how its texture is read (src/gpu/material.cuh:18-40)
+template <typename Real>
+struct alignas(16) material_rec
+{
+    int32_t kind;
+    int32_t texture; // -1: one solid color, in albedo
+    uint32_t flags;
+    int32_t unused;
+    Real albedo[3];
+    Real fuzz;
+    Real refraction_index;
+    Real unused2[3];
+};
+
+template <typename Real>
+struct alignas(16) texture_rec
+{
+    int32_t kind;
+    int32_t even;
+    int32_t odd;
+    int32_t table; // image: index into the image infos; noise: index into the noise tables
+    Real rgb[3];
+    Real inv_scale;
+};
+
+struct alignas(16) image_info
+{
+    int32_t offset;
+    int32_t width;
+    int32_t height;
+    int32_t unused;
+};
+
+struct alignas(16) texel
+{
+    float r, g, b, unused;
+};
+
+template <typename Real>
+struct alignas(16) noise_rec
+{
+    Real gradients[256][4];
+    uint8_t perm_x[256];
+    uint8_t perm_y[256];
+    uint8_t perm_z[256];
+    Real scale;
+    Real unused[3];
+};
+
+// A volume of smoke (src/gpu/constant_medium.cuh:14-26)
+template <typename Real>
+struct alignas(16) medium_rec
+{
+    Real box_min[3];
+    Real neg_inv_density;
+    Real box_max[3];
+    Real unused;
+    Real center[3];
+    Real radius;
+    Real displacement[3];
+    Real unused2;
+    int32_t material;
+    int32_t first_quad;
+    int32_t quad_count;
+    int32_t unused3;
+};
+
+// A shape of the lights list (src/gpu/lights.cuh:18-39)
+template <typename Real>
+struct alignas(16) light_rec
+{
+    quad_rec<Real> quad;
+    Real u[3];
+    Real area;
+    Real v[3];
+    Real cdf;
+    Real center[3];
+    Real radius;
+    Real weight;
+    int32_t kind;
+    int32_t twin;
+    int32_t unused;
+};
+
+// The camera as the kernels read it (src/gpu/camera.cuh:33-51)
+template <typename Real>
+struct camera_rec
+{
+    int32_t image_width;
+    int32_t image_height;
+    int32_t sqrt_spp;
+    int32_t max_depth;
+    Real recip_sqrt_spp;
+    Real center[3];
+    Real center_to_pixel00[3];
+    Real pixel_delta_u[3];
+    Real pixel_delta_v[3];
+    int32_t defocus;
+    Real defocus_disk_u[3];
+    Real defocus_disk_v[3];
+    Real sky_bottom[3];
+    Real sky_top[3];
+};
+
+// Where the scene's arrays are: the kernels take it by value
+template <typename Real>
+struct scene_view
+{
+    const wide_node<Real>* nodes;
+    int32_t root_ref;  // the root: an inner node's index (0), a leaf reference, or -1 with no primitive at all
+    int32_t stack_depth;
+    const sphere_rec<Real>* spheres;
+    const moving_sphere_rec<Real>* moving_spheres;
+    const quad_rec<Real>* quads;
+    const triangle_rec<Real>* triangles;
+    int32_t moving_base;
+    int32_t quad_base;
+    int32_t triangle_base;
+    int32_t media_base;
+    const int32_t* prim_materials;
+    const sphere_frame<Real>* sphere_frames;
+    const material_rec<Real>* materials;
+    const texture_rec<Real>* textures;
+    const image_info* images;
+    const texel* texels;
+    const noise_rec<Real>* noises;
+    const medium_rec<Real>* media;
+    const quad_rec<Real>* boundary_quads;
+    int32_t medium_count;
+    int32_t light_count;
+    const light_rec<Real>* lights;
+    int32_t sampling; // rt_sampling_strategy
+    int32_t motion;   // some sphere or medium boundary moves: camera rays draw a time
+};
+
+// The work of one launch: passes [first_pass, first_pass + pass_count) of rows [first_row, first_row + row_count),
+// within one round of strata
+struct launch_work
+{
+    int32_t first_pass;
+    int32_t pass_count;
+    int32_t first_row;
+    int32_t row_count;
+};
+}

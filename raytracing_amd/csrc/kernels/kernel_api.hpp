@@ -1,0 +1,39 @@
+// The host's view of the kernels: launch functions with plain arguments, defined in the .hip files. The host runtime
+// (session.cpp) calls HIP only through these and the HIP runtime API.
+#pragma once
+
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+#include "../device_types.hpp"
+
+namespace rt
+{
+struct kernel_attributes
+{
+    int registers = 0;
+    int local_bytes = 0;
+    int lds_bytes = 0;
+    int blocks_per_cu = 0;
+};
+
+template <typename Real>
+size_t stack_lds_bytes(int depth, int threads);
+
+template <typename Real>
+hipError_t launch_megakernel(bool all_books, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
+                             uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays,
+                             int block_w, int block_h, hipStream_t stream, unsigned long long* tallies = nullptr);
+
+template <typename Real>
+hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origins, const Real* directions,
+                              const Real* times, const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index,
+                              Real* t, uint8_t* entering, hipStream_t stream);
+
+hipError_t launch_display(const double* sums, int64_t pixels, double inverse_passes, uint8_t* rgba,
+                          hipStream_t stream);
+
+template <typename Real>
+kernel_attributes megakernel_attributes(bool all_books, int block_threads, int depth);
+}

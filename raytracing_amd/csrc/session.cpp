@@ -1,0 +1,458 @@
+// The session (see session.hpp): the scene's device copy, the view's double sums, the launch planner, and the two
+// integrators behind one interface.
+#include "session.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "device_scene.hpp"
+#include "kernels/kernel_api.hpp"
+#include "wavefront.hpp"
+
+namespace rt
+{
+namespace
+{
+constexpr uint64_t round_seed_step = uint64_t{1} << 32;
+
+class hip_event
+{
+public:
+    hip_event() { check(hipEventCreate(&e_), "creating a HIP event"); }
+    ~hip_event() { (void)hipEventDestroy(e_); }
+    hip_event(const hip_event&) = delete;
+    hip_event& operator=(const hip_event&) = delete;
+    void record(hipStream_t s) { check(hipEventRecord(e_, s), "recording a HIP event"); }
+    void synchronize() { check(hipEventSynchronize(e_), "rendering on the GPU"); }
+    static double seconds_between(const hip_event& a, const hip_event& b)
+    {
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, a.e_, b.e_), "timing the GPU");
+        return ms / 1000.0;
+    }
+
+private:
+    hipEvent_t e_ = nullptr;
+};
+
+class stream_owner
+{
+public:
+    stream_owner() { check(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "creating a HIP stream"); }
+    ~stream_owner() { (void)hipStreamDestroy(s_); }
+    stream_owner(const stream_owner&) = delete;
+    stream_owner& operator=(const stream_owner&) = delete;
+    hipStream_t get() const { return s_; }
+
+private:
+    hipStream_t s_ = nullptr;
+};
+
+template <typename Real>
+class session_impl final : public session
+{
+public:
+    session_impl(const rt_scene& scene, const rt_session_options& options)
+        : scene_(scene_tables::copy_of(scene)), options_(options), dscene_(scene_.view())
+    {
+        view_scene_ = scene_.view();
+        all_books_ = !first_book_scene(view_scene_);
+        launch_seconds_ = options.launch_seconds > 0 ? options.launch_seconds : 0.05;
+        const int threads = options.block_size > 0 ? options.block_size : 128;
+        block_w_ = 16;
+        block_h_ = std::max(1, threads / 16);
+        use_wavefront_ = options.integrator == RT_INTEGRATOR_WAVEFRONT ||
+                         (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>());
+        rays_ = device_array<unsigned long long>(1);
+        start_view(scene.camera);
+    }
+
+    void set_camera(const rt_camera_desc& camera) override
+    {
+        validate_camera(camera);
+        if (camera.sampling != scene_.camera.sampling)
+        {
+            throw std::invalid_argument("a new view must keep the scene's sampling strategy");
+        }
+        has_view_ = false;
+        start_view(camera);
+    }
+
+    accumulation state() const override { return state_; }
+
+    rt_render_report render_to(int64_t target_passes, int32_t target_rows) override
+    {
+        require_view();
+        const int height = camera_.image_height;
+        if (target_passes < 0 || target_rows < 0 || target_rows >= height ||
+            target_passes < state_.passes || (target_passes == state_.passes && target_rows < state_.rows))
+        {
+            throw std::invalid_argument("the target is behind the image or past its rows");
+        }
+        rt_render_report report{};
+        while (state_.passes < target_passes || (state_.passes == target_passes && state_.rows < target_rows))
+        {
+            dev::launch_work work{};
+            const int64_t pass = state_.passes;
+            const int64_t round_size = static_cast<int64_t>(camera_.sqrt_spp) * camera_.sqrt_spp;
+            const int64_t round = pass / round_size;
+            work.first_pass = static_cast<int32_t>(pass % round_size);
+            if (state_.rows > 0 || pass == target_passes)
+            {
+                // a band of rows of the pass in progress
+                const int end_row = pass < target_passes ? height : target_rows;
+                work.pass_count = 1;
+                work.first_row = state_.rows;
+                work.row_count = end_row - state_.rows;
+            }
+            else
+            {
+                const int64_t left_in_round = round_size - work.first_pass;
+                int64_t count = seconds_per_pass_ > 0 ? static_cast<int64_t>(launch_seconds_ / seconds_per_pass_) : 1;
+                count = std::max<int64_t>(1, std::min({count, left_in_round, target_passes - pass}));
+                work.pass_count = static_cast<int32_t>(count);
+                work.first_row = 0;
+                work.row_count = height;
+            }
+            const double seconds = launch(options_.seed + static_cast<uint64_t>(round) * round_seed_step, work);
+            report.launches += 1;
+            report.gpu_seconds += seconds;
+            report.longest_launch_seconds = std::max(report.longest_launch_seconds, seconds);
+            if (work.first_row == 0 && work.row_count == height)
+            {
+                seconds_per_pass_ = seconds / work.pass_count;
+                state_.passes += work.pass_count;
+            }
+            else if (work.first_row + work.row_count == height)
+            {
+                state_.passes += 1;
+                state_.rows = 0;
+            }
+            else
+            {
+                state_.rows = work.first_row + work.row_count;
+            }
+        }
+        report.passes = state_.passes;
+        report.rows = state_.rows;
+        return report;
+    }
+
+    rt_render_report render_for(double seconds, int64_t pass_limit) override
+    {
+        require_view();
+        if (!(seconds >= 0 && seconds <= 3600) || pass_limit < 0)
+        {
+            throw std::invalid_argument("a time from 0 to 3600 seconds and a pass limit of 0 or more");
+        }
+        rt_render_report report{};
+        while (state_.passes < pass_limit && (report.gpu_seconds < seconds || state_.passes == 0))
+        {
+            const int64_t remaining = pass_limit - state_.passes;
+            int64_t step = seconds_per_pass_ > 0
+                               ? static_cast<int64_t>(std::max(1.0, (seconds - report.gpu_seconds) / seconds_per_pass_))
+                               : 1;
+            step = std::min(step, remaining);
+            const rt_render_report r = render_to(state_.passes + step, 0);
+            report.launches += r.launches;
+            report.gpu_seconds += r.gpu_seconds;
+            report.longest_launch_seconds = std::max(report.longest_launch_seconds, r.longest_launch_seconds);
+        }
+        report.passes = state_.passes;
+        report.rows = state_.rows;
+        return report;
+    }
+
+    void pixels(double* out, std::size_t count) override
+    {
+        require_view();
+        if (state_.passes == 0 || state_.rows != 0)
+        {
+            throw std::logic_error("the image has no complete pass, or a pass in progress");
+        }
+        sums(out, count);
+        const double inverse = 1.0 / static_cast<double>(state_.passes);
+        for (std::size_t i = 0; i < values(); ++i)
+        {
+            out[i] *= inverse;
+        }
+    }
+
+    void sums(double* out, std::size_t count) override
+    {
+        require_view();
+        if (count < values())
+        {
+            throw std::invalid_argument("the output buffer is smaller than width x height x 3");
+        }
+        check(hipStreamSynchronize(stream_.get()), "waiting for the GPU");
+        check(hipMemcpy(out, sums_.data(), values() * sizeof(double), hipMemcpyDeviceToHost), "copying the image");
+    }
+
+    void display(uint8_t* out, std::size_t count) override
+    {
+        require_view();
+        if (state_.passes == 0)
+        {
+            throw std::logic_error("the image has no complete pass");
+        }
+        const std::size_t pixels = values() / 3;
+        if (count < 4 * pixels)
+        {
+            throw std::invalid_argument("the output buffer is smaller than width x height x 4");
+        }
+        if (rgba_.size() != 4 * pixels)
+        {
+            rgba_ = device_array<uint8_t>(4 * pixels);
+        }
+        check(launch_display(sums_.data(), static_cast<int64_t>(pixels), 1.0 / static_cast<double>(state_.passes),
+                             rgba_.data(), stream_.get()),
+              "launching the display kernel");
+        check(hipStreamSynchronize(stream_.get()), "computing the display");
+        check(hipMemcpy(out, rgba_.data(), 4 * pixels, hipMemcpyDeviceToHost), "copying the display");
+    }
+
+    void counters(uint64_t* rays, uint64_t* shadow_rays, double* kernel_seconds) const override
+    {
+        require_view();
+        unsigned long long n = 0;
+        check(hipMemcpy(&n, rays_.data(), sizeof n, hipMemcpyDeviceToHost), "reading the ray counter");
+        if (rays)
+        {
+            *rays = n;
+        }
+        if (shadow_rays)
+        {
+            *shadow_rays = 0;
+        }
+        if (kernel_seconds)
+        {
+            *kernel_seconds = kernel_seconds_;
+        }
+    }
+
+    rt_kernel_info kernel() const override
+    {
+        rt_kernel_info info{};
+        kernel_attributes a{};
+        const char* name = nullptr;
+        if (use_wavefront_)
+        {
+            a = wavefront_->attributes();
+            name = "wavefront";
+            info.threads_per_block = 256;
+        }
+        else
+        {
+            a = megakernel_attributes<Real>(all_books_, block_w_ * block_h_, dscene_.view().stack_depth);
+            name = all_books_ ? "all books" : "first book";
+            info.threads_per_block = block_w_ * block_h_;
+        }
+        std::snprintf(info.name, sizeof info.name, "%s%s", name, sizeof(Real) == 8 ? " fp64" : "");
+        info.registers = a.registers;
+        info.local_bytes = a.local_bytes;
+        info.lds_bytes = a.lds_bytes;
+        info.blocks_per_cu = a.blocks_per_cu;
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) == hipSuccess)
+        {
+            info.compute_units = cus;
+        }
+        return info;
+    }
+
+    void closest_hit(const float* origins, const float* directions, const float* times, const int32_t* from,
+                     const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) override
+    {
+        if constexpr (sizeof(Real) != sizeof(float))
+        {
+            throw std::invalid_argument("the closest-hit probe runs in float32 sessions only");
+        }
+        else
+        {
+            if (n <= 0)
+            {
+                return;
+            }
+            const std::size_t m = static_cast<std::size_t>(n);
+            device_array<float> o(3 * m), d(3 * m), tm(m), tt(m);
+            device_array<int32_t> f(m), idx(m);
+            device_array<uint8_t> in(m), ent(m);
+            check(hipMemcpy(o.data(), origins, o.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(d.data(), directions, d.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(tm.data(), times, tm.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(f.data(), from, f.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(in.data(), inward, in.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(launch_closest_hit<float>(dscene_.view(), o.data(), d.data(), tm.data(), f.data(), in.data(), n,
+                                            idx.data(), tt.data(), ent.data(), stream_.get()),
+                  "launching the closest-hit probe");
+            check(hipStreamSynchronize(stream_.get()), "tracing the probe rays");
+            check(hipMemcpy(index, idx.data(), idx.bytes(), hipMemcpyDeviceToHost), "copying hits");
+            check(hipMemcpy(t, tt.data(), tt.bytes(), hipMemcpyDeviceToHost), "copying hits");
+            check(hipMemcpy(entering, ent.data(), ent.bytes(), hipMemcpyDeviceToHost), "copying hits");
+        }
+    }
+
+    rt_traffic traffic(int64_t passes) override
+    {
+        require_view();
+        const int64_t round_size = static_cast<int64_t>(camera_.sqrt_spp) * camera_.sqrt_spp;
+        if (passes < 1 || passes > round_size)
+        {
+            throw std::invalid_argument("traffic is counted over 1 to sqrt_spp^2 passes of the first round");
+        }
+        device_array<double> scratch(values());
+        device_array<unsigned long long> counts(1 + 16);
+        check(hipMemsetAsync(scratch.data(), 0, scratch.bytes(), stream_.get()), "clearing scratch");
+        check(hipMemsetAsync(counts.data(), 0, counts.bytes(), stream_.get()), "clearing counters");
+        const dev::launch_work work{0, static_cast<int32_t>(passes), 0, camera_.image_height};
+        check(launch_megakernel<Real>(all_books_, cam_, dscene_.view(), options_.seed, work, scratch.data(),
+                                      counts.data(), block_w_, block_h_, stream_.get(), counts.data() + 1),
+              "launching the counting kernel");
+        check(hipStreamSynchronize(stream_.get()), "counting traffic");
+        unsigned long long host[17] = {};
+        check(hipMemcpy(host, counts.data(), sizeof host, hipMemcpyDeviceToHost), "reading counters");
+        rt_traffic t{};
+        t.paths = static_cast<uint64_t>(passes) * static_cast<uint64_t>(values() / 3);
+        t.rays = host[0];
+        const double sizes[RT_TRAFFIC_EVENTS] = {
+            sizeof(dev::wide_node<Real>), sizeof(dev::sphere_rec<Real>), sizeof(dev::moving_sphere_rec<Real>),
+            sizeof(dev::quad_rec<Real>), sizeof(dev::medium_rec<Real>), sizeof(dev::quad_rec<Real>),
+            4.0 + sizeof(dev::material_rec<Real>), sizeof(dev::sphere_frame<Real>), sizeof(dev::texture_rec<Real>),
+            sizeof(dev::texel), 4.0 * sizeof(Real) + 3.0, sizeof(dev::light_rec<Real>)};
+        double bytes = 0;
+        for (int e = 0; e < RT_TRAFFIC_EVENTS; ++e)
+        {
+            t.events[e] = host[1 + e];
+            t.event_bytes[e] = sizes[e];
+            bytes += sizes[e] * static_cast<double>(host[1 + e]);
+        }
+        // the pixel's double sums, read and written once per launch of `passes` passes
+        bytes += 48.0 * static_cast<double>(values() / 3);
+        t.bytes_per_path = bytes / static_cast<double>(t.paths);
+        return t;
+    }
+
+    std::size_t device_bytes() const override
+    {
+        return dscene_.bytes() + sums_.bytes() + (wavefront_ ? wavefront_->bytes() : 0);
+    }
+
+private:
+    scene_tables scene_;
+    rt_scene view_scene_{};
+    rt_session_options options_;
+    device_scene<Real> dscene_;
+    stream_owner stream_;
+    hip_event start_, stop_;
+    bool all_books_ = true;
+    bool use_wavefront_ = false;
+    std::unique_ptr<wavefront<Real>> wavefront_;
+    double launch_seconds_ = 0.05;
+    int block_w_ = 16, block_h_ = 8;
+    rt_camera_desc camera_{};
+    dev::camera_rec<Real> cam_{};
+    device_array<double> sums_;
+    device_array<unsigned long long> rays_;
+    device_array<uint8_t> rgba_;
+    bool has_view_ = false;
+    accumulation state_{};
+    double kernel_seconds_ = 0;
+    double seconds_per_pass_ = 0;
+
+    std::size_t values() const
+    {
+        return 3ull * static_cast<std::size_t>(camera_.image_width) * static_cast<std::size_t>(camera_.image_height);
+    }
+    void require_view() const
+    {
+        if (!has_view_)
+        {
+            throw std::logic_error("the session has no view: set a camera first");
+        }
+    }
+
+    void start_view(const rt_camera_desc& camera)
+    {
+        camera_ = camera;
+        cam_ = to_camera<Real>(camera);
+        const std::size_t n = values();
+        if (sums_.size() != n)
+        {
+            sums_ = device_array<double>();
+            sums_ = device_array<double>(n);
+        }
+        check(hipMemsetAsync(sums_.data(), 0, sums_.bytes(), stream_.get()), "clearing the image");
+        check(hipMemsetAsync(rays_.data(), 0, sizeof(unsigned long long), stream_.get()), "clearing the counter");
+        if (use_wavefront_)
+        {
+            const int slots = options_.wavefront_slots;
+            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, all_books_, slots, stream_.get());
+        }
+        check(hipStreamSynchronize(stream_.get()), "starting the view");
+        state_ = {};
+        kernel_seconds_ = 0;
+        seconds_per_pass_ = 0;
+        has_view_ = true;
+    }
+
+    // One launch of the view's work, timed on the GPU
+    double launch(uint64_t seed, const dev::launch_work& work)
+    {
+        start_.record(stream_.get());
+        if (use_wavefront_)
+        {
+            wavefront_->render(seed, work, sums_.data(), rays_.data());
+        }
+        else
+        {
+            check(launch_megakernel<Real>(all_books_, cam_, dscene_.view(), seed, work, sums_.data(), rays_.data(),
+                                          block_w_, block_h_, stream_.get()),
+                  "launching the kernel");
+        }
+        stop_.record(stream_.get());
+        stop_.synchronize();
+        const double seconds = hip_event::seconds_between(start_, stop_);
+        kernel_seconds_ += seconds;
+        return seconds;
+    }
+};
+}
+
+std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_options& options)
+{
+    validate_scene(scene);
+    if (options.precision != RT_FLOAT32 && options.precision != RT_FLOAT64)
+    {
+        throw std::invalid_argument("unknown precision");
+    }
+    if (options.integrator < RT_INTEGRATOR_AUTO || options.integrator > RT_INTEGRATOR_WAVEFRONT)
+    {
+        throw std::invalid_argument("unknown integrator");
+    }
+    if (options.launch_seconds < 0 || options.launch_seconds > 0.5)
+    {
+        throw std::invalid_argument("launch_seconds must be in (0, 0.5]");
+    }
+    if (options.block_size != 0 && (options.block_size < 64 || options.block_size > 256 || options.block_size % 64 != 0))
+    {
+        throw std::invalid_argument("the block size must be 64, 128, 192 or 256 threads");
+    }
+    if (scene.mesh_count != 0 || scene.microfacet_count != 0 || scene.environment != nullptr ||
+        scene.combination != RT_COMBINE_BSDF_ONLY)
+    {
+        if (options.precision != RT_FLOAT32)
+        {
+            throw std::invalid_argument("triangles, GGX, environment maps and light samples render in float32 only");
+        }
+    }
+    if (options.precision == RT_FLOAT64)
+    {
+        return std::make_unique<session_impl<double>>(scene, options);
+    }
+    return std::make_unique<session_impl<float>>(scene, options);
+}
+}

@@ -1,0 +1,43 @@
+// A render that goes on (gpu::session, src/gpu/renderer.h:568-673): the scene stays in the GPU's memory, and the image
+// of one view gains passes on demand. Host code; it reaches the GPU through the HIP runtime API and the launch
+// functions of kernels/kernel_api.hpp.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/raytracing_amd.h"
+#include "scene_tables.hpp"
+
+namespace rt
+{
+struct accumulation
+{
+    int64_t passes = 0;
+    int32_t rows = 0;
+};
+
+class session
+{
+public:
+    virtual ~session() = default;
+    virtual void set_camera(const rt_camera_desc& camera) = 0;
+    virtual rt_render_report render_to(int64_t passes, int32_t rows) = 0;
+    virtual rt_render_report render_for(double seconds, int64_t pass_limit) = 0;
+    virtual accumulation state() const = 0;
+    virtual void pixels(double* out, std::size_t count) = 0;
+    virtual void sums(double* out, std::size_t count) = 0;
+    virtual void display(uint8_t* out, std::size_t count) = 0;
+    virtual void counters(uint64_t* rays, uint64_t* shadow_rays, double* kernel_seconds) const = 0;
+    virtual rt_kernel_info kernel() const = 0;
+    virtual void closest_hit(const float* origins, const float* directions, const float* times, const int32_t* from,
+                             const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) = 0;
+    virtual std::size_t device_bytes() const = 0;
+    virtual rt_traffic traffic(int64_t passes) = 0;
+};
+
+// Validates the scene (before any GPU work), then builds its BVH and uploads it
+std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_options& options);
+}

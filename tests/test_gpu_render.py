@@ -1,0 +1,156 @@
+"""GPU parity tests: the HIP kernels through the C ABI against the oracle.
+
+- closest-hit ids, distances and sides of seeded ray sets, bit for bit against the GPU-semantics twin;
+- the furnace scene (src/gpu_scenes.ixx:33-55): every pixel exactly 1.0, in every kernel and precision;
+- ImageCompare of GPU renders against the CPU restatement of the reference renderer, with the reference's re-run rule
+  (scripts/verify/compare.ps1: a second, independent pair of seeds decides when the first pair fails);
+- byte reproducibility: a fixed seed gives the same sums whatever the launch grouping or session history.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import image_compare
+from tests.rays import scene_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare_with_rerun(name, width, spp, sampling=None, precision="float32", integrator="auto", depth=0):
+    import raytracing_amd as rt
+
+    samp = -1 if sampling is None else rt.SAMPLING[sampling]
+    verdicts = []
+    for cpu_seed, gpu_seed in ((0, 1), (2, 3)):
+        scene = rt.build_scene(name, width=width, samples=spp, sampling=sampling, max_depth=depth, earth_width=512)
+        gpu = rt.render(scene, seed=gpu_seed, precision=precision, integrator=integrator).pixels
+        cpu, _ = oracle.render(name, width=width, spp=spp, seed=cpu_seed, sampling=samp, max_depth=depth,
+                               earth_width=512)
+        v = image_compare.compare(cpu, gpu)
+        verdicts.append(v)
+        print(f"{name} {width}w {spp}spp {precision} {integrator}: CPU {cpu.mean():.6f} GPU {gpu.mean():.6f}: "
+              f"{v.summary()}", flush=True)
+        if v.consistent:
+            return v
+    raise AssertionError(f"{name}: GPU and CPU differ beyond noise twice: " + " | ".join(v.summary() for v in verdicts))
+
+
+@pytest.mark.parametrize("name", ["weekend", "bouncing", "glass", "final", "cornell", "smoke", "light"])
+def test_closest_hit_bit_exact(rt, name):
+    scene = rt.build_scene(name, width=64, samples=16, earth_width=512)
+    o, d, tm, f, inw = scene_rays(scene, 60000, seed=zlib.crc32(name.encode()) % 1000, width=64, spp=16, earth_width=512)
+    with rt.Session(scene, integrator="megakernel") as s:
+        ids, t, ent = s.closest_hit(o, d, tm, f, inw)
+    tids, tt, tent = oracle.twin_closest_hit(name, o, d, tm, f, inw, 64, 16, 512)
+    assert (tids >= 0).sum() > len(tids) // 4, "the ray set should hit something"
+    mism = np.nonzero((ids != tids) | (t.view(np.uint32) != tt.view(np.uint32)) | (ent != tent))[0]
+    assert len(mism) == 0, f"{len(mism)} of {len(ids)} rays differ, first {mism[:5]}: gpu {ids[mism[:5]]} {t[mism[:5]]} twin {tids[mism[:5]]} {tt[mism[:5]]}"
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+@pytest.mark.parametrize("integrator", ["megakernel", "auto"])
+def test_furnace_exactly_one(rt, precision, integrator):
+    scene = rt.build_scene("furnace", width=240, samples=16)
+    img = rt.render(scene, seed=7, precision=precision, integrator=integrator).pixels
+    assert np.all(img == 1.0), f"{np.count_nonzero(img != 1.0)} values differ from 1, e.g. {img[img != 1.0][:5]}"
+
+
+def test_weekend_config0_vs_cpu(rt):
+    """Config 0: weekend, 400x225, 64 spp, depth 50, against the CPU render."""
+    _compare_with_rerun("weekend", 400, 64)
+
+
+@pytest.mark.parametrize("name,width,spp,sampling", [
+    ("materials", 160, 64, None), ("bouncing", 160, 64, None), ("checker", 160, 64, None), ("earth", 160, 64, None),
+    ("perlin", 160, 64, None), ("quads", 120, 64, None), ("light", 160, 64, None), ("cornell", 120, 64, None),
+    ("smoke", 120, 64, None), ("final", 120, 64, None), ("aluminum", 120, 64, None), ("glass", 120, 64, None),
+    ("glass", 120, 64, "material"), ("cornell", 120, 64, "lights"), ("cornell", 120, 64, "uniform"),
+    ("final", 120, 64, "material"),
+])
+def test_scene_vs_cpu(rt, name, width, spp, sampling):
+    _compare_with_rerun(name, width, spp, sampling)
+
+
+def test_fp64_vs_cpu(rt):
+    _compare_with_rerun("final", 100, 64, precision="float64")
+    _compare_with_rerun("weekend", 160, 64, precision="float64")
+
+
+def test_same_seed_same_bytes(rt):
+    scene = rt.build_scene("final", width=96, samples=16, earth_width=512)
+    with rt.Session(scene, seed=5, launch_seconds=0.5) as a:
+        a.render_to(16)
+        sa = a.sums()
+    with rt.Session(scene, seed=5, launch_seconds=0.0001) as b:
+        for p in (1, 2, 5, 9, 16):
+            b.render_to(p)
+        sb = b.sums()
+    with rt.Session(scene, seed=5) as c:
+        c.render_to(7, 40)  # a band of rows of pass 7
+        c.render_to(16)
+        sc = c.sums()
+    assert np.array_equal(sa.view(np.uint64), sb.view(np.uint64))
+    assert np.array_equal(sa.view(np.uint64), sc.view(np.uint64))
+    with rt.Session(scene, seed=6) as d:
+        d.render_to(16)
+        assert not np.array_equal(sa, d.sums())
+
+
+def test_rounds_of_strata(rt):
+    """More passes than one grid: round r draws seed + r * 2^32 (src/gpu/renderer.h:564-584)"""
+    scene = rt.build_scene("weekend", width=64, samples=50, strata=4)
+    assert scene.passes == 50 and scene.desc.camera.sqrt_spp == 4
+    r = rt.render(scene, seed=0)
+    with rt.Session(scene, seed=0) as s:
+        s.render_to(16)
+        first_round = s.sums()
+        s.render_to(50)
+        assert np.array_equal(s.pixels(), r.pixels)
+    with rt.Session(scene, seed=1 << 32) as s2:
+        s2.render_to(16)  # round 1 of seed 0 is round 0 of seed 2^32
+        round1 = s2.sums()
+    with rt.Session(scene, seed=0) as s3:
+        s3.render_to(32)
+        assert np.allclose(s3.sums(), first_round + round1, rtol=0, atol=1e-12)
+
+
+def test_session_errors(rt):
+    scene = rt.build_scene("glass", width=32, samples=4)
+    s = rt.Session(scene)
+    with pytest.raises(rt.RenderLogicError):
+        s.pixels()
+    s.render_to(2)
+    with pytest.raises(ValueError):
+        s.render_to(1)
+    cam = rt.CameraDesc.from_buffer_copy(scene.desc.camera)
+    cam.sampling = rt.SAMPLING["material"]
+    with pytest.raises(ValueError):
+        s.set_camera(cam)
+    s.close()
+
+
+def test_display_bytes(rt):
+    scene = rt.build_scene("weekend", width=80, samples=4)
+    with rt.Session(scene) as s:
+        s.render_to(4)
+        px = s.pixels()
+        rgba = s.display()
+    expect = (256 * np.clip(np.sqrt(np.maximum(px, 0)), 0, 0.999)).astype(np.uint8)
+    assert np.array_equal(rgba[:, :, :3], expect)
+    assert np.all(rgba[:, :, 3] == 255)
+
+
+def test_traffic_counts(rt):
+    scene = rt.build_scene("final", width=128, samples=16, earth_width=512)
+    with rt.Session(scene, integrator="megakernel") as s:
+        t = s.traffic(4)
+        rays, _, _ = s.counters()
+    assert t.paths == 4 * 128 * 128
+    assert t.rays > t.paths
+    events = dict(zip(rt.TRAFFIC_EVENTS, list(t.events)))
+    assert events["node"] > t.rays  # several nodes per ray
+    assert events["medium"] >= t.rays  # final's fog is tested by every ray
+    assert 100 < t.bytes_per_path < 1e6
+    assert rays == 0  # the view's counters are untouched
