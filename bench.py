@@ -68,7 +68,16 @@ def main():
     ap.add_argument("--width", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-megakernel", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run the plain copy kernel over this many bytes (the HBM counters' calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import raytracing_amd as rt
+
+        seconds = rt.copy_probe(args.copy_probe, 5)
+        print(json.dumps({"copy_bytes": args.copy_probe, "seconds": seconds,
+                          "GB/s": 2 * args.copy_probe / seconds / 1e9}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} asked)",
               file=sys.stderr)

@@ -477,6 +477,10 @@ typedef struct rt_device_info
 /* gpu::query_device (src/gpu/renderer.h:516): describes device 0 and starts the runtime on it */
 int rt_query_device(rt_device_info* info);
 
+/* A device-to-device copy of `bytes` bytes with a plain dwordx4 copy kernel, `repeats` times, and its average kernel
+ * seconds: the calibration the HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled by. No counterpart in the reference. */
+int rt_copy_probe(uint64_t bytes, int32_t repeats, double* seconds);
+
 #ifdef __cplusplus
 }
 #endif

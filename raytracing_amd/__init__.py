@@ -208,6 +208,7 @@ _SIGNATURES = {
     "rt_server_pixels": (C.c_int, [C.c_void_p, C.c_int64, _P(C.c_double), C.c_size_t]),
     "rt_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
     "rt_query_device": (C.c_int, [_P(DeviceInfo)]),
+    "rt_copy_probe": (C.c_int, [C.c_uint64, C.c_int32, _P(C.c_double)]),
 }
 for _name, (_res, _args) in _SIGNATURES.items():
     _f = getattr(_lib, _name)
@@ -414,6 +415,13 @@ def query_device() -> DeviceInfo:
     info = DeviceInfo()
     _check(_lib.rt_query_device(C.byref(info)))
     return info
+
+
+def copy_probe(nbytes: int, repeats: int = 5) -> float:
+    """Average seconds of a plain device-to-device copy kernel over `nbytes` (the HBM counters' calibration)."""
+    seconds = C.c_double()
+    _check(_lib.rt_copy_probe(nbytes, repeats, C.byref(seconds)))
+    return seconds.value
 
 
 class Server:

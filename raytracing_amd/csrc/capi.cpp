@@ -9,6 +9,7 @@
 
 #include "../../include/raytracing_amd.h"
 #include "bvh_build.hpp"
+#include "kernels/kernel_api.hpp"
 #include "device_scene.hpp"
 #include "scene_build.hpp"
 #include "server.hpp"
@@ -404,4 +405,35 @@ int rt_query_device(rt_device_info* info)
         rt::check(hipFree(nullptr), "starting the runtime");
     });
 }
+}
+
+extern "C" int rt_copy_probe(uint64_t bytes, int32_t repeats, double* seconds)
+{
+    return guarded([&] {
+        need(seconds, "seconds");
+        if (bytes < 16 || bytes % 16 != 0 || repeats < 1)
+        {
+            throw std::invalid_argument("a copy of a positive multiple of 16 bytes, at least once");
+        }
+        rt::device_array<uint8_t> in(bytes), out(bytes);
+        hipStream_t stream = nullptr;
+        rt::check(hipStreamCreate(&stream), "creating a stream");
+        hipEvent_t start = nullptr, stop = nullptr;
+        rt::check(hipEventCreate(&start), "creating an event");
+        rt::check(hipEventCreate(&stop), "creating an event");
+        rt::check(hipMemsetAsync(in.data(), 1, bytes, stream), "filling the source");
+        rt::check(hipEventRecord(start, stream), "recording an event");
+        for (int r = 0; r < repeats; ++r)
+        {
+            rt::check(rt::launch_copy(in.data(), out.data(), bytes, stream), "launching the copy");
+        }
+        rt::check(hipEventRecord(stop, stream), "recording an event");
+        rt::check(hipEventSynchronize(stop), "copying");
+        float ms = 0;
+        rt::check(hipEventElapsedTime(&ms, start, stop), "timing the copy");
+        *seconds = ms / 1000.0 / repeats;
+        (void)hipEventDestroy(start);
+        (void)hipEventDestroy(stop);
+        (void)hipStreamDestroy(stream);
+    });
 }

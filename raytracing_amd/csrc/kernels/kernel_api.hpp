@@ -34,6 +34,8 @@ hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origi
 hipError_t launch_display(const double* sums, int64_t pixels, double inverse_passes, uint8_t* rgba,
                           hipStream_t stream);
 
+hipError_t launch_copy(const void* in, void* out, size_t bytes, hipStream_t stream);
+
 template <typename Real>
 kernel_attributes megakernel_attributes(bool all_books, int block_threads, int depth);
 }

@@ -240,3 +240,28 @@ template kernel_attributes megakernel_attributes<double>(bool, int, int);
 template size_t stack_lds_bytes<float>(int, int);
 template size_t stack_lds_bytes<double>(int, int);
 }
+
+namespace rt::dev
+{
+// A plain copy, one 16-byte word per thread per step: the calibration kernel of the HBM counters
+__global__ void __launch_bounds__(256) copy_kernel(const uint4* __restrict__ in, uint4* __restrict__ out, size_t n)
+{
+    for (size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+         k += static_cast<size_t>(gridDim.x) * blockDim.x)
+    {
+        out[k] = in[k];
+    }
+}
+}
+
+namespace rt
+{
+hipError_t launch_copy(const void* in, void* out, size_t bytes, hipStream_t stream)
+{
+    const size_t n = bytes / 16;
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 256 * 64));
+    hipLaunchKernelGGL(dev::copy_kernel, dim3(blocks), dim3(256), 0, stream, static_cast<const uint4*>(in),
+                       static_cast<uint4*>(out), n);
+    return hipGetLastError();
+}
+}

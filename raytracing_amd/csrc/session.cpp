@@ -405,7 +405,15 @@ private:
         start_.record(stream_.get());
         if (use_wavefront_)
         {
-            wavefront_->render(seed, work, sums_.data(), rays_.data());
+            // a pool smaller than the launch's rows takes them in bands: each band's pixels are its own
+            const int band = wavefront_->rows_capacity();
+            for (int row = work.first_row; row < work.first_row + work.row_count; row += band)
+            {
+                dev::launch_work part = work;
+                part.first_row = row;
+                part.row_count = std::min(band, work.first_row + work.row_count - row);
+                wavefront_->render(seed, part, sums_.data(), rays_.data());
+            }
         }
         else
         {

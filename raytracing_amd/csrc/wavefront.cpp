@@ -1,2 +1,148 @@
-// The wavefront integrator's host driver: see wavefront.hpp
+// The wavefront integrator's host driver (see wavefront.hpp)
 #include "wavefront.hpp"
+
+#include <algorithm>
+
+namespace rt
+{
+namespace
+{
+int blocks_for(const kernel_attributes& a)
+{
+    int cus = 0;
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "reading the device's properties");
+    return std::max(1, cus * std::max(1, a.blocks_per_cu));
+}
+
+class graph_owner
+{
+public:
+    ~graph_owner()
+    {
+        if (exec)
+        {
+            (void)hipGraphExecDestroy(exec);
+        }
+        if (graph)
+        {
+            (void)hipGraphDestroy(graph);
+        }
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+}
+
+template <typename Real>
+wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, bool, int slots,
+                           hipStream_t stream)
+    : stream_(stream)
+{
+    if constexpr (!wavefront_supported<Real>())
+    {
+        throw std::invalid_argument("the wavefront integrator renders in float32 only");
+    }
+    else
+    {
+        sc_ = sc;
+        cam_ = cam;
+        const uint64_t pixels = static_cast<uint64_t>(cam.image_width) * static_cast<uint64_t>(cam.image_height);
+        const uint64_t want = slots > 0 ? static_cast<uint64_t>(slots) : pixels;
+        if (want < static_cast<uint64_t>(cam.image_width))
+        {
+            throw std::invalid_argument("the wavefront pool needs at least one row of slots");
+        }
+        if (pixels > 0x7fffffffull)
+        {
+            throw std::invalid_argument("the image has too many pixels for the wavefront pool");
+        }
+        // whole rows: a launch's band of rows must fit the pool
+        capacity_ = static_cast<uint32_t>(std::min(want, pixels) / cam.image_width * cam.image_width);
+        ray_o_ = device_array<float4>(capacity_);
+        ray_d_ = device_array<float4>(capacity_);
+        thr_ = device_array<float4>(capacity_);
+        rng_ = device_array<uint4>(capacity_);
+        hit_ = device_array<int4>(capacity_);
+        pass_ = device_array<int32_t>(capacity_);
+        active0_ = device_array<uint32_t>(capacity_);
+        active1_ = device_array<uint32_t>(capacity_);
+        queue_ = device_array<uint32_t>(static_cast<std::size_t>(capacity_) * dev::wf_classes);
+        counters_ = device_array<uint32_t>(2 * dev::wf_counter_stride);
+        check(hipMemsetAsync(counters_.data(), 0, counters_.bytes(), stream_), "clearing counters");
+        check(hipHostMalloc(reinterpret_cast<void**>(&host_count_), sizeof(uint32_t), hipHostMallocDefault),
+              "allocating pinned memory");
+        pool_ = {ray_o_.data(), ray_d_.data(), thr_.data(), rng_.data(), hit_.data(), pass_.data(),
+                 {active0_.data(), active1_.data()}, queue_.data(), counters_.data(), capacity_};
+        bytes_ = ray_o_.bytes() * 3 + rng_.bytes() + hit_.bytes() + pass_.bytes() + active0_.bytes() * 2 +
+                 queue_.bytes() + counters_.bytes();
+        extend_blocks_ = blocks_for(wf_attributes(sc.stack_depth, 0));
+        shade_blocks_ = blocks_for(wf_attributes(sc.stack_depth, 1));
+    }
+}
+
+template <typename Real>
+wavefront<Real>::~wavefront()
+{
+    if (host_count_)
+    {
+        (void)hipHostFree(host_count_);
+    }
+}
+
+template <typename Real>
+void wavefront<Real>::render(uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays)
+{
+    if constexpr (wavefront_supported<Real>())
+    {
+        if (static_cast<uint64_t>(work.row_count) * static_cast<uint64_t>(cam_.image_width) > capacity_)
+        {
+            throw std::invalid_argument("the launch's rows don't fit the wavefront pool");
+        }
+        check(wf_launch_init(cam_, sc_, seed, work, pool_, stream_), "launching the wavefront's first paths");
+        // the bounce loop: a graph of an even number of bounces (parity 0, 1, 0, 1...), replayed until a replay ends
+        // with no active slot (the count of the next parity-0 list)
+        graph_owner g;
+        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capturing the bounce loop");
+        hipError_t launched = hipSuccess;
+        for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
+        {
+            launched = wf_launch_bounce(cam_, sc_, seed, work, pool_, b & 1, sums, rays, extend_blocks_,
+                                        shade_blocks_, stream_);
+        }
+        const hipError_t ended = hipStreamEndCapture(stream_, &g.graph);
+        check(launched, "launching the bounce loop's kernels");
+        check(ended, "capturing the bounce loop");
+        check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "instantiating the bounce loop");
+        // a path is at most max_depth rays, and a slot traces pass_count paths: that bounds the replays
+        const long long max_bounces = static_cast<long long>(cam_.max_depth) * work.pass_count + bounces_per_graph_;
+        for (long long done = 0;; done += bounces_per_graph_)
+        {
+            if (done > max_bounces)
+            {
+                throw std::runtime_error("the wavefront loop didn't drain");
+            }
+            check(hipGraphLaunch(g.exec, stream_), "launching the bounce loop");
+            check(hipMemcpyAsync(host_count_, counters_.data(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+                  "reading the active count");
+            check(hipStreamSynchronize(stream_), "rendering on the GPU");
+            if (*host_count_ == 0)
+            {
+                break;
+            }
+        }
+    }
+}
+
+template <typename Real>
+kernel_attributes wavefront<Real>::attributes() const
+{
+    kernel_attributes a = wf_attributes(sc_.stack_depth, 0);
+    const kernel_attributes s = wf_attributes(sc_.stack_depth, 1);
+    a.registers = std::max(a.registers, s.registers);
+    a.local_bytes = std::max(a.local_bytes, s.local_bytes);
+    return a;
+}
+
+template class wavefront<float>;
+template class wavefront<double>;
+}

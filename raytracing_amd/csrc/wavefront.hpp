@@ -1,32 +1,59 @@
-// The wavefront integrator's host driver (see wavefront.cpp)
+// The wavefront integrator's host driver: the pool of path slots in device memory, and the bounce loop, captured in a
+// hipGraph of several bounces that is replayed until no slot is active (kernels in kernels/wavefront.hip)
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 
 #include <hip/hip_runtime.h>
 
+#include "device_scene.hpp"
 #include "device_types.hpp"
 #include "kernels/kernel_api.hpp"
+#include "kernels/wavefront_api.hpp"
 
 namespace rt
 {
 template <typename Real>
 constexpr bool wavefront_supported()
 {
-    return false;
+    return sizeof(Real) == sizeof(float);
 }
 
 template <typename Real>
 class wavefront
 {
 public:
-    wavefront(const dev::scene_view<Real>&, const dev::camera_rec<Real>&, bool, int, hipStream_t)
-    {
-        throw std::invalid_argument("the wavefront integrator is not built in this library");
-    }
-    void render(uint64_t, const dev::launch_work&, double*, unsigned long long*) {}
-    kernel_attributes attributes() const { return {}; }
-    std::size_t bytes() const { return 0; }
+    // slots: the pool's size in paths (0: one per pixel of the image)
+    wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, bool all_books, int slots,
+              hipStream_t stream);
+    ~wavefront();
+    wavefront(const wavefront&) = delete;
+    wavefront& operator=(const wavefront&) = delete;
+
+    // The launch's work, added to sums; waits until it is done
+    void render(uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays);
+    kernel_attributes attributes() const;
+    std::size_t bytes() const { return bytes_; }
+    int bounces_per_graph() const { return bounces_per_graph_; }
+    // How many image rows one render call may cover
+    int rows_capacity() const { return static_cast<int>(capacity_ / static_cast<uint32_t>(cam_.image_width)); }
+
+private:
+    dev::scene_view<float> sc_{};
+    dev::camera_rec<float> cam_{};
+    hipStream_t stream_ = nullptr;
+    uint32_t capacity_ = 0;
+    device_array<float4> ray_o_, ray_d_, thr_;
+    device_array<uint4> rng_;
+    device_array<int4> hit_;
+    device_array<int32_t> pass_;
+    device_array<uint32_t> active0_, active1_, queue_, counters_;
+    uint32_t* host_count_ = nullptr; // pinned: the active count after each replay
+    dev::wf_pool pool_{};
+    int extend_blocks_ = 0, shade_blocks_ = 0;
+    int bounces_per_graph_ = 8;
+    std::size_t bytes_ = 0;
 };
 }

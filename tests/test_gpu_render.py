@@ -154,3 +154,36 @@ def test_traffic_counts(rt):
     assert events["medium"] >= t.rays  # final's fog is tested by every ray
     assert 100 < t.bytes_per_path < 1e6
     assert rays == 0  # the view's counters are untouched
+
+
+@pytest.mark.parametrize("name", ["weekend", "final", "glass", "cornell", "smoke", "bouncing", "earth", "perlin",
+                                  "light", "materials"])
+def test_wavefront_equals_megakernel(rt, name):
+    """The two integrators trace the same paths with the same random numbers: the same sums, bit for bit, and the
+    same ray counts"""
+    scene = rt.build_scene(name, width=96, samples=16, earth_width=512)
+    with rt.Session(scene, seed=3, integrator="megakernel") as a:
+        a.render_to(16)
+        sa = a.sums()
+        ra = a.counters()[0]
+    with rt.Session(scene, seed=3, integrator="wavefront") as b:
+        assert b.kernel().name.decode().startswith("wavefront")
+        b.render_to(5)
+        b.render_to(16)
+        sb = b.sums()
+        rb = b.counters()[0]
+    assert ra == rb
+    mism = np.count_nonzero(sa.view(np.uint64) != sb.view(np.uint64))
+    assert mism == 0, f"{mism} of {sa.size} sums differ; max |diff| {np.abs(sa - sb).max()}"
+
+
+def test_wavefront_small_pool(rt):
+    """A pool smaller than the image: the launches go in bands of rows, and the image is the same"""
+    scene = rt.build_scene("final", width=64, samples=16, earth_width=512)
+    with rt.Session(scene, seed=2, integrator="megakernel") as a:
+        a.render_to(16)
+        sa = a.sums()
+    with rt.Session(scene, seed=2, integrator="wavefront", wavefront_slots=64 * 10) as b:
+        b.render_to(16)
+        sb = b.sums()
+    assert np.array_equal(sa.view(np.uint64), sb.view(np.uint64))
