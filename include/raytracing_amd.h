@@ -404,17 +404,32 @@ int rt_session_closest_hit(rt_session* session, const float* origins, const floa
 int rt_scene_bvh(const rt_scene* scene, float* node_boxes, int32_t* node_ints, int64_t* node_count,
                  int32_t* sphere_ids, int32_t* quad_ids, int32_t* depth);
 
+/* The quality integrator's light set as the kernels read it, on the host (build_lamps, src/gpu/lamps.cuh:159 and
+ * src/gpu/lamp_table.cu:60-119): the scene's glowing quads and still spheres, 80 bytes each (20 floats: corner[3],
+ * size, u[3], kind, v[3], primitive, normal[3], 0, radiance[3], 0; kind and primitive are int32 bit patterns). Call
+ * with lamps NULL to learn the count. */
+int rt_scene_lamps(const rt_scene* scene, float* lamps, int64_t* count);
+
+/* An environment map's tables, on the host (build_environment, src/gpu/environment_data.h:58 and
+ * src/gpu/environment_table.cu:139-225): per texel r, g, b and the density per steradian of its cell's directions;
+ * the row borders' cosines (height + 1); the alias table's coins and aliases (2^log2_bins bins, none for a black map);
+ * and S, the map's luminance over the sphere. Call with texels NULL to learn log2_bins alone. */
+int rt_environment_tables(const rt_environment_desc* environment, float* texels, float* row_cos, float* bin_q,
+                          int32_t* bin_alias, int32_t* log2_bins, double* total);
+
 /* The scene data the view's paths read, counted by an instrumented instance of the megakernel over passes
  * [0, passes) of the view's first round, into scratch sums (the view's image and counters don't change). events[k] counts
  * reads of one kind of record (in order: wide BVH nodes, spheres, moving spheres, quads, media, media boundary quads,
- * hits, sphere frames, textures, texels, Perlin gradients, lights) and event_bytes[k] is that record's size; bytes_per_path
- * is their weighted sum per path, plus the double sums' read and write of each pixel once per pass. This is the
- * algorithmic traffic the roofline is measured with. No counterpart in the reference. */
-#define RT_TRAFFIC_EVENTS 12
+ * hits, sphere frames, textures, texels, Perlin gradients, the books' lights, triangles, lamps of the light set,
+ * environment texels, alias bins) and event_bytes[k] is that record's size; bytes_per_path is their weighted sum per
+ * path, plus the double sums' read and write of each pixel once per launch. This is the algorithmic traffic the
+ * roofline is measured with. No counterpart in the reference. */
+#define RT_TRAFFIC_EVENTS 16
 typedef struct rt_traffic
 {
     uint64_t paths;
     uint64_t rays;
+    uint64_t shadow_rays;
     uint64_t events[16];
     double event_bytes[16];
     double bytes_per_path;

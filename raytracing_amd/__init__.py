@@ -149,12 +149,12 @@ class RenderStats(C.Structure):
 
 
 class Traffic(C.Structure):
-    _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("events", C.c_uint64 * 16),
+    _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("events", C.c_uint64 * 16),
                 ("event_bytes", C.c_double * 16), ("bytes_per_path", C.c_double)]
 
 
 TRAFFIC_EVENTS = ["node", "sphere", "moving_sphere", "quad", "medium", "boundary_quad", "hit", "frame", "texture",
-                  "texel", "perlin", "light"]
+                  "texel", "perlin", "light", "triangle", "lamp", "env_texel", "alias_bin"]
 
 
 class ServerOptions(C.Structure):
@@ -200,6 +200,9 @@ _SIGNATURES = {
     "rt_session_traffic": (C.c_int, [C.c_void_p, C.c_int64, _P(Traffic)]),
     "rt_scene_bvh": (C.c_int, [_P(Scene), _P(C.c_float), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32),
                                _P(C.c_int32), _P(C.c_int32)]),
+    "rt_scene_lamps": (C.c_int, [_P(Scene), _P(C.c_float), _P(C.c_int64)]),
+    "rt_environment_tables": (C.c_int, [_P(EnvironmentDesc), _P(C.c_float), _P(C.c_float), _P(C.c_float),
+                                        _P(C.c_int32), _P(C.c_int32), _P(C.c_double)]),
     "rt_server_create": (C.c_int, [_P(ServerOptions), _P(C.c_void_p)]),
     "rt_server_destroy": (None, [C.c_void_p]),
     "rt_server_submit": (C.c_int, [C.c_void_p, _P(Scene), C.c_uint64, C.c_int64, _P(C.c_int64)]),
@@ -409,6 +412,47 @@ def scene_bvh(scene: BuiltScene):
                              sphere_ids.ctypes.data_as(_P(C.c_int32)), quad_ids.ctypes.data_as(_P(C.c_int32)),
                              C.byref(depth)))
     return boxes, ints, sphere_ids, quad_ids, depth.value
+
+
+LAMP_FIELDS = 20  # floats per lamp record: corner[3], size, u[3], kind, v[3], primitive, normal[3], 0, radiance[3], 0
+
+
+def scene_lamps(scene: BuiltScene) -> np.ndarray:
+    """The quality integrator's light set (lamps.cuh build_lamps), one row of 20 float32 per lamp (host only)."""
+    count = C.c_int64()
+    _check(_lib.rt_scene_lamps(C.byref(scene.desc), None, C.byref(count)))
+    out = np.zeros((count.value, LAMP_FIELDS), np.float32)
+    _check(_lib.rt_scene_lamps(C.byref(scene.desc), out.ctypes.data_as(_P(C.c_float)), C.byref(count)))
+    return out
+
+
+@dataclass
+class EnvironmentTables:
+    texels: np.ndarray   # (height, width, 4): r, g, b, density per steradian
+    row_cos: np.ndarray  # (height + 1,)
+    bin_q: np.ndarray    # (2^log2_bins,) coins
+    bin_alias: np.ndarray
+    log2_bins: int
+    total: float
+
+
+def environment_tables(rgb: np.ndarray, yaw_degrees: float = 0.0, scale: float = 1.0) -> EnvironmentTables:
+    """An environment map's host tables (environment_data.h build_environment) for an (H, W, 3) float32 image."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    env = EnvironmentDesc(ImageDesc(w, h, rgb.ctypes.data_as(_P(C.c_float))), yaw_degrees, scale)
+    log2 = C.c_int32()
+    total = C.c_double()
+    _check(_lib.rt_environment_tables(C.byref(env), None, None, None, None, C.byref(log2), C.byref(total)))
+    bins = (1 << log2.value) if log2.value > 0 else 0
+    texels = np.zeros((h, w, 4), np.float32)
+    row_cos = np.zeros(h + 1, np.float32)
+    q = np.zeros(max(bins, 1), np.float32)
+    alias = np.zeros(max(bins, 1), np.int32)
+    _check(_lib.rt_environment_tables(C.byref(env), texels.ctypes.data_as(_P(C.c_float)),
+                                      row_cos.ctypes.data_as(_P(C.c_float)), q.ctypes.data_as(_P(C.c_float)),
+                                      alias.ctypes.data_as(_P(C.c_int32)), C.byref(log2), C.byref(total)))
+    return EnvironmentTables(texels, row_cos, q[:bins], alias[:bins], log2.value, total.value)
 
 
 def query_device() -> DeviceInfo:

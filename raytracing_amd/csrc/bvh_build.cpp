@@ -2,7 +2,9 @@
 #include "bvh_build.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <limits>
 #include <numeric>
 #include <span>
 #include <stdexcept>
@@ -113,6 +115,13 @@ public:
                                scene.quads[i].material, {}, quad_box(scene.quads[i])});
             quads_.push_back(make_quad_rec<Real>(scene.quads[i]));
         }
+        out_.triangle_base = static_cast<int32_t>(inputs_.size());
+        for (const world_triangle& w : world_triangles(scene))
+        {
+            inputs_.push_back({dev::prim_triangle, static_cast<int>(triangles_.size()), -1, w.material, {},
+                               triangle_box(w)});
+            triangles_.push_back(make_triangle_rec(w));
+        }
         out_.sphere_ids.resize(n_spheres);
         out_.quad_ids.resize(static_cast<std::size_t>(scene.quad_count));
     }
@@ -125,7 +134,23 @@ public:
         std::iota(order.begin(), order.end(), 0);
         if (!order.empty())
         {
-            add_subtree(order, 1);
+            if (triangles_.empty())
+            {
+                add_subtree(order, 1);
+            }
+            else
+            {
+                centroids_.resize(inputs_.size());
+                for (std::size_t i = 0; i < inputs_.size(); ++i)
+                {
+                    for (int a = 0; a < 3; ++a)
+                    {
+                        centroids_[i][a] = 0.5 * (inputs_[i].box.min[a] + inputs_[i].box.max[a]);
+                    }
+                }
+                out_.triangles.reserve(triangles_.size());
+                add_subtree_sah(order, 1);
+            }
             pack_wide();
         }
         return std::move(out_);
@@ -146,13 +171,60 @@ private:
     std::vector<dev::sphere_rec<Real>> spheres_;
     std::vector<dev::moving_sphere_rec<Real>> moving_;
     std::vector<dev::quad_rec<Real>> quads_;
+    std::vector<dev::triangle_rec<Real>> triangles_;
+    std::vector<std::array<double, 3>> centroids_;
     bvh_arrays<Real> out_;
 
     int array_size(int kind) const
     {
-        return kind == dev::prim_sphere ? static_cast<int>(out_.spheres.size())
+        return kind == dev::prim_sphere          ? static_cast<int>(out_.spheres.size())
                : kind == dev::prim_moving_sphere ? static_cast<int>(out_.moving_spheres.size())
-                                                 : static_cast<int>(out_.quads.size());
+               : kind == dev::prim_quad          ? static_cast<int>(out_.quads.size())
+                                                 : static_cast<int>(out_.triangles.size());
+    }
+
+    // A triangle's box in double, padded as a quad's (a flat box would let rounding in the slab test lose its hits)
+    static box_d triangle_box(const world_triangle& w)
+    {
+        box_d box;
+        double largest = 0;
+        for (const auto& p : w.p)
+        {
+            for (int a = 0; a < 3; ++a)
+            {
+                box.min[a] = std::min(box.min[a], p[a]);
+                box.max[a] = std::max(box.max[a], p[a]);
+                largest = std::max(largest, std::fabs(p[a]));
+            }
+        }
+        const double margin = 1e-6 * (1 + largest);
+        for (int a = 0; a < 3; ++a)
+        {
+            box.min[a] -= margin;
+            box.max[a] += margin;
+            if (box.max[a] - box.min[a] < 0.0001)
+            {
+                box.min[a] -= 0.0001 / 2;
+                box.max[a] += 0.0001 / 2;
+            }
+        }
+        return box;
+    }
+
+    // v0, the edges from it and the unit normal, in double from the world vertices, each rounded once
+    static dev::triangle_rec<Real> make_triangle_rec(const world_triangle& w)
+    {
+        const v3d p0{w.p[0][0], w.p[0][1], w.p[0][2]};
+        const v3d e1{w.p[1][0] - p0.x, w.p[1][1] - p0.y, w.p[1][2] - p0.z};
+        const v3d e2{w.p[2][0] - p0.x, w.p[2][1] - p0.y, w.p[2][2] - p0.z};
+        const v3d n = cross(e1, e2);
+        const double length = std::sqrt(dot(n, n));
+        dev::triangle_rec<Real> t{};
+        store(t.v0, p0);
+        store(t.e1, e1);
+        store(t.e2, e2);
+        store(t.normal, length > 0 ? scale(1 / length, n) : v3d{0, 0, 0});
+        return t;
     }
 
     void add_to_leaf(const input& in)
@@ -168,6 +240,11 @@ private:
             id = out_.moving_base + static_cast<int>(out_.moving_spheres.size());
             out_.moving_spheres.push_back(moving_[in.index]);
             break;
+        case dev::prim_triangle:
+            id = out_.triangle_base + static_cast<int>(out_.triangles.size());
+            out_.triangles.push_back(triangles_[in.index]);
+            out_.prim_materials[id] = in.material;
+            return;
         default:
             id = out_.quad_base + static_cast<int>(out_.quads.size());
             out_.quads.push_back(quads_[in.index]);
@@ -231,6 +308,168 @@ private:
         }
         add_subtree(indices.first(middle), level + 1);
         out_.nodes[index].index = add_subtree(indices.subspan(middle), level + 1);
+        return index;
+    }
+
+    static double half_area(const box_d& b)
+    {
+        const double x = b.max[0] - b.min[0], y = b.max[1] - b.min[1], z = b.max[2] - b.min[2];
+        return x * y + y * z + z * x;
+    }
+
+    // The tree of a scene with triangles: binned surface-area-heuristic splits (16 bins on each axis of the
+    // centroids' box), leaves of one kind and at most sah_leaf primitives where a split wouldn't pay. Below
+    // sah_levels levels, object-median splits bound the depth (and the traversal stack).
+    static constexpr int sah_bins = 16;
+    static constexpr int sah_leaf = 8;
+    static constexpr int sah_levels = 40;
+    static constexpr int max_levels = 64;
+
+    int add_subtree_sah(std::span<int> indices, int level)
+    {
+        if (level > max_levels)
+        {
+            throw std::runtime_error("the BVH is too deep for the traversal stack");
+        }
+        out_.depth = std::max(out_.depth, level);
+        box_d box, centers;
+        bool one_kind = true;
+        for (const int i : indices)
+        {
+            box.enclose(inputs_[i].box);
+            one_kind = one_kind && inputs_[i].kind == inputs_[indices[0]].kind;
+            for (int a = 0; a < 3; ++a)
+            {
+                centers.min[a] = std::min(centers.min[a], centroids_[i][a]);
+                centers.max[a] = std::max(centers.max[a], centroids_[i][a]);
+            }
+        }
+        const auto index = static_cast<int>(out_.nodes.size());
+        ref_node<Real> node{};
+        for (int k = 0; k < 3; ++k)
+        {
+            node.box_min[k] = round_down<Real>(box.min[k]);
+            node.box_max[k] = round_up<Real>(box.max[k]);
+        }
+        out_.nodes.push_back(node);
+        const std::size_t n = indices.size();
+        if (n == 1 || (one_kind && n <= 2))
+        {
+            return leaf(index, indices);
+        }
+
+        // the best plane over the three axes: cost = A_left N_left + A_right N_right, relative to the node's area
+        int best_axis = -1, best_bin = 0;
+        double best_cost = std::numeric_limits<double>::infinity();
+        if (level < sah_levels)
+        {
+            for (int a = 0; a < 3; ++a)
+            {
+                const double extent = centers.max[a] - centers.min[a];
+                if (!(extent > 0))
+                {
+                    continue;
+                }
+                const double to_bin = sah_bins / extent;
+                box_d bin_box[sah_bins];
+                std::size_t bin_count[sah_bins] = {};
+                for (const int i : indices)
+                {
+                    const int b = std::min(sah_bins - 1, static_cast<int>((centroids_[i][a] - centers.min[a]) * to_bin));
+                    bin_box[b].enclose(inputs_[i].box);
+                    ++bin_count[b];
+                }
+                double right_area[sah_bins];
+                std::size_t right_count[sah_bins];
+                box_d acc;
+                std::size_t count = 0;
+                for (int b = sah_bins - 1; b > 0; --b)
+                {
+                    acc.enclose(bin_box[b]);
+                    count += bin_count[b];
+                    right_area[b] = count ? half_area(acc) : 0;
+                    right_count[b] = count;
+                }
+                acc = box_d{};
+                count = 0;
+                for (int b = 0; b + 1 < sah_bins; ++b)
+                {
+                    acc.enclose(bin_box[b]);
+                    count += bin_count[b];
+                    if (count == 0 || right_count[b + 1] == 0)
+                    {
+                        continue;
+                    }
+                    const double cost = half_area(acc) * static_cast<double>(count) +
+                                        right_area[b + 1] * static_cast<double>(right_count[b + 1]);
+                    if (cost < best_cost)
+                    {
+                        best_cost = cost;
+                        best_axis = a;
+                        best_bin = b;
+                    }
+                }
+            }
+        }
+        const double parent = half_area(box);
+        const double leaf_cost = static_cast<double>(n);
+        const double split_cost = parent > 0 ? 1.0 + best_cost / parent : std::numeric_limits<double>::infinity();
+        if (n <= static_cast<std::size_t>(sah_leaf) && one_kind && (best_axis < 0 || leaf_cost <= split_cost))
+        {
+            return leaf(index, indices);
+        }
+        std::size_t middle = 0;
+        if (n <= static_cast<std::size_t>(sah_leaf) && !one_kind && (best_axis < 0 || leaf_cost <= split_cost))
+        {
+            // few primitives of several kinds: one subtree per kind, as the reference's builder splits a small leaf
+            int kind = 3;
+            for (const int i : indices)
+            {
+                kind = std::min(kind, inputs_[i].kind);
+            }
+            middle = static_cast<std::size_t>(
+                std::stable_partition(indices.begin(), indices.end(), [&](int i) { return inputs_[i].kind == kind; }) -
+                indices.begin());
+        }
+        else if (best_axis >= 0)
+        {
+            const double extent = centers.max[best_axis] - centers.min[best_axis];
+            const double to_bin = sah_bins / extent;
+            const auto split = std::partition(indices.begin(), indices.end(), [&](int i) {
+                const int b = std::min(sah_bins - 1,
+                                       static_cast<int>((centroids_[i][best_axis] - centers.min[best_axis]) * to_bin));
+                return b <= best_bin;
+            });
+            middle = static_cast<std::size_t>(split - indices.begin());
+        }
+        if (middle == 0 || middle == n)
+        {
+            // no plane separates the centroids (or below the SAH levels): halves by the longest axis
+            const int axis = longest_axis(centers.max[0] > centers.min[0] || centers.max[1] > centers.min[1] ||
+                                                  centers.max[2] > centers.min[2]
+                                              ? centers
+                                              : box);
+            middle = n / 2;
+            std::nth_element(indices.begin(), indices.begin() + static_cast<std::ptrdiff_t>(middle), indices.end(),
+                             [&](int a, int b) {
+                                 return centroids_[a][axis] < centroids_[b][axis] ||
+                                        (centroids_[a][axis] == centroids_[b][axis] && a < b);
+                             });
+        }
+        add_subtree_sah(indices.first(middle), level + 1);
+        out_.nodes[index].index = add_subtree_sah(indices.subspan(middle), level + 1);
+        return index;
+    }
+
+    int leaf(int index, std::span<int> indices)
+    {
+        const int kind = inputs_[indices[0]].kind;
+        out_.nodes[index].index = array_size(kind);
+        out_.nodes[index].count = static_cast<int32_t>(indices.size()) | kind << 8;
+        for (const int i : indices)
+        {
+            add_to_leaf(inputs_[i]);
+        }
         return index;
     }
 
@@ -356,6 +595,42 @@ dev::quad_rec<Real> make_quad_rec(const rt_quad_desc& d)
 }
 
 int leaf_size_for(std::size_t shapes) { return shapes <= 8 ? 8 : 1; }
+
+std::vector<world_triangle> world_triangles(const rt_scene& scene)
+{
+    std::vector<world_triangle> out;
+    const auto place = [&](const rt_mesh_desc& m, const double* transform) {
+        for (int64_t k = 0; k < m.triangle_count; ++k)
+        {
+            world_triangle w{};
+            w.material = m.material;
+            for (int c = 0; c < 3; ++c)
+            {
+                const float* p = m.positions + 3 * static_cast<std::size_t>(m.indices[3 * k + c]);
+                const double x = p[0], y = p[1], z = p[2];
+                for (int a = 0; a < 3; ++a)
+                {
+                    w.p[c][a] = transform ? transform[4 * a] * x + transform[4 * a + 1] * y + transform[4 * a + 2] * z +
+                                                transform[4 * a + 3]
+                                          : (a == 0 ? x : (a == 1 ? y : z));
+                }
+            }
+            out.push_back(w);
+        }
+    };
+    if (scene.instance_count == 0)
+    {
+        for (int64_t i = 0; i < scene.mesh_count; ++i)
+        {
+            place(scene.meshes[i], nullptr);
+        }
+    }
+    for (int64_t i = 0; i < scene.instance_count; ++i)
+    {
+        place(scene.meshes[scene.instances[i].mesh], scene.instances[i].object_to_world);
+    }
+    return out;
+}
 
 template <typename Real>
 bvh_arrays<Real> build_bvh(const rt_scene& scene, int max_leaf_size)

@@ -31,10 +31,12 @@ struct bvh_arrays
     std::vector<dev::sphere_rec<Real>> spheres;
     std::vector<dev::moving_sphere_rec<Real>> moving_spheres;
     std::vector<dev::quad_rec<Real>> quads;
+    std::vector<dev::triangle_rec<Real>> triangles;
     std::vector<int32_t> prim_materials;
     std::vector<dev::sphere_frame<Real>> sphere_frames;
     int32_t moving_base = 0;
     int32_t quad_base = 0;
+    int32_t triangle_base = 0;
     std::vector<int32_t> sphere_ids; // the global id of each sphere of the scene, by its index there
     std::vector<int32_t> quad_ids;
 };
@@ -43,8 +45,21 @@ struct bvh_arrays
 // (src/gpu/renderer.cu:75-82)
 int leaf_size_for(std::size_t shapes);
 
+// The tree over every sphere, quad and triangle of a scene. A scene without triangles gets the reference's tree
+// (object-median splits, leaves of max_leaf_size); one with triangles (meshes baked into world coordinates through
+// their instances) gets a surface-area-heuristic tree of small leaves, as the reference's object-median split on
+// millions of triangles would make a tree several times slower to walk.
 template <typename Real>
 bvh_arrays<Real> build_bvh(const rt_scene& scene, int max_leaf_size);
+
+// The world triangles of a scene's meshes, through their instances (or each mesh once, as it is, when the scene has
+// no instances), with their materials: what build_bvh puts in the tree, in that order before it sorts them
+struct world_triangle
+{
+    double p[3][3];
+    int32_t material;
+};
+std::vector<world_triangle> world_triangles(const rt_scene& scene);
 
 // The quad's record from its description, in double, rounded once (src/gpu/quad.cuh:33-43)
 template <typename Real>

@@ -11,6 +11,7 @@
 #include "bvh_build.hpp"
 #include "kernels/kernel_api.hpp"
 #include "device_scene.hpp"
+#include "quality_tables.hpp"
 #include "scene_build.hpp"
 #include "server.hpp"
 #include "session.hpp"
@@ -313,6 +314,59 @@ int rt_scene_bvh(const rt_scene* scene, float* node_boxes, int32_t* node_ints, i
         if (quad_ids)
         {
             std::memcpy(quad_ids, tree.quad_ids.data(), tree.quad_ids.size() * sizeof(int32_t));
+        }
+    });
+}
+
+int rt_scene_lamps(const rt_scene* scene, float* lamps, int64_t* count)
+{
+    return guarded([&] {
+        need(scene, "scene");
+        need(count, "count");
+        rt::validate_scene(*scene);
+        const auto tree = rt::build_bvh<float>(
+            *scene, rt::leaf_size_for(static_cast<size_t>(scene->sphere_count + scene->quad_count)));
+        const rt::lamp_set set = rt::build_lamp_set(*scene, tree.sphere_ids, tree.quad_ids, tree.prim_materials.size());
+        *count = static_cast<int64_t>(set.lamps.size());
+        if (lamps)
+        {
+            std::memcpy(lamps, set.lamps.data(), set.lamps.size() * sizeof(rt::dev::emitter_rec));
+        }
+    });
+}
+
+int rt_environment_tables(const rt_environment_desc* environment, float* texels, float* row_cos, float* bin_q,
+                          int32_t* bin_alias, int32_t* log2_bins, double* total)
+{
+    return guarded([&] {
+        need(environment, "environment");
+        const rt::environment_tables t = rt::build_environment_tables(*environment);
+        if (log2_bins)
+        {
+            *log2_bins = t.bins.empty() ? 0 : t.log2_bins;
+        }
+        if (total)
+        {
+            *total = t.total;
+        }
+        if (texels)
+        {
+            std::memcpy(texels, t.texels.data(), t.texels.size() * sizeof(rt::dev::env_texel));
+        }
+        if (row_cos)
+        {
+            std::memcpy(row_cos, t.row_cos.data(), t.row_cos.size() * sizeof(float));
+        }
+        for (std::size_t k = 0; k < t.bins.size(); ++k)
+        {
+            if (bin_q)
+            {
+                bin_q[k] = t.bins[k].q;
+            }
+            if (bin_alias)
+            {
+                bin_alias[k] = t.bins[k].alias;
+            }
         }
     });
 }

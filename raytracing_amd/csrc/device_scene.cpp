@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <numbers>
 
+#include "quality_tables.hpp"
 #include "scene_build.hpp"
 
 namespace rt
@@ -266,6 +268,62 @@ void validate_scene(const rt_scene& s)
             throw std::runtime_error("an environment map's yaw and scale must be finite numbers, the scale >= 0");
         }
     }
+    if (s.mesh_count < 0 || s.instance_count < 0)
+    {
+        throw std::runtime_error("a table of the scene has a count but no data");
+    }
+    std::size_t triangles = 0;
+    for (int64_t i = 0; i < s.mesh_count; ++i)
+    {
+        const rt_mesh_desc& m = s.meshes[i];
+        if (m.vertex_count < 0 || m.triangle_count < 0 || (m.vertex_count > 0 && m.positions == nullptr) ||
+            (m.triangle_count > 0 && m.indices == nullptr))
+        {
+            throw std::runtime_error("a mesh's vertices or triangles don't match their counts");
+        }
+        if (m.material < 0 || m.material >= s.material_count)
+        {
+            throw std::runtime_error("a mesh refers to a material that doesn't exist");
+        }
+        for (int64_t k = 0; k < 3 * m.vertex_count; ++k)
+        {
+            if (!std::isfinite(m.positions[k]))
+            {
+                throw std::runtime_error("a mesh's vertex positions must be finite numbers");
+            }
+        }
+        for (int64_t k = 0; k < 3 * m.triangle_count; ++k)
+        {
+            if (static_cast<int64_t>(m.indices[k]) >= m.vertex_count)
+            {
+                throw std::runtime_error("a mesh's triangle refers to a vertex that doesn't exist");
+            }
+        }
+        if (s.instance_count == 0)
+        {
+            triangles += static_cast<std::size_t>(m.triangle_count);
+        }
+    }
+    for (int64_t i = 0; i < s.instance_count; ++i)
+    {
+        const rt_mesh_instance_desc& in = s.instances[i];
+        if (in.mesh < 0 || in.mesh >= s.mesh_count)
+        {
+            throw std::runtime_error("a mesh instance refers to a mesh that doesn't exist");
+        }
+        for (const double x : in.object_to_world)
+        {
+            if (!std::isfinite(x))
+            {
+                throw std::runtime_error("a mesh instance's transform must hold finite numbers");
+            }
+        }
+        triangles += static_cast<std::size_t>(s.meshes[in.mesh].triangle_count);
+    }
+    if (triangles > static_cast<std::size_t>(dev::max_leaf_first))
+    {
+        throw std::runtime_error("the scene has more triangles than the GPU renderer's tree can address");
+    }
 }
 
 bool first_book_scene(const rt_scene& s)
@@ -496,21 +554,61 @@ device_scene<Real>::device_scene(const rt_scene& s)
     media_ = device_array<dev::medium_rec<Real>>(media);
     boundary_quads_ = device_array<dev::quad_rec<Real>>(boundary);
     lights_ = device_array<dev::light_rec<Real>>(lights);
+    triangles_ = device_array<dev::triangle_rec<Real>>(tree_.triangles);
     bytes_ = bytes_of(nodes_) + bytes_of(spheres_) + bytes_of(moving_) + bytes_of(quads_) + bytes_of(prim_materials_) +
              bytes_of(frames_) + bytes_of(materials_) + bytes_of(textures_) + bytes_of(images_) + bytes_of(texels_) +
-             bytes_of(noises_) + bytes_of(media_) + bytes_of(boundary_quads_) + bytes_of(lights_);
+             bytes_of(noises_) + bytes_of(media_) + bytes_of(boundary_quads_) + bytes_of(lights_) +
+             bytes_of(triangles_);
 
     dev::scene_view<Real>& v = view_;
+    quality_ = needs_quality(s);
+    if (quality_ && sizeof(Real) == sizeof(float))
+    {
+        // the quality integrator's tables (src/gpu/quality.cu:112-170)
+        const lamp_set set = build_lamp_set(s, tree_.sphere_ids, tree_.quad_ids, tree_.prim_materials.size());
+        lamps_ = device_array<dev::emitter_rec>(set.lamps);
+        lamp_of_ = device_array<int32_t>(set.lamp_of);
+        microfacets_ = device_array<dev::microfacet_rec>(build_microfacets(s));
+        dev::quality_view& q = v.quality;
+        q.combination = s.combination;
+        q.lamp_count = static_cast<int32_t>(set.lamps.size());
+        q.lamps = lamps_.data();
+        q.lamp_of = lamp_of_.data();
+        q.microfacets = microfacets_.size() > 0 ? microfacets_.data() : nullptr;
+        int lights_sampled = q.lamp_count;
+        if (s.environment)
+        {
+            const environment_tables env = build_environment_tables(*s.environment);
+            env_texels_ = device_array<dev::env_texel>(env.texels);
+            env_bins_ = device_array<dev::alias_bin>(env.bins);
+            env_row_cos_ = device_array<float>(env.row_cos);
+            const double yaw = s.environment->yaw_degrees * std::numbers::pi / 180;
+            q.env_present = 1;
+            q.env_texels = env_texels_.data();
+            q.env_bins = env_bins_.size() > 0 ? env_bins_.data() : nullptr;
+            q.env_row_cos = env_row_cos_.data();
+            q.env_width = env.width;
+            q.env_height = env.height;
+            q.env_log2_bins = env.log2_bins;
+            q.env_cos_yaw = static_cast<float>(std::cos(yaw));
+            q.env_sin_yaw = static_cast<float>(std::sin(yaw));
+            q.env_scale = static_cast<float>(s.environment->scale);
+            lights_sampled += env_bins_.size() > 0 ? 1 : 0;
+        }
+        q.light_pmf = lights_sampled > 0 ? static_cast<float>(1.0 / lights_sampled) : 0.0f;
+        bytes_ += bytes_of(lamps_) + bytes_of(lamp_of_) + bytes_of(microfacets_) + bytes_of(env_texels_) +
+                  bytes_of(env_bins_) + bytes_of(env_row_cos_);
+    }
     v.nodes = nodes_.data();
     v.root_ref = tree_.root_ref;
     v.stack_depth = tree_.nodes.empty() ? 0 : tree_.depth;
     v.spheres = spheres_.data();
     v.moving_spheres = moving_.data();
     v.quads = quads_.data();
-    v.triangles = nullptr;
+    v.triangles = triangles_.data();
     v.moving_base = tree_.moving_base;
     v.quad_base = tree_.quad_base;
-    v.triangle_base = static_cast<int32_t>(tree_.prim_materials.size());
+    v.triangle_base = tree_.triangle_base;
     v.media_base = static_cast<int32_t>(tree_.prim_materials.size());
     v.prim_materials = prim_materials_.data();
     v.sphere_frames = frames_.data();

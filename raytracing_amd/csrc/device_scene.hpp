@@ -93,6 +93,8 @@ public:
     const dev::scene_view<Real>& view() const { return view_; }
     const bvh_arrays<Real>& tree() const { return tree_; }
     std::size_t bytes() const { return bytes_; }
+    // whether the scene needs the quality integrator (light samples, GGX, an environment map or triangles)
+    bool quality() const { return quality_; }
 
 private:
     bvh_arrays<Real> tree_;
@@ -110,6 +112,15 @@ private:
     device_array<dev::medium_rec<Real>> media_;
     device_array<dev::quad_rec<Real>> boundary_quads_;
     device_array<dev::light_rec<Real>> lights_;
+    device_array<dev::triangle_rec<Real>> triangles_;
+    // the quality integrator's tables (float scenes only)
+    device_array<dev::emitter_rec> lamps_;
+    device_array<int32_t> lamp_of_;
+    device_array<dev::microfacet_rec> microfacets_;
+    device_array<dev::env_texel> env_texels_;
+    device_array<dev::alias_bin> env_bins_;
+    device_array<float> env_row_cos_;
+    bool quality_ = false;
     dev::scene_view<Real> view_{};
     std::size_t bytes_ = 0;
 };

@@ -81,15 +81,16 @@ struct alignas(16) quad_rec
 };
 static_assert(sizeof(quad_rec<float>) == 48);
 
-// A triangle (Moller-Trumbore): a vertex and two edges, with the material in the last word
+// A triangle for the Moller-Trumbore test: a vertex, the two edges from it, and the unit normal e1 x e2 (its front)
 template <typename Real>
 struct alignas(16) triangle_rec
 {
     Real v0[3];
     Real e1[3];
     Real e2[3];
-    Real unused[3];
+    Real normal[3];
 };
+static_assert(sizeof(triangle_rec<float>) == 48);
 
 // What only a sphere hit reads besides its material (src/gpu/sphere.cuh:36-46)
 template <typename Real>
@@ -211,10 +212,69 @@ struct camera_rec
     Real sky_top[3];
 };
 
+// A lamp of the quality integrator's light set: a glowing quad or sphere of the world (src/gpu/lamps.cuh:41-54)
+struct alignas(16) emitter_rec
+{
+    float corner[3]; // quad: q; sphere: the center
+    float size;      // quad: the area; sphere: the radius
+    float u[3];
+    int32_t kind;    // RT_LIGHT_QUAD or RT_LIGHT_SPHERE
+    float v[3];
+    int32_t primitive; // its global id: its own shadow ray doesn't count it as a blocker
+    float normal[3];
+    float unused0;
+    float radiance[3];
+    float unused1;
+};
+static_assert(sizeof(emitter_rec) == 80);
+
+// A texel of the environment map with its cell's density per steradian, and a bin of the map's alias table
+// (src/gpu/environment_data.h:24-57)
+struct alignas(16) env_texel
+{
+    float r, g, b, pdf;
+};
+struct alignas(8) alias_bin
+{
+    float q;
+    int32_t alias;
+};
+
+// A material's GGX side entry as the kernels read it
+struct alignas(16) microfacet_rec
+{
+    int32_t kind;    // RT_MICROFACET_*
+    float alpha;     // r^2, rounded once
+    float roughness; // r
+    int32_t unused;
+};
+
+// What only the quality integrator reads: the light combination, the lamps, the environment, the GGX side entries
+struct quality_view
+{
+    int32_t combination;  // RT_COMBINE_*
+    int32_t lamp_count;
+    const emitter_rec* lamps;
+    const int32_t* lamp_of; // per surface primitive: its lamp, or -1
+    float light_pmf;        // 1 / (lamps + the map when it is sampled)
+    int32_t env_present;
+    const microfacet_rec* microfacets; // one per material, or nullptr
+    const env_texel* env_texels;
+    const alias_bin* env_bins;          // nullptr: the map isn't sampled (a black map)
+    const float* env_row_cos;
+    int32_t env_width;
+    int32_t env_height;
+    int32_t env_log2_bins;
+    float env_cos_yaw;
+    float env_sin_yaw;
+    float env_scale;
+};
+
 // Where the scene's arrays are: the kernels take it by value
 template <typename Real>
 struct scene_view
 {
+    quality_view quality;
     const wide_node<Real>* nodes;
     int32_t root_ref;  // the root: an inner node's index (0), a leaf reference, or -1 with no primitive at all
     int32_t stack_depth;
@@ -240,6 +300,16 @@ struct scene_view
     const light_rec<Real>* lights;
     int32_t sampling; // rt_sampling_strategy
     int32_t motion;   // some sphere or medium boundary moves: camera rays draw a time
+};
+
+// Which integrator a scene's kernels run (src/gpu/renderer.cu:1011-1023, src/gpu/quality.cu:53-109): the first book's
+// (spheres that stand still, no texture, the material strategy), the three books', or the quality integrator's (light
+// samples weighed by MIS, GGX, the environment map, triangles; float only)
+enum kernel_mode : int32_t
+{
+    mode_first_book = 0,
+    mode_all_books = 1,
+    mode_quality = 2,
 };
 
 // The work of one launch: passes [first_pass, first_pass + pass_count) of rows [first_row, first_row + row_count),

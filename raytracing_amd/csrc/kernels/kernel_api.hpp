@@ -22,7 +22,7 @@ template <typename Real>
 size_t stack_lds_bytes(int depth, int threads);
 
 template <typename Real>
-hipError_t launch_megakernel(bool all_books, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
+hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
                              uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays,
                              int block_w, int block_h, hipStream_t stream, unsigned long long* tallies = nullptr);
 
@@ -37,5 +37,5 @@ hipError_t launch_display(const double* sums, int64_t pixels, double inverse_pas
 hipError_t launch_copy(const void* in, void* out, size_t bytes, hipStream_t stream);
 
 template <typename Real>
-kernel_attributes megakernel_attributes(bool all_books, int block_threads, int depth);
+kernel_attributes megakernel_attributes(int mode, int block_threads, int depth);
 }

@@ -7,6 +7,7 @@
 // wave with DPP/permute shuffles, then one atomic per wave.
 #include "kernel_api.hpp"
 #include "path_tracer.hpp"
+#include "quality.hpp"
 
 namespace rt::dev
 {
@@ -28,6 +29,38 @@ __device__ inline lds_stack<Real> block_stack(int depth)
     int32_t* refs = reinterpret_cast<int32_t*>(lds);
     Real* ts = reinterpret_cast<Real*>(lds + ((static_cast<size_t>(depth) * threads * 4 + 15) & ~size_t(15)));
     return {refs + tid, ts + tid, threads};
+}
+
+static_assert(tally_events == RT_TRAFFIC_EVENTS, "the traffic report lists every tally event");
+
+// The wave's counts: the rays (and shadow rays) summed across the wave, then one atomic per wave
+template <bool Count, typename Tally>
+__device__ inline void add_counts(unsigned long long rays, unsigned long long shadow_rays,
+                                  unsigned long long* __restrict__ ray_count, unsigned long long* __restrict__ tallies,
+                                  const Tally& tl)
+{
+    rays = wave_sum(rays);
+    shadow_rays = wave_sum(shadow_rays);
+    const bool lane0 = ((threadIdx.y * blockDim.x + threadIdx.x) & 63) == 0;
+    if (lane0)
+    {
+        atomicAdd(ray_count, rays);
+        if (shadow_rays)
+        {
+            atomicAdd(ray_count + 1, shadow_rays);
+        }
+    }
+    if constexpr (Count)
+    {
+        for (int e = 0; e < tally_events; ++e)
+        {
+            const unsigned long long total = wave_sum(static_cast<unsigned long long>(tl.n[e]));
+            if (lane0)
+            {
+                atomicAdd(tallies + e, total);
+            }
+        }
+    }
 }
 
 template <typename Real, bool AllBooks, bool Count>
@@ -74,23 +107,56 @@ __global__ void __launch_bounds__(256) megakernel(const camera_rec<Real> cam, co
         sum[1] = green;
         sum[2] = blue;
     }
-    rays = wave_sum(rays);
-    const bool lane0 = ((threadIdx.y * blockDim.x + threadIdx.x) & 63) == 0;
-    if (lane0)
+    add_counts<Count>(rays, 0, ray_count, tallies, tl);
+}
+
+// The quality integrator's megakernel (src/gpu/quality.cu:53-109): the same loop over the launch's passes, with light
+// samples and their shadow rays at every vertex that isn't a mirror's (quality.hpp)
+template <bool Count>
+__global__ void __launch_bounds__(256) quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc,
+                                                          const uint64_t seed, const launch_work work,
+                                                          double* __restrict__ sums,
+                                                          unsigned long long* __restrict__ ray_count,
+                                                          unsigned long long* __restrict__ tallies)
+{
+    const lds_stack<float> stack = block_stack<float>(sc.stack_depth);
+    tally<Count> tl;
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = work.first_row + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+    unsigned long long rays = 0;
+    uint32_t shadow_rays = 0;
+    if (i < cam.image_width && j < work.first_row + work.row_count)
     {
-        atomicAdd(ray_count, rays);
-    }
-    if constexpr (Count)
-    {
-        for (int e = 0; e < tally_events; ++e)
+        const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
+        double* const sum = sums + 3 * pixel;
+        double red = sum[0], green = sum[1], blue = sum[2];
+        int pass = work.first_pass;
+        const int end_pass = work.first_pass + work.pass_count;
+        pcg32 g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
+        quality_path q = start_quality(cam, sc.motion != 0, i, j, pass, g);
+        while (true)
         {
-            const unsigned long long total = wave_sum(static_cast<unsigned long long>(tl.n[e]));
-            if (lane0)
+            ++rays;
+            vec3<float> sample;
+            if (!trace_quality(q, sc, cam, g, sample, stack, tl, shadow_rays))
             {
-                atomicAdd(tallies + e, total);
+                continue;
             }
+            red += without_nan(sample.x);
+            green += without_nan(sample.y);
+            blue += without_nan(sample.z);
+            if (++pass == end_pass)
+            {
+                break;
+            }
+            g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
+            q = start_quality(cam, sc.motion != 0, i, j, pass, g);
         }
+        sum[0] = red;
+        sum[1] = green;
+        sum[2] = blue;
     }
+    add_counts<Count>(rays, shadow_rays, ray_count, tallies, tl);
 }
 
 // The parity probe: closest_hit on a batch of given rays, with the traversal of the all-books kernel
@@ -150,38 +216,42 @@ size_t stack_lds_bytes(int depth, int threads)
     return ((static_cast<size_t>(depth) * threads * 4 + 15) & ~size_t(15)) + static_cast<size_t>(depth) * threads * sizeof(Real);
 }
 
+namespace
+{
+template <typename Real, bool Count>
+const void* megakernel_for(int mode)
+{
+    if constexpr (sizeof(Real) == sizeof(float))
+    {
+        if (mode == dev::mode_quality)
+        {
+            return reinterpret_cast<const void*>(&dev::quality_megakernel<Count>);
+        }
+    }
+    return mode == dev::mode_all_books ? reinterpret_cast<const void*>(&dev::megakernel<Real, true, Count>)
+                                       : reinterpret_cast<const void*>(&dev::megakernel<Real, false, Count>);
+}
+}
+
 template <typename Real>
-hipError_t launch_megakernel(bool all_books, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
+hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
                              uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays,
                              int block_w, int block_h, hipStream_t stream, unsigned long long* tallies)
 {
+    if (mode == dev::mode_quality && sizeof(Real) != sizeof(float))
+    {
+        return hipErrorInvalidValue;
+    }
     const dim3 block(block_w, block_h);
     const dim3 grid((cam.image_width + block_w - 1) / block_w, (work.row_count + block_h - 1) / block_h);
     const size_t lds = stack_lds_bytes<Real>(sc.stack_depth, block_w * block_h);
-    if (tallies)
-    {
-        if (all_books)
-        {
-            hipLaunchKernelGGL((dev::megakernel<Real, true, true>), grid, block, lds, stream, cam, sc, seed, work, sums,
-                               rays, tallies);
-        }
-        else
-        {
-            hipLaunchKernelGGL((dev::megakernel<Real, false, true>), grid, block, lds, stream, cam, sc, seed, work,
-                               sums, rays, tallies);
-        }
-    }
-    else if (all_books)
-    {
-        hipLaunchKernelGGL((dev::megakernel<Real, true, false>), grid, block, lds, stream, cam, sc, seed, work, sums,
-                           rays, tallies);
-    }
-    else
-    {
-        hipLaunchKernelGGL((dev::megakernel<Real, false, false>), grid, block, lds, stream, cam, sc, seed, work, sums,
-                           rays, tallies);
-    }
-    return hipGetLastError();
+    const void* f = tallies ? megakernel_for<Real, true>(mode) : megakernel_for<Real, false>(mode);
+    dev::camera_rec<Real> c = cam;
+    dev::scene_view<Real> v = sc;
+    uint64_t s = seed;
+    dev::launch_work w = work;
+    void* args[] = {&c, &v, &s, &w, &sums, &rays, &tallies};
+    return hipLaunchKernel(f, grid, block, args, lds, stream);
 }
 
 template <typename Real>
@@ -206,11 +276,10 @@ hipError_t launch_display(const double* sums, int64_t pixels, double inverse_pas
 }
 
 template <typename Real>
-kernel_attributes megakernel_attributes(bool all_books, int block_threads, int depth)
+kernel_attributes megakernel_attributes(int mode, int block_threads, int depth)
 {
     hipFuncAttributes a{};
-    const void* f = all_books ? reinterpret_cast<const void*>(&dev::megakernel<Real, true, false>)
-                              : reinterpret_cast<const void*>(&dev::megakernel<Real, false, false>);
+    const void* f = megakernel_for<Real, false>(mode);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)
     {
@@ -226,17 +295,17 @@ kernel_attributes megakernel_attributes(bool all_books, int block_threads, int d
     return k;
 }
 
-template hipError_t launch_megakernel<float>(bool, const dev::camera_rec<float>&, const dev::scene_view<float>&,
+template hipError_t launch_megakernel<float>(int, const dev::camera_rec<float>&, const dev::scene_view<float>&,
                                              uint64_t, const dev::launch_work&, double*, unsigned long long*, int, int,
                                              hipStream_t, unsigned long long*);
-template hipError_t launch_megakernel<double>(bool, const dev::camera_rec<double>&, const dev::scene_view<double>&,
+template hipError_t launch_megakernel<double>(int, const dev::camera_rec<double>&, const dev::scene_view<double>&,
                                               uint64_t, const dev::launch_work&, double*, unsigned long long*, int, int,
                                               hipStream_t, unsigned long long*);
 template hipError_t launch_closest_hit<float>(const dev::scene_view<float>&, const float*, const float*, const float*,
                                               const int32_t*, const uint8_t*, int64_t, int32_t*, float*, uint8_t*,
                                               hipStream_t);
-template kernel_attributes megakernel_attributes<float>(bool, int, int);
-template kernel_attributes megakernel_attributes<double>(bool, int, int);
+template kernel_attributes megakernel_attributes<float>(int, int, int);
+template kernel_attributes megakernel_attributes<double>(int, int, int);
 template size_t stack_lds_bytes<float>(int, int);
 template size_t stack_lds_bytes<double>(int, int);
 }

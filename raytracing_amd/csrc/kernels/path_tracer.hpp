@@ -328,6 +328,48 @@ __device__ inline void quad_uv(const quad_rec<Real>& s, const ray<Real>& r, Real
     v = dot(p, load3(s.beta_axis));
 }
 
+// The ray against a triangle (Moller and Trumbore 1997), closed [t_min, t_max] as a quad's; front: the ray meets the
+// side its normal e1 x e2 points to. Barycentric (u, v) of the hit for its texture coordinates.
+template <typename Real>
+__device__ inline bool hit_triangle(const triangle_rec<Real>& tri, const ray<Real>& r, Real t_min, Real t_max, Real& t,
+                                    bool& front_face, Real* bary = nullptr)
+{
+    const vec3<Real> e1 = load3(tri.e1);
+    const vec3<Real> e2 = load3(tri.e2);
+    const vec3<Real> d = r.direction;
+    const vec3<Real> pv{d.y * e2.z - d.z * e2.y, d.z * e2.x - d.x * e2.z, d.x * e2.y - d.y * e2.x};
+    const Real det = dot(e1, pv);
+    if (!(m_fabs(det) > Real(1e-30)))
+    {
+        return false;
+    }
+    const Real inv = Real(1) / det;
+    const vec3<Real> s = r.origin - load3(tri.v0);
+    const Real u = dot(s, pv) * inv;
+    if (!(u >= Real(0) && u <= Real(1)))
+    {
+        return false;
+    }
+    const vec3<Real> qv{s.y * e1.z - s.z * e1.y, s.z * e1.x - s.x * e1.z, s.x * e1.y - s.y * e1.x};
+    const Real v = dot(d, qv) * inv;
+    if (!(v >= Real(0) && u + v <= Real(1)))
+    {
+        return false;
+    }
+    t = dot(e2, qv) * inv;
+    if (!(t_min <= t && t <= t_max))
+    {
+        return false;
+    }
+    front_face = det > Real(0);
+    if (bary)
+    {
+        bary[0] = u;
+        bary[1] = v;
+    }
+    return true;
+}
+
 template <typename Real>
 struct hit
 {
@@ -352,6 +394,10 @@ enum tally_event
     tally_texel,          // an image texel
     tally_perlin,         // a Perlin gradient (and its three permutation bytes)
     tally_light,          // a light record
+    tally_triangle,       // a triangle record tested
+    tally_lamp,           // a lamp of the quality integrator's light set
+    tally_env_texel,      // an environment map texel (radiance and density)
+    tally_alias_bin,      // a bin of the environment map's alias table
     tally_events
 };
 template <bool On>
@@ -405,7 +451,26 @@ __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Re
             const int kind = leaf_kind(ref);
             const int first = leaf_first(ref);
             const int end = first + leaf_count(ref);
-            if (AllKinds && kind == prim_quad)
+            if (AllKinds && kind == prim_triangle)
+            {
+                for (int i = first; i < end; ++i)
+                {
+                    const int id = sc.triangle_base + i;
+                    if (id == from)
+                    {
+                        continue; // a ray never meets the flat face it leaves
+                    }
+                    const triangle_rec<Real> tri = sc.triangles[i];
+                    tl.add(tally_triangle);
+                    Real t;
+                    bool front;
+                    if (hit_triangle(tri, r, t_min, closest.t, t, front))
+                    {
+                        closest = {id, t, front};
+                    }
+                }
+            }
+            else if (AllKinds && kind == prim_quad)
             {
                 for (int i = first; i < end; ++i)
                 {
@@ -824,7 +889,7 @@ __device__ inline surface<Real> surface_at(const scene_view<Real>& sc, const ray
     s.u = Real(0);
     s.v = Real(0);
     s.outward = {Real(0), Real(0), Real(0)};
-    const bool on_quad = h.index >= sc.quad_base && h.index < sc.media_base;
+    const bool on_quad = h.index >= sc.quad_base && h.index < sc.triangle_base;
     s.rec.p = on_quad ? r.at_unfused(h.t) : r.at(h.t);
     s.texture_point = s.rec.p;
     s.from = h.index;
@@ -842,6 +907,22 @@ __device__ inline surface<Real> surface_at(const scene_view<Real>& sc, const ray
     s.material = sc.prim_materials[h.index];
     const uint32_t flags = sc.materials[s.material].flags;
     s.texture_index = sc.materials[s.material].texture;
+    if (h.index >= sc.triangle_base)
+    {
+        const triangle_rec<Real> tri = sc.triangles[h.index - sc.triangle_base];
+        tl.add(tally_triangle);
+        s.outward = load3(tri.normal);
+        s.rec.normal = s.rec.front_face ? s.outward : -s.outward;
+        if (flags & needs_uv)
+        {
+            Real t, bary[2];
+            bool front;
+            hit_triangle(tri, r, -inf<Real>, inf<Real>, t, front, bary);
+            s.u = bary[0];
+            s.v = bary[1];
+        }
+        return s;
+    }
     if (h.index >= sc.quad_base)
     {
         const quad_rec<Real> face = sc.quads[h.index - sc.quad_base];

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <numbers>
+#include <optional>
 #include <random>
 #include <span>
 #include <stdexcept>
@@ -189,6 +190,8 @@ struct material_def
     d3 albedo;
     double fuzz = 0;
     double refraction_index = 1;
+    int microfacet = RT_MICROFACET_NONE; // a GGX side entry (src/gpu_scenes_materials.ixx:35-60)
+    double roughness = 0;
 };
 using material_ref = std::shared_ptr<const material_def>;
 material_ref lambertian(texture_ref t)
@@ -220,6 +223,33 @@ material_ref lamp_light(d3 c)
     m->texture = solid(c);
     return m;
 }
+// A GGX rough metal of reflectance F0 at normal incidence: the books' metal without fuzz, completed by its side entry
+material_ref rough_metal(d3 f0, double roughness)
+{
+    auto m = std::make_shared<material_def>();
+    m->kind = RT_MATERIAL_METAL;
+    m->albedo = f0;
+    m->microfacet = RT_MICROFACET_CONDUCTOR;
+    m->roughness = roughness;
+    return m;
+}
+// A glossy coat (GGX, F0 0.04) over a matte color
+material_ref glossy(d3 base, double roughness)
+{
+    auto m = std::make_shared<material_def>();
+    m->texture = solid(base);
+    m->microfacet = RT_MICROFACET_GLOSS;
+    m->roughness = roughness;
+    return m;
+}
+
+// A triangle mesh in its own coordinates (vertices shared by its triangles) and the material all of it wears
+struct mesh_def
+{
+    std::vector<float> positions;
+    std::vector<uint32_t> indices;
+    material_ref material;
+};
 
 // --- objects --------------------------------------------------------------------------------------------------------
 struct object;
@@ -437,11 +467,22 @@ camera_settings cornell_camera()
     return c;
 }
 
+struct environment_def
+{
+    std::shared_ptr<const image_data> image;
+    double yaw_degrees = 0;
+    double scale = 1;
+};
+
 struct built
 {
     list_builder world;
     list_builder lights;
     camera_settings camera;
+    std::vector<mesh_def> meshes;
+    std::vector<rt_mesh_instance_desc> instances;
+    std::optional<environment_def> environment;
+    int combination = -1; // the light combination the scene asks for by default (-p), or -1 for the books' strategy
 };
 
 list_builder random_spheres(scene_random& rng, bool bouncing)
@@ -513,6 +554,185 @@ object_ref short_block(const material_ref& m)
     return make_translate(make_rotate_y(make_box({0, 0, 0}, {165, 165, 165}, m), -18), {130, 0, 65});
 }
 object_ref cornell_lamp(material_ref m) { return make_quad({343, 554, 332}, {-130, 0, 0}, {0, 0, -105}, std::move(m)); }
+
+// --- generated content: meshes and skies (the triangle scene and the light-sample tests) -----------------------------
+// The unit icosphere: the icosahedron's 20 faces, each split into 4 `level` times, every new vertex (an edge's midpoint)
+// pushed out onto the sphere; 20 * 4^level triangles, counter-clockwise seen from outside
+struct sphere_mesh
+{
+    std::vector<d3> vertices;
+    std::vector<std::array<uint32_t, 3>> faces;
+};
+sphere_mesh icosphere(int level)
+{
+    const double g = (1 + std::sqrt(5.0)) / 2;
+    sphere_mesh m;
+    m.vertices = {{-1, g, 0}, {1, g, 0}, {-1, -g, 0}, {1, -g, 0}, {0, -1, g}, {0, 1, g},
+                  {0, -1, -g}, {0, 1, -g}, {g, 0, -1}, {g, 0, 1}, {-g, 0, -1}, {-g, 0, 1}};
+    for (d3& v : m.vertices)
+    {
+        v = unit(v);
+    }
+    m.faces = {{0, 11, 5}, {0, 5, 1},  {0, 1, 7},   {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4},
+               {11, 10, 2}, {10, 7, 6}, {7, 1, 8},   {3, 9, 4},  {3, 4, 2},   {3, 2, 6}, {3, 6, 8},
+               {3, 8, 9},  {4, 9, 5},  {2, 4, 11},  {6, 2, 10}, {8, 6, 7},   {9, 8, 1}};
+    for (int l = 0; l < level; ++l)
+    {
+        std::unordered_map<uint64_t, uint32_t> middle;
+        const auto mid = [&](uint32_t a, uint32_t b) {
+            const uint64_t key = (static_cast<uint64_t>(std::min(a, b)) << 32) | std::max(a, b);
+            if (const auto f = middle.find(key); f != middle.end())
+            {
+                return f->second;
+            }
+            const auto k = static_cast<uint32_t>(m.vertices.size());
+            m.vertices.push_back(unit(0.5 * (m.vertices[a] + m.vertices[b])));
+            middle.emplace(key, k);
+            return k;
+        };
+        std::vector<std::array<uint32_t, 3>> next;
+        next.reserve(4 * m.faces.size());
+        for (const auto& f : m.faces)
+        {
+            const uint32_t ab = mid(f[0], f[1]), bc = mid(f[1], f[2]), ca = mid(f[2], f[0]);
+            next.push_back({f[0], ab, ca});
+            next.push_back({f[1], bc, ab});
+            next.push_back({f[2], ca, bc});
+            next.push_back({ab, bc, ca});
+        }
+        m.faces = std::move(next);
+    }
+    return m;
+}
+
+// A smooth seeded field on the unit sphere: the mean of `waves` plane waves sin(f (d . p) + phase), directions uniform
+// on the sphere, frequencies in [f_lo, f_hi], from std::mt19937_64(seed); values in [-1, 1]
+class wave_field
+{
+public:
+    wave_field(uint64_t seed, int waves, double f_lo, double f_hi)
+    {
+        std::mt19937_64 g(seed);
+        std::uniform_real_distribution<double> u(0.0, 1.0);
+        for (int k = 0; k < waves; ++k)
+        {
+            const double z = 2 * u(g) - 1, a = 2 * std::numbers::pi * u(g), s = std::sqrt(1 - z * z);
+            waves_.push_back({{s * std::cos(a), s * std::sin(a), z}, f_lo + (f_hi - f_lo) * u(g),
+                              2 * std::numbers::pi * u(g)});
+        }
+    }
+    double operator()(d3 p) const
+    {
+        double sum = 0;
+        for (const wave& w : waves_)
+        {
+            sum += std::sin(w.frequency * dot(w.direction, p) + w.phase);
+        }
+        return sum / static_cast<double>(waves_.size());
+    }
+
+private:
+    struct wave
+    {
+        d3 direction;
+        double frequency, phase;
+    };
+    std::vector<wave> waves_;
+};
+
+// An icosphere of radius `radius` around `center`, each vertex moved along its direction by amplitude * field
+mesh_def displaced_sphere(int level, const wave_field& field, double amplitude, double radius, d3 center,
+                          material_ref material)
+{
+    const sphere_mesh s = icosphere(level);
+    mesh_def m;
+    m.material = std::move(material);
+    m.positions.reserve(3 * s.vertices.size());
+    for (const d3& v : s.vertices)
+    {
+        const d3 p = center + (radius * (1 + amplitude * field(v))) * v;
+        m.positions.insert(m.positions.end(), {static_cast<float>(p.x), static_cast<float>(p.y), static_cast<float>(p.z)});
+    }
+    m.indices.reserve(3 * s.faces.size());
+    for (const auto& f : s.faces)
+    {
+        m.indices.insert(m.indices.end(), f.begin(), f.end());
+    }
+    return m;
+}
+
+// An equirectangular HDR sky, +y up, row 0 at +y (the environment map's convention): above the horizon a gradient from
+// a bright haze to a blue zenith with a faint seeded variation, below it a dim ground, and the sun, a disk of
+// `sun_radiance` and angular radius `sun_degrees` at the given elevation and azimuth (degrees; azimuth 0 is +x)
+std::shared_ptr<const image_data> sun_sky(int width, int height, uint64_t seed, double elevation, double azimuth,
+                                          double sun_degrees, double sun_radiance)
+{
+    auto img = std::make_shared<image_data>();
+    img->width = width;
+    img->height = height;
+    img->rgb.resize(3 * static_cast<std::size_t>(width) * height);
+    const wave_field clouds(seed, 12, 2.0, 6.0);
+    const double el = radians(elevation), az = radians(azimuth);
+    const d3 sun{std::cos(el) * std::cos(az), std::sin(el), std::cos(el) * std::sin(az)};
+    const double cos_disk = std::cos(radians(sun_degrees));
+    for (int j = 0; j < height; ++j)
+    {
+        const double theta = std::numbers::pi * (j + 0.5) / height;
+        for (int i = 0; i < width; ++i)
+        {
+            const double phi = 2 * std::numbers::pi * (i + 0.5) / width - std::numbers::pi;
+            const d3 d{std::sin(theta) * std::cos(phi), std::cos(theta), std::sin(theta) * std::sin(phi)};
+            d3 c;
+            if (d.y >= 0)
+            {
+                const double up = std::sqrt(d.y);
+                const d3 haze{1.1, 1.05, 1.0}, zenith{0.25, 0.45, 0.95};
+                c = (1 - up) * haze + up * zenith;
+                c = (1 + 0.15 * clouds(d)) * c;
+            }
+            else
+            {
+                c = d3{0.18, 0.16, 0.14};
+            }
+            if (dot(d, sun) >= cos_disk)
+            {
+                c = d3{sun_radiance, sun_radiance, sun_radiance};
+            }
+            float* px = img->rgb.data() + 3 * (static_cast<std::size_t>(j) * width + i);
+            px[0] = static_cast<float>(c.x);
+            px[1] = static_cast<float>(c.y);
+            px[2] = static_cast<float>(c.z);
+        }
+    }
+    return img;
+}
+
+// A map of radiance 1 everywhere (env-furnace)
+std::shared_ptr<const image_data> white_map()
+{
+    auto img = std::make_shared<image_data>();
+    img->width = 16;
+    img->height = 8;
+    img->rgb.assign(3 * 16 * 8, 1.0f);
+    return img;
+}
+
+// The furnaces' camera (src/gpu_scenes_materials.ixx:62-75): nearly orthographic from 60 away, a white sky
+camera_settings furnace_camera()
+{
+    camera_settings c;
+    c.aspect_ratio = 16.0 / 9.0;
+    c.image_width = 480;
+    c.samples_per_pixel = 256;
+    c.max_depth = 10;
+    c.sampling = RT_SAMPLING_MATERIAL;
+    c.vfov = 6;
+    c.lookfrom = {0, 0, 60};
+    c.lookat = {0, 0, 0};
+    c.sky_bottom = {1, 1, 1};
+    c.sky_top = {1, 1, 1};
+    return c;
+}
 
 built build_named(const std::string& name, scene_random& rng, const scene_options& opts)
 {
@@ -691,6 +911,183 @@ built build_named(const std::string& name, scene_random& rng, const scene_option
         s.camera.sky_bottom = {1, 1, 1};
         s.camera.sky_top = {1, 1, 1};
     }
+    else if (name == "ggx-furnace")
+    {
+        // Five white metals (F0 = 1) under a white sky, the mirror and GGX at alpha = r^2 = 0.05, 0.3, 0.6, 1
+        // (src/gpu_scenes_materials.ixx:86-102): each sphere's centre shows the light its lobe keeps, 1 with
+        // Turquin's compensation
+        const std::array<double, 5> roughness{0.0, std::sqrt(0.05), std::sqrt(0.3), std::sqrt(0.6), 1.0};
+        for (std::size_t i = 0; i < roughness.size(); ++i)
+        {
+            s.world.add(make_sphere({-4.0 + 2.0 * static_cast<double>(i), 0, 0}, 0.4,
+                                    rough_metal({1, 1, 1}, roughness[i])));
+        }
+        s.camera = furnace_camera();
+    }
+    else if (name == "gloss-furnace")
+    {
+        // Four white matte spheres under glossy coats at alpha = 0.01, 0.1, 0.3, 1 (src/gpu_scenes_materials.ixx:104-116)
+        const std::array<double, 4> roughness{std::sqrt(0.01), std::sqrt(0.1), std::sqrt(0.3), 1.0};
+        for (std::size_t i = 0; i < roughness.size(); ++i)
+        {
+            s.world.add(make_sphere({-3.0 + 2.0 * static_cast<double>(i), 0, 0}, 0.4, glossy({1, 1, 1}, roughness[i])));
+        }
+        s.camera = furnace_camera();
+    }
+    else if (name == "env-furnace")
+    {
+        // furnace's sphere and camera under a map of radiance 1 and a black sky
+        // (src/gpu_scenes_environments.ixx:102-118): every pixel is 1
+        s.world.add(make_sphere({0, 0, 0}, 1.0, lambertian(d3{1, 1, 1})));
+        s.camera.aspect_ratio = 16.0 / 9.0;
+        s.camera.image_width = 480;
+        s.camera.samples_per_pixel = 64;
+        s.camera.max_depth = 10;
+        s.camera.sampling = RT_SAMPLING_MATERIAL;
+        s.camera.vfov = 30;
+        s.camera.lookfrom = {0, 0, 6};
+        s.camera.lookat = {0, 0, 0};
+        s.environment = environment_def{white_map(), 0, 1};
+    }
+    else if (name == "cornell-ggx")
+    {
+        // The Cornell box with a tall block of rough aluminum (r = 0.3) and a short one glossy white (r = 0.2)
+        // (src/gpu_scenes_materials.ixx:181-204): -p nee and -p mis converge to -p material's image
+        cornell_walls(s.world);
+        s.world.add(cornell_lamp(lamp_light({15, 15, 15})));
+        s.world.add(tall_block(rough_metal({0.913, 0.921, 0.925}, 0.3)));
+        s.world.add(short_block(glossy({0.73, 0.73, 0.73}, 0.2)));
+        s.camera = cornell_camera();
+        s.camera.samples_per_pixel = 1024;
+        s.camera.sampling = RT_SAMPLING_MATERIAL;
+    }
+    else if (name == "veach")
+    {
+        // Veach's four plates under four sphere lamps of one power (src/gpu_scenes_materials.ixx:237-270)
+        const d3 eye{0, 2, 15}, lamp_row{0, 2.5, -3.5};
+        const std::array<std::pair<d3, double>, 4> plates{
+            {{{0, 0.4, -2.4}, 0.05}, {{0, -0.5, -0.9}, 0.1}, {{0, -1.2, 0.7}, 0.2}, {{0, -1.7, 2.2}, 0.35}}};
+        for (const auto& [centre, roughness] : plates)
+        {
+            const d3 n = unit(unit(eye - centre) + unit(lamp_row - centre));
+            const d3 u{9, 0, 0};
+            const d3 v = unit(cross(n, d3{1, 0, 0}));
+            s.world.add(make_quad(centre - 0.5 * u - 0.5 * v, u, v, rough_metal({0.95, 0.93, 0.88}, roughness)));
+        }
+        const std::array<std::pair<double, d3>, 4> lamps{
+            {{0.05, {1.0, 0.5, 0.2}}, {0.15, {1.0, 0.8, 0.55}}, {0.4, {0.8, 0.9, 1.0}}, {1.0, {0.5, 0.7, 1.0}}}};
+        for (std::size_t i = 0; i < lamps.size(); ++i)
+        {
+            const auto& [radius, tint] = lamps[i];
+            const double y = 0.2126 * tint.x + 0.7152 * tint.y + 0.0722 * tint.z;
+            s.world.add(make_sphere({-3.75 + 2.5 * static_cast<double>(i), 2.5, -3.5}, radius,
+                                    lamp_light((4 / (radius * radius) / y) * tint)));
+        }
+        s.world.add(make_quad({-15, -10, -6}, {30, 0, 0}, {0, 25, 0}, lambertian(d3{0.1, 0.1, 0.1})));
+        s.camera.aspect_ratio = 16.0 / 9.0;
+        s.camera.image_width = 1920;
+        s.camera.samples_per_pixel = 1024;
+        s.camera.max_depth = 10;
+        s.camera.sampling = RT_SAMPLING_MATERIAL;
+        s.camera.vfov = 28;
+        s.camera.lookfrom = eye;
+        s.camera.lookat = {0, 0.4, 0};
+        s.combination = RT_COMBINE_POWER_HEURISTIC;
+    }
+    else if (name == "sun-test")
+    {
+        // A grey floor and three matte spheres (albedo 0.8, 0.5, 0.2) in sunlight (src/gpu_scenes_environments.ixx:
+        // 172-190), under this library's seeded sun sky (the reference's is a photographed HDR asset)
+        s.world.add(make_quad({-100, 0, -100}, {0, 0, 200}, {200, 0, 0}, lambertian(d3{0.5, 0.5, 0.5})));
+        const std::array<std::pair<double, double>, 3> balls{{{-2.2, 0.8}, {0, 0.5}, {2.2, 0.2}}};
+        for (const auto& [x, albedo] : balls)
+        {
+            s.world.add(make_sphere({x, 1, 0}, 1.0, lambertian(d3{albedo, albedo, albedo})));
+        }
+        s.camera.aspect_ratio = 16.0 / 9.0;
+        s.camera.image_width = 480;
+        s.camera.samples_per_pixel = 1024;
+        s.camera.max_depth = 10;
+        s.camera.sampling = RT_SAMPLING_MATERIAL;
+        s.camera.vfov = 30;
+        s.camera.lookfrom = {0, 2, 9};
+        s.camera.lookat = {0, 0.9, 0};
+        s.environment = environment_def{sun_sky(1024, 512, 48, 48, 30, 0.6, 10000), 0, 1};
+        s.combination = RT_COMBINE_POWER_HEURISTIC;
+    }
+    else if (name == "mesh-furnace")
+    {
+        // Two icospheres of 5120 triangles under a white sky, one white matte, one a white mirror
+        // (src/gpu_scenes.ixx:84-85): every path ends in the sky with its throughput 1, so every pixel is exactly 1
+        s.meshes.push_back(displaced_sphere(4, wave_field(1, 1, 1.0, 1.0), 0.0, 1.0, {-1.2, 0, 0},
+                                            lambertian(d3{1, 1, 1})));
+        s.meshes.push_back(displaced_sphere(4, wave_field(1, 1, 1.0, 1.0), 0.0, 1.0, {1.2, 0, 0}, metal({1, 1, 1}, 0)));
+        s.camera.aspect_ratio = 16.0 / 9.0;
+        s.camera.image_width = 480;
+        s.camera.samples_per_pixel = 64;
+        s.camera.max_depth = 50;
+        s.camera.sampling = RT_SAMPLING_MATERIAL;
+        s.camera.vfov = 30;
+        s.camera.lookfrom = {0, 0, 6};
+        s.camera.lookat = {0, 0, 0};
+        s.camera.sky_bottom = {1, 1, 1};
+        s.camera.sky_top = {1, 1, 1};
+    }
+    else if (name == "triangles")
+    {
+        // The triangle-heavy scene: a seeded displaced icosphere of 1,310,720 triangles (level 8) in a GGX metal of
+        // roughness U[0.05, 0.6], 64 instances of a seeded 20,480-triangle blob (level 5) in matte and glossy-coat
+        // materials, on a ground quad, under a seeded 1024x512 HDR sky with a sun disk of radiance 10^4, lit by the
+        // map's light samples weighed by MIS. 1920x1080, 256 samples, depth 16.
+        std::mt19937_64 g(20250603);
+        std::uniform_real_distribution<double> u(0.0, 1.0);
+        const double metal_roughness = 0.05 + 0.55 * u(g);
+        s.world.add(make_quad({-20, 0, -20}, {0, 0, 40}, {40, 0, 0}, lambertian(d3{0.5, 0.5, 0.5})));
+        s.meshes.push_back(displaced_sphere(8, wave_field(11, 24, 3.0, 12.0), 0.05, 1.6, {0, 1.75, 0},
+                                            rough_metal({0.955, 0.637, 0.538}, metal_roughness)));
+        const wave_field blob_field(12, 10, 2.0, 7.0);
+        const d3 matte_color{0.2 + 0.6 * u(g), 0.2 + 0.6 * u(g), 0.2 + 0.6 * u(g)};
+        const d3 coat_color{0.1 + 0.6 * u(g), 0.1 + 0.6 * u(g), 0.1 + 0.6 * u(g)};
+        const double coat_roughness = 0.1 + 0.4 * u(g);
+        s.meshes.push_back(displaced_sphere(5, blob_field, 0.25, 1.0, {0, 0, 0}, lambertian(matte_color)));
+        s.meshes.push_back(displaced_sphere(5, blob_field, 0.25, 1.0, {0, 0, 0}, glossy(coat_color, coat_roughness)));
+        for (int k = 0; k < 64; ++k)
+        {
+            // an 8 x 8 grid of cells 2.4 wide, each blob jittered in its cell; the four middle cells move out of the
+            // large sphere's way
+            const int row = k / 8, column = k % 8;
+            const double scale = 0.35 + 0.25 * u(g);
+            double x = -8.4 + 2.4 * column + 0.6 * (u(g) - 0.5);
+            double z = -8.4 + 2.4 * row + 0.6 * (u(g) - 0.5);
+            if (std::fabs(x) < 2.6 && std::fabs(z) < 2.6)
+            {
+                x *= 2.6;
+                z *= 2.6;
+            }
+            const double yaw = 2 * std::numbers::pi * u(g);
+            rt_mesh_instance_desc in{};
+            in.mesh = 1 + (k & 1);
+            const double c = scale * std::cos(yaw), sn = scale * std::sin(yaw);
+            const double m[12] = {c, 0, sn, x, 0, scale, 0, 1.25 * scale, -sn, 0, c, z};
+            std::copy(m, m + 12, in.object_to_world);
+            s.instances.push_back(in);
+        }
+        rt_mesh_instance_desc big{};
+        big.mesh = 0;
+        const double identity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        std::copy(identity, identity + 12, big.object_to_world);
+        s.instances.push_back(big);
+        s.camera.aspect_ratio = 16.0 / 9.0;
+        s.camera.image_width = 1920;
+        s.camera.samples_per_pixel = 256;
+        s.camera.max_depth = 16;
+        s.camera.sampling = RT_SAMPLING_MATERIAL;
+        s.camera.vfov = 40;
+        s.camera.lookfrom = {0, 5.5, 15};
+        s.camera.lookat = {0, 1.2, 0};
+        s.environment = environment_def{sun_sky(1024, 512, 7, 35, 60, 0.6, 10000), 0, 1};
+        s.combination = RT_COMBINE_POWER_HEURISTIC;
+    }
     else
     {
         throw std::invalid_argument("unknown scene '" + name + "'");
@@ -829,10 +1226,29 @@ public:
         }
     }
 
+    // A mesh's triangles keep their vertices; its material joins the table like any other
+    void add_mesh(const mesh_def& m)
+    {
+        rt_mesh_desc d{};
+        d.material = material_index(m.material.get());
+        out_.mesh_positions.push_back(m.positions);
+        out_.mesh_indices.push_back(m.indices);
+        out_.meshes.push_back(d);
+    }
+
+    // The GGX side entries, one per material, when some material has one (src/gpu_scene.ixx:700-740)
+    void finish_microfacets()
+    {
+        const bool any = std::any_of(sides_.begin(), sides_.end(),
+                                     [](const rt_microfacet_desc& s) { return s.kind != RT_MICROFACET_NONE; });
+        out_.microfacets = any ? sides_ : std::vector<rt_microfacet_desc>{};
+    }
+
 private:
     scene_tables& out_;
     std::unordered_map<const material_def*, int> materials_;
     std::unordered_map<const texture_def*, int> textures_;
+    std::vector<rt_microfacet_desc> sides_;
 
     void add_leaf(const object& o, const placement& where)
     {
@@ -924,6 +1340,7 @@ private:
         }
         const int index = static_cast<int>(out_.materials.size());
         out_.materials.push_back(d);
+        sides_.push_back({m->microfacet, 0, m->roughness});
         materials_.emplace(m, index);
         return index;
     }
@@ -1126,6 +1543,10 @@ built_scene build_scene(const std::string& name, const rt_scene_overrides& overr
     // ask for light samples (src/gpu_scene.ixx:1375-1440)
     int requested = overrides.sampling;
     int combination = RT_COMBINE_BSDF_ONLY;
+    if (requested == -1 && s.combination >= 0)
+    {
+        requested = s.combination == RT_COMBINE_LIGHTS_ONLY ? 4 : (s.combination == RT_COMBINE_POWER_HEURISTIC ? 5 : 1);
+    }
     if (requested >= 0 && requested <= 3)
     {
         cam.sampling = requested;
@@ -1158,6 +1579,20 @@ built_scene build_scene(const std::string& name, const rt_scene_overrides& overr
         const object_ref lights = s.lights.finish();
         f.add_lights(*lights, placement{}, 1.0);
     }
+    for (const mesh_def& m : s.meshes)
+    {
+        f.add_mesh(m);
+    }
+    t.instances = s.instances;
+    f.finish_microfacets();
+    if (s.environment)
+    {
+        t.has_environment = true;
+        t.environment_pixels = s.environment->image->rgb;
+        t.environment.image = {s.environment->image->width, s.environment->image->height, nullptr};
+        t.environment.yaw_degrees = s.environment->yaw_degrees;
+        t.environment.scale = s.environment->scale;
+    }
     t.combination = combination;
     t.relink();
     result.passes = overrides.strata > 0 ? samples
@@ -1168,6 +1603,6 @@ built_scene build_scene(const std::string& name, const rt_scene_overrides& overr
 const char* scene_names()
 {
     return "materials,weekend,bouncing,checker,earth,perlin,quads,light,cornell,smoke,final,aluminum,glass,furnace,"
-           "triangles";
+           "ggx-furnace,gloss-furnace,env-furnace,mesh-furnace,cornell-ggx,veach,sun-test,triangles";
 }
 }

@@ -59,14 +59,22 @@ public:
         : scene_(scene_tables::copy_of(scene)), options_(options), dscene_(scene_.view())
     {
         view_scene_ = scene_.view();
-        all_books_ = !first_book_scene(view_scene_);
+        mode_ = dscene_.quality()                 ? dev::mode_quality
+                : first_book_scene(view_scene_) ? dev::mode_first_book
+                                                : dev::mode_all_books;
         launch_seconds_ = options.launch_seconds > 0 ? options.launch_seconds : 0.05;
         const int threads = options.block_size > 0 ? options.block_size : 128;
         block_w_ = 16;
         block_h_ = std::max(1, threads / 16);
-        use_wavefront_ = options.integrator == RT_INTEGRATOR_WAVEFRONT ||
-                         (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>());
-        rays_ = device_array<unsigned long long>(1);
+        if (options.integrator == RT_INTEGRATOR_WAVEFRONT && mode_ == dev::mode_quality)
+        {
+            throw std::invalid_argument("the wavefront integrator doesn't take light samples, GGX, environment maps "
+                                        "or triangles yet: use the megakernel");
+        }
+        use_wavefront_ = mode_ != dev::mode_quality &&
+                         (options.integrator == RT_INTEGRATOR_WAVEFRONT ||
+                          (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>()));
+        rays_ = device_array<unsigned long long>(2);
         start_view(scene.camera);
     }
 
@@ -218,15 +226,15 @@ public:
     void counters(uint64_t* rays, uint64_t* shadow_rays, double* kernel_seconds) const override
     {
         require_view();
-        unsigned long long n = 0;
-        check(hipMemcpy(&n, rays_.data(), sizeof n, hipMemcpyDeviceToHost), "reading the ray counter");
+        unsigned long long n[2] = {};
+        check(hipMemcpy(n, rays_.data(), sizeof n, hipMemcpyDeviceToHost), "reading the ray counters");
         if (rays)
         {
-            *rays = n;
+            *rays = n[0];
         }
         if (shadow_rays)
         {
-            *shadow_rays = 0;
+            *shadow_rays = n[1];
         }
         if (kernel_seconds)
         {
@@ -247,8 +255,8 @@ public:
         }
         else
         {
-            a = megakernel_attributes<Real>(all_books_, block_w_ * block_h_, dscene_.view().stack_depth);
-            name = all_books_ ? "all books" : "first book";
+            a = megakernel_attributes<Real>(mode_, block_w_ * block_h_, dscene_.view().stack_depth);
+            name = mode_ == dev::mode_quality ? "quality" : (mode_ == dev::mode_all_books ? "all books" : "first book");
             info.threads_per_block = block_w_ * block_h_;
         }
         std::snprintf(info.name, sizeof info.name, "%s%s", name, sizeof(Real) == 8 ? " fp64" : "");
@@ -305,30 +313,33 @@ public:
             throw std::invalid_argument("traffic is counted over 1 to sqrt_spp^2 passes of the first round");
         }
         device_array<double> scratch(values());
-        device_array<unsigned long long> counts(1 + 16);
+        // [rays, shadow rays, the tally events]
+        device_array<unsigned long long> counts(2 + RT_TRAFFIC_EVENTS);
         check(hipMemsetAsync(scratch.data(), 0, scratch.bytes(), stream_.get()), "clearing scratch");
         check(hipMemsetAsync(counts.data(), 0, counts.bytes(), stream_.get()), "clearing counters");
         const dev::launch_work work{0, static_cast<int32_t>(passes), 0, camera_.image_height};
-        check(launch_megakernel<Real>(all_books_, cam_, dscene_.view(), options_.seed, work, scratch.data(),
-                                      counts.data(), block_w_, block_h_, stream_.get(), counts.data() + 1),
+        check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), options_.seed, work, scratch.data(),
+                                      counts.data(), block_w_, block_h_, stream_.get(), counts.data() + 2),
               "launching the counting kernel");
         check(hipStreamSynchronize(stream_.get()), "counting traffic");
-        unsigned long long host[17] = {};
+        unsigned long long host[2 + RT_TRAFFIC_EVENTS] = {};
         check(hipMemcpy(host, counts.data(), sizeof host, hipMemcpyDeviceToHost), "reading counters");
         rt_traffic t{};
         t.paths = static_cast<uint64_t>(passes) * static_cast<uint64_t>(values() / 3);
         t.rays = host[0];
+        t.shadow_rays = host[1];
         const double sizes[RT_TRAFFIC_EVENTS] = {
             sizeof(dev::wide_node<Real>), sizeof(dev::sphere_rec<Real>), sizeof(dev::moving_sphere_rec<Real>),
             sizeof(dev::quad_rec<Real>), sizeof(dev::medium_rec<Real>), sizeof(dev::quad_rec<Real>),
             4.0 + sizeof(dev::material_rec<Real>), sizeof(dev::sphere_frame<Real>), sizeof(dev::texture_rec<Real>),
-            sizeof(dev::texel), 4.0 * sizeof(Real) + 3.0, sizeof(dev::light_rec<Real>)};
+            sizeof(dev::texel), 4.0 * sizeof(Real) + 3.0, sizeof(dev::light_rec<Real>),
+            sizeof(dev::triangle_rec<Real>), sizeof(dev::emitter_rec), sizeof(dev::env_texel), sizeof(dev::alias_bin)};
         double bytes = 0;
         for (int e = 0; e < RT_TRAFFIC_EVENTS; ++e)
         {
-            t.events[e] = host[1 + e];
+            t.events[e] = host[2 + e];
             t.event_bytes[e] = sizes[e];
-            bytes += sizes[e] * static_cast<double>(host[1 + e]);
+            bytes += sizes[e] * static_cast<double>(host[2 + e]);
         }
         // the pixel's double sums, read and written once per launch of `passes` passes
         bytes += 48.0 * static_cast<double>(values() / 3);
@@ -348,7 +359,7 @@ private:
     device_scene<Real> dscene_;
     stream_owner stream_;
     hip_event start_, stop_;
-    bool all_books_ = true;
+    int mode_ = dev::mode_all_books;
     bool use_wavefront_ = false;
     std::unique_ptr<wavefront<Real>> wavefront_;
     double launch_seconds_ = 0.05;
@@ -386,11 +397,12 @@ private:
             sums_ = device_array<double>(n);
         }
         check(hipMemsetAsync(sums_.data(), 0, sums_.bytes(), stream_.get()), "clearing the image");
-        check(hipMemsetAsync(rays_.data(), 0, sizeof(unsigned long long), stream_.get()), "clearing the counter");
+        check(hipMemsetAsync(rays_.data(), 0, rays_.bytes(), stream_.get()), "clearing the counters");
         if (use_wavefront_)
         {
             const int slots = options_.wavefront_slots;
-            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, all_books_, slots, stream_.get());
+            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, mode_ == dev::mode_all_books, slots,
+                                                           stream_.get());
         }
         check(hipStreamSynchronize(stream_.get()), "starting the view");
         state_ = {};
@@ -417,7 +429,7 @@ private:
         }
         else
         {
-            check(launch_megakernel<Real>(all_books_, cam_, dscene_.view(), seed, work, sums_.data(), rays_.data(),
+            check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), rays_.data(),
                                           block_w_, block_h_, stream_.get()),
                   "launching the kernel");
         }
