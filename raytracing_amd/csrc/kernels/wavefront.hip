@@ -16,6 +16,7 @@
 // (read from device memory, so the loop needs no host synchronisation and can be captured in a hipGraph) runs out.
 #include "kernel_api.hpp"
 #include "path_tracer.hpp"
+#include "quality.hpp"
 #include "wavefront_api.hpp"
 
 namespace rt::dev
@@ -38,6 +39,7 @@ struct slot_path
 {
     path<float> p;
     pcg32 g;
+    bool specular = false; // quality integrator: the last vertex was a mirror's (or the camera)
 };
 
 __device__ inline slot_path load_path(const wf_pool& pool, uint32_t s)
@@ -52,9 +54,10 @@ __device__ inline slot_path load_path(const wf_pool& pool, uint32_t s)
     sp.p.r.direction = {d.x, d.y, d.z};
     sp.p.from = __float_as_int(d.w);
     sp.p.throughput = {t.x, t.y, t.z};
-    const int packed = __float_as_int(t.w);
-    sp.p.rays_left = packed & 0x3fffffff;
-    sp.p.inward = (packed >> 30) & 1;
+    const uint32_t packed = __float_as_uint(t.w);
+    sp.p.rays_left = static_cast<int>(packed & 0x3fffffffu);
+    sp.p.inward = (packed >> 30) & 1u;
+    sp.specular = (packed >> 31) != 0;
     sp.g.state = (static_cast<uint64_t>(r.y) << 32) | r.x;
     sp.g.inc = (static_cast<uint64_t>(r.w) << 32) | r.z;
     return sp;
@@ -70,8 +73,9 @@ __device__ inline void store_path(const wf_pool& pool, uint32_t s, const slot_pa
 {
     pool.ray_o[s] = make_float4(sp.p.r.origin.x, sp.p.r.origin.y, sp.p.r.origin.z, sp.p.r.time);
     pool.ray_d[s] = make_float4(sp.p.r.direction.x, sp.p.r.direction.y, sp.p.r.direction.z, __int_as_float(sp.p.from));
-    pool.thr[s] = make_float4(sp.p.throughput.x, sp.p.throughput.y, sp.p.throughput.z,
-                              __int_as_float(sp.p.rays_left | (sp.p.inward ? 1 << 30 : 0)));
+    const uint32_t packed = static_cast<uint32_t>(sp.p.rays_left) | (sp.p.inward ? 1u << 30 : 0u) |
+                            (sp.specular ? 1u << 31 : 0u);
+    pool.thr[s] = make_float4(sp.p.throughput.x, sp.p.throughput.y, sp.p.throughput.z, __uint_as_float(packed));
     store_rng(pool, s, sp.g);
 }
 
@@ -123,6 +127,18 @@ __device__ inline int hit_class(const scene_view<float>& sc, int index)
         return wf_class_miss;
     }
     const int m = index >= sc.media_base ? sc.media[index - sc.media_base].material : sc.prim_materials[index];
+    if (sc.quality.microfacets)
+    {
+        const int side = sc.quality.microfacets[m].kind;
+        if (side == RT_MICROFACET_CONDUCTOR)
+        {
+            return wf_class_ggx;
+        }
+        if (side == RT_MICROFACET_GLOSS)
+        {
+            return wf_class_gloss;
+        }
+    }
     const material_rec<float>& mat = sc.materials[m];
     switch (mat.kind)
     {
@@ -143,13 +159,18 @@ __device__ inline int hit_class(const scene_view<float>& sc, int index)
     }
 }
 
-// The first path of a slot: its pixel's pass `pass`
+// The first path of a slot: its pixel's pass `pass`. A quality path starts as the camera's: specular, no light yet.
 __device__ inline slot_path start_slot(const camera_rec<float>& cam, const scene_view<float>& sc, uint64_t seed,
-                                       uint64_t pixel, int i, int j, int pass)
+                                       uint64_t pixel, int i, int j, int pass, const wf_pool& pool, uint32_t s)
 {
     slot_path sp;
     sp.g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
     sp.p = start_path(cam, sc.motion != 0, i, j, pass, sp.g);
+    if (pool.qst)
+    {
+        sp.specular = true;
+        pool.qst[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     return sp;
 }
 }
@@ -170,7 +191,7 @@ __global__ void __launch_bounds__(256) wf_init(const camera_rec<float> cam, cons
     const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
     const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
     const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
-    const slot_path sp = start_slot(cam, sc, seed, pixel, i, j, work.first_pass);
+    const slot_path sp = start_slot(cam, sc, seed, pixel, i, j, work.first_pass, pool, s);
     store_path(pool, s, sp);
     pool.pass[s] = work.first_pass;
     pool.active[0][s] = s;
@@ -305,7 +326,7 @@ __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, con
                 if (next < end_pass)
                 {
                     pool.pass[s] = next;
-                    store_path(pool, s, start_slot(cam, sc, seed, pixel, i, j, next));
+                    store_path(pool, s, start_slot(cam, sc, seed, pixel, i, j, next, pool, s));
                     keep = true;
                 }
             }
@@ -315,6 +336,90 @@ __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, con
         {
             pool.active[1 - p][at] = s;
         }
+    }
+}
+
+// The quality integrator's shade: the same queues in class order, each vertex's light sample and its shadow ray
+// traced here (an any-hit walk with this thread's LDS stack), in the megakernel's order of random numbers
+__global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> cam, const scene_view<float> sc,
+                                                        const uint64_t seed, const launch_work work, wf_pool pool,
+                                                        int p, double* __restrict__ sums,
+                                                        unsigned long long* __restrict__ ray_count)
+{
+    const lds_stack<float> stack = thread_stack(sc.stack_depth);
+    uint32_t* in = pool.counters + wf_counter_stride * p;
+    uint32_t* out = pool.counters + wf_counter_stride * (1 - p);
+    uint32_t offsets[wf_classes + 1];
+    offsets[0] = 0;
+    for (int c = 0; c < wf_classes; ++c)
+    {
+        offsets[c + 1] = offsets[c] + in[wf_counter_class + c];
+    }
+    const uint32_t total = offsets[wf_classes];
+    const int end_pass = work.first_pass + work.pass_count;
+    tally<false> tl;
+    uint32_t shadow_rays = 0;
+    uint32_t base;
+    while (wave_fetch(in + wf_counter_shade_work, total, base))
+    {
+        const uint32_t g = base + static_cast<uint32_t>(lane_id());
+        bool keep = false;
+        uint32_t s = 0;
+        if (g < total)
+        {
+            int c = 0;
+            while (g >= offsets[c + 1])
+            {
+                ++c;
+            }
+            s = pool.queue[static_cast<size_t>(c) * pool.capacity + (g - offsets[c])];
+            slot_path sp = load_path(pool, s);
+            const float4 st = pool.qst[s];
+            quality_path q;
+            q.base = sp.p;
+            q.radiance = {st.x, st.y, st.z};
+            q.bsdf_pdf = st.w;
+            q.specular = sp.specular;
+            const int4 hv = pool.hit[s];
+            const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
+            vec3<float> sample;
+            if (!shade_quality(q, sc, cam, h, sp.g, sample, stack, tl, shadow_rays))
+            {
+                sp.p = q.base;
+                sp.specular = q.specular;
+                pool.qst[s] = make_float4(q.radiance.x, q.radiance.y, q.radiance.z, q.bsdf_pdf);
+                store_path(pool, s, sp);
+                keep = true;
+            }
+            else
+            {
+                const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
+                const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
+                const uint64_t pixel =
+                    static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
+                double* const sum = sums + 3 * pixel;
+                sum[0] += without_nan(sample.x);
+                sum[1] += without_nan(sample.y);
+                sum[2] += without_nan(sample.z);
+                const int next = pool.pass[s] + 1;
+                if (next < end_pass)
+                {
+                    pool.pass[s] = next;
+                    store_path(pool, s, start_slot(cam, sc, seed, pixel, i, j, next, pool, s));
+                    keep = true;
+                }
+            }
+        }
+        const uint32_t at = wave_append(out, keep);
+        if (keep)
+        {
+            pool.active[1 - p][at] = s;
+        }
+    }
+    const unsigned long long shadows = wsum(shadow_rays);
+    if (lane_id() == 0 && shadows)
+    {
+        atomicAdd(ray_count + 1, shadows);
     }
 }
 }
@@ -331,28 +436,38 @@ hipError_t wf_launch_init(const dev::camera_rec<float>& cam, const dev::scene_vi
 
 hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
                             const dev::launch_work& work, const dev::wf_pool& pool, int parity, double* sums,
-                            unsigned long long* rays, int extend_blocks, int shade_blocks, hipStream_t stream)
+                            unsigned long long* rays, int extend_blocks, int shade_blocks, bool quality,
+                            hipStream_t stream)
 {
     hipLaunchKernelGGL(dev::wf_reset, dim3(1), dim3(64), 0, stream, pool, parity);
     const size_t lds = stack_lds_bytes<float>(sc.stack_depth, 256);
     hipLaunchKernelGGL(dev::wf_extend, dim3(extend_blocks), dim3(256), lds, stream, sc, pool, parity, rays);
-    hipLaunchKernelGGL(dev::wf_shade, dim3(shade_blocks), dim3(256), 0, stream, cam, sc, seed, work, pool, parity,
-                       sums);
+    if (quality)
+    {
+        hipLaunchKernelGGL(dev::wf_shade_quality, dim3(shade_blocks), dim3(256), lds, stream, cam, sc, seed, work,
+                           pool, parity, sums, rays);
+    }
+    else
+    {
+        hipLaunchKernelGGL(dev::wf_shade, dim3(shade_blocks), dim3(256), 0, stream, cam, sc, seed, work, pool,
+                           parity, sums);
+    }
     return hipGetLastError();
 }
 
 kernel_attributes wf_attributes(int depth, int which)
 {
     hipFuncAttributes a{};
-    const void* f = which == 0 ? reinterpret_cast<const void*>(&dev::wf_extend)
-                               : reinterpret_cast<const void*>(&dev::wf_shade);
+    const void* f = which == 0   ? reinterpret_cast<const void*>(&dev::wf_extend)
+                    : which == 1 ? reinterpret_cast<const void*>(&dev::wf_shade)
+                                 : reinterpret_cast<const void*>(&dev::wf_shade_quality);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)
     {
         k.registers = a.numRegs;
         k.local_bytes = static_cast<int>(a.localSizeBytes);
     }
-    k.lds_bytes = which == 0 ? static_cast<int>(stack_lds_bytes<float>(depth, 256)) : 0;
+    k.lds_bytes = which != 1 ? static_cast<int>(stack_lds_bytes<float>(depth, 256)) : 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, 256, k.lds_bytes) == hipSuccess)
     {

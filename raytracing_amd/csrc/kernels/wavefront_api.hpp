@@ -22,6 +22,8 @@ enum wf_class : int
     wf_class_smoke,
     wf_class_metal,
     wf_class_glass,
+    wf_class_ggx,   // quality integrator: a GGX conductor
+    wf_class_gloss, // quality integrator: a glossy coat
     wf_classes
 };
 
@@ -37,7 +39,8 @@ struct wf_pool
 {
     float4* ray_o;  // origin, time
     float4* ray_d;  // direction, the primitive the ray leaves (int bits)
-    float4* thr;    // throughput, rays left | inward << 30 (int bits)
+    float4* thr;    // throughput, rays left | inward << 30 | specular << 31 (int bits)
+    float4* qst;    // quality integrator only (else nullptr): the light gathered so far, the last bsdf density
     uint4* rng;     // pcg32 state and increment
     int4* hit;      // the closest event: primitive id, t (bits), entering
     int32_t* pass;  // the pass the slot's path belongs to
@@ -52,9 +55,11 @@ namespace rt
 {
 hipError_t wf_launch_init(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
                           const dev::launch_work& work, const dev::wf_pool& pool, hipStream_t stream);
+// rays: [0] the rays traced, [1] the shadow rays (quality)
 hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
                             const dev::launch_work& work, const dev::wf_pool& pool, int parity, double* sums,
-                            unsigned long long* rays, int extend_blocks, int shade_blocks, hipStream_t stream);
-// which: 0 extend, 1 shade
+                            unsigned long long* rays, int extend_blocks, int shade_blocks, bool quality,
+                            hipStream_t stream);
+// which: 0 extend, 1 shade, 2 the quality integrator's shade
 kernel_attributes wf_attributes(int depth, int which);
 }

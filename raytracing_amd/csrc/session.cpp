@@ -66,14 +66,8 @@ public:
         const int threads = options.block_size > 0 ? options.block_size : 128;
         block_w_ = 16;
         block_h_ = std::max(1, threads / 16);
-        if (options.integrator == RT_INTEGRATOR_WAVEFRONT && mode_ == dev::mode_quality)
-        {
-            throw std::invalid_argument("the wavefront integrator doesn't take light samples, GGX, environment maps "
-                                        "or triangles yet: use the megakernel");
-        }
-        use_wavefront_ = mode_ != dev::mode_quality &&
-                         (options.integrator == RT_INTEGRATOR_WAVEFRONT ||
-                          (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>()));
+        use_wavefront_ = options.integrator == RT_INTEGRATOR_WAVEFRONT ||
+                         (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>());
         rays_ = device_array<unsigned long long>(2);
         start_view(scene.camera);
     }
@@ -401,8 +395,7 @@ private:
         if (use_wavefront_)
         {
             const int slots = options_.wavefront_slots;
-            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, mode_ == dev::mode_all_books, slots,
-                                                           stream_.get());
+            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, mode_, slots, stream_.get());
         }
         check(hipStreamSynchronize(stream_.get()), "starting the view");
         state_ = {};

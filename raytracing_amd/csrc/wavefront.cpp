@@ -34,9 +34,9 @@ public:
 }
 
 template <typename Real>
-wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, bool, int slots,
+wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, int mode, int slots,
                            hipStream_t stream)
-    : stream_(stream)
+    : stream_(stream), quality_(mode == dev::mode_quality)
 {
     if constexpr (!wavefront_supported<Real>())
     {
@@ -61,6 +61,10 @@ wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_re
         ray_o_ = device_array<float4>(capacity_);
         ray_d_ = device_array<float4>(capacity_);
         thr_ = device_array<float4>(capacity_);
+        if (quality_)
+        {
+            qst_ = device_array<float4>(capacity_);
+        }
         rng_ = device_array<uint4>(capacity_);
         hit_ = device_array<int4>(capacity_);
         pass_ = device_array<int32_t>(capacity_);
@@ -71,12 +75,13 @@ wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_re
         check(hipMemsetAsync(counters_.data(), 0, counters_.bytes(), stream_), "clearing counters");
         check(hipHostMalloc(reinterpret_cast<void**>(&host_count_), sizeof(uint32_t), hipHostMallocDefault),
               "allocating pinned memory");
-        pool_ = {ray_o_.data(), ray_d_.data(), thr_.data(), rng_.data(), hit_.data(), pass_.data(),
-                 {active0_.data(), active1_.data()}, queue_.data(), counters_.data(), capacity_};
-        bytes_ = ray_o_.bytes() * 3 + rng_.bytes() + hit_.bytes() + pass_.bytes() + active0_.bytes() * 2 +
-                 queue_.bytes() + counters_.bytes();
+        pool_ = {ray_o_.data(), ray_d_.data(), thr_.data(), qst_.size() > 0 ? qst_.data() : nullptr, rng_.data(),
+                 hit_.data(), pass_.data(), {active0_.data(), active1_.data()}, queue_.data(), counters_.data(),
+                 capacity_};
+        bytes_ = ray_o_.bytes() * 3 + qst_.bytes() + rng_.bytes() + hit_.bytes() + pass_.bytes() +
+                 active0_.bytes() * 2 + queue_.bytes() + counters_.bytes();
         extend_blocks_ = blocks_for(wf_attributes(sc.stack_depth, 0));
-        shade_blocks_ = blocks_for(wf_attributes(sc.stack_depth, 1));
+        shade_blocks_ = blocks_for(wf_attributes(sc.stack_depth, quality_ ? 2 : 1));
     }
 }
 
@@ -107,7 +112,7 @@ void wavefront<Real>::render(uint64_t seed, const dev::launch_work& work, double
         for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
         {
             launched = wf_launch_bounce(cam_, sc_, seed, work, pool_, b & 1, sums, rays, extend_blocks_,
-                                        shade_blocks_, stream_);
+                                        shade_blocks_, quality_, stream_);
         }
         const hipError_t ended = hipStreamEndCapture(stream_, &g.graph);
         check(launched, "launching the bounce loop's kernels");
@@ -137,7 +142,7 @@ template <typename Real>
 kernel_attributes wavefront<Real>::attributes() const
 {
     kernel_attributes a = wf_attributes(sc_.stack_depth, 0);
-    const kernel_attributes s = wf_attributes(sc_.stack_depth, 1);
+    const kernel_attributes s = wf_attributes(sc_.stack_depth, quality_ ? 2 : 1);
     a.registers = std::max(a.registers, s.registers);
     a.local_bytes = std::max(a.local_bytes, s.local_bytes);
     return a;

@@ -25,8 +25,8 @@ template <typename Real>
 class wavefront
 {
 public:
-    // slots: the pool's size in paths (0: one per pixel of the image)
-    wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, bool all_books, int slots,
+    // slots: the pool's size in paths (0: one per pixel of the image); mode: the scene's kernel_mode
+    wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, int mode, int slots,
               hipStream_t stream);
     ~wavefront();
     wavefront(const wavefront&) = delete;
@@ -44,8 +44,9 @@ private:
     dev::scene_view<float> sc_{};
     dev::camera_rec<float> cam_{};
     hipStream_t stream_ = nullptr;
+    bool quality_ = false;
     uint32_t capacity_ = 0;
-    device_array<float4> ray_o_, ray_d_, thr_;
+    device_array<float4> ray_o_, ray_d_, thr_, qst_;
     device_array<uint4> rng_;
     device_array<int4> hit_;
     device_array<int32_t> pass_;
