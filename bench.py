@@ -5,12 +5,20 @@ image's double sums resident in device memory. `value` is paths per second over 
 them; every launch is waited for). Also reported:
   - roofline: the dominant kernel's algorithmic bytes per launch (bytes per path, counted by the instrumented kernel
     over 16 passes of the same image, times the paths of one launch) over its average launch time (HIP events on the
-    session's own stream, over the timed region), against 8 TB/s of HBM;
+    session's own stream, over the timed region), against 8 TB/s of HBM; `traffic` is the HBM bytes per launch that
+    rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured for the same workload, when profiles/hbm_traffic.txt
+    holds a line for it (scaled by the counters' reading for a copy kernel of known size);
   - cpu_baseline: the CPU restatement of the reference renderer (double, mt19937_64 per row, the reference's
-    algorithms) timed on this box's host cores on a bounded sample: the same scene at 1080x1080 with 16 spp;
-  - megakernel: the plain FP32 megakernel's Mpaths/s on the same workload, beside the integrator measured.
+    algorithms) timed on this box's host cores on a bounded sample: the same scene at 1080x1080 with 64 spp;
+  - megakernel: the plain FP32 megakernel's Mpaths/s on the same workload, beside the integrator measured. The two
+    integrators trace the same paths with the same random numbers (their images are the same bytes), so at equal
+    spp they are at equal RMSE, and the ratio of their Mpaths/s is the ratio at equal RMSE.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--integrator auto|megakernel|wavefront] [--spp N] [--no-cpu]
+Other workloads (--scene): weekend (400x225, 64 spp, depth 50), glass (600x600, 1024 spp, depth 50, -p mixture),
+glass-mis (the same with -p mis), triangles (1920x1080, 256 spp, depth 16, -p mis).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--integrator auto|megakernel|wavefront] [--scene NAME]
+                [--spp N] [--no-cpu] [--no-megakernel]
 """
 import argparse
 import json
@@ -22,6 +30,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+# name: (scene, width, spp, sampling, description)
+WORKLOADS = {
+    "final": ("final", 1080, 1024, None, "final 1080x1080, 1024 spp (32x32 strata), depth 40, FP32, -p mixture, "
+                                         "seeded 2048x1024 noise texture"),
+    "weekend": ("weekend", 400, 64, None, "weekend 400x225, 64 spp, depth 50, FP32"),
+    "glass": ("glass", 600, 1024, None, "glass 600x600, 1024 spp, depth 50, FP32, -p mixture"),
+    "glass-mis": ("glass", 600, 1024, "mis", "glass 600x600, 1024 spp, depth 50, FP32, -p mis"),
+    "triangles": ("triangles", 1920, 256, None, "triangles 1920x1080, 256 spp, depth 16, FP32, -p mis, "
+                                                "2.62 M triangles, 1024x512 HDR sky with a sun"),
+}
 
 
 def measure(rt, scene, integrator, steps, warmup, seed=0):
@@ -45,17 +64,29 @@ def measure(rt, scene, integrator, steps, warmup, seed=0):
     return value, 1000 * elapsed / steps, gpu_seconds / max(launches, 1), launches / steps, s
 
 
-def cpu_baseline(width: int, spp: int):
+def cpu_baseline(name: str, width: int, spp: int):
     import oracle
 
     threads = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
-    img, rays = oracle.render("final", width=width, spp=spp, seed=0, threads=threads, earth_width=2048)
+    img, rays = oracle.render(name, width=width, spp=spp, seed=0, threads=threads, earth_width=2048)
     seconds = time.perf_counter() - t0
     paths = img.shape[0] * img.shape[1] * spp
     return {"value": round(paths / seconds / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"final {img.shape[1]}x{img.shape[0]}, {spp} spp, depth 40, double, {seconds:.1f} s, "
+            "sample": f"{name} {img.shape[1]}x{img.shape[0]}, {spp} spp, double, {seconds:.1f} s, "
                       f"{rays / paths:.2f} rays/path"}
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch for this workload and kernel, from the rocprofv3 counter summary, or None"""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 4 and not line.startswith("#") and parts[0] == workload and parts[1] == kernel:
+            return float(parts[2]) + float(parts[3])
+    return None
 
 
 def main():
@@ -64,8 +95,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--integrator", default="auto", choices=["auto", "megakernel", "wavefront"])
-    ap.add_argument("--spp", type=int, default=1024)
-    ap.add_argument("--width", type=int, default=1080)
+    ap.add_argument("--scene", default="final", choices=sorted(WORKLOADS))
+    ap.add_argument("--spp", type=int, default=0)
+    ap.add_argument("--width", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-megakernel", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -85,38 +117,43 @@ def main():
 
     import raytracing_amd as rt
 
-    scene = rt.build_scene("final", width=args.width, samples=args.spp)
-    print(f"final {scene.width}x{scene.height}, {scene.passes} spp; device {rt.query_device().name.decode()}",
-          file=sys.stderr, flush=True)
+    name, width, spp, sampling, description = WORKLOADS[args.scene]
+    width = args.width or width
+    spp = args.spp or spp
+    scene = rt.build_scene(name, width=width, samples=spp, sampling=sampling)
+    device = rt.query_device()
+    print(f"{args.scene}: {scene.width}x{scene.height}, {scene.passes} spp; device {device.name.decode()} "
+          f"({device.compute_units} CUs)", file=sys.stderr, flush=True)
     value, ms, avg_launch, launches_per_step, session = measure(rt, scene, args.integrator, args.steps, args.warmup)
-    kernel = session.kernel()
+    kernel = session.kernel().name.decode()
     print(f"{args.integrator}: {value:.1f} Mpaths/s, {ms:.1f} ms/step, {launches_per_step:.1f} launches/step, "
-          f"kernel {kernel.name.decode()}", file=sys.stderr, flush=True)
+          f"kernel {kernel}", file=sys.stderr, flush=True)
 
     # algorithmic bytes per path, counted over 16 passes of the same image by the instrumented megakernel
     traffic = session.traffic(16)
     bytes_per_path = traffic.bytes_per_path
     paths_per_launch = scene.width * scene.height * scene.passes / launches_per_step
     achieved = bytes_per_path * paths_per_launch / avg_launch / 1e9
-    events = {name: int(traffic.events[i]) for i, name in enumerate(rt.TRAFFIC_EVENTS)}
-    print(f"traffic: {bytes_per_path:.0f} B/path, {traffic.rays / traffic.paths:.2f} rays/path, {events}",
-          file=sys.stderr, flush=True)
+    events = {n: int(traffic.events[i]) for i, n in enumerate(rt.TRAFFIC_EVENTS) if traffic.events[i]}
+    print(f"traffic: {bytes_per_path:.0f} B/path, {traffic.rays / traffic.paths:.2f} rays/path, "
+          f"{traffic.shadow_rays / traffic.paths:.2f} shadow rays/path, {events}", file=sys.stderr, flush=True)
     session.close()
 
     mega = None
     if args.integrator != "megakernel" and not args.no_megakernel:
-        mv, mms, _, _, ms_ = measure(rt, scene, "megakernel", 1, 0)
+        mv, _, _, _, ms_ = measure(rt, scene, "megakernel", 1, 0)
         ms_.close()
         mega = round(mv, 2)
         print(f"megakernel: {mv:.1f} Mpaths/s", file=sys.stderr, flush=True)
 
     cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(1080, 16)
+    if not args.no_cpu and name != "triangles" and sampling is None:
+        cpu = cpu_baseline(name, width, 64 if name == "final" else 16)
         print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
+    hbm = measured_traffic(args.scene, kernel.split()[0])
     line = {
-        "metric": "Mpaths/s (1080^2 final scene)",
+        "metric": "Mpaths/s (1080^2 final scene)" if args.scene == "final" else f"Mpaths/s ({args.scene})",
         "value": round(value, 2),
         "unit": "Mpaths/s",
         "n_gpus": 1,
@@ -128,15 +165,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"final {scene.width}x{scene.height}, {scene.passes} spp (32x32 strata), depth 40, "
-                               f"FP32, mixture sampling, seeded 2048x1024 noise texture",
-                   "integrator": args.integrator, "kernel": kernel.name.decode(),
+        "config": {"workload": description, "integrator": args.integrator, "kernel": kernel,
                    "paths_per_step": scene.width * scene.height * scene.passes,
                    "rays_per_path": round(traffic.rays / traffic.paths, 3),
+                   "shadow_rays_per_path": round(traffic.shadow_rays / traffic.paths, 3),
                    "megakernel_mpaths_s": mega},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_path": round(bytes_per_path, 1), "avg_launch_ms": round(1000 * avg_launch, 3)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if hbm is None else round(hbm),
+                     "bytes_per_path": round(bytes_per_path, 1), "avg_launch_ms": round(1000 * avg_launch, 3),
+                     "paths_per_launch": round(paths_per_launch)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

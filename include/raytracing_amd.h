@@ -326,6 +326,10 @@ typedef struct rt_session_options
     double launch_seconds;  /* how long one launch is planned to take: (0, 0.5]; 0 means 0.05 */
     int32_t block_size;     /* megakernel threads per block: 0 for the tuned default */
     int32_t wavefront_slots;/* wavefront path slots: 0 for the tuned default */
+    int32_t wavefront_graph;/* wavefront bounces per captured hipGraph: 0 for the tuned default, -1 for none (every
+                               kernel launched on its own, as profilers that don't follow graphs need) */
+    int32_t wavefront_drain;/* wavefront: when fewer slots than this are active, each finishes its pixel's passes on
+                               its own (0: the tuned default, a quarter of the pool; -1: never) */
 } rt_session_options;
 
 typedef struct rt_render_report
@@ -335,6 +339,7 @@ typedef struct rt_render_report
     int32_t launches;
     double gpu_seconds;
     double longest_launch_seconds;
+    int64_t bounces;             /* wavefront: bounce steps launched by the call (extend + shade over the pool) */
 } rt_render_report;
 
 typedef struct rt_kernel_info

@@ -339,6 +339,97 @@ __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, con
     }
 }
 
+// The drain: when few slots are left active, each of them (a thread each) traces its current path and its pixel's
+// remaining passes of the launch to the end, one after the other, as the megakernel does: the bounces with a nearly
+// empty pool would cost a launch each for little work. The sums stay in pass order, so the image is the same bytes.
+template <int Mode>
+__global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, const scene_view<float> sc,
+                                                const uint64_t seed, const launch_work work, wf_pool pool, int p,
+                                                double* __restrict__ sums, unsigned long long* __restrict__ ray_count)
+{
+    const lds_stack<float> stack = thread_stack(sc.stack_depth);
+    const uint32_t n = pool.counters[wf_counter_stride * p];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long rays = 0;
+    uint32_t shadow_rays = 0;
+    if (k < n)
+    {
+        const uint32_t s = pool.active[p][k];
+        slot_path sp = load_path(pool, s);
+        const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
+        const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
+        const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
+        double* const sum = sums + 3 * pixel;
+        double red = sum[0], green = sum[1], blue = sum[2];
+        const int end_pass = work.first_pass + work.pass_count;
+        tally<false> tl;
+        quality_path q;
+        if constexpr (Mode == mode_quality)
+        {
+            const float4 st = pool.qst[s];
+            q.base = sp.p;
+            q.radiance = {st.x, st.y, st.z};
+            q.bsdf_pdf = st.w;
+            q.specular = sp.specular;
+        }
+        int pass = pool.pass[s];
+        while (true)
+        {
+            ++rays;
+            vec3<float> sample;
+            bool done;
+            if constexpr (Mode == mode_quality)
+            {
+                done = trace_quality(q, sc, cam, sp.g, sample, stack, tl, shadow_rays);
+            }
+            else if constexpr (Mode == mode_all_books)
+            {
+                done = trace_all_books(sp.p, sc, cam, sp.g, sample, stack, tl);
+            }
+            else
+            {
+                done = trace_first_book(sp.p, sc, cam, sp.g, sample, stack, tl);
+            }
+            if (!done)
+            {
+                continue;
+            }
+            red += without_nan(sample.x);
+            green += without_nan(sample.y);
+            blue += without_nan(sample.z);
+            if (++pass == end_pass)
+            {
+                break;
+            }
+            sp.g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
+            if constexpr (Mode == mode_quality)
+            {
+                q = start_quality(cam, sc.motion != 0, i, j, pass, sp.g);
+            }
+            else
+            {
+                sp.p = start_path(cam, sc.motion != 0, i, j, pass, sp.g);
+            }
+        }
+        sum[0] = red;
+        sum[1] = green;
+        sum[2] = blue;
+    }
+    rays = wsum(rays);
+    const unsigned long long shadows = wsum(shadow_rays);
+    if (lane_id() == 0)
+    {
+        if (rays)
+        {
+            atomicAdd(ray_count, rays);
+        }
+        if (shadows)
+        {
+            atomicAdd(ray_count + 1, shadows);
+        }
+    }
+}
+
 // The quality integrator's shade: the same queues in class order, each vertex's light sample and its shadow ray
 // traced here (an any-hit walk with this thread's LDS stack), in the megakernel's order of random numbers
 __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> cam, const scene_view<float> sc,
@@ -451,6 +542,31 @@ hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_
     {
         hipLaunchKernelGGL(dev::wf_shade, dim3(shade_blocks), dim3(256), 0, stream, cam, sc, seed, work, pool,
                            parity, sums);
+    }
+    return hipGetLastError();
+}
+
+hipError_t wf_launch_drain(int mode, const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
+                           const dev::launch_work& work, const dev::wf_pool& pool, uint32_t count, double* sums,
+                           unsigned long long* rays, hipStream_t stream)
+{
+    const int threads = 128;
+    const dim3 grid((count + threads - 1) / threads);
+    const size_t lds = stack_lds_bytes<float>(sc.stack_depth, threads);
+    if (mode == dev::mode_quality)
+    {
+        hipLaunchKernelGGL((dev::wf_drain<dev::mode_quality>), grid, dim3(threads), lds, stream, cam, sc, seed, work,
+                           pool, 0, sums, rays);
+    }
+    else if (mode == dev::mode_all_books)
+    {
+        hipLaunchKernelGGL((dev::wf_drain<dev::mode_all_books>), grid, dim3(threads), lds, stream, cam, sc, seed, work,
+                           pool, 0, sums, rays);
+    }
+    else
+    {
+        hipLaunchKernelGGL((dev::wf_drain<dev::mode_first_book>), grid, dim3(threads), lds, stream, cam, sc, seed,
+                           work, pool, 0, sums, rays);
     }
     return hipGetLastError();
 }

@@ -60,6 +60,11 @@ hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_
                             const dev::launch_work& work, const dev::wf_pool& pool, int parity, double* sums,
                             unsigned long long* rays, int extend_blocks, int shade_blocks, bool quality,
                             hipStream_t stream);
+// The drain: the `count` slots of the parity-0 active list finish their paths and their pixels' remaining passes, a
+// thread each (mode: the scene's kernel_mode)
+hipError_t wf_launch_drain(int mode, const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
+                           const dev::launch_work& work, const dev::wf_pool& pool, uint32_t count, double* sums,
+                           unsigned long long* rays, hipStream_t stream);
 // which: 0 extend, 1 shade, 2 the quality integrator's shade
 kernel_attributes wf_attributes(int depth, int which);
 }

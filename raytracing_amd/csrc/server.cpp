@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <thread>
+#include <vector>
 
 #include "device_scene.hpp"
 
@@ -16,10 +18,28 @@ std::size_t request_bytes(const rt_scene& s)
     {
         texels += static_cast<std::size_t>(s.images[i].width) * s.images[i].height;
     }
-    const std::size_t prims = static_cast<std::size_t>(s.sphere_count + s.quad_count);
+    std::size_t prims = static_cast<std::size_t>(s.sphere_count + s.quad_count);
+    for (int64_t i = 0; i < s.instance_count; ++i)
+    {
+        prims += static_cast<std::size_t>(s.meshes[s.instances[i].mesh].triangle_count);
+    }
+    if (s.instance_count == 0)
+    {
+        for (int64_t i = 0; i < s.mesh_count; ++i)
+        {
+            prims += static_cast<std::size_t>(s.meshes[i].triangle_count);
+        }
+    }
+    std::size_t env = 0;
+    if (s.environment)
+    {
+        env = static_cast<std::size_t>(s.environment->image.width) * s.environment->image.height;
+    }
     return 24 * pixels                                  // double sums
+           + 16 * 8 * pixels                            // the wavefront's slots
            + 16 * texels                                // float4 texels
            + prims * (64 + 48 + 16 + 4)                 // nodes, records, frames, materials ids
+           + env * (16 + 16)                            // map texels and alias bins (padded to a power of two)
            + static_cast<std::size_t>(s.texture_count) * 4200 + (1u << 20);
 }
 
@@ -109,15 +129,17 @@ void server::admit()
 int server::step()
 {
     admit();
-    int active = 0;
+    // every active request renders its step at once, each from a host thread of its own on its session's own
+    // stream, so their kernels overlap on the device; a request's step ends at its next power-of-two milestone
+    std::vector<request*> active;
     for (auto& [id, rp] : requests_)
     {
-        request& r = *rp;
-        if (r.st.state != 1)
+        if (rp->st.state == 1)
         {
-            continue;
+            active.push_back(rp.get());
         }
-        // up to the next power-of-two milestone, at most passes_per_step at a time
+    }
+    const auto run = [this](request& r) {
         const int64_t done = r.st.passes_done;
         int64_t next_milestone = 1;
         while (next_milestone <= done)
@@ -125,7 +147,16 @@ int server::step()
             next_milestone *= 2;
         }
         const int64_t target = std::min({r.passes, done + options_.passes_per_step, std::max(next_milestone, done + 1)});
-        r.work->render_to(target, 0);
+        try
+        {
+            r.work->render_to(target, 0);
+        }
+        catch (const std::exception&)
+        {
+            r.st.state = 3;
+            r.st.finish_time = now();
+            return;
+        }
         r.st.passes_done = target;
         if ((target & (target - 1)) == 0 || target == r.passes)
         {
@@ -137,12 +168,30 @@ int server::step()
             r.st.state = 2;
             r.st.finish_time = now();
         }
-        else
+    };
+    if (active.size() == 1)
+    {
+        run(*active[0]);
+    }
+    else
+    {
+        std::vector<std::thread> workers;
+        workers.reserve(active.size());
+        for (request* r : active)
         {
-            ++active;
+            workers.emplace_back([&run, r] { run(*r); });
+        }
+        for (std::thread& w : workers)
+        {
+            w.join();
         }
     }
-    return active;
+    int still = 0;
+    for (const request* r : active)
+    {
+        still += r->st.state == 1 ? 1 : 0;
+    }
+    return still;
 }
 
 rt_request_status server::status(int64_t id) const

@@ -121,6 +121,7 @@ public:
             }
             const double seconds = launch(options_.seed + static_cast<uint64_t>(round) * round_seed_step, work);
             report.launches += 1;
+            report.bounces += last_bounces_;
             report.gpu_seconds += seconds;
             report.longest_launch_seconds = std::max(report.longest_launch_seconds, seconds);
             if (work.first_row == 0 && work.row_count == height)
@@ -160,6 +161,7 @@ public:
             step = std::min(step, remaining);
             const rt_render_report r = render_to(state_.passes + step, 0);
             report.launches += r.launches;
+            report.bounces += r.bounces;
             report.gpu_seconds += r.gpu_seconds;
             report.longest_launch_seconds = std::max(report.longest_launch_seconds, r.longest_launch_seconds);
         }
@@ -367,6 +369,7 @@ private:
     accumulation state_{};
     double kernel_seconds_ = 0;
     double seconds_per_pass_ = 0;
+    int64_t last_bounces_ = 0; // the wavefront's bounce steps in the last launch
 
     std::size_t values() const
     {
@@ -394,8 +397,7 @@ private:
         check(hipMemsetAsync(rays_.data(), 0, rays_.bytes(), stream_.get()), "clearing the counters");
         if (use_wavefront_)
         {
-            const int slots = options_.wavefront_slots;
-            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, mode_, slots, stream_.get());
+            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, mode_, options_, stream_.get());
         }
         check(hipStreamSynchronize(stream_.get()), "starting the view");
         state_ = {};
@@ -408,6 +410,7 @@ private:
     double launch(uint64_t seed, const dev::launch_work& work)
     {
         start_.record(stream_.get());
+        last_bounces_ = 0;
         if (use_wavefront_)
         {
             // a pool smaller than the launch's rows takes them in bands: each band's pixels are its own
@@ -417,7 +420,7 @@ private:
                 dev::launch_work part = work;
                 part.first_row = row;
                 part.row_count = std::min(band, work.first_row + work.row_count - row);
-                wavefront_->render(seed, part, sums_.data(), rays_.data());
+                last_bounces_ += wavefront_->render(seed, part, sums_.data(), rays_.data());
             }
         }
         else

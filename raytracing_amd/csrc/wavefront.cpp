@@ -34,9 +34,9 @@ public:
 }
 
 template <typename Real>
-wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, int mode, int slots,
-                           hipStream_t stream)
-    : stream_(stream), quality_(mode == dev::mode_quality)
+wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, int mode,
+                           const rt_session_options& options, hipStream_t stream)
+    : stream_(stream), mode_(mode), quality_(mode == dev::mode_quality)
 {
     if constexpr (!wavefront_supported<Real>())
     {
@@ -46,6 +46,18 @@ wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_re
     {
         sc_ = sc;
         cam_ = cam;
+        const int slots = options.wavefront_slots;
+        const int graph_bounces = options.wavefront_graph;
+        if (slots < 0 || options.wavefront_drain < -1)
+        {
+            throw std::invalid_argument("the wavefront's slots must be 0 or more, its drain count -1 or more");
+        }
+        if (graph_bounces < -1 || graph_bounces > 64)
+        {
+            throw std::invalid_argument("the wavefront's bounces per graph must be -1 (none), 0 (default) or 1 to 64");
+        }
+        use_graph_ = graph_bounces != -1;
+        bounces_per_graph_ = graph_bounces > 0 ? 2 * ((graph_bounces + 1) / 2) : 8; // even: each replay ends on parity 0
         const uint64_t pixels = static_cast<uint64_t>(cam.image_width) * static_cast<uint64_t>(cam.image_height);
         const uint64_t want = slots > 0 ? static_cast<uint64_t>(slots) : pixels;
         if (want < static_cast<uint64_t>(cam.image_width))
@@ -58,6 +70,9 @@ wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_re
         }
         // whole rows: a launch's band of rows must fit the pool
         capacity_ = static_cast<uint32_t>(std::min(want, pixels) / cam.image_width * cam.image_width);
+        drain_below_ = options.wavefront_drain > 0    ? static_cast<uint32_t>(options.wavefront_drain)
+                       : options.wavefront_drain == 0 ? capacity_ / 4
+                                                      : 0;
         ray_o_ = device_array<float4>(capacity_);
         ray_d_ = device_array<float4>(capacity_);
         thr_ = device_array<float4>(capacity_);
@@ -95,38 +110,58 @@ wavefront<Real>::~wavefront()
 }
 
 template <typename Real>
-void wavefront<Real>::render(uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays)
+int64_t wavefront<Real>::render(uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays)
 {
-    if constexpr (wavefront_supported<Real>())
+    if constexpr (!wavefront_supported<Real>())
+    {
+        return 0;
+    }
+    else
     {
         if (static_cast<uint64_t>(work.row_count) * static_cast<uint64_t>(cam_.image_width) > capacity_)
         {
             throw std::invalid_argument("the launch's rows don't fit the wavefront pool");
         }
         check(wf_launch_init(cam_, sc_, seed, work, pool_, stream_), "launching the wavefront's first paths");
-        // the bounce loop: a graph of an even number of bounces (parity 0, 1, 0, 1...), replayed until a replay ends
-        // with no active slot (the count of the next parity-0 list)
+        // the bounce loop: an even number of bounces (parity 0, 1, 0, 1...) at a time, captured once in a graph (or
+        // launched one by one), repeated until a round ends with no active slot (the count of the next parity-0 list)
+        const auto bounces = [&] {
+            hipError_t launched = hipSuccess;
+            for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
+            {
+                launched = wf_launch_bounce(cam_, sc_, seed, work, pool_, b & 1, sums, rays, extend_blocks_,
+                                            shade_blocks_, quality_, stream_);
+            }
+            return launched;
+        };
         graph_owner g;
-        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capturing the bounce loop");
-        hipError_t launched = hipSuccess;
-        for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
+        if (use_graph_)
         {
-            launched = wf_launch_bounce(cam_, sc_, seed, work, pool_, b & 1, sums, rays, extend_blocks_,
-                                        shade_blocks_, quality_, stream_);
+            check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capturing the bounce loop");
+            const hipError_t launched = bounces();
+            const hipError_t ended = hipStreamEndCapture(stream_, &g.graph);
+            check(launched, "launching the bounce loop's kernels");
+            check(ended, "capturing the bounce loop");
+            check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "instantiating the bounce loop");
         }
-        const hipError_t ended = hipStreamEndCapture(stream_, &g.graph);
-        check(launched, "launching the bounce loop's kernels");
-        check(ended, "capturing the bounce loop");
-        check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "instantiating the bounce loop");
-        // a path is at most max_depth rays, and a slot traces pass_count paths: that bounds the replays
+        // a path is at most max_depth rays, and a slot traces pass_count paths: that bounds the rounds
         const long long max_bounces = static_cast<long long>(cam_.max_depth) * work.pass_count + bounces_per_graph_;
-        for (long long done = 0;; done += bounces_per_graph_)
+        int64_t done = 0;
+        while (true)
         {
             if (done > max_bounces)
             {
                 throw std::runtime_error("the wavefront loop didn't drain");
             }
-            check(hipGraphLaunch(g.exec, stream_), "launching the bounce loop");
+            if (use_graph_)
+            {
+                check(hipGraphLaunch(g.exec, stream_), "launching the bounce loop");
+            }
+            else
+            {
+                check(bounces(), "launching the bounce loop's kernels");
+            }
+            done += bounces_per_graph_;
             check(hipMemcpyAsync(host_count_, counters_.data(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
                   "reading the active count");
             check(hipStreamSynchronize(stream_), "rendering on the GPU");
@@ -134,7 +169,15 @@ void wavefront<Real>::render(uint64_t seed, const dev::launch_work& work, double
             {
                 break;
             }
+            if (*host_count_ < drain_below_)
+            {
+                check(wf_launch_drain(mode_, cam_, sc_, seed, work, pool_, *host_count_, sums, rays, stream_),
+                      "launching the wavefront's drain");
+                check(hipStreamSynchronize(stream_), "rendering on the GPU");
+                break;
+            }
         }
+        return done;
     }
 }
 

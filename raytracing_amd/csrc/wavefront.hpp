@@ -25,15 +25,17 @@ template <typename Real>
 class wavefront
 {
 public:
-    // slots: the pool's size in paths (0: one per pixel of the image); mode: the scene's kernel_mode
-    wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, int mode, int slots,
-              hipStream_t stream);
+    // mode: the scene's kernel_mode; options: the session's (wavefront_slots: the pool's size in paths, 0 for one per
+    // pixel; wavefront_graph: bounces per captured hipGraph, 0 for the default, -1 for none; wavefront_drain: the
+    // active count below which the pool drains, 0 for the default, -1 for never)
+    wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, int mode,
+              const rt_session_options& options, hipStream_t stream);
     ~wavefront();
     wavefront(const wavefront&) = delete;
     wavefront& operator=(const wavefront&) = delete;
 
-    // The launch's work, added to sums; waits until it is done
-    void render(uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays);
+    // The launch's work, added to sums; waits until it is done. Returns the bounce steps it launched.
+    int64_t render(uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays);
     kernel_attributes attributes() const;
     std::size_t bytes() const { return bytes_; }
     int bounces_per_graph() const { return bounces_per_graph_; }
@@ -44,7 +46,9 @@ private:
     dev::scene_view<float> sc_{};
     dev::camera_rec<float> cam_{};
     hipStream_t stream_ = nullptr;
+    int mode_ = dev::mode_all_books;
     bool quality_ = false;
+    uint32_t drain_below_ = 0;
     uint32_t capacity_ = 0;
     device_array<float4> ray_o_, ray_d_, thr_, qst_;
     device_array<uint4> rng_;
@@ -55,6 +59,7 @@ private:
     dev::wf_pool pool_{};
     int extend_blocks_ = 0, shade_blocks_ = 0;
     int bounces_per_graph_ = 8;
+    bool use_graph_ = true;
     std::size_t bytes_ = 0;
 };
 }

@@ -130,6 +130,28 @@ def test_triangles_small(rt):
     _agree_with_rerun(rt, "triangles", 96, 64, "nee", "mis")
 
 
+@pytest.mark.parametrize("name,sampling", [("cornell-ggx", "mis"), ("cornell-ggx", "nee"), ("sun-test", "mis"),
+                                           ("env-furnace", "mis"), ("veach", "mis"), ("glass", "mis"),
+                                           ("final", "mis"), ("triangles", "mis")])
+def test_wavefront_equals_megakernel_quality(rt, name, sampling):
+    """The quality integrator in both forms: the same sums bit for bit, the same ray and shadow-ray counts"""
+    scene = rt.build_scene(name, width=64, samples=16, sampling=sampling, earth_width=512)
+    with rt.Session(scene, seed=6, integrator="megakernel") as a:
+        assert a.kernel().name.decode().startswith("quality")
+        a.render_to(16)
+        sa = a.sums()
+        ca = a.counters()
+    with rt.Session(scene, seed=6, integrator="wavefront") as b:
+        assert b.kernel().name.decode().startswith("wavefront")
+        b.render_to(7)
+        b.render_to(16)
+        sb = b.sums()
+        cb = b.counters()
+    assert ca[0] == cb[0] and ca[1] == cb[1], f"rays {ca[:2]} vs {cb[:2]}"
+    mism = np.count_nonzero(sa.view(np.uint64) != sb.view(np.uint64))
+    assert mism == 0, f"{mism} of {sa.size} sums differ; max |diff| {np.abs(sa - sb).max()}"
+
+
 def test_quality_same_seed_same_bytes(rt):
     a = _render(rt, "cornell-ggx", 64, 16, "mis", seed=4).pixels
     b = _render(rt, "cornell-ggx", 64, 16, "mis", seed=4).pixels

@@ -177,6 +177,24 @@ def test_wavefront_equals_megakernel(rt, name):
     assert mism == 0, f"{mism} of {sa.size} sums differ; max |diff| {np.abs(sa - sb).max()}"
 
 
+@pytest.mark.parametrize("name", ["weekend", "final", "cornell"])
+@pytest.mark.parametrize("drain,graph", [(-1, 0), (1 << 30, 0), (0, -1), (5000, 2)])
+def test_wavefront_drain_and_graph_equal(rt, name, drain, graph):
+    """The drain (slots finishing their passes on their own), direct launches and short graphs: the same sums"""
+    scene = rt.build_scene(name, width=96, samples=16, earth_width=512)
+    with rt.Session(scene, seed=8, integrator="megakernel") as a:
+        a.render_to(16)
+        sa = a.sums()
+        ra = a.counters()[0]
+    with rt.Session(scene, seed=8, integrator="wavefront", wavefront_drain=drain, wavefront_graph=graph) as b:
+        r = b.render_to(16)
+        sb = b.sums()
+        rb = b.counters()[0]
+    assert r.bounces > 0
+    assert ra == rb
+    assert np.array_equal(sa.view(np.uint64), sb.view(np.uint64))
+
+
 def test_wavefront_small_pool(rt):
     """A pool smaller than the image: the launches go in bands of rows, and the image is the same"""
     scene = rt.build_scene("final", width=64, samples=16, earth_width=512)
