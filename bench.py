@@ -43,11 +43,11 @@ WORKLOADS = {
 }
 
 
-def measure(rt, scene, integrator, steps, warmup, seed=0):
+def measure(rt, scene, integrator, steps, warmup, seed=0, **tuning):
     """(Mpaths/s, ms per step, average launch seconds, launches per step, session) over `steps` timed renders"""
     passes = scene.passes
     paths_per_step = scene.width * scene.height * passes
-    s = rt.Session(scene, seed=seed, integrator=integrator, launch_seconds=0.1)
+    s = rt.Session(scene, seed=seed, integrator=integrator, launch_seconds=0.1, **tuning)
     for _ in range(warmup):
         s.set_camera(scene.desc.camera)
         s.render_to(passes)
@@ -89,6 +89,91 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
+SERVING_SCENES = ["weekend", "glass", "final", "triangles"]
+SERVING_RESOLUTIONS = [(256, 256), (512, 512), (1080, 1080), (1920, 1080)]
+
+
+def serving(rt, rates, per_rate, seed, tau):
+    """Config 4: a seeded Poisson stream of requests at each rate; each picks a scene of configs 0-3, a resolution and
+    a log-uniform spp in [16, 1024], and is rendered progressively (power-of-two milestones) by the server, with the
+    four scenes resident. Reports p50/p99 of the time from submission to the first milestone whose estimated relative
+    RMSE is below tau (the request's last milestone when its spp never gets there), p50/p99 time to completion, and
+    the aggregate Mpaths/s. The relative RMSE of m passes is sigma/sqrt(m), sigma per scene from a pilot: two
+    independent 16-pass renders at 128 pixels wide, mean((A - B)^2) / 2 = sigma^2 / 16 per value, over the mean^2."""
+    import numpy as np
+
+    views = {}
+    scenes = {}
+    for name in SERVING_SCENES:
+        for w, h in SERVING_RESOLUTIONS:
+            views[(name, w, h)] = rt.build_scene(name, width=w, height=h, samples=1).desc.camera
+    sigma = {}
+    for name in SERVING_SCENES:
+        pilot = rt.build_scene(name, width=128, samples=1)
+        imgs = []
+        for s in (101, 202):
+            with rt.Session(pilot, seed=s) as sess:
+                sess.render_to(16)
+                imgs.append(sess.pixels())
+        var = 16 * np.mean((imgs[0] - imgs[1]) ** 2) / 2
+        sigma[name] = float(np.sqrt(var) / max(np.mean(imgs[0] + imgs[1]) / 2, 1e-9))
+        print(f"serving pilot {name}: relative sigma per pass {sigma[name]:.3f}", file=sys.stderr, flush=True)
+    server = rt.Server(memory_budget_bytes=64 << 30, max_active=8, passes_per_step=8)
+    for name in SERVING_SCENES:
+        scenes[name] = server.add_scene(rt.build_scene(name, width=256, height=256, samples=1))
+    results = {}
+    rng = np.random.default_rng(seed)
+    for rate in rates:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, per_rate))
+        picks = [(SERVING_SCENES[rng.integers(4)], SERVING_RESOLUTIONS[rng.integers(4)],
+                  int(round(np.exp(rng.uniform(np.log(16), np.log(1024)))))) for _ in range(per_rate)]
+        pending = list(range(per_rate))
+        live = {}
+        paths = 0
+        t0 = time.perf_counter()
+        milestones = {}
+        while pending or live:
+            now = time.perf_counter() - t0
+            while pending and arrivals[pending[0]] <= now:
+                k = pending.pop(0)
+                name, (w, h), spp = picks[k]
+                rid = server.submit_view(scenes[name], views[(name, w, h)], 1000 + k, spp)
+                live[rid] = (k, now)
+                milestones[rid] = []
+                paths += w * h * spp
+            if not live:
+                time.sleep(max(0.0, arrivals[pending[0]] - (time.perf_counter() - t0)))
+                continue
+            server.step()
+            now = time.perf_counter() - t0
+            for rid in list(live):
+                st = server.status(rid)
+                if not milestones[rid] or milestones[rid][-1][0] != st.milestone:
+                    milestones[rid].append((st.milestone, now))
+                if st.state >= 2:
+                    live[rid] = live[rid] + (now, st.state)
+                    k = live[rid][0]
+                    name = picks[k][0]
+                    need = (sigma[name] / tau) ** 2
+                    hit = next((t for m, t in milestones[rid] if m >= need), milestones[rid][-1][1])
+                    results.setdefault(rate, []).append((hit - live[rid][1], now - live[rid][1], st.state))
+                    server.release(rid)
+                    del live[rid]
+        wall = time.perf_counter() - t0
+        r = results[rate]
+        target = np.array([x[0] for x in r])
+        done = np.array([x[1] for x in r])
+        results[rate] = {"rate_req_s": rate, "requests": per_rate, "failed": sum(1 for x in r if x[2] != 2),
+                         "p50_time_to_target_s": round(float(np.percentile(target, 50)), 3),
+                         "p99_time_to_target_s": round(float(np.percentile(target, 99)), 3),
+                         "p50_time_to_done_s": round(float(np.percentile(done, 50)), 3),
+                         "p99_time_to_done_s": round(float(np.percentile(done, 99)), 3),
+                         "aggregate_mpaths_s": round(paths / wall / 1e6, 1), "wall_s": round(wall, 2)}
+        print(json.dumps(results[rate]), file=sys.stderr, flush=True)
+    server.close()
+    return [results[r] for r in rates], sigma
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,7 +187,26 @@ def main():
     ap.add_argument("--no-megakernel", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel over this many bytes (the HBM counters' calibration)")
+    ap.add_argument("--wavefront-graph", type=int, default=0, help="bounces per hipGraph (-1: none, e.g. under a profiler)")
+    ap.add_argument("--wavefront-drain", type=int, default=0, help="active slots below which the pool drains")
+    ap.add_argument("--wavefront-slots", type=int, default=0, help="the wavefront pool's size")
+    ap.add_argument("--serving", action="store_true", help="config 4: the serving loop under Poisson request streams")
+    ap.add_argument("--rates", default="1,4,16,64")
+    ap.add_argument("--requests", type=int, default=16, help="requests per rate (serving)")
+    ap.add_argument("--tau", type=float, default=0.05, help="target relative RMSE (serving)")
     args = ap.parse_args()
+    if args.serving:
+        import raytracing_amd as rt
+
+        rows, sigma = serving(rt, [float(r) for r in args.rates.split(",")], args.requests, 4, args.tau)
+        print(json.dumps({"metric": "serving p99 time to target noise (config 4)", "value": rows[-1]["p99_time_to_target_s"],
+                          "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+                          "config": {"workload": f"Poisson streams, {args.requests} requests per rate, scenes "
+                                                 f"{SERVING_SCENES}, resolutions {SERVING_RESOLUTIONS}, spp "
+                                                 f"log-uniform [16, 1024], target relative RMSE {args.tau}",
+                                     "sigma_per_pass": sigma},
+                          "rates": rows}), flush=True)
+        return
     if args.copy_probe:
         import raytracing_amd as rt
 
@@ -124,7 +228,10 @@ def main():
     device = rt.query_device()
     print(f"{args.scene}: {scene.width}x{scene.height}, {scene.passes} spp; device {device.name.decode()} "
           f"({device.compute_units} CUs)", file=sys.stderr, flush=True)
-    value, ms, avg_launch, launches_per_step, session = measure(rt, scene, args.integrator, args.steps, args.warmup)
+    tuning = {k: v for k, v in (("wavefront_graph", args.wavefront_graph), ("wavefront_drain", args.wavefront_drain),
+                                ("wavefront_slots", args.wavefront_slots)) if v}
+    value, ms, avg_launch, launches_per_step, session = measure(rt, scene, args.integrator, args.steps, args.warmup,
+                                                                **tuning)
     kernel = session.kernel().name.decode()
     print(f"{args.integrator}: {value:.1f} Mpaths/s, {ms:.1f} ms/step, {launches_per_step:.1f} launches/step, "
           f"kernel {kernel}", file=sys.stderr, flush=True)

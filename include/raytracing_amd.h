@@ -469,7 +469,15 @@ typedef struct rt_request_status
 
 int rt_server_create(const rt_server_options* options, rt_server** out);
 void rt_server_destroy(rt_server* server);
-/* Admits a request: a scene (copied), its seed and its passes. Returns its id in *id. */
+/* Makes a scene resident: copied, its tree built and uploaded once, shared by every request that views it. Its
+ * device memory counts against the budget until it is removed and its last request released. */
+int rt_server_add_scene(rt_server* server, const rt_scene* scene, int64_t* scene_id);
+int rt_server_remove_scene(rt_server* server, int64_t scene_id);
+/* Queues a request: a view of a resident scene (a camera of any resolution, with the scene's sampling strategy), its
+ * seed and its passes. Returns its id in *id. */
+int rt_server_submit_view(rt_server* server, int64_t scene_id, const rt_camera_desc* camera, uint64_t seed,
+                          int64_t passes, int64_t* id);
+/* Queues a request with a scene of its own (copied and uploaded for it, freed when it is released) */
 int rt_server_submit(rt_server* server, const rt_scene* scene, uint64_t seed, int64_t passes, int64_t* id);
 /* One scheduling step: admits queued requests as memory and slots free up, and renders a step of every active one */
 int rt_server_step(rt_server* server, int32_t* active);

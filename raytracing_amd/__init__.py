@@ -207,6 +207,10 @@ _SIGNATURES = {
     "rt_server_create": (C.c_int, [_P(ServerOptions), _P(C.c_void_p)]),
     "rt_server_destroy": (None, [C.c_void_p]),
     "rt_server_submit": (C.c_int, [C.c_void_p, _P(Scene), C.c_uint64, C.c_int64, _P(C.c_int64)]),
+    "rt_server_add_scene": (C.c_int, [C.c_void_p, _P(Scene), _P(C.c_int64)]),
+    "rt_server_remove_scene": (C.c_int, [C.c_void_p, C.c_int64]),
+    "rt_server_submit_view": (C.c_int, [C.c_void_p, C.c_int64, _P(CameraDesc), C.c_uint64, C.c_int64,
+                                        _P(C.c_int64)]),
     "rt_server_step": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
     "rt_server_status": (C.c_int, [C.c_void_p, C.c_int64, _P(RequestStatus)]),
     "rt_server_pixels": (C.c_int, [C.c_void_p, C.c_int64, _P(C.c_double), C.c_size_t]),
@@ -491,6 +495,21 @@ class Server:
         rid = C.c_int64()
         _check(_lib.rt_server_submit(self._h, C.byref(scene.desc), seed, passes, C.byref(rid)))
         self._scenes[rid.value] = (scene.width, scene.height)
+        return rid.value
+
+    def add_scene(self, scene: BuiltScene) -> int:
+        """Makes a scene resident (uploaded once, shared by its views)"""
+        sid = C.c_int64()
+        _check(_lib.rt_server_add_scene(self._h, C.byref(scene.desc), C.byref(sid)))
+        return sid.value
+
+    def remove_scene(self, scene_id: int) -> None:
+        _check(_lib.rt_server_remove_scene(self._h, scene_id))
+
+    def submit_view(self, scene_id: int, camera: CameraDesc, seed: int, passes: int) -> int:
+        rid = C.c_int64()
+        _check(_lib.rt_server_submit_view(self._h, scene_id, C.byref(camera), seed, passes, C.byref(rid)))
+        self._scenes[rid.value] = (camera.image_width, camera.image_height)
         return rid.value
 
     def step(self) -> int:

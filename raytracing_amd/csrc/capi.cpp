@@ -388,6 +388,35 @@ int rt_server_create(const rt_server_options* options, rt_server** out)
 
 void rt_server_destroy(rt_server* server) { delete server; }
 
+int rt_server_add_scene(rt_server* server, const rt_scene* scene, int64_t* scene_id)
+{
+    return guarded([&] {
+        need(server, "server");
+        need(scene, "scene");
+        need(scene_id, "scene_id");
+        *scene_id = server->impl->add_scene(*scene);
+    });
+}
+
+int rt_server_remove_scene(rt_server* server, int64_t scene_id)
+{
+    return guarded([&] {
+        need(server, "server");
+        server->impl->remove_scene(scene_id);
+    });
+}
+
+int rt_server_submit_view(rt_server* server, int64_t scene_id, const rt_camera_desc* camera, uint64_t seed,
+                          int64_t passes, int64_t* id)
+{
+    return guarded([&] {
+        need(server, "server");
+        need(camera, "camera");
+        need(id, "id");
+        *id = server->impl->submit_view(scene_id, *camera, seed, passes);
+    });
+}
+
 int rt_server_submit(rt_server* server, const rt_scene* scene, uint64_t seed, int64_t passes, int64_t* id)
 {
     return guarded([&] {

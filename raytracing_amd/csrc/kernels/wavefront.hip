@@ -4,16 +4,18 @@
 // next pass in place. So every pixel has one path in flight and adds its samples in pass order, as the megakernel does,
 // and every path draws the same random numbers in the same order: the two integrators' images are the same bytes.
 //
-// One bounce of every active slot is three kernels:
-//   reset:  clears the counters the bounce fills;
+// One bounce of every active slot is four kernels:
+//   begin:  the active list's prefix sums over its stripes, and the counters the bounce fills cleared (one block);
 //   extend: the closest event of each active slot's ray (BVH walk with an LDS stack, then the media), written to the
-//           slot, and the slot appended to the queue of its hit's material class (wave-aggregated atomics: one atomic
-//           per class per wave);
-//   shade:  the queues in class order (miss, lamp, matte, textured, marble, smoke, metal, glass), so that a wave
-//           shades one material at a time: the bounce (or the end of the path, its sample, and the pixel's next pass),
-//           and the slot appended to the next bounce's active list.
-// extend and shade are persistent: a fixed grid whose waves fetch 64 items at a time from a counter, until the count
-// (read from device memory, so the loop needs no host synchronisation and can be captured in a hipGraph) runs out.
+//           slot, and the slot appended to the queue of its hit's material class;
+//   sort:   the queues' prefix sums, class by class (one block);
+//   shade:  the queues in class order (miss, lamp, matte, textured, marble, smoke, metal, glass, GGX, gloss), so that
+//           a wave shades one material at a time: the bounce (or the end of the path, its sample, and the pixel's next
+//           pass), and the slot appended to the next bounce's active list.
+// Every thread of extend and shade takes one item by its index (no work counter); appends are counted per wave with
+// ballots, per block in LDS, then one device-scope atomic per block and list on a stripe of the list (wf_stripes).
+// The lengths live in device memory, so a bounce needs no host synchronisation and a run of bounces is captured in a
+// hipGraph. When few slots are left, the drain kernel finishes them a thread each.
 #include "kernel_api.hpp"
 #include "path_tracer.hpp"
 #include "quality.hpp"
@@ -33,6 +35,7 @@ __device__ inline unsigned long long wsum(unsigned long long v)
 }
 
 __device__ inline int lane_id() { return static_cast<int>(threadIdx.x & 63); }
+__device__ inline int stripe_of_block() { return static_cast<int>(blockIdx.x % wf_stripes); }
 
 // A slot's path in the SoA pool
 struct slot_path
@@ -79,35 +82,102 @@ __device__ inline void store_path(const wf_pool& pool, uint32_t s, const slot_pa
     store_rng(pool, s, sp.g);
 }
 
-// Appends `want` (this lane's vote) to a list with one atomic per wave; returns this lane's position
-__device__ inline uint32_t wave_append(uint32_t* counter, bool want)
+// The segment of item g in a list of `segments` whose prefix sums (segments + 1 of them, the last the length) are at
+// `prefix`: the largest k with prefix[k] <= g (empty segments are skipped). Requires g < prefix[segments].
+__device__ inline int find_segment(const uint32_t* prefix, int segments, uint32_t g)
 {
-    const uint64_t mask = __ballot(want);
-    if (mask == 0)
+    int lo = 0, hi = segments - 1;
+    while (lo < hi)
     {
-        return 0;
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= g)
+        {
+            lo = mid;
+        }
+        else
+        {
+            hi = mid - 1;
+        }
     }
-    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-    uint32_t base = 0;
-    if (lane_id() == leader)
-    {
-        base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
-    }
-    base = __shfl(base, leader, 64);
-    const uint64_t below = mask & ((uint64_t{1} << lane_id()) - 1);
-    return base + static_cast<uint32_t>(__popcll(below));
+    return lo;
 }
 
-// The next 64 items for this wave, or false when the count is exhausted
-__device__ inline bool wave_fetch(uint32_t* work, uint32_t count, uint32_t& base)
+// Appends each thread's item to list `list` (none for list < 0) of `Lists` striped lists: counted per wave with a
+// ballot, per block in LDS, then one device-scope atomic per list present in the block, on this block's stripe. Every
+// thread of the block must call it (it has barriers).
+template <int Lists>
+__device__ inline void block_append(int list, uint32_t item, uint32_t* counters, uint32_t* storage, uint32_t segment)
 {
-    uint32_t b = 0;
-    if (lane_id() == 0)
+    __shared__ uint32_t count[Lists];
+    __shared__ uint32_t base[Lists];
+    const int t = static_cast<int>(threadIdx.x);
+    if (t < Lists)
     {
-        b = atomicAdd(work, 64u);
+        count[t] = 0;
     }
-    base = __shfl(b, 0, 64);
-    return base < count;
+    __syncthreads();
+    uint32_t at = 0;
+    for (int c = 0; c < Lists; ++c)
+    {
+        const uint64_t mask = __ballot(list == c);
+        if (mask == 0)
+        {
+            continue;
+        }
+        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        uint32_t wave_base = 0;
+        if (lane_id() == leader)
+        {
+            wave_base = atomicAdd(&count[c], static_cast<uint32_t>(__popcll(mask)));
+        }
+        wave_base = __shfl(wave_base, leader, 64);
+        if (list == c)
+        {
+            at = wave_base + static_cast<uint32_t>(__popcll(mask & ((uint64_t{1} << lane_id()) - 1)));
+        }
+    }
+    __syncthreads();
+    const int stripe = stripe_of_block();
+    if (t < Lists && count[t] > 0)
+    {
+        base[t] = atomicAdd(&counters[t * wf_stripes + stripe], count[t]);
+    }
+    __syncthreads();
+    if (list >= 0)
+    {
+        storage[static_cast<size_t>(list * wf_stripes + stripe) * segment + base[list] + at] = item;
+    }
+}
+
+// Adds a count of every thread of the block to one of the launch's stripe tallies: one atomic per block
+__device__ inline void block_tally(unsigned long long* tallies, int which, unsigned long long v)
+{
+    __shared__ unsigned long long total;
+    if (threadIdx.x == 0)
+    {
+        total = 0;
+    }
+    __syncthreads();
+    v = wsum(v);
+    if (lane_id() == 0 && v)
+    {
+        atomicAdd(&total, v);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && total)
+    {
+        atomicAdd(&tallies[2 * stripe_of_block() + which], total);
+    }
+}
+
+// Loads `n` prefix sums into LDS (every thread of the block calls it)
+__device__ inline void load_prefix(uint32_t* lds, const uint32_t* prefix, int n)
+{
+    for (int k = static_cast<int>(threadIdx.x); k < n; k += static_cast<int>(blockDim.x))
+    {
+        lds[k] = prefix[k];
+    }
+    __syncthreads();
 }
 
 __device__ inline lds_stack<float> thread_stack(int depth)
@@ -173,17 +243,62 @@ __device__ inline slot_path start_slot(const camera_rec<float>& cam, const scene
     }
     return sp;
 }
+
+// The slot behind item g of the active list of parity p (its stripes' prefix sums in LDS)
+__device__ inline uint32_t active_slot(const wf_pool& pool, int p, const uint32_t* prefix, uint32_t g)
+{
+    const int stripe = find_segment(prefix, wf_stripes, g);
+    return pool.active[p][static_cast<size_t>(stripe) * pool.segment + (g - prefix[stripe])];
 }
 
+// A finished path's sample into its pixel's sums, and the pixel's next pass started in the slot; false when the
+// pixel has no pass left in the launch
+__device__ inline bool finish_sample(const camera_rec<float>& cam, const scene_view<float>& sc, uint64_t seed,
+                                     const launch_work& work, const wf_pool& pool, uint32_t s, vec3<float> sample,
+                                     double* __restrict__ sums)
+{
+    const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
+    const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
+    const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
+    double* const sum = sums + 3 * pixel;
+    sum[0] += without_nan(sample.x);
+    sum[1] += without_nan(sample.y);
+    sum[2] += without_nan(sample.z);
+    const int next = pool.pass[s] + 1;
+    if (next >= work.first_pass + work.pass_count)
+    {
+        return false;
+    }
+    pool.pass[s] = next;
+    store_path(pool, s, start_slot(cam, sc, seed, pixel, i, j, next, pool, s));
+    return true;
+}
+}
+
+// Every slot's first path, and the parity-0 active list holding every slot: block b fills its stripe's segment at
+// (b / wf_stripes) * wf_block, so the list is written without atomics and its stripes' lengths have a closed form
 __global__ void __launch_bounds__(256) wf_init(const camera_rec<float> cam, const scene_view<float> sc,
                                                const uint64_t seed, const launch_work work, wf_pool pool)
 {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t slots = static_cast<uint32_t>(cam.image_width) * static_cast<uint32_t>(work.row_count);
-    if (s == 0)
+    const uint32_t blocks = (slots + wf_block - 1) / wf_block;
+    if (blockIdx.x == 0 && threadIdx.x < wf_stripes)
     {
-        pool.counters[0] = slots; // the active list of parity 0 holds every slot
+        const uint32_t stripe = threadIdx.x;
+        uint32_t length = 0;
+        if (stripe < blocks)
+        {
+            const uint32_t last = blocks - 1;
+            const uint32_t mine = (last - stripe) / wf_stripes + 1; // blocks stripe, stripe + S, ... <= last
+            length = mine * wf_block;
+            if (last % wf_stripes == stripe)
+            {
+                length -= blocks * wf_block - slots; // the last block is short
+            }
+        }
+        pool.counters[wf_counter_active + stripe] = length;
     }
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= slots)
     {
         return;
@@ -194,149 +309,180 @@ __global__ void __launch_bounds__(256) wf_init(const camera_rec<float> cam, cons
     const slot_path sp = start_slot(cam, sc, seed, pixel, i, j, work.first_pass, pool, s);
     store_path(pool, s, sp);
     pool.pass[s] = work.first_pass;
-    pool.active[0][s] = s;
+    const uint32_t stripe = blockIdx.x % wf_stripes;
+    pool.active[0][static_cast<size_t>(stripe) * pool.segment + (blockIdx.x / wf_stripes) * wf_block + threadIdx.x] = s;
 }
 
-// Clears what one bounce of parity p fills: the next active list's count, the class counts and the work counters
-__global__ void wf_reset(wf_pool pool, int p)
+// The start of a bounce of parity p: the active list's prefix sums; the class queues of parity p and the next active
+// list cleared (one block)
+__global__ void __launch_bounds__(256) wf_begin(wf_pool pool, int p)
 {
-    const int k = threadIdx.x;
     uint32_t* in = pool.counters + wf_counter_stride * p;
     uint32_t* out = pool.counters + wf_counter_stride * (1 - p);
-    if (k == 0)
+    const int t = static_cast<int>(threadIdx.x);
+    if (t == 0)
     {
-        out[0] = 0;
+        uint32_t sum = 0;
+        for (int k = 0; k < wf_stripes; ++k)
+        {
+            pool.prefix[wf_prefix_active + k] = sum;
+            sum += in[wf_counter_active + k];
+        }
+        pool.prefix[wf_prefix_active + wf_stripes] = sum;
     }
-    if (k >= 1 && k < wf_counter_stride)
+    for (int k = t; k < wf_classes * wf_stripes; k += static_cast<int>(blockDim.x))
     {
-        in[k] = 0;
+        in[wf_counter_class + k] = 0;
+    }
+    if (t < wf_stripes)
+    {
+        out[wf_counter_active + t] = 0;
     }
 }
 
-__global__ void __launch_bounds__(256) wf_extend(const scene_view<float> sc, wf_pool pool, int p,
-                                                 unsigned long long* ray_count)
+// The queues' prefix sums, class-major (one block)
+__global__ void __launch_bounds__(64) wf_sort(wf_pool pool, int p)
 {
+    const uint32_t* in = pool.counters + wf_counter_stride * p + wf_counter_class;
+    if (threadIdx.x == 0)
+    {
+        uint32_t sum = 0;
+        for (int k = 0; k < wf_classes * wf_stripes; ++k)
+        {
+            pool.prefix[wf_prefix_class + k] = sum;
+            sum += in[k];
+        }
+        pool.prefix[wf_prefix_class + wf_classes * wf_stripes] = sum;
+    }
+}
+
+__global__ void __launch_bounds__(256) wf_extend(const scene_view<float> sc, wf_pool pool, int p)
+{
+    __shared__ uint32_t prefix[wf_stripes + 1];
+    load_prefix(prefix, pool.prefix + wf_prefix_active, wf_stripes + 1);
     const lds_stack<float> stack = thread_stack(sc.stack_depth);
-    uint32_t* in = pool.counters + wf_counter_stride * p;
-    const uint32_t n = in[0];
-    const uint32_t* active = pool.active[p];
-    unsigned long long rays = 0;
-    tally<false> tl;
-    uint32_t base;
-    while (wave_fetch(in + wf_counter_extend_work, n, base))
+    const uint32_t n = prefix[wf_stripes];
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    int cls = -1;
+    uint32_t s = 0;
+    if (g < n)
     {
-        const uint32_t k = base + static_cast<uint32_t>(lane_id());
-        const bool valid = k < n;
-        int cls = -1;
-        uint32_t s = 0;
-        if (valid)
+        s = active_slot(pool, p, prefix, g);
+        slot_path sp = load_path(pool, s);
+        tally<false> tl;
+        const hit<float> h = closest_event(sc, sp.p.r, sp.p.from, sp.p.inward, sp.g, stack, tl);
+        pool.hit[s] = make_int4(h.index, __float_as_int(h.t), h.entering ? 1 : 0, 0);
+        if (sc.medium_count > 0)
         {
-            s = active[k];
-            slot_path sp = load_path(pool, s);
-            const hit<float> h = closest_event(sc, sp.p.r, sp.p.from, sp.p.inward, sp.g, stack, tl);
-            pool.hit[s] = make_int4(h.index, __float_as_int(h.t), h.entering ? 1 : 0, 0);
-            if (sc.medium_count > 0)
-            {
-                store_rng(pool, s, sp.g); // the media drew their free flights from the path's generator
-            }
-            cls = hit_class(sc, h.index);
-            ++rays;
+            store_rng(pool, s, sp.g); // the media drew their free flights from the path's generator
         }
-        // the slot waits in its class's queue: one atomic per class present in the wave
-        for (int c = 0; c < wf_classes; ++c)
-        {
-            const bool mine = cls == c;
-            if (__ballot(mine) == 0)
-            {
-                continue;
-            }
-            const uint32_t at = wave_append(in + wf_counter_class + c, mine);
-            if (mine)
-            {
-                pool.queue[static_cast<size_t>(c) * pool.capacity + at] = s;
-            }
-        }
+        cls = hit_class(sc, h.index);
     }
-    rays = wsum(rays);
-    if (lane_id() == 0 && rays)
+    if (blockIdx.x * blockDim.x >= n)
     {
-        atomicAdd(ray_count, rays);
+        return; // the whole block is past the list (uniform across the block: no barrier below is skipped by some)
     }
+    block_tally(pool.tallies, 0, cls >= 0 ? 1 : 0);
+    block_append<wf_classes>(cls, s, pool.counters + wf_counter_stride * p + wf_counter_class, pool.queue,
+                             pool.segment);
 }
 
 __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, const scene_view<float> sc,
                                                 const uint64_t seed, const launch_work work, wf_pool pool, int p,
                                                 double* __restrict__ sums)
 {
-    uint32_t* in = pool.counters + wf_counter_stride * p;
-    uint32_t* out = pool.counters + wf_counter_stride * (1 - p);
-    uint32_t offsets[wf_classes + 1];
-    offsets[0] = 0;
-    for (int c = 0; c < wf_classes; ++c)
+    __shared__ uint32_t prefix[wf_classes * wf_stripes + 1];
+    load_prefix(prefix, pool.prefix + wf_prefix_class, wf_classes * wf_stripes + 1);
+    const uint32_t total = prefix[wf_classes * wf_stripes];
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= total)
     {
-        offsets[c + 1] = offsets[c] + in[wf_counter_class + c];
+        return;
     }
-    const uint32_t total = offsets[wf_classes];
-    const int end_pass = work.first_pass + work.pass_count;
-    tally<false> tl;
-    uint32_t base;
-    while (wave_fetch(in + wf_counter_shade_work, total, base))
+    bool keep = false;
+    uint32_t s = 0;
+    if (g < total)
     {
-        const uint32_t g = base + static_cast<uint32_t>(lane_id());
-        bool keep = false;
-        uint32_t s = 0;
-        if (g < total)
+        const int list = find_segment(prefix, wf_classes * wf_stripes, g);
+        s = pool.queue[static_cast<size_t>(list) * pool.segment + (g - prefix[list])];
+        slot_path sp = load_path(pool, s);
+        const int4 hv = pool.hit[s];
+        const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
+        tally<false> tl;
+        vec3<float> sample;
+        bool done;
+        if (h.index < 0)
         {
-            int c = 0;
-            while (g >= offsets[c + 1])
-            {
-                ++c;
-            }
-            s = pool.queue[static_cast<size_t>(c) * pool.capacity + (g - offsets[c])];
-            slot_path sp = load_path(pool, s);
-            const int4 hv = pool.hit[s];
-            const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
-            vec3<float> sample;
-            bool done;
-            if (h.index < 0)
-            {
-                sample = sp.p.throughput * sky(cam, sp.p.r.direction);
-                done = true;
-            }
-            else
-            {
-                done = shade(sp.p, sc, h, sp.g, sample, tl);
-            }
-            if (!done)
-            {
-                store_path(pool, s, sp);
-                keep = true;
-            }
-            else
-            {
-                const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
-                const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
-                const uint64_t pixel =
-                    static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
-                double* const sum = sums + 3 * pixel;
-                sum[0] += without_nan(sample.x);
-                sum[1] += without_nan(sample.y);
-                sum[2] += without_nan(sample.z);
-                const int next = pool.pass[s] + 1;
-                if (next < end_pass)
-                {
-                    pool.pass[s] = next;
-                    store_path(pool, s, start_slot(cam, sc, seed, pixel, i, j, next, pool, s));
-                    keep = true;
-                }
-            }
+            sample = sp.p.throughput * sky(cam, sp.p.r.direction);
+            done = true;
         }
-        const uint32_t at = wave_append(out, keep);
-        if (keep)
+        else
         {
-            pool.active[1 - p][at] = s;
+            done = shade(sp.p, sc, h, sp.g, sample, tl);
+        }
+        if (!done)
+        {
+            store_path(pool, s, sp);
+            keep = true;
+        }
+        else
+        {
+            keep = finish_sample(cam, sc, seed, work, pool, s, sample, sums);
         }
     }
+    block_append<1>(keep ? 0 : -1, s, pool.counters + wf_counter_stride * (1 - p) + wf_counter_active,
+                    pool.active[1 - p], pool.segment);
+}
+
+// The quality integrator's shade: the same queues in class order, each vertex's light sample and its shadow ray
+// traced here (an any-hit walk with this thread's LDS stack), in the megakernel's order of random numbers
+__global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> cam, const scene_view<float> sc,
+                                                        const uint64_t seed, const launch_work work, wf_pool pool,
+                                                        int p, double* __restrict__ sums)
+{
+    __shared__ uint32_t prefix[wf_classes * wf_stripes + 1];
+    load_prefix(prefix, pool.prefix + wf_prefix_class, wf_classes * wf_stripes + 1);
+    const lds_stack<float> stack = thread_stack(sc.stack_depth);
+    const uint32_t total = prefix[wf_classes * wf_stripes];
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= total)
+    {
+        return;
+    }
+    bool keep = false;
+    uint32_t s = 0;
+    uint32_t shadow_rays = 0;
+    if (g < total)
+    {
+        const int list = find_segment(prefix, wf_classes * wf_stripes, g);
+        s = pool.queue[static_cast<size_t>(list) * pool.segment + (g - prefix[list])];
+        slot_path sp = load_path(pool, s);
+        const float4 st = pool.qst[s];
+        quality_path q;
+        q.base = sp.p;
+        q.radiance = {st.x, st.y, st.z};
+        q.bsdf_pdf = st.w;
+        q.specular = sp.specular;
+        const int4 hv = pool.hit[s];
+        const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
+        tally<false> tl;
+        vec3<float> sample;
+        if (!shade_quality(q, sc, cam, h, sp.g, sample, stack, tl, shadow_rays))
+        {
+            sp.p = q.base;
+            sp.specular = q.specular;
+            pool.qst[s] = make_float4(q.radiance.x, q.radiance.y, q.radiance.z, q.bsdf_pdf);
+            store_path(pool, s, sp);
+            keep = true;
+        }
+        else
+        {
+            keep = finish_sample(cam, sc, seed, work, pool, s, sample, sums);
+        }
+    }
+    block_tally(pool.tallies, 1, shadow_rays);
+    block_append<1>(keep ? 0 : -1, s, pool.counters + wf_counter_stride * (1 - p) + wf_counter_active,
+                    pool.active[1 - p], pool.segment);
 }
 
 // The drain: when few slots are left active, each of them (a thread each) traces its current path and its pixel's
@@ -344,17 +490,19 @@ __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, con
 // empty pool would cost a launch each for little work. The sums stay in pass order, so the image is the same bytes.
 template <int Mode>
 __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, const scene_view<float> sc,
-                                                const uint64_t seed, const launch_work work, wf_pool pool, int p,
-                                                double* __restrict__ sums, unsigned long long* __restrict__ ray_count)
+                                                const uint64_t seed, const launch_work work, wf_pool pool,
+                                                double* __restrict__ sums)
 {
+    __shared__ uint32_t prefix[wf_stripes + 1];
+    load_prefix(prefix, pool.prefix + wf_prefix_active, wf_stripes + 1);
     const lds_stack<float> stack = thread_stack(sc.stack_depth);
-    const uint32_t n = pool.counters[wf_counter_stride * p];
+    const uint32_t n = prefix[wf_stripes];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long rays = 0;
     uint32_t shadow_rays = 0;
     if (k < n)
     {
-        const uint32_t s = pool.active[p][k];
+        const uint32_t s = active_slot(pool, 0, prefix, k);
         slot_path sp = load_path(pool, s);
         const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
         const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
@@ -415,102 +563,28 @@ __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, con
         sum[1] = green;
         sum[2] = blue;
     }
-    rays = wsum(rays);
-    const unsigned long long shadows = wsum(shadow_rays);
-    if (lane_id() == 0)
-    {
-        if (rays)
-        {
-            atomicAdd(ray_count, rays);
-        }
-        if (shadows)
-        {
-            atomicAdd(ray_count + 1, shadows);
-        }
-    }
+    block_tally(pool.tallies, 0, rays);
+    block_tally(pool.tallies, 1, shadow_rays);
 }
 
-// The quality integrator's shade: the same queues in class order, each vertex's light sample and its shadow ray
-// traced here (an any-hit walk with this thread's LDS stack), in the megakernel's order of random numbers
-__global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> cam, const scene_view<float> sc,
-                                                        const uint64_t seed, const launch_work work, wf_pool pool,
-                                                        int p, double* __restrict__ sums,
-                                                        unsigned long long* __restrict__ ray_count)
+// The launch's ray counts into the session's counters, and the stripe tallies cleared (one block)
+__global__ void __launch_bounds__(64) wf_finish(wf_pool pool, unsigned long long* rays)
 {
-    const lds_stack<float> stack = thread_stack(sc.stack_depth);
-    uint32_t* in = pool.counters + wf_counter_stride * p;
-    uint32_t* out = pool.counters + wf_counter_stride * (1 - p);
-    uint32_t offsets[wf_classes + 1];
-    offsets[0] = 0;
-    for (int c = 0; c < wf_classes; ++c)
+    const int t = static_cast<int>(threadIdx.x);
+    unsigned long long mine[2] = {0, 0};
+    if (t < wf_stripes)
     {
-        offsets[c + 1] = offsets[c] + in[wf_counter_class + c];
+        mine[0] = pool.tallies[2 * t];
+        mine[1] = pool.tallies[2 * t + 1];
+        pool.tallies[2 * t] = 0;
+        pool.tallies[2 * t + 1] = 0;
     }
-    const uint32_t total = offsets[wf_classes];
-    const int end_pass = work.first_pass + work.pass_count;
-    tally<false> tl;
-    uint32_t shadow_rays = 0;
-    uint32_t base;
-    while (wave_fetch(in + wf_counter_shade_work, total, base))
+    const unsigned long long r = wsum(mine[0]);
+    const unsigned long long sh = wsum(mine[1]);
+    if (t == 0)
     {
-        const uint32_t g = base + static_cast<uint32_t>(lane_id());
-        bool keep = false;
-        uint32_t s = 0;
-        if (g < total)
-        {
-            int c = 0;
-            while (g >= offsets[c + 1])
-            {
-                ++c;
-            }
-            s = pool.queue[static_cast<size_t>(c) * pool.capacity + (g - offsets[c])];
-            slot_path sp = load_path(pool, s);
-            const float4 st = pool.qst[s];
-            quality_path q;
-            q.base = sp.p;
-            q.radiance = {st.x, st.y, st.z};
-            q.bsdf_pdf = st.w;
-            q.specular = sp.specular;
-            const int4 hv = pool.hit[s];
-            const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
-            vec3<float> sample;
-            if (!shade_quality(q, sc, cam, h, sp.g, sample, stack, tl, shadow_rays))
-            {
-                sp.p = q.base;
-                sp.specular = q.specular;
-                pool.qst[s] = make_float4(q.radiance.x, q.radiance.y, q.radiance.z, q.bsdf_pdf);
-                store_path(pool, s, sp);
-                keep = true;
-            }
-            else
-            {
-                const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
-                const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
-                const uint64_t pixel =
-                    static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
-                double* const sum = sums + 3 * pixel;
-                sum[0] += without_nan(sample.x);
-                sum[1] += without_nan(sample.y);
-                sum[2] += without_nan(sample.z);
-                const int next = pool.pass[s] + 1;
-                if (next < end_pass)
-                {
-                    pool.pass[s] = next;
-                    store_path(pool, s, start_slot(cam, sc, seed, pixel, i, j, next, pool, s));
-                    keep = true;
-                }
-            }
-        }
-        const uint32_t at = wave_append(out, keep);
-        if (keep)
-        {
-            pool.active[1 - p][at] = s;
-        }
-    }
-    const unsigned long long shadows = wsum(shadow_rays);
-    if (lane_id() == 0 && shadows)
-    {
-        atomicAdd(ray_count + 1, shadows);
+        rays[0] += r;
+        rays[1] += sh;
     }
 }
 }
@@ -521,53 +595,61 @@ hipError_t wf_launch_init(const dev::camera_rec<float>& cam, const dev::scene_vi
                           const dev::launch_work& work, const dev::wf_pool& pool, hipStream_t stream)
 {
     const uint32_t slots = static_cast<uint32_t>(cam.image_width) * static_cast<uint32_t>(work.row_count);
-    hipLaunchKernelGGL(dev::wf_init, dim3((slots + 255) / 256), dim3(256), 0, stream, cam, sc, seed, work, pool);
+    hipLaunchKernelGGL(dev::wf_init, dim3(wf_blocks(slots)), dim3(dev::wf_block), 0, stream, cam, sc, seed, work, pool);
     return hipGetLastError();
 }
 
 hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
                             const dev::launch_work& work, const dev::wf_pool& pool, int parity, double* sums,
-                            unsigned long long* rays, int extend_blocks, int shade_blocks, bool quality,
-                            hipStream_t stream)
+                            bool quality, hipStream_t stream)
 {
-    hipLaunchKernelGGL(dev::wf_reset, dim3(1), dim3(64), 0, stream, pool, parity);
-    const size_t lds = stack_lds_bytes<float>(sc.stack_depth, 256);
-    hipLaunchKernelGGL(dev::wf_extend, dim3(extend_blocks), dim3(256), lds, stream, sc, pool, parity, rays);
+    const dim3 grid(wf_blocks(pool.capacity));
+    const size_t lds = stack_lds_bytes<float>(sc.stack_depth, dev::wf_block);
+    hipLaunchKernelGGL(dev::wf_begin, dim3(1), dim3(256), 0, stream, pool, parity);
+    hipLaunchKernelGGL(dev::wf_extend, grid, dim3(dev::wf_block), lds, stream, sc, pool, parity);
+    hipLaunchKernelGGL(dev::wf_sort, dim3(1), dim3(64), 0, stream, pool, parity);
     if (quality)
     {
-        hipLaunchKernelGGL(dev::wf_shade_quality, dim3(shade_blocks), dim3(256), lds, stream, cam, sc, seed, work,
-                           pool, parity, sums, rays);
+        hipLaunchKernelGGL(dev::wf_shade_quality, grid, dim3(dev::wf_block), lds, stream, cam, sc, seed, work, pool,
+                           parity, sums);
     }
     else
     {
-        hipLaunchKernelGGL(dev::wf_shade, dim3(shade_blocks), dim3(256), 0, stream, cam, sc, seed, work, pool,
-                           parity, sums);
+        hipLaunchKernelGGL(dev::wf_shade, grid, dim3(dev::wf_block), 0, stream, cam, sc, seed, work, pool, parity,
+                           sums);
     }
     return hipGetLastError();
 }
 
 hipError_t wf_launch_drain(int mode, const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
                            const dev::launch_work& work, const dev::wf_pool& pool, uint32_t count, double* sums,
-                           unsigned long long* rays, hipStream_t stream)
+                           hipStream_t stream)
 {
+    hipLaunchKernelGGL(dev::wf_begin, dim3(1), dim3(256), 0, stream, pool, 0);
     const int threads = 128;
     const dim3 grid((count + threads - 1) / threads);
     const size_t lds = stack_lds_bytes<float>(sc.stack_depth, threads);
     if (mode == dev::mode_quality)
     {
         hipLaunchKernelGGL((dev::wf_drain<dev::mode_quality>), grid, dim3(threads), lds, stream, cam, sc, seed, work,
-                           pool, 0, sums, rays);
+                           pool, sums);
     }
     else if (mode == dev::mode_all_books)
     {
         hipLaunchKernelGGL((dev::wf_drain<dev::mode_all_books>), grid, dim3(threads), lds, stream, cam, sc, seed, work,
-                           pool, 0, sums, rays);
+                           pool, sums);
     }
     else
     {
         hipLaunchKernelGGL((dev::wf_drain<dev::mode_first_book>), grid, dim3(threads), lds, stream, cam, sc, seed,
-                           work, pool, 0, sums, rays);
+                           work, pool, sums);
     }
+    return hipGetLastError();
+}
+
+hipError_t wf_launch_finish(const dev::wf_pool& pool, unsigned long long* rays, hipStream_t stream)
+{
+    hipLaunchKernelGGL(dev::wf_finish, dim3(1), dim3(64), 0, stream, pool, rays);
     return hipGetLastError();
 }
 
@@ -583,9 +665,9 @@ kernel_attributes wf_attributes(int depth, int which)
         k.registers = a.numRegs;
         k.local_bytes = static_cast<int>(a.localSizeBytes);
     }
-    k.lds_bytes = which != 1 ? static_cast<int>(stack_lds_bytes<float>(depth, 256)) : 0;
+    k.lds_bytes = which != 1 ? static_cast<int>(stack_lds_bytes<float>(depth, dev::wf_block)) : 0;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, 256, k.lds_bytes) == hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, dev::wf_block, k.lds_bytes) == hipSuccess)
     {
         k.blocks_per_cu = blocks;
     }

@@ -10,37 +10,12 @@
 
 namespace rt
 {
-std::size_t request_bytes(const rt_scene& s)
+std::size_t view_bytes(const rt_camera_desc& camera)
 {
-    const std::size_t pixels = static_cast<std::size_t>(s.camera.image_width) * s.camera.image_height;
-    std::size_t texels = 0;
-    for (int64_t i = 0; i < s.image_count; ++i)
-    {
-        texels += static_cast<std::size_t>(s.images[i].width) * s.images[i].height;
-    }
-    std::size_t prims = static_cast<std::size_t>(s.sphere_count + s.quad_count);
-    for (int64_t i = 0; i < s.instance_count; ++i)
-    {
-        prims += static_cast<std::size_t>(s.meshes[s.instances[i].mesh].triangle_count);
-    }
-    if (s.instance_count == 0)
-    {
-        for (int64_t i = 0; i < s.mesh_count; ++i)
-        {
-            prims += static_cast<std::size_t>(s.meshes[i].triangle_count);
-        }
-    }
-    std::size_t env = 0;
-    if (s.environment)
-    {
-        env = static_cast<std::size_t>(s.environment->image.width) * s.environment->image.height;
-    }
-    return 24 * pixels                                  // double sums
-           + 16 * 8 * pixels                            // the wavefront's slots
-           + 16 * texels                                // float4 texels
-           + prims * (64 + 48 + 16 + 4)                 // nodes, records, frames, materials ids
-           + env * (16 + 16)                            // map texels and alias bins (padded to a power of two)
-           + static_cast<std::size_t>(s.texture_count) * 4200 + (1u << 20);
+    const std::size_t pixels = static_cast<std::size_t>(camera.image_width) * camera.image_height;
+    return 24 * pixels          // the double sums
+           + 8 * 16 * pixels    // the wavefront's slots (rays, throughput, light, generator, hit, lists, queues)
+           + (1u << 20);
 }
 
 server::server(const rt_server_options& options) : options_(options)
@@ -64,20 +39,65 @@ double server::now() const
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - start_).count();
 }
 
-int64_t server::submit(const rt_scene& scene, uint64_t seed, int64_t passes)
+int64_t server::add_scene(const rt_scene& scene)
 {
-    validate_scene(scene);
+    std::shared_ptr<const shared_scene> s = upload_scene(scene, RT_FLOAT32);
+    const std::size_t bytes = device_bytes(*s);
+    if (used_bytes_ + bytes > options_.memory_budget_bytes)
+    {
+        throw std::invalid_argument("the scene doesn't fit the server's memory budget beside what is resident");
+    }
+    used_bytes_ += bytes;
+    const int64_t id = next_scene_++;
+    scenes_.emplace(id, resident{std::move(s), bytes, 0, false});
+    return id;
+}
+
+void server::drop_scene_if_unused(int64_t scene_id)
+{
+    const auto f = scenes_.find(scene_id);
+    if (f != scenes_.end() && f->second.removed && f->second.users == 0)
+    {
+        used_bytes_ -= f->second.bytes;
+        scenes_.erase(f);
+    }
+}
+
+void server::remove_scene(int64_t scene_id)
+{
+    const auto f = scenes_.find(scene_id);
+    if (f == scenes_.end() || f->second.removed)
+    {
+        throw std::invalid_argument("no resident scene with this id");
+    }
+    f->second.removed = true;
+    drop_scene_if_unused(scene_id);
+}
+
+int64_t server::submit_view(int64_t scene_id, const rt_camera_desc& camera, uint64_t seed, int64_t passes)
+{
+    const auto f = scenes_.find(scene_id);
+    if (f == scenes_.end() || f->second.removed)
+    {
+        throw std::invalid_argument("no resident scene with this id");
+    }
+    validate_camera(camera);
+    if (camera.sampling != scene_of(*f->second.scene).camera.sampling)
+    {
+        throw std::invalid_argument("a view must keep the scene's sampling strategy");
+    }
     if (passes <= 0)
     {
         throw std::invalid_argument("a request must ask for at least one pass");
     }
     auto r = std::make_unique<request>();
     r->id = next_id_++;
-    r->scene = scene_tables::copy_of(scene);
+    r->scene_id = scene_id;
+    r->camera = camera;
     r->seed = seed;
     r->passes = passes;
-    r->bytes = request_bytes(scene);
-    if (r->bytes > options_.memory_budget_bytes)
+    r->bytes = view_bytes(camera);
+    if (r->bytes + f->second.bytes > options_.memory_budget_bytes)
     {
         throw std::invalid_argument("the request needs more device memory than the server's whole budget");
     }
@@ -85,9 +105,28 @@ int64_t server::submit(const rt_scene& scene, uint64_t seed, int64_t passes)
     r->st.state = 0;
     r->st.passes_total = passes;
     r->st.submit_time = now();
+    ++f->second.users;
     const int64_t id = r->id;
     requests_.emplace(id, std::move(r));
     queue_.push_back(id);
+    return id;
+}
+
+int64_t server::submit(const rt_scene& scene, uint64_t seed, int64_t passes)
+{
+    const int64_t scene_id = add_scene(scene);
+    int64_t id = 0;
+    try
+    {
+        id = submit_view(scene_id, scene.camera, seed, passes);
+    }
+    catch (...)
+    {
+        remove_scene(scene_id);
+        throw;
+    }
+    requests_.at(id)->owns_scene = true;
+    scenes_.at(scene_id).removed = true; // freed with its request
     return id;
 }
 
@@ -111,8 +150,7 @@ void server::admit()
         {
             rt_session_options o{};
             o.seed = r.seed;
-            const rt_scene view = r.scene.view();
-            r.work = make_session(view, o);
+            r.work = make_session(scenes_.at(r.scene_id).scene, r.camera, o);
             used_bytes_ += r.bytes;
             r.st.state = 1;
             r.st.start_time = now();
@@ -229,7 +267,14 @@ void server::release(int64_t id)
     {
         used_bytes_ -= f->second->bytes;
     }
+    const int64_t scene_id = f->second->scene_id;
     queue_.erase(std::remove(queue_.begin(), queue_.end(), id), queue_.end());
     requests_.erase(f);
+    const auto s = scenes_.find(scene_id);
+    if (s != scenes_.end())
+    {
+        --s->second.users;
+        drop_scene_if_unused(scene_id);
+    }
 }
 }

@@ -51,14 +51,71 @@ private:
     hipStream_t s_ = nullptr;
 };
 
+}
+
+class shared_scene
+{
+public:
+    shared_scene(const rt_scene& scene, int32_t precision) : tables(scene_tables::copy_of(scene))
+    {
+        view = tables.view();
+        if (precision == RT_FLOAT64)
+        {
+            f64 = std::make_unique<device_scene<double>>(view);
+        }
+        else
+        {
+            f32 = std::make_unique<device_scene<float>>(view);
+        }
+    }
+    template <typename Real>
+    const device_scene<Real>& device() const
+    {
+        if constexpr (sizeof(Real) == sizeof(double))
+        {
+            return *f64;
+        }
+        else
+        {
+            return *f32;
+        }
+    }
+    scene_tables tables;
+    rt_scene view{};
+    std::unique_ptr<device_scene<float>> f32;
+    std::unique_ptr<device_scene<double>> f64;
+};
+
+std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision)
+{
+    validate_scene(scene);
+    if (precision != RT_FLOAT32 && precision != RT_FLOAT64)
+    {
+        throw std::invalid_argument("unknown precision");
+    }
+    if (precision != RT_FLOAT32 && (scene.mesh_count != 0 || scene.microfacet_count != 0 ||
+                                    scene.environment != nullptr || scene.combination != RT_COMBINE_BSDF_ONLY))
+    {
+        throw std::invalid_argument("triangles, GGX, environment maps and light samples render in float32 only");
+    }
+    return std::make_shared<const shared_scene>(scene, precision);
+}
+
+std::size_t device_bytes(const shared_scene& scene) { return scene.f32 ? scene.f32->bytes() : scene.f64->bytes(); }
+
+const rt_scene& scene_of(const shared_scene& scene) { return scene.view; }
+
+namespace
+{
 template <typename Real>
 class session_impl final : public session
 {
 public:
-    session_impl(const rt_scene& scene, const rt_session_options& options)
-        : scene_(scene_tables::copy_of(scene)), options_(options), dscene_(scene_.view())
+    session_impl(std::shared_ptr<const shared_scene> scene, const rt_camera_desc& camera,
+                 const rt_session_options& options)
+        : shared_(std::move(scene)), scene_(shared_->tables), options_(options), dscene_(shared_->device<Real>())
     {
-        view_scene_ = scene_.view();
+        view_scene_ = shared_->view;
         mode_ = dscene_.quality()                 ? dev::mode_quality
                 : first_book_scene(view_scene_) ? dev::mode_first_book
                                                 : dev::mode_all_books;
@@ -69,7 +126,12 @@ public:
         use_wavefront_ = options.integrator == RT_INTEGRATOR_WAVEFRONT ||
                          (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>());
         rays_ = device_array<unsigned long long>(2);
-        start_view(scene.camera);
+        validate_camera(camera);
+        if (camera.sampling != scene_.camera.sampling)
+        {
+            throw std::invalid_argument("a view must keep the scene's sampling strategy");
+        }
+        start_view(camera);
     }
 
     void set_camera(const rt_camera_desc& camera) override
@@ -349,10 +411,11 @@ public:
     }
 
 private:
-    scene_tables scene_;
+    std::shared_ptr<const shared_scene> shared_;
+    const scene_tables& scene_;
     rt_scene view_scene_{};
     rt_session_options options_;
-    device_scene<Real> dscene_;
+    const device_scene<Real>& dscene_;
     stream_owner stream_;
     hip_event start_, stop_;
     int mode_ = dev::mode_all_books;
@@ -438,9 +501,10 @@ private:
 };
 }
 
-std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_options& options)
+namespace
 {
-    validate_scene(scene);
+void check_options(const rt_session_options& options)
+{
     if (options.precision != RT_FLOAT32 && options.precision != RT_FLOAT64)
     {
         throw std::invalid_argument("unknown precision");
@@ -457,18 +521,28 @@ std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_op
     {
         throw std::invalid_argument("the block size must be 64, 128, 192 or 256 threads");
     }
-    if (scene.mesh_count != 0 || scene.microfacet_count != 0 || scene.environment != nullptr ||
-        scene.combination != RT_COMBINE_BSDF_ONLY)
+}
+}
+
+std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_options& options)
+{
+    validate_scene(scene);
+    check_options(options);
+    return make_session(upload_scene(scene, options.precision), scene.camera, options);
+}
+
+std::unique_ptr<session> make_session(std::shared_ptr<const shared_scene> scene, const rt_camera_desc& camera,
+                                      const rt_session_options& options)
+{
+    check_options(options);
+    if ((options.precision == RT_FLOAT64) != (scene->f64 != nullptr))
     {
-        if (options.precision != RT_FLOAT32)
-        {
-            throw std::invalid_argument("triangles, GGX, environment maps and light samples render in float32 only");
-        }
+        throw std::invalid_argument("the session's precision must be the resident scene's");
     }
     if (options.precision == RT_FLOAT64)
     {
-        return std::make_unique<session_impl<double>>(scene, options);
+        return std::make_unique<session_impl<double>>(std::move(scene), camera, options);
     }
-    return std::make_unique<session_impl<float>>(scene, options);
+    return std::make_unique<session_impl<float>>(std::move(scene), camera, options);
 }
 }

@@ -38,6 +38,16 @@ public:
     virtual rt_traffic traffic(int64_t passes) = 0;
 };
 
+// A scene resident on the device: its tables and its arrays (tree, records, textures) in one precision, shared by
+// every session rendered from it (the server keeps scenes resident while requests of any resolution come and go)
+class shared_scene;
+std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision);
+std::size_t device_bytes(const shared_scene& scene);
+const rt_scene& scene_of(const shared_scene& scene);
+
 // Validates the scene (before any GPU work), then builds its BVH and uploads it
 std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_options& options);
+// A session over a resident scene, with its own view (camera), image and stream
+std::unique_ptr<session> make_session(std::shared_ptr<const shared_scene> scene, const rt_camera_desc& camera,
+                                      const rt_session_options& options);
 }

@@ -7,13 +7,6 @@ namespace rt
 {
 namespace
 {
-int blocks_for(const kernel_attributes& a)
-{
-    int cus = 0;
-    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "reading the device's properties");
-    return std::max(1, cus * std::max(1, a.blocks_per_cu));
-}
-
 class graph_owner
 {
 public:
@@ -83,20 +76,26 @@ wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_re
         rng_ = device_array<uint4>(capacity_);
         hit_ = device_array<int4>(capacity_);
         pass_ = device_array<int32_t>(capacity_);
-        active0_ = device_array<uint32_t>(capacity_);
-        active1_ = device_array<uint32_t>(capacity_);
-        queue_ = device_array<uint32_t>(static_cast<std::size_t>(capacity_) * dev::wf_classes);
+        // each list is wf_stripes segments, each as long as the items a stripe's blocks can append
+        const uint32_t blocks = wf_blocks(capacity_);
+        const uint32_t segment = (blocks + dev::wf_stripes - 1) / dev::wf_stripes * dev::wf_block;
+        const std::size_t list = static_cast<std::size_t>(segment) * dev::wf_stripes;
+        active0_ = device_array<uint32_t>(list);
+        active1_ = device_array<uint32_t>(list);
+        queue_ = device_array<uint32_t>(list * dev::wf_classes);
         counters_ = device_array<uint32_t>(2 * dev::wf_counter_stride);
+        prefix_ = device_array<uint32_t>(dev::wf_prefix_size);
+        tallies_ = device_array<unsigned long long>(2 * dev::wf_stripes);
         check(hipMemsetAsync(counters_.data(), 0, counters_.bytes(), stream_), "clearing counters");
-        check(hipHostMalloc(reinterpret_cast<void**>(&host_count_), sizeof(uint32_t), hipHostMallocDefault),
+        check(hipMemsetAsync(tallies_.data(), 0, tallies_.bytes(), stream_), "clearing counters");
+        check(hipHostMalloc(reinterpret_cast<void**>(&host_count_), dev::wf_stripes * sizeof(uint32_t),
+                            hipHostMallocDefault),
               "allocating pinned memory");
         pool_ = {ray_o_.data(), ray_d_.data(), thr_.data(), qst_.size() > 0 ? qst_.data() : nullptr, rng_.data(),
                  hit_.data(), pass_.data(), {active0_.data(), active1_.data()}, queue_.data(), counters_.data(),
-                 capacity_};
+                 prefix_.data(), tallies_.data(), capacity_, segment};
         bytes_ = ray_o_.bytes() * 3 + qst_.bytes() + rng_.bytes() + hit_.bytes() + pass_.bytes() +
-                 active0_.bytes() * 2 + queue_.bytes() + counters_.bytes();
-        extend_blocks_ = blocks_for(wf_attributes(sc.stack_depth, 0));
-        shade_blocks_ = blocks_for(wf_attributes(sc.stack_depth, quality_ ? 2 : 1));
+                 active0_.bytes() * 2 + queue_.bytes() + counters_.bytes() + prefix_.bytes() + tallies_.bytes();
     }
 }
 
@@ -129,8 +128,7 @@ int64_t wavefront<Real>::render(uint64_t seed, const dev::launch_work& work, dou
             hipError_t launched = hipSuccess;
             for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
             {
-                launched = wf_launch_bounce(cam_, sc_, seed, work, pool_, b & 1, sums, rays, extend_blocks_,
-                                            shade_blocks_, quality_, stream_);
+                launched = wf_launch_bounce(cam_, sc_, seed, work, pool_, b & 1, sums, quality_, stream_);
             }
             return launched;
         };
@@ -162,21 +160,30 @@ int64_t wavefront<Real>::render(uint64_t seed, const dev::launch_work& work, dou
                 check(bounces(), "launching the bounce loop's kernels");
             }
             done += bounces_per_graph_;
-            check(hipMemcpyAsync(host_count_, counters_.data(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+            // the parity-0 active list's stripes: the next round's slots
+            check(hipMemcpyAsync(host_count_, counters_.data() + dev::wf_counter_active,
+                                 dev::wf_stripes * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
                   "reading the active count");
             check(hipStreamSynchronize(stream_), "rendering on the GPU");
-            if (*host_count_ == 0)
+            uint64_t active = 0;
+            for (int k = 0; k < dev::wf_stripes; ++k)
+            {
+                active += host_count_[k];
+            }
+            if (active == 0)
             {
                 break;
             }
-            if (*host_count_ < drain_below_)
+            if (active < drain_below_)
             {
-                check(wf_launch_drain(mode_, cam_, sc_, seed, work, pool_, *host_count_, sums, rays, stream_),
+                check(wf_launch_drain(mode_, cam_, sc_, seed, work, pool_, static_cast<uint32_t>(active), sums,
+                                      stream_),
                       "launching the wavefront's drain");
-                check(hipStreamSynchronize(stream_), "rendering on the GPU");
                 break;
             }
         }
+        check(wf_launch_finish(pool_, rays, stream_), "counting the launch's rays");
+        check(hipStreamSynchronize(stream_), "rendering on the GPU");
         return done;
     }
 }

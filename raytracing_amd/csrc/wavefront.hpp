@@ -54,10 +54,10 @@ private:
     device_array<uint4> rng_;
     device_array<int4> hit_;
     device_array<int32_t> pass_;
-    device_array<uint32_t> active0_, active1_, queue_, counters_;
-    uint32_t* host_count_ = nullptr; // pinned: the active count after each replay
+    device_array<uint32_t> active0_, active1_, queue_, counters_, prefix_;
+    device_array<unsigned long long> tallies_;
+    uint32_t* host_count_ = nullptr; // pinned: the active list's stripe lengths after each round
     dev::wf_pool pool_{};
-    int extend_blocks_ = 0, shade_blocks_ = 0;
     int bounces_per_graph_ = 8;
     bool use_graph_ = true;
     std::size_t bytes_ = 0;

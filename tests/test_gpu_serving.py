@@ -1,0 +1,100 @@
+"""GPU tests of the serving loop (config 4): requests of mixed scenes and resolutions admitted by a memory budget and
+rendered together, with their kernels overlapping on the device. Each request is a session of its own (its own
+stream, image and seed), and a pixel's samples are summed in pass order, so a result is the same bytes whether the
+request ran alone or in the mix: the stated tolerance is zero.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(server, ids):
+    while server.step() > 0:
+        pass
+    while any(server.status(i).state < 2 for i in ids):
+        server.step()
+
+
+def _view(rt, name, width, height):
+    return rt.build_scene(name, width=width, height=height, samples=1)
+
+
+def test_same_alone_and_in_mix(rt):
+    a = _view(rt, "final", 96, 96)
+    mix = [_view(rt, "weekend", 160, 90), _view(rt, "glass", 64, 64), _view(rt, "cornell-ggx", 80, 80),
+           _view(rt, "sun-test", 128, 72)]
+    alone = rt.Server()
+    sid = alone.add_scene(a)
+    ra = alone.submit_view(sid, a.desc.camera, 5, 24)
+    _run(alone, [ra])
+    img_alone = alone.pixels(ra)
+    st = alone.status(ra)
+    assert st.state == 2 and st.passes_done == 24 and st.milestone == 24
+    alone.close()
+
+    server = rt.Server(max_active=4, passes_per_step=3)
+    others = [server.submit(s, 11 + k, 16 + 8 * k) for k, s in enumerate(mix)]
+    sid = server.add_scene(a)
+    ra = server.submit_view(sid, a.desc.camera, 5, 24)
+    _run(server, others + [ra])
+    img_mix = server.pixels(ra)
+    for rid in others:
+        st = server.status(rid)
+        assert st.state == 2, f"request {rid} state {st.state}"
+    assert np.array_equal(img_alone.view(np.uint64), img_mix.view(np.uint64))
+    # a request alone equals a plain session of the same view and seed
+    with rt.Session(mix[0], seed=11) as s:
+        s.render_to(16)
+        ref = s.pixels()
+    assert np.array_equal(ref.view(np.uint64), server.pixels(others[0]).view(np.uint64))
+    server.close()
+
+
+def test_views_of_one_resident_scene(rt):
+    """Two resolutions of one resident scene: each equals its own session"""
+    server = rt.Server()
+    small, large = _view(rt, "cornell", 48, 48), _view(rt, "cornell", 96, 64)
+    sid = server.add_scene(small)
+    r1 = server.submit_view(sid, small.desc.camera, 1, 8)
+    r2 = server.submit_view(sid, large.desc.camera, 2, 8)
+    server.remove_scene(sid)  # stays resident until its requests are released
+    _run(server, [r1, r2])
+    for rid, view, seed in ((r1, small, 1), (r2, large, 2)):
+        with rt.Session(view, seed=seed) as s:
+            s.render_to(8)
+            assert np.array_equal(s.pixels().view(np.uint64), server.pixels(rid).view(np.uint64))
+    server.release(r1)
+    server.release(r2)
+    with pytest.raises(ValueError):
+        server.submit_view(sid, small.desc.camera, 1, 8)
+    server.close()
+
+
+def test_admission_by_memory_budget(rt):
+    """A request waits while the budget is full and starts once an earlier one is released"""
+    view = _view(rt, "weekend", 200, 112)
+    server = rt.Server(memory_budget_bytes=6 << 20, max_active=8)
+    sid = server.add_scene(view)
+    first = server.submit_view(sid, view.desc.camera, 1, 4)
+    second = server.submit_view(sid, view.desc.camera, 2, 4)
+    server.step()
+    assert server.status(first).state >= 1
+    assert server.status(second).state == 0, "the second view doesn't fit beside the first"
+    _run(server, [first])
+    server.release(first)
+    _run(server, [second])
+    assert server.status(second).state == 2
+    server.close()
+
+
+def test_milestones_are_powers_of_two(rt):
+    view = _view(rt, "glass", 48, 48)
+    server = rt.Server(passes_per_step=100)
+    rid = server.submit(view, 3, 40)
+    seen = []
+    while server.status(rid).state < 2:
+        server.step()
+        seen.append(server.status(rid).milestone)
+    assert seen == [1, 2, 4, 8, 16, 32, 40]
+    server.close()
