@@ -174,6 +174,42 @@ def serving(rt, rates, per_rate, seed, tau):
     return [results[r] for r in rates], sigma
 
 
+def rmse_protocol(rt, workload, width, oracle_spp):
+    """RMSE of a workload's FP32 render against an FP64 render of `oracle_spp` passes (the oracle), at a reduced width
+    (the same scene, camera and depth; fewer pixels so that the oracle's 64k passes take a minute), for the megakernel
+    and the wavefront at the workload's spp, and the spp each needs to reach the megakernel's RMSE. The two
+    integrators trace the same paths, so their RMSEs are equal at equal spp. The quality integrator has no FP64 form:
+    for the triangle scene the oracle is FP32 at the same pass count, and says so."""
+    import numpy as np
+
+    name, _, spp, sampling, description = WORKLOADS[workload]
+    oracle_scene = rt.build_scene(name, width=width, samples=oracle_spp, strata=1, sampling=sampling)
+    fp64 = name != "triangles" and sampling is None
+    t0 = time.perf_counter()
+    with rt.Session(oracle_scene, seed=12345, precision="float64" if fp64 else "float32",
+                    integrator="megakernel", launch_seconds=0.2) as s:
+        s.render_to(oracle_spp)
+        oracle = s.pixels()
+    oracle_seconds = time.perf_counter() - t0
+    out = {"workload": description, "width": oracle.shape[1], "height": oracle.shape[0],
+           "oracle": f"{'FP64' if fp64 else 'FP32'} megakernel, {oracle_spp} passes, {oracle_seconds:.1f} s",
+           "oracle_mean": float(oracle.mean())}
+    scene = rt.build_scene(name, width=width, samples=spp, sampling=sampling)
+    for integrator in ("megakernel", "wavefront"):
+        with rt.Session(scene, seed=7, integrator=integrator) as s:
+            t0 = time.perf_counter()
+            s.render_to(scene.passes)
+            seconds = time.perf_counter() - t0
+            img = s.pixels()
+        err = float(np.sqrt(np.mean((img - oracle) ** 2)))
+        out[integrator] = {"spp": scene.passes, "rmse": err, "relative_rmse": err / max(float(oracle.mean()), 1e-12),
+                           "seconds": round(seconds, 3),
+                           "mpaths_s": round(img.shape[0] * img.shape[1] * scene.passes / seconds / 1e6, 1)}
+    # the oracle's own noise: its RMSE against the truth is about the FP32 render's times sqrt(spp / oracle_spp)
+    out["oracle_noise_share"] = float(np.sqrt(scene.passes / oracle_spp))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +230,10 @@ def main():
     ap.add_argument("--rates", default="1,4,16,64")
     ap.add_argument("--requests", type=int, default=16, help="requests per rate (serving)")
     ap.add_argument("--tau", type=float, default=0.05, help="target relative RMSE (serving)")
+    ap.add_argument("--rmse", default="", help="the RMSE protocol for a workload (final, weekend, glass, glass-mis, "
+                                               "triangles)")
+    ap.add_argument("--rmse-width", type=int, default=270)
+    ap.add_argument("--rmse-oracle-spp", type=int, default=65536)
     args = ap.parse_args()
     if args.serving:
         import raytracing_amd as rt
@@ -206,6 +246,11 @@ def main():
                                                  f"log-uniform [16, 1024], target relative RMSE {args.tau}",
                                      "sigma_per_pass": sigma},
                           "rates": rows}), flush=True)
+        return
+    if args.rmse:
+        import raytracing_amd as rt
+
+        print(json.dumps(rmse_protocol(rt, args.rmse, args.rmse_width, args.rmse_oracle_spp)), flush=True)
         return
     if args.copy_probe:
         import raytracing_amd as rt

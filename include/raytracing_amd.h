@@ -330,7 +330,14 @@ typedef struct rt_session_options
                                kernel launched on its own, as profilers that don't follow graphs need) */
     int32_t wavefront_drain;/* wavefront: when fewer slots than this are active, each finishes its pixel's passes on
                                its own (0: the tuned default, a quarter of the pool; -1: never) */
+    int32_t megakernel_waves;/* megakernel: the waves per SIMD its registers are budgeted for: 0 (the compiler's
+                               choice), 5 or 6 (float only) */
+    int32_t reserved;
 } rt_session_options;
+/* RT_INTEGRATOR_AUTO (float32): the session's autotuner renders its first full passes with each candidate (the
+ * megakernel at each register budget and block size, and the wavefront), one pass each, and keeps the fastest for the
+ * rest. Every candidate traces the same paths with the same random numbers and sums them in the same order, so the
+ * image doesn't depend on which one renders which pass. */
 
 typedef struct rt_render_report
 {

@@ -126,7 +126,8 @@ class SceneOverrides(C.Structure):
 class SessionOptions(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("precision", C.c_int32), ("integrator", C.c_int32),
                 ("launch_seconds", C.c_double), ("block_size", C.c_int32), ("wavefront_slots", C.c_int32),
-                ("wavefront_graph", C.c_int32), ("wavefront_drain", C.c_int32)]
+                ("wavefront_graph", C.c_int32), ("wavefront_drain", C.c_int32), ("megakernel_waves", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class RenderReport(C.Structure):
@@ -284,9 +285,10 @@ def build_scene(name: str, **overrides) -> BuiltScene:
 
 
 def _session_options(seed=0, precision="float32", integrator="auto", launch_seconds=0.0, block_size=0,
-                     wavefront_slots=0, wavefront_graph=0, wavefront_drain=0) -> SessionOptions:
+                     wavefront_slots=0, wavefront_graph=0, wavefront_drain=0, megakernel_waves=0) -> SessionOptions:
     return SessionOptions(seed, 1 if precision in ("float64", "fp64") else 0, INTEGRATORS[integrator],
-                          launch_seconds, block_size, wavefront_slots, wavefront_graph, wavefront_drain)
+                          launch_seconds, block_size, wavefront_slots, wavefront_graph, wavefront_drain,
+                          megakernel_waves, 0)
 
 
 @dataclass

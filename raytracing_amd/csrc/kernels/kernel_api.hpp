@@ -21,10 +21,14 @@ struct kernel_attributes
 template <typename Real>
 size_t stack_lds_bytes(int depth, int threads);
 
+// waves: the occupancy the kernel's register budget targets (0: the compiler's choice, 5 or 6 waves per SIMD; the
+// FP64 and counting instances take 0 whatever is asked)
 template <typename Real>
 hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
                              uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays,
-                             int block_w, int block_h, hipStream_t stream, unsigned long long* tallies = nullptr);
+                             int block_w, int block_h, hipStream_t stream, unsigned long long* tallies = nullptr,
+                             int waves = 0);
+bool megakernel_waves_supported(int waves);
 
 template <typename Real>
 hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origins, const Real* directions,
@@ -37,5 +41,5 @@ hipError_t launch_display(const double* sums, int64_t pixels, double inverse_pas
 hipError_t launch_copy(const void* in, void* out, size_t bytes, hipStream_t stream);
 
 template <typename Real>
-kernel_attributes megakernel_attributes(int mode, int block_threads, int depth);
+kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves = 0);
 }

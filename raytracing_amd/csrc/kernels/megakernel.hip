@@ -63,8 +63,11 @@ __device__ inline void add_counts(unsigned long long rays, unsigned long long sh
     }
 }
 
-template <typename Real, bool AllBooks, bool Count>
-__global__ void __launch_bounds__(256) megakernel(const camera_rec<Real> cam, const scene_view<Real> sc,
+// Waves: the waves per SIMD the register allocation must leave room for (0: the compiler's choice); a tighter budget
+// trades spills for occupancy, and the session's autotuner picks per scene
+template <typename Real, bool AllBooks, bool Count, int Waves>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
+megakernel(const camera_rec<Real> cam, const scene_view<Real> sc,
                                                   const uint64_t seed, const launch_work work, double* __restrict__ sums,
                                                   unsigned long long* __restrict__ ray_count,
                                                   unsigned long long* __restrict__ tallies)
@@ -112,8 +115,9 @@ __global__ void __launch_bounds__(256) megakernel(const camera_rec<Real> cam, co
 
 // The quality integrator's megakernel (src/gpu/quality.cu:53-109): the same loop over the launch's passes, with light
 // samples and their shadow rays at every vertex that isn't a mirror's (quality.hpp)
-template <bool Count>
-__global__ void __launch_bounds__(256) quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc,
+template <bool Count, int Waves>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
+quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc,
                                                           const uint64_t seed, const launch_work work,
                                                           double* __restrict__ sums,
                                                           unsigned long long* __restrict__ ray_count,
@@ -218,25 +222,45 @@ size_t stack_lds_bytes(int depth, int threads)
 
 namespace
 {
-template <typename Real, bool Count>
-const void* megakernel_for(int mode)
+template <typename Real, bool Count, int Waves>
+const void* megakernel_for_waves(int mode)
 {
     if constexpr (sizeof(Real) == sizeof(float))
     {
         if (mode == dev::mode_quality)
         {
-            return reinterpret_cast<const void*>(&dev::quality_megakernel<Count>);
+            return reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves>);
         }
     }
-    return mode == dev::mode_all_books ? reinterpret_cast<const void*>(&dev::megakernel<Real, true, Count>)
-                                       : reinterpret_cast<const void*>(&dev::megakernel<Real, false, Count>);
+    return mode == dev::mode_all_books ? reinterpret_cast<const void*>(&dev::megakernel<Real, true, Count, Waves>)
+                                       : reinterpret_cast<const void*>(&dev::megakernel<Real, false, Count, Waves>);
+}
+
+// The instance for a mode and an occupancy target: the counting instances and the FP64 ones at the compiler's choice
+template <typename Real, bool Count>
+const void* megakernel_for(int mode, int waves)
+{
+    if constexpr (!Count && sizeof(Real) == sizeof(float))
+    {
+        if (waves == 5)
+        {
+            return megakernel_for_waves<Real, Count, 5>(mode);
+        }
+        if (waves == 6)
+        {
+            return megakernel_for_waves<Real, Count, 6>(mode);
+        }
+    }
+    return megakernel_for_waves<Real, Count, 0>(mode);
 }
 }
+
+bool megakernel_waves_supported(int waves) { return waves == 0 || waves == 5 || waves == 6; }
 
 template <typename Real>
 hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
                              uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays,
-                             int block_w, int block_h, hipStream_t stream, unsigned long long* tallies)
+                             int block_w, int block_h, hipStream_t stream, unsigned long long* tallies, int waves)
 {
     if (mode == dev::mode_quality && sizeof(Real) != sizeof(float))
     {
@@ -245,7 +269,7 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
     const dim3 block(block_w, block_h);
     const dim3 grid((cam.image_width + block_w - 1) / block_w, (work.row_count + block_h - 1) / block_h);
     const size_t lds = stack_lds_bytes<Real>(sc.stack_depth, block_w * block_h);
-    const void* f = tallies ? megakernel_for<Real, true>(mode) : megakernel_for<Real, false>(mode);
+    const void* f = tallies ? megakernel_for<Real, true>(mode, 0) : megakernel_for<Real, false>(mode, waves);
     dev::camera_rec<Real> c = cam;
     dev::scene_view<Real> v = sc;
     uint64_t s = seed;
@@ -276,10 +300,10 @@ hipError_t launch_display(const double* sums, int64_t pixels, double inverse_pas
 }
 
 template <typename Real>
-kernel_attributes megakernel_attributes(int mode, int block_threads, int depth)
+kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves)
 {
     hipFuncAttributes a{};
-    const void* f = megakernel_for<Real, false>(mode);
+    const void* f = megakernel_for<Real, false>(mode, waves);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)
     {
@@ -297,15 +321,15 @@ kernel_attributes megakernel_attributes(int mode, int block_threads, int depth)
 
 template hipError_t launch_megakernel<float>(int, const dev::camera_rec<float>&, const dev::scene_view<float>&,
                                              uint64_t, const dev::launch_work&, double*, unsigned long long*, int, int,
-                                             hipStream_t, unsigned long long*);
+                                             hipStream_t, unsigned long long*, int);
 template hipError_t launch_megakernel<double>(int, const dev::camera_rec<double>&, const dev::scene_view<double>&,
                                               uint64_t, const dev::launch_work&, double*, unsigned long long*, int, int,
-                                              hipStream_t, unsigned long long*);
+                                              hipStream_t, unsigned long long*, int);
 template hipError_t launch_closest_hit<float>(const dev::scene_view<float>&, const float*, const float*, const float*,
                                               const int32_t*, const uint8_t*, int64_t, int32_t*, float*, uint8_t*,
                                               hipStream_t);
-template kernel_attributes megakernel_attributes<float>(int, int, int);
-template kernel_attributes megakernel_attributes<double>(int, int, int);
+template kernel_attributes megakernel_attributes<float>(int, int, int, int);
+template kernel_attributes megakernel_attributes<double>(int, int, int, int);
 template size_t stack_lds_bytes<float>(int, int);
 template size_t stack_lds_bytes<double>(int, int);
 }

@@ -120,11 +120,29 @@ public:
                 : first_book_scene(view_scene_) ? dev::mode_first_book
                                                 : dev::mode_all_books;
         launch_seconds_ = options.launch_seconds > 0 ? options.launch_seconds : 0.05;
-        const int threads = options.block_size > 0 ? options.block_size : 128;
-        block_w_ = 16;
-        block_h_ = std::max(1, threads / 16);
-        use_wavefront_ = options.integrator == RT_INTEGRATOR_WAVEFRONT ||
-                         (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>());
+        if (!megakernel_waves_supported(options.megakernel_waves))
+        {
+            throw std::invalid_argument("the megakernel's waves per SIMD must be 0, 5 or 6");
+        }
+        if (options.integrator == RT_INTEGRATOR_WAVEFRONT && !wavefront_supported<Real>())
+        {
+            throw std::invalid_argument("the wavefront integrator renders in float32 only");
+        }
+        current_ = {options.integrator == RT_INTEGRATOR_WAVEFRONT, options.megakernel_waves,
+                    options.block_size > 0 ? options.block_size : 128};
+        if (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>())
+        {
+            // the autotuner's candidates, in the order they render the first full passes
+            for (const int waves : {0, 5, 6})
+            {
+                for (const int block : {128, 256})
+                {
+                    candidates_.push_back({false, waves, block});
+                }
+            }
+            candidates_.push_back({true, 0, 256});
+            current_ = candidates_.front();
+        }
         rays_ = device_array<unsigned long long>(2);
         validate_camera(camera);
         if (camera.sampling != scene_.camera.sampling)
@@ -177,19 +195,34 @@ public:
                 const int64_t left_in_round = round_size - work.first_pass;
                 int64_t count = seconds_per_pass_ > 0 ? static_cast<int64_t>(launch_seconds_ / seconds_per_pass_) : 1;
                 count = std::max<int64_t>(1, std::min({count, left_in_round, target_passes - pass}));
-                work.pass_count = static_cast<int32_t>(count);
+                work.pass_count = tuning() ? 1 : static_cast<int32_t>(count);
                 work.first_row = 0;
                 work.row_count = height;
             }
-            const double seconds = launch(options_.seed + static_cast<uint64_t>(round) * round_seed_step, work);
+            const bool whole = work.first_row == 0 && work.row_count == height;
+            const bool probe = whole && tuning();
+            const variant v = probe ? candidates_[tuned_] : current_;
+            const double seconds = launch(options_.seed + static_cast<uint64_t>(round) * round_seed_step, work, v);
             report.launches += 1;
             report.bounces += last_bounces_;
             report.gpu_seconds += seconds;
             report.longest_launch_seconds = std::max(report.longest_launch_seconds, seconds);
-            if (work.first_row == 0 && work.row_count == height)
+            if (whole)
             {
                 seconds_per_pass_ = seconds / work.pass_count;
                 state_.passes += work.pass_count;
+                if (probe)
+                {
+                    // a candidate's pass timed: once all are, the fastest renders the rest
+                    tune_seconds_.push_back(seconds);
+                    if (++tuned_ == candidates_.size())
+                    {
+                        const auto best = static_cast<std::size_t>(
+                            std::min_element(tune_seconds_.begin(), tune_seconds_.end()) - tune_seconds_.begin());
+                        current_ = candidates_[best];
+                        seconds_per_pass_ = tune_seconds_[best];
+                    }
+                }
             }
             else if (work.first_row + work.row_count == height)
             {
@@ -304,20 +337,30 @@ public:
     {
         rt_kernel_info info{};
         kernel_attributes a{};
-        const char* name = nullptr;
-        if (use_wavefront_)
+        const int depth = dscene_.view().stack_depth;
+        if (current_.wavefront)
         {
-            a = wavefront_->attributes();
-            name = "wavefront";
-            info.threads_per_block = 256;
+            a = wf_attributes(depth, 0);
+            const kernel_attributes s = wf_attributes(depth, mode_ == dev::mode_quality ? 2 : 1);
+            a.registers = std::max(a.registers, s.registers);
+            a.local_bytes = std::max(a.local_bytes, s.local_bytes);
+            info.threads_per_block = dev::wf_block;
+            std::snprintf(info.name, sizeof info.name, "wavefront");
         }
         else
         {
-            a = megakernel_attributes<Real>(mode_, block_w_ * block_h_, dscene_.view().stack_depth);
-            name = mode_ == dev::mode_quality ? "quality" : (mode_ == dev::mode_all_books ? "all books" : "first book");
-            info.threads_per_block = block_w_ * block_h_;
+            a = megakernel_attributes<Real>(mode_, current_.block, depth, current_.waves);
+            const char* name =
+                mode_ == dev::mode_quality ? "quality" : (mode_ == dev::mode_all_books ? "all books" : "first book");
+            info.threads_per_block = current_.block;
+            char budget[16] = "";
+            if (current_.waves > 0)
+            {
+                std::snprintf(budget, sizeof budget, " w%d", current_.waves);
+            }
+            std::snprintf(info.name, sizeof info.name, "%s%s%s%s", name, sizeof(Real) == 8 ? " fp64" : "", budget,
+                          current_.block != 128 ? (current_.block == 256 ? " b256" : " b64") : "");
         }
-        std::snprintf(info.name, sizeof info.name, "%s%s", name, sizeof(Real) == 8 ? " fp64" : "");
         info.registers = a.registers;
         info.local_bytes = a.local_bytes;
         info.lds_bytes = a.lds_bytes;
@@ -377,7 +420,7 @@ public:
         check(hipMemsetAsync(counts.data(), 0, counts.bytes(), stream_.get()), "clearing counters");
         const dev::launch_work work{0, static_cast<int32_t>(passes), 0, camera_.image_height};
         check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), options_.seed, work, scratch.data(),
-                                      counts.data(), block_w_, block_h_, stream_.get(), counts.data() + 2),
+                                      counts.data(), 16, 8, stream_.get(), counts.data() + 2),
               "launching the counting kernel");
         check(hipStreamSynchronize(stream_.get()), "counting traffic");
         unsigned long long host[2 + RT_TRAFFIC_EVENTS] = {};
@@ -419,10 +462,20 @@ private:
     stream_owner stream_;
     hip_event start_, stop_;
     int mode_ = dev::mode_all_books;
-    bool use_wavefront_ = false;
-    std::unique_ptr<wavefront<Real>> wavefront_;
+    // what renders a launch: the megakernel (its register budget and block size) or the wavefront
+    struct variant
+    {
+        bool wavefront = false;
+        int waves = 0;
+        int block = 128;
+    };
+    variant current_{};
+    std::vector<variant> candidates_; // the autotuner's, each timed on one full pass
+    std::size_t tuned_ = 0;           // candidates timed so far
+    std::vector<double> tune_seconds_;
+    bool tuning() const { return tuned_ < candidates_.size(); }
+    std::unique_ptr<wavefront<Real>> wavefront_; // made at the first launch that needs it in a view
     double launch_seconds_ = 0.05;
-    int block_w_ = 16, block_h_ = 8;
     rt_camera_desc camera_{};
     dev::camera_rec<Real> cam_{};
     device_array<double> sums_;
@@ -458,10 +511,7 @@ private:
         }
         check(hipMemsetAsync(sums_.data(), 0, sums_.bytes(), stream_.get()), "clearing the image");
         check(hipMemsetAsync(rays_.data(), 0, rays_.bytes(), stream_.get()), "clearing the counters");
-        if (use_wavefront_)
-        {
-            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, mode_, options_, stream_.get());
-        }
+        wavefront_.reset(); // its pool is the view's size
         check(hipStreamSynchronize(stream_.get()), "starting the view");
         state_ = {};
         kernel_seconds_ = 0;
@@ -470,11 +520,15 @@ private:
     }
 
     // One launch of the view's work, timed on the GPU
-    double launch(uint64_t seed, const dev::launch_work& work)
+    double launch(uint64_t seed, const dev::launch_work& work, const variant& v)
     {
+        if (v.wavefront && !wavefront_)
+        {
+            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, mode_, options_, stream_.get());
+        }
         start_.record(stream_.get());
         last_bounces_ = 0;
-        if (use_wavefront_)
+        if (v.wavefront)
         {
             // a pool smaller than the launch's rows takes them in bands: each band's pixels are its own
             const int band = wavefront_->rows_capacity();
@@ -488,8 +542,8 @@ private:
         }
         else
         {
-            check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), rays_.data(),
-                                          block_w_, block_h_, stream_.get()),
+            check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), rays_.data(), 16,
+                                          std::max(1, v.block / 16), stream_.get(), nullptr, v.waves),
                   "launching the kernel");
         }
         stop_.record(stream_.get());
