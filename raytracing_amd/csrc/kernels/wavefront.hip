@@ -1,21 +1,25 @@
-// The wavefront integrator's kernels (float). A launch's work, passes [first_pass, first_pass + pass_count) of rows
-// [first_row, first_row + row_count), is a pool of path slots, one per pixel. A slot traces its pixel's passes one
-// after the other: when its path ends, the sample is added to the pixel's double sums and the slot starts the pixel's
-// next pass in place. So every pixel has one path in flight and adds its samples in pass order, as the megakernel does,
-// and every path draws the same random numbers in the same order: the two integrators' images are the same bytes.
+// The wavefront integrator's kernels (float). A launch's work is a list of work items, pass-major over the launch's
+// pixels (item k: pass first_pass + k / pixels of pixel k % pixels), rendered by a pool of path slots: a slot whose
+// path ends writes the path's sample to the item's place and takes the next item, so the pool stays full until the
+// items run out, whatever the paths' lengths. At the launch's end each pixel's samples are added to its double sums in
+// pass order, as the megakernel adds them, and every path draws the same random numbers: the two integrators' images
+// are the same bytes.
 //
-// One bounce of every active slot is four kernels:
-//   begin:  the active list's prefix sums over its stripes, and the counters the bounce fills cleared (one block);
+// One bounce of every active slot is five kernels:
+//   begin:  the free slots get the next items (as many as there are, or items left), the active list's prefix sums
+//           (its stripes, then the refilled slots), and the counters the bounce fills cleared (one block);
+//   refill: each refilled slot starts its item's path;
 //   extend: the closest event of each active slot's ray (BVH walk with an LDS stack, then the media), written to the
 //           slot, and the slot appended to the queue of its hit's material class;
 //   sort:   the queues' prefix sums, class by class (one block);
 //   shade:  the queues in class order (miss, lamp, matte, textured, marble, smoke, metal, glass, GGX, gloss), so that
-//           a wave shades one material at a time: the bounce (or the end of the path, its sample, and the pixel's next
-//           pass), and the slot appended to the next bounce's active list.
-// Every thread of extend and shade takes one item by its index (no work counter); appends are counted per wave with
-// ballots, per block in LDS, then one device-scope atomic per block and list on a stripe of the list (wf_stripes).
-// The lengths live in device memory, so a bounce needs no host synchronisation and a run of bounces is captured in a
-// hipGraph. When few slots are left, the drain kernel finishes them a thread each.
+//           a wave shades one material at a time: the bounce, or the end of the path (its sample written, the slot
+//           appended to the free list); a path that goes on is appended to the next bounce's active list.
+// Every thread of refill, extend and shade takes one item by its index (no work counter); appends are counted per
+// wave with ballots, per block in LDS, then one device-scope atomic per block and list on a stripe of the list
+// (wf_stripes). The lengths live in device memory, so a bounce needs no host synchronisation and a run of bounces is
+// captured in a hipGraph. When the items are handed out and few slots are left, the drain kernel finishes them a
+// thread each.
 #include "kernel_api.hpp"
 #include "path_tracer.hpp"
 #include "quality.hpp"
@@ -36,6 +40,42 @@ __device__ inline unsigned long long wsum(unsigned long long v)
 
 __device__ inline int lane_id() { return static_cast<int>(threadIdx.x & 63); }
 __device__ inline int stripe_of_block() { return static_cast<int>(blockIdx.x % wf_stripes); }
+
+// An exclusive prefix sum of n counters into out[0..n] (out[n]: the total) by one wave; other waves return
+__device__ inline void wave_prefix(const uint32_t* in, uint32_t* out, int n)
+{
+    if (threadIdx.x >= 64)
+    {
+        return;
+    }
+    const int lane = lane_id();
+    const int per = (n + 63) / 64;
+    const int first = lane * per;
+    uint32_t mine = 0;
+    for (int k = first; k < first + per && k < n; ++k)
+    {
+        mine += in[k];
+    }
+    uint32_t scan = mine; // inclusive scan across the wave
+    for (int offset = 1; offset < 64; offset <<= 1)
+    {
+        const uint32_t up = __shfl_up(scan, offset, 64);
+        if (lane >= offset)
+        {
+            scan += up;
+        }
+    }
+    uint32_t run = scan - mine;
+    for (int k = first; k < first + per && k < n; ++k)
+    {
+        out[k] = run;
+        run += in[k];
+    }
+    if (lane == 63)
+    {
+        out[n] = scan;
+    }
+}
 
 // A slot's path in the SoA pool
 struct slot_path
@@ -229,10 +269,18 @@ __device__ inline int hit_class(const scene_view<float>& sc, int index)
     }
 }
 
-// The first path of a slot: its pixel's pass `pass`. A quality path starts as the camera's: specular, no light yet.
-__device__ inline slot_path start_slot(const camera_rec<float>& cam, const scene_view<float>& sc, uint64_t seed,
-                                       uint64_t pixel, int i, int j, int pass, const wf_pool& pool, uint32_t s)
+// A slot's new path: work item `item` of the launch (pass first_pass + item / pixels, pixel item % pixels of the
+// launch's rows). A quality path starts as the camera's: specular, no light yet.
+__device__ inline slot_path start_item(const camera_rec<float>& cam, const scene_view<float>& sc, uint64_t seed,
+                                       const launch_work& work, const wf_pool& pool, uint32_t s, uint32_t item)
 {
+    const uint32_t width = static_cast<uint32_t>(cam.image_width);
+    const uint32_t pixels = width * static_cast<uint32_t>(work.row_count);
+    const uint32_t local = item % pixels;
+    const int pass = work.first_pass + static_cast<int>(item / pixels);
+    const int i = static_cast<int>(local % width);
+    const int j = work.first_row + static_cast<int>(local / width);
+    const uint64_t pixel = static_cast<uint64_t>(j) * width + static_cast<uint64_t>(i);
     slot_path sp;
     sp.g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
     sp.p = start_path(cam, sc.motion != 0, i, j, pass, sp.g);
@@ -244,43 +292,20 @@ __device__ inline slot_path start_slot(const camera_rec<float>& cam, const scene
     return sp;
 }
 
-// The slot behind item g of the active list of parity p (its stripes' prefix sums in LDS)
+// The slot behind entry g of the active list of parity p (its segments' prefix sums in LDS)
 __device__ inline uint32_t active_slot(const wf_pool& pool, int p, const uint32_t* prefix, uint32_t g)
 {
-    const int stripe = find_segment(prefix, wf_stripes, g);
-    return pool.active[p][static_cast<size_t>(stripe) * pool.segment + (g - prefix[stripe])];
-}
-
-// A finished path's sample into its pixel's sums, and the pixel's next pass started in the slot; false when the
-// pixel has no pass left in the launch
-__device__ inline bool finish_sample(const camera_rec<float>& cam, const scene_view<float>& sc, uint64_t seed,
-                                     const launch_work& work, const wf_pool& pool, uint32_t s, vec3<float> sample,
-                                     double* __restrict__ sums)
-{
-    const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
-    const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
-    const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
-    double* const sum = sums + 3 * pixel;
-    sum[0] += without_nan(sample.x);
-    sum[1] += without_nan(sample.y);
-    sum[2] += without_nan(sample.z);
-    const int next = pool.pass[s] + 1;
-    if (next >= work.first_pass + work.pass_count)
-    {
-        return false;
-    }
-    pool.pass[s] = next;
-    store_path(pool, s, start_slot(cam, sc, seed, pixel, i, j, next, pool, s));
-    return true;
+    const int segment = find_segment(prefix, wf_stripes + 1, g);
+    return pool.active[p][static_cast<size_t>(segment) * pool.segment + (g - prefix[segment])];
 }
 }
 
-// Every slot's first path, and the parity-0 active list holding every slot: block b fills its stripe's segment at
-// (b / wf_stripes) * wf_block, so the list is written without atomics and its stripes' lengths have a closed form
-__global__ void __launch_bounds__(256) wf_init(const camera_rec<float> cam, const scene_view<float> sc,
-                                               const uint64_t seed, const launch_work work, wf_pool pool)
+// A launch's start: its items; every slot on the free list (block b fills its stripe's segment at
+// (b / wf_stripes) * wf_block, so the list is written without atomics and its stripes' lengths have a closed form);
+// no slot active
+__global__ void __launch_bounds__(256) wf_init(wf_pool pool, uint32_t items)
 {
-    const uint32_t slots = static_cast<uint32_t>(cam.image_width) * static_cast<uint32_t>(work.row_count);
+    const uint32_t slots = pool.capacity;
     const uint32_t blocks = (slots + wf_block - 1) / wf_block;
     if (blockIdx.x == 0 && threadIdx.x < wf_stripes)
     {
@@ -296,40 +321,47 @@ __global__ void __launch_bounds__(256) wf_init(const camera_rec<float> cam, cons
                 length -= blocks * wf_block - slots; // the last block is short
             }
         }
-        pool.counters[wf_counter_active + stripe] = length;
+        pool.counters[wf_counter_free + stripe] = length;
+        pool.counters[wf_counter_active + stripe] = 0; // parity 0: nothing active yet
+        if (stripe == 0)
+        {
+            pool.counters[wf_counter_next] = 0;
+            pool.counters[wf_counter_items] = items;
+        }
     }
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= slots)
+    if (s < slots)
     {
-        return;
+        const uint32_t stripe = blockIdx.x % wf_stripes;
+        pool.free[static_cast<size_t>(stripe) * pool.segment + (blockIdx.x / wf_stripes) * wf_block + threadIdx.x] = s;
     }
-    const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
-    const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
-    const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
-    const slot_path sp = start_slot(cam, sc, seed, pixel, i, j, work.first_pass, pool, s);
-    store_path(pool, s, sp);
-    pool.pass[s] = work.first_pass;
-    const uint32_t stripe = blockIdx.x % wf_stripes;
-    pool.active[0][static_cast<size_t>(stripe) * pool.segment + (blockIdx.x / wf_stripes) * wf_block + threadIdx.x] = s;
 }
 
-// The start of a bounce of parity p: the active list's prefix sums; the class queues of parity p and the next active
-// list cleared (one block)
+// The start of a bounce of parity p (one block): the free slots get the next items, as many as there are of either;
+// the active list's prefix sums (its stripes, then the refilled slots); the queues of parity p, the next active list
+// and the free list cleared for the bounce to fill
 __global__ void __launch_bounds__(256) wf_begin(wf_pool pool, int p)
 {
     uint32_t* in = pool.counters + wf_counter_stride * p;
     uint32_t* out = pool.counters + wf_counter_stride * (1 - p);
-    const int t = static_cast<int>(threadIdx.x);
-    if (t == 0)
+    __shared__ uint32_t active_total;
+    wave_prefix(pool.counters + wf_counter_free, pool.prefix + wf_prefix_free, wf_stripes);
+    wave_prefix(in + wf_counter_active, pool.prefix + wf_prefix_active, wf_stripes);
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
-        uint32_t sum = 0;
-        for (int k = 0; k < wf_stripes; ++k)
-        {
-            pool.prefix[wf_prefix_active + k] = sum;
-            sum += in[wf_counter_active + k];
-        }
-        pool.prefix[wf_prefix_active + wf_stripes] = sum;
+        const uint32_t free = pool.prefix[wf_prefix_free + wf_stripes];
+        const uint32_t next = pool.counters[wf_counter_next];
+        const uint32_t left = pool.counters[wf_counter_items] - next;
+        const uint32_t refill = free < left ? free : left;
+        pool.counters[wf_counter_refill] = refill;
+        pool.counters[wf_counter_refill_base] = next;
+        pool.counters[wf_counter_next] = next + refill;
+        active_total = pool.prefix[wf_prefix_active + wf_stripes];
+        pool.prefix[wf_prefix_active + wf_stripes + 1] = active_total + refill;
     }
+    __syncthreads();
+    const int t = static_cast<int>(threadIdx.x);
     for (int k = t; k < wf_classes * wf_stripes; k += static_cast<int>(blockDim.x))
     {
         in[wf_counter_class + k] = 0;
@@ -337,32 +369,48 @@ __global__ void __launch_bounds__(256) wf_begin(wf_pool pool, int p)
     if (t < wf_stripes)
     {
         out[wf_counter_active + t] = 0;
+        pool.counters[wf_counter_free + t] = 0;
     }
 }
 
-// The queues' prefix sums, class-major (one block)
+// The refilled slots start their items' paths and join the active list's last segment
+__global__ void __launch_bounds__(256) wf_refill(const camera_rec<float> cam, const scene_view<float> sc,
+                                                 const uint64_t seed, const launch_work work, wf_pool pool, int p)
+{
+    __shared__ uint32_t prefix[wf_stripes + 1];
+    load_prefix(prefix, pool.prefix + wf_prefix_free, wf_stripes + 1);
+    const uint32_t refill = pool.counters[wf_counter_refill];
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= refill)
+    {
+        return;
+    }
+    const int stripe = find_segment(prefix, wf_stripes, g);
+    const uint32_t s = pool.free[static_cast<size_t>(stripe) * pool.segment + (g - prefix[stripe])];
+    const uint32_t item = pool.counters[wf_counter_refill_base] + g;
+    store_path(pool, s, start_item(cam, sc, seed, work, pool, s, item));
+    pool.item[s] = item;
+    pool.active[p][static_cast<size_t>(wf_stripes) * pool.segment + g] = s;
+}
+
+// The queues' prefix sums, class-major (one wave)
 __global__ void __launch_bounds__(64) wf_sort(wf_pool pool, int p)
 {
-    const uint32_t* in = pool.counters + wf_counter_stride * p + wf_counter_class;
-    if (threadIdx.x == 0)
-    {
-        uint32_t sum = 0;
-        for (int k = 0; k < wf_classes * wf_stripes; ++k)
-        {
-            pool.prefix[wf_prefix_class + k] = sum;
-            sum += in[k];
-        }
-        pool.prefix[wf_prefix_class + wf_classes * wf_stripes] = sum;
-    }
+    wave_prefix(pool.counters + wf_counter_stride * p + wf_counter_class, pool.prefix + wf_prefix_class,
+                wf_classes * wf_stripes);
 }
 
 __global__ void __launch_bounds__(256) wf_extend(const scene_view<float> sc, wf_pool pool, int p)
 {
-    __shared__ uint32_t prefix[wf_stripes + 1];
-    load_prefix(prefix, pool.prefix + wf_prefix_active, wf_stripes + 1);
+    __shared__ uint32_t prefix[wf_stripes + 2];
+    load_prefix(prefix, pool.prefix + wf_prefix_active, wf_stripes + 2);
     const lds_stack<float> stack = thread_stack(sc.stack_depth);
-    const uint32_t n = prefix[wf_stripes];
+    const uint32_t n = prefix[wf_stripes + 1];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= n)
+    {
+        return; // the whole block is past the list (uniform across the block: no barrier below is skipped by some)
+    }
     int cls = -1;
     uint32_t s = 0;
     if (g < n)
@@ -378,18 +426,33 @@ __global__ void __launch_bounds__(256) wf_extend(const scene_view<float> sc, wf_
         }
         cls = hit_class(sc, h.index);
     }
-    if (blockIdx.x * blockDim.x >= n)
-    {
-        return; // the whole block is past the list (uniform across the block: no barrier below is skipped by some)
-    }
     block_tally(pool.tallies, 0, cls >= 0 ? 1 : 0);
     block_append<wf_classes>(cls, s, pool.counters + wf_counter_stride * p + wf_counter_class, pool.queue,
                              pool.segment);
 }
 
+// A queue entry's slot (the queues' prefix sums in LDS)
+__device__ inline uint32_t queued_slot(const wf_pool& pool, const uint32_t* prefix, uint32_t g)
+{
+    const int list = find_segment(prefix, wf_classes * wf_stripes, g);
+    return pool.queue[static_cast<size_t>(list) * pool.segment + (g - prefix[list])];
+}
+
+// After shading: a path that goes on joins the next active list; one that ended leaves its sample and joins the free
+// list (every thread of the block calls it)
+__device__ inline void after_shade(const wf_pool& pool, int p, uint32_t s, bool valid, bool done, vec3<float> sample)
+{
+    if (valid && done)
+    {
+        pool.samples[pool.item[s]] = make_float4(sample.x, sample.y, sample.z, 0.0f);
+    }
+    block_append<1>(valid && !done ? 0 : -1, s, pool.counters + wf_counter_stride * (1 - p) + wf_counter_active,
+                    pool.active[1 - p], pool.segment);
+    block_append<1>(valid && done ? 0 : -1, s, pool.counters + wf_counter_free, pool.free, pool.segment);
+}
+
 __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, const scene_view<float> sc,
-                                                const uint64_t seed, const launch_work work, wf_pool pool, int p,
-                                                double* __restrict__ sums)
+                                                const uint64_t seed, const launch_work work, wf_pool pool, int p)
 {
     __shared__ uint32_t prefix[wf_classes * wf_stripes + 1];
     load_prefix(prefix, pool.prefix + wf_prefix_class, wf_classes * wf_stripes + 1);
@@ -399,18 +462,16 @@ __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, con
     {
         return;
     }
-    bool keep = false;
+    bool done = false;
     uint32_t s = 0;
+    vec3<float> sample{0.0f, 0.0f, 0.0f};
     if (g < total)
     {
-        const int list = find_segment(prefix, wf_classes * wf_stripes, g);
-        s = pool.queue[static_cast<size_t>(list) * pool.segment + (g - prefix[list])];
+        s = queued_slot(pool, prefix, g);
         slot_path sp = load_path(pool, s);
         const int4 hv = pool.hit[s];
         const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
         tally<false> tl;
-        vec3<float> sample;
-        bool done;
         if (h.index < 0)
         {
             sample = sp.p.throughput * sky(cam, sp.p.r.direction);
@@ -423,22 +484,16 @@ __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, con
         if (!done)
         {
             store_path(pool, s, sp);
-            keep = true;
-        }
-        else
-        {
-            keep = finish_sample(cam, sc, seed, work, pool, s, sample, sums);
         }
     }
-    block_append<1>(keep ? 0 : -1, s, pool.counters + wf_counter_stride * (1 - p) + wf_counter_active,
-                    pool.active[1 - p], pool.segment);
+    after_shade(pool, p, s, g < total, done, sample);
 }
 
 // The quality integrator's shade: the same queues in class order, each vertex's light sample and its shadow ray
 // traced here (an any-hit walk with this thread's LDS stack), in the megakernel's order of random numbers
 __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> cam, const scene_view<float> sc,
                                                         const uint64_t seed, const launch_work work, wf_pool pool,
-                                                        int p, double* __restrict__ sums)
+                                                        int p)
 {
     __shared__ uint32_t prefix[wf_classes * wf_stripes + 1];
     load_prefix(prefix, pool.prefix + wf_prefix_class, wf_classes * wf_stripes + 1);
@@ -449,13 +504,13 @@ __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> 
     {
         return;
     }
-    bool keep = false;
+    bool done = false;
     uint32_t s = 0;
     uint32_t shadow_rays = 0;
+    vec3<float> sample{0.0f, 0.0f, 0.0f};
     if (g < total)
     {
-        const int list = find_segment(prefix, wf_classes * wf_stripes, g);
-        s = pool.queue[static_cast<size_t>(list) * pool.segment + (g - prefix[list])];
+        s = queued_slot(pool, prefix, g);
         slot_path sp = load_path(pool, s);
         const float4 st = pool.qst[s];
         quality_path q;
@@ -466,37 +521,29 @@ __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> 
         const int4 hv = pool.hit[s];
         const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
         tally<false> tl;
-        vec3<float> sample;
-        if (!shade_quality(q, sc, cam, h, sp.g, sample, stack, tl, shadow_rays))
+        done = shade_quality(q, sc, cam, h, sp.g, sample, stack, tl, shadow_rays);
+        if (!done)
         {
             sp.p = q.base;
             sp.specular = q.specular;
             pool.qst[s] = make_float4(q.radiance.x, q.radiance.y, q.radiance.z, q.bsdf_pdf);
             store_path(pool, s, sp);
-            keep = true;
-        }
-        else
-        {
-            keep = finish_sample(cam, sc, seed, work, pool, s, sample, sums);
         }
     }
     block_tally(pool.tallies, 1, shadow_rays);
-    block_append<1>(keep ? 0 : -1, s, pool.counters + wf_counter_stride * (1 - p) + wf_counter_active,
-                    pool.active[1 - p], pool.segment);
+    after_shade(pool, p, s, g < total, done, sample);
 }
 
-// The drain: when few slots are left active, each of them (a thread each) traces its current path and its pixel's
-// remaining passes of the launch to the end, one after the other, as the megakernel does: the bounces with a nearly
-// empty pool would cost a launch each for little work. The sums stay in pass order, so the image is the same bytes.
+// The drain: once every item is handed out and few slots are left active, each of them (a thread each) traces its
+// path to the end, as the megakernel does: the bounces of a nearly empty pool would cost a launch each for little work
 template <int Mode>
 __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, const scene_view<float> sc,
-                                                const uint64_t seed, const launch_work work, wf_pool pool,
-                                                double* __restrict__ sums)
+                                                const uint64_t seed, const launch_work work, wf_pool pool)
 {
-    __shared__ uint32_t prefix[wf_stripes + 1];
-    load_prefix(prefix, pool.prefix + wf_prefix_active, wf_stripes + 1);
+    __shared__ uint32_t prefix[wf_stripes + 2];
+    load_prefix(prefix, pool.prefix + wf_prefix_active, wf_stripes + 2);
     const lds_stack<float> stack = thread_stack(sc.stack_depth);
-    const uint32_t n = prefix[wf_stripes];
+    const uint32_t n = prefix[wf_stripes + 1];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long rays = 0;
     uint32_t shadow_rays = 0;
@@ -504,12 +551,6 @@ __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, con
     {
         const uint32_t s = active_slot(pool, 0, prefix, k);
         slot_path sp = load_path(pool, s);
-        const int i = static_cast<int>(s % static_cast<uint32_t>(cam.image_width));
-        const int j = work.first_row + static_cast<int>(s / static_cast<uint32_t>(cam.image_width));
-        const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(i);
-        double* const sum = sums + 3 * pixel;
-        double red = sum[0], green = sum[1], blue = sum[2];
-        const int end_pass = work.first_pass + work.pass_count;
         tally<false> tl;
         quality_path q;
         if constexpr (Mode == mode_quality)
@@ -520,11 +561,10 @@ __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, con
             q.bsdf_pdf = st.w;
             q.specular = sp.specular;
         }
-        int pass = pool.pass[s];
+        vec3<float> sample;
         while (true)
         {
             ++rays;
-            vec3<float> sample;
             bool done;
             if constexpr (Mode == mode_quality)
             {
@@ -538,92 +578,96 @@ __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, con
             {
                 done = trace_first_book(sp.p, sc, cam, sp.g, sample, stack, tl);
             }
-            if (!done)
-            {
-                continue;
-            }
-            red += without_nan(sample.x);
-            green += without_nan(sample.y);
-            blue += without_nan(sample.z);
-            if (++pass == end_pass)
+            if (done)
             {
                 break;
             }
-            sp.g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
-            if constexpr (Mode == mode_quality)
-            {
-                q = start_quality(cam, sc.motion != 0, i, j, pass, sp.g);
-            }
-            else
-            {
-                sp.p = start_path(cam, sc.motion != 0, i, j, pass, sp.g);
-            }
         }
-        sum[0] = red;
-        sum[1] = green;
-        sum[2] = blue;
+        pool.samples[pool.item[s]] = make_float4(sample.x, sample.y, sample.z, 0.0f);
     }
     block_tally(pool.tallies, 0, rays);
     block_tally(pool.tallies, 1, shadow_rays);
 }
 
-// The launch's ray counts into the session's counters, and the stripe tallies cleared (one block)
-__global__ void __launch_bounds__(64) wf_finish(wf_pool pool, unsigned long long* rays)
+// The launch's end: each pixel's samples added to its sums in pass order (NaN counted as 0, as the megakernel adds
+// them); block 0 also adds the ray counts to the session's counters and clears the tallies
+__global__ void __launch_bounds__(256) wf_finish(const camera_rec<float> cam, const launch_work work, wf_pool pool,
+                                                 double* __restrict__ sums, unsigned long long* rays)
 {
-    const int t = static_cast<int>(threadIdx.x);
-    unsigned long long mine[2] = {0, 0};
-    if (t < wf_stripes)
+    const uint32_t width = static_cast<uint32_t>(cam.image_width);
+    const uint32_t pixels = width * static_cast<uint32_t>(work.row_count);
+    const uint32_t local = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < 64)
     {
-        mine[0] = pool.tallies[2 * t];
-        mine[1] = pool.tallies[2 * t + 1];
-        pool.tallies[2 * t] = 0;
-        pool.tallies[2 * t + 1] = 0;
+        const int t = static_cast<int>(threadIdx.x);
+        unsigned long long mine[2] = {0, 0};
+        if (t < wf_stripes)
+        {
+            mine[0] = pool.tallies[2 * t];
+            mine[1] = pool.tallies[2 * t + 1];
+            pool.tallies[2 * t] = 0;
+            pool.tallies[2 * t + 1] = 0;
+        }
+        const unsigned long long r = wsum(mine[0]);
+        const unsigned long long sh = wsum(mine[1]);
+        if (t == 0)
+        {
+            rays[0] += r;
+            rays[1] += sh;
+        }
     }
-    const unsigned long long r = wsum(mine[0]);
-    const unsigned long long sh = wsum(mine[1]);
-    if (t == 0)
+    if (local >= pixels)
     {
-        rays[0] += r;
-        rays[1] += sh;
+        return;
     }
+    const uint64_t pixel = static_cast<uint64_t>(work.first_row) * width + local;
+    double* const sum = sums + 3 * pixel;
+    double red = sum[0], green = sum[1], blue = sum[2];
+    for (int q = 0; q < work.pass_count; ++q)
+    {
+        const float4 v = pool.samples[static_cast<size_t>(q) * pixels + local];
+        red += without_nan(v.x);
+        green += without_nan(v.y);
+        blue += without_nan(v.z);
+    }
+    sum[0] = red;
+    sum[1] = green;
+    sum[2] = blue;
 }
 }
 
 namespace rt
 {
-hipError_t wf_launch_init(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
-                          const dev::launch_work& work, const dev::wf_pool& pool, hipStream_t stream)
+hipError_t wf_launch_init(const dev::wf_pool& pool, uint32_t items, hipStream_t stream)
 {
-    const uint32_t slots = static_cast<uint32_t>(cam.image_width) * static_cast<uint32_t>(work.row_count);
-    hipLaunchKernelGGL(dev::wf_init, dim3(wf_blocks(slots)), dim3(dev::wf_block), 0, stream, cam, sc, seed, work, pool);
+    hipLaunchKernelGGL(dev::wf_init, dim3(wf_blocks(pool.capacity)), dim3(dev::wf_block), 0, stream, pool, items);
     return hipGetLastError();
 }
 
 hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
-                            const dev::launch_work& work, const dev::wf_pool& pool, int parity, double* sums,
-                            bool quality, hipStream_t stream)
+                            const dev::launch_work& work, const dev::wf_pool& pool, int parity, bool quality,
+                            hipStream_t stream)
 {
     const dim3 grid(wf_blocks(pool.capacity));
     const size_t lds = stack_lds_bytes<float>(sc.stack_depth, dev::wf_block);
     hipLaunchKernelGGL(dev::wf_begin, dim3(1), dim3(256), 0, stream, pool, parity);
+    hipLaunchKernelGGL(dev::wf_refill, grid, dim3(dev::wf_block), 0, stream, cam, sc, seed, work, pool, parity);
     hipLaunchKernelGGL(dev::wf_extend, grid, dim3(dev::wf_block), lds, stream, sc, pool, parity);
     hipLaunchKernelGGL(dev::wf_sort, dim3(1), dim3(64), 0, stream, pool, parity);
     if (quality)
     {
         hipLaunchKernelGGL(dev::wf_shade_quality, grid, dim3(dev::wf_block), lds, stream, cam, sc, seed, work, pool,
-                           parity, sums);
+                           parity);
     }
     else
     {
-        hipLaunchKernelGGL(dev::wf_shade, grid, dim3(dev::wf_block), 0, stream, cam, sc, seed, work, pool, parity,
-                           sums);
+        hipLaunchKernelGGL(dev::wf_shade, grid, dim3(dev::wf_block), 0, stream, cam, sc, seed, work, pool, parity);
     }
     return hipGetLastError();
 }
 
 hipError_t wf_launch_drain(int mode, const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
-                           const dev::launch_work& work, const dev::wf_pool& pool, uint32_t count, double* sums,
-                           hipStream_t stream)
+                           const dev::launch_work& work, const dev::wf_pool& pool, uint32_t count, hipStream_t stream)
 {
     hipLaunchKernelGGL(dev::wf_begin, dim3(1), dim3(256), 0, stream, pool, 0);
     const int threads = 128;
@@ -632,24 +676,27 @@ hipError_t wf_launch_drain(int mode, const dev::camera_rec<float>& cam, const de
     if (mode == dev::mode_quality)
     {
         hipLaunchKernelGGL((dev::wf_drain<dev::mode_quality>), grid, dim3(threads), lds, stream, cam, sc, seed, work,
-                           pool, sums);
+                           pool);
     }
     else if (mode == dev::mode_all_books)
     {
         hipLaunchKernelGGL((dev::wf_drain<dev::mode_all_books>), grid, dim3(threads), lds, stream, cam, sc, seed, work,
-                           pool, sums);
+                           pool);
     }
     else
     {
         hipLaunchKernelGGL((dev::wf_drain<dev::mode_first_book>), grid, dim3(threads), lds, stream, cam, sc, seed,
-                           work, pool, sums);
+                           work, pool);
     }
     return hipGetLastError();
 }
 
-hipError_t wf_launch_finish(const dev::wf_pool& pool, unsigned long long* rays, hipStream_t stream)
+hipError_t wf_launch_finish(const dev::camera_rec<float>& cam, const dev::launch_work& work, const dev::wf_pool& pool,
+                            double* sums, unsigned long long* rays, hipStream_t stream)
 {
-    hipLaunchKernelGGL(dev::wf_finish, dim3(1), dim3(64), 0, stream, pool, rays);
+    const uint32_t pixels = static_cast<uint32_t>(cam.image_width) * static_cast<uint32_t>(work.row_count);
+    hipLaunchKernelGGL(dev::wf_finish, dim3(wf_blocks(pixels)), dim3(dev::wf_block), 0, stream, cam, work, pool, sums,
+                       rays);
     return hipGetLastError();
 }
 

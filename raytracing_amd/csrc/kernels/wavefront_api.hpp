@@ -27,35 +27,50 @@ enum wf_class : int
     wf_classes
 };
 
-// Every list (the active slots, each class's queue) is cut into wf_stripes stripes: block b of a kernel appends to
-// stripe b % wf_stripes, with one device-scope atomic per block and list, so that no counter is the target of more
-// than a few hundred atomics per kernel (the eight XCDs' L2s don't share a line: a device-scope atomic is served
-// beyond them, and atomics on one address queue there). A stripe's items lie in its own segment of the list.
+// Every list (the active slots, each class's queue, the free slots) is cut into wf_stripes stripes: block b of a
+// kernel appends to stripe b % wf_stripes, with one device-scope atomic per block and list, so that no counter is the
+// target of more than a few hundred atomics per kernel (the eight XCDs' L2s don't share a line: a device-scope atomic
+// is served beyond them, and atomics on one address queue there). A stripe's items lie in its own segment of the
+// list. The active list has one more segment, after the stripes: the slots refilled with new work at the bounce's
+// start.
 constexpr int wf_stripes = 32;
 constexpr int wf_block = 256; // threads per block of every pool kernel
 
-// The counters of one parity: the active list's stripes, then each class's stripes (class-major)
+// The counters: per parity, the active list's stripes, then each class's stripes (class-major); after both
+// parities, the free list's stripes and the launch's work state
 constexpr int wf_counter_active = 0;
 constexpr int wf_counter_class = wf_stripes;
 constexpr int wf_counter_stride = wf_stripes * (1 + wf_classes);
-// The prefix sums a bounce reads: the active stripes' (wf_stripes + 1), then the class stripes'
+constexpr int wf_counter_free = 2 * wf_counter_stride;
+constexpr int wf_counter_next = wf_counter_free + wf_stripes; // the next work item to hand out
+constexpr int wf_counter_items = wf_counter_next + 1;         // the launch's work items
+constexpr int wf_counter_refill = wf_counter_next + 2;        // slots refilled at this bounce's start
+constexpr int wf_counter_refill_base = wf_counter_next + 3;   // their first work item
+constexpr int wf_counters = wf_counter_next + 4;
+// The prefix sums a bounce reads: the active list's (stripes and the refill segment: wf_stripes + 2), the queues'
+// (class-major), the free list's
 constexpr int wf_prefix_active = 0;
-constexpr int wf_prefix_class = wf_stripes + 1;
-constexpr int wf_prefix_size = wf_prefix_class + wf_classes * wf_stripes + 1;
+constexpr int wf_prefix_class = wf_stripes + 2;
+constexpr int wf_prefix_free = wf_prefix_class + wf_classes * wf_stripes + 1;
+constexpr int wf_prefix_size = wf_prefix_free + wf_stripes + 1;
 
-// The slots' state, structure of arrays, 16-byte records
+// The slots' state, structure of arrays, 16-byte records. A slot renders work items: item k of a launch is pass
+// first_pass + k / pixels of the launch's pixel k % pixels (pixels in row order from its first row); its sample goes
+// to samples[k], and the launch ends by adding each pixel's samples to its sums in pass order.
 struct wf_pool
 {
-    float4* ray_o;  // origin, time
-    float4* ray_d;  // direction, the primitive the ray leaves (int bits)
-    float4* thr;    // throughput, rays left | inward << 30 | specular << 31 (int bits)
-    float4* qst;    // quality integrator only (else nullptr): the light gathered so far, the last bsdf density
-    uint4* rng;     // pcg32 state and increment
-    int4* hit;      // the closest event: primitive id, t (bits), entering
-    int32_t* pass;  // the pass the slot's path belongs to
-    uint32_t* active[2]; // wf_stripes segments of `segment` slots
+    float4* ray_o;   // origin, time
+    float4* ray_d;   // direction, the primitive the ray leaves (int bits)
+    float4* thr;     // throughput, rays left | inward << 30 | specular << 31 (int bits)
+    float4* qst;     // quality integrator only (else nullptr): the light gathered so far, the last bsdf density
+    uint4* rng;      // pcg32 state and increment
+    int4* hit;       // the closest event: primitive id, t (bits), entering
+    uint32_t* item;  // the work item the slot's path renders
+    float4* samples; // per work item of the launch: its path's sample
+    uint32_t* active[2]; // wf_stripes segments of `segment` slots, then the refill segment (`capacity` slots)
+    uint32_t* free;      // wf_stripes segments of `segment` slots
     uint32_t* queue;     // wf_classes x wf_stripes segments of `segment` slots
-    uint32_t* counters;  // 2 parities x wf_counter_stride
+    uint32_t* counters;  // wf_counters
     uint32_t* prefix;    // wf_prefix_size
     unsigned long long* tallies; // wf_stripes x 2: rays and shadow rays, added to the session's counters per launch
     uint32_t capacity;
@@ -68,19 +83,21 @@ namespace rt
 // Blocks of a pool kernel over `capacity` slots
 inline uint32_t wf_blocks(uint32_t capacity) { return (capacity + dev::wf_block - 1) / dev::wf_block; }
 
-hipError_t wf_launch_init(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
-                          const dev::launch_work& work, const dev::wf_pool& pool, hipStream_t stream);
-// One bounce of parity `parity`: the active list's prefix sums, extend, the queues' prefix sums, shade
+// A launch's start: `items` work items (passes x pixels of work), every slot free, no slot active
+hipError_t wf_launch_init(const dev::wf_pool& pool, uint32_t items, hipStream_t stream);
+// One bounce of parity `parity`: plan (refills, the active list's prefix sums), refill, extend, the queues' prefix
+// sums, shade
 hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
-                            const dev::launch_work& work, const dev::wf_pool& pool, int parity, double* sums,
-                            bool quality, hipStream_t stream);
-// The drain: the slots of the parity-0 active list finish their paths and their pixels' remaining passes, a thread
-// each (mode: the scene's kernel_mode; count: at least the list's length)
+                            const dev::launch_work& work, const dev::wf_pool& pool, int parity, bool quality,
+                            hipStream_t stream);
+// The drain: the slots of the parity-0 active list finish their paths, a thread each (mode: the scene's kernel_mode;
+// count: at least the list's length). Only when every work item has been handed out.
 hipError_t wf_launch_drain(int mode, const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
-                           const dev::launch_work& work, const dev::wf_pool& pool, uint32_t count, double* sums,
-                           hipStream_t stream);
-// Adds the launch's ray counts to the session's counters (rays: [0] rays, [1] shadow rays) and clears them
-hipError_t wf_launch_finish(const dev::wf_pool& pool, unsigned long long* rays, hipStream_t stream);
+                           const dev::launch_work& work, const dev::wf_pool& pool, uint32_t count, hipStream_t stream);
+// The launch's end: each pixel's samples added to its sums in pass order, and the ray counts to the session's
+// counters (rays: [0] rays, [1] shadow rays)
+hipError_t wf_launch_finish(const dev::camera_rec<float>& cam, const dev::launch_work& work, const dev::wf_pool& pool,
+                            double* sums, unsigned long long* rays, hipStream_t stream);
 // which: 0 extend, 1 shade, 2 the quality integrator's shade
 kernel_attributes wf_attributes(int depth, int which);
 }

@@ -64,7 +64,7 @@ wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_re
         // whole rows: a launch's band of rows must fit the pool
         capacity_ = static_cast<uint32_t>(std::min(want, pixels) / cam.image_width * cam.image_width);
         drain_below_ = options.wavefront_drain > 0    ? static_cast<uint32_t>(options.wavefront_drain)
-                       : options.wavefront_drain == 0 ? capacity_ / 4
+                       : options.wavefront_drain == 0 ? capacity_ / 16
                                                       : 0;
         ray_o_ = device_array<float4>(capacity_);
         ray_d_ = device_array<float4>(capacity_);
@@ -75,36 +75,42 @@ wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_re
         }
         rng_ = device_array<uint4>(capacity_);
         hit_ = device_array<int4>(capacity_);
-        pass_ = device_array<int32_t>(capacity_);
-        // each list is wf_stripes segments, each as long as the items a stripe's blocks can append
+        item_ = device_array<uint32_t>(capacity_);
+        // a launch renders at most max_passes_ passes at a time: their samples wait in a buffer of at most 1 GiB
+        max_passes_ = static_cast<int>(std::clamp<uint64_t>((1ull << 30) / (16ull * capacity_), 1, 64));
+        samples_ = device_array<float4>(static_cast<std::size_t>(max_passes_) * capacity_);
+        // each list is wf_stripes segments, each as long as the items a stripe's blocks can append; the active list
+        // has the refill segment after them
         const uint32_t blocks = wf_blocks(capacity_);
         const uint32_t segment = (blocks + dev::wf_stripes - 1) / dev::wf_stripes * dev::wf_block;
         const std::size_t list = static_cast<std::size_t>(segment) * dev::wf_stripes;
-        active0_ = device_array<uint32_t>(list);
-        active1_ = device_array<uint32_t>(list);
+        active0_ = device_array<uint32_t>(list + capacity_);
+        active1_ = device_array<uint32_t>(list + capacity_);
+        free_ = device_array<uint32_t>(list);
         queue_ = device_array<uint32_t>(list * dev::wf_classes);
-        counters_ = device_array<uint32_t>(2 * dev::wf_counter_stride);
+        counters_ = device_array<uint32_t>(dev::wf_counters);
         prefix_ = device_array<uint32_t>(dev::wf_prefix_size);
         tallies_ = device_array<unsigned long long>(2 * dev::wf_stripes);
         check(hipMemsetAsync(counters_.data(), 0, counters_.bytes(), stream_), "clearing counters");
         check(hipMemsetAsync(tallies_.data(), 0, tallies_.bytes(), stream_), "clearing counters");
-        check(hipHostMalloc(reinterpret_cast<void**>(&host_count_), dev::wf_stripes * sizeof(uint32_t),
+        check(hipHostMalloc(reinterpret_cast<void**>(&host_counters_), dev::wf_counters * sizeof(uint32_t),
                             hipHostMallocDefault),
               "allocating pinned memory");
         pool_ = {ray_o_.data(), ray_d_.data(), thr_.data(), qst_.size() > 0 ? qst_.data() : nullptr, rng_.data(),
-                 hit_.data(), pass_.data(), {active0_.data(), active1_.data()}, queue_.data(), counters_.data(),
-                 prefix_.data(), tallies_.data(), capacity_, segment};
-        bytes_ = ray_o_.bytes() * 3 + qst_.bytes() + rng_.bytes() + hit_.bytes() + pass_.bytes() +
-                 active0_.bytes() * 2 + queue_.bytes() + counters_.bytes() + prefix_.bytes() + tallies_.bytes();
+                 hit_.data(), item_.data(), samples_.data(), {active0_.data(), active1_.data()}, free_.data(),
+                 queue_.data(), counters_.data(), prefix_.data(), tallies_.data(), capacity_, segment};
+        bytes_ = ray_o_.bytes() * 3 + qst_.bytes() + rng_.bytes() + hit_.bytes() + item_.bytes() + samples_.bytes() +
+                 active0_.bytes() * 2 + free_.bytes() + queue_.bytes() + counters_.bytes() + prefix_.bytes() +
+                 tallies_.bytes();
     }
 }
 
 template <typename Real>
 wavefront<Real>::~wavefront()
 {
-    if (host_count_)
+    if (host_counters_)
     {
-        (void)hipHostFree(host_count_);
+        (void)hipHostFree(host_counters_);
     }
 }
 
@@ -117,75 +123,94 @@ int64_t wavefront<Real>::render(uint64_t seed, const dev::launch_work& work, dou
     }
     else
     {
-        if (static_cast<uint64_t>(work.row_count) * static_cast<uint64_t>(cam_.image_width) > capacity_)
+        const uint64_t pixels = static_cast<uint64_t>(work.row_count) * static_cast<uint64_t>(cam_.image_width);
+        if (pixels > capacity_)
         {
             throw std::invalid_argument("the launch's rows don't fit the wavefront pool");
         }
-        check(wf_launch_init(cam_, sc_, seed, work, pool_, stream_), "launching the wavefront's first paths");
-        // the bounce loop: an even number of bounces (parity 0, 1, 0, 1...) at a time, captured once in a graph (or
-        // launched one by one), repeated until a round ends with no active slot (the count of the next parity-0 list)
-        const auto bounces = [&] {
-            hipError_t launched = hipSuccess;
-            for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
-            {
-                launched = wf_launch_bounce(cam_, sc_, seed, work, pool_, b & 1, sums, quality_, stream_);
-            }
-            return launched;
-        };
-        graph_owner g;
-        if (use_graph_)
-        {
-            check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capturing the bounce loop");
-            const hipError_t launched = bounces();
-            const hipError_t ended = hipStreamEndCapture(stream_, &g.graph);
-            check(launched, "launching the bounce loop's kernels");
-            check(ended, "capturing the bounce loop");
-            check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "instantiating the bounce loop");
-        }
-        // a path is at most max_depth rays, and a slot traces pass_count paths: that bounds the rounds
-        const long long max_bounces = static_cast<long long>(cam_.max_depth) * work.pass_count + bounces_per_graph_;
         int64_t done = 0;
-        while (true)
+        // at most max_passes_ passes at a time: their samples fit the buffer
+        for (int first = 0; first < work.pass_count; first += max_passes_)
         {
-            if (done > max_bounces)
-            {
-                throw std::runtime_error("the wavefront loop didn't drain");
-            }
-            if (use_graph_)
-            {
-                check(hipGraphLaunch(g.exec, stream_), "launching the bounce loop");
-            }
-            else
-            {
-                check(bounces(), "launching the bounce loop's kernels");
-            }
-            done += bounces_per_graph_;
-            // the parity-0 active list's stripes: the next round's slots
-            check(hipMemcpyAsync(host_count_, counters_.data() + dev::wf_counter_active,
-                                 dev::wf_stripes * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
-                  "reading the active count");
-            check(hipStreamSynchronize(stream_), "rendering on the GPU");
-            uint64_t active = 0;
-            for (int k = 0; k < dev::wf_stripes; ++k)
-            {
-                active += host_count_[k];
-            }
-            if (active == 0)
-            {
-                break;
-            }
-            if (active < drain_below_)
-            {
-                check(wf_launch_drain(mode_, cam_, sc_, seed, work, pool_, static_cast<uint32_t>(active), sums,
-                                      stream_),
-                      "launching the wavefront's drain");
-                break;
-            }
+            dev::launch_work part = work;
+            part.first_pass = work.first_pass + first;
+            part.pass_count = std::min(max_passes_, work.pass_count - first);
+            done += render_part(seed, part, sums, rays);
         }
-        check(wf_launch_finish(pool_, rays, stream_), "counting the launch's rays");
-        check(hipStreamSynchronize(stream_), "rendering on the GPU");
         return done;
     }
+}
+
+template <typename Real>
+int64_t wavefront<Real>::render_part(uint64_t seed, const dev::launch_work& work, double* sums,
+                                     unsigned long long* rays)
+{
+    const uint32_t items =
+        static_cast<uint32_t>(work.pass_count) * static_cast<uint32_t>(work.row_count) * static_cast<uint32_t>(cam_.image_width);
+    check(wf_launch_init(pool_, items, stream_), "starting the wavefront's launch");
+    // the bounce loop: an even number of bounces (parity 0, 1, 0, 1...) at a time, captured once in a graph (or
+    // launched one by one), repeated until the items are all handed out and no slot is active
+    const auto bounces = [&] {
+        hipError_t launched = hipSuccess;
+        for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
+        {
+            launched = wf_launch_bounce(cam_, sc_, seed, work, pool_, b & 1, quality_, stream_);
+        }
+        return launched;
+    };
+    graph_owner g;
+    if (use_graph_)
+    {
+        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capturing the bounce loop");
+        const hipError_t launched = bounces();
+        const hipError_t ended = hipStreamEndCapture(stream_, &g.graph);
+        check(launched, "launching the bounce loop's kernels");
+        check(ended, "capturing the bounce loop");
+        check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "instantiating the bounce loop");
+    }
+    // every bounce either advances a path or hands out items to every free slot: a safe bound on the rounds
+    const long long max_bounces =
+        (static_cast<long long>(items) / capacity_ + 2) * static_cast<long long>(cam_.max_depth + 1) * 2 + 64;
+    int64_t done = 0;
+    while (true)
+    {
+        if (done > max_bounces)
+        {
+            throw std::runtime_error("the wavefront loop didn't drain");
+        }
+        if (use_graph_)
+        {
+            check(hipGraphLaunch(g.exec, stream_), "launching the bounce loop");
+        }
+        else
+        {
+            check(bounces(), "launching the bounce loop's kernels");
+        }
+        done += bounces_per_graph_;
+        check(hipMemcpyAsync(host_counters_, counters_.data(), dev::wf_counters * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, stream_),
+              "reading the wavefront's counters");
+        check(hipStreamSynchronize(stream_), "rendering on the GPU");
+        uint64_t active = 0; // the parity-0 active list's stripes: the paths going on into the next round
+        for (int k = 0; k < dev::wf_stripes; ++k)
+        {
+            active += host_counters_[dev::wf_counter_active + k];
+        }
+        const bool handed_out = host_counters_[dev::wf_counter_next] == host_counters_[dev::wf_counter_items];
+        if (handed_out && active == 0)
+        {
+            break;
+        }
+        if (handed_out && active < drain_below_)
+        {
+            check(wf_launch_drain(mode_, cam_, sc_, seed, work, pool_, static_cast<uint32_t>(active), stream_),
+                  "launching the wavefront's drain");
+            break;
+        }
+    }
+    check(wf_launch_finish(cam_, work, pool_, sums, rays, stream_), "adding the launch's samples");
+    check(hipStreamSynchronize(stream_), "rendering on the GPU");
+    return done;
 }
 
 template <typename Real>

@@ -53,10 +53,13 @@ private:
     device_array<float4> ray_o_, ray_d_, thr_, qst_;
     device_array<uint4> rng_;
     device_array<int4> hit_;
-    device_array<int32_t> pass_;
-    device_array<uint32_t> active0_, active1_, queue_, counters_, prefix_;
+    device_array<uint32_t> item_;
+    device_array<float4> samples_;
+    device_array<uint32_t> active0_, active1_, free_, queue_, counters_, prefix_;
     device_array<unsigned long long> tallies_;
-    uint32_t* host_count_ = nullptr; // pinned: the active list's stripe lengths after each round
+    uint32_t* host_counters_ = nullptr; // pinned: the counters after each round
+    int max_passes_ = 1;                // passes one launch of the pool renders
+    int64_t render_part(uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays);
     dev::wf_pool pool_{};
     int bounces_per_graph_ = 8;
     bool use_graph_ = true;
