@@ -18,7 +18,8 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY_PATH = os.path.join(_HERE, "librt_amd.so")
+# RT_AMD_LIBRARY names another build of the same library (a variant built for a measurement); it must exist too
+LIBRARY_PATH = os.environ.get("RT_AMD_LIBRARY") or os.path.join(_HERE, "librt_amd.so")
 
 if not os.path.exists(LIBRARY_PATH):
     raise ImportError(

@@ -427,8 +427,121 @@ struct lds_stack
     }
 };
 
+// The primitives of a leaf against the ray, the closest hit so far updated (src/gpu/bvh.cuh:612-680)
+template <typename Real, bool AllKinds, typename Tally>
+__device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Real>& r, int32_t ref, int from,
+                                       bool inward, Real inverse_a, Real t_min, hit<Real>& closest, Tally& tl)
+{
+    const int kind = leaf_kind(ref);
+    const int first = leaf_first(ref);
+    const int end = first + leaf_count(ref);
+    if (AllKinds && kind == prim_triangle)
+    {
+        for (int i = first; i < end; ++i)
+        {
+            const int id = sc.triangle_base + i;
+            if (id == from)
+            {
+                continue; // a ray never meets the flat face it leaves
+            }
+            const triangle_rec<Real> tri = sc.triangles[i];
+            tl.add(tally_triangle);
+            Real t;
+            bool front;
+            if (hit_triangle(tri, r, t_min, closest.t, t, front))
+            {
+                closest = {id, t, front};
+            }
+        }
+    }
+    else if (AllKinds && kind == prim_quad)
+    {
+        for (int i = first; i < end; ++i)
+        {
+            const int id = sc.quad_base + i;
+            if (id == from)
+            {
+                continue;
+            }
+            const quad_rec<Real> face = sc.quads[i];
+            tl.add(tally_quad);
+            Real t;
+            bool front;
+            if (hit_quad(face, r, t_min, closest.t, t, front))
+            {
+                closest = {id, t, front};
+            }
+        }
+    }
+    else
+    {
+        const bool moving = AllKinds && kind == prim_moving_sphere;
+        for (int i = first; i < end; ++i)
+        {
+            const int id = moving ? sc.moving_base + i : i;
+            const bool own = id == from;
+            if (own && !inward)
+            {
+                continue;
+            }
+            const sphere<Real> s = moving ? moving_at(sc.moving_spheres[i], r.time) : load_sphere(sc.spheres, i);
+            tl.add(moving ? tally_moving : tally_sphere);
+            Real t;
+            bool entering;
+            if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering))
+            {
+                closest = {id, t, entering};
+            }
+        }
+    }
+}
+
+// An inner node: both children's boxes tested; the farther one entered goes on the stack, and the nearer one is the
+// next reference (true), or neither is entered (false). The nearer of equal entries is the first child.
+template <typename Real, typename Tally>
+__device__ inline bool descend(const scene_view<Real>& sc, int32_t& ref, vec3<Real> inv, vec3<Real> os, Real t_min,
+                               Real t_max, const lds_stack<Real>& stack, int& size, Tally& tl)
+{
+    const wide_node<Real> node = sc.nodes[ref];
+    tl.add(tally_node);
+    const Real t0 = box_entry(node.lo0, node.hi0, inv, os, t_min, t_max);
+    const Real t1 = box_entry(node.lo1, node.hi1, inv, os, t_min, t_max);
+    const bool second_nearer = t1 < t0;
+    const Real t_far = m_max(t0, t1);
+    if (t_far < inf<Real>)
+    {
+        stack.store(size++, second_nearer ? node.ref0 : node.ref1, t_far);
+    }
+    if (m_min(t0, t1) < inf<Real>)
+    {
+        ref = second_nearer ? node.ref1 : node.ref0;
+        return true;
+    }
+    return false;
+}
+
+// The next stack entry the ray enters before the closest hit so far, or false when there is none
+template <typename Real>
+__device__ inline bool pop(const lds_stack<Real>& stack, int& size, Real t_max, int32_t& ref)
+{
+    while (size > 0)
+    {
+        --size;
+        const Real t = stack.ts[size * stack.stride];
+        if (t < t_max)
+        {
+            ref = stack.refs[size * stack.stride];
+            return true;
+        }
+    }
+    return false;
+}
+
 // The closest primitive a ray hits at t in (0.001, ∞), with the self-intersection rule for the primitive it leaves
-// (src/gpu/bvh.cuh:576-706). AllKinds false: a tree of spheres that stand still only (the first book's scenes).
+// (src/gpu/bvh.cuh:576-706). AllKinds false: a tree of spheres that stand still only (the first book's scenes). Nodes
+// are visited near first and leaves tested in the reference's order; the loop either takes one node or one leaf per
+// iteration, or (RT_TRAVERSAL_WHILE_WHILE) walks down inner nodes in an inner loop until every lane of the wave has a
+// leaf, then tests the leaves (Aila and Laine 2009): the same decisions in the same order per ray either way.
 template <typename Real, bool AllKinds, typename Tally>
 __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
                                         const lds_stack<Real>& stack, Tally& tl)
@@ -444,109 +557,39 @@ __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Re
     const vec3<Real> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
     int size = 0;
     int32_t ref = sc.root_ref;
+#if RT_TRAVERSAL_WHILE_WHILE
     while (true)
     {
-        if (ref < 0)
+        while (ref >= 0)
         {
-            const int kind = leaf_kind(ref);
-            const int first = leaf_first(ref);
-            const int end = first + leaf_count(ref);
-            if (AllKinds && kind == prim_triangle)
+            if (!descend(sc, ref, inv, os, t_min, closest.t, stack, size, tl) && !pop(stack, size, closest.t, ref))
             {
-                for (int i = first; i < end; ++i)
-                {
-                    const int id = sc.triangle_base + i;
-                    if (id == from)
-                    {
-                        continue; // a ray never meets the flat face it leaves
-                    }
-                    const triangle_rec<Real> tri = sc.triangles[i];
-                    tl.add(tally_triangle);
-                    Real t;
-                    bool front;
-                    if (hit_triangle(tri, r, t_min, closest.t, t, front))
-                    {
-                        closest = {id, t, front};
-                    }
-                }
-            }
-            else if (AllKinds && kind == prim_quad)
-            {
-                for (int i = first; i < end; ++i)
-                {
-                    const int id = sc.quad_base + i;
-                    if (id == from)
-                    {
-                        continue;
-                    }
-                    const quad_rec<Real> face = sc.quads[i];
-                    tl.add(tally_quad);
-                    Real t;
-                    bool front;
-                    if (hit_quad(face, r, t_min, closest.t, t, front))
-                    {
-                        closest = {id, t, front};
-                    }
-                }
-            }
-            else
-            {
-                const bool moving = AllKinds && kind == prim_moving_sphere;
-                for (int i = first; i < end; ++i)
-                {
-                    const int id = moving ? sc.moving_base + i : i;
-                    const bool own = id == from;
-                    if (own && !inward)
-                    {
-                        continue;
-                    }
-                    const sphere<Real> s = moving ? moving_at(sc.moving_spheres[i], r.time) : load_sphere(sc.spheres, i);
-                    tl.add(moving ? tally_moving : tally_sphere);
-                    Real t;
-                    bool entering;
-                    if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering))
-                    {
-                        closest = {id, t, entering};
-                    }
-                }
+                return closest;
             }
         }
-        else
-        {
-            const wide_node<Real> node = sc.nodes[ref];
-            tl.add(tally_node);
-            const Real t0 = box_entry(node.lo0, node.hi0, inv, os, t_min, closest.t);
-            const Real t1 = box_entry(node.lo1, node.hi1, inv, os, t_min, closest.t);
-            const bool second_nearer = t1 < t0;
-            const Real t_far = m_max(t0, t1);
-            if (t_far < inf<Real>)
-            {
-                stack.store(size++, second_nearer ? node.ref0 : node.ref1, t_far);
-            }
-            if (m_min(t0, t1) < inf<Real>)
-            {
-                ref = second_nearer ? node.ref1 : node.ref0;
-                continue;
-            }
-        }
-        // nowhere to descend: the next entry the ray enters before the closest hit so far
-        bool found = false;
-        while (size > 0)
-        {
-            --size;
-            const Real t = stack.ts[size * stack.stride];
-            if (t < closest.t)
-            {
-                ref = stack.refs[size * stack.stride];
-                found = true;
-                break;
-            }
-        }
-        if (!found)
+        closest_in_leaf<Real, AllKinds>(sc, r, ref, from, inward, inverse_a, t_min, closest, tl);
+        if (!pop(stack, size, closest.t, ref))
         {
             return closest;
         }
     }
+#else
+    while (true)
+    {
+        if (ref < 0)
+        {
+            closest_in_leaf<Real, AllKinds>(sc, r, ref, from, inward, inverse_a, t_min, closest, tl);
+        }
+        else if (descend(sc, ref, inv, os, t_min, closest.t, stack, size, tl))
+        {
+            continue;
+        }
+        if (!pop(stack, size, closest.t, ref))
+        {
+            return closest;
+        }
+    }
+#endif
 }
 
 // --- sphere surface helpers (src/gpu/sphere.cuh:125-168) ---------------------------------------------------------------

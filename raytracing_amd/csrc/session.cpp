@@ -142,6 +142,17 @@ public:
             }
             candidates_.push_back({true, 0, 256});
             current_ = candidates_.front();
+            // each candidate renders two passes, in a forward then a backward sweep, so that a drift in the timings
+            // (clocks, caches warming) weighs on all of them alike
+            for (std::size_t k = 0; k < candidates_.size(); ++k)
+            {
+                schedule_.push_back(k);
+            }
+            for (std::size_t k = candidates_.size(); k-- > 0;)
+            {
+                schedule_.push_back(k);
+            }
+            tune_seconds_.assign(candidates_.size(), 0.0);
         }
         rays_ = device_array<unsigned long long>(2);
         validate_camera(camera);
@@ -201,7 +212,7 @@ public:
             }
             const bool whole = work.first_row == 0 && work.row_count == height;
             const bool probe = whole && tuning();
-            const variant v = probe ? candidates_[tuned_] : current_;
+            const variant v = probe ? candidates_[schedule_[tuned_]] : current_;
             const double seconds = launch(options_.seed + static_cast<uint64_t>(round) * round_seed_step, work, v);
             report.launches += 1;
             report.bounces += last_bounces_;
@@ -213,14 +224,15 @@ public:
                 state_.passes += work.pass_count;
                 if (probe)
                 {
-                    // a candidate's pass timed: once all are, the fastest renders the rest
-                    tune_seconds_.push_back(seconds);
-                    if (++tuned_ == candidates_.size())
+                    // a candidate's pass timed: once the schedule is done, the fastest renders the rest
+                    tune_seconds_[schedule_[tuned_]] += seconds;
+                    if (++tuned_ == schedule_.size())
                     {
                         const auto best = static_cast<std::size_t>(
                             std::min_element(tune_seconds_.begin(), tune_seconds_.end()) - tune_seconds_.begin());
                         current_ = candidates_[best];
-                        seconds_per_pass_ = tune_seconds_[best];
+                        seconds_per_pass_ = tune_seconds_[best] * static_cast<double>(candidates_.size()) /
+                                            static_cast<double>(schedule_.size());
                     }
                 }
             }
@@ -470,10 +482,11 @@ private:
         int block = 128;
     };
     variant current_{};
-    std::vector<variant> candidates_; // the autotuner's, each timed on one full pass
-    std::size_t tuned_ = 0;           // candidates timed so far
-    std::vector<double> tune_seconds_;
-    bool tuning() const { return tuned_ < candidates_.size(); }
+    std::vector<variant> candidates_;   // the autotuner's
+    std::vector<std::size_t> schedule_; // which candidate renders each of the first full passes
+    std::size_t tuned_ = 0;             // passes of the schedule rendered so far
+    std::vector<double> tune_seconds_;  // per candidate, its passes' seconds
+    bool tuning() const { return tuned_ < schedule_.size(); }
     std::unique_ptr<wavefront<Real>> wavefront_; // made at the first launch that needs it in a view
     double launch_seconds_ = 0.05;
     rt_camera_desc camera_{};
