@@ -106,7 +106,8 @@ def serving(rt, rates, per_rate, seed, tau):
     scenes = {}
     for name in SERVING_SCENES:
         for w, h in SERVING_RESOLUTIONS:
-            views[(name, w, h)] = rt.build_scene(name, width=w, height=h, samples=1).desc.camera
+            built = rt.build_scene(name, width=w, height=h, samples=1)
+            views[(name, w, h)] = rt.CameraDesc.from_buffer_copy(built.desc.camera)  # a copy: the scene is freed
     sigma = {}
     for name in SERVING_SCENES:
         pilot = rt.build_scene(name, width=128, samples=1)
@@ -226,6 +227,8 @@ def main():
     ap.add_argument("--wavefront-graph", type=int, default=0, help="bounces per hipGraph (-1: none, e.g. under a profiler)")
     ap.add_argument("--wavefront-drain", type=int, default=0, help="active slots below which the pool drains")
     ap.add_argument("--wavefront-slots", type=int, default=0, help="the wavefront pool's size")
+    ap.add_argument("--tree", default="default", choices=["default", "median", "sah"],
+                    help="the BVH's splits: the library's default, the reference's object median, or the SAH")
     ap.add_argument("--serving", action="store_true", help="config 4: the serving loop under Poisson request streams")
     ap.add_argument("--rates", default="1,4,16,64")
     ap.add_argument("--requests", type=int, default=16, help="requests per rate (serving)")
@@ -275,6 +278,7 @@ def main():
           f"({device.compute_units} CUs)", file=sys.stderr, flush=True)
     tuning = {k: v for k, v in (("wavefront_graph", args.wavefront_graph), ("wavefront_drain", args.wavefront_drain),
                                 ("wavefront_slots", args.wavefront_slots)) if v}
+    tuning["tree"] = args.tree
     value, ms, avg_launch, launches_per_step, session = measure(rt, scene, args.integrator, args.steps, args.warmup,
                                                                 **tuning)
     kernel = session.kernel().name.decode()
@@ -293,7 +297,7 @@ def main():
 
     mega = None
     if args.integrator != "megakernel" and not args.no_megakernel:
-        mv, _, _, _, ms_ = measure(rt, scene, "megakernel", 1, 0)
+        mv, _, _, _, ms_ = measure(rt, scene, "megakernel", 1, 0, tree=args.tree)
         ms_.close()
         mega = round(mv, 2)
         print(f"megakernel: {mv:.1f} Mpaths/s", file=sys.stderr, flush=True)
@@ -317,7 +321,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": description, "integrator": args.integrator, "kernel": kernel,
+        "config": {"workload": description, "integrator": args.integrator, "tree": args.tree, "kernel": kernel,
                    "paths_per_step": scene.width * scene.height * scene.passes,
                    "rays_per_path": round(traffic.rays / traffic.paths, 3),
                    "shadow_rays_per_path": round(traffic.shadow_rays / traffic.paths, 3),

@@ -329,11 +329,21 @@ typedef struct rt_session_options
     int32_t wavefront_graph;/* wavefront bounces per captured hipGraph: 0 for the tuned default, -1 for none (every
                                kernel launched on its own, as profilers that don't follow graphs need) */
     int32_t wavefront_drain;/* wavefront: when fewer slots than this are active, each finishes its pixel's passes on
-                               its own (0: the tuned default, a quarter of the pool; -1: never) */
+                               its own (0: the tuned default, a sixteenth of the pool; -1: never) */
     int32_t megakernel_waves;/* megakernel: the waves per SIMD its registers are budgeted for: 0 (the compiler's
                                choice), 5 or 6 (float only) */
-    int32_t reserved;
+    int32_t tree;           /* rt_tree: how the session's BVH is split */
 } rt_session_options;
+
+/* The BVH's splits. Both trees hold every primitive and find the same closest hit; they differ in how many boxes a
+ * ray tests on the way (and which of two primitives hit at exactly the same distance is kept). */
+enum rt_tree
+{
+    RT_TREE_DEFAULT = 0, /* the surface-area heuristic */
+    RT_TREE_MEDIAN = 1,  /* the reference's object-median splits on the longest axis, with its leaf sizes
+                            (src/gpu/bvh.cuh:382-444); scenes with triangles take the surface-area tree */
+    RT_TREE_SAH = 2,     /* binned surface-area-heuristic splits, leaves of one kind and at most 8 primitives */
+};
 /* RT_INTEGRATOR_AUTO (float32): the session's autotuner renders its first full passes with each candidate (the
  * megakernel at each register budget and block size, and the wavefront), one pass each, and keeps the fastest for the
  * rest. Every candidate traces the same paths with the same random numbers and sums them in the same order, so the
@@ -410,10 +420,11 @@ int rt_session_closest_hit(rt_session* session, const float* origins, const floa
                            const int32_t* from, const uint8_t* inward, int64_t count, int32_t* index, float* t,
                            uint8_t* entering);
 
-/* The flat BVH this library builds for a scene in float32, on the host (no GPU needed), in the reference's depth-first
- * node order (src/gpu/bvh.cuh:29-38,382-435): per node box_min[3], box_max[3] as floats, and index, count as ints;
- * and the global ids of the scene's spheres and quads. Call with NULL buffers to learn the counts. */
-int rt_scene_bvh(const rt_scene* scene, float* node_boxes, int32_t* node_ints, int64_t* node_count,
+/* The flat BVH this library builds for a scene in float32 with the splits of `tree` (an rt_tree), on the host (no GPU
+ * needed), in the reference's depth-first node layout (src/gpu/bvh.cuh:29-38,382-435): per node box_min[3], box_max[3]
+ * as floats, and index, count as ints; and the global ids of the scene's spheres and quads. Call with NULL buffers to
+ * learn the counts. */
+int rt_scene_bvh(const rt_scene* scene, int32_t tree, float* node_boxes, int32_t* node_ints, int64_t* node_count,
                  int32_t* sphere_ids, int32_t* quad_ids, int32_t* depth);
 
 /* The quality integrator's light set as the kernels read it, on the host (build_lamps, src/gpu/lamps.cuh:159 and

@@ -128,7 +128,7 @@ class SessionOptions(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("precision", C.c_int32), ("integrator", C.c_int32),
                 ("launch_seconds", C.c_double), ("block_size", C.c_int32), ("wavefront_slots", C.c_int32),
                 ("wavefront_graph", C.c_int32), ("wavefront_drain", C.c_int32), ("megakernel_waves", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("tree", C.c_int32)]
 
 
 class RenderReport(C.Structure):
@@ -201,7 +201,7 @@ _SIGNATURES = {
     "rt_session_closest_hit": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_int32),
                                          _P(C.c_uint8), C.c_int64, _P(C.c_int32), _P(C.c_float), _P(C.c_uint8)]),
     "rt_session_traffic": (C.c_int, [C.c_void_p, C.c_int64, _P(Traffic)]),
-    "rt_scene_bvh": (C.c_int, [_P(Scene), _P(C.c_float), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32),
+    "rt_scene_bvh": (C.c_int, [_P(Scene), C.c_int32, _P(C.c_float), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32),
                                _P(C.c_int32), _P(C.c_int32)]),
     "rt_scene_lamps": (C.c_int, [_P(Scene), _P(C.c_float), _P(C.c_int64)]),
     "rt_environment_tables": (C.c_int, [_P(EnvironmentDesc), _P(C.c_float), _P(C.c_float), _P(C.c_float),
@@ -228,6 +228,7 @@ for _name, (_res, _args) in _SIGNATURES.items():
 RT_OK, RT_ERROR_RUNTIME, RT_ERROR_INVALID_ARGUMENT, RT_ERROR_LOGIC = 0, 1, 2, 3
 SAMPLING = {"uniform": 0, "material": 1, "lights": 2, "mixture": 3, "nee": 4, "mis": 5}
 INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2}
+TREES = {"default": 0, "median": 1, "sah": 2}
 
 
 def library() -> C.CDLL:
@@ -286,10 +287,11 @@ def build_scene(name: str, **overrides) -> BuiltScene:
 
 
 def _session_options(seed=0, precision="float32", integrator="auto", launch_seconds=0.0, block_size=0,
-                     wavefront_slots=0, wavefront_graph=0, wavefront_drain=0, megakernel_waves=0) -> SessionOptions:
+                     wavefront_slots=0, wavefront_graph=0, wavefront_drain=0, megakernel_waves=0,
+                     tree="default") -> SessionOptions:
     return SessionOptions(seed, 1 if precision in ("float64", "fp64") else 0, INTEGRATORS[integrator],
                           launch_seconds, block_size, wavefront_slots, wavefront_graph, wavefront_drain,
-                          megakernel_waves, 0)
+                          megakernel_waves, TREES[tree] if isinstance(tree, str) else int(tree))
 
 
 @dataclass
@@ -406,16 +408,18 @@ class Session:
         return ids, t, ent
 
 
-def scene_bvh(scene: BuiltScene):
-    """The float32 BVH this library builds for a scene, in the reference's node layout (host only)."""
+def scene_bvh(scene: BuiltScene, tree: str = "median"):
+    """The float32 BVH this library builds for a scene with the splits of `tree` ("default", "median" for the
+    reference's, "sah"), in the reference's node layout (host only)."""
     count = C.c_int64()
     depth = C.c_int32()
-    _check(_lib.rt_scene_bvh(C.byref(scene.desc), None, None, C.byref(count), None, None, C.byref(depth)))
+    kind = TREES[tree] if isinstance(tree, str) else int(tree)
+    _check(_lib.rt_scene_bvh(C.byref(scene.desc), kind, None, None, C.byref(count), None, None, C.byref(depth)))
     boxes = np.zeros((count.value, 6), np.float32)
     ints = np.zeros((count.value, 2), np.int32)
     sphere_ids = np.zeros(scene.desc.sphere_count, np.int32)
     quad_ids = np.zeros(scene.desc.quad_count, np.int32)
-    _check(_lib.rt_scene_bvh(C.byref(scene.desc), boxes.ctypes.data_as(_P(C.c_float)),
+    _check(_lib.rt_scene_bvh(C.byref(scene.desc), kind, boxes.ctypes.data_as(_P(C.c_float)),
                              ints.ctypes.data_as(_P(C.c_int32)), C.byref(count),
                              sphere_ids.ctypes.data_as(_P(C.c_int32)), quad_ids.ctypes.data_as(_P(C.c_int32)),
                              C.byref(depth)))

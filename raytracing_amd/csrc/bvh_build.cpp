@@ -64,7 +64,7 @@ template <typename Real>
 class builder
 {
 public:
-    builder(const rt_scene& scene, int max_leaf) : max_leaf_(max_leaf)
+    builder(const rt_scene& scene, int max_leaf, bool sah) : max_leaf_(max_leaf), sah_(sah)
     {
         const auto n_spheres = static_cast<std::size_t>(scene.sphere_count);
         for (std::size_t i = 0; i < n_spheres; ++i)
@@ -134,7 +134,7 @@ public:
         std::iota(order.begin(), order.end(), 0);
         if (!order.empty())
         {
-            if (triangles_.empty())
+            if (triangles_.empty() && !sah_)
             {
                 add_subtree(order, 1);
             }
@@ -167,6 +167,7 @@ private:
         box_d box;
     };
     int max_leaf_;
+    bool sah_;
     std::vector<input> inputs_;
     std::vector<dev::sphere_rec<Real>> spheres_;
     std::vector<dev::moving_sphere_rec<Real>> moving_;
@@ -633,13 +634,17 @@ std::vector<world_triangle> world_triangles(const rt_scene& scene)
 }
 
 template <typename Real>
-bvh_arrays<Real> build_bvh(const rt_scene& scene, int max_leaf_size)
+bvh_arrays<Real> build_bvh(const rt_scene& scene, int max_leaf_size, int32_t tree)
 {
-    return builder<Real>(scene, max_leaf_size).build();
+    if (tree < RT_TREE_DEFAULT || tree > RT_TREE_SAH)
+    {
+        throw std::invalid_argument("unknown tree");
+    }
+    return builder<Real>(scene, max_leaf_size, tree != RT_TREE_MEDIAN).build();
 }
 
-template bvh_arrays<float> build_bvh<float>(const rt_scene&, int);
-template bvh_arrays<double> build_bvh<double>(const rt_scene&, int);
+template bvh_arrays<float> build_bvh<float>(const rt_scene&, int, int32_t);
+template bvh_arrays<double> build_bvh<double>(const rt_scene&, int, int32_t);
 template dev::quad_rec<float> make_quad_rec<float>(const rt_quad_desc&);
 template dev::quad_rec<double> make_quad_rec<double>(const rt_quad_desc&);
 template float round_down<float>(double);

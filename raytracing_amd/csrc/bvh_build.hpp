@@ -45,12 +45,13 @@ struct bvh_arrays
 // (src/gpu/renderer.cu:75-82)
 int leaf_size_for(std::size_t shapes);
 
-// The tree over every sphere, quad and triangle of a scene. A scene without triangles gets the reference's tree
-// (object-median splits, leaves of max_leaf_size); one with triangles (meshes baked into world coordinates through
-// their instances) gets a surface-area-heuristic tree of small leaves, as the reference's object-median split on
-// millions of triangles would make a tree several times slower to walk.
+// The tree over every sphere, quad and triangle of a scene (tree: an rt_tree). RT_TREE_MEDIAN on a scene without
+// triangles gives the reference's tree (object-median splits, leaves of max_leaf_size); otherwise, and for every scene
+// with triangles (meshes baked into world coordinates through their instances), a surface-area-heuristic tree of
+// small leaves: the reference's object-median split on millions of triangles would make a tree several times slower
+// to walk.
 template <typename Real>
-bvh_arrays<Real> build_bvh(const rt_scene& scene, int max_leaf_size);
+bvh_arrays<Real> build_bvh(const rt_scene& scene, int max_leaf_size, int32_t tree);
 
 // The world triangles of a scene's meshes, through their instances (or each mesh once, as it is, when the scene has
 // no instances), with their materials: what build_bvh puts in the tree, in that order before it sorts them

@@ -282,7 +282,7 @@ int rt_session_traffic(rt_session* session, int64_t passes, rt_traffic* traffic)
     });
 }
 
-int rt_scene_bvh(const rt_scene* scene, float* node_boxes, int32_t* node_ints, int64_t* node_count,
+int rt_scene_bvh(const rt_scene* scene, int32_t kind, float* node_boxes, int32_t* node_ints, int64_t* node_count,
                  int32_t* sphere_ids, int32_t* quad_ids, int32_t* depth)
 {
     return guarded([&] {
@@ -290,7 +290,7 @@ int rt_scene_bvh(const rt_scene* scene, float* node_boxes, int32_t* node_ints, i
         need(node_count, "node_count");
         rt::validate_scene(*scene);
         const auto tree = rt::build_bvh<float>(
-            *scene, rt::leaf_size_for(static_cast<size_t>(scene->sphere_count + scene->quad_count)));
+            *scene, rt::leaf_size_for(static_cast<size_t>(scene->sphere_count + scene->quad_count)), kind);
         *node_count = static_cast<int64_t>(tree.nodes.size());
         if (depth)
         {
@@ -325,7 +325,7 @@ int rt_scene_lamps(const rt_scene* scene, float* lamps, int64_t* count)
         need(count, "count");
         rt::validate_scene(*scene);
         const auto tree = rt::build_bvh<float>(
-            *scene, rt::leaf_size_for(static_cast<size_t>(scene->sphere_count + scene->quad_count)));
+            *scene, rt::leaf_size_for(static_cast<size_t>(scene->sphere_count + scene->quad_count)), RT_TREE_DEFAULT);
         const rt::lamp_set set = rt::build_lamp_set(*scene, tree.sphere_ids, tree.quad_ids, tree.prim_materials.size());
         *count = static_cast<int64_t>(set.lamps.size());
         if (lamps)

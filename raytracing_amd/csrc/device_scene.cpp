@@ -376,8 +376,8 @@ dev::camera_rec<Real> to_camera(const rt_camera_desc& d)
 }
 
 template <typename Real>
-device_scene<Real>::device_scene(const rt_scene& s)
-    : tree_(build_bvh<Real>(s, leaf_size_for(static_cast<std::size_t>(s.sphere_count + s.quad_count))))
+device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
+    : tree_(build_bvh<Real>(s, leaf_size_for(static_cast<std::size_t>(s.sphere_count + s.quad_count)), tree))
 {
     // materials: a texture of one color becomes the material's albedo (src/gpu/renderer.cu:548-601)
     std::vector<dev::material_rec<Real>> materials;

@@ -89,7 +89,7 @@ template <typename Real>
 class device_scene
 {
 public:
-    explicit device_scene(const rt_scene& scene);
+    device_scene(const rt_scene& scene, int32_t tree); // tree: an rt_tree
     const dev::scene_view<Real>& view() const { return view_; }
     const bvh_arrays<Real>& tree() const { return tree_; }
     std::size_t bytes() const { return bytes_; }

@@ -539,9 +539,9 @@ __device__ inline bool pop(const lds_stack<Real>& stack, int& size, Real t_max, 
 
 // The closest primitive a ray hits at t in (0.001, ∞), with the self-intersection rule for the primitive it leaves
 // (src/gpu/bvh.cuh:576-706). AllKinds false: a tree of spheres that stand still only (the first book's scenes). Nodes
-// are visited near first and leaves tested in the reference's order; the loop either takes one node or one leaf per
-// iteration, or (RT_TRAVERSAL_WHILE_WHILE) walks down inner nodes in an inner loop until every lane of the wave has a
-// leaf, then tests the leaves (Aila and Laine 2009): the same decisions in the same order per ray either way.
+// are visited near first and leaves tested in the reference's order; the loop takes one node or one leaf per
+// iteration (on MI355X this measured faster than Aila and Laine's while-while form, which walks inner nodes in an
+// inner loop until every lane of the wave has a leaf).
 template <typename Real, bool AllKinds, typename Tally>
 __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
                                         const lds_stack<Real>& stack, Tally& tl)
@@ -557,23 +557,6 @@ __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Re
     const vec3<Real> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
     int size = 0;
     int32_t ref = sc.root_ref;
-#if RT_TRAVERSAL_WHILE_WHILE
-    while (true)
-    {
-        while (ref >= 0)
-        {
-            if (!descend(sc, ref, inv, os, t_min, closest.t, stack, size, tl) && !pop(stack, size, closest.t, ref))
-            {
-                return closest;
-            }
-        }
-        closest_in_leaf<Real, AllKinds>(sc, r, ref, from, inward, inverse_a, t_min, closest, tl);
-        if (!pop(stack, size, closest.t, ref))
-        {
-            return closest;
-        }
-    }
-#else
     while (true)
     {
         if (ref < 0)
@@ -589,7 +572,6 @@ __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Re
             return closest;
         }
     }
-#endif
 }
 
 // --- sphere surface helpers (src/gpu/sphere.cuh:125-168) ---------------------------------------------------------------

@@ -56,16 +56,16 @@ private:
 class shared_scene
 {
 public:
-    shared_scene(const rt_scene& scene, int32_t precision) : tables(scene_tables::copy_of(scene))
+    shared_scene(const rt_scene& scene, int32_t precision, int32_t tree) : tables(scene_tables::copy_of(scene))
     {
         view = tables.view();
         if (precision == RT_FLOAT64)
         {
-            f64 = std::make_unique<device_scene<double>>(view);
+            f64 = std::make_unique<device_scene<double>>(view, tree);
         }
         else
         {
-            f32 = std::make_unique<device_scene<float>>(view);
+            f32 = std::make_unique<device_scene<float>>(view, tree);
         }
     }
     template <typename Real>
@@ -86,7 +86,7 @@ public:
     std::unique_ptr<device_scene<double>> f64;
 };
 
-std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision)
+std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision, int32_t tree)
 {
     validate_scene(scene);
     if (precision != RT_FLOAT32 && precision != RT_FLOAT64)
@@ -98,7 +98,7 @@ std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t 
     {
         throw std::invalid_argument("triangles, GGX, environment maps and light samples render in float32 only");
     }
-    return std::make_shared<const shared_scene>(scene, precision);
+    return std::make_shared<const shared_scene>(scene, precision, tree);
 }
 
 std::size_t device_bytes(const shared_scene& scene) { return scene.f32 ? scene.f32->bytes() : scene.f64->bytes(); }
@@ -588,6 +588,10 @@ void check_options(const rt_session_options& options)
     {
         throw std::invalid_argument("the block size must be 64, 128, 192 or 256 threads");
     }
+    if (options.tree < RT_TREE_DEFAULT || options.tree > RT_TREE_SAH)
+    {
+        throw std::invalid_argument("unknown tree");
+    }
 }
 }
 
@@ -595,7 +599,7 @@ std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_op
 {
     validate_scene(scene);
     check_options(options);
-    return make_session(upload_scene(scene, options.precision), scene.camera, options);
+    return make_session(upload_scene(scene, options.precision, options.tree), scene.camera, options);
 }
 
 std::unique_ptr<session> make_session(std::shared_ptr<const shared_scene> scene, const rt_camera_desc& camera,

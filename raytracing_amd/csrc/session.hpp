@@ -39,9 +39,11 @@ public:
 };
 
 // A scene resident on the device: its tables and its arrays (tree, records, textures) in one precision, shared by
-// every session rendered from it (the server keeps scenes resident while requests of any resolution come and go)
+// every session rendered from it (the server keeps scenes resident while requests of any resolution come and go);
+// tree is an rt_tree
 class shared_scene;
-std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision);
+std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision,
+                                                 int32_t tree = RT_TREE_DEFAULT);
 std::size_t device_bytes(const shared_scene& scene);
 const rt_scene& scene_of(const shared_scene& scene);
 

@@ -85,6 +85,40 @@ def test_bvh_node_counts():
     assert rt.scene_bvh(rt.build_scene("final", earth_width=256))[0].shape[0] == 6813
 
 
+@pytest.mark.parametrize("name", ["weekend", "final", "cornell", "smoke"])
+def test_surface_area_tree_holds_every_primitive(rt, name):
+    s = rt.build_scene(name, width=64, samples=16, earth_width=256)
+    boxes, ints, sphere_ids, quad_ids, depth = rt.scene_bvh(s, "sah")
+    median = rt.scene_bvh(s, "median")
+    # the same primitives: each sphere and quad of the scene once, in one leaf of its kind
+    assert sorted(sphere_ids) == sorted(median[2]) and sorted(quad_ids) == sorted(median[3])
+    leaf = ints[:, 1] != 0
+    counts, kinds = ints[leaf, 1] & 255, ints[leaf, 1] >> 8
+    assert counts.sum() == len(sphere_ids) + len(quad_ids)
+    assert counts.max() <= 8 and depth <= 64
+    for kind in range(3):
+        starts = np.sort(ints[leaf][kinds == kind, 0])
+        sizes = counts[kinds == kind][np.argsort(ints[leaf][kinds == kind, 0])]
+        assert np.array_equal(starts, np.concatenate([[0], np.cumsum(sizes)[:-1]])) if len(starts) else True
+    # every inner node's box holds both children's; fewer nodes than the reference's leaf-per-primitive tree of a
+    # scene of more than 8 shapes
+    inner = np.nonzero(~leaf)[0]
+    for child in (inner + 1, ints[inner, 0]):
+        assert np.all(boxes[child, :3] >= boxes[inner, :3]) and np.all(boxes[child, 3:] <= boxes[inner, 3:])
+    if counts.sum() > 8:
+        assert len(boxes) < len(median[0])
+
+
+def test_unknown_tree_is_refused(rt):
+    s = rt.build_scene("weekend", width=32, samples=1)
+    with pytest.raises(ValueError, match="tree"):
+        rt.Session(s, tree=3)
+    with pytest.raises(ValueError, match="tree"):
+        rt.render(s, tree=-1)
+    with pytest.raises(ValueError, match="tree"):
+        rt.scene_bvh(s, 9)
+
+
 def test_twin_traversal_equals_brute_force():
     import raytracing_amd as rt
     from tests.rays import scene_rays

@@ -41,12 +41,63 @@ def _compare_with_rerun(name, width, spp, sampling=None, precision="float32", in
 def test_closest_hit_bit_exact(rt, name):
     scene = rt.build_scene(name, width=64, samples=16, earth_width=512)
     o, d, tm, f, inw = scene_rays(scene, 60000, seed=zlib.crc32(name.encode()) % 1000, width=64, spp=16, earth_width=512)
-    with rt.Session(scene, integrator="megakernel") as s:
+    with rt.Session(scene, integrator="megakernel", tree="median") as s:
         ids, t, ent = s.closest_hit(o, d, tm, f, inw)
     tids, tt, tent = oracle.twin_closest_hit(name, o, d, tm, f, inw, 64, 16, 512)
     assert (tids >= 0).sum() > len(tids) // 4, "the ray set should hit something"
     mism = np.nonzero((ids != tids) | (t.view(np.uint32) != tt.view(np.uint32)) | (ent != tent))[0]
     assert len(mism) == 0, f"{len(mism)} of {len(ids)} rays differ, first {mism[:5]}: gpu {ids[mism[:5]]} {t[mism[:5]]} twin {tids[mism[:5]]} {tt[mism[:5]]}"
+
+
+def _to_tree(rt, scene, ids, frm: str, to: str):
+    """Global primitive ids of one tree as the ids of the same spheres and quads in another (-1 stays -1)."""
+    _, _, s_from, q_from, _ = rt.scene_bvh(scene, frm)
+    _, _, s_to, q_to, _ = rt.scene_bvh(scene, to)
+    table = np.full(len(s_from) + len(q_from) + 1, -1, np.int64)
+    table[np.concatenate([s_from, q_from])] = np.concatenate([s_to, q_to])
+    return np.where(ids >= 0, table[np.maximum(ids, 0)], -1).astype(np.int32)
+
+
+@pytest.mark.parametrize("name", ["weekend", "final", "cornell", "smoke"])
+def test_surface_area_tree_finds_the_same_hits(rt, name):
+    """The surface-area tree tests other boxes in another order, but every ray's closest hit is the same primitive at
+    the same distance (bit for bit) as in the reference's median tree; only a tie (two primitives at exactly the same
+    distance) may keep the other one."""
+    scene = rt.build_scene(name, width=64, samples=16, earth_width=512)
+    o, d, tm, f, inw = scene_rays(scene, 60000, seed=7 + zlib.crc32(name.encode()) % 1000, width=64, spp=16,
+                                  earth_width=512)
+    with rt.Session(scene, integrator="megakernel", tree="median") as s:
+        ids_m, t_m, ent_m = s.closest_hit(o, d, tm, f, inw)
+    with rt.Session(scene, integrator="megakernel", tree="sah") as s:
+        ids_s, t_s, ent_s = s.closest_hit(o, d, tm, _to_tree(rt, scene, f, "median", "sah"), inw)
+    assert np.array_equal(t_m.view(np.uint32), t_s.view(np.uint32))
+    ids_s = _to_tree(rt, scene, ids_s, "sah", "median")
+    same = ids_s == ids_m
+    assert np.array_equal(ent_m[same], ent_s[same])
+    # the rays that keep another primitive hit two coplanar quads (the shared faces of the ground's boxes, met from
+    # inside by the bounce rays' random directions) at the same distance: ties, which either tree may break its way
+    differ = np.nonzero(~same)[0]
+    assert len(differ) <= len(ids_m) // 20, f"{len(differ)} rays keep another primitive"
+    if len(differ):
+        _, _, sphere_ids, quad_ids, _ = rt.scene_bvh(scene, "median")
+        source = np.full(len(sphere_ids) + len(quad_ids), -1, np.int64)
+        source[quad_ids] = np.arange(len(quad_ids))
+        d = scene.desc
+        quads = [d.quads[k] for k in range(d.quad_count)]
+
+        def plane(k):
+            q = quads[k]
+            u, v = np.array([q.u.x, q.u.y, q.u.z]), np.array([q.v.x, q.v.y, q.v.z])
+            n = np.cross(u, v)
+            n /= np.linalg.norm(n)
+            return n, float(n @ np.array([q.q.x, q.q.y, q.q.z]))
+
+        for k in differ:
+            a, b = source[ids_m[k]], source[ids_s[k]]
+            assert a >= 0 and b >= 0, f"ray {k}: a tie between {ids_m[k]} and {ids_s[k]} that aren't both quads"
+            (na, da), (nb, db) = plane(a), plane(b)
+            assert abs(abs(na @ nb) - 1) < 1e-12 and abs(da - np.sign(na @ nb) * db) < 1e-9 * (1 + abs(da)), \
+                f"ray {k}: quads {a} and {b} aren't coplanar"
 
 
 @pytest.mark.parametrize("precision", ["float32", "float64"])

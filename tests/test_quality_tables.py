@@ -140,7 +140,7 @@ def test_lamps_cornell(rt):
     assert lamp[3] == np.float32(130 * 105)
     assert np.all(lamp[12:15] == np.float32([0, -1, 0]))
     assert np.all(lamp[16:19] == 15)
-    _, _, _, quad_ids, _ = rt.scene_bvh(s)
+    _, _, _, quad_ids, _ = rt.scene_bvh(s, "default")
     assert lamp[11].view(np.int32) == quad_ids[5], "the lamp is the world's sixth quad"
 
 
