@@ -474,56 +474,88 @@ private:
         return index;
     }
 
-    int32_t ref_of(int node, const std::vector<int32_t>& wide_index) const
+    int32_t leaf_ref_of(int node) const
     {
         const ref_node<Real>& n = out_.nodes[node];
-        if (n.count > 0)
+        const int count = n.count & 255;
+        if (n.index > dev::max_leaf_first || count > 16)
         {
-            const int count = n.count & 255;
-            if (n.index > dev::max_leaf_first || count > 16)
-            {
-                throw std::runtime_error("a BVH leaf beyond the packed reference's range");
-            }
-            return dev::make_leaf_ref(n.count >> 8, count, n.index);
+            throw std::runtime_error("a BVH leaf beyond the packed reference's range");
         }
-        return wide_index[node];
+        return dev::make_leaf_ref(n.count >> 8, count, n.index);
     }
 
-    // Each inner node becomes a wide node holding its two children's boxes, numbered in depth-first order
+    double node_area(int node) const
+    {
+        const ref_node<Real>& n = out_.nodes[node];
+        const double x = static_cast<double>(n.box_max[0]) - n.box_min[0];
+        const double y = static_cast<double>(n.box_max[1]) - n.box_min[1];
+        const double z = static_cast<double>(n.box_max[2]) - n.box_min[2];
+        return x * y + y * z + z * x;
+    }
+
+    // The binary tree collapsed four-wide. A wide node starts from its binary node's two children and, while it has
+    // fewer than four, replaces its inner child of largest surface area by that child's own two children (in place,
+    // so the children keep the binary tree's depth-first order). Wide nodes are numbered in depth-first order from the
+    // root. `pushed` is what the walk can have on its stack on arrival; the stack's exact worst case is the largest
+    // pushed + (children - 1) over the wide nodes.
+    int32_t collapse(int node, int pushed)
+    {
+        std::array<int, dev::wide_children> kids{node + 1, out_.nodes[node].index, -1, -1};
+        int count = 2;
+        while (count < dev::wide_children)
+        {
+            int best = -1;
+            double best_area = -1;
+            for (int c = 0; c < count; ++c)
+            {
+                if (out_.nodes[kids[c]].count == 0 && node_area(kids[c]) > best_area)
+                {
+                    best = c;
+                    best_area = node_area(kids[c]);
+                }
+            }
+            if (best < 0)
+            {
+                break;
+            }
+            const int inner = kids[best];
+            for (int c = count; c > best + 1; --c)
+            {
+                kids[c] = kids[c - 1];
+            }
+            kids[best] = inner + 1;
+            kids[best + 1] = out_.nodes[inner].index;
+            ++count;
+        }
+        const auto me = static_cast<int32_t>(out_.wide.size());
+        out_.wide.emplace_back();
+        out_.stack_entries = std::max(out_.stack_entries, pushed + count - 1);
+        int32_t refs[dev::wide_children];
+        for (int c = 0; c < dev::wide_children; ++c)
+        {
+            refs[c] = c >= count                       ? dev::no_child
+                      : out_.nodes[kids[c]].count != 0 ? leaf_ref_of(kids[c])
+                                                       : collapse(kids[c], pushed + count - 1);
+        }
+        dev::wide_node<Real>& w = out_.wide[static_cast<std::size_t>(me)];
+        for (int c = 0; c < dev::wide_children; ++c)
+        {
+            w.ref[c] = refs[c];
+            for (int a = 0; a < 3; ++a)
+            {
+                w.lo[a][c] = c < count ? out_.nodes[kids[c]].box_min[a] : std::numeric_limits<Real>::infinity();
+                w.hi[a][c] = c < count ? out_.nodes[kids[c]].box_max[a] : std::numeric_limits<Real>::infinity();
+            }
+        }
+        return me;
+    }
+
     void pack_wide()
     {
-        std::vector<int32_t> wide_index(out_.nodes.size(), -1);
-        int32_t next = 0;
-        for (std::size_t i = 0; i < out_.nodes.size(); ++i)
-        {
-            if (out_.nodes[i].count == 0)
-            {
-                wide_index[i] = next++;
-            }
-        }
-        out_.wide.resize(static_cast<std::size_t>(next));
-        for (std::size_t i = 0; i < out_.nodes.size(); ++i)
-        {
-            const ref_node<Real>& n = out_.nodes[i];
-            if (n.count != 0)
-            {
-                continue;
-            }
-            const int first = static_cast<int>(i) + 1;
-            const int second = n.index;
-            dev::wide_node<Real> w{};
-            for (int k = 0; k < 3; ++k)
-            {
-                w.lo0[k] = out_.nodes[first].box_min[k];
-                w.hi0[k] = out_.nodes[first].box_max[k];
-                w.lo1[k] = out_.nodes[second].box_min[k];
-                w.hi1[k] = out_.nodes[second].box_max[k];
-            }
-            w.ref0 = ref_of(first, wide_index);
-            w.ref1 = ref_of(second, wide_index);
-            out_.wide[wide_index[i]] = w;
-        }
-        out_.root_ref = ref_of(0, wide_index);
+        out_.wide.clear();
+        out_.stack_entries = 1;
+        out_.root_ref = out_.nodes[0].count != 0 ? leaf_ref_of(0) : collapse(0, 0);
     }
 };
 }

@@ -1,5 +1,6 @@
-// The BVH builder (host code): an object-median split over every sphere and quad of a scene, as the reference builds
-// its GPU tree (src/gpu/bvh.cuh:100-444), then packed into the wide layout the kernels walk (device_types.hpp).
+// The BVH builder (host code): a binary tree over every sphere, quad and triangle of a scene (the reference's
+// object-median splits, src/gpu/bvh.cuh:100-444, or binned surface-area-heuristic ones), then collapsed into the
+// four-wide layout the kernels walk (device_types.hpp).
 #pragma once
 
 #include <cstdint>
@@ -24,10 +25,11 @@ struct ref_node
 template <typename Real>
 struct bvh_arrays
 {
-    std::vector<ref_node<Real>> nodes; // depth first, the reference's order
-    std::vector<dev::wide_node<Real>> wide;
+    std::vector<ref_node<Real>> nodes; // the binary tree, depth first, the reference's order
+    std::vector<dev::wide_node<Real>> wide; // the binary tree collapsed four-wide: what the kernels walk
     int32_t root_ref = -1;
-    int32_t depth = 0; // levels: the most entries the traversal stack holds
+    int32_t depth = 0;         // the binary tree's levels
+    int32_t stack_entries = 0; // the most entries the four-wide walk's stack can hold
     std::vector<dev::sphere_rec<Real>> spheres;
     std::vector<dev::moving_sphere_rec<Real>> moving_spheres;
     std::vector<dev::quad_rec<Real>> quads;

@@ -601,7 +601,7 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
     }
     v.nodes = nodes_.data();
     v.root_ref = tree_.root_ref;
-    v.stack_depth = tree_.nodes.empty() ? 0 : tree_.depth;
+    v.stack_depth = tree_.nodes.empty() ? 0 : tree_.stack_entries;
     v.spheres = spheres_.data();
     v.moving_spheres = moving_.data();
     v.quads = quads_.data();

@@ -3,10 +3,12 @@
 // read with dwordx4 loads.
 //
 // Layout choices (the reference's are in src/gpu/bvh.cuh:29-38, sphere.cuh:11-46, quad.cuh:19-30):
-//   - BVH: a node holds the boxes of BOTH its children (64 bytes in float: one cache-line-sized fetch per visited
-//     node), where the reference's 32-byte node holds its own box and the visit reads the two children's records, one
-//     of them elsewhere in memory. A child is an inner node's index, or a packed leaf reference (leaf_ref below). The
-//     traversal makes the same decisions in the same order as the reference's (the same boxes, the same slab test).
+//   - BVH: four-wide. The binary tree (the reference's object-median one, or the surface-area one) is collapsed so
+//     that a node holds the boxes of up to FOUR children (128 bytes in float: one L2 line per visited node), where the
+//     reference's 32-byte node holds its own box and a visit reads two children's records from two places. A child is
+//     an inner node's index, or a packed leaf reference (leaf_ref below). The boxes are the binary tree's, the slab
+//     test is the reference's, and the closest hit is the same primitive at the same distance (two primitives at
+//     exactly the same distance may resolve either way: the walk's order differs).
 //   - primitives keep the reference's global ids: spheres that stand still [0, moving_base), moving spheres
 //     [moving_base, quad_base), quads [quad_base, ...), and the media after the surfaces.
 #pragma once
@@ -39,20 +41,21 @@ __host__ __device__ constexpr int32_t leaf_kind(int32_t ref) { return (ref >> 28
 __host__ __device__ constexpr int32_t leaf_count(int32_t ref) { return ((ref >> 24) & 15) + 1; }
 __host__ __device__ constexpr int32_t leaf_first(int32_t ref) { return ref & max_leaf_first; }
 
-// An inner node: the boxes of its two children (in the reference's order: the first child, then the second) and their
-// references. 64 bytes in float.
+// An inner node of the four-wide tree: the boxes of up to four children, one array per bound and axis (lo[axis][child],
+// so the four slab tests are the same instructions on different registers), and the children's references in the
+// binary tree's depth-first order. A missing child has a box of +infinity corners, which box_entry never enters (each
+// slab gives +inf or -inf), so the walk needs no test for it. 128 bytes in float, 256 in double.
+constexpr int wide_children = 4;
+constexpr int32_t no_child = 0x7fffffff;
 template <typename Real>
 struct alignas(16) wide_node
 {
-    Real lo0[3];
-    Real hi0[3];
-    Real lo1[3];
-    Real hi1[3];
-    int32_t ref0;
-    int32_t ref1;
-    int32_t unused[2];
+    Real lo[3][wide_children];
+    Real hi[3][wide_children];
+    int32_t ref[wide_children];
+    int32_t unused[sizeof(Real) == 4 ? 4 : 12];
 };
-static_assert(sizeof(wide_node<float>) == 64);
+static_assert(sizeof(wide_node<float>) == 128 && sizeof(wide_node<double>) == 256);
 
 template <typename Real>
 struct alignas(16) sphere_rec
@@ -277,7 +280,7 @@ struct scene_view
     quality_view quality;
     const wide_node<Real>* nodes;
     int32_t root_ref;  // the root: an inner node's index (0), a leaf reference, or -1 with no primitive at all
-    int32_t stack_depth;
+    int32_t stack_depth; // the most entries the traversal stack can hold (at least 1), or 0 with no primitive at all
     const sphere_rec<Real>* spheres;
     const moving_sphere_rec<Real>* moving_spheres;
     const quad_rec<Real>* quads;

@@ -20,15 +20,10 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-template <typename Real>
-__device__ inline lds_stack<Real> block_stack(int depth)
+__device__ inline lds_stack block_stack(int entries)
 {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int threads = blockDim.x * blockDim.y;
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    int32_t* refs = reinterpret_cast<int32_t*>(lds);
-    Real* ts = reinterpret_cast<Real*>(lds + ((static_cast<size_t>(depth) * threads * 4 + 15) & ~size_t(15)));
-    return {refs + tid, ts + tid, threads};
+    return stack_in(lds, entries, blockDim.x * blockDim.y, threadIdx.y * blockDim.x + threadIdx.x);
 }
 
 static_assert(tally_events == RT_TRAFFIC_EVENTS, "the traffic report lists every tally event");
@@ -72,7 +67,7 @@ megakernel(const camera_rec<Real> cam, const scene_view<Real> sc,
                                                   unsigned long long* __restrict__ ray_count,
                                                   unsigned long long* __restrict__ tallies)
 {
-    const lds_stack<Real> stack = block_stack<Real>(sc.stack_depth);
+    const lds_stack stack = block_stack(sc.stack_depth);
     tally<Count> tl;
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int j = work.first_row + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
@@ -123,7 +118,7 @@ quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc,
                                                           unsigned long long* __restrict__ ray_count,
                                                           unsigned long long* __restrict__ tallies)
 {
-    const lds_stack<float> stack = block_stack<float>(sc.stack_depth);
+    const lds_stack stack = block_stack(sc.stack_depth);
     tally<Count> tl;
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int j = work.first_row + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
@@ -170,7 +165,7 @@ __global__ void __launch_bounds__(256) closest_hit_kernel(const scene_view<Real>
                                                           const int32_t* from, const uint8_t* inward, int64_t n,
                                                           int32_t* index, Real* t, uint8_t* entering)
 {
-    const lds_stack<Real> stack = block_stack<Real>(sc.stack_depth);
+    const lds_stack stack = block_stack(sc.stack_depth);
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n)
     {
@@ -215,9 +210,9 @@ __global__ void __launch_bounds__(256) display_kernel(const double* sums, int64_
 namespace rt
 {
 template <typename Real>
-size_t stack_lds_bytes(int depth, int threads)
+size_t stack_lds_bytes(int entries, int threads)
 {
-    return ((static_cast<size_t>(depth) * threads * 4 + 15) & ~size_t(15)) + static_cast<size_t>(depth) * threads * sizeof(Real);
+    return dev::stack_bytes(entries, threads);
 }
 
 namespace

@@ -382,7 +382,7 @@ struct hit
 // kernels with tally<true>); tally<false> compiles to nothing
 enum tally_event
 {
-    tally_node = 0,       // a wide node: both children's boxes
+    tally_node = 0,       // a wide node: its four children's boxes
     tally_sphere,         // a sphere record tested
     tally_moving,         // a moving sphere record tested
     tally_quad,           // a quad record tested
@@ -412,20 +412,44 @@ struct tally<true>
     __device__ void add(tally_event e, uint32_t k = 1) { n[e] += k; }
 };
 
-// The traversal stack of one thread: entries (a node or leaf reference, the distance the ray enters it) in LDS,
-// interleaved across the block's threads so that a wave's 64 accesses to one level fall into distinct banks
-template <typename Real>
+// A distance on the traversal stack in 16 bits: the upper half of a float no larger than t (t > 0, so dropping the low
+// bits rounds down). The pop's test `entry < closest` can then keep an entry the exact distance would have dropped
+// (a wasted visit: every box and primitive below it is still tested against the closest distance), never the reverse.
+__device__ inline uint16_t stack_distance(float t) { return static_cast<uint16_t>(__float_as_uint(t) >> 16); }
+__device__ inline uint16_t stack_distance(double t) { return stack_distance(__double2float_rd(t)); }
+__device__ inline float stack_distance_value(uint16_t bits) { return __uint_as_float(static_cast<uint32_t>(bits) << 16); }
+
+// The traversal stack of one thread in LDS: entries of a node or leaf reference (4 bytes) and the distance the ray
+// enters it (2 bytes), interleaved across the block's threads so that a wave's 64 accesses to one level fall into
+// distinct banks. 6 bytes an entry keep the four-wide walk's deeper stack within the LDS the binary walk's 8-byte
+// entries took, and so the occupancy.
 struct lds_stack
 {
     int32_t* refs;
-    Real* ts;
+    uint16_t* ts;
     int stride; // threads per block
+    template <typename Real>
     __device__ void store(int level, int32_t ref, Real t) const
     {
         refs[level * stride] = ref;
-        ts[level * stride] = t;
+        ts[level * stride] = stack_distance(t);
     }
 };
+
+// LDS bytes of a block's stacks, and one thread's stack within them
+__host__ __device__ constexpr size_t stack_ref_bytes(int entries, int threads)
+{
+    return (static_cast<size_t>(entries) * static_cast<size_t>(threads) * 4 + 15) & ~size_t(15);
+}
+__host__ __device__ constexpr size_t stack_bytes(int entries, int threads)
+{
+    return stack_ref_bytes(entries, threads) + static_cast<size_t>(entries) * static_cast<size_t>(threads) * 2;
+}
+__device__ inline lds_stack stack_in(unsigned char* lds, int entries, int threads, int tid)
+{
+    return {reinterpret_cast<int32_t*>(lds) + tid,
+            reinterpret_cast<uint16_t*>(lds + stack_ref_bytes(entries, threads)) + tid, threads};
+}
 
 // The primitives of a leaf against the ray, the closest hit so far updated (src/gpu/bvh.cuh:612-680)
 template <typename Real, bool AllKinds, typename Tally>
@@ -496,39 +520,76 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
     }
 }
 
-// An inner node: both children's boxes tested; the farther one entered goes on the stack, and the nearer one is the
-// next reference (true), or neither is entered (false). The nearer of equal entries is the first child.
+// Where the ray enters a wide node's child c, or infinity (a missing child's +infinity box included)
+template <typename Real>
+__device__ inline Real child_entry(const wide_node<Real>& node, int c, vec3<Real> inv, vec3<Real> os, Real t_min,
+                                   Real t_max)
+{
+    const Real lo[3] = {node.lo[0][c], node.lo[1][c], node.lo[2][c]};
+    const Real hi[3] = {node.hi[0][c], node.hi[1][c], node.hi[2][c]};
+    return box_entry(lo, hi, inv, os, t_min, t_max);
+}
+
+// One compare-exchange of the sorting network: the nearer entry first
+template <typename Real>
+__device__ inline void order(Real& ta, int32_t& ra, Real& tb, int32_t& rb)
+{
+    const bool swap = tb < ta;
+    const Real t = swap ? tb : ta;
+    const int32_t r = swap ? rb : ra;
+    tb = swap ? ta : tb;
+    rb = swap ? ra : rb;
+    ta = t;
+    ra = r;
+}
+
+// An inner node: its four children's boxes tested and sorted by entry distance (a five-exchange network); the entered
+// ones but the nearest go on the stack, farthest first, and the nearest is the next reference (true), or none is
+// entered (false)
 template <typename Real, typename Tally>
 __device__ inline bool descend(const scene_view<Real>& sc, int32_t& ref, vec3<Real> inv, vec3<Real> os, Real t_min,
-                               Real t_max, const lds_stack<Real>& stack, int& size, Tally& tl)
+                               Real t_max, const lds_stack& stack, int& size, Tally& tl)
 {
     const wide_node<Real> node = sc.nodes[ref];
     tl.add(tally_node);
-    const Real t0 = box_entry(node.lo0, node.hi0, inv, os, t_min, t_max);
-    const Real t1 = box_entry(node.lo1, node.hi1, inv, os, t_min, t_max);
-    const bool second_nearer = t1 < t0;
-    const Real t_far = m_max(t0, t1);
-    if (t_far < inf<Real>)
+    Real t0 = child_entry(node, 0, inv, os, t_min, t_max);
+    Real t1 = child_entry(node, 1, inv, os, t_min, t_max);
+    Real t2 = child_entry(node, 2, inv, os, t_min, t_max);
+    Real t3 = child_entry(node, 3, inv, os, t_min, t_max);
+    int32_t r0 = node.ref[0], r1 = node.ref[1], r2 = node.ref[2], r3 = node.ref[3];
+    order(t0, r0, t1, r1);
+    order(t2, r2, t3, r3);
+    order(t0, r0, t2, r2);
+    order(t1, r1, t3, r3);
+    order(t1, r1, t2, r2);
+    if (t3 < inf<Real>)
     {
-        stack.store(size++, second_nearer ? node.ref0 : node.ref1, t_far);
+        stack.store(size++, r3, t3);
     }
-    if (m_min(t0, t1) < inf<Real>)
+    if (t2 < inf<Real>)
     {
-        ref = second_nearer ? node.ref1 : node.ref0;
+        stack.store(size++, r2, t2);
+    }
+    if (t1 < inf<Real>)
+    {
+        stack.store(size++, r1, t1);
+    }
+    if (t0 < inf<Real>)
+    {
+        ref = r0;
         return true;
     }
     return false;
 }
 
-// The next stack entry the ray enters before the closest hit so far, or false when there is none
+// The next stack entry the ray may enter before the closest hit so far, or false when there is none
 template <typename Real>
-__device__ inline bool pop(const lds_stack<Real>& stack, int& size, Real t_max, int32_t& ref)
+__device__ inline bool pop(const lds_stack& stack, int& size, Real t_max, int32_t& ref)
 {
     while (size > 0)
     {
         --size;
-        const Real t = stack.ts[size * stack.stride];
-        if (t < t_max)
+        if (stack_distance_value(stack.ts[size * stack.stride]) < t_max)
         {
             ref = stack.refs[size * stack.stride];
             return true;
@@ -544,7 +605,7 @@ __device__ inline bool pop(const lds_stack<Real>& stack, int& size, Real t_max, 
 // inner loop until every lane of the wave has a leaf).
 template <typename Real, bool AllKinds, typename Tally>
 __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
-                                        const lds_stack<Real>& stack, Tally& tl)
+                                        const lds_stack& stack, Tally& tl)
 {
     const Real t_min = Real(0.001);
     hit<Real> closest{-1, inf<Real>, false};
@@ -869,7 +930,7 @@ __device__ inline bool medium_collision(const medium_rec<Real>& m, const quad_re
 // order, each against the closest event so far (src/gpu/camera.cuh:259-284)
 template <typename Real, typename Tally>
 __device__ inline hit<Real> closest_event(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
-                                          pcg32& g, const lds_stack<Real>& stack, Tally& tl)
+                                          pcg32& g, const lds_stack& stack, Tally& tl)
 {
     hit<Real> closest = closest_hit<Real, true>(sc, r, from, inward, stack, tl);
     if (sc.medium_count == 0)
@@ -1163,7 +1224,7 @@ __device__ inline bounce diffuse_bounce(const scene_view<Real>& sc, bool matte, 
 // True when the sample is complete (its color in `sample`).
 template <typename Real, typename Tally>
 __device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                        pcg32& g, vec3<Real>& sample, const lds_stack<Real>& stack, Tally& tl)
+                                        pcg32& g, vec3<Real>& sample, const lds_stack& stack, Tally& tl)
 {
     const hit<Real> h = closest_hit<Real, false>(sc, p.r, p.from, p.inward, stack, tl);
     if (h.index < 0)
@@ -1251,7 +1312,7 @@ __device__ inline bool shade(path<Real>& p, const scene_view<Real>& sc, const hi
 // One ray of a path with every feature of the three books (src/gpu/camera.cuh:364-437)
 template <typename Real, typename Tally>
 __device__ inline bool trace_all_books(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                       pcg32& g, vec3<Real>& sample, const lds_stack<Real>& stack, Tally& tl)
+                                       pcg32& g, vec3<Real>& sample, const lds_stack& stack, Tally& tl)
 {
     const hit<Real> h = closest_event(sc, p.r, p.from, p.inward, g, stack, tl);
     if (h.index < 0)

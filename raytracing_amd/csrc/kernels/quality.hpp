@@ -594,7 +594,7 @@ __device__ inline float lamp_hit_pdf(const quality_view& q, int id, vec3<float> 
 // Anything in (0.001, t_max) but the primitive `skip`? closest_hit's walk with a fixed t_max, ending at the first hit
 template <typename Tally>
 __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r, float t_max, int from, bool inward,
-                                int skip, const lds_stack<float>& stack, Tally& tl)
+                                int skip, const lds_stack& stack, Tally& tl)
 {
     const float t_min = 0.001f;
     if (sc.stack_depth == 0)
@@ -673,18 +673,28 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
         }
         else
         {
+            // any entered child will do: the first one entered is next, the others go on the stack
             const wide_node<float> node = sc.nodes[ref];
             tl.add(tally_node);
-            const float t0 = box_entry(node.lo0, node.hi0, inv, os, t_min, t_max);
-            const float t1 = box_entry(node.lo1, node.hi1, inv, os, t_min, t_max);
-            const bool second_nearer = t1 < t0;
-            if (fmaxf(t0, t1) < inf<float>)
+            int32_t next = no_child;
+#pragma unroll
+            for (int c = 0; c < wide_children; ++c)
             {
-                stack.store(size++, second_nearer ? node.ref0 : node.ref1, 0.0f);
+                if (child_entry(node, c, inv, os, t_min, t_max) < inf<float>)
+                {
+                    if (next == no_child)
+                    {
+                        next = node.ref[c];
+                    }
+                    else
+                    {
+                        stack.store(size++, node.ref[c], t_min);
+                    }
+                }
             }
-            if (fminf(t0, t1) < inf<float>)
+            if (next != no_child)
             {
-                ref = second_nearer ? node.ref1 : node.ref0;
+                ref = next;
                 continue;
             }
         }
@@ -770,7 +780,7 @@ __device__ inline bool light_contribution(const scene_view<float>& sc, const bsd
 // What a hit (or a miss) does to a quality path: the part of trace_quality_ray after the closest event
 template <typename Tally>
 __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
-                                     const hit<float>& h, pcg32& g, vec3<float>& sample, const lds_stack<float>& stack,
+                                     const hit<float>& h, pcg32& g, vec3<float>& sample, const lds_stack& stack,
                                      Tally& tl, uint32_t& shadow_rays)
 {
     path<float>& p = q.base;
@@ -856,7 +866,7 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
 
 template <typename Tally>
 __device__ inline bool trace_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
-                                     pcg32& g, vec3<float>& sample, const lds_stack<float>& stack, Tally& tl,
+                                     pcg32& g, vec3<float>& sample, const lds_stack& stack, Tally& tl,
                                      uint32_t& shadow_rays)
 {
     const hit<float> h = closest_event(sc, q.base.r, q.base.from, q.base.inward, g, stack, tl);

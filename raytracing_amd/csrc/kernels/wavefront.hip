@@ -220,13 +220,10 @@ __device__ inline void load_prefix(uint32_t* lds, const uint32_t* prefix, int n)
     __syncthreads();
 }
 
-__device__ inline lds_stack<float> thread_stack(int depth)
+__device__ inline lds_stack thread_stack(int entries)
 {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int threads = blockDim.x;
-    int32_t* refs = reinterpret_cast<int32_t*>(lds);
-    float* ts = reinterpret_cast<float*>(lds + ((static_cast<size_t>(depth) * threads * 4 + 15) & ~size_t(15)));
-    return {refs + threadIdx.x, ts + threadIdx.x, threads};
+    return stack_in(lds, entries, blockDim.x, threadIdx.x);
 }
 
 // The shading class of a hit: which queue it waits in
@@ -404,7 +401,7 @@ __global__ void __launch_bounds__(256) wf_extend(const scene_view<float> sc, wf_
 {
     __shared__ uint32_t prefix[wf_stripes + 2];
     load_prefix(prefix, pool.prefix + wf_prefix_active, wf_stripes + 2);
-    const lds_stack<float> stack = thread_stack(sc.stack_depth);
+    const lds_stack stack = thread_stack(sc.stack_depth);
     const uint32_t n = prefix[wf_stripes + 1];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x * blockDim.x >= n)
@@ -497,7 +494,7 @@ __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> 
 {
     __shared__ uint32_t prefix[wf_classes * wf_stripes + 1];
     load_prefix(prefix, pool.prefix + wf_prefix_class, wf_classes * wf_stripes + 1);
-    const lds_stack<float> stack = thread_stack(sc.stack_depth);
+    const lds_stack stack = thread_stack(sc.stack_depth);
     const uint32_t total = prefix[wf_classes * wf_stripes];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x * blockDim.x >= total)
@@ -542,7 +539,7 @@ __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, con
 {
     __shared__ uint32_t prefix[wf_stripes + 2];
     load_prefix(prefix, pool.prefix + wf_prefix_active, wf_stripes + 2);
-    const lds_stack<float> stack = thread_stack(sc.stack_depth);
+    const lds_stack stack = thread_stack(sc.stack_depth);
     const uint32_t n = prefix[wf_stripes + 1];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long rays = 0;
