@@ -77,14 +77,24 @@ def cpu_baseline(name: str, width: int, spp: int):
                       f"{rays / paths:.2f} rays/path"}
 
 
-def measured_traffic(workload: str, kernel: str):
-    """HBM bytes per launch for this workload and kernel, from the rocprofv3 counter summary, or None"""
+def traffic_key(kernel: str) -> str:
+    """The kernel's key in profiles/hbm_traffic.txt: "wavefront", or "megakernel-w<waves>" (the register budget sets
+    the spills, which are HBM writes; the block size doesn't change the traffic)"""
+    if kernel == "wavefront":
+        return kernel
+    waves = next((w[1:] for w in kernel.split() if w.startswith("w") and w[1:].isdigit()), "0")
+    return f"megakernel-w{waves}"
+
+
+def measured_traffic(workload: str, key: str):
+    """HBM bytes per launch for this workload and kernel (traffic_key), from the rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs of the same command, scaled by the copy kernel's calibration), or None"""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 4 and not line.startswith("#") and parts[0] == workload and parts[1] == kernel:
+        if len(parts) >= 4 and not line.startswith("#") and parts[0] == workload and parts[1] == key:
             return float(parts[2]) + float(parts[3])
     return None
 
@@ -307,7 +317,7 @@ def main():
         cpu = cpu_baseline(name, width, 64 if name == "final" else 16)
         print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
-    hbm = measured_traffic(args.scene, kernel.split()[0])
+    hbm = measured_traffic(args.scene, traffic_key(kernel))
     line = {
         "metric": "Mpaths/s (1080^2 final scene)" if args.scene == "final" else f"Mpaths/s ({args.scene})",
         "value": round(value, 2),

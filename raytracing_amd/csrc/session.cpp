@@ -6,6 +6,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
+#include <numeric>
 #include <stdexcept>
 
 #include "device_scene.hpp"
@@ -142,15 +144,10 @@ public:
             }
             candidates_.push_back({true, 0, 256});
             current_ = candidates_.front();
-            // each candidate renders two passes, in a forward then a backward sweep, so that a drift in the timings
-            // (clocks, caches warming) weighs on all of them alike
+            // the first round: every candidate renders one pass (a screen; a one-pass launch is mostly its tail)
             for (std::size_t k = 0; k < candidates_.size(); ++k)
             {
-                schedule_.push_back(k);
-            }
-            for (std::size_t k = candidates_.size(); k-- > 0;)
-            {
-                schedule_.push_back(k);
+                schedule_.push_back({k, 1});
             }
             tune_seconds_.assign(candidates_.size(), 0.0);
         }
@@ -206,13 +203,15 @@ public:
                 const int64_t left_in_round = round_size - work.first_pass;
                 int64_t count = seconds_per_pass_ > 0 ? static_cast<int64_t>(launch_seconds_ / seconds_per_pass_) : 1;
                 count = std::max<int64_t>(1, std::min({count, left_in_round, target_passes - pass}));
-                work.pass_count = tuning() ? 1 : static_cast<int32_t>(count);
+                work.pass_count = tuning() ? static_cast<int32_t>(std::min<int64_t>(schedule_[tuned_].passes, std::min(
+                                                 left_in_round, target_passes - pass)))
+                                           : static_cast<int32_t>(count);
                 work.first_row = 0;
                 work.row_count = height;
             }
             const bool whole = work.first_row == 0 && work.row_count == height;
             const bool probe = whole && tuning();
-            const variant v = probe ? candidates_[schedule_[tuned_]] : current_;
+            const variant v = probe ? candidates_[schedule_[tuned_].candidate] : current_;
             const double seconds = launch(options_.seed + static_cast<uint64_t>(round) * round_seed_step, work, v);
             report.launches += 1;
             report.bounces += last_bounces_;
@@ -224,15 +223,10 @@ public:
                 state_.passes += work.pass_count;
                 if (probe)
                 {
-                    // a candidate's pass timed: once the schedule is done, the fastest renders the rest
-                    tune_seconds_[schedule_[tuned_]] += seconds;
+                    tune_seconds_[schedule_[tuned_].candidate] += seconds / work.pass_count;
                     if (++tuned_ == schedule_.size())
                     {
-                        const auto best = static_cast<std::size_t>(
-                            std::min_element(tune_seconds_.begin(), tune_seconds_.end()) - tune_seconds_.begin());
-                        current_ = candidates_[best];
-                        seconds_per_pass_ = tune_seconds_[best] * static_cast<double>(candidates_.size()) /
-                                            static_cast<double>(schedule_.size());
+                        next_tuning_round();
                     }
                 }
             }
@@ -482,11 +476,61 @@ private:
         int block = 128;
     };
     variant current_{};
-    std::vector<variant> candidates_;   // the autotuner's
-    std::vector<std::size_t> schedule_; // which candidate renders each of the first full passes
-    std::size_t tuned_ = 0;             // passes of the schedule rendered so far
-    std::vector<double> tune_seconds_;  // per candidate, its passes' seconds
+    struct probe
+    {
+        std::size_t candidate;
+        int passes; // at most: a launch never crosses a round of strata or the render's target
+    };
+    std::vector<variant> candidates_;  // the autotuner's
+    std::vector<probe> schedule_;      // the first full launches: which candidate renders each, how many passes
+    std::size_t tuned_ = 0;            // launches of the schedule rendered so far
+    int tuning_round_ = 0;
+    std::vector<double> tune_seconds_; // per candidate, the sum of its probes' seconds per pass this round
     bool tuning() const { return tuned_ < schedule_.size(); }
+
+    // After the screen (one pass each), the three fastest candidates are timed again on launches of several passes
+    // (about a fifth of a planned launch: the steady state, not the tail), in a forward then a backward sweep so that
+    // a drift in the timings (clocks, caches warming) weighs on all of them alike; then the fastest renders the rest
+    void next_tuning_round()
+    {
+        std::vector<std::size_t> order(candidates_.size());
+        std::iota(order.begin(), order.end(), std::size_t{0});
+        std::vector<double> per_pass(candidates_.size(), std::numeric_limits<double>::infinity());
+        std::vector<int> probes(candidates_.size(), 0);
+        for (const probe& p : schedule_)
+        {
+            ++probes[p.candidate];
+        }
+        for (std::size_t k = 0; k < candidates_.size(); ++k)
+        {
+            if (probes[k] > 0)
+            {
+                per_pass[k] = tune_seconds_[k] / probes[k];
+            }
+        }
+        std::stable_sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) { return per_pass[a] < per_pass[b]; });
+        if (tuning_round_ == 0 && candidates_.size() > 1)
+        {
+            tuning_round_ = 1;
+            const std::size_t finalists = std::min<std::size_t>(3, candidates_.size());
+            const int passes = static_cast<int>(
+                std::clamp(0.2 * launch_seconds_ / std::max(per_pass[order[0]], 1e-9), 1.0, 16.0));
+            schedule_.clear();
+            tuned_ = 0;
+            for (std::size_t k = 0; k < finalists; ++k)
+            {
+                schedule_.push_back({order[k], passes});
+            }
+            for (std::size_t k = finalists; k-- > 0;)
+            {
+                schedule_.push_back({order[k], passes});
+            }
+            tune_seconds_.assign(candidates_.size(), 0.0);
+            return;
+        }
+        current_ = candidates_[order[0]];
+        seconds_per_pass_ = per_pass[order[0]];
+    }
     std::unique_ptr<wavefront<Real>> wavefront_; // made at the first launch that needs it in a view
     double launch_seconds_ = 0.05;
     rt_camera_desc camera_{};
