@@ -62,7 +62,7 @@ def _load() -> C.CDLL:
     lib.twin_brute_force.restype = C.c_int
     lib.twin_brute_force.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
                                      P(C.c_float), P(C.c_int32), P(C.c_uint8), C.c_int64, P(C.c_int32),
-                                     P(C.c_float)]
+                                     P(C.c_float), P(C.c_int32)]
     return lib
 
 
@@ -201,8 +201,9 @@ def twin_brute_force(name: str, origins, directions, times, from_ids, inward, wi
     n = len(tm)
     ids = np.zeros(n, np.int32)
     t = np.zeros(n, np.float32)
+    tied = np.zeros(n, np.int32)
     if L.twin_brute_force(name.encode(), width, spp, earth_width, _ptr(o, C.c_float), _ptr(d, C.c_float),
                           _ptr(tm, C.c_float), _ptr(f, C.c_int32), _ptr(inw, C.c_uint8), n, _ptr(ids, C.c_int32),
-                          _ptr(t, C.c_float)):
+                          _ptr(t, C.c_float), _ptr(tied, C.c_int32)):
         raise RuntimeError(L.twin_last_error().decode())
-    return ids, t
+    return ids, t, tied

@@ -3,9 +3,13 @@
 // and its closest-hit traversal (src/gpu/bvh.cuh:483-706, src/gpu/sphere.cuh:70-117, src/gpu/quad.cuh:33-84). It is the
 // bit-exact oracle for the GPU's closest-hit ids and random numbers.
 //
-// Two choices are fixed here where the reference leaves them to its tools, and the HIP kernels make the same ones:
+// Three choices are fixed here where the reference leaves them to its tools or its walk, and the HIP kernels make the
+// same ones:
 //   - the BVH builder sorts with std::stable_sort (the reference's std::ranges::sort leaves the order of equal keys to
 //     the standard library);
+//   - ties: of two primitives hit at exactly the same distance the one of lower index in the scene is the closest hit
+//     (`better` below), where the reference keeps the one its walk happens to meet first (spheres) or last (quads), so
+//     the closest hit doesn't depend on the tree or the walk, and the brute-force search finds the same one;
 //   - multiply-adds: the reference's float code is compiled by nvcc, which fuses a·b + c into one fma wherever it
 //     chooses. Here every fusion is written out and nothing else is fused (built with -ffp-contract=off):
 //       dot(u, v)        = fma(u.x, v.x, fma(u.y, v.y, u.z·v.z))
@@ -200,6 +204,7 @@ struct tree
     std::vector<int> materials;
     int moving_base = 0, quad_base = 0;
     std::vector<int> sphere_ids, quad_ids;
+    std::vector<int> ranks; // by global id: the primitive's index in the scene (spheres, then quads)
     int depth = 0;
 };
 
@@ -254,6 +259,15 @@ public:
         if (!order.empty())
         {
             add_subtree(order, 1);
+        }
+        t_.ranks.resize(inputs_.size());
+        for (std::size_t k = 0; k < t_.sphere_ids.size(); ++k)
+        {
+            t_.ranks[static_cast<std::size_t>(t_.sphere_ids[k])] = static_cast<int>(k);
+        }
+        for (std::size_t k = 0; k < t_.quad_ids.size(); ++k)
+        {
+            t_.ranks[static_cast<std::size_t>(t_.quad_ids[k])] = static_cast<int>(t_.sphere_ids.size() + k);
         }
         return std::move(t_);
     }
@@ -388,8 +402,9 @@ inline bool sphere_roots(const sphere& s, const ray& r, float inverse_a, float& 
     far = h + half_chord;
     return true;
 }
+// closed: accept t = t_max too (the closest-hit walk's tie rule below); the reference's test is open
 inline bool hit_sphere(const sphere& s, const ray& r, float inverse_a, float t_min, float t_max, bool far_only,
-                       float& t, bool& entering)
+                       float& t, bool& entering, bool closed = false)
 {
     float near, far;
     if (!sphere_roots(s, r, inverse_a, near, far))
@@ -402,7 +417,7 @@ inline bool hit_sphere(const sphere& s, const ray& r, float inverse_a, float t_m
     {
         t = far;
     }
-    return t_min < t && t < t_max;
+    return t_min < t && (t < t_max || (closed && t == t_max));
 }
 inline bool hit_quad(const quad& s, const ray& r, float t_min, float t_max, float& t, bool& front)
 {
@@ -435,6 +450,16 @@ struct hit
     float t = INFINITY;
     bool entering = false;
 };
+
+// Ties: a primitive hit at exactly the closest distance so far replaces the closest one when its index in the scene is
+// lower. The reference keeps whichever its walk meets first (a sphere) or last (a quad), which depends on the walk's
+// order; this rule makes the closest hit a function of the ray and the scene alone, the same for any tree and any
+// walk, and equal to the brute-force search's. Away from exact ties the two rules agree.
+inline bool better(const tree& tr, int id, float t, const hit& closest)
+{
+    return t < closest.t || (t == closest.t && tr.ranks[static_cast<std::size_t>(id)] <
+                                                   tr.ranks[static_cast<std::size_t>(closest.index)]);
+}
 
 struct counters
 {
@@ -483,7 +508,7 @@ inline hit closest_hit(const tree& tr, const ray& r, int from, bool inward, coun
                     }
                     float t;
                     bool front;
-                    if (hit_quad(tr.quads[i], r, t_min, closest.t, t, front))
+                    if (hit_quad(tr.quads[i], r, t_min, closest.t, t, front) && better(tr, id, t, closest))
                     {
                         closest = {id, t, front};
                     }
@@ -504,7 +529,8 @@ inline hit closest_hit(const tree& tr, const ray& r, int from, bool inward, coun
                     const sphere s = moving ? tr.moving[i].at(r.time) : tr.spheres[i];
                     float t;
                     bool entering;
-                    if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering))
+                    if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering, true) &&
+                        better(tr, id, t, closest))
                     {
                         closest = {id, t, entering};
                     }
@@ -533,7 +559,7 @@ inline hit closest_hit(const tree& tr, const ray& r, int from, bool inward, coun
         while (n < 0 && size > 0)
         {
             const entry e = stack[--size];
-            if (e.t < closest.t)
+            if (e.t <= closest.t) // a box entered at exactly the closest distance may hold a tie
             {
                 n = e.node;
             }
@@ -661,10 +687,11 @@ int twin_closest_hit(const char* name, int width, int spp, int earth_width, cons
 }
 
 // Brute force over every primitive of the twin's tree: the closest hit as a loop over all of them (the reference's
-// T7 check, tests/lamp_tests.cu:1618-1660), to pin the traversal itself. Same arguments as twin_closest_hit.
+// T7 check, tests/lamp_tests.cu:1618-1660), to pin the traversal itself. Same arguments as twin_closest_hit, and
+// `tied` (may be null): per ray, how many primitives are hit at exactly the closest distance.
 int twin_brute_force(const char* name, int width, int spp, int earth_width, const float* origins,
                      const float* directions, const float* times, const std::int32_t* from, const std::uint8_t* inward,
-                     std::int64_t n, std::int32_t* index, float* t)
+                     std::int64_t n, std::int32_t* index, float* t, std::int32_t* tied)
 {
     try
     {
@@ -688,7 +715,7 @@ int twin_brute_force(const char* name, int width, int spp, int earth_width, cons
                 const sphere s = id < tr.moving_base ? tr.spheres[i] : tr.moving[i - tr.moving_base].at(r.time);
                 float tt;
                 bool e;
-                if (hit_sphere(s, r, inverse_a, 0.001f, best.t, own, tt, e))
+                if (hit_sphere(s, r, inverse_a, 0.001f, best.t, own, tt, e, true) && better(tr, id, tt, best))
                 {
                     best = {id, tt, e};
                 }
@@ -702,13 +729,40 @@ int twin_brute_force(const char* name, int width, int spp, int earth_width, cons
                 }
                 float tt;
                 bool f;
-                if (hit_quad(tr.quads[i], r, 0.001f, best.t, tt, f))
+                if (hit_quad(tr.quads[i], r, 0.001f, best.t, tt, f) && better(tr, id, tt, best))
                 {
                     best = {id, tt, f};
                 }
             }
             index[k] = best.index;
             t[k] = best.t;
+            if (tied && best.index >= 0)
+            {
+                // a second sweep with the closest distance known: every primitive hit exactly there
+                int count = 0;
+                for (std::size_t i = 0; i < tr.spheres.size() + tr.moving.size(); ++i)
+                {
+                    const int id = static_cast<int>(i);
+                    const bool own = id == from[k];
+                    const sphere s = id < tr.moving_base ? tr.spheres[i] : tr.moving[i - tr.moving_base].at(r.time);
+                    float tt;
+                    bool e;
+                    count += (!own || inward[k]) && hit_sphere(s, r, inverse_a, 0.001f, best.t, own, tt, e, true) &&
+                             tt == best.t;
+                }
+                for (std::size_t i = 0; i < tr.quads.size(); ++i)
+                {
+                    float tt;
+                    bool f;
+                    count += tr.quad_base + static_cast<int>(i) != from[k] &&
+                             hit_quad(tr.quads[i], r, 0.001f, best.t, tt, f) && tt == best.t;
+                }
+                tied[k] = count;
+            }
+            else if (tied)
+            {
+                tied[k] = 0;
+            }
         }
         return 0;
     }

@@ -129,6 +129,7 @@ public:
     bvh_arrays<Real> build() &&
     {
         out_.prim_materials.resize(inputs_.size());
+        out_.prim_ranks.resize(inputs_.size());
         out_.sphere_frames.resize(static_cast<std::size_t>(out_.quad_base));
         std::vector<int> order(inputs_.size());
         std::iota(order.begin(), order.end(), 0);
@@ -245,15 +246,18 @@ private:
             id = out_.triangle_base + static_cast<int>(out_.triangles.size());
             out_.triangles.push_back(triangles_[in.index]);
             out_.prim_materials[id] = in.material;
+            out_.prim_ranks[id] = static_cast<int32_t>(out_.sphere_ids.size() + out_.quad_ids.size()) + in.index;
             return;
         default:
             id = out_.quad_base + static_cast<int>(out_.quads.size());
             out_.quads.push_back(quads_[in.index]);
             out_.prim_materials[id] = in.material;
+            out_.prim_ranks[id] = static_cast<int32_t>(out_.sphere_ids.size()) + in.source;
             out_.quad_ids[in.source] = id;
             return;
         }
         out_.prim_materials[id] = in.material;
+        out_.prim_ranks[id] = in.source;
         out_.sphere_frames[id] = in.frame;
         out_.sphere_ids[in.source] = id;
     }

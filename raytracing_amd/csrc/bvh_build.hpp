@@ -35,6 +35,7 @@ struct bvh_arrays
     std::vector<dev::quad_rec<Real>> quads;
     std::vector<dev::triangle_rec<Real>> triangles;
     std::vector<int32_t> prim_materials;
+    std::vector<int32_t> prim_ranks; // by global id: the primitive's index in the scene (spheres, quads, triangles)
     std::vector<dev::sphere_frame<Real>> sphere_frames;
     int32_t moving_base = 0;
     int32_t quad_base = 0;

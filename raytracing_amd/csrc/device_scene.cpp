@@ -545,6 +545,7 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
     moving_ = device_array<dev::moving_sphere_rec<Real>>(tree_.moving_spheres);
     quads_ = device_array<dev::quad_rec<Real>>(tree_.quads);
     prim_materials_ = device_array<int32_t>(tree_.prim_materials);
+    prim_ranks_ = device_array<int32_t>(tree_.prim_ranks);
     frames_ = device_array<dev::sphere_frame<Real>>(tree_.sphere_frames);
     materials_ = device_array<dev::material_rec<Real>>(materials);
     textures_ = device_array<dev::texture_rec<Real>>(textures);
@@ -556,6 +557,7 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
     lights_ = device_array<dev::light_rec<Real>>(lights);
     triangles_ = device_array<dev::triangle_rec<Real>>(tree_.triangles);
     bytes_ = bytes_of(nodes_) + bytes_of(spheres_) + bytes_of(moving_) + bytes_of(quads_) + bytes_of(prim_materials_) +
+             bytes_of(prim_ranks_) +
              bytes_of(frames_) + bytes_of(materials_) + bytes_of(textures_) + bytes_of(images_) + bytes_of(texels_) +
              bytes_of(noises_) + bytes_of(media_) + bytes_of(boundary_quads_) + bytes_of(lights_) +
              bytes_of(triangles_);
@@ -611,6 +613,7 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
     v.triangle_base = tree_.triangle_base;
     v.media_base = static_cast<int32_t>(tree_.prim_materials.size());
     v.prim_materials = prim_materials_.data();
+    v.prim_ranks = prim_ranks_.data();
     v.sphere_frames = frames_.data();
     v.materials = materials_.data();
     v.textures = textures_.data();

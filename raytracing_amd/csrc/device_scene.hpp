@@ -103,6 +103,7 @@ private:
     device_array<dev::moving_sphere_rec<Real>> moving_;
     device_array<dev::quad_rec<Real>> quads_;
     device_array<int32_t> prim_materials_;
+    device_array<int32_t> prim_ranks_;
     device_array<dev::sphere_frame<Real>> frames_;
     device_array<dev::material_rec<Real>> materials_;
     device_array<dev::texture_rec<Real>> textures_;

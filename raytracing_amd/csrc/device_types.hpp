@@ -290,6 +290,7 @@ struct scene_view
     int32_t triangle_base;
     int32_t media_base;
     const int32_t* prim_materials;
+    const int32_t* prim_ranks; // by global id: the primitive's index in the scene, which breaks exact ties
     const sphere_frame<Real>* sphere_frames;
     const material_rec<Real>* materials;
     const texture_rec<Real>* textures;

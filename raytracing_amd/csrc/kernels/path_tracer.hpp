@@ -260,8 +260,10 @@ __device__ inline bool sphere_roots(const sphere<Real>& s, const ray<Real>& r, R
 
 // The nearest root in (t_min, t_max), the far one only for a ray leaving this sphere inward (src/gpu/sphere.cuh:99-117)
 template <typename Real>
+// closed: accept t = t_max too (the closest-hit walk's tie rule, `better`); the reference's test is open, as the shadow
+// rays' stays
 __device__ inline bool hit_sphere(const sphere<Real>& s, const ray<Real>& r, Real inverse_a, Real t_min, Real t_max,
-                                  bool far_only, Real& t, bool& entering)
+                                  bool far_only, Real& t, bool& entering, bool closed = false)
 {
     Real near, far;
     if (!sphere_roots(s, r, inverse_a, near, far))
@@ -274,7 +276,7 @@ __device__ inline bool hit_sphere(const sphere<Real>& s, const ray<Real>& r, Rea
     {
         t = far;
     }
-    return t_min < t && t < t_max;
+    return t_min < t && (t < t_max || (closed && t == t_max));
 }
 
 template <typename Real>
@@ -451,6 +453,17 @@ __device__ inline lds_stack stack_in(unsigned char* lds, int entries, int thread
             reinterpret_cast<uint16_t*>(lds + stack_ref_bytes(entries, threads)) + tid, threads};
 }
 
+// Ties: a primitive hit at exactly the closest distance so far (every test accepts t = t_max in the closest-hit walk)
+// replaces the closest one when its index in the scene is lower. The reference keeps whichever its walk meets first
+// (a sphere) or last (a quad), which depends on the walk's order; with this rule the closest hit is a function of the
+// ray and the scene alone, the same for any tree and for the four-wide walk as for the twin's binary one. The ranks
+// are read only on an exact tie.
+template <typename Real>
+__device__ inline bool better(const scene_view<Real>& sc, int id, Real t, const hit<Real>& closest)
+{
+    return t < closest.t || sc.prim_ranks[id] < sc.prim_ranks[closest.index];
+}
+
 // The primitives of a leaf against the ray, the closest hit so far updated (src/gpu/bvh.cuh:612-680)
 template <typename Real, bool AllKinds, typename Tally>
 __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Real>& r, int32_t ref, int from,
@@ -472,7 +485,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
             tl.add(tally_triangle);
             Real t;
             bool front;
-            if (hit_triangle(tri, r, t_min, closest.t, t, front))
+            if (hit_triangle(tri, r, t_min, closest.t, t, front) && better(sc, id, t, closest))
             {
                 closest = {id, t, front};
             }
@@ -491,7 +504,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
             tl.add(tally_quad);
             Real t;
             bool front;
-            if (hit_quad(face, r, t_min, closest.t, t, front))
+            if (hit_quad(face, r, t_min, closest.t, t, front) && better(sc, id, t, closest))
             {
                 closest = {id, t, front};
             }
@@ -512,7 +525,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
             tl.add(moving ? tally_moving : tally_sphere);
             Real t;
             bool entering;
-            if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering))
+            if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering, true) && better(sc, id, t, closest))
             {
                 closest = {id, t, entering};
             }
@@ -582,14 +595,15 @@ __device__ inline bool descend(const scene_view<Real>& sc, int32_t& ref, vec3<Re
     return false;
 }
 
-// The next stack entry the ray may enter before the closest hit so far, or false when there is none
+// The next stack entry the ray may enter no farther than the closest hit so far (at exactly that distance it may hold
+// a tie), or false when there is none
 template <typename Real>
 __device__ inline bool pop(const lds_stack& stack, int& size, Real t_max, int32_t& ref)
 {
     while (size > 0)
     {
         --size;
-        if (stack_distance_value(stack.ts[size * stack.stride]) < t_max)
+        if (stack_distance_value(stack.ts[size * stack.stride]) <= t_max)
         {
             ref = stack.refs[size * stack.stride];
             return true;
