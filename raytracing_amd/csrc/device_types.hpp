@@ -316,6 +316,13 @@ enum kernel_mode : int32_t
     mode_quality = 2,
 };
 
+// One path's sample, staged by the megakernel until its pixel's samples are added in pass order (16 bytes in float)
+template <typename Real>
+struct alignas(4 * sizeof(Real)) staged_sample
+{
+    Real x, y, z, unused;
+};
+
 // The work of one launch: passes [first_pass, first_pass + pass_count) of rows [first_row, first_row + row_count),
 // within one round of strata
 struct launch_work

@@ -943,10 +943,9 @@ __device__ inline bool medium_collision(const medium_rec<Real>& m, const quad_re
 // The nearest surface, or a particle of smoke in front of it: the media are tested after the surfaces, in their list's
 // order, each against the closest event so far (src/gpu/camera.cuh:259-284)
 template <typename Real, typename Tally>
-__device__ inline hit<Real> closest_event(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
-                                          pcg32& g, const lds_stack& stack, Tally& tl)
+__device__ inline hit<Real> media_events(const scene_view<Real>& sc, const ray<Real>& r, hit<Real> closest, pcg32& g,
+                                         Tally& tl)
 {
-    hit<Real> closest = closest_hit<Real, true>(sc, r, from, inward, stack, tl);
     if (sc.medium_count == 0)
     {
         return closest;
@@ -965,6 +964,12 @@ __device__ inline hit<Real> closest_event(const scene_view<Real>& sc, const ray<
         }
     }
     return closest;
+}
+template <typename Real, typename Tally>
+__device__ inline hit<Real> closest_event(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
+                                          pcg32& g, const lds_stack& stack, Tally& tl)
+{
+    return media_events(sc, r, closest_hit<Real, true>(sc, r, from, inward, stack, tl), g, tl);
 }
 
 // --- the surface at a hit (src/gpu/camera.cuh:155-253) -------------------------------------------------------------------
@@ -1237,10 +1242,9 @@ __device__ inline bounce diffuse_bounce(const scene_view<Real>& sc, bool matte, 
 // One ray of a first-book path: spheres that stand still, lambertian of one color, metal, glass, and the sky.
 // True when the sample is complete (its color in `sample`).
 template <typename Real, typename Tally>
-__device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                        pcg32& g, vec3<Real>& sample, const lds_stack& stack, Tally& tl)
+__device__ inline bool first_book_after_walk(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
+                                             pcg32& g, vec3<Real>& sample, const hit<Real>& h, Tally& tl)
 {
-    const hit<Real> h = closest_hit<Real, false>(sc, p.r, p.from, p.inward, stack, tl);
     if (h.index < 0)
     {
         sample = p.throughput * sky(cam, p.r.direction);
@@ -1269,6 +1273,13 @@ __device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& s
     p.from = h.index;
     p.inward = dot(direction, outward) < Real(0);
     return false;
+}
+template <typename Real, typename Tally>
+__device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
+                                        pcg32& g, vec3<Real>& sample, const lds_stack& stack, Tally& tl)
+{
+    return first_book_after_walk(p, sc, cam, g, sample, closest_hit<Real, false>(sc, p.r, p.from, p.inward, stack, tl),
+                                 tl);
 }
 
 // What a hit does to the path once it is known (the part of trace_ray after closest_event)
@@ -1325,16 +1336,23 @@ __device__ inline bool shade(path<Real>& p, const scene_view<Real>& sc, const hi
 
 // One ray of a path with every feature of the three books (src/gpu/camera.cuh:364-437)
 template <typename Real, typename Tally>
-__device__ inline bool trace_all_books(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                       pcg32& g, vec3<Real>& sample, const lds_stack& stack, Tally& tl)
+__device__ inline bool all_books_after_walk(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
+                                            pcg32& g, vec3<Real>& sample, const hit<Real>& surface_hit, Tally& tl)
 {
-    const hit<Real> h = closest_event(sc, p.r, p.from, p.inward, g, stack, tl);
+    const hit<Real> h = media_events(sc, p.r, surface_hit, g, tl);
     if (h.index < 0)
     {
         sample = p.throughput * sky(cam, p.r.direction);
         return true;
     }
     return shade(p, sc, h, g, sample, tl);
+}
+template <typename Real, typename Tally>
+__device__ inline bool trace_all_books(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
+                                       pcg32& g, vec3<Real>& sample, const lds_stack& stack, Tally& tl)
+{
+    return all_books_after_walk(p, sc, cam, g, sample, closest_hit<Real, true>(sc, p.r, p.from, p.inward, stack, tl),
+                                tl);
 }
 
 // A NaN channel of a sample counts as 0 (src/gpu/renderer.cu:311-316)

@@ -202,7 +202,9 @@ public:
             {
                 const int64_t left_in_round = round_size - work.first_pass;
                 int64_t count = seconds_per_pass_ > 0 ? static_cast<int64_t>(launch_seconds_ / seconds_per_pass_) : 1;
-                count = std::max<int64_t>(1, std::min({count, left_in_round, target_passes - pass}));
+                // and no more passes than the megakernel's staging holds (one kernel a launch)
+                const int64_t staged = current_.wavefront ? count : max_staged_passes();
+                count = std::max<int64_t>(1, std::min({count, staged, left_in_round, target_passes - pass}));
                 work.pass_count = tuning() ? static_cast<int32_t>(std::min<int64_t>(schedule_[tuned_].passes, std::min(
                                                  left_in_round, target_passes - pass)))
                                            : static_cast<int32_t>(count);
@@ -425,8 +427,10 @@ public:
         check(hipMemsetAsync(scratch.data(), 0, scratch.bytes(), stream_.get()), "clearing scratch");
         check(hipMemsetAsync(counts.data(), 0, counts.bytes(), stream_.get()), "clearing counters");
         const dev::launch_work work{0, static_cast<int32_t>(passes), 0, camera_.image_height};
+        ensure_staging();
         check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), options_.seed, work, scratch.data(),
-                                      counts.data(), 16, 8, stream_.get(), counts.data() + 2),
+                                      staging_.data(), staging_samples_, counts.data(), 16, 8, stream_.get(),
+                                      counts.data() + 2),
               "launching the counting kernel");
         check(hipStreamSynchronize(stream_.get()), "counting traffic");
         unsigned long long host[2 + RT_TRAFFIC_EVENTS] = {};
@@ -456,7 +460,7 @@ public:
 
     std::size_t device_bytes() const override
     {
-        return dscene_.bytes() + sums_.bytes() + (wavefront_ ? wavefront_->bytes() : 0);
+        return dscene_.bytes() + sums_.bytes() + staging_.bytes() + (wavefront_ ? wavefront_->bytes() : 0);
     }
 
 private:
@@ -537,6 +541,30 @@ private:
     dev::camera_rec<Real> cam_{};
     device_array<double> sums_;
     device_array<unsigned long long> rays_;
+    // the megakernel's staged samples: one per grid thread per pass, for up to staging_passes_ passes a kernel
+    static constexpr std::size_t staging_bytes_cap = std::size_t{1} << 30;
+    device_array<dev::staged_sample<Real>> staging_;
+    std::size_t staging_samples_ = 0;
+    int staging_passes_ = 0;
+    int64_t max_staged_passes() const
+    {
+        const std::size_t per_pass = megakernel_staging_per_pass(camera_.image_width, camera_.image_height);
+        return static_cast<int64_t>(
+            std::clamp<std::size_t>(staging_bytes_cap / (per_pass * sizeof(dev::staged_sample<Real>)), 1, 1024));
+    }
+    void ensure_staging()
+    {
+        if (staging_samples_ != 0)
+        {
+            return;
+        }
+        const std::size_t per_pass = megakernel_staging_per_pass(camera_.image_width, camera_.image_height);
+        const std::size_t passes =
+            std::clamp<std::size_t>(staging_bytes_cap / (per_pass * sizeof(dev::staged_sample<Real>)), 1, 1024);
+        staging_ = device_array<dev::staged_sample<Real>>(per_pass * passes);
+        staging_samples_ = per_pass * passes;
+        staging_passes_ = static_cast<int>(passes);
+    }
     device_array<uint8_t> rgba_;
     bool has_view_ = false;
     accumulation state_{};
@@ -569,6 +597,11 @@ private:
         check(hipMemsetAsync(sums_.data(), 0, sums_.bytes(), stream_.get()), "clearing the image");
         check(hipMemsetAsync(rays_.data(), 0, rays_.bytes(), stream_.get()), "clearing the counters");
         wavefront_.reset(); // its pool is the view's size
+        if (staging_samples_ != megakernel_staging_per_pass(camera.image_width, camera.image_height) * staging_passes_)
+        {
+            staging_ = device_array<dev::staged_sample<Real>>(); // the megakernel's, sized again at its next launch
+            staging_samples_ = 0;
+        }
         check(hipStreamSynchronize(stream_.get()), "starting the view");
         state_ = {};
         kernel_seconds_ = 0;
@@ -599,8 +632,10 @@ private:
         }
         else
         {
-            check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), rays_.data(), 16,
-                                          std::max(1, v.block / 16), stream_.get(), nullptr, v.waves),
+            ensure_staging();
+            check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), staging_.data(),
+                                          staging_samples_, rays_.data(), 16, std::max(1, v.block / 16),
+                                          stream_.get(), nullptr, v.waves),
                   "launching the kernel");
         }
         stop_.record(stream_.get());
