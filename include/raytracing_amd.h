@@ -296,9 +296,18 @@ typedef struct rt_scene_overrides
 
 typedef struct rt_scene_handle rt_scene_handle;
 
+/* name: a built-in scene (rt_scene_names), or a glTF 2.0 model's file, FILE.gltf or FILE.glb, which renders in a
+ * studio: its triangle meshes placed by its nodes, with their metallic-roughness factors, on a ground under a seeded
+ * sky with a sun, framed by the camera (as RayTracingGPU takes a model's file for its scene, src/main_gpu.cpp:141-143,
+ * whose models the reference renders through OptiX, src/gltf.ixx:34). A model the loader refuses (a file cut short, a
+ * required extension, an index past its vertices, a cycle of nodes...) is RT_ERROR_RUNTIME, one without a triangle
+ * RT_ERROR_INVALID_ARGUMENT, with the reason in rt_last_error(). */
 int rt_scene_build(const char* name, const rt_scene_overrides* overrides, rt_scene_handle** out);
 const rt_scene* rt_scene_get(const rt_scene_handle* handle);
 int64_t rt_scene_passes(const rt_scene_handle* handle); /* sqrt_spp^2, or the samples asked for with strata */
+/* A model's warnings, one line each ("" for a built-in scene): what the file holds that isn't rendered (textures,
+ * points and lines, skins). Valid while the handle is. */
+const char* rt_scene_warnings(const rt_scene_handle* handle);
 void rt_scene_free(rt_scene_handle* handle);
 /* The names of the built-in scenes, separated by commas */
 const char* rt_scene_names(void);

@@ -185,6 +185,7 @@ _SIGNATURES = {
     "rt_scene_build": (C.c_int, [C.c_char_p, _P(SceneOverrides), _P(C.c_void_p)]),
     "rt_scene_get": (_P(Scene), [C.c_void_p]),
     "rt_scene_passes": (C.c_int64, [C.c_void_p]),
+    "rt_scene_warnings": (C.c_char_p, [C.c_void_p]),
     "rt_scene_free": (None, [C.c_void_p]),
     "rt_session_create": (C.c_int, [_P(Scene), _P(SessionOptions), _P(C.c_void_p)]),
     "rt_session_destroy": (None, [C.c_void_p]),
@@ -227,6 +228,7 @@ for _name, (_res, _args) in _SIGNATURES.items():
 
 RT_OK, RT_ERROR_RUNTIME, RT_ERROR_INVALID_ARGUMENT, RT_ERROR_LOGIC = 0, 1, 2, 3
 SAMPLING = {"uniform": 0, "material": 1, "lights": 2, "mixture": 3, "nee": 4, "mis": 5}
+MATERIAL_KINDS = {"lambertian": 0, "metal": 1, "dielectric": 2, "diffuse_light": 3, "isotropic": 4}
 INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2}
 TREES = {"default": 0, "median": 1, "sah": 2}
 
@@ -255,7 +257,9 @@ def scene_names() -> list[str]:
 
 
 class BuiltScene:
-    """A scene built by name and flattened (build_gpu_scene): ``desc`` is the rt_scene view over its tables."""
+    """A scene built by name and flattened (build_gpu_scene): ``desc`` is the rt_scene view over its tables. A name that
+    ends in .gltf or .glb is a glTF 2.0 model's file, rendered in a studio; ``warnings`` lists what the model holds
+    that isn't rendered."""
 
     def __init__(self, name: str, width: int = 0, height: int = 0, samples: int = 0, strata: int = 0,
                  sampling: Optional[str] = None, max_depth: int = 0, earth_width: int = 0, earth_seed: int = 0):
@@ -267,6 +271,7 @@ class BuiltScene:
         self.name = name
         self.desc: Scene = _lib.rt_scene_get(handle).contents
         self.passes: int = _lib.rt_scene_passes(handle)
+        self.warnings: list = [w for w in _lib.rt_scene_warnings(handle).decode().split("\n") if w]
 
     @property
     def width(self) -> int:

@@ -20,6 +20,7 @@ struct rt_scene_handle
 {
     rt::built_scene built;
     rt_scene view{};
+    std::string warnings; // the model's warnings, one line each
 };
 
 struct rt_session
@@ -98,6 +99,10 @@ int rt_scene_build(const char* name, const rt_scene_overrides* overrides, rt_sce
         auto h = std::make_unique<rt_scene_handle>();
         h->built = rt::build_scene(name, o);
         h->view = h->built.tables.view();
+        for (const std::string& w : h->built.warnings)
+        {
+            h->warnings += w + "\n";
+        }
         *out = h.release();
     });
 }
@@ -105,6 +110,8 @@ int rt_scene_build(const char* name, const rt_scene_overrides* overrides, rt_sce
 const rt_scene* rt_scene_get(const rt_scene_handle* handle) { return handle ? &handle->view : nullptr; }
 
 int64_t rt_scene_passes(const rt_scene_handle* handle) { return handle ? handle->built.passes : 0; }
+
+const char* rt_scene_warnings(const rt_scene_handle* handle) { return handle ? handle->warnings.c_str() : ""; }
 
 void rt_scene_free(rt_scene_handle* handle) { delete handle; }
 

@@ -1,6 +1,6 @@
-// rt_gpu: the RayTracingGPU command line (src/main_gpu.cpp:32-294,362-640) over the C ABI: a built-in scene, its
-// overrides, a render, the numbers the reference prints, and the image as PFM (linear floats) or PNG (the books'
-// bytes).
+// rt_gpu: the RayTracingGPU command line (src/main_gpu.cpp:32-294,362-640) over the C ABI: a built-in scene or a glTF
+// 2.0 model's file (FILE.glb or FILE.gltf, rendered in a studio), its overrides, a render, the numbers the reference
+// prints, and the image as PFM (linear floats) or PNG (the books' bytes).
 //
 //   rt_gpu [scene] [-w N] [--height N] [-s N] [--strata K] [-p uniform|material|lights|mixture|nee|mis]
 //          [--seed N] [--fp64] [--integrator auto|megakernel|wavefront] [--launch-ms N] [-o FILE.pfm|FILE.png]...
@@ -155,7 +155,8 @@ int main(int argc, char** argv)
         {
             std::printf("Usage: rt_gpu [scene] [-w N] [--height N] [-s N] [--strata K] [-p STRATEGY] [--seed N]\n"
                         "              [--fp64] [--integrator auto|megakernel|wavefront] [--launch-ms N] [-o FILE]\n"
-                        "Scenes: %s\n",
+                        "Scenes: %s\n"
+                        "Or a glTF 2.0 model, FILE.glb or FILE.gltf, rendered in a studio\n",
                         rt_scene_names());
             return 0;
         }
@@ -220,6 +221,11 @@ int main(int argc, char** argv)
         return fail("building the scene");
     }
     const rt_scene* s = rt_scene_get(h);
+    // what a model holds that isn't rendered, one line each, before it renders (src/main_gpu.cpp:489)
+    if (const char* warnings = rt_scene_warnings(h); warnings[0] != '\0')
+    {
+        std::printf("The model's parts that aren't rendered:\n%s", warnings);
+    }
     rt_device_info dev{};
     if (rt_query_device(&dev) != RT_OK)
     {

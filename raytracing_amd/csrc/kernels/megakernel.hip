@@ -134,6 +134,23 @@ __device__ inline wave_share share_of(const camera_rec<Real>& cam, const launch_
     return w;
 }
 
+// A staged sample is written once and read once: non-temporal stores and loads, so that the samples don't push the
+// tree and the scene's records out of L2 (with cached ones the L2 hit rate of C2 fell from 99 % to 89 %)
+template <typename Real>
+__device__ inline void stage(staged_sample<Real>* at, const vec3<Real>& s)
+{
+    typedef Real quad __attribute__((ext_vector_type(4)));
+    const quad v = {s.x, s.y, s.z, Real(0)};
+    __builtin_nontemporal_store(v, reinterpret_cast<quad*>(at));
+}
+template <typename Real>
+__device__ inline staged_sample<Real> unstage(const staged_sample<Real>* at)
+{
+    typedef Real quad __attribute__((ext_vector_type(4)));
+    const quad v = __builtin_nontemporal_load(reinterpret_cast<const quad*>(at));
+    return {v.x, v.y, v.z, Real(0)};
+}
+
 // Adds this lane's own pixel's staged samples to its double sums, in pass order (after the wave's loop: every lane of
 // the wave is past its last store)
 template <typename Real>
@@ -150,7 +167,7 @@ __device__ inline void add_samples(const wave_share& w, const camera_rec<Real>& 
     double red = sum[0], green = sum[1], blue = sum[2];
     for (int q = 0; q < work.pass_count; ++q)
     {
-        const staged_sample<Real> s = staged[static_cast<size_t>(q) * 64 + w.lane];
+        const staged_sample<Real> s = unstage(staged + static_cast<size_t>(q) * 64 + w.lane);
         red += without_nan(s.x);
         green += without_nan(s.y);
         blue += without_nan(s.z);
@@ -204,7 +221,7 @@ megakernel(const camera_rec<Real> cam, const scene_view<Real> sc, const uint64_t
                                    : trace_first_book(p, sc, cam, g, sample, stack, tl);
         if (done)
         {
-            staged[at] = {sample.x, sample.y, sample.z, Real(0)};
+            stage(staged + at, sample);
             at = -1;
         }
     }
@@ -252,7 +269,7 @@ quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc, cons
         vec3<float> sample;
         if (trace_quality(q, sc, cam, g, sample, stack, tl, shadow_rays))
         {
-            staged[at] = {sample.x, sample.y, sample.z, 0.0f};
+            stage(staged + at, sample);
             at = -1;
         }
     }

@@ -9,6 +9,8 @@
 // materials, the same Perlin tables.
 #include "scene_build.hpp"
 
+#include "gltf.hpp"
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -732,6 +734,76 @@ camera_settings furnace_camera()
     c.sky_bottom = {1, 1, 1};
     c.sky_top = {1, 1, 1};
     return c;
+}
+
+// A glTF model in a studio (RayTracingGPU's FILE.glb / FILE.gltf scene, src/main_gpu.cpp:141-143): its triangle meshes
+// placed by its nodes, on a gray ground under the seeded sky with a sun (the triangle scene's), framed by a camera that
+// looks at the model's bounds from the front, the right and above. Materials by their factors: metallic ones (>= 0.5) a
+// GGX conductor of the base color and roughness, the others a glossy coat over the base color, or plain lambertian at
+// roughness 1. 1280 wide (16:9), 64 samples, depth 16, light samples weighed by MIS.
+built build_model(const std::string& path, std::vector<std::string>& warnings)
+{
+    const gltf_model model = load_gltf(path);
+    warnings = model.warnings;
+    if (model.placements.empty())
+    {
+        std::string why = path + ": the model has no triangles to render";
+        for (const std::string& w : model.warnings)
+        {
+            why += "; " + w;
+        }
+        throw std::invalid_argument(why);
+    }
+    built s;
+    std::vector<material_ref> materials;
+    for (const gltf_material& m : model.materials)
+    {
+        const d3 base{m.base_color[0], m.base_color[1], m.base_color[2]};
+        materials.push_back(m.metallic >= 0.5  ? rough_metal(base, m.roughness)
+                            : m.roughness < 1 ? glossy(base, m.roughness)
+                                              : lambertian(base));
+    }
+    const material_ref fallback = rough_metal({1, 1, 1}, 1.0); // glTF's default material: metallic 1, roughness 1
+    for (const gltf_mesh& m : model.meshes)
+    {
+        s.meshes.push_back({m.positions, m.indices, m.material >= 0 ? materials[static_cast<std::size_t>(m.material)] : fallback});
+    }
+    // the bounds of every placed vertex
+    d3 lo{1e300, 1e300, 1e300}, hi{-1e300, -1e300, -1e300};
+    for (const gltf_placement& p : model.placements)
+    {
+        rt_mesh_instance_desc in{};
+        in.mesh = p.mesh;
+        std::copy(p.object_to_world, p.object_to_world + 12, in.object_to_world);
+        s.instances.push_back(in);
+        const std::vector<float>& pos = model.meshes[static_cast<std::size_t>(p.mesh)].positions;
+        const double* m = p.object_to_world;
+        for (std::size_t k = 0; k + 2 < pos.size(); k += 3)
+        {
+            const double x = pos[k], y = pos[k + 1], z = pos[k + 2];
+            const d3 w{m[0] * x + m[1] * y + m[2] * z + m[3], m[4] * x + m[5] * y + m[6] * z + m[7],
+                       m[8] * x + m[9] * y + m[10] * z + m[11]};
+            lo = {std::min(lo.x, w.x), std::min(lo.y, w.y), std::min(lo.z, w.z)};
+            hi = {std::max(hi.x, w.x), std::max(hi.y, w.y), std::max(hi.z, w.z)};
+        }
+    }
+    const d3 center = 0.5 * (lo + hi);
+    const double radius = std::max(0.5 * length(hi - lo), 1e-6);
+    const double ground = 40 * radius;
+    s.world.add(make_quad({center.x - ground / 2, lo.y, center.z - ground / 2}, {0, 0, ground}, {ground, 0, 0},
+                          lambertian(d3{0.5, 0.5, 0.5})));
+    s.camera.aspect_ratio = 16.0 / 9.0;
+    s.camera.image_width = 1280;
+    s.camera.samples_per_pixel = 64;
+    s.camera.max_depth = 16;
+    s.camera.sampling = RT_SAMPLING_MATERIAL;
+    s.camera.vfov = 35;
+    s.camera.lookat = center;
+    const d3 view{0.55, 0.35, 1.0};
+    s.camera.lookfrom = center + (3.2 * radius / length(view)) * view;
+    s.environment = environment_def{sun_sky(1024, 512, 7, 35, 60, 0.6, 10000), 0, 1};
+    s.combination = RT_COMBINE_POWER_HEURISTIC;
+    return s;
 }
 
 built build_named(const std::string& name, scene_random& rng, const scene_options& opts)
@@ -1516,7 +1588,8 @@ built_scene build_scene(const std::string& name, const rt_scene_overrides& overr
     }
     scene_options opts{overrides.earth_width, overrides.earth_seed};
     scene_random rng; // every build starts the stream over: the scene of a fresh process (src/gpu_scene.ixx:1355)
-    built s = build_named(name, rng, opts);
+    std::vector<std::string> warnings;
+    built s = is_model_path(name) ? build_model(name, warnings) : build_named(name, rng, opts);
     camera_settings& cam = s.camera;
     if (overrides.image_width > 0)
     {
@@ -1595,6 +1668,7 @@ built_scene build_scene(const std::string& name, const rt_scene_overrides& overr
     }
     t.combination = combination;
     t.relink();
+    result.warnings = std::move(warnings);
     result.passes = overrides.strata > 0 ? samples
                                          : static_cast<std::int64_t>(t.camera.sqrt_spp) * t.camera.sqrt_spp;
     return result;

@@ -24,10 +24,12 @@ struct image_pixels
 struct built_scene
 {
     scene_tables tables;
-    std::int64_t passes = 0; // sqrt_spp^2, or the samples asked for with strata
+    std::int64_t passes = 0;           // sqrt_spp^2, or the samples asked for with strata
+    std::vector<std::string> warnings; // a model's: what it holds that isn't rendered, one line each
 };
 
-// The scene called `name` with the overrides applied, flattened (build_gpu_scene, src/gpu_scene.ixx:1333-1445).
+// The scene called `name` with the overrides applied, flattened (build_gpu_scene, src/gpu_scene.ixx:1333-1445). A name
+// that ends in .gltf or .glb is a model's file, rendered in a studio (gltf.hpp).
 // Throws std::invalid_argument for an unknown name or options out of range, std::runtime_error for a scene that can't
 // be flattened.
 built_scene build_scene(const std::string& name, const rt_scene_overrides& overrides);
