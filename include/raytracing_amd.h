@@ -323,7 +323,7 @@ enum rt_precision
 enum rt_integrator
 {
     RT_INTEGRATOR_AUTO = 0,       /* the wavefront integrator where it is available */
-    RT_INTEGRATOR_MEGAKERNEL = 1, /* one thread per pixel, the whole path in one kernel */
+    RT_INTEGRATOR_MEGAKERNEL = 1, /* one thread per path, the whole path in one kernel */
     RT_INTEGRATOR_WAVEFRONT = 2,  /* per-bounce stages over queues of paths */
 };
 
@@ -398,6 +398,22 @@ int rt_session_pixels(rt_session* session, double* out, size_t count);
 int rt_session_sums(rt_session* session, double* out, size_t count);
 /* gpu::session::display: the books' bytes (sqrt, clamp 0.999, x256), RGBA, rows from the top */
 int rt_session_display_rgba8(rt_session* session, uint8_t* out, size_t count);
+
+/* Tone maps beyond the books' bytes (gpu::tone_map, src/gpu/renderer.h:282-303) */
+enum rt_tone_map
+{
+    RT_TONE_BOOKS = 0,   /* the books' bytes, as rt_session_display_rgba8 (no exposure) */
+    RT_TONE_NONE = 1,    /* the light times 2^exposure, clipped to [0, 1], the sRGB curve */
+    RT_TONE_AGX = 2,     /* AgX (Troy Sobotka's; Minimal AgX's matrices, a closed-form curve) */
+    RT_TONE_NEUTRAL = 3, /* Khronos PBR Neutral, then the sRGB curve */
+};
+#define RT_MAX_EXPOSURE 24.0 /* stops, either way */
+/* display(display_options) (src/gpu/renderer.h:624-635): each pixel's light (its sums / passes rounded to float, times
+ * 2^exposure) through the tone map, dithered with interleaved gradient noise and quantized, RGBA, rows from the top.
+ * RT_ERROR_INVALID_ARGUMENT for an exposure that isn't finite and within RT_MAX_EXPOSURE, for RT_TONE_BOOKS with an
+ * exposure other than 0, and for a tone map other than RT_TONE_BOOKS in a float64 session (the curves compute in
+ * float only), as the reference refuses them. */
+int rt_session_display_toned(rt_session* session, int32_t tone_map, double exposure, uint8_t* out, size_t count);
 /* gpu::session::rays, shadow_rays, kernel_seconds */
 int rt_session_counters(const rt_session* session, uint64_t* rays, uint64_t* shadow_rays, double* kernel_seconds);
 /* gpu::session::kernel */

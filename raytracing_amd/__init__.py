@@ -196,6 +196,7 @@ _SIGNATURES = {
     "rt_session_pixels": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_size_t]),
     "rt_session_sums": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_size_t]),
     "rt_session_display_rgba8": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_size_t]),
+    "rt_session_display_toned": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _P(C.c_uint8), C.c_size_t]),
     "rt_session_counters": (C.c_int, [C.c_void_p, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_double)]),
     "rt_session_kernel_info": (C.c_int, [C.c_void_p, _P(KernelInfo)]),
     "rt_render": (C.c_int, [_P(Scene), _P(RenderOptions), _P(C.c_double), C.c_size_t, _P(RenderStats)]),
@@ -231,6 +232,7 @@ SAMPLING = {"uniform": 0, "material": 1, "lights": 2, "mixture": 3, "nee": 4, "m
 MATERIAL_KINDS = {"lambertian": 0, "metal": 1, "dielectric": 2, "diffuse_light": 3, "isotropic": 4}
 INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2}
 TREES = {"default": 0, "median": 1, "sah": 2}
+TONE_MAPS = {"books": 0, "none": 1, "agx": 2, "neutral": 3}
 
 
 def library() -> C.CDLL:
@@ -374,9 +376,15 @@ class Session:
     def sums(self) -> np.ndarray:
         return self._image(_lib.rt_session_sums)
 
-    def display(self) -> np.ndarray:
+    def display(self, tone_map: str = "books", exposure: float = 0.0) -> np.ndarray:
+        """RGBA bytes: the books' (sqrt, clamp 0.999, x256) or a tone map's ("none", "agx", "neutral") at an exposure
+        in stops"""
         out = np.zeros((self.camera.image_height, self.camera.image_width, 4), dtype=np.uint8)
-        _check(_lib.rt_session_display_rgba8(self._h, out.ctypes.data_as(_P(C.c_uint8)), out.size))
+        if tone_map == "books" and exposure == 0:
+            _check(_lib.rt_session_display_rgba8(self._h, out.ctypes.data_as(_P(C.c_uint8)), out.size))
+        else:
+            op = TONE_MAPS[tone_map] if isinstance(tone_map, str) else int(tone_map)
+            _check(_lib.rt_session_display_toned(self._h, op, exposure, out.ctypes.data_as(_P(C.c_uint8)), out.size))
         return out
 
     def counters(self) -> tuple[int, int, float]:

@@ -209,6 +209,15 @@ int rt_session_display_rgba8(rt_session* session, uint8_t* out, size_t count)
     });
 }
 
+int rt_session_display_toned(rt_session* session, int32_t tone_map, double exposure, uint8_t* out, size_t count)
+{
+    return guarded([&] {
+        need(session, "session");
+        need(out, "out");
+        session->impl->display_toned(tone_map, exposure, out, count);
+    });
+}
+
 int rt_session_counters(const rt_session* session, uint64_t* rays, uint64_t* shadow_rays, double* kernel_seconds)
 {
     return guarded([&] {

@@ -140,6 +140,8 @@ int main(int argc, char** argv)
     ov.sampling = -1;
     rt_session_options so{};
     std::vector<std::string> outputs;
+    int32_t tone_map = RT_TONE_BOOKS;
+    double exposure = 0;
     for (int i = 1; i < argc; ++i)
     {
         const std::string a = argv[i];
@@ -155,6 +157,7 @@ int main(int argc, char** argv)
         {
             std::printf("Usage: rt_gpu [scene] [-w N] [--height N] [-s N] [--strata K] [-p STRATEGY] [--seed N]\n"
                         "              [--fp64] [--integrator auto|megakernel|wavefront] [--launch-ms N] [-o FILE]\n"
+                        "              [--tonemap books|none|agx|neutral] [--exposure EV]\n"
                         "Scenes: %s\n"
                         "Or a glTF 2.0 model, FILE.glb or FILE.gltf, rendered in a studio\n",
                         rt_scene_names());
@@ -178,6 +181,26 @@ int main(int argc, char** argv)
             so.launch_seconds = std::stoi(value()) / 1000.0;
         else if (a == "-o" || a == "--output")
             outputs.push_back(value());
+        else if (a == "--exposure")
+            exposure = std::stod(value());
+        else if (a == "--tonemap")
+        {
+            const std::string v = value();
+            const char* names[] = {"books", "none", "agx", "neutral"};
+            tone_map = -1;
+            for (int k = 0; k < 4; ++k)
+            {
+                if (v == names[k])
+                {
+                    tone_map = k;
+                }
+            }
+            if (tone_map < 0)
+            {
+                std::printf("Error: unknown tone map '%s'\n", v.c_str());
+                return 2;
+            }
+        }
         else if (a == "--integrator")
         {
             const std::string v = value();
@@ -281,7 +304,8 @@ int main(int argc, char** argv)
         }
         else
         {
-            ok = rt_session_display_rgba8(session, rgba.data(), rgba.size()) == RT_OK && save_png(out, w, hgt, rgba);
+            ok = rt_session_display_toned(session, tone_map, exposure, rgba.data(), rgba.size()) == RT_OK &&
+                 save_png(out, w, hgt, rgba);
         }
         std::printf("%s: %s\n", ok ? "Saved" : "Error: could not save", out.c_str());
     }

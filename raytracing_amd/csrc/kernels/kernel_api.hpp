@@ -40,6 +40,9 @@ hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origi
                               const Real* times, const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index,
                               Real* t, uint8_t* entering, hipStream_t stream);
 
+// A tone map other than the books' (op: rt_tone_map) of the image's sums, width pixels a row
+hipError_t launch_tone_map(const double* sums, int width, int64_t pixels, double inverse_passes, int op,
+                           float exposure_scale, uint8_t* rgba, hipStream_t stream);
 hipError_t launch_display(const double* sums, int64_t pixels, double inverse_passes, uint8_t* rgba,
                           hipStream_t stream);
 

@@ -1,10 +1,11 @@
-// The megakernel: one thread per pixel traces the launch's passes of its pixel, one ray per loop iteration, and adds
-// each finished sample to the pixel's double sums in pass order (src/gpu/renderer.cu:340-397). It is the plain
-// baseline the wavefront integrator is measured against, and the path the FP64 renders take.
+// The megakernel: the launch's passes of a block of pixels, every path traced by one thread, one ray per loop
+// iteration, and each pixel's samples added to its double sums in pass order (src/gpu/renderer.cu:340-397). It is the
+// plain baseline the wavefront integrator is measured against, and the path the FP64 renders take.
 //
-// MI355X mapping: a block of 16x8 pixels is two wave64s of 16x4; the BVH stack lives in LDS, one column of entries per
-// thread (path_tracer.hpp, lds_stack), sized by the scene's tree depth at launch; the ray counts are summed across the
-// wave with DPP/permute shuffles, then one atomic per wave.
+// MI355X mapping: a block of 16x8 (or 16x16) pixels is two (four) wave64s of 16x4; a wave's lanes share its 64 pixels'
+// passes as items (wave_share) and stage their samples until the wave's items are done; the BVH stack lives in LDS, one
+// column of entries per thread (path_tracer.hpp, lds_stack), sized by the tree's worst case at launch; the ray counts
+// are summed across the wave with shuffles, then one atomic per wave.
 #include <algorithm>
 
 #include "kernel_api.hpp"

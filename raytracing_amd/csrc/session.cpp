@@ -322,6 +322,51 @@ public:
         check(hipMemcpy(out, rgba_.data(), 4 * pixels, hipMemcpyDeviceToHost), "copying the display");
     }
 
+    void display_toned(int32_t tone_map, double exposure, uint8_t* out, std::size_t count) override
+    {
+        if (tone_map < RT_TONE_BOOKS || tone_map > RT_TONE_NEUTRAL)
+        {
+            throw std::invalid_argument("unknown tone map");
+        }
+        if (!std::isfinite(exposure) || std::fabs(exposure) > RT_MAX_EXPOSURE)
+        {
+            throw std::invalid_argument("the exposure must be a finite number of stops within +-24");
+        }
+        if (tone_map == RT_TONE_BOOKS)
+        {
+            if (exposure != 0)
+            {
+                throw std::invalid_argument("the books' bytes take no exposure");
+            }
+            display(out, count);
+            return;
+        }
+        if constexpr (sizeof(Real) != sizeof(float))
+        {
+            throw std::invalid_argument("tone maps other than the books' compute in float32 sessions only");
+        }
+        require_view();
+        if (state_.passes == 0)
+        {
+            throw std::logic_error("the image has no complete pass");
+        }
+        const std::size_t pixels = values() / 3;
+        if (count < 4 * pixels)
+        {
+            throw std::invalid_argument("the output buffer is smaller than width x height x 4");
+        }
+        if (rgba_.size() != 4 * pixels)
+        {
+            rgba_ = device_array<uint8_t>(4 * pixels);
+        }
+        check(launch_tone_map(sums_.data(), camera_.image_width, static_cast<int64_t>(pixels),
+                              1.0 / static_cast<double>(state_.passes), tone_map,
+                              static_cast<float>(std::exp2(exposure)), rgba_.data(), stream_.get()),
+              "launching the tone map");
+        check(hipStreamSynchronize(stream_.get()), "computing the display");
+        check(hipMemcpy(out, rgba_.data(), 4 * pixels, hipMemcpyDeviceToHost), "copying the display");
+    }
+
     void counters(uint64_t* rays, uint64_t* shadow_rays, double* kernel_seconds) const override
     {
         require_view();

@@ -30,6 +30,8 @@ public:
     virtual void pixels(double* out, std::size_t count) = 0;
     virtual void sums(double* out, std::size_t count) = 0;
     virtual void display(uint8_t* out, std::size_t count) = 0;
+    // tone_map: an rt_tone_map; exposure in stops
+    virtual void display_toned(int32_t tone_map, double exposure, uint8_t* out, std::size_t count) = 0;
     virtual void counters(uint64_t* rays, uint64_t* shadow_rays, double* kernel_seconds) const = 0;
     virtual rt_kernel_info kernel() const = 0;
     virtual void closest_hit(const float* origins, const float* directions, const float* times, const int32_t* from,
