@@ -237,6 +237,11 @@ def main():
     ap.add_argument("--wavefront-graph", type=int, default=0, help="bounces per hipGraph (-1: none, e.g. under a profiler)")
     ap.add_argument("--wavefront-drain", type=int, default=0, help="active slots below which the pool drains")
     ap.add_argument("--wavefront-slots", type=int, default=0, help="the wavefront pool's size")
+    ap.add_argument("--megakernel-waves", type=int, default=0, choices=[0, 5, 6],
+                    help="the megakernel's register budget in waves per SIMD (0: the compiler's choice), with "
+                         "--integrator megakernel")
+    ap.add_argument("--block-size", type=int, default=0, choices=[0, 64, 128, 192, 256],
+                    help="the megakernel's threads per block (0: the tuned default), with --integrator megakernel")
     ap.add_argument("--tree", default="default", choices=["default", "median", "sah"],
                     help="the BVH's splits: the library's default, the reference's object median, or the SAH")
     ap.add_argument("--serving", action="store_true", help="config 4: the serving loop under Poisson request streams")
@@ -287,7 +292,8 @@ def main():
     print(f"{args.scene}: {scene.width}x{scene.height}, {scene.passes} spp; device {device.name.decode()} "
           f"({device.compute_units} CUs)", file=sys.stderr, flush=True)
     tuning = {k: v for k, v in (("wavefront_graph", args.wavefront_graph), ("wavefront_drain", args.wavefront_drain),
-                                ("wavefront_slots", args.wavefront_slots)) if v}
+                                ("wavefront_slots", args.wavefront_slots),
+                                ("megakernel_waves", args.megakernel_waves), ("block_size", args.block_size)) if v}
     tuning["tree"] = args.tree
     value, ms, avg_launch, launches_per_step, session = measure(rt, scene, args.integrator, args.steps, args.warmup,
                                                                 **tuning)
