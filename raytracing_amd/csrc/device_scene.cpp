@@ -540,6 +540,11 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
         motion = motion || (s.media[i].boundary == RT_BOUNDARY_SPHERE && moves(s.media[i].sphere));
     }
 
+    // the kernels address a node's planes as the array's base plus a 32-bit byte offset
+    if (tree_.wide.size() > std::numeric_limits<uint32_t>::max() / sizeof(dev::wide_node<Real>))
+    {
+        throw std::runtime_error("the scene's BVH is larger than 4 GiB");
+    }
     nodes_ = device_array<dev::wide_node<Real>>(tree_.wide);
     spheres_ = device_array<dev::sphere_rec<Real>>(tree_.spheres);
     moving_ = device_array<dev::moving_sphere_rec<Real>>(tree_.moving_spheres);

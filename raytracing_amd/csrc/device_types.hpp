@@ -13,6 +13,7 @@
 //     [moving_base, quad_base), quads [quad_base, ...), and the media after the surfaces.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 #include <hip/hip_runtime.h>
@@ -56,6 +57,10 @@ struct alignas(16) wide_node
     int32_t unused[sizeof(Real) == 4 ? 4 : 12];
 };
 static_assert(sizeof(wide_node<float>) == 128 && sizeof(wide_node<double>) == 256);
+// the kernels read the planes and references at these offsets (child_entries, child_refs)
+static_assert(offsetof(wide_node<float>, hi) == 12 * sizeof(float) && offsetof(wide_node<float>, ref) == 24 * sizeof(float));
+static_assert(offsetof(wide_node<double>, hi) == 12 * sizeof(double) &&
+              offsetof(wide_node<double>, ref) == 24 * sizeof(double));
 
 template <typename Real>
 struct alignas(16) sphere_rec

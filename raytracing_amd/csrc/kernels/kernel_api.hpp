@@ -21,17 +21,18 @@ struct kernel_attributes
 template <typename Real>
 size_t stack_lds_bytes(int depth, int threads);
 
-// The megakernel's staging buffer holds one sample (dev::staged_sample<Real>) per thread of the grid per pass; the
-// grid of a launch over a whole image of width x height has at most this many threads at any block shape used here
+// The megakernel's staging buffer holds one sample (dev::staged_sample<Real>) per pixel of a launch's rows per pass; a
+// launch over a whole image of width x height stages at most this many samples per pass
 size_t megakernel_staging_per_pass(int width, int height);
 
 // waves: the occupancy the kernel's register budget targets (0: the compiler's choice, 5 or 6 waves per SIMD; the
 // FP64 and counting instances take 0 whatever is asked). staging: staging_samples samples; a launch of more passes
-// than fit is split into several.
+// than fit is split into several. queue: one device word, the chunk queue's head (cleared by the launch itself).
+// block_threads: 64, 128, 192 or 256 (one to four waves a block; the waves work independently).
 template <typename Real>
 hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
                              uint64_t seed, const dev::launch_work& work, double* sums, void* staging,
-                             size_t staging_samples, unsigned long long* rays, int block_w, int block_h,
+                             size_t staging_samples, uint32_t* queue, unsigned long long* rays, int block_threads,
                              hipStream_t stream, unsigned long long* tallies = nullptr, int waves = 0);
 bool megakernel_waves_supported(int waves);
 

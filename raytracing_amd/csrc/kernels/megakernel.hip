@@ -1,12 +1,15 @@
-// The megakernel: the launch's passes of a block of pixels, every path traced by one thread, one ray per loop
-// iteration, and each pixel's samples added to its double sums in pass order (src/gpu/renderer.cu:340-397). It is the
-// plain baseline the wavefront integrator is measured against, and the path the FP64 renders take.
+// The megakernel: every path traced by one thread, one ray per loop iteration, and each pixel's samples added to its
+// double sums in pass order (src/gpu/renderer.cu:340-397). It is the plain baseline the wavefront integrator is
+// measured against, and the path the FP64 renders take.
 //
-// MI355X mapping: a block of 16x8 (or 16x16) pixels is two (four) wave64s of 16x4; a wave's lanes share its 64 pixels'
-// passes as items (wave_share) and stage their samples until the wave's items are done; the BVH stack lives in LDS, one
-// column of entries per thread (path_tracer.hpp, lds_stack), sized by the tree's worst case at launch; the ray counts
-// are summed across the wave with shuffles, then one atomic per wave.
+// MI355X mapping: the grid is as many wave64s as fit the chip at once (persistent); they take a launch's work from a
+// queue of chunks, each a 16x4 tile of pixels over a group of passes, with one atomic per chunk, and share each chunk's
+// items among their lanes (wave_queue); each sample is staged in HBM and a second kernel adds them to the sums in pass
+// order (add_staged). The BVH stack lives in LDS, one column of entries per thread (path_tracer.hpp, lds_stack), sized
+// by the tree's worst case at launch; the ray counts are summed across the wave with shuffles, then one atomic per
+// wave.
 #include <algorithm>
+#include <cstdint>
 
 #include "kernel_api.hpp"
 #include "path_tracer.hpp"
@@ -61,78 +64,140 @@ __device__ inline void add_counts(unsigned long long rays, unsigned long long sh
     }
 }
 
-// A wave's share of a launch. Its slots are its 64 threads' pixels that lie in the image and the launch's rows; its
-// items are the launch's passes over those slots, pass-major: item k is pass k / count of the (k % count)-th slot. A
-// lane whose path ends claims the wave's next item (a ballot and a popcount: the counter is the wave's own, no
-// atomic), so the lanes stay busy while the wave has items, however long their pixels' paths are; a lane no longer
-// waits for the longest pixel of its wave. Each item's sample goes to the wave's own staging region (no other wave
-// touches its cache lines); when the wave's items are done, each lane adds its own pixel's samples to the double sums
-// in pass order: the same additions in the same order as one thread looping over its pixel's passes, so the image is
-// the same bytes.
-struct wave_share
+// A launch's paths as a queue of chunks. Chunk c is one 16x4 tile of the launch's rows (tile c % tiles, row-major)
+// over a group of the launch's passes (group c / tiles): pass-major, so every tile's first passes come before any
+// tile's later ones, the costly tiles (long paths) are spread over the launch, and the launch ends on short chunks
+// of every kind instead of on its last rows.
+struct chunk_queue
 {
-    int lane;         // this thread's lane
-    int first;        // the wave's first thread in the block
-    uint64_t below;   // the lanes below this one
-    uint32_t count;   // slots in the image
-    uint32_t total;   // items
-    uint32_t next;    // the wave's next unclaimed item (the same in every lane)
-    const uint8_t* slots; // the wave's slots, in lane order
-    bool inside;      // this thread's own pixel is a slot
-    int i, j;         // this thread's own pixel
-    size_t region;    // the wave's staging region: 64 samples per pass, pass-major
+    uint32_t* next_chunk; // the queue's head: zero when the launch starts
+    uint32_t tiles_x;     // tiles in a row of tiles
+    uint32_t tiles;       // tiles in the launch's rows
+    uint32_t chunks;      // tiles x groups
+    int32_t group;        // passes in a chunk (the last group's may be fewer)
+    int32_t width;        // the image's width
+    launch_work work;
+};
 
-    // Claims an item for every lane that has none (needy); true with this lane's item in (pass_local, slot) when it
-    // got one, false when the wave's items are all claimed
-    __device__ bool claim(bool needy, uint32_t& pass_local, int& slot)
+// A chunk's tile and passes (the same in every lane); no items for a chunk past the queue's end
+struct chunk_span
+{
+    int32_t x0, y0;  // the tile's first pixel, y0 counted from the launch's first row
+    int32_t wx, wy;  // the tile's pixels in the image: 16x4 but at the image's right and the rows' bottom edge
+    int32_t pass0;   // the chunk's first pass, counted from the launch's first
+    uint32_t items;  // pixels x passes
+};
+
+__device__ inline chunk_span span_of(const chunk_queue& q, uint32_t c)
+{
+    chunk_span s{0, 0, 0, 0, 0, 0};
+    if (c >= q.chunks)
+    {
+        return s;
+    }
+    const uint32_t g = c / q.tiles;
+    const uint32_t t = c - g * q.tiles;
+    const uint32_t ty = t / q.tiles_x;
+    s.x0 = static_cast<int32_t>(16 * (t - ty * q.tiles_x));
+    s.y0 = static_cast<int32_t>(4 * ty);
+    s.wx = min(16, q.width - s.x0);
+    s.wy = min(4, q.work.row_count - s.y0);
+    s.pass0 = static_cast<int32_t>(g) * q.group;
+    s.items = static_cast<uint32_t>(s.wx * s.wy * min(q.group, q.work.pass_count - s.pass0));
+    return s;
+}
+
+// Item k of a chunk, pass-major over the tile's pixels in row-major order: its pixel (i, j) and pass (counted from
+// the launch's first)
+__device__ inline void item_of(const chunk_span& s, uint32_t k, int first_row, int& i, int& j, int& pass_local)
+{
+    uint32_t p, at, row;
+    if (s.wx == 16 && s.wy == 4)
+    {
+        p = k >> 6;
+        at = k & 63;
+        row = at >> 4;
+    }
+    else
+    {
+        const uint32_t n = static_cast<uint32_t>(s.wx * s.wy);
+        p = k / n;
+        at = k - p * n;
+        row = at / static_cast<uint32_t>(s.wx);
+    }
+    i = s.x0 + static_cast<int>(at - row * static_cast<uint32_t>(s.wx));
+    j = first_row + s.y0 + static_cast<int>(row);
+    pass_local = s.pass0 + static_cast<int>(p);
+}
+
+// A wave's place in the queue: the chunk it takes items from and the chunk's next item, the same in every lane. A
+// lane whose path ends claims the wave's next item (a ballot and a popcount, no atomic); once the wave's chunk is used
+// up, the wave takes the queue's next chunk (one atomic, by its first active lane) and its lanes go on with that
+// chunk's items. So no lane waits for the wave's longer paths and no wave waits for another wave's costly tile: the
+// lanes stay busy until the queue is empty. Any lane may trace any pixel's pass: a sample depends on the pixel and the
+// pass only, and the samples are added to the sums in pass order afterwards (add_staged).
+struct wave_queue
+{
+    uint64_t below;  // the lanes below this one
+    chunk_span span; // the wave's chunk
+    uint32_t next;   // its next unclaimed item
+    bool drained;    // the queue has no chunk left
+
+    __device__ static wave_queue start()
+    {
+        const int lane = static_cast<int>(threadIdx.x & 63);
+        return {(uint64_t{1} << lane) - 1, {0, 0, 0, 0, 0, 0}, 0, false};
+    }
+
+    // Claims an item for every lane that has none (needy): true with this lane's pixel and pass when it got one
+    __device__ bool claim(const chunk_queue& q, bool needy, int& i, int& j, int& pass_local)
     {
         const uint64_t ask = __ballot(needy);
-        const uint32_t k = next + static_cast<uint32_t>(__popcll(ask & below));
-        next += static_cast<uint32_t>(__popcll(ask));
-        if (!needy || k >= total)
+        const uint32_t n = static_cast<uint32_t>(__popcll(ask));
+        const uint32_t rank = static_cast<uint32_t>(__popcll(ask & below));
+        const uint32_t avail = span.items - next;
+        chunk_span from = span;
+        uint32_t k = next + rank;
+        if (n > avail)
+        {
+            chunk_span fresh{0, 0, 0, 0, 0, 0};
+            if (!drained)
+            {
+                uint32_t c = 0;
+                if ((__ballot(true) & below) == 0)
+                {
+                    c = atomicAdd(q.next_chunk, 1u);
+                }
+                c = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(c)));
+                fresh = span_of(q, c);
+                drained = c >= q.chunks;
+            }
+            if (rank >= avail)
+            {
+                from = fresh;
+                k = rank - avail;
+            }
+            next = min(n - avail, fresh.items);
+            span = fresh;
+        }
+        else
+        {
+            next += n;
+        }
+        if (!needy || k >= from.items)
         {
             return false;
         }
-        pass_local = k / count;
-        slot = slots[k - pass_local * count];
+        item_of(from, k, q.work.first_row, i, j, pass_local);
         return true;
-    }
-    // A slot's pixel
-    __device__ void pixel_of(int slot, const launch_work& work, int& si, int& sj) const
-    {
-        const int t = first + slot;
-        si = static_cast<int>(blockIdx.x * blockDim.x) + t % static_cast<int>(blockDim.x);
-        sj = work.first_row + static_cast<int>(blockIdx.y * blockDim.y) + t / static_cast<int>(blockDim.x);
     }
 };
 
-template <typename Real>
-__device__ inline wave_share share_of(const camera_rec<Real>& cam, const launch_work& work)
+// Where a sample is staged: pass-major over the launch's pixels (add_staged reads a pass's samples of consecutive
+// pixels together)
+__device__ inline int32_t staged_at(const chunk_queue& q, int i, int j, int pass_local)
 {
-    __shared__ uint8_t slot_tables[4][64]; // up to 256 threads: four waves
-    wave_share w;
-    const int tid = static_cast<int>(threadIdx.y * blockDim.x + threadIdx.x);
-    w.lane = tid & 63;
-    w.first = tid - w.lane;
-    w.below = (uint64_t{1} << w.lane) - 1;
-    w.i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    w.j = work.first_row + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
-    w.inside = w.i < cam.image_width && w.j < work.first_row + work.row_count;
-    const uint64_t valid = __ballot(w.inside);
-    uint8_t* const table = slot_tables[tid >> 6];
-    if (w.inside)
-    {
-        table[__popcll(valid & w.below)] = static_cast<uint8_t>(w.lane);
-    }
-    __syncthreads();
-    w.slots = table;
-    w.count = static_cast<uint32_t>(__popcll(valid));
-    w.total = w.count * static_cast<uint32_t>(work.pass_count);
-    w.next = 0;
-    const size_t waves_per_block = blockDim.x * blockDim.y / 64;
-    w.region = ((static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x) * waves_per_block + (tid >> 6)) * 64 *
-               static_cast<size_t>(work.pass_count);
-    return w;
+    return (pass_local * q.work.row_count + (j - q.work.first_row)) * q.width + i;
 }
 
 // A staged sample is written once and read once: non-temporal stores and loads, so that the samples don't push the
@@ -152,23 +217,24 @@ __device__ inline staged_sample<Real> unstage(const staged_sample<Real>* at)
     return {v.x, v.y, v.z, Real(0)};
 }
 
-// Adds this lane's own pixel's staged samples to its double sums, in pass order (after the wave's loop: every lane of
-// the wave is past its last store)
+// After a megakernel launch: each pixel of the launch's rows adds its staged samples to its double sums in pass order,
+// the same additions in the same order as one thread looping over its pixel's passes, so the image is the same bytes
+// whichever lane traced which sample (src/gpu/renderer.cu:340-397: the sums of a pixel's passes, NaN as 0)
 template <typename Real>
-__device__ inline void add_samples(const wave_share& w, const camera_rec<Real>& cam, const launch_work& work,
-                                   const staged_sample<Real>* __restrict__ staged, double* __restrict__ sums)
+__global__ void __launch_bounds__(256) add_staged(const staged_sample<Real>* __restrict__ staged,
+                                                  double* __restrict__ sums, int width, launch_work work)
 {
-    __threadfence_block();
-    if (!w.inside)
+    const int64_t pixels = static_cast<int64_t>(width) * work.row_count;
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= pixels)
     {
         return;
     }
-    double* const sum = sums + 3 * (static_cast<uint64_t>(w.j) * static_cast<uint64_t>(cam.image_width) +
-                                    static_cast<uint64_t>(w.i));
+    double* const sum = sums + 3 * (static_cast<int64_t>(work.first_row) * width + k);
     double red = sum[0], green = sum[1], blue = sum[2];
     for (int q = 0; q < work.pass_count; ++q)
     {
-        const staged_sample<Real> s = unstage(staged + static_cast<size_t>(q) * 64 + w.lane);
+        const staged_sample<Real> s = unstage(staged + q * pixels + k);
         red += without_nan(s.x);
         green += without_nan(s.y);
         blue += without_nan(s.z);
@@ -178,43 +244,47 @@ __device__ inline void add_samples(const wave_share& w, const camera_rec<Real>& 
     sum[2] = blue;
 }
 
+// The queue's head, cleared by a kernel before each launch (no memset node where a caller captures the launches)
+__global__ void clear_queue(uint32_t* next_chunk) { *next_chunk = 0; }
+
 // Waves: the waves per SIMD the register allocation must leave room for (0: the compiler's choice); a tighter budget
-// trades spills for occupancy, and the session's autotuner picks per scene. Each lane follows its path ray by ray
-// (src/gpu/renderer.cu:340-397), and a lane whose path ends starts the wave's next item (wave_share). (Walking the
-// tree in bursts of a few steps per iteration, so that a lane whose walk is over shades while a long walk goes on,
-// measured slower here: the walk's state then stays live across the shading code, which costs registers.)
+// trades spills for occupancy, and the session's autotuner picks per scene. The grid is as many waves as fit the chip
+// at once; each lane follows its path ray by ray (src/gpu/renderer.cu:340-397) and claims its next item from its
+// wave's place in the queue as the path ends (wave_queue). (Walking the tree in bursts of a few steps per iteration,
+// so that a lane whose walk is over shades while a long walk goes on, measured slower here: the walk's state then
+// stays live across the shading code, which costs registers.)
 template <typename Real, bool AllBooks, bool Count, int Waves>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
-megakernel(const camera_rec<Real> cam, const scene_view<Real> sc, const uint64_t seed, const launch_work work,
-           double* __restrict__ sums, staged_sample<Real>* __restrict__ staging,
-           unsigned long long* __restrict__ ray_count, unsigned long long* __restrict__ tallies)
+megakernel(const camera_rec<Real> cam, const scene_view<Real> sc, const uint64_t seed, const chunk_queue q,
+           staged_sample<Real>* __restrict__ staging, unsigned long long* __restrict__ ray_count,
+           unsigned long long* __restrict__ tallies)
 {
     const lds_stack stack = block_stack(sc.stack_depth);
     tally<Count> tl;
-    wave_share w = share_of(cam, work);
-    staged_sample<Real>* const staged = staging + w.region;
+    wave_queue w = wave_queue::start();
     unsigned long long rays = 0;
-    int at = -1; // where this lane's path's sample goes in the wave's region, or -1 between items
+    int32_t at = -1; // where this lane's path's sample goes, or -1 between items
     path<Real> p{};
     pcg32 g{};
     while (true)
     {
-        uint32_t pass_local;
-        int slot;
-        if (w.claim(at < 0, pass_local, slot))
+        int si, sj, pass_local;
+        if (w.claim(q, at < 0, si, sj, pass_local))
         {
-            int si, sj;
-            w.pixel_of(slot, work, si, sj);
-            const int pass = work.first_pass + static_cast<int>(pass_local);
+            const int pass = q.work.first_pass + pass_local;
             const uint64_t pixel =
                 static_cast<uint64_t>(sj) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(si);
             g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
             p = start_path(cam, AllBooks && sc.motion != 0, si, sj, pass, g);
-            at = static_cast<int>(pass_local) * 64 + slot;
+            at = staged_at(q, si, sj, pass_local);
         }
-        else if (at < 0)
+        if (at < 0)
         {
-            break;
+            if (w.drained)
+            {
+                break;
+            }
+            continue; // the wave's chunk ran out for this lane in this round; it claims again
         }
         ++rays;
         vec3<Real> sample;
@@ -222,59 +292,57 @@ megakernel(const camera_rec<Real> cam, const scene_view<Real> sc, const uint64_t
                                    : trace_first_book(p, sc, cam, g, sample, stack, tl);
         if (done)
         {
-            stage(staged + at, sample);
+            stage(staging + at, sample);
             at = -1;
         }
     }
-    add_samples(w, cam, work, staged, sums);
     add_counts<Count>(rays, 0, ray_count, tallies, tl);
 }
 
-// The quality integrator's megakernel (src/gpu/quality.cu:53-109): the same loop over the wave's items, with light
+// The quality integrator's megakernel (src/gpu/quality.cu:53-109): the same loop over the queue's items, with light
 // samples and their shadow rays at every vertex that isn't a mirror's (quality.hpp)
 template <bool Count, int Waves>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
-quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc, const uint64_t seed,
-                   const launch_work work, double* __restrict__ sums, staged_sample<float>* __restrict__ staging,
-                   unsigned long long* __restrict__ ray_count, unsigned long long* __restrict__ tallies)
+quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc, const uint64_t seed, const chunk_queue q,
+                   staged_sample<float>* __restrict__ staging, unsigned long long* __restrict__ ray_count,
+                   unsigned long long* __restrict__ tallies)
 {
     const lds_stack stack = block_stack(sc.stack_depth);
     tally<Count> tl;
-    wave_share w = share_of(cam, work);
-    staged_sample<float>* const staged = staging + w.region;
+    wave_queue w = wave_queue::start();
     unsigned long long rays = 0;
     uint32_t shadow_rays = 0;
-    int at = -1;
-    quality_path q{};
+    int32_t at = -1;
+    quality_path qp{};
     pcg32 g{};
     while (true)
     {
-        uint32_t pass_local;
-        int slot;
-        if (w.claim(at < 0, pass_local, slot))
+        int si, sj, pass_local;
+        if (w.claim(q, at < 0, si, sj, pass_local))
         {
-            int si, sj;
-            w.pixel_of(slot, work, si, sj);
-            const int pass = work.first_pass + static_cast<int>(pass_local);
+            const int pass = q.work.first_pass + pass_local;
             const uint64_t pixel =
                 static_cast<uint64_t>(sj) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(si);
             g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
-            q = start_quality(cam, sc.motion != 0, si, sj, pass, g);
-            at = static_cast<int>(pass_local) * 64 + slot;
+            qp = start_quality(cam, sc.motion != 0, si, sj, pass, g);
+            at = staged_at(q, si, sj, pass_local);
         }
-        else if (at < 0)
+        if (at < 0)
         {
-            break;
+            if (w.drained)
+            {
+                break;
+            }
+            continue;
         }
         ++rays;
         vec3<float> sample;
-        if (trace_quality(q, sc, cam, g, sample, stack, tl, shadow_rays))
+        if (trace_quality(qp, sc, cam, g, sample, stack, tl, shadow_rays))
         {
-            stage(staged + at, sample);
+            stage(staging + at, sample);
             at = -1;
         }
     }
-    add_samples(w, cam, work, staged, sums);
     add_counts<Count>(rays, shadow_rays, ray_count, tallies, tl);
 }
 
@@ -374,44 +442,76 @@ bool megakernel_waves_supported(int waves) { return waves == 0 || waves == 5 || 
 
 size_t megakernel_staging_per_pass(int width, int height)
 {
-    const size_t w = (static_cast<size_t>(width) + 15) / 16 * 16;
-    const size_t h = (static_cast<size_t>(height) + 15) / 16 * 16;
-    return w * h;
+    return static_cast<size_t>(width) * static_cast<size_t>(height);
 }
+
+// Passes in a queue chunk: a chunk is 64 x 4 paths for a wave, and a launch of C2 has ~128 K chunks
+constexpr int megakernel_pass_group = 4;
 
 template <typename Real>
 hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
                              uint64_t seed, const dev::launch_work& work, double* sums, void* staging,
-                             size_t staging_samples, unsigned long long* rays, int block_w, int block_h,
+                             size_t staging_samples, uint32_t* queue, unsigned long long* rays, int block_threads,
                              hipStream_t stream, unsigned long long* tallies, int waves)
 {
-    if ((mode == dev::mode_quality && sizeof(Real) != sizeof(float)) || block_w * block_h % 64 != 0 ||
-        block_w * block_h > 256 || work.pass_count < 1 || work.row_count < 1)
+    if ((mode == dev::mode_quality && sizeof(Real) != sizeof(float)) || block_threads % 64 != 0 ||
+        block_threads < 64 || block_threads > 256 || work.pass_count < 1 || work.row_count < 1 ||
+        staging_samples > static_cast<size_t>(INT32_MAX))
     {
         return hipErrorInvalidValue;
     }
-    const dim3 block(block_w, block_h);
-    const dim3 grid((cam.image_width + block_w - 1) / block_w, (work.row_count + block_h - 1) / block_h);
-    const size_t threads = static_cast<size_t>(grid.x) * grid.y * block.x * block.y;
-    // the staging buffer holds `threads` samples per pass: a launch of more passes goes as several
-    const int chunk = static_cast<int>(std::min<size_t>(staging_samples / threads, static_cast<size_t>(work.pass_count)));
+    // the staging buffer holds a sample per pixel of the launch's rows per pass: a launch of more passes goes as several
+    const size_t pixels = static_cast<size_t>(cam.image_width) * static_cast<size_t>(work.row_count);
+    const int chunk = static_cast<int>(std::min<size_t>(staging_samples / pixels, static_cast<size_t>(work.pass_count)));
     if (chunk < 1)
     {
         return hipErrorInvalidValue;
     }
-    const size_t lds = stack_lds_bytes<Real>(sc.stack_depth, block_w * block_h);
+    const size_t lds = stack_lds_bytes<Real>(sc.stack_depth, block_threads);
     const void* f = tallies ? megakernel_for<Real, true>(mode, 0) : megakernel_for<Real, false>(mode, waves);
+    // as many blocks as fit the chip at once
+    int device = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e == hipSuccess)
+    {
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    }
+    if (e == hipSuccess)
+    {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, block_threads, lds);
+    }
+    if (e != hipSuccess)
+    {
+        return e;
+    }
+    const uint32_t tiles_x = static_cast<uint32_t>((cam.image_width + 15) / 16);
+    const uint32_t tiles = tiles_x * static_cast<uint32_t>((work.row_count + 3) / 4);
+    auto* staged = static_cast<dev::staged_sample<Real>*>(staging);
     for (int done = 0; done < work.pass_count; done += chunk)
     {
-        dev::camera_rec<Real> c = cam;
-        dev::scene_view<Real> v = sc;
-        uint64_t s = seed;
         dev::launch_work w = work;
         w.first_pass = work.first_pass + done;
         w.pass_count = std::min(chunk, work.pass_count - done);
-        auto* staged = static_cast<dev::staged_sample<Real>*>(staging);
-        void* args[] = {&c, &v, &s, &w, &sums, &staged, &rays, &tallies};
-        const hipError_t e = hipLaunchKernel(f, grid, block, args, lds, stream);
+        const int group = std::min(megakernel_pass_group, w.pass_count);
+        const uint32_t groups = static_cast<uint32_t>((w.pass_count + group - 1) / group);
+        dev::chunk_queue q{queue, tiles_x, tiles, tiles * groups, group, cam.image_width, w};
+        const uint32_t waves_per_block = static_cast<uint32_t>(block_threads / 64);
+        const uint32_t blocks = std::max<uint32_t>(
+            1, std::min<uint32_t>(static_cast<uint32_t>(std::max(1, per_cu * cus)),
+                                  (q.chunks + waves_per_block - 1) / waves_per_block));
+        hipLaunchKernelGGL(dev::clear_queue, dim3(1), dim3(1), 0, stream, queue);
+        dev::camera_rec<Real> c = cam;
+        dev::scene_view<Real> v = sc;
+        uint64_t s = seed;
+        void* args[] = {&c, &v, &s, &q, &staged, &rays, &tallies};
+        e = hipLaunchKernel(f, dim3(blocks), dim3(block_threads), args, lds, stream);
+        if (e != hipSuccess)
+        {
+            return e;
+        }
+        hipLaunchKernelGGL(dev::add_staged<Real>, dim3(static_cast<unsigned>((pixels + 255) / 256)), dim3(256), 0,
+                           stream, staged, sums, cam.image_width, w);
+        e = hipGetLastError();
         if (e != hipSuccess)
         {
             return e;
@@ -462,11 +562,11 @@ kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, 
 }
 
 template hipError_t launch_megakernel<float>(int, const dev::camera_rec<float>&, const dev::scene_view<float>&,
-                                             uint64_t, const dev::launch_work&, double*, void*, size_t,
-                                             unsigned long long*, int, int, hipStream_t, unsigned long long*, int);
+                                             uint64_t, const dev::launch_work&, double*, void*, size_t, uint32_t*,
+                                             unsigned long long*, int, hipStream_t, unsigned long long*, int);
 template hipError_t launch_megakernel<double>(int, const dev::camera_rec<double>&, const dev::scene_view<double>&,
-                                              uint64_t, const dev::launch_work&, double*, void*, size_t,
-                                              unsigned long long*, int, int, hipStream_t, unsigned long long*, int);
+                                              uint64_t, const dev::launch_work&, double*, void*, size_t, uint32_t*,
+                                              unsigned long long*, int, hipStream_t, unsigned long long*, int);
 template hipError_t launch_closest_hit<float>(const dev::scene_view<float>&, const float*, const float*, const float*,
                                               const int32_t*, const uint8_t*, int64_t, int32_t*, float*, uint8_t*,
                                               hipStream_t);

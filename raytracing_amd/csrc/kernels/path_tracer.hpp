@@ -533,14 +533,56 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
     }
 }
 
-// Where the ray enters a wide node's child c, or infinity (a missing child's +infinity box included)
-template <typename Real>
-__device__ inline Real child_entry(const wide_node<Real>& node, int c, vec3<Real> inv, vec3<Real> os, Real t_min,
-                                   Real t_max)
+// Which plane of each axis a ray meets first in every box: the lo plane where its inverse direction is positive, the
+// hi plane where it is negative (safe_inverse never returns zero). Kept per ray as byte offsets of the four children's
+// near-plane vectors in a wide_node; the far plane's vector is the other one of the axis.
+struct slab_order
 {
-    const Real lo[3] = {node.lo[0][c], node.lo[1][c], node.lo[2][c]};
-    const Real hi[3] = {node.hi[0][c], node.hi[1][c], node.hi[2][c]};
-    return box_entry(lo, hi, inv, os, t_min, t_max);
+    uint32_t near[3];
+};
+template <typename Real>
+__device__ inline slab_order slab_order_of(vec3<Real> inv)
+{
+    const uint32_t r = sizeof(Real);
+    return {{(signbit(inv.x) ? 12u : 0u) * r, (signbit(inv.y) ? 16u : 4u) * r, (signbit(inv.z) ? 20u : 8u) * r}};
+}
+
+// Where the ray enters each of a wide node's four children, or infinity (a missing child's +infinity box included).
+// The near and far planes are loaded by the ray's slab order, so each slab is one multiply-add per plane and no min or
+// max: a multiply-add is monotonic in its first operand, so the near plane's distance is the smaller of the two, and
+// the entries are box_entry's bit for bit.
+template <typename Real>
+__device__ inline void child_entries(const scene_view<Real>& sc, int32_t ref, const slab_order& so, vec3<Real> inv,
+                                     vec3<Real> os, Real t_min, Real t_max, Real (&t)[wide_children])
+{
+    typedef Real real4 __attribute__((ext_vector_type(4)));
+    const char* const nodes = reinterpret_cast<const char*>(sc.nodes);
+    const uint32_t at = static_cast<uint32_t>(ref) * static_cast<uint32_t>(sizeof(wide_node<Real>));
+    const uint32_t r = sizeof(Real);
+    const real4 nx = *reinterpret_cast<const real4*>(nodes + (at + so.near[0]));
+    const real4 ny = *reinterpret_cast<const real4*>(nodes + (at + so.near[1]));
+    const real4 nz = *reinterpret_cast<const real4*>(nodes + (at + so.near[2]));
+    const real4 fx = *reinterpret_cast<const real4*>(nodes + (at + (12u * r - so.near[0])));
+    const real4 fy = *reinterpret_cast<const real4*>(nodes + (at + (20u * r - so.near[1])));
+    const real4 fz = *reinterpret_cast<const real4*>(nodes + (at + (28u * r - so.near[2])));
+#pragma unroll
+    for (int c = 0; c < wide_children; ++c)
+    {
+        const Real enter = m_max(m_max(m_fma(nx[c], inv.x, -os.x), m_fma(ny[c], inv.y, -os.y)),
+                                 m_max(m_fma(nz[c], inv.z, -os.z), t_min));
+        const Real exit = m_min(m_min(m_fma(fx[c], inv.x, -os.x), m_fma(fy[c], inv.y, -os.y)),
+                                m_min(m_fma(fz[c], inv.z, -os.z), t_max));
+        t[c] = enter <= exit ? enter : inf<Real>;
+    }
+}
+
+// A wide node's four child references (read as the planes are: the node array's base plus a 32-bit offset)
+template <typename Real>
+__device__ inline int4 child_refs(const scene_view<Real>& sc, int32_t ref)
+{
+    const char* const nodes = reinterpret_cast<const char*>(sc.nodes);
+    const uint32_t at = static_cast<uint32_t>(ref) * static_cast<uint32_t>(sizeof(wide_node<Real>));
+    return *reinterpret_cast<const int4*>(nodes + (at + static_cast<uint32_t>(24 * sizeof(Real))));
 }
 
 // One compare-exchange of the sorting network: the nearer entry first
@@ -560,16 +602,15 @@ __device__ inline void order(Real& ta, int32_t& ra, Real& tb, int32_t& rb)
 // ones but the nearest go on the stack, farthest first, and the nearest is the next reference (true), or none is
 // entered (false)
 template <typename Real, typename Tally>
-__device__ inline bool descend(const scene_view<Real>& sc, int32_t& ref, vec3<Real> inv, vec3<Real> os, Real t_min,
-                               Real t_max, const lds_stack& stack, int& size, Tally& tl)
+__device__ inline bool descend(const scene_view<Real>& sc, int32_t& ref, const slab_order& so, vec3<Real> inv,
+                               vec3<Real> os, Real t_min, Real t_max, const lds_stack& stack, int& size, Tally& tl)
 {
-    const wide_node<Real> node = sc.nodes[ref];
     tl.add(tally_node);
-    Real t0 = child_entry(node, 0, inv, os, t_min, t_max);
-    Real t1 = child_entry(node, 1, inv, os, t_min, t_max);
-    Real t2 = child_entry(node, 2, inv, os, t_min, t_max);
-    Real t3 = child_entry(node, 3, inv, os, t_min, t_max);
-    int32_t r0 = node.ref[0], r1 = node.ref[1], r2 = node.ref[2], r3 = node.ref[3];
+    Real t[wide_children];
+    child_entries(sc, ref, so, inv, os, t_min, t_max, t);
+    Real t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
+    const int4 refs = child_refs(sc, ref);
+    int32_t r0 = refs.x, r1 = refs.y, r2 = refs.z, r3 = refs.w;
     order(t0, r0, t1, r1);
     order(t2, r2, t3, r3);
     order(t0, r0, t2, r2);
@@ -630,6 +671,7 @@ __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Re
     const Real inverse_a = Real(1) / length_squared(r.direction);
     const vec3<Real> inv{safe_inverse(r.direction.x), safe_inverse(r.direction.y), safe_inverse(r.direction.z)};
     const vec3<Real> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
+    const slab_order so = slab_order_of(inv);
     int size = 0;
     int32_t ref = sc.root_ref;
     while (true)
@@ -638,7 +680,7 @@ __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Re
         {
             closest_in_leaf<Real, AllKinds>(sc, r, ref, from, inward, inverse_a, t_min, closest, tl);
         }
-        else if (descend(sc, ref, inv, os, t_min, closest.t, stack, size, tl))
+        else if (descend(sc, ref, so, inv, os, t_min, closest.t, stack, size, tl))
         {
             continue;
         }

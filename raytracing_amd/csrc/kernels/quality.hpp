@@ -604,6 +604,7 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
     const float inverse_a = 1.0f / length_squared(r.direction);
     const vec3<float> inv{safe_inverse(r.direction.x), safe_inverse(r.direction.y), safe_inverse(r.direction.z)};
     const vec3<float> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
+    const slab_order so = slab_order_of(inv);
     int size = 0;
     int32_t ref = sc.root_ref;
     while (true)
@@ -674,21 +675,24 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
         else
         {
             // any entered child will do: the first one entered is next, the others go on the stack
-            const wide_node<float> node = sc.nodes[ref];
             tl.add(tally_node);
+            float t[wide_children];
+            child_entries(sc, ref, so, inv, os, t_min, t_max, t);
+            const int4 refs = child_refs(sc, ref);
+            const int32_t child[wide_children] = {refs.x, refs.y, refs.z, refs.w};
             int32_t next = no_child;
 #pragma unroll
             for (int c = 0; c < wide_children; ++c)
             {
-                if (child_entry(node, c, inv, os, t_min, t_max) < inf<float>)
+                if (t[c] < inf<float>)
                 {
                     if (next == no_child)
                     {
-                        next = node.ref[c];
+                        next = child[c];
                     }
                     else
                     {
-                        stack.store(size++, node.ref[c], t_min);
+                        stack.store(size++, child[c], t_min);
                     }
                 }
             }

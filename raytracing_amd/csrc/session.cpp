@@ -131,13 +131,13 @@ public:
             throw std::invalid_argument("the wavefront integrator renders in float32 only");
         }
         current_ = {options.integrator == RT_INTEGRATOR_WAVEFRONT, options.megakernel_waves,
-                    options.block_size > 0 ? options.block_size : 128};
+                    options.block_size > 0 ? options.block_size : 64};
         if (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>())
         {
             // the autotuner's candidates, in the order they render the first full passes
             for (const int waves : {0, 5, 6})
             {
-                for (const int block : {128, 256})
+                for (const int block : {64, 128, 256})
                 {
                     candidates_.push_back({false, waves, block});
                 }
@@ -152,6 +152,7 @@ public:
             tune_seconds_.assign(candidates_.size(), 0.0);
         }
         rays_ = device_array<unsigned long long>(2);
+        queue_ = device_array<uint32_t>(1);
         validate_camera(camera);
         if (camera.sampling != scene_.camera.sampling)
         {
@@ -474,8 +475,8 @@ public:
         const dev::launch_work work{0, static_cast<int32_t>(passes), 0, camera_.image_height};
         ensure_staging();
         check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), options_.seed, work, scratch.data(),
-                                      staging_.data(), staging_samples_, counts.data(), 16, 8, stream_.get(),
-                                      counts.data() + 2),
+                                      staging_.data(), staging_samples_, queue_.data(), counts.data(), 128,
+                                      stream_.get(), counts.data() + 2),
               "launching the counting kernel");
         check(hipStreamSynchronize(stream_.get()), "counting traffic");
         unsigned long long host[2 + RT_TRAFFIC_EVENTS] = {};
@@ -586,7 +587,9 @@ private:
     dev::camera_rec<Real> cam_{};
     device_array<double> sums_;
     device_array<unsigned long long> rays_;
-    // the megakernel's staged samples: one per grid thread per pass, for up to staging_passes_ passes a kernel
+    // the megakernel's chunk queue head, and its staged samples: one per pixel per pass, for up to staging_passes_
+    // passes a kernel
+    device_array<uint32_t> queue_;
     static constexpr std::size_t staging_bytes_cap = std::size_t{1} << 30;
     device_array<dev::staged_sample<Real>> staging_;
     std::size_t staging_samples_ = 0;
@@ -679,8 +682,8 @@ private:
         {
             ensure_staging();
             check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), staging_.data(),
-                                          staging_samples_, rays_.data(), 16, std::max(1, v.block / 16),
-                                          stream_.get(), nullptr, v.waves),
+                                          staging_samples_, queue_.data(), rays_.data(), v.block, stream_.get(),
+                                          nullptr, v.waves),
                   "launching the kernel");
         }
         stop_.record(stream_.get());
