@@ -135,12 +135,10 @@ public:
         if (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>())
         {
             // the autotuner's candidates, in the order they render the first full passes
+            // (one wave a block: the megakernel's waves are independent, and a bigger block only holds more LDS)
             for (const int waves : {0, 5, 6})
             {
-                for (const int block : {64, 128, 256})
-                {
-                    candidates_.push_back({false, waves, block});
-                }
+                candidates_.push_back({false, waves, 64});
             }
             candidates_.push_back({true, 0, 256});
             current_ = candidates_.front();
@@ -412,8 +410,8 @@ public:
             {
                 std::snprintf(budget, sizeof budget, " w%d", current_.waves);
             }
-            std::snprintf(info.name, sizeof info.name, "%s%s%s%s", name, sizeof(Real) == 8 ? " fp64" : "", budget,
-                          current_.block != 128 ? (current_.block == 256 ? " b256" : " b64") : "");
+            std::snprintf(info.name, sizeof info.name, "%s%s%s b%d", name, sizeof(Real) == 8 ? " fp64" : "", budget,
+                          current_.block);
         }
         info.registers = a.registers;
         info.local_bytes = a.local_bytes;
