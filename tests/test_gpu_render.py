@@ -232,3 +232,40 @@ def test_wavefront_small_pool(rt):
         b.render_to(16)
         sb = b.sums()
     assert np.array_equal(sa.view(np.uint64), sb.view(np.uint64))
+
+
+@pytest.mark.parametrize("name,width", [("final", 37), ("weekend", 100), ("cornell", 50)])
+def test_megakernel_queue_shapes(rt, name, width):
+    """The megakernel's chunk queue (16x4 tiles x groups of 4 passes): image sides that aren't multiples of the tile,
+    pass counts that aren't multiples of the group, launches of one pass up to all of them, a band of rows, every
+    register budget and block size: the same sums as the wavefront, bit for bit, and the same ray counts"""
+    scene = rt.build_scene(name, width=width, samples=7, strata=4, earth_width=512)
+    assert scene.passes == 7
+    with rt.Session(scene, seed=11, integrator="wavefront") as w:
+        w.render_to(7)
+        reference = w.sums()
+        rays = w.counters()[0]
+    variants = [dict(), dict(megakernel_waves=5, block_size=128), dict(megakernel_waves=6, block_size=256),
+                dict(block_size=192, launch_seconds=0.0001)]
+    for options in variants:
+        with rt.Session(scene, seed=11, integrator="megakernel", **options) as m:
+            m.render_to(1)
+            m.render_to(3, 5)  # a band of the third pass's rows
+            m.render_to(7)
+            got = m.sums()
+            assert m.counters()[0] == rays, options
+        mism = np.count_nonzero(got.view(np.uint64) != reference.view(np.uint64))
+        assert mism == 0, f"{options}: {mism} of {got.size} sums differ"
+
+
+def test_megakernel_fp64_launch_splits(rt):
+    """FP64 (megakernel only): one launch or a launch per pass, the same bytes"""
+    scene = rt.build_scene("final", width=37, samples=6, strata=4, earth_width=512)
+    with rt.Session(scene, seed=4, precision="float64", launch_seconds=0.5) as a:
+        a.render_to(6)
+        sa = a.sums()
+    with rt.Session(scene, seed=4, precision="float64", launch_seconds=0.0001) as b:
+        for p in (1, 2, 6):
+            b.render_to(p)
+        sb = b.sums()
+    assert np.array_equal(sa.view(np.uint64), sb.view(np.uint64))
