@@ -237,7 +237,7 @@ def main():
     ap.add_argument("--wavefront-graph", type=int, default=0, help="bounces per hipGraph (-1: none, e.g. under a profiler)")
     ap.add_argument("--wavefront-drain", type=int, default=0, help="active slots below which the pool drains")
     ap.add_argument("--wavefront-slots", type=int, default=0, help="the wavefront pool's size")
-    ap.add_argument("--megakernel-waves", type=int, default=0, choices=[0, 5, 6],
+    ap.add_argument("--megakernel-waves", type=int, default=0, choices=[0, 5, 6, 7],
                     help="the megakernel's register budget in waves per SIMD (0: the compiler's choice), with "
                          "--integrator megakernel")
     ap.add_argument("--block-size", type=int, default=0, choices=[0, 64, 128, 192, 256],

@@ -340,7 +340,7 @@ typedef struct rt_session_options
     int32_t wavefront_drain;/* wavefront: when fewer slots than this are active, each finishes its pixel's passes on
                                its own (0: the tuned default, a sixteenth of the pool; -1: never) */
     int32_t megakernel_waves;/* megakernel: the waves per SIMD its registers are budgeted for: 0 (the compiler's
-                               choice), 5 or 6 (float only) */
+                               choice), 5, 6 or 7 (float only) */
     int32_t tree;           /* rt_tree: how the session's BVH is split */
 } rt_session_options;
 

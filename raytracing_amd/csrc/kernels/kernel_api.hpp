@@ -25,15 +25,19 @@ size_t stack_lds_bytes(int depth, int threads);
 // launch over a whole image of width x height stages at most this many samples per pass
 size_t megakernel_staging_per_pass(int width, int height);
 
-// waves: the occupancy the kernel's register budget targets (0: the compiler's choice, 5 or 6 waves per SIMD; the
+// waves: the occupancy the kernel's register budget targets (0: the compiler's choice, 5, 6 or 7 waves per SIMD; the
 // FP64 and counting instances take 0 whatever is asked). staging: staging_samples samples; a launch of more passes
 // than fit is split into several. queue: one device word, the chunk queue's head (cleared by the launch itself).
-// block_threads: 64, 128, 192 or 256 (one to four waves a block; the waves work independently).
+// block_threads: 64, 128, 192 or 256 (one to four waves a block; the waves work independently). far_buffer (may be
+// null): far_bytes of device memory where the quality megakernel keeps its stacks' deeper levels when the whole
+// stacks in LDS would cost occupancy (megakernel_far_stack_bytes(depth) bytes always suffice).
 template <typename Real>
 hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
                              uint64_t seed, const dev::launch_work& work, double* sums, void* staging,
                              size_t staging_samples, uint32_t* queue, unsigned long long* rays, int block_threads,
-                             hipStream_t stream, unsigned long long* tallies = nullptr, int waves = 0);
+                             hipStream_t stream, unsigned long long* tallies = nullptr, int waves = 0,
+                             void* far_buffer = nullptr, size_t far_bytes = 0);
+size_t megakernel_far_stack_bytes(int depth);
 bool megakernel_waves_supported(int waves);
 
 template <typename Real>

@@ -299,15 +299,24 @@ megakernel(const camera_rec<Real> cam, const scene_view<Real> sc, const uint64_t
     add_counts<Count>(rays, 0, ray_count, tallies, tl);
 }
 
-// The quality integrator's megakernel (src/gpu/quality.cu:53-109): the same loop over the queue's items, with light
-// samples and their shadow rays at every vertex that isn't a mirror's (quality.hpp)
-template <bool Count, int Waves>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
-quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc, const uint64_t seed, const chunk_queue q,
-                   staged_sample<float>* __restrict__ staging, unsigned long long* __restrict__ ray_count,
-                   unsigned long long* __restrict__ tallies)
+// The deeper levels of the quality megakernel's traversal stacks, in HBM (split_stack): levels [near, depth) of each
+// grid thread, refs then distances, one column per thread
+struct far_stack
 {
-    const lds_stack stack = block_stack(sc.stack_depth);
+    int32_t near;     // levels kept in LDS
+    int32_t* refs;    // (depth - near) x stride
+    uint16_t* ts;     // (depth - near) x stride
+    uint32_t stride;  // threads in the grid
+};
+
+// The quality integrator's paths (src/gpu/quality.cu:53-109): the same loop over the queue's items, with light
+// samples and their shadow rays at every vertex that isn't a mirror's (quality.hpp)
+template <bool Count, typename Stack>
+__device__ inline void quality_paths(const camera_rec<float>& cam, const scene_view<float>& sc, uint64_t seed,
+                                     const chunk_queue& q, staged_sample<float>* __restrict__ staging,
+                                     unsigned long long* __restrict__ ray_count,
+                                     unsigned long long* __restrict__ tallies, const Stack& stack)
+{
     tally<Count> tl;
     wave_queue w = wave_queue::start();
     unsigned long long rays = 0;
@@ -344,6 +353,26 @@ quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc, cons
         }
     }
     add_counts<Count>(rays, shadow_rays, ray_count, tallies, tl);
+}
+
+// Split: the stack's first far.near levels in LDS, the rest in HBM (a deep tree, such as a large mesh's, would
+// otherwise hold so much LDS per wave that fewer waves fit than the registers allow)
+template <bool Count, int Waves, bool Split>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
+quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc, const uint64_t seed, const chunk_queue q,
+                   staged_sample<float>* __restrict__ staging, unsigned long long* __restrict__ ray_count,
+                   unsigned long long* __restrict__ tallies, const far_stack far)
+{
+    if constexpr (Split)
+    {
+        const uint32_t column = blockIdx.x * blockDim.x + threadIdx.x;
+        const split_stack stack{block_stack(far.near), far.near, far.refs + column, far.ts + column, far.stride};
+        quality_paths<Count>(cam, sc, seed, q, staging, ray_count, tallies, stack);
+    }
+    else
+    {
+        quality_paths<Count>(cam, sc, seed, q, staging, ray_count, tallies, block_stack(sc.stack_depth));
+    }
 }
 
 // The parity probe: closest_hit on a batch of given rays, with the traversal of the all-books kernel
@@ -406,43 +435,80 @@ size_t stack_lds_bytes(int entries, int threads)
 namespace
 {
 template <typename Real, bool Count, int Waves>
-const void* megakernel_for_waves(int mode)
+const void* megakernel_for_waves(int mode, bool split)
 {
     if constexpr (sizeof(Real) == sizeof(float))
     {
         if (mode == dev::mode_quality)
         {
-            return reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves>);
+            return split ? reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, true>)
+                         : reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, false>);
         }
     }
     return mode == dev::mode_all_books ? reinterpret_cast<const void*>(&dev::megakernel<Real, true, Count, Waves>)
                                        : reinterpret_cast<const void*>(&dev::megakernel<Real, false, Count, Waves>);
 }
 
-// The instance for a mode and an occupancy target: the counting instances and the FP64 ones at the compiler's choice
+// The instance for a mode and an occupancy target: the counting instances and the FP64 ones at the compiler's choice.
+// split: the quality megakernel with its stack's deeper levels in HBM
 template <typename Real, bool Count>
-const void* megakernel_for(int mode, int waves)
+const void* megakernel_for(int mode, int waves, bool split = false)
 {
     if constexpr (!Count && sizeof(Real) == sizeof(float))
     {
         if (waves == 5)
         {
-            return megakernel_for_waves<Real, Count, 5>(mode);
+            return megakernel_for_waves<Real, Count, 5>(mode, split);
         }
         if (waves == 6)
         {
-            return megakernel_for_waves<Real, Count, 6>(mode);
+            return megakernel_for_waves<Real, Count, 6>(mode, split);
+        }
+        if (waves == 7)
+        {
+            return megakernel_for_waves<Real, Count, 7>(mode, split);
         }
     }
-    return megakernel_for_waves<Real, Count, 0>(mode);
+    return megakernel_for_waves<Real, Count, 0>(mode, split);
+}
+
+// The stack levels a split quality megakernel keeps in LDS: as many as let the waves its registers allow share a CU's
+// LDS (at least 8); the whole stack (no split) when it fits that way already
+int near_levels(const void* split_kernel, int depth, int block_threads, int device)
+{
+    hipFuncAttributes a{};
+    int lds_per_cu = 0;
+    if (hipFuncGetAttributes(&a, split_kernel) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        lds_per_cu <= 0)
+    {
+        return depth;
+    }
+    const int registers = std::max(8, (a.numRegs + 7) / 8 * 8);
+    const int waves_per_cu = 4 * std::min(8, 512 / registers);
+    const int blocks_per_cu = std::max(1, waves_per_cu / (block_threads / 64));
+    const int levels = lds_per_cu / blocks_per_cu / (block_threads * 6);
+    return levels >= depth ? depth : std::max(8, levels);
 }
 }
 
-bool megakernel_waves_supported(int waves) { return waves == 0 || waves == 5 || waves == 6; }
+bool megakernel_waves_supported(int waves) { return waves == 0 || (waves >= 5 && waves <= 7); }
 
 size_t megakernel_staging_per_pass(int width, int height)
 {
     return static_cast<size_t>(width) * static_cast<size_t>(height);
+}
+
+size_t megakernel_far_stack_bytes(int depth)
+{
+    int device = 0, cus = 0;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    {
+        return 0;
+    }
+    // at most 32 waves a CU, 6 bytes a level
+    return static_cast<size_t>(depth) * static_cast<size_t>(cus) * 32 * 64 * 6;
 }
 
 // Passes in a queue chunk: a chunk is 64 x 4 paths for a wave, and a launch of C2 has ~128 K chunks
@@ -452,7 +518,8 @@ template <typename Real>
 hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
                              uint64_t seed, const dev::launch_work& work, double* sums, void* staging,
                              size_t staging_samples, uint32_t* queue, unsigned long long* rays, int block_threads,
-                             hipStream_t stream, unsigned long long* tallies, int waves)
+                             hipStream_t stream, unsigned long long* tallies, int waves, void* far_buffer,
+                             size_t far_bytes)
 {
     if ((mode == dev::mode_quality && sizeof(Real) != sizeof(float)) || block_threads % 64 != 0 ||
         block_threads < 64 || block_threads > 256 || work.pass_count < 1 || work.row_count < 1 ||
@@ -467,7 +534,7 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
     {
         return hipErrorInvalidValue;
     }
-    const size_t lds = stack_lds_bytes<Real>(sc.stack_depth, block_threads);
+    size_t lds = stack_lds_bytes<Real>(sc.stack_depth, block_threads);
     const void* f = tallies ? megakernel_for<Real, true>(mode, 0) : megakernel_for<Real, false>(mode, waves);
     // as many blocks as fit the chip at once
     int device = 0, cus = 0, per_cu = 0;
@@ -483,6 +550,34 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
     if (e != hipSuccess)
     {
         return e;
+    }
+    // the quality megakernel on a deep tree: the stack's deeper levels in HBM when the whole stack in LDS would leave
+    // room for fewer waves than the registers allow, and the buffer holds them for every thread of the grid
+    dev::far_stack far{sc.stack_depth, nullptr, nullptr, 0};
+    if (!tallies && mode == dev::mode_quality && far_buffer != nullptr && sc.stack_depth > 0)
+    {
+        const void* split = megakernel_for<Real, false>(mode, waves, true);
+        const int near = near_levels(split, sc.stack_depth, block_threads, device);
+        const size_t split_lds = stack_lds_bytes<Real>(near, block_threads);
+        int split_per_cu = 0;
+        // (worth it for a quarter more waves or better: a split costs a branch at every push and pop, and one wave
+        // more per CU measured slower on C3 than the whole stack in LDS)
+        if (near < sc.stack_depth &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&split_per_cu, split, block_threads, split_lds) == hipSuccess &&
+            4 * split_per_cu >= 5 * per_cu)
+        {
+            const size_t stride = static_cast<size_t>(split_per_cu) * static_cast<size_t>(cus) * block_threads;
+            const size_t levels = static_cast<size_t>(sc.stack_depth - near);
+            if (levels * stride * 6 <= far_bytes && stride <= UINT32_MAX)
+            {
+                f = split;
+                lds = split_lds;
+                per_cu = split_per_cu;
+                far = {near, static_cast<int32_t*>(far_buffer),
+                       reinterpret_cast<uint16_t*>(static_cast<char*>(far_buffer) + levels * stride * 4),
+                       static_cast<uint32_t>(stride)};
+            }
+        }
     }
     const uint32_t tiles_x = static_cast<uint32_t>((cam.image_width + 15) / 16);
     const uint32_t tiles = tiles_x * static_cast<uint32_t>((work.row_count + 3) / 4);
@@ -503,8 +598,10 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
         dev::camera_rec<Real> c = cam;
         dev::scene_view<Real> v = sc;
         uint64_t s = seed;
-        void* args[] = {&c, &v, &s, &q, &staged, &rays, &tallies};
-        e = hipLaunchKernel(f, dim3(blocks), dim3(block_threads), args, lds, stream);
+        void* args[] = {&c, &v, &s, &q, &staged, &rays, &tallies, &far};
+        void* book_args[] = {&c, &v, &s, &q, &staged, &rays, &tallies}; // the books' kernels take no far_stack
+        e = hipLaunchKernel(f, dim3(blocks), dim3(block_threads), mode == dev::mode_quality ? args : book_args, lds,
+                            stream);
         if (e != hipSuccess)
         {
             return e;
@@ -563,10 +660,12 @@ kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, 
 
 template hipError_t launch_megakernel<float>(int, const dev::camera_rec<float>&, const dev::scene_view<float>&,
                                              uint64_t, const dev::launch_work&, double*, void*, size_t, uint32_t*,
-                                             unsigned long long*, int, hipStream_t, unsigned long long*, int);
+                                             unsigned long long*, int, hipStream_t, unsigned long long*, int, void*,
+                                             size_t);
 template hipError_t launch_megakernel<double>(int, const dev::camera_rec<double>&, const dev::scene_view<double>&,
                                               uint64_t, const dev::launch_work&, double*, void*, size_t, uint32_t*,
-                                              unsigned long long*, int, hipStream_t, unsigned long long*, int);
+                                              unsigned long long*, int, hipStream_t, unsigned long long*, int, void*,
+                                              size_t);
 template hipError_t launch_closest_hit<float>(const dev::scene_view<float>&, const float*, const float*, const float*,
                                               const int32_t*, const uint8_t*, int64_t, int32_t*, float*, uint8_t*,
                                               hipStream_t);

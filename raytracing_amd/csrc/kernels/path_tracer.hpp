@@ -436,6 +436,43 @@ struct lds_stack
         refs[level * stride] = ref;
         ts[level * stride] = stack_distance(t);
     }
+    __device__ int32_t ref_at(int level) const { return refs[level * stride]; }
+    __device__ float distance_at(int level) const { return stack_distance_value(ts[level * stride]); }
+};
+
+// A stack whose first `near` levels are in LDS and whose deeper ones are in HBM, one column per thread of the grid
+// (interleaved like the LDS part): a deep tree's worst case then no longer sets the LDS a wave holds, and so the
+// occupancy; rays rarely go that deep.
+struct split_stack
+{
+    lds_stack top;
+    int near;         // levels in LDS
+    int32_t* refs;    // the deeper levels, this thread's column
+    uint16_t* ts;
+    uint32_t stride;  // threads in the grid
+    template <typename Real>
+    __device__ void store(int level, int32_t ref, Real t) const
+    {
+        if (level < near)
+        {
+            top.store(level, ref, t);
+        }
+        else
+        {
+            const size_t at = static_cast<size_t>(level - near) * stride;
+            refs[at] = ref;
+            ts[at] = stack_distance(t);
+        }
+    }
+    __device__ int32_t ref_at(int level) const
+    {
+        return level < near ? top.ref_at(level) : refs[static_cast<size_t>(level - near) * stride];
+    }
+    __device__ float distance_at(int level) const
+    {
+        return level < near ? top.distance_at(level)
+                            : stack_distance_value(ts[static_cast<size_t>(level - near) * stride]);
+    }
 };
 
 // LDS bytes of a block's stacks, and one thread's stack within them
@@ -601,9 +638,9 @@ __device__ inline void order(Real& ta, int32_t& ra, Real& tb, int32_t& rb)
 // An inner node: its four children's boxes tested and sorted by entry distance (a five-exchange network); the entered
 // ones but the nearest go on the stack, farthest first, and the nearest is the next reference (true), or none is
 // entered (false)
-template <typename Real, typename Tally>
+template <typename Real, typename Tally, typename Stack>
 __device__ inline bool descend(const scene_view<Real>& sc, int32_t& ref, const slab_order& so, vec3<Real> inv,
-                               vec3<Real> os, Real t_min, Real t_max, const lds_stack& stack, int& size, Tally& tl)
+                               vec3<Real> os, Real t_min, Real t_max, const Stack& stack, int& size, Tally& tl)
 {
     tl.add(tally_node);
     Real t[wide_children];
@@ -638,15 +675,15 @@ __device__ inline bool descend(const scene_view<Real>& sc, int32_t& ref, const s
 
 // The next stack entry the ray may enter no farther than the closest hit so far (at exactly that distance it may hold
 // a tie), or false when there is none
-template <typename Real>
-__device__ inline bool pop(const lds_stack& stack, int& size, Real t_max, int32_t& ref)
+template <typename Real, typename Stack>
+__device__ inline bool pop(const Stack& stack, int& size, Real t_max, int32_t& ref)
 {
     while (size > 0)
     {
         --size;
-        if (stack_distance_value(stack.ts[size * stack.stride]) <= t_max)
+        if (stack.distance_at(size) <= t_max)
         {
-            ref = stack.refs[size * stack.stride];
+            ref = stack.ref_at(size);
             return true;
         }
     }
@@ -658,9 +695,9 @@ __device__ inline bool pop(const lds_stack& stack, int& size, Real t_max, int32_
 // are visited near first and leaves tested in the reference's order; the loop takes one node or one leaf per
 // iteration (on MI355X this measured faster than Aila and Laine's while-while form, which walks inner nodes in an
 // inner loop until every lane of the wave has a leaf).
-template <typename Real, bool AllKinds, typename Tally>
+template <typename Real, bool AllKinds, typename Tally, typename Stack>
 __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
-                                        const lds_stack& stack, Tally& tl)
+                                        const Stack& stack, Tally& tl)
 {
     const Real t_min = Real(0.001);
     hit<Real> closest{-1, inf<Real>, false};
@@ -1007,9 +1044,9 @@ __device__ inline hit<Real> media_events(const scene_view<Real>& sc, const ray<R
     }
     return closest;
 }
-template <typename Real, typename Tally>
+template <typename Real, typename Tally, typename Stack>
 __device__ inline hit<Real> closest_event(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
-                                          pcg32& g, const lds_stack& stack, Tally& tl)
+                                          pcg32& g, const Stack& stack, Tally& tl)
 {
     return media_events(sc, r, closest_hit<Real, true>(sc, r, from, inward, stack, tl), g, tl);
 }
@@ -1316,9 +1353,9 @@ __device__ inline bool first_book_after_walk(path<Real>& p, const scene_view<Rea
     p.inward = dot(direction, outward) < Real(0);
     return false;
 }
-template <typename Real, typename Tally>
+template <typename Real, typename Tally, typename Stack>
 __device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                        pcg32& g, vec3<Real>& sample, const lds_stack& stack, Tally& tl)
+                                        pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
 {
     return first_book_after_walk(p, sc, cam, g, sample, closest_hit<Real, false>(sc, p.r, p.from, p.inward, stack, tl),
                                  tl);
@@ -1389,9 +1426,9 @@ __device__ inline bool all_books_after_walk(path<Real>& p, const scene_view<Real
     }
     return shade(p, sc, h, g, sample, tl);
 }
-template <typename Real, typename Tally>
+template <typename Real, typename Tally, typename Stack>
 __device__ inline bool trace_all_books(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                       pcg32& g, vec3<Real>& sample, const lds_stack& stack, Tally& tl)
+                                       pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
 {
     return all_books_after_walk(p, sc, cam, g, sample, closest_hit<Real, true>(sc, p.r, p.from, p.inward, stack, tl),
                                 tl);

@@ -592,9 +592,9 @@ __device__ inline float lamp_hit_pdf(const quality_view& q, int id, vec3<float> 
 
 // --- shadow rays (src/gpu/shadow.cuh:26-165) -------------------------------------------------------------------------
 // Anything in (0.001, t_max) but the primitive `skip`? closest_hit's walk with a fixed t_max, ending at the first hit
-template <typename Tally>
+template <typename Tally, typename Stack>
 __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r, float t_max, int from, bool inward,
-                                int skip, const lds_stack& stack, Tally& tl)
+                                int skip, const Stack& stack, Tally& tl)
 {
     const float t_min = 0.001f;
     if (sc.stack_depth == 0)
@@ -707,7 +707,7 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
             return false;
         }
         --size;
-        ref = stack.refs[size * stack.stride];
+        ref = stack.ref_at(size);
     }
 }
 // The chance the segment crosses every medium without a collision: e^(-sum rho L), the expected value of the media's
@@ -782,9 +782,9 @@ __device__ inline bool light_contribution(const scene_view<float>& sc, const bsd
 }
 
 // What a hit (or a miss) does to a quality path: the part of trace_quality_ray after the closest event
-template <typename Tally>
+template <typename Tally, typename Stack>
 __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
-                                     const hit<float>& h, pcg32& g, vec3<float>& sample, const lds_stack& stack,
+                                     const hit<float>& h, pcg32& g, vec3<float>& sample, const Stack& stack,
                                      Tally& tl, uint32_t& shadow_rays)
 {
     path<float>& p = q.base;
@@ -868,9 +868,9 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
     return false;
 }
 
-template <typename Tally>
+template <typename Tally, typename Stack>
 __device__ inline bool trace_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
-                                     pcg32& g, vec3<float>& sample, const lds_stack& stack, Tally& tl,
+                                     pcg32& g, vec3<float>& sample, const Stack& stack, Tally& tl,
                                      uint32_t& shadow_rays)
 {
     const hit<float> h = closest_event(sc, q.base.r, q.base.from, q.base.inward, g, stack, tl);

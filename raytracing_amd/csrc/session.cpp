@@ -124,7 +124,7 @@ public:
         launch_seconds_ = options.launch_seconds > 0 ? options.launch_seconds : 0.05;
         if (!megakernel_waves_supported(options.megakernel_waves))
         {
-            throw std::invalid_argument("the megakernel's waves per SIMD must be 0, 5 or 6");
+            throw std::invalid_argument("the megakernel's waves per SIMD must be 0, 5, 6 or 7");
         }
         if (options.integrator == RT_INTEGRATOR_WAVEFRONT && !wavefront_supported<Real>())
         {
@@ -136,7 +136,7 @@ public:
         {
             // the autotuner's candidates, in the order they render the first full passes
             // (one wave a block: the megakernel's waves are independent, and a bigger block only holds more LDS)
-            for (const int waves : {0, 5, 6})
+            for (const int waves : {0, 5, 6, 7})
             {
                 candidates_.push_back({false, waves, 64});
             }
@@ -504,7 +504,8 @@ public:
 
     std::size_t device_bytes() const override
     {
-        return dscene_.bytes() + sums_.bytes() + staging_.bytes() + (wavefront_ ? wavefront_->bytes() : 0);
+        return dscene_.bytes() + sums_.bytes() + staging_.bytes() + far_stack_.bytes() +
+               (wavefront_ ? wavefront_->bytes() : 0);
     }
 
 private:
@@ -537,7 +538,8 @@ private:
     bool tuning() const { return tuned_ < schedule_.size(); }
 
     // After the screen (one pass each), the three fastest candidates are timed again on launches of several passes
-    // (about a fifth of a planned launch: the steady state, not the tail), in a forward then a backward sweep so that
+    // (about half a planned launch, up to 64 passes: close to the launches they will render, whose end weighs on the
+    // persistent megakernel's candidates differently), in a forward then a backward sweep so that
     // a drift in the timings (clocks, caches warming) weighs on all of them alike; then the fastest renders the rest
     void next_tuning_round()
     {
@@ -562,7 +564,7 @@ private:
             tuning_round_ = 1;
             const std::size_t finalists = std::min<std::size_t>(3, candidates_.size());
             const int passes = static_cast<int>(
-                std::clamp(0.2 * launch_seconds_ / std::max(per_pass[order[0]], 1e-9), 1.0, 16.0));
+                std::clamp(0.5 * launch_seconds_ / std::max(per_pass[order[0]], 1e-9), 1.0, 64.0));
             schedule_.clear();
             tuned_ = 0;
             for (std::size_t k = 0; k < finalists; ++k)
@@ -610,6 +612,16 @@ private:
         staging_ = device_array<dev::staged_sample<Real>>(per_pass * passes);
         staging_samples_ = per_pass * passes;
         staging_passes_ = static_cast<int>(passes);
+    }
+    // the quality megakernel's stacks' deeper levels on a deep tree (a large mesh's), allocated at its first launch
+    device_array<uint8_t> far_stack_;
+    void ensure_far_stack()
+    {
+        const int depth = dscene_.view().stack_depth;
+        if (mode_ == dev::mode_quality && depth > 16 && far_stack_.size() == 0)
+        {
+            far_stack_ = device_array<uint8_t>(megakernel_far_stack_bytes(depth));
+        }
     }
     device_array<uint8_t> rgba_;
     bool has_view_ = false;
@@ -679,9 +691,10 @@ private:
         else
         {
             ensure_staging();
+            ensure_far_stack();
             check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), staging_.data(),
                                           staging_samples_, queue_.data(), rays_.data(), v.block, stream_.get(),
-                                          nullptr, v.waves),
+                                          nullptr, v.waves, far_stack_.data(), far_stack_.bytes()),
                   "launching the kernel");
         }
         stop_.record(stream_.get());

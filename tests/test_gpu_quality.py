@@ -215,3 +215,22 @@ def test_triangle_closest_hit_brute_force(rt):
     rel = np.abs(t[both] - bf[both]) / bf[both]
     assert np.quantile(rel, 0.999) < 1e-5 and rel.max() < 1e-3, f"t differs: max {rel.max()}"
     assert np.all(ent[both] == 1), "rays from outside meet the spheres' fronts"
+
+
+@pytest.mark.parametrize("waves", [0, 5, 6, 7])
+def test_split_stack_equals_wavefront(rt, waves):
+    """The triangle scene's tree is deep (36 stack levels): the quality megakernel keeps its stacks' deeper levels in
+    HBM (more of them at a tighter register budget, since more waves share the LDS); the sums are the wavefront's,
+    bit for bit"""
+    scene = rt.build_scene("triangles", width=48, samples=9, strata=3, sampling="mis")
+    with rt.Session(scene, seed=2, integrator="wavefront") as w:
+        w.render_to(9)
+        reference = w.sums()
+        counts = w.counters()[:2]
+    with rt.Session(scene, seed=2, integrator="megakernel", megakernel_waves=waves) as m:
+        m.render_to(4)
+        m.render_to(9)
+        got = m.sums()
+        assert m.counters()[:2] == counts
+    mism = np.count_nonzero(got.view(np.uint64) != reference.view(np.uint64))
+    assert mism == 0, f"{mism} of {got.size} sums differ"
