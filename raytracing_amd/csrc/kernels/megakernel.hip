@@ -511,8 +511,9 @@ size_t megakernel_far_stack_bytes(int depth)
     return static_cast<size_t>(depth) * static_cast<size_t>(cus) * 32 * 64 * 6;
 }
 
-// Passes in a queue chunk: a chunk is 64 x 4 paths for a wave, and a launch of C2 has ~128 K chunks
-constexpr int megakernel_pass_group = 4;
+// Passes in a queue chunk: a chunk is 64 x 2 paths for a wave, and a launch of C2 has ~530 K chunks (groups of 2
+// measured 1 % faster on C2 than 4, and within 1 % on C1 and C3; 8 and 16 6 % and 11 % slower)
+constexpr int megakernel_pass_group = 2;
 
 template <typename Real>
 hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
