@@ -345,7 +345,8 @@ typedef struct rt_session_options
 } rt_session_options;
 
 /* The BVH's splits. Both trees hold every primitive and find the same closest hit; they differ in how many boxes a
- * ray tests on the way (and which of two primitives hit at exactly the same distance is kept). */
+ * ray tests on the way (of two primitives hit at exactly the same distance, the one of lower index in the scene is
+ * kept with either). */
 enum rt_tree
 {
     RT_TREE_DEFAULT = 0, /* the surface-area heuristic */

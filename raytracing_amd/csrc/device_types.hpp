@@ -7,8 +7,8 @@
 //     that a node holds the boxes of up to FOUR children (128 bytes in float: one L2 line per visited node), where the
 //     reference's 32-byte node holds its own box and a visit reads two children's records from two places. A child is
 //     an inner node's index, or a packed leaf reference (leaf_ref below). The boxes are the binary tree's, the slab
-//     test is the reference's, and the closest hit is the same primitive at the same distance (two primitives at
-//     exactly the same distance may resolve either way: the walk's order differs).
+//     test is the reference's, and the closest hit is the same primitive at the same distance (of two primitives hit
+//     at exactly the same distance, the one of lower index in the scene, whatever the walk's order).
 //   - primitives keep the reference's global ids: spheres that stand still [0, moving_base), moving spheres
 //     [moving_base, quad_base), quads [quad_base, ...), and the media after the surfaces.
 #pragma once
