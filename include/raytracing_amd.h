@@ -355,7 +355,7 @@ enum rt_tree
     RT_TREE_SAH = 2,     /* binned surface-area-heuristic splits, leaves of one kind and at most 8 primitives */
 };
 /* RT_INTEGRATOR_AUTO (float32): the session's autotuner renders its first full passes with each candidate (the
- * megakernel at each register budget, one wave a block, and the wavefront), one pass each; the three fastest then render
+ * megakernel at each register budget, one wave a block, and the wavefront), a few passes each; the three fastest render
  * launches of several passes each, in a forward and a backward sweep, and the fastest of those renders the rest.
  * Every candidate traces the same paths with the same random numbers and sums them in the same order, so the image
  * doesn't depend on which one renders which pass. */

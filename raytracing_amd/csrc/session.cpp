@@ -142,10 +142,22 @@ public:
             }
             candidates_.push_back({true, 0, 256});
             current_ = candidates_.front();
-            // the first round: every candidate renders one pass (a screen; a one-pass launch is mostly its tail)
+            // the first round: every candidate renders a few passes (a screen), enough that the launch's chunk queue
+            // holds ~4 chunks for every wave the chip can hold (2048 threads a CU): with fewer, a budget of more waves
+            // wins the screen on its shorter tail alone (one pass of C1 screened out the compiler's budget, which
+            // renders C1 -p mis 9 % faster than the budget picked)
+            int cus = 0, device = 0;
+            if (hipGetDevice(&device) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
+            {
+                cus = 256;
+            }
+            const double pixels =
+                std::max(1.0, static_cast<double>(camera.image_width) * static_cast<double>(camera.image_height));
+            const int screen = static_cast<int>(std::clamp(std::ceil(4.0 * cus * 2048.0 / pixels), 1.0, 16.0));
             for (std::size_t k = 0; k < candidates_.size(); ++k)
             {
-                schedule_.push_back({k, 1});
+                schedule_.push_back({k, screen});
             }
             tune_seconds_.assign(candidates_.size(), 0.0);
         }
