@@ -155,6 +155,9 @@ public:
             const double pixels =
                 std::max(1.0, static_cast<double>(camera.image_width) * static_cast<double>(camera.image_height));
             const int screen = static_cast<int>(std::clamp(std::ceil(4.0 * cus * 2048.0 / pixels), 1.0, 16.0));
+            // the session's first launch pays one-time costs (the kernels' code loaded, the staging buffer's pages
+            // first touched): it renders the first candidate's passes untimed, and the screen follows
+            schedule_.push_back({0, screen, false});
             for (std::size_t k = 0; k < candidates_.size(); ++k)
             {
                 schedule_.push_back({k, screen});
@@ -236,7 +239,10 @@ public:
                 state_.passes += work.pass_count;
                 if (probe)
                 {
-                    tune_seconds_[schedule_[tuned_].candidate] += seconds / work.pass_count;
+                    if (schedule_[tuned_].timed)
+                    {
+                        tune_seconds_[schedule_[tuned_].candidate] += seconds / work.pass_count;
+                    }
                     if (++tuned_ == schedule_.size())
                     {
                         next_tuning_round();
@@ -540,7 +546,8 @@ private:
     struct probe
     {
         std::size_t candidate;
-        int passes; // at most: a launch never crosses a round of strata or the render's target
+        int passes;        // at most: a launch never crosses a round of strata or the render's target
+        bool timed = true; // false: a launch that renders its passes but whose time is not compared
     };
     std::vector<variant> candidates_;  // the autotuner's
     std::vector<probe> schedule_;      // the first full launches: which candidate renders each, how many passes
@@ -550,8 +557,7 @@ private:
     bool tuning() const { return tuned_ < schedule_.size(); }
 
     // After the screen (one pass each), the three fastest candidates are timed again on launches of several passes
-    // (about half a planned launch, up to 64 passes: close to the launches they will render, whose end weighs on the
-    // persistent megakernel's candidates differently), in a forward then a backward sweep so that
+    // (a planned launch's worth, up to 256 passes: the launches they will render), in a forward then a backward sweep so that
     // a drift in the timings (clocks, caches warming) weighs on all of them alike; then the fastest renders the rest
     void next_tuning_round()
     {
@@ -561,7 +567,7 @@ private:
         std::vector<int> probes(candidates_.size(), 0);
         for (const probe& p : schedule_)
         {
-            ++probes[p.candidate];
+            probes[p.candidate] += p.timed ? 1 : 0;
         }
         for (std::size_t k = 0; k < candidates_.size(); ++k)
         {
@@ -576,7 +582,7 @@ private:
             tuning_round_ = 1;
             const std::size_t finalists = std::min<std::size_t>(3, candidates_.size());
             const int passes = static_cast<int>(
-                std::clamp(0.5 * launch_seconds_ / std::max(per_pass[order[0]], 1e-9), 1.0, 64.0));
+                std::clamp(launch_seconds_ / std::max(per_pass[order[0]], 1e-9), 1.0, 256.0));
             schedule_.clear();
             tuned_ = 0;
             for (std::size_t k = 0; k < finalists; ++k)
