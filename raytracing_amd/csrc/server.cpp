@@ -150,6 +150,9 @@ void server::admit()
         {
             rt_session_options o{};
             o.seed = r.seed;
+            // the megakernel at the compiler's register budget: a request is too short to pay for the autotuner's
+            // probes (most of a 16-pass request would be rendered by them, the wavefront's among them)
+            o.integrator = RT_INTEGRATOR_MEGAKERNEL;
             r.work = make_session(scenes_.at(r.scene_id).scene, r.camera, o);
             used_bytes_ += r.bytes;
             r.st.state = 1;
