@@ -278,7 +278,7 @@ def main():
                           "GB/s": 2 * args.copy_probe / seconds / 1e9}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} asked)",
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (--gpus {args.gpus} asked)",
               file=sys.stderr)
         sys.exit(2)
 
