@@ -524,8 +524,22 @@ int rt_server_submit_view(rt_server* server, int64_t scene_id, const rt_camera_d
                           int64_t passes, int64_t* id);
 /* Queues a request with a scene of its own (copied and uploaded for it, freed when it is released) */
 int rt_server_submit(rt_server* server, const rt_scene* scene, uint64_t seed, int64_t passes, int64_t* id);
-/* One scheduling step: admits queued requests as memory and slots free up, and renders a step of every active one */
+/* One scheduling step (continuous batching): admits queued requests as memory and slots free up, enqueues the next step
+ * of every active request that has none in flight (up to its next power-of-two milestone, at most passes_per_step
+ * passes, on one of the server's few streams, so the steps of different requests run side by side on the device),
+ * then returns as soon as one step in flight has ended, with the number of requests still active in *active */
 int rt_server_step(rt_server* server, int32_t* active);
+
+/* The server's device memory: its budget; what it has reserved (resident scenes, and for each active request what its
+ * session may hold: image, staging for one step, stacks, counters); and what is held now (the scenes and the sessions'
+ * actual allocations, never more than reserved). No counterpart in the reference. */
+typedef struct rt_server_memory
+{
+    uint64_t budget_bytes;
+    uint64_t reserved_bytes;
+    uint64_t held_bytes;
+} rt_server_memory;
+int rt_server_memory_use(const rt_server* server, rt_server_memory* memory);
 int rt_server_status(const rt_server* server, int64_t id, rt_request_status* status);
 /* The image of a request's complete passes so far (W*H*3 doubles) */
 int rt_server_pixels(rt_server* server, int64_t id, double* out, size_t count);

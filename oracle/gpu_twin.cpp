@@ -457,8 +457,8 @@ struct hit
 // walk, and equal to the brute-force search's. Away from exact ties the two rules agree.
 inline bool better(const tree& tr, int id, float t, const hit& closest)
 {
-    return t < closest.t || (t == closest.t && tr.ranks[static_cast<std::size_t>(id)] <
-                                                   tr.ranks[static_cast<std::size_t>(closest.index)]);
+    return t < closest.t || (t == closest.t && closest.index >= 0 &&
+                             tr.ranks[static_cast<std::size_t>(id)] < tr.ranks[static_cast<std::size_t>(closest.index)]);
 }
 
 struct counters

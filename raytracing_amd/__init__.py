@@ -170,6 +170,10 @@ class RequestStatus(C.Structure):
                 ("start_time", C.c_double), ("milestone_time", C.c_double), ("finish_time", C.c_double)]
 
 
+class ServerMemory(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("reserved_bytes", C.c_uint64), ("held_bytes", C.c_uint64)]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 128), ("arch", C.c_char * 64), ("compute_units", C.c_int32),
                 ("reserved", C.c_int32), ("memory_gib", C.c_double), ("runtime_version", C.c_int32),
@@ -217,6 +221,7 @@ _SIGNATURES = {
                                         _P(C.c_int64)]),
     "rt_server_step": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
     "rt_server_status": (C.c_int, [C.c_void_p, C.c_int64, _P(RequestStatus)]),
+    "rt_server_memory_use": (C.c_int, [C.c_void_p, _P(ServerMemory)]),
     "rt_server_pixels": (C.c_int, [C.c_void_p, C.c_int64, _P(C.c_double), C.c_size_t]),
     "rt_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
     "rt_query_device": (C.c_int, [_P(DeviceInfo)]),
@@ -536,6 +541,11 @@ class Server:
         active = C.c_int32()
         _check(_lib.rt_server_step(self._h, C.byref(active)))
         return active.value
+
+    def memory(self) -> ServerMemory:
+        m = ServerMemory()
+        _check(_lib.rt_server_memory_use(self._h, C.byref(m)))
+        return m
 
     def status(self, rid: int) -> RequestStatus:
         st = RequestStatus()

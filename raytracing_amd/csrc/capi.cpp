@@ -455,6 +455,15 @@ int rt_server_step(rt_server* server, int32_t* active)
     });
 }
 
+int rt_server_memory_use(const rt_server* server, rt_server_memory* memory)
+{
+    return guarded([&] {
+        need(server, "server");
+        need(memory, "memory");
+        *memory = server->impl->memory();
+    });
+}
+
 int rt_server_status(const rt_server* server, int64_t id, rt_request_status* status)
 {
     return guarded([&] {

@@ -378,6 +378,10 @@ std::string percent_decoded(const std::string& uri)
     return out;
 }
 
+// The most elements one accessor may hold: 2^28 (a mesh of 268 M vertices or 89 M triangles), far beyond any model
+// this renderer's memory holds, and small enough that element counts times components or strides never wrap
+constexpr std::size_t max_accessor_elements = std::size_t{1} << 28;
+
 std::size_t index_of(const json* j, const char* what)
 {
     if (!j)
@@ -697,11 +701,17 @@ private:
             throw std::runtime_error("an accessor of component type " + std::to_string(component_type));
         }
         const std::size_t count = index_of(a.find("count"), "accessor count");
-        std::vector<double> out(count * static_cast<std::size_t>(components), 0.0);
+        // Every bound below is checked by subtraction or division, never by a sum or product that could wrap modulo
+        // 2^64 (a crafted count or stride would otherwise pass the check and the reads would leave the buffer)
+        if (count > max_accessor_elements)
+        {
+            throw std::runtime_error("an accessor of " + std::to_string(count) + " elements, more than this loader reads");
+        }
         const json* view_index = a.find("bufferView");
         if (!view_index)
         {
-            return out; // no buffer view: zeros (glTF 2.0 section 3.6.2)
+            // no buffer view: zeros (glTF 2.0 section 3.6.2)
+            return std::vector<double>(count * static_cast<std::size_t>(components), 0.0);
         }
         const json_array& views = array_or_empty(root_, "bufferViews");
         const std::size_t vi = index_of(view_index, "bufferView");
@@ -715,19 +725,32 @@ private:
         {
             throw std::runtime_error("a buffer index past the buffers");
         }
+        const std::size_t buffer_size = buffers_[buffer].size();
         const std::size_t view_offset = v.find("byteOffset") ? index_of(v.find("byteOffset"), "byteOffset") : 0;
         const std::size_t view_length = index_of(v.find("byteLength"), "bufferView byteLength");
         const std::size_t element = size * static_cast<std::size_t>(components);
-        const std::size_t stride = v.find("byteStride") ? index_of(v.find("byteStride"), "byteStride") : element;
+        const bool strided = v.find("byteStride") != nullptr;
+        const std::size_t stride = strided ? index_of(v.find("byteStride"), "byteStride") : element;
         const std::size_t offset = a.find("byteOffset") ? index_of(a.find("byteOffset"), "byteOffset") : 0;
-        if (view_offset + view_length > buffers_[buffer].size() || stride < element)
+        if (view_offset > buffer_size || view_length > buffer_size - view_offset)
         {
             throw std::runtime_error("a buffer view past its buffer");
         }
-        if (count > 0 && offset + (count - 1) * stride + element > view_length)
+        // glTF 2.0 section 5.11: byteStride in [4, 252], and no smaller than an element
+        if (strided && (stride < 4 || stride > 252))
+        {
+            throw std::runtime_error("a buffer view of byteStride " + std::to_string(stride) + ", outside [4, 252]");
+        }
+        if (stride < element)
+        {
+            throw std::runtime_error("a buffer view whose byteStride is smaller than its accessor's elements");
+        }
+        if (count > 0 && (offset > view_length || element > view_length - offset ||
+                          count - 1 > (view_length - offset - element) / stride))
         {
             throw std::runtime_error("an accessor past its buffer view");
         }
+        std::vector<double> out(count * static_cast<std::size_t>(components), 0.0);
         const uint8_t* base = buffers_[buffer].data() + view_offset + offset;
         for (std::size_t e = 0; e < count; ++e)
         {

@@ -30,13 +30,15 @@ size_t megakernel_staging_per_pass(int width, int height);
 // than fit is split into several. queue: one device word, the chunk queue's head (cleared by the launch itself).
 // block_threads: 64, 128, 192 or 256 (one to four waves a block; the waves work independently). far_buffer (may be
 // null): far_bytes of device memory where the quality megakernel keeps its stacks' deeper levels when the whole
-// stacks in LDS would cost occupancy (megakernel_far_stack_bytes(depth) bytes always suffice).
+// stacks in LDS would cost occupancy (megakernel_far_stack_bytes(depth) bytes always suffice). grid_ways: the
+// persistent grid takes a 1/grid_ways share of the waves the chip holds (several launches on different streams then
+// run side by side, as the serving loop's do).
 template <typename Real>
 hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
                              uint64_t seed, const dev::launch_work& work, double* sums, void* staging,
                              size_t staging_samples, uint32_t* queue, unsigned long long* rays, int block_threads,
                              hipStream_t stream, unsigned long long* tallies = nullptr, int waves = 0,
-                             void* far_buffer = nullptr, size_t far_bytes = 0);
+                             void* far_buffer = nullptr, size_t far_bytes = 0, int grid_ways = 1);
 size_t megakernel_far_stack_bytes(int depth);
 bool megakernel_waves_supported(int waves);
 

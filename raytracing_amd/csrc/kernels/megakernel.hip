@@ -448,9 +448,9 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
                              uint64_t seed, const dev::launch_work& work, double* sums, void* staging,
                              size_t staging_samples, uint32_t* queue, unsigned long long* rays, int block_threads,
                              hipStream_t stream, unsigned long long* tallies, int waves, void* far_buffer,
-                             size_t far_bytes)
+                             size_t far_bytes, int grid_ways)
 {
-    if ((mode == dev::mode_quality && sizeof(Real) != sizeof(float)) || block_threads % 64 != 0 ||
+    if ((mode == dev::mode_quality && sizeof(Real) != sizeof(float)) || block_threads % 64 != 0 || grid_ways < 1 ||
         block_threads < 64 || block_threads > 256 || work.pass_count < 1 || work.row_count < 1 ||
         staging_samples > static_cast<size_t>(INT32_MAX))
     {
@@ -521,7 +521,7 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
         dev::chunk_queue q{queue, tiles_x, tiles, tiles * groups, group, cam.image_width, w};
         const uint32_t waves_per_block = static_cast<uint32_t>(block_threads / 64);
         const uint32_t blocks = std::max<uint32_t>(
-            1, std::min<uint32_t>(static_cast<uint32_t>(std::max(1, per_cu * cus)),
+            1, std::min<uint32_t>(static_cast<uint32_t>(std::max(1, per_cu * cus / grid_ways)),
                                   (q.chunks + waves_per_block - 1) / waves_per_block));
         hipLaunchKernelGGL(dev::clear_queue, dim3(1), dim3(1), 0, stream, queue);
         dev::camera_rec<Real> c = cam;
@@ -590,11 +590,11 @@ kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, 
 template hipError_t launch_megakernel<float>(int, const dev::camera_rec<float>&, const dev::scene_view<float>&,
                                              uint64_t, const dev::launch_work&, double*, void*, size_t, uint32_t*,
                                              unsigned long long*, int, hipStream_t, unsigned long long*, int, void*,
-                                             size_t);
+                                             size_t, int);
 template hipError_t launch_megakernel<double>(int, const dev::camera_rec<double>&, const dev::scene_view<double>&,
                                               uint64_t, const dev::launch_work&, double*, void*, size_t, uint32_t*,
                                               unsigned long long*, int, hipStream_t, unsigned long long*, int, void*,
-                                              size_t);
+                                              size_t, int);
 template hipError_t launch_closest_hit<float>(const dev::scene_view<float>&, const float*, const float*, const float*,
                                               const int32_t*, const uint8_t*, int64_t, int32_t*, float*, uint8_t*,
                                               hipStream_t);

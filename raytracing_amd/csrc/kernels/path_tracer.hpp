@@ -500,10 +500,12 @@ __device__ inline lds_stack stack_in(unsigned char* lds, int entries, int thread
 // (a sphere) or last (a quad), which depends on the walk's order; with this rule the closest hit is a function of the
 // ray and the scene alone, the same for any tree and for the four-wide walk as for the twin's binary one. The ranks
 // are read only on an exact tie.
+// (A first candidate at an infinite t, which an overflowing quad or triangle distance could give while nothing is hit
+// yet, is no hit: the ranks are read only once a closest hit exists.)
 template <typename Real>
 __device__ inline bool better(const scene_view<Real>& sc, int id, Real t, const hit<Real>& closest)
 {
-    return t < closest.t || sc.prim_ranks[id] < sc.prim_ranks[closest.index];
+    return t < closest.t || (closest.index >= 0 && sc.prim_ranks[id] < sc.prim_ranks[closest.index]);
 }
 
 // The primitives of a leaf against the ray, the closest hit so far updated (src/gpu/bvh.cuh:612-680)

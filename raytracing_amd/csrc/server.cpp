@@ -10,14 +10,6 @@
 
 namespace rt
 {
-std::size_t view_bytes(const rt_camera_desc& camera)
-{
-    const std::size_t pixels = static_cast<std::size_t>(camera.image_width) * camera.image_height;
-    return 24 * pixels          // the double sums
-           + 8 * 16 * pixels    // the wavefront's slots (rays, throughput, light, generator, hit, lists, queues)
-           + (1u << 20);
-}
-
 server::server(const rt_server_options& options) : options_(options)
 {
     if (options_.memory_budget_bytes == 0)
@@ -31,6 +23,12 @@ server::server(const rt_server_options& options) : options_(options)
     if (options_.passes_per_step <= 0)
     {
         options_.passes_per_step = 4;
+    }
+    // no more streams than the hardware queues a process gets by default (4): more would only contend for them
+    const int streams = std::min(4, options_.max_active);
+    for (int k = 0; k < streams; ++k)
+    {
+        streams_.emplace_back();
     }
 }
 
@@ -96,8 +94,8 @@ int64_t server::submit_view(int64_t scene_id, const rt_camera_desc& camera, uint
     r->camera = camera;
     r->seed = seed;
     r->passes = passes;
-    r->bytes = view_bytes(camera);
-    if (r->bytes + f->second.bytes > options_.memory_budget_bytes)
+    if (view_device_bytes(*f->second.scene, camera, options_.passes_per_step) + f->second.bytes >
+        options_.memory_budget_bytes)
     {
         throw std::invalid_argument("the request needs more device memory than the server's whole budget");
     }
@@ -141,7 +139,8 @@ void server::admit()
     while (!queue_.empty() && active < options_.max_active)
     {
         request& r = *requests_.at(queue_.front());
-        if (used_bytes_ + r.bytes > options_.memory_budget_bytes)
+        const std::size_t bytes = view_device_bytes(*scenes_.at(r.scene_id).scene, r.camera, options_.passes_per_step);
+        if (used_bytes_ + bytes > options_.memory_budget_bytes)
         {
             break;
         }
@@ -154,24 +153,49 @@ void server::admit()
             // probes (most of a 16-pass request would be rendered by them, the wavefront's among them)
             o.integrator = RT_INTEGRATOR_MEGAKERNEL;
             r.work = make_session(scenes_.at(r.scene_id).scene, r.camera, o);
-            used_bytes_ += r.bytes;
+            // a stream of the server's few, shared in turn; staging for one step's passes at most
+            r.stream = next_stream_;
+            next_stream_ = (next_stream_ + 1) % streams_.size();
+            r.work->serve_on(streams_[r.stream].get(), options_.passes_per_step);
+            r.bytes = bytes;
+            used_bytes_ += bytes;
             r.st.state = 1;
             r.st.start_time = now();
             ++active;
         }
         catch (const std::exception&)
         {
+            r.work.reset();
             r.st.state = 3;
             r.st.finish_time = now();
         }
     }
 }
 
+// A request's step ended: its passes counted, its milestone and completion stamped
+void server::finish_step(request& r)
+{
+    const int64_t target = r.in_flight;
+    r.in_flight = 0;
+    r.st.passes_done = target;
+    if ((target & (target - 1)) == 0 || target == r.passes)
+    {
+        r.st.milestone = target;
+        r.st.milestone_time = now();
+    }
+    if (target >= r.passes)
+    {
+        r.st.state = 2;
+        r.st.finish_time = now();
+    }
+}
+
 int server::step()
 {
     admit();
-    // every active request renders its step at once, each from a host thread of its own on its session's own
-    // stream, so their kernels overlap on the device; a request's step ends at its next power-of-two milestone
+    // Every active request without a step in flight gets its next one, enqueued on its stream without waiting: up to
+    // the next power-of-two milestone, at most passes_per_step passes. The steps of requests on different streams run
+    // side by side on the device, each persistent grid on a share of the chip.
     std::vector<request*> active;
     for (auto& [id, rp] : requests_)
     {
@@ -180,51 +204,64 @@ int server::step()
             active.push_back(rp.get());
         }
     }
-    const auto run = [this](request& r) {
-        const int64_t done = r.st.passes_done;
+    const int ways = static_cast<int>(std::min(active.size(), streams_.size()));
+    for (request* r : active)
+    {
+        if (r->in_flight != 0)
+        {
+            continue;
+        }
+        const int64_t done = r->st.passes_done;
         int64_t next_milestone = 1;
         while (next_milestone <= done)
         {
             next_milestone *= 2;
         }
-        const int64_t target = std::min({r.passes, done + options_.passes_per_step, std::max(next_milestone, done + 1)});
+        const int64_t target = std::min({r->passes, done + options_.passes_per_step, std::max(next_milestone, done + 1)});
         try
         {
-            r.work->render_to(target, 0);
+            r->work->enqueue_to(target, std::max(1, ways));
+            r->in_flight = target;
         }
         catch (const std::exception&)
         {
-            r.st.state = 3;
-            r.st.finish_time = now();
-            return;
+            r->st.state = 3;
+            r->st.finish_time = now();
         }
-        r.st.passes_done = target;
-        if ((target & (target - 1)) == 0 || target == r.passes)
-        {
-            r.st.milestone = target;
-            r.st.milestone_time = now();
-        }
-        if (target >= r.passes)
-        {
-            r.st.state = 2;
-            r.st.finish_time = now();
-        }
-    };
-    if (active.size() == 1)
-    {
-        run(*active[0]);
     }
-    else
+    // Then wait for the first step to end (any request's, not all of them): its request is done or steps again at the
+    // next call, and a request that arrived meanwhile is admitted as soon as memory and a slot are free
+    bool ended = active.empty();
+    while (!ended)
     {
-        std::vector<std::thread> workers;
-        workers.reserve(active.size());
         for (request* r : active)
         {
-            workers.emplace_back([&run, r] { run(*r); });
+            if (r->st.state != 1 || r->in_flight == 0)
+            {
+                continue;
+            }
+            bool over = false;
+            try
+            {
+                over = r->work->poll();
+            }
+            catch (const std::exception&)
+            {
+                r->in_flight = 0;
+                r->st.state = 3;
+                r->st.finish_time = now();
+                ended = true;
+                continue;
+            }
+            if (over)
+            {
+                finish_step(*r);
+                ended = true;
+            }
         }
-        for (std::thread& w : workers)
+        if (!ended)
         {
-            w.join();
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
         }
     }
     int still = 0;
@@ -233,6 +270,24 @@ int server::step()
         still += r->st.state == 1 ? 1 : 0;
     }
     return still;
+}
+
+rt_server_memory server::memory() const
+{
+    rt_server_memory m{};
+    m.budget_bytes = options_.memory_budget_bytes;
+    m.reserved_bytes = used_bytes_;
+    uint64_t held = 0;
+    for (const auto& [id, sc] : scenes_)
+    {
+        held += sc.bytes;
+    }
+    for (const auto& [id, r] : requests_)
+    {
+        held += r->work ? r->work->view_device_bytes() : 0;
+    }
+    m.held_bytes = held;
+    return m;
 }
 
 rt_request_status server::status(int64_t id) const

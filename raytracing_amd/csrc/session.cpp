@@ -29,6 +29,8 @@ public:
     hip_event& operator=(const hip_event&) = delete;
     void record(hipStream_t s) { check(hipEventRecord(e_, s), "recording a HIP event"); }
     void synchronize() { check(hipEventSynchronize(e_), "rendering on the GPU"); }
+    hipError_t query() const { return hipEventQuery(e_); }
+    hipEvent_t get() const { return e_; }
     static double seconds_between(const hip_event& a, const hip_event& b)
     {
         float ms = 0;
@@ -109,6 +111,45 @@ const rt_scene& scene_of(const shared_scene& scene) { return scene.view; }
 
 namespace
 {
+// The passes the megakernel's staging buffer holds: at most 1 GiB of staged samples and 1024 passes, or `cap` passes
+// (a serving session's step) when that is fewer
+template <typename Real>
+std::size_t staged_passes(int width, int height, int cap)
+{
+    constexpr std::size_t staging_bytes_cap = std::size_t{1} << 30;
+    const std::size_t per_pass = megakernel_staging_per_pass(width, height);
+    std::size_t passes = std::clamp<std::size_t>(staging_bytes_cap / (per_pass * sizeof(dev::staged_sample<Real>)), 1, 1024);
+    if (cap > 0)
+    {
+        passes = std::min(passes, static_cast<std::size_t>(cap));
+    }
+    return passes;
+}
+
+// The quality megakernel's stacks go partly to HBM on a tree deeper than this (session_impl::ensure_far_stack)
+constexpr int far_stack_depth = 16;
+}
+
+std::size_t view_device_bytes(const shared_scene& scene, const rt_camera_desc& camera, int staging_passes)
+{
+    const std::size_t pixels = static_cast<std::size_t>(camera.image_width) * static_cast<std::size_t>(camera.image_height);
+    const bool f32 = scene.f32 != nullptr;
+    const std::size_t staged = f32 ? staged_passes<float>(camera.image_width, camera.image_height, staging_passes) *
+                                         sizeof(dev::staged_sample<float>)
+                                   : staged_passes<double>(camera.image_width, camera.image_height, staging_passes) *
+                                         sizeof(dev::staged_sample<double>);
+    std::size_t bytes = 3 * sizeof(double) * pixels                                        // the sums
+                        + staged * megakernel_staging_per_pass(camera.image_width, camera.image_height) // staging
+                        + 2 * sizeof(unsigned long long) + sizeof(uint32_t);               // counters, queue head
+    if (f32 && scene.f32->quality() && scene.f32->view().stack_depth > far_stack_depth)
+    {
+        bytes += megakernel_far_stack_bytes(scene.f32->view().stack_depth);
+    }
+    return bytes;
+}
+
+namespace
+{
 template <typename Real>
 class session_impl final : public session
 {
@@ -174,8 +215,17 @@ public:
         start_view(camera);
     }
 
+    ~session_impl() override
+    {
+        if (pending_ != 0)
+        {
+            (void)hipEventSynchronize(done_.get()); // a step in flight ends before its buffers are freed
+        }
+    }
+
     void set_camera(const rt_camera_desc& camera) override
     {
+        settle();
         validate_camera(camera);
         if (camera.sampling != scene_.camera.sampling)
         {
@@ -189,6 +239,7 @@ public:
 
     rt_render_report render_to(int64_t target_passes, int32_t target_rows) override
     {
+        settle();
         require_view();
         const int height = camera_.image_height;
         if (target_passes < 0 || target_rows < 0 || target_rows >= height ||
@@ -307,17 +358,19 @@ public:
 
     void sums(double* out, std::size_t count) override
     {
+        settle();
         require_view();
         if (count < values())
         {
             throw std::invalid_argument("the output buffer is smaller than width x height x 3");
         }
-        check(hipStreamSynchronize(stream_.get()), "waiting for the GPU");
+        check(hipStreamSynchronize(stream()), "waiting for the GPU");
         check(hipMemcpy(out, sums_.data(), values() * sizeof(double), hipMemcpyDeviceToHost), "copying the image");
     }
 
     void display(uint8_t* out, std::size_t count) override
     {
+        settle();
         require_view();
         if (state_.passes == 0)
         {
@@ -333,9 +386,9 @@ public:
             rgba_ = device_array<uint8_t>(4 * pixels);
         }
         check(launch_display(sums_.data(), static_cast<int64_t>(pixels), 1.0 / static_cast<double>(state_.passes),
-                             rgba_.data(), stream_.get()),
+                             rgba_.data(), stream()),
               "launching the display kernel");
-        check(hipStreamSynchronize(stream_.get()), "computing the display");
+        check(hipStreamSynchronize(stream()), "computing the display");
         check(hipMemcpy(out, rgba_.data(), 4 * pixels, hipMemcpyDeviceToHost), "copying the display");
     }
 
@@ -378,9 +431,9 @@ public:
         }
         check(launch_tone_map(sums_.data(), camera_.image_width, static_cast<int64_t>(pixels),
                               1.0 / static_cast<double>(state_.passes), tone_map,
-                              static_cast<float>(std::exp2(exposure)), rgba_.data(), stream_.get()),
+                              static_cast<float>(std::exp2(exposure)), rgba_.data(), stream()),
               "launching the tone map");
-        check(hipStreamSynchronize(stream_.get()), "computing the display");
+        check(hipStreamSynchronize(stream()), "computing the display");
         check(hipMemcpy(out, rgba_.data(), 4 * pixels, hipMemcpyDeviceToHost), "copying the display");
     }
 
@@ -466,9 +519,9 @@ public:
             check(hipMemcpy(f.data(), from, f.bytes(), hipMemcpyHostToDevice), "copying rays");
             check(hipMemcpy(in.data(), inward, in.bytes(), hipMemcpyHostToDevice), "copying rays");
             check(launch_closest_hit<float>(dscene_.view(), o.data(), d.data(), tm.data(), f.data(), in.data(), n,
-                                            idx.data(), tt.data(), ent.data(), stream_.get()),
+                                            idx.data(), tt.data(), ent.data(), stream()),
                   "launching the closest-hit probe");
-            check(hipStreamSynchronize(stream_.get()), "tracing the probe rays");
+            check(hipStreamSynchronize(stream()), "tracing the probe rays");
             check(hipMemcpy(index, idx.data(), idx.bytes(), hipMemcpyDeviceToHost), "copying hits");
             check(hipMemcpy(t, tt.data(), tt.bytes(), hipMemcpyDeviceToHost), "copying hits");
             check(hipMemcpy(entering, ent.data(), ent.bytes(), hipMemcpyDeviceToHost), "copying hits");
@@ -477,6 +530,7 @@ public:
 
     rt_traffic traffic(int64_t passes) override
     {
+        settle();
         require_view();
         const int64_t round_size = static_cast<int64_t>(camera_.sqrt_spp) * camera_.sqrt_spp;
         if (passes < 1 || passes > round_size)
@@ -486,15 +540,15 @@ public:
         device_array<double> scratch(values());
         // [rays, shadow rays, the tally events]
         device_array<unsigned long long> counts(2 + RT_TRAFFIC_EVENTS);
-        check(hipMemsetAsync(scratch.data(), 0, scratch.bytes(), stream_.get()), "clearing scratch");
-        check(hipMemsetAsync(counts.data(), 0, counts.bytes(), stream_.get()), "clearing counters");
+        check(hipMemsetAsync(scratch.data(), 0, scratch.bytes(), stream()), "clearing scratch");
+        check(hipMemsetAsync(counts.data(), 0, counts.bytes(), stream()), "clearing counters");
         const dev::launch_work work{0, static_cast<int32_t>(passes), 0, camera_.image_height};
         ensure_staging();
         check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), options_.seed, work, scratch.data(),
                                       staging_.data(), staging_samples_, queue_.data(), counts.data(), 128,
-                                      stream_.get(), counts.data() + 2),
+                                      stream(), counts.data() + 2),
               "launching the counting kernel");
-        check(hipStreamSynchronize(stream_.get()), "counting traffic");
+        check(hipStreamSynchronize(stream()), "counting traffic");
         unsigned long long host[2 + RT_TRAFFIC_EVENTS] = {};
         check(hipMemcpy(host, counts.data(), sizeof host, hipMemcpyDeviceToHost), "reading counters");
         rt_traffic t{};
@@ -520,13 +574,106 @@ public:
         return t;
     }
 
-    std::size_t device_bytes() const override
+    std::size_t device_bytes() const override { return dscene_.bytes() + view_device_bytes(); }
+
+    std::size_t view_device_bytes() const override
     {
-        return dscene_.bytes() + sums_.bytes() + staging_.bytes() + far_stack_.bytes() +
+        return sums_.bytes() + staging_.bytes() + far_stack_.bytes() + rays_.bytes() + queue_.bytes() + rgba_.bytes() +
                (wavefront_ ? wavefront_->bytes() : 0);
     }
 
+    void serve_on(void* shared_stream, int staging_passes) override
+    {
+        if (staging_passes < 0)
+        {
+            throw std::invalid_argument("0 or more staged passes");
+        }
+        settle();
+        check(hipStreamSynchronize(stream()), "waiting for the GPU");
+        serve_stream_ = static_cast<hipStream_t>(shared_stream);
+        if (staging_passes != staging_cap_)
+        {
+            staging_cap_ = staging_passes;
+            staging_ = device_array<dev::staged_sample<Real>>(); // sized again at the next launch
+            staging_samples_ = 0;
+        }
+    }
+
+    void enqueue_to(int64_t target, int grid_ways) override
+    {
+        require_view();
+        if (grid_ways < 1)
+        {
+            throw std::invalid_argument("a share of the chip of 1/n with n >= 1");
+        }
+        if (pending_ != 0)
+        {
+            throw std::logic_error("the session's last step hasn't been collected");
+        }
+        if (state_.rows != 0 || target <= state_.passes)
+        {
+            throw std::invalid_argument("a step adds whole passes to an image without a pass in progress");
+        }
+        if (current_.wavefront || tuning())
+        {
+            throw std::logic_error("steps render on the megakernel of a session that isn't tuning");
+        }
+        ensure_staging();
+        ensure_far_stack();
+        const int64_t round_size = static_cast<int64_t>(camera_.sqrt_spp) * camera_.sqrt_spp;
+        for (int64_t pass = state_.passes; pass < target;)
+        {
+            dev::launch_work work{};
+            work.first_pass = static_cast<int32_t>(pass % round_size);
+            work.pass_count = static_cast<int32_t>(std::min(target - pass, round_size - work.first_pass));
+            work.first_row = 0;
+            work.row_count = camera_.image_height;
+            const uint64_t seed = options_.seed + static_cast<uint64_t>(pass / round_size) * round_seed_step;
+            check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), staging_.data(),
+                                          staging_samples_, queue_.data(), rays_.data(), current_.block, stream(),
+                                          nullptr, current_.waves, far_stack_.data(), far_stack_.bytes(), grid_ways),
+                  "launching the kernel");
+            pass += work.pass_count;
+        }
+        done_.record(stream());
+        pending_ = target;
+    }
+
+    bool poll() override
+    {
+        if (pending_ == 0)
+        {
+            return true;
+        }
+        const hipError_t e = done_.query();
+        if (e == hipErrorNotReady)
+        {
+            return false;
+        }
+        check(e, "rendering on the GPU");
+        state_.passes = pending_;
+        pending_ = 0;
+        return true;
+    }
+
 private:
+    // serving: the stream the launches go to (null: the session's own), the staged passes a launch is capped at (0:
+    // the default), and the passes of the step in flight (0: none)
+    hipStream_t serve_stream_ = nullptr;
+    int staging_cap_ = 0;
+    int64_t pending_ = 0;
+    hip_event done_;
+    hipStream_t stream() const { return serve_stream_ ? serve_stream_ : stream_.get(); }
+    // a step in flight waited for, and its passes added to the image's state
+    void settle()
+    {
+        if (pending_ != 0)
+        {
+            done_.synchronize();
+            poll();
+        }
+    }
+
     std::shared_ptr<const shared_scene> shared_;
     const scene_tables& scene_;
     rt_scene view_scene_{};
@@ -608,15 +755,12 @@ private:
     // the megakernel's chunk queue head, and its staged samples: one per pixel per pass, for up to staging_passes_
     // passes a kernel
     device_array<uint32_t> queue_;
-    static constexpr std::size_t staging_bytes_cap = std::size_t{1} << 30;
     device_array<dev::staged_sample<Real>> staging_;
     std::size_t staging_samples_ = 0;
     int staging_passes_ = 0;
     int64_t max_staged_passes() const
     {
-        const std::size_t per_pass = megakernel_staging_per_pass(camera_.image_width, camera_.image_height);
-        return static_cast<int64_t>(
-            std::clamp<std::size_t>(staging_bytes_cap / (per_pass * sizeof(dev::staged_sample<Real>)), 1, 1024));
+        return static_cast<int64_t>(staged_passes<Real>(camera_.image_width, camera_.image_height, staging_cap_));
     }
     void ensure_staging()
     {
@@ -625,8 +769,7 @@ private:
             return;
         }
         const std::size_t per_pass = megakernel_staging_per_pass(camera_.image_width, camera_.image_height);
-        const std::size_t passes =
-            std::clamp<std::size_t>(staging_bytes_cap / (per_pass * sizeof(dev::staged_sample<Real>)), 1, 1024);
+        const std::size_t passes = staged_passes<Real>(camera_.image_width, camera_.image_height, staging_cap_);
         staging_ = device_array<dev::staged_sample<Real>>(per_pass * passes);
         staging_samples_ = per_pass * passes;
         staging_passes_ = static_cast<int>(passes);
@@ -636,7 +779,7 @@ private:
     void ensure_far_stack()
     {
         const int depth = dscene_.view().stack_depth;
-        if (mode_ == dev::mode_quality && depth > 16 && far_stack_.size() == 0)
+        if (mode_ == dev::mode_quality && depth > far_stack_depth && far_stack_.size() == 0)
         {
             far_stack_ = device_array<uint8_t>(megakernel_far_stack_bytes(depth));
         }
@@ -670,15 +813,15 @@ private:
             sums_ = device_array<double>();
             sums_ = device_array<double>(n);
         }
-        check(hipMemsetAsync(sums_.data(), 0, sums_.bytes(), stream_.get()), "clearing the image");
-        check(hipMemsetAsync(rays_.data(), 0, rays_.bytes(), stream_.get()), "clearing the counters");
+        check(hipMemsetAsync(sums_.data(), 0, sums_.bytes(), stream()), "clearing the image");
+        check(hipMemsetAsync(rays_.data(), 0, rays_.bytes(), stream()), "clearing the counters");
         wavefront_.reset(); // its pool is the view's size
         if (staging_samples_ != megakernel_staging_per_pass(camera.image_width, camera.image_height) * staging_passes_)
         {
             staging_ = device_array<dev::staged_sample<Real>>(); // the megakernel's, sized again at its next launch
             staging_samples_ = 0;
         }
-        check(hipStreamSynchronize(stream_.get()), "starting the view");
+        check(hipStreamSynchronize(stream()), "starting the view");
         state_ = {};
         kernel_seconds_ = 0;
         seconds_per_pass_ = 0;
@@ -690,9 +833,9 @@ private:
     {
         if (v.wavefront && !wavefront_)
         {
-            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, mode_, options_, stream_.get());
+            wavefront_ = std::make_unique<wavefront<Real>>(dscene_.view(), cam_, mode_, options_, stream());
         }
-        start_.record(stream_.get());
+        start_.record(stream());
         last_bounces_ = 0;
         if (v.wavefront)
         {
@@ -711,11 +854,11 @@ private:
             ensure_staging();
             ensure_far_stack();
             check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), staging_.data(),
-                                          staging_samples_, queue_.data(), rays_.data(), v.block, stream_.get(),
+                                          staging_samples_, queue_.data(), rays_.data(), v.block, stream(),
                                           nullptr, v.waves, far_stack_.data(), far_stack_.bytes()),
                   "launching the kernel");
         }
-        stop_.record(stream_.get());
+        stop_.record(stream());
         stop_.synchronize();
         const double seconds = hip_event::seconds_between(start_, stop_);
         kernel_seconds_ += seconds;

@@ -38,6 +38,19 @@ public:
                              const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) = 0;
     virtual std::size_t device_bytes() const = 0;
     virtual rt_traffic traffic(int64_t passes) = 0;
+
+    // Serving (the server steps many sessions at once). The session's launches go to `stream` (a stream the caller
+    // owns and may share between sessions; null: the session's own), and its megakernel stages at most staging_passes
+    // passes a launch (0: the session's default).
+    virtual void serve_on(void* stream, int staging_passes) = 0;
+    // Whole passes up to `passes` enqueued on the stream without waiting for them (megakernel launches, each within a
+    // round of strata, with a persistent grid of a 1/grid_ways share of the chip); a step already in flight must have
+    // been collected first (RT_ERROR_LOGIC otherwise)
+    virtual void enqueue_to(int64_t passes, int grid_ways) = 0;
+    // Whether the enqueued step has ended; once it has, the image holds its passes (state(), pixels())
+    virtual bool poll() = 0;
+    // The device memory the session holds beside its (shared) scene: its image, staging, stacks, counters
+    virtual std::size_t view_device_bytes() const = 0;
 };
 
 // A scene resident on the device: its tables and its arrays (tree, records, textures) in one precision, shared by
@@ -48,6 +61,10 @@ std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t 
                                                  int32_t tree = RT_TREE_DEFAULT);
 std::size_t device_bytes(const shared_scene& scene);
 const rt_scene& scene_of(const shared_scene& scene);
+
+// The device memory a megakernel session of this view of a resident scene holds beside the scene once it has
+// rendered, with its staging sized for staging_passes passes a launch: an upper bound of view_device_bytes()
+std::size_t view_device_bytes(const shared_scene& scene, const rt_camera_desc& camera, int staging_passes);
 
 // Validates the scene (before any GPU work), then builds its BVH and uploads it
 std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_options& options);

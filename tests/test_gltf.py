@@ -265,6 +265,42 @@ def test_refusals(rt, tmp_path):
         rt.build_scene(str(tmp_path / "nowhere.glb"))
 
 
+def test_crafted_bounds_are_refused(rt, tmp_path):
+    """Accessor and view bounds whose sums or products would wrap modulo 2^64 are refused cleanly, before any read
+    (count 32769 with byteStride 2^49 passed a summed check and read far past the buffer)"""
+    def model():
+        w = Writer()
+        w.mesh({"attributes": {"POSITION": w.positions(TRIANGLE)}, "indices": w.indices([0, 1, 2])})
+        w.node(mesh=0)
+        return w
+
+    w = model()
+    w.accessors[0]["count"] = 32769
+    w.views[0]["byteStride"] = 2 ** 49
+    with pytest.raises(RuntimeError, match="byteStride"):
+        rt.build_scene(w.save(str(tmp_path / "wrap.gltf")))
+    w = model()
+    w.views[0]["byteStride"] = 2  # below glTF's minimum of 4
+    with pytest.raises(RuntimeError, match="byteStride"):
+        rt.build_scene(w.save(str(tmp_path / "stride2.gltf")))
+    w = model()
+    w.views[0]["byteStride"] = 8  # within [4, 252] but smaller than a VEC3 of floats
+    with pytest.raises(RuntimeError, match="smaller than"):
+        rt.build_scene(w.save(str(tmp_path / "stride8.gltf")))
+    w = model()
+    w.accessors[0]["byteOffset"] = 10 ** 15  # far past the view
+    with pytest.raises(RuntimeError, match="past its buffer view"):
+        rt.build_scene(w.save(str(tmp_path / "offset.gltf")))
+    w = model()
+    w.views[0]["byteOffset"] = 10 ** 15  # far past the buffer
+    with pytest.raises(RuntimeError, match="past its buffer"):
+        rt.build_scene(w.save(str(tmp_path / "view.gltf")))
+    w = model()
+    w.accessors[0]["count"] = 10 ** 12
+    with pytest.raises(RuntimeError, match="elements"):
+        rt.build_scene(w.save(str(tmp_path / "count.gltf")))
+
+
 @pytest.mark.gpu
 def test_model_renders_alike_in_both_integrators(rt, tmp_path):
     """A model (an octahedron of two materials, placed twice) renders on the GPU: finite, lit, the same bytes from the

@@ -72,9 +72,18 @@ def test_views_of_one_resident_scene(rt):
 
 
 def test_admission_by_memory_budget(rt):
-    """A request waits while the budget is full and starts once an earlier one is released"""
+    """A request waits while the budget is full and starts once an earlier one is released; the budget is set from
+    the server's own accounting so that one view fits beside the scene and two don't"""
     view = _view(rt, "weekend", 200, 112)
-    server = rt.Server(memory_budget_bytes=6 << 20, max_active=8)
+    probe = rt.Server(max_active=8)
+    sid = probe.add_scene(view)
+    scene_bytes = probe.memory().reserved_bytes
+    probe.submit_view(sid, view.desc.camera, 1, 4)
+    probe.step()
+    request_bytes = probe.memory().reserved_bytes - scene_bytes
+    probe.close()
+    assert request_bytes > 0
+    server = rt.Server(memory_budget_bytes=scene_bytes + request_bytes + request_bytes // 2, max_active=8)
     sid = server.add_scene(view)
     first = server.submit_view(sid, view.desc.camera, 1, 4)
     second = server.submit_view(sid, view.desc.camera, 2, 4)
@@ -85,6 +94,34 @@ def test_admission_by_memory_budget(rt):
     server.release(first)
     _run(server, [second])
     assert server.status(second).state == 2
+    server.close()
+
+
+def test_memory_held_within_budget(rt):
+    """What the active requests' sessions actually hold (image, staging, stacks, counters) never exceeds what the
+    server reserved for them, nor the budget, at any step of a mix that the budget admits only part of at once"""
+    views = [_view(rt, "final", 256, 256), _view(rt, "weekend", 512, 288), _view(rt, "triangles", 192, 108),
+             _view(rt, "glass", 128, 128)]
+    probe = rt.Server()
+    scene_ids = [probe.add_scene(v) for v in views]
+    scenes = probe.memory().reserved_bytes
+    probe.close()
+    # (a view of the triangle scene holds ~113 MB of far stack levels for its 36-level tree: one such view fits at
+    # a time beside a few others)
+    server = rt.Server(memory_budget_bytes=scenes + (160 << 20), max_active=8, passes_per_step=8)
+    scene_ids = [server.add_scene(v) for v in views]
+    ids = [server.submit_view(scene_ids[k % 4], views[k % 4].desc.camera, k, 8 + 4 * k) for k in range(10)]
+    peak_active = 0
+    while any(server.status(i).state < 2 for i in ids):
+        active = server.step()
+        m = server.memory()
+        assert m.held_bytes <= m.reserved_bytes <= m.budget_bytes, (m.held_bytes, m.reserved_bytes, m.budget_bytes)
+        peak_active = max(peak_active, active)
+        for i in ids:
+            if server.status(i).state == 2:
+                server.release(i)
+                ids.remove(i)
+    assert peak_active >= 1
     server.close()
 
 
