@@ -447,6 +447,25 @@ int rt_session_closest_hit(rt_session* session, const float* origins, const floa
                            const int32_t* from, const uint8_t* inward, int64_t count, int32_t* index, float* t,
                            uint8_t* entering);
 
+/* The reference's own traversal, for reference-exact closest hits (src/gpu/bvh.cuh:576-706): the session's binary tree
+ * (with RT_TREE_MEDIAN, the reference's tree) walked near first in the reference's order (the first child when both
+ * are entered at the same distance; the farther child pushed; a popped node visited only when entered before the
+ * closest hit), spheres tested on the open interval (0.001, closest) and quads on the closed one (src/gpu/sphere.cuh:116,
+ * src/gpu/quad.cuh:70). A tie then goes to the first sphere and the last quad the walk meets, as in the reference,
+ * where rt_session_closest_hit gives it to the primitive of lower scene index. Same arguments as
+ * rt_session_closest_hit; float32 sessions only. */
+int rt_session_reference_hits(rt_session* session, const float* origins, const float* directions, const float* times,
+                              const int32_t* from, const uint8_t* inward, int64_t count, int32_t* index, float* t,
+                              uint8_t* entering);
+
+/* The per-sample random numbers the kernels draw (src/gpu/random.cuh:25-139), on device 0: for each of `count` items,
+ * the generator of (seed, pixels[k], passes[k]) (SplitMix64 hashes into PCG32) and, each from a fresh generator of the
+ * item, `draws` raw 32-bit words, `draws` float draws (the top 24 bits of a word times 2^-24), `draws` double draws
+ * (53 bits of two words, the first one high), and nine floats: a uniform unit vector, a uniform point of the unit disk
+ * (z = 0) and a cosine-weighted direction about +z, as the paths draw them. Any output may be NULL; draws <= 4096. */
+int rt_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passes, int64_t count, int32_t draws,
+                      uint32_t* words, float* floats, double* doubles, float* directions);
+
 /* The flat BVH this library builds for a scene in float32 with the splits of `tree` (an rt_tree), on the host (no GPU
  * needed), in the reference's depth-first node layout (src/gpu/bvh.cuh:29-38,382-435): per node box_min[3], box_max[3]
  * as floats, and index, count as ints; and the global ids of the scene's spheres and quads. Call with NULL buffers to

@@ -4,14 +4,19 @@ The checker the tests, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline l
 reference's CPU renderer (cpu_model.hpp, cpu_reference.cpp), the GPU-semantics twin for bit-exact closest hits and
 random numbers (gpu_twin.cpp), and ImageCompare (image_compare.py). Nothing in the product imports this package.
 
-Parity status per function (the checks that pin each are in tests/test_oracle.py and tests/test_parity_cpu.py):
+Parity status per function (the checks that pin each are in tests/test_oracle.py and tests/test_abi.py):
   random numbers (mt19937_64, PCG32, SplitMix64)   pinned by published known answers
   scene building and flattening                    pinned against the product's independent builder, bit for bit, and
                                                    the reference's documented counts and values
   CPU renderer (ray_color)                         pinned by the furnace closed form (exactly 1), CPU-vs-CPU
                                                    consistency, a planted fault, and the reference's published mean
                                                    radiance of final (docs/gpu.md:66)
-  twin closest_hit                                 pinned by brute force over all primitives
+  twin closest_hit, scene-order ties               pinned by brute force over all primitives
+  twin closest_hit, the reference's walk-order ties pinned by brute force away from ties, by the tied set on tied
+                                                   rays, and by hand-built scenes whose tie outcome the reference's
+                                                   intervals and near-first order fix (tests/test_oracle.py)
+  twin sample words, floats and doubles            pinned by the PCG32 demo stream and SplitMix64 known answers, and
+                                                   the draws' bit layout (floor(f 2^24) = w >> 8, floor(d 2^32) = w)
   ImageCompare                                     pinned by synthetic images: equal expectations pass, a 0.1% bias and
                                                    a shifted image fail
 """
@@ -59,6 +64,16 @@ def _load() -> C.CDLL:
     lib.twin_closest_hit.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
                                      P(C.c_float), P(C.c_int32), P(C.c_uint8), C.c_int64, P(C.c_int32),
                                      P(C.c_float), P(C.c_uint8), P(C.c_uint64)]
+    lib.twin_closest_hit_walk_order.restype = C.c_int
+    lib.twin_closest_hit_walk_order.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
+                                                P(C.c_float), P(C.c_int32), P(C.c_uint8), C.c_int64, P(C.c_int32),
+                                                P(C.c_float), P(C.c_uint8)]
+    lib.twin_custom_closest_hit.restype = C.c_int
+    lib.twin_custom_closest_hit.argtypes = [P(C.c_double), C.c_int64, P(C.c_double), C.c_int64, C.c_int,
+                                            P(C.c_float), P(C.c_float), C.c_int64, P(C.c_int32), P(C.c_float),
+                                            P(C.c_uint8), P(C.c_int32), P(C.c_int32)]
+    lib.twin_sample_doubles.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, P(C.c_double)]
+    lib.twin_sample_words.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, P(C.c_uint32)]
     lib.twin_brute_force.restype = C.c_int
     lib.twin_brute_force.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
                                      P(C.c_float), P(C.c_int32), P(C.c_uint8), C.c_int64, P(C.c_int32),
@@ -188,6 +203,60 @@ def twin_closest_hit(name: str, origins, directions, times, from_ids, inward, wi
                           _ptr(t, C.c_float), _ptr(ent, C.c_uint8), _ptr(st, C.c_uint64) if stats else None):
         raise RuntimeError(L.twin_last_error().decode())
     return (ids, t, ent, st) if stats else (ids, t, ent)
+
+
+def twin_closest_hit_walk_order(name: str, origins, directions, times, from_ids, inward, width: int = 0,
+                                spp: int = 0, earth_width: int = 0):
+    """twin_closest_hit under the reference's tie rule: the binary near-first walk keeps the first sphere met at a
+    distance (open interval) and the last quad (closed interval), src/gpu/bvh.cuh:576-706, sphere.cuh:116, quad.cuh:70"""
+    L = lib()
+    o = np.ascontiguousarray(origins, np.float32)
+    d = np.ascontiguousarray(directions, np.float32)
+    tm = np.ascontiguousarray(times, np.float32)
+    f = np.ascontiguousarray(from_ids, np.int32)
+    inw = np.ascontiguousarray(inward, np.uint8)
+    n = len(tm)
+    ids = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    ent = np.zeros(n, np.uint8)
+    if L.twin_closest_hit_walk_order(name.encode(), width, spp, earth_width, _ptr(o, C.c_float), _ptr(d, C.c_float),
+                                     _ptr(tm, C.c_float), _ptr(f, C.c_int32), _ptr(inw, C.c_uint8), n,
+                                     _ptr(ids, C.c_int32), _ptr(t, C.c_float), _ptr(ent, C.c_uint8)):
+        raise RuntimeError(L.twin_last_error().decode())
+    return ids, t, ent
+
+
+def twin_custom_closest_hit(spheres, quads, origins, directions, walk_order: bool):
+    """Closest hits in a scene of plain spheres (x, y, z, r rows) and quads (q, u, v rows): (ids, t, entering,
+    sphere_ids, quad_ids), ids in the twin tree's numbering"""
+    L = lib()
+    sp = np.ascontiguousarray(np.asarray(spheres, np.float64).reshape(-1, 4))
+    qd = np.ascontiguousarray(np.asarray(quads, np.float64).reshape(-1, 9))
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+    n = len(o)
+    ids = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    ent = np.zeros(n, np.uint8)
+    sid = np.zeros(len(sp), np.int32)
+    qid = np.zeros(len(qd), np.int32)
+    if L.twin_custom_closest_hit(_ptr(sp, C.c_double), len(sp), _ptr(qd, C.c_double), len(qd), 1 if walk_order else 0,
+                                 _ptr(o, C.c_float), _ptr(d, C.c_float), n, _ptr(ids, C.c_int32), _ptr(t, C.c_float),
+                                 _ptr(ent, C.c_uint8), _ptr(sid, C.c_int32), _ptr(qid, C.c_int32)):
+        raise RuntimeError(L.twin_last_error().decode())
+    return ids, t, ent, sid, qid
+
+
+def sample_doubles(seed: int, pixel: int, pass_: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float64)
+    lib().twin_sample_doubles(seed, pixel, pass_, n, _ptr(out, C.c_double))
+    return out
+
+
+def sample_words(seed: int, pixel: int, pass_: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().twin_sample_words(seed, pixel, pass_, n, _ptr(out, C.c_uint32))
+    return out
 
 
 def twin_brute_force(name: str, origins, directions, times, from_ids, inward, width: int = 0, spp: int = 0,

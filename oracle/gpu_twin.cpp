@@ -7,9 +7,12 @@
 // same ones:
 //   - the BVH builder sorts with std::stable_sort (the reference's std::ranges::sort leaves the order of equal keys to
 //     the standard library);
-//   - ties: of two primitives hit at exactly the same distance the one of lower index in the scene is the closest hit
-//     (`better` below), where the reference keeps the one its walk happens to meet first (spheres) or last (quads), so
-//     the closest hit doesn't depend on the tree or the walk, and the brute-force search finds the same one;
+//   - ties, two rules (tie_rule below). The product's closest-hit walk: of two primitives hit at exactly the same
+//     distance the one of lower index in the scene is the closest hit (`better`), so the closest hit doesn't depend on
+//     the tree or the walk, and the brute-force search finds the same one. The reference's own (walk_order): its walk
+//     keeps the first sphere it meets at a distance (an open interval, src/gpu/sphere.cuh:116) and the last quad (a
+//     closed one, src/gpu/quad.cuh:70), in the order of its binary near-first walk (src/gpu/bvh.cuh:576-706), so a quad
+//     always wins a tie against a sphere;
 //   - multiply-adds: the reference's float code is compiled by nvcc, which fuses a·b + c into one fma wherever it
 //     chooses. Here every fusion is written out and nothing else is fused (built with -ffp-contract=off):
 //       dot(u, v)        = fma(u.x, v.x, fma(u.y, v.y, u.z·v.z))
@@ -461,14 +464,23 @@ inline bool better(const tree& tr, int id, float t, const hit& closest)
                              tr.ranks[static_cast<std::size_t>(id)] < tr.ranks[static_cast<std::size_t>(closest.index)]);
 }
 
+// Which primitive a tie goes to (see the header)
+enum class tie_rule
+{
+    scene_order, // the lower index in the scene; every test closed
+    walk_order,  // the reference's: spheres open (the first met stays), quads closed (the last met wins)
+};
+
 struct counters
 {
     std::uint64_t node_visits = 0; // inner nodes whose two children's boxes were tested
     std::uint64_t sphere_tests = 0, moving_tests = 0, quad_tests = 0;
 };
 
-inline hit closest_hit(const tree& tr, const ray& r, int from, bool inward, counters* count)
+inline hit closest_hit(const tree& tr, const ray& r, int from, bool inward, counters* count,
+                       tie_rule rule = tie_rule::scene_order)
 {
+    const bool by_walk = rule == tie_rule::walk_order;
     const float t_min = 0.001f;
     hit closest;
     if (tr.nodes.empty())
@@ -508,7 +520,7 @@ inline hit closest_hit(const tree& tr, const ray& r, int from, bool inward, coun
                     }
                     float t;
                     bool front;
-                    if (hit_quad(tr.quads[i], r, t_min, closest.t, t, front) && better(tr, id, t, closest))
+                    if (hit_quad(tr.quads[i], r, t_min, closest.t, t, front) && (by_walk || better(tr, id, t, closest)))
                     {
                         closest = {id, t, front};
                     }
@@ -529,8 +541,8 @@ inline hit closest_hit(const tree& tr, const ray& r, int from, bool inward, coun
                     const sphere s = moving ? tr.moving[i].at(r.time) : tr.spheres[i];
                     float t;
                     bool entering;
-                    if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering, true) &&
-                        better(tr, id, t, closest))
+                    if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering, !by_walk) &&
+                        (by_walk || better(tr, id, t, closest)))
                     {
                         closest = {id, t, entering};
                     }
@@ -559,7 +571,9 @@ inline hit closest_hit(const tree& tr, const ray& r, int from, bool inward, coun
         while (n < 0 && size > 0)
         {
             const entry e = stack[--size];
-            if (e.t <= closest.t) // a box entered at exactly the closest distance may hold a tie
+            // scene order: a box entered at exactly the closest distance may hold a tie; the reference's walk pops
+            // only boxes entered before it
+            if (by_walk ? e.t < closest.t : e.t <= closest.t)
             {
                 n = e.node;
             }
@@ -683,6 +697,107 @@ int twin_closest_hit(const char* name, int width, int spp, int earth_width, cons
     {
         last_error = e.what();
         return -1;
+    }
+}
+
+// twin_closest_hit under the reference's tie rule (tie_rule::walk_order)
+int twin_closest_hit_walk_order(const char* name, int width, int spp, int earth_width, const float* origins,
+                                const float* directions, const float* times, const std::int32_t* from,
+                                const std::uint8_t* inward, std::int64_t n, std::int32_t* index, float* t,
+                                std::uint8_t* entering)
+{
+    try
+    {
+        const scene_tree st = make(name, width, spp, earth_width);
+        for (std::int64_t i = 0; i < n; ++i)
+        {
+            const ray r{{origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]},
+                        {directions[3 * i], directions[3 * i + 1], directions[3 * i + 2]},
+                        times[i]};
+            const hit h = closest_hit(st.t, r, from[i], inward[i] != 0, nullptr, tie_rule::walk_order);
+            index[i] = h.index;
+            t[i] = h.t;
+            entering[i] = h.entering ? 1 : 0;
+        }
+        return 0;
+    }
+    catch (const std::exception& e)
+    {
+        last_error = e.what();
+        return -1;
+    }
+}
+
+// closest_hit in a scene given as plain primitives (spheres standing still: center x, y, z, radius; quads: q, u, v,
+// 9 doubles each), its tree built with the reference's leaf sizes; rule 0 scene order, 1 the reference's walk order.
+// Ids are the tree's global ids; sphere_ids and quad_ids (may be null) receive each input primitive's id.
+int twin_custom_closest_hit(const double* spheres, std::int64_t sphere_count, const double* quads,
+                            std::int64_t quad_count, int rule, const float* origins, const float* directions,
+                            std::int64_t n, std::int32_t* index, float* t, std::uint8_t* entering,
+                            std::int32_t* sphere_ids, std::int32_t* quad_ids)
+{
+    try
+    {
+        std::vector<oracle::flat_sphere> fs;
+        for (std::int64_t k = 0; k < sphere_count; ++k)
+        {
+            const double* v = spheres + 4 * k;
+            fs.push_back({{v[0], v[1], v[2]}, v[3], 0, {0, 0, 0}});
+        }
+        std::vector<oracle::flat_quad> fq;
+        for (std::int64_t k = 0; k < quad_count; ++k)
+        {
+            const double* v = quads + 9 * k;
+            fq.push_back({{v[0], v[1], v[2]}, {v[3], v[4], v[5]}, {v[6], v[7], v[8]}, 0});
+        }
+        const tree tr = builder(fs, fq, leaf_size(fs.size() + fq.size())).build();
+        for (std::int64_t i = 0; i < n; ++i)
+        {
+            const ray r{{origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]},
+                        {directions[3 * i], directions[3 * i + 1], directions[3 * i + 2]},
+                        0.0f};
+            const hit h = closest_hit(tr, r, -1, false, nullptr, rule == 1 ? tie_rule::walk_order : tie_rule::scene_order);
+            index[i] = h.index;
+            t[i] = h.t;
+            entering[i] = h.entering ? 1 : 0;
+        }
+        if (sphere_ids)
+        {
+            std::copy(tr.sphere_ids.begin(), tr.sphere_ids.end(), sphere_ids);
+        }
+        if (quad_ids)
+        {
+            std::copy(tr.quad_ids.begin(), tr.quad_ids.end(), quad_ids);
+        }
+        return 0;
+    }
+    catch (const std::exception& e)
+    {
+        last_error = e.what();
+        return -1;
+    }
+}
+
+// The first n random doubles of sample_generator(seed, pixel, pass): 53 bits from two outputs, the high one first
+// (src/gpu/random.cuh:76-90)
+void twin_sample_doubles(std::uint64_t seed, std::uint64_t pixel, std::uint64_t pass, std::int64_t n, double* out)
+{
+    pcg32 g = sample_generator(seed, pixel, pass);
+    for (std::int64_t i = 0; i < n; ++i)
+    {
+        const std::uint64_t high = g.next();
+        const std::uint64_t low = g.next();
+        out[i] = static_cast<double>((high << 21u) | (low >> 11u)) * 0x1p-53;
+    }
+}
+
+// The first n raw 32-bit outputs of sample_generator(seed, pixel, pass)
+void twin_sample_words(std::uint64_t seed, std::uint64_t pixel, std::uint64_t pass, std::int64_t n, std::uint32_t* out)
+{
+    pcg32 g = sample_generator(seed, pixel, pass);
+    for (std::int64_t i = 0; i < n; ++i)
+    {
+        out[i] = g.next();
     }
 }
 

@@ -206,6 +206,11 @@ _SIGNATURES = {
     "rt_render": (C.c_int, [_P(Scene), _P(RenderOptions), _P(C.c_double), C.c_size_t, _P(RenderStats)]),
     "rt_session_closest_hit": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_int32),
                                          _P(C.c_uint8), C.c_int64, _P(C.c_int32), _P(C.c_float), _P(C.c_uint8)]),
+    "rt_session_reference_hits": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float),
+                                            _P(C.c_int32), _P(C.c_uint8), C.c_int64, _P(C.c_int32), _P(C.c_float),
+                                            _P(C.c_uint8)]),
+    "rt_sample_numbers": (C.c_int, [C.c_uint64, _P(C.c_uint64), _P(C.c_uint64), C.c_int64, C.c_int32,
+                                    _P(C.c_uint32), _P(C.c_float), _P(C.c_double), _P(C.c_float)]),
     "rt_session_traffic": (C.c_int, [C.c_void_p, C.c_int64, _P(Traffic)]),
     "rt_scene_bvh": (C.c_int, [_P(Scene), C.c_int32, _P(C.c_float), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32),
                                _P(C.c_int32), _P(C.c_int32)]),
@@ -330,7 +335,7 @@ class Session:
     """gpu::session: the scene stays on the GPU and the image of one view gains passes on demand."""
 
     def __init__(self, scene, **options):
-        desc = scene.desc if isinstance(scene, BuiltScene) else scene
+        desc = getattr(scene, "desc", scene)  # a built scene, anything holding an rt_scene as .desc, or the struct
         self._keep = scene
         handle = C.c_void_p()
         _check(_lib.rt_session_create(C.byref(desc), C.byref(_session_options(**options)), C.byref(handle)))
@@ -410,6 +415,13 @@ class Session:
 
     def closest_hit(self, origins, directions, times, from_ids, inward):
         """The session's BVH traversal on given rays: (ids, t, entering)."""
+        return self._hits(_lib.rt_session_closest_hit, origins, directions, times, from_ids, inward)
+
+    def reference_hits(self, origins, directions, times, from_ids, inward):
+        """The reference's own walk of the session's binary tree, with its tie rule: (ids, t, entering)."""
+        return self._hits(_lib.rt_session_reference_hits, origins, directions, times, from_ids, inward)
+
+    def _hits(self, fn, origins, directions, times, from_ids, inward):
         o = np.ascontiguousarray(origins, dtype=np.float32)
         d = np.ascontiguousarray(directions, dtype=np.float32)
         tm = np.ascontiguousarray(times, dtype=np.float32)
@@ -419,11 +431,28 @@ class Session:
         ids = np.zeros(n, np.int32)
         t = np.zeros(n, np.float32)
         ent = np.zeros(n, np.uint8)
-        _check(_lib.rt_session_closest_hit(self._h, o.ctypes.data_as(_P(C.c_float)), d.ctypes.data_as(_P(C.c_float)),
-                                           tm.ctypes.data_as(_P(C.c_float)), f.ctypes.data_as(_P(C.c_int32)),
-                                           inw.ctypes.data_as(_P(C.c_uint8)), n, ids.ctypes.data_as(_P(C.c_int32)),
-                                           t.ctypes.data_as(_P(C.c_float)), ent.ctypes.data_as(_P(C.c_uint8))))
+        _check(fn(self._h, o.ctypes.data_as(_P(C.c_float)), d.ctypes.data_as(_P(C.c_float)),
+                  tm.ctypes.data_as(_P(C.c_float)), f.ctypes.data_as(_P(C.c_int32)), inw.ctypes.data_as(_P(C.c_uint8)),
+                  n, ids.ctypes.data_as(_P(C.c_int32)), t.ctypes.data_as(_P(C.c_float)),
+                  ent.ctypes.data_as(_P(C.c_uint8))))
         return ids, t, ent
+
+
+def sample_numbers(seed: int, pixels, passes, draws: int):
+    """The per-sample generator's outputs on the device for items (pixels[k], passes[k]): (words, floats, doubles,
+    directions), words/floats/doubles of shape (n, draws), directions (n, 3, 3): a unit vector, a disk point and a
+    cosine-weighted direction about +z"""
+    px = np.ascontiguousarray(pixels, dtype=np.uint64)
+    ps = np.ascontiguousarray(passes, dtype=np.uint64)
+    n = len(px)
+    words = np.zeros((n, draws), np.uint32)
+    floats = np.zeros((n, draws), np.float32)
+    doubles = np.zeros((n, draws), np.float64)
+    dirs = np.zeros((n, 3, 3), np.float32)
+    _check(_lib.rt_sample_numbers(seed, px.ctypes.data_as(_P(C.c_uint64)), ps.ctypes.data_as(_P(C.c_uint64)), n,
+                                  draws, words.ctypes.data_as(_P(C.c_uint32)), floats.ctypes.data_as(_P(C.c_float)),
+                                  doubles.ctypes.data_as(_P(C.c_double)), dirs.ctypes.data_as(_P(C.c_float))))
+    return words, floats, doubles, dirs
 
 
 def scene_bvh(scene: BuiltScene, tree: str = "median"):

@@ -289,6 +289,33 @@ int rt_session_closest_hit(rt_session* session, const float* origins, const floa
     });
 }
 
+int rt_session_reference_hits(rt_session* session, const float* origins, const float* directions, const float* times,
+                              const int32_t* from, const uint8_t* inward, int64_t count, int32_t* index, float* t,
+                              uint8_t* entering)
+{
+    return guarded([&] {
+        need(session, "session");
+        if (count > 0)
+        {
+            need(origins, "origins");
+            need(directions, "directions");
+            need(times, "times");
+            need(from, "from");
+            need(inward, "inward");
+            need(index, "index");
+            need(t, "t");
+            need(entering, "entering");
+        }
+        session->impl->reference_hits(origins, directions, times, from, inward, count, index, t, entering);
+    });
+}
+
+int rt_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passes, int64_t count, int32_t draws,
+                      uint32_t* words, float* floats, double* doubles, float* directions)
+{
+    return guarded([&] { rt::sample_numbers(seed, pixels, passes, count, draws, words, floats, doubles, directions); });
+}
+
 int rt_session_traffic(rt_session* session, int64_t passes, rt_traffic* traffic)
 {
     return guarded([&] {

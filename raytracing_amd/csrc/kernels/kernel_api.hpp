@@ -47,6 +47,19 @@ hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origi
                               const Real* times, const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index,
                               Real* t, uint8_t* entering, hipStream_t stream);
 
+// The reference's own closest-hit walk over the binary tree (probes.hip): binary_nodes holds node_count nodes of
+// binary_node_bytes() each in the reference's layout (bvh_build.hpp ref_node<float>); overflow: one device word, counts
+// rays whose walk outgrew the probe's stack
+hipError_t launch_reference_walk(const dev::scene_view<float>& sc, const void* binary_nodes, int node_count,
+                                 const float* origins, const float* directions, const float* times,
+                                 const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index, float* t,
+                                 uint8_t* entering, int32_t* overflow, hipStream_t stream);
+size_t binary_node_bytes();
+// The per-sample generator's outputs for n (pixel, pass) items (probes.hip); any output may be null
+hipError_t launch_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passes, int64_t n, int draws,
+                                 uint32_t* words, float* floats, double* doubles, float* directions,
+                                 hipStream_t stream);
+
 // A tone map other than the books' (op: rt_tone_map) of the image's sums, width pixels a row
 hipError_t launch_tone_map(const double* sums, int width, int64_t pixels, double inverse_passes, int op,
                            float exposure_scale, uint8_t* rgba, hipStream_t stream);

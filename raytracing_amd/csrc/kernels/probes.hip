@@ -1,0 +1,249 @@
+// Parity probes: kernels that expose one part of the path on its own, so that tests can check it against the oracle
+// bit for bit (the reference's GpuTests drive its device functions through test kernels the same way,
+// tests/test_scene.h).
+//
+//   - reference_walk_kernel: the closest hit as the reference's own traversal finds it. The product walks a four-wide
+//     tree and gives a tie to the primitive of lower scene index; this kernel walks the session's binary tree in the
+//     reference's order with the reference's intervals, so that on tied rays too its ids are the reference's.
+//   - sample_numbers_kernel: the per-sample generator's raw words, its float and double draws, and the direction
+//     samplers built on them, for given (seed, pixel, pass).
+#include <algorithm>
+#include <cstdint>
+
+#include "kernel_api.hpp"
+#include "path_tracer.hpp"
+
+namespace rt::dev
+{
+// The reference's 32-byte node (src/gpu/bvh.cuh:29-38), as bvh_build.hpp's ref_node<float> lays it out: an inner
+// node's second child is `index` and its first child follows it; a leaf holds count & 255 primitives of kind
+// count >> 8 from `index` on
+struct binary_node
+{
+    float box_min[3];
+    int32_t index;
+    float box_max[3];
+    int32_t count;
+};
+
+// The binary near-first walk of src/gpu/bvh.cuh:576-706, in its order: both children's boxes tested against the
+// closest hit so far; the nearer one next (the first child when they are entered at the same distance), the farther
+// one pushed if entered; a popped node is visited only if it is entered before the closest hit. Spheres are tested on
+// the open interval (0.001, closest) and quads on the closed one [0.001, closest] (src/gpu/sphere.cuh:116,
+// quad.cuh:70), and a primitive that passes replaces the closest hit: a tie goes to the first sphere and to the last
+// quad the walk meets, and to a quad over a sphere. Triangles, which the reference's walk doesn't hold, are tested as
+// quads are. The stack is private (the reference's 32 entries; 64 here, with a check).
+__global__ void __launch_bounds__(256) reference_walk_kernel(const scene_view<float> sc,
+                                                             const binary_node* __restrict__ nodes, int node_count,
+                                                             const float* __restrict__ origins,
+                                                             const float* __restrict__ directions,
+                                                             const float* __restrict__ times,
+                                                             const int32_t* __restrict__ from,
+                                                             const uint8_t* __restrict__ inward, int64_t n,
+                                                             int32_t* __restrict__ index, float* __restrict__ t_out,
+                                                             uint8_t* __restrict__ entering,
+                                                             int32_t* __restrict__ overflow)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+    {
+        return;
+    }
+    const ray<float> r{{origins[3 * k], origins[3 * k + 1], origins[3 * k + 2]},
+                       {directions[3 * k], directions[3 * k + 1], directions[3 * k + 2]},
+                       times[k]};
+    const int own_id = from[k];
+    const bool going_in = inward[k] != 0;
+    const float t_min = 0.001f;
+    hit<float> closest{-1, inf<float>, false};
+    if (node_count > 0)
+    {
+        const float inverse_a = 1.0f / length_squared(r.direction);
+        const vec3<float> inv{safe_inverse(r.direction.x), safe_inverse(r.direction.y), safe_inverse(r.direction.z)};
+        const vec3<float> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
+        constexpr int capacity = 64;
+        int pending[capacity];
+        float pending_t[capacity];
+        int size = 0;
+        int at = 0;
+        while (at >= 0)
+        {
+            const binary_node nd = nodes[at];
+            if (nd.count > 0)
+            {
+                const int kind = nd.count >> 8;
+                const int end = nd.index + (nd.count & 255);
+                for (int i = nd.index; i < end; ++i)
+                {
+                    float t;
+                    bool side;
+                    if (kind == prim_quad || kind == prim_triangle)
+                    {
+                        const int id = (kind == prim_quad ? sc.quad_base : sc.triangle_base) + i;
+                        if (id == own_id)
+                        {
+                            continue;
+                        }
+                        const bool met = kind == prim_quad ? hit_quad(sc.quads[i], r, t_min, closest.t, t, side)
+                                                           : hit_triangle(sc.triangles[i], r, t_min, closest.t, t, side);
+                        if (met)
+                        {
+                            closest = {id, t, side};
+                        }
+                    }
+                    else
+                    {
+                        const bool moving = kind == prim_moving_sphere;
+                        const int id = moving ? sc.moving_base + i : i;
+                        const bool own = id == own_id;
+                        if (own && !going_in)
+                        {
+                            continue;
+                        }
+                        const sphere<float> s =
+                            moving ? moving_at(sc.moving_spheres[i], r.time) : load_sphere(sc.spheres, i);
+                        if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, side))
+                        {
+                            closest = {id, t, side};
+                        }
+                    }
+                }
+                at = -1;
+            }
+            else
+            {
+                const int first = at + 1;
+                const int second = nd.index;
+                const binary_node a = nodes[first];
+                const binary_node b = nodes[second];
+                const float ta = box_entry(a.box_min, a.box_max, inv, os, t_min, closest.t);
+                const float tb = box_entry(b.box_min, b.box_max, inv, os, t_min, closest.t);
+                const bool second_first = tb < ta;
+                const float far_t = m_max(ta, tb);
+                if (far_t < inf<float>)
+                {
+                    if (size == capacity)
+                    {
+                        atomicAdd(overflow, 1);
+                        break;
+                    }
+                    pending[size] = second_first ? first : second;
+                    pending_t[size] = far_t;
+                    ++size;
+                }
+                at = m_min(ta, tb) < inf<float> ? (second_first ? second : first) : -1;
+            }
+            while (at < 0 && size > 0)
+            {
+                --size;
+                if (pending_t[size] < closest.t)
+                {
+                    at = pending[size];
+                }
+            }
+        }
+    }
+    index[k] = closest.index;
+    t_out[k] = closest.t;
+    entering[k] = closest.entering ? 1 : 0;
+}
+
+// Item k: the generator of (seed, pixels[k], passes[k]) and what the path draws from it. Each output comes from a
+// fresh generator of the item: `draws` raw words, `draws` float draws, `draws` double draws (two words each), and nine
+// floats: a uniform unit vector, a uniform point of the unit disk and a cosine-weighted direction about +z
+__global__ void __launch_bounds__(256) sample_numbers_kernel(uint64_t seed, const uint64_t* __restrict__ pixels,
+                                                             const uint64_t* __restrict__ passes, int64_t n, int draws,
+                                                             uint32_t* __restrict__ words, float* __restrict__ floats,
+                                                             double* __restrict__ doubles,
+                                                             float* __restrict__ directions)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+    {
+        return;
+    }
+    const uint64_t pixel = pixels[k];
+    const uint64_t pass = passes[k];
+    const int64_t base = k * draws;
+    if (words)
+    {
+        pcg32 g = sample_generator(seed, pixel, pass);
+        for (int i = 0; i < draws; ++i)
+        {
+            words[base + i] = g.next();
+        }
+    }
+    if (floats)
+    {
+        pcg32 g = sample_generator(seed, pixel, pass);
+        for (int i = 0; i < draws; ++i)
+        {
+            floats[base + i] = random_real<float>(g);
+        }
+    }
+    if (doubles)
+    {
+        pcg32 g = sample_generator(seed, pixel, pass);
+        for (int i = 0; i < draws; ++i)
+        {
+            doubles[base + i] = random_real<double>(g);
+        }
+    }
+    if (directions)
+    {
+        pcg32 g = sample_generator(seed, pixel, pass);
+        const vec3<float> u = random_unit_vector<float>(g);
+        g = sample_generator(seed, pixel, pass);
+        const vec3<float> d = random_in_unit_disk<float>(g);
+        g = sample_generator(seed, pixel, pass);
+        const vec3<float> c = random_cosine_direction<float>(g);
+        float* out = directions + 9 * k;
+        out[0] = u.x;
+        out[1] = u.y;
+        out[2] = u.z;
+        out[3] = d.x;
+        out[4] = d.y;
+        out[5] = d.z;
+        out[6] = c.x;
+        out[7] = c.y;
+        out[8] = c.z;
+    }
+}
+}
+
+namespace rt
+{
+hipError_t launch_reference_walk(const dev::scene_view<float>& sc, const void* binary_nodes, int node_count,
+                                 const float* origins, const float* directions, const float* times,
+                                 const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index, float* t,
+                                 uint8_t* entering, int32_t* overflow, hipStream_t stream)
+{
+    if (n <= 0)
+    {
+        return hipSuccess;
+    }
+    const int threads = 256;
+    const dim3 grid(static_cast<unsigned>((n + threads - 1) / threads));
+    hipLaunchKernelGGL(dev::reference_walk_kernel, grid, dim3(threads), 0, stream, sc,
+                       static_cast<const dev::binary_node*>(binary_nodes), node_count, origins, directions, times,
+                       from, inward, n, index, t, entering, overflow);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passes, int64_t n, int draws,
+                                 uint32_t* words, float* floats, double* doubles, float* directions,
+                                 hipStream_t stream)
+{
+    if (n <= 0)
+    {
+        return hipSuccess;
+    }
+    const int threads = 256;
+    const dim3 grid(static_cast<unsigned>((n + threads - 1) / threads));
+    hipLaunchKernelGGL(dev::sample_numbers_kernel, grid, dim3(threads), 0, stream, seed, pixels, passes, n, draws,
+                       words, floats, doubles, directions);
+    return hipGetLastError();
+}
+
+size_t binary_node_bytes() { return sizeof(dev::binary_node); }
+}

@@ -528,6 +528,56 @@ public:
         }
     }
 
+    void reference_hits(const float* origins, const float* directions, const float* times, const int32_t* from,
+                        const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) override
+    {
+        if constexpr (sizeof(Real) != sizeof(float))
+        {
+            throw std::invalid_argument("the reference-walk probe runs in float32 sessions only");
+        }
+        else
+        {
+            if (n <= 0)
+            {
+                return;
+            }
+            const auto& nodes = dscene_.tree().nodes;
+            static_assert(sizeof(ref_node<float>) == 32, "the probe reads the reference's 32-byte node");
+            if (binary_node_bytes() != sizeof(ref_node<float>) || nodes.size() > static_cast<std::size_t>(INT32_MAX))
+            {
+                throw std::runtime_error("the binary tree's layout differs from the probe's");
+            }
+            if (binary_nodes_.size() != nodes.size())
+            {
+                binary_nodes_ = device_array<ref_node<float>>(nodes);
+            }
+            const std::size_t m = static_cast<std::size_t>(n);
+            device_array<float> o(3 * m), d(3 * m), tm(m), tt(m);
+            device_array<int32_t> f(m), idx(m), overflow(1);
+            device_array<uint8_t> in(m), ent(m);
+            check(hipMemcpy(o.data(), origins, o.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(d.data(), directions, d.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(tm.data(), times, tm.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(f.data(), from, f.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(in.data(), inward, in.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemset(overflow.data(), 0, overflow.bytes()), "clearing the probe's counter");
+            check(launch_reference_walk(dscene_.view(), binary_nodes_.data(), static_cast<int>(nodes.size()), o.data(),
+                                        d.data(), tm.data(), f.data(), in.data(), n, idx.data(), tt.data(), ent.data(),
+                                        overflow.data(), stream()),
+                  "launching the reference-walk probe");
+            check(hipStreamSynchronize(stream()), "tracing the probe rays");
+            int32_t overflowed = 0;
+            check(hipMemcpy(&overflowed, overflow.data(), sizeof overflowed, hipMemcpyDeviceToHost), "reading hits");
+            if (overflowed != 0)
+            {
+                throw std::runtime_error("the reference walk's stack overflowed on some rays");
+            }
+            check(hipMemcpy(index, idx.data(), idx.bytes(), hipMemcpyDeviceToHost), "copying hits");
+            check(hipMemcpy(t, tt.data(), tt.bytes(), hipMemcpyDeviceToHost), "copying hits");
+            check(hipMemcpy(entering, ent.data(), ent.bytes(), hipMemcpyDeviceToHost), "copying hits");
+        }
+    }
+
     rt_traffic traffic(int64_t passes) override
     {
         settle();
@@ -785,6 +835,7 @@ private:
         }
     }
     device_array<uint8_t> rgba_;
+    device_array<ref_node<float>> binary_nodes_; // the reference-walk probe's copy of the binary tree
     bool has_view_ = false;
     accumulation state_{};
     double kernel_seconds_ = 0;
@@ -865,6 +916,47 @@ private:
         return seconds;
     }
 };
+}
+
+void sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passes, int64_t n, int draws,
+                    uint32_t* words, float* floats, double* doubles, float* directions)
+{
+    if (n < 0 || draws < 0 || draws > 4096 || (n > 0 && (!pixels || !passes)))
+    {
+        throw std::invalid_argument("0 or more items with their pixels and passes, and 0 to 4096 draws");
+    }
+    if (n == 0)
+    {
+        return;
+    }
+    const std::size_t m = static_cast<std::size_t>(n);
+    const std::size_t k = m * static_cast<std::size_t>(draws);
+    device_array<uint64_t> px(m), ps(m);
+    device_array<uint32_t> w(words ? k : 0);
+    device_array<float> fl(floats ? k : 0), dir(directions ? 9 * m : 0);
+    device_array<double> db(doubles ? k : 0);
+    check(hipMemcpy(px.data(), pixels, px.bytes(), hipMemcpyHostToDevice), "copying items");
+    check(hipMemcpy(ps.data(), passes, ps.bytes(), hipMemcpyHostToDevice), "copying items");
+    check(launch_sample_numbers(seed, px.data(), ps.data(), n, draws, w.data(), fl.data(), db.data(), dir.data(),
+                                nullptr),
+          "launching the random-number probe");
+    check(hipDeviceSynchronize(), "drawing the numbers");
+    if (words && k)
+    {
+        check(hipMemcpy(words, w.data(), w.bytes(), hipMemcpyDeviceToHost), "copying words");
+    }
+    if (floats && k)
+    {
+        check(hipMemcpy(floats, fl.data(), fl.bytes(), hipMemcpyDeviceToHost), "copying floats");
+    }
+    if (doubles && k)
+    {
+        check(hipMemcpy(doubles, db.data(), db.bytes(), hipMemcpyDeviceToHost), "copying doubles");
+    }
+    if (directions)
+    {
+        check(hipMemcpy(directions, dir.data(), dir.bytes(), hipMemcpyDeviceToHost), "copying directions");
+    }
 }
 
 namespace

@@ -36,6 +36,9 @@ public:
     virtual rt_kernel_info kernel() const = 0;
     virtual void closest_hit(const float* origins, const float* directions, const float* times, const int32_t* from,
                              const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) = 0;
+    // The reference's own traversal of the session's binary tree, with its tie rule (probes.hip)
+    virtual void reference_hits(const float* origins, const float* directions, const float* times, const int32_t* from,
+                                const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) = 0;
     virtual std::size_t device_bytes() const = 0;
     virtual rt_traffic traffic(int64_t passes) = 0;
 
@@ -65,6 +68,11 @@ const rt_scene& scene_of(const shared_scene& scene);
 // The device memory a megakernel session of this view of a resident scene holds beside the scene once it has
 // rendered, with its staging sized for staging_passes passes a launch: an upper bound of view_device_bytes()
 std::size_t view_device_bytes(const shared_scene& scene, const rt_camera_desc& camera, int staging_passes);
+
+// The per-sample generator's outputs on device 0 (the numbers the kernels draw): n items of (pixel, pass) for one
+// seed; any output may be null (probes.hip, sample_numbers_kernel)
+void sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passes, int64_t n, int draws,
+                    uint32_t* words, float* floats, double* doubles, float* directions);
 
 // Validates the scene (before any GPU work), then builds its BVH and uploads it
 std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_options& options);

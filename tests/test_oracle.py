@@ -49,6 +49,20 @@ def test_sample_floats_are_top_24_bits():
     assert np.array_equal(oracle.sample_floats(seed, pixel, pass_, 5), (raw >> 8).astype(np.float32) * 2.0 ** -24)
 
 
+def test_sample_words_and_doubles():
+    # the raw words are pcg32 of the mix64 hashes; a double is 53 bits of two words, the first one high
+    # (src/gpu/random.cuh:62-90): floor(d 2^32) is the first word, and d 2^53 is an integer below 2^53
+    seed, pixel, pass_ = 5 + (2 << 32), 777, 31
+    m = oracle.mix64
+    raw = oracle.pcg32(m(m(m(seed) ^ pixel) ^ pass_), m(m(m(seed ^ 0x9E3779B97F4A7C15) ^ pass_) ^ pixel), 8)
+    assert np.array_equal(oracle.sample_words(seed, pixel, pass_, 8), raw)
+    d = oracle.sample_doubles(seed, pixel, pass_, 4)
+    assert np.array_equal(np.floor(d * 2.0 ** 32).astype(np.uint64), raw[0::2].astype(np.uint64))
+    scaled = d * 2.0 ** 53
+    assert np.all(scaled == np.floor(scaled)) and np.all(scaled < 2.0 ** 53)
+    assert np.array_equal(scaled.astype(np.uint64) & np.uint64(2 ** 21 - 1), raw[1::2].astype(np.uint64) >> np.uint64(11))
+
+
 def test_noise_image_stream():
     img = oracle.noise_image(64, 9)
     raw = oracle.pcg32(0x853C49E6748FEA9B ^ 9, 0xDA3E39CB94B95BDB, 2)
