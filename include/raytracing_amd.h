@@ -390,6 +390,16 @@ void rt_session_destroy(rt_session* session);
 int rt_session_set_camera(rt_session* session, const rt_camera_desc* camera);
 /* gpu::session::render_to: render until the image holds `passes` complete passes (rows must be 0) */
 int rt_session_render_to(rt_session* session, int64_t passes, int32_t rows, rt_render_report* report);
+/* rt_session_render_to without device allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that must not call hipMalloc). The session's
+ * whole passes up to `passes` render on the megakernel (at the session's register budget, or the compiler's when the
+ * session renders on the wavefront or is still tuning), with the chunk queue's head and the staged samples in a static
+ * device array of 64 MiB that the library holds on each GPU it runs on until the process ends; the image's sums, the
+ * counters and the scene are the session's, allocated when it was created. Same image bytes as rt_session_render_to.
+ * Limit: one pass of the image must fit the array, W x H x 16 bytes (W x H x 32 in float64): about 4.19 M pixels in
+ * float32; a larger image is RT_ERROR_INVALID_ARGUMENT. Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (sessions on other threads) race, and must be serialized by the caller. */
+int rt_session_render_to_static(rt_session* session, int64_t passes, rt_render_report* report);
 /* gpu::session::render_for: about `seconds` of GPU time, never past pass_limit */
 int rt_session_render_for(rt_session* session, double seconds, int64_t pass_limit, rt_render_report* report);
 int rt_session_state(const rt_session* session, int64_t* passes, int32_t* rows);

@@ -195,6 +195,7 @@ _SIGNATURES = {
     "rt_session_destroy": (None, [C.c_void_p]),
     "rt_session_set_camera": (C.c_int, [C.c_void_p, _P(CameraDesc)]),
     "rt_session_render_to": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _P(RenderReport)]),
+    "rt_session_render_to_static": (C.c_int, [C.c_void_p, C.c_int64, _P(RenderReport)]),
     "rt_session_render_for": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, _P(RenderReport)]),
     "rt_session_state": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int32)]),
     "rt_session_pixels": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_size_t]),
@@ -363,6 +364,12 @@ class Session:
     def render_to(self, passes: int, rows: int = 0) -> RenderReport:
         r = RenderReport()
         _check(_lib.rt_session_render_to(self._h, passes, rows, C.byref(r)))
+        return r
+
+    def render_to_static(self, passes: int) -> RenderReport:
+        """render_to's whole passes without device allocation (the library's static scratch)"""
+        r = RenderReport()
+        _check(_lib.rt_session_render_to_static(self._h, passes, C.byref(r)))
         return r
 
     def render_for(self, seconds: float, pass_limit: int) -> RenderReport:

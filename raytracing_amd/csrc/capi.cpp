@@ -154,6 +154,18 @@ int rt_session_render_to(rt_session* session, int64_t passes, int32_t rows, rt_r
     });
 }
 
+int rt_session_render_to_static(rt_session* session, int64_t passes, rt_render_report* report)
+{
+    return guarded([&] {
+        need(session, "session");
+        const rt_render_report r = session->impl->render_to_static(passes);
+        if (report)
+        {
+            *report = r;
+        }
+    });
+}
+
 int rt_session_render_for(rt_session* session, double seconds, int64_t pass_limit, rt_render_report* report)
 {
     return guarded([&] {

@@ -40,6 +40,9 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
                              hipStream_t stream, unsigned long long* tallies = nullptr, int waves = 0,
                              void* far_buffer = nullptr, size_t far_bytes = 0, int grid_ways = 1);
 size_t megakernel_far_stack_bytes(int depth);
+// The static scratch of the no-allocation entry point on the current device (its address there, and its size in
+// *bytes), or null if the runtime can't give its address
+void* static_scratch(size_t* bytes);
 bool megakernel_waves_supported(int waves);
 
 template <typename Real>

@@ -10,6 +10,8 @@
 // wave.
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
+#include <vector>
 
 #include "chunk_queue.hpp"
 #include "kernel_api.hpp"
@@ -352,12 +354,50 @@ __global__ void __launch_bounds__(256) display_kernel(const double* sums, int64_
 }
 }
 
+namespace rt::dev
+{
+// The scratch memory of rt_session_render_to_static (the entry point for callers that can't allocate): the chunk
+// queue's head in its first 256 bytes, the staged samples after it. One array per GPU, held until the process ends;
+// every call on a GPU shares it, so calls that overlap on one GPU race.
+constexpr size_t static_scratch_bytes = size_t{64} << 20;
+static __device__ __align__(256) unsigned char scratch[static_scratch_bytes];
+}
+
 namespace rt
 {
 template <typename Real>
 size_t stack_lds_bytes(int entries, int threads)
 {
     return dev::stack_bytes(entries, threads);
+}
+
+// The static scratch's device address on the current device: looked up once per device with that device current
+// (the array's name is only a host-side stand-in), then kept
+void* static_scratch(size_t* bytes)
+{
+    static std::mutex guard;
+    static std::vector<void*> address;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess || device < 0)
+    {
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(guard);
+    if (address.size() <= static_cast<size_t>(device))
+    {
+        address.resize(static_cast<size_t>(device) + 1, nullptr);
+    }
+    if (address[static_cast<size_t>(device)] == nullptr)
+    {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(dev::scratch)) != hipSuccess || p == nullptr)
+        {
+            return nullptr;
+        }
+        address[static_cast<size_t>(device)] = p;
+    }
+    *bytes = dev::static_scratch_bytes;
+    return address[static_cast<size_t>(device)];
 }
 
 namespace

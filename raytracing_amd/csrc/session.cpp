@@ -9,6 +9,7 @@
 #include <limits>
 #include <numeric>
 #include <stdexcept>
+#include <string>
 
 #include "device_scene.hpp"
 #include "kernels/kernel_api.hpp"
@@ -309,6 +310,64 @@ public:
             {
                 state_.rows = work.first_row + work.row_count;
             }
+        }
+        report.passes = state_.passes;
+        report.rows = state_.rows;
+        return report;
+    }
+
+    rt_render_report render_to_static(int64_t target_passes) override
+    {
+        require_view();
+        settle();
+        if (target_passes < state_.passes || state_.rows != 0)
+        {
+            throw std::invalid_argument("the target is behind the image, or a pass is in progress");
+        }
+        std::size_t bytes = 0;
+        void* const scratch = static_scratch(&bytes);
+        if (scratch == nullptr)
+        {
+            throw std::runtime_error("the static scratch's device address is not available");
+        }
+        // the queue's head in the first 256 bytes, then the staged samples: one pass of the image must fit
+        constexpr std::size_t head_bytes = 256;
+        const std::size_t staged = (bytes - head_bytes) / sizeof(dev::staged_sample<Real>);
+        const std::size_t per_pass = megakernel_staging_per_pass(camera_.image_width, camera_.image_height);
+        if (per_pass > staged)
+        {
+            throw std::invalid_argument("an image of more than " + std::to_string(staged) +
+                                        " pixels doesn't fit the static scratch (64 MiB)");
+        }
+        auto* const head = static_cast<uint32_t*>(scratch);
+        void* const staging = static_cast<unsigned char*>(scratch) + head_bytes;
+        // the megakernel of the session's register budget; the wavefront and the autotuner's candidates need memory
+        // of their own, and every variant gives the same bytes
+        const int waves = current_.wavefront ? 0 : current_.waves;
+        rt_render_report report{};
+        const int64_t round_size = static_cast<int64_t>(camera_.sqrt_spp) * camera_.sqrt_spp;
+        while (state_.passes < target_passes)
+        {
+            dev::launch_work work{};
+            const int64_t pass = state_.passes;
+            work.first_pass = static_cast<int32_t>(pass % round_size);
+            // (launch_megakernel runs a launch of more passes than the scratch stages as several kernels)
+            work.pass_count = static_cast<int32_t>(std::min(target_passes - pass, round_size - work.first_pass));
+            work.first_row = 0;
+            work.row_count = camera_.image_height;
+            const uint64_t seed = options_.seed + static_cast<uint64_t>(pass / round_size) * round_seed_step;
+            start_.record(stream());
+            check(launch_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), staging, staged, head,
+                                          rays_.data(), 64, stream(), nullptr, waves),
+                  "launching the kernel");
+            stop_.record(stream());
+            stop_.synchronize();
+            const double seconds = hip_event::seconds_between(start_, stop_);
+            kernel_seconds_ += seconds;
+            report.launches += 1;
+            report.gpu_seconds += seconds;
+            report.longest_launch_seconds = std::max(report.longest_launch_seconds, seconds);
+            state_.passes += work.pass_count;
         }
         report.passes = state_.passes;
         report.rows = state_.rows;

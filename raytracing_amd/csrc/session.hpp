@@ -26,6 +26,9 @@ public:
     virtual void set_camera(const rt_camera_desc& camera) = 0;
     virtual rt_render_report render_to(int64_t passes, int32_t rows) = 0;
     virtual rt_render_report render_for(double seconds, int64_t pass_limit) = 0;
+    // render_to's whole passes on the megakernel with its staging and queue head in the library's static scratch:
+    // allocates no device memory (the caller under graph capture or in a loop that can't allocate)
+    virtual rt_render_report render_to_static(int64_t passes) = 0;
     virtual accumulation state() const = 0;
     virtual void pixels(double* out, std::size_t count) = 0;
     virtual void sums(double* out, std::size_t count) = 0;
