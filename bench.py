@@ -8,16 +8,21 @@ them; every launch is waited for). Also reported:
     session's own stream, over the timed region), against 8 TB/s of HBM; `traffic` is the HBM bytes per launch that
     rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured for the same workload, when profiles/hbm_traffic.txt
     holds a line for it (scaled by the counters' reading for a copy kernel of known size);
+    `hbm_frac` is that measured traffic over the live launch time against 8 TB/s, and `issue` the kernel's active
+    lanes per VALU instruction, VALU busy share and waves per SIMD from rocprofv3's SQ counters
+    (profiles/valu_issue.txt) for the same workload;
   - cpu_baseline: the CPU restatement of the reference renderer (double, mt19937_64 per row, the reference's
-    algorithms) timed on this box's host cores on a bounded sample: the same scene at 1080x1080 with 64 spp;
-  - megakernel: the plain FP32 megakernel's Mpaths/s on the same workload, beside the integrator measured. The two
-    integrators trace the same paths with the same random numbers (their images are the same bytes), so at equal
-    spp they are at equal RMSE, and the ratio of their Mpaths/s is the ratio at equal RMSE.
+    algorithms, a thread per processor of this job's share of the host, at most 16) timed on a bounded sample: the
+    same scene at 1080x1080 with 64 spp; `host` gives nproc, the affinity, the cgroup quota and the CPU model;
+  - the plain megakernel (the reference's mapping: a thread per pixel of a 16x8 block, all of a launch's passes in
+    its loop), the wavefront and the persistent queue megakernel on the same workload, and the measured integrator's
+    ratio to the plain one. All trace the same paths with the same random numbers (their images are the same bytes),
+    so at equal spp they are at equal RMSE, and the ratio of their Mpaths/s is the ratio at equal RMSE.
 
 Other workloads (--scene): weekend (400x225, 64 spp, depth 50), glass (600x600, 1024 spp, depth 50, -p mixture),
 glass-mis (the same with -p mis), triangles (1920x1080, 256 spp, depth 16, -p mis).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--integrator auto|megakernel|wavefront] [--scene NAME]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--integrator auto|megakernel|wavefront|plain] [--scene NAME]
                 [--spp N] [--no-cpu] [--no-megakernel]
 """
 import argparse
@@ -64,17 +69,47 @@ def measure(rt, scene, integrator, steps, warmup, seed=0, **tuning):
     return value, 1000 * elapsed / steps, gpu_seconds / max(launches, 1), launches / steps, s
 
 
+def host_cpus():
+    """The host's processors as this process sees them: nproc (online), the ones it may run on (affinity), the cgroup's
+    CPU quota (cpu.max, None when unlimited), the model name, and the threads the CPU baseline uses: one per
+    processor of this job's share (the smallest of those), at most 16, the CPU share of one GPU's job on the GPU host"""
+    online = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = online
+    quota = None
+    try:
+        parts = open("/sys/fs/cgroup/cpu.max").read().split()
+        if parts and parts[0] != "max":
+            quota = max(1, int(int(parts[0]) / int(parts[1])))
+    except (OSError, ValueError, IndexError):
+        pass
+    model = ""
+    try:
+        model = next((line.split(":", 1)[1].strip() for line in open("/proc/cpuinfo") if line.startswith("model name")),
+                     "")
+    except OSError:
+        pass
+    threads = max(1, min(16, affinity, quota or affinity))
+    return {"nproc": online, "affinity": affinity, "cgroup_quota": quota, "model": model, "threads": threads}
+
+
 def cpu_baseline(name: str, width: int, spp: int):
+    """The oracle's C++ restatement of the reference CPU renderer (src/camera.ixx:204-272: a thread per hardware thread,
+    rows handed out dynamically), on the job's share of the host's processors"""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    cpus = host_cpus()
+    threads = cpus["threads"]
     t0 = time.perf_counter()
     img, rays = oracle.render(name, width=width, spp=spp, seed=0, threads=threads, earth_width=2048)
     seconds = time.perf_counter() - t0
     paths = img.shape[0] * img.shape[1] * spp
     return {"value": round(paths / seconds / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
             "sample": f"{name} {img.shape[1]}x{img.shape[0]}, {spp} spp, double, {seconds:.1f} s, "
-                      f"{rays / paths:.2f} rays/path"}
+                      f"{rays / paths:.2f} rays/path",
+            "host": {k: cpus[k] for k in ("nproc", "affinity", "cgroup_quota", "model")}}
 
 
 def traffic_key(kernel: str) -> str:
@@ -84,6 +119,19 @@ def traffic_key(kernel: str) -> str:
         return kernel
     waves = next((w[1:] for w in kernel.split() if w.startswith("w") and w[1:].isdigit()), "0")
     return f"megakernel-w{waves}"
+
+
+def measured_issue(workload: str, key: str):
+    """Lanes per VALU instruction, VALU busy share and waves per SIMD of the dominant kernel, from the rocprofv3 SQ
+    counter summary (profiles/valu_issue.txt), or None"""
+    path = os.path.join(ROOT, "profiles", "valu_issue.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 5 and not line.startswith("#") and parts[0] == workload and parts[1] == key:
+            return {"lanes_per_valu": float(parts[2]), "valu_busy": float(parts[3]), "waves_per_simd": float(parts[4])}
+    return None
 
 
 def measured_traffic(workload: str, key: str):
@@ -226,15 +274,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--integrator", default="auto", choices=["auto", "megakernel", "wavefront"])
+    ap.add_argument("--integrator", default="auto", choices=["auto", "megakernel", "wavefront", "plain"])
     ap.add_argument("--scene", default="final", choices=sorted(WORKLOADS))
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--width", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-megakernel", action="store_true")
+    ap.add_argument("--no-megakernel", action="store_true",
+                    help="skip the comparison renders (the plain megakernel, the wavefront, the queue megakernel)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel over this many bytes (the HBM counters' calibration)")
-    ap.add_argument("--wavefront-graph", type=int, default=0, help="bounces per hipGraph (-1: none, e.g. under a profiler)")
+    ap.add_argument("--wavefront-graph", type=int, default=0, help="bounces per hipGraph (-1: none, every kernel launched on its own)")
     ap.add_argument("--wavefront-drain", type=int, default=0, help="active slots below which the pool drains")
     ap.add_argument("--wavefront-slots", type=int, default=0, help="the wavefront pool's size")
     ap.add_argument("--megakernel-waves", type=int, default=0, choices=[0, 5, 6, 7],
@@ -311,12 +360,19 @@ def main():
           f"{traffic.shadow_rays / traffic.paths:.2f} shadow rays/path, {events}", file=sys.stderr, flush=True)
     session.close()
 
-    mega = None
-    if args.integrator != "megakernel" and not args.no_megakernel:
-        mv, _, _, _, ms_ = measure(rt, scene, "megakernel", 1, 0, tree=args.tree)
-        ms_.close()
-        mega = round(mv, 2)
-        print(f"megakernel: {mv:.1f} Mpaths/s", file=sys.stderr, flush=True)
+    # the other integrators on the same workload: the plain megakernel (the reference's mapping, the baseline), the
+    # wavefront and the persistent queue megakernel. All trace the same paths and sum them in the same order (the same
+    # image bytes), so at equal spp they are at equal RMSE and the ratio of their Mpaths/s is the ratio at equal RMSE.
+    others = {}
+    if not args.no_megakernel:
+        for other in ("plain", "wavefront", "megakernel"):
+            if other == args.integrator:
+                others[other] = round(value, 2)
+                continue
+            ov, _, _, _, os_ = measure(rt, scene, other, 1, 1, tree=args.tree)
+            os_.close()
+            others[other] = round(ov, 2)
+            print(f"{other}: {ov:.1f} Mpaths/s", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu and name != "triangles" and sampling is None:
@@ -324,6 +380,8 @@ def main():
         print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     hbm = measured_traffic(args.scene, traffic_key(kernel))
+    issue = measured_issue(args.scene, traffic_key(kernel))
+    plain = others.get("plain")
     line = {
         "metric": "Mpaths/s (1080^2 final scene)" if args.scene == "final" else f"Mpaths/s ({args.scene})",
         "value": round(value, 2),
@@ -341,10 +399,18 @@ def main():
                    "paths_per_step": scene.width * scene.height * scene.passes,
                    "rays_per_path": round(traffic.rays / traffic.paths, 3),
                    "shadow_rays_per_path": round(traffic.shadow_rays / traffic.paths, 3),
-                   "megakernel_mpaths_s": mega},
+                   "plain_megakernel_mpaths_s": plain,
+                   "wavefront_mpaths_s": others.get("wavefront"),
+                   "megakernel_mpaths_s": others.get("megakernel"),
+                   "vs_plain_megakernel": round(value / plain, 3) if plain else None,
+                   "wavefront_vs_plain_megakernel": round(others["wavefront"] / plain, 3)
+                   if plain and others.get("wavefront") else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if hbm is None else round(hbm),
+                     # measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE) over the live launch time
+                     "hbm_frac": None if hbm is None else round(hbm / avg_launch / 1e9 / HBM_PEAK_GBS, 4),
+                     "issue": issue,
                      "bytes_per_path": round(bytes_per_path, 1), "avg_launch_ms": round(1000 * avg_launch, 3),
                      "paths_per_launch": round(paths_per_launch)},
         "cpu_baseline": cpu,

@@ -325,6 +325,9 @@ enum rt_integrator
     RT_INTEGRATOR_AUTO = 0,       /* the wavefront integrator where it is available */
     RT_INTEGRATOR_MEGAKERNEL = 1, /* one thread per path, the whole path in one kernel */
     RT_INTEGRATOR_WAVEFRONT = 2,  /* per-bounce stages over queues of paths */
+    RT_INTEGRATOR_PLAIN = 3,      /* the reference's mapping (src/gpu/renderer.cu:340-397,421-451): one thread per pixel
+                                     of a 16x8 block, all of a launch's passes in its loop; the baseline the others are
+                                     measured against. Same image bytes as the others. */
 };
 
 typedef struct rt_session_options

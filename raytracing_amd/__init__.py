@@ -241,7 +241,7 @@ for _name, (_res, _args) in _SIGNATURES.items():
 RT_OK, RT_ERROR_RUNTIME, RT_ERROR_INVALID_ARGUMENT, RT_ERROR_LOGIC = 0, 1, 2, 3
 SAMPLING = {"uniform": 0, "material": 1, "lights": 2, "mixture": 3, "nee": 4, "mis": 5}
 MATERIAL_KINDS = {"lambertian": 0, "metal": 1, "dielectric": 2, "diffuse_light": 3, "isotropic": 4}
-INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2}
+INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2, "plain": 3}
 TREES = {"default": 0, "median": 1, "sah": 2}
 TONE_MAPS = {"books": 0, "none": 1, "agx": 2, "neutral": 3}
 

@@ -45,6 +45,18 @@ size_t megakernel_far_stack_bytes(int depth);
 void* static_scratch(size_t* bytes);
 bool megakernel_waves_supported(int waves);
 
+// The plain megakernel (the reference's mapping, src/gpu/renderer.cu:340-397,421-451): one thread per pixel of a
+// plain_block_x x plain_block_y block, the launch's passes in the thread's loop, added straight to the sums. The
+// baseline the batched integrators are measured against; its sums equal theirs bit for bit.
+constexpr int plain_block_x = 16;
+constexpr int plain_block_y = 8;
+template <typename Real>
+hipError_t launch_plain_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
+                                   uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays,
+                                   hipStream_t stream);
+template <typename Real>
+kernel_attributes plain_megakernel_attributes(int mode, int depth);
+
 template <typename Real>
 hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origins, const Real* directions,
                               const Real* times, const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index,

@@ -305,6 +305,61 @@ quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc, cons
     }
 }
 
+// The plain megakernel, the reference's own mapping (src/gpu/renderer.cu:340-397,421-451): one thread per pixel of a
+// 16x8 block, all of the launch's passes in the thread's loop, one path after the other, each added to the pixel's
+// double sums as it ends. No queue and no staging: a wave waits for its longest pixel, and a launch for its costliest
+// blocks. It is the baseline the batched integrators are measured against; its sums are the others' bytes (the same
+// samples added in the same order).
+template <typename Real, int Mode>
+__global__ void __launch_bounds__(128) plain_megakernel(const camera_rec<Real> cam, const scene_view<Real> sc,
+                                                        const uint64_t seed, const launch_work work,
+                                                        double* __restrict__ sums,
+                                                        unsigned long long* __restrict__ ray_count)
+{
+    const lds_stack stack = block_stack(sc.stack_depth);
+    tally<false> tl;
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = work.first_row + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+    unsigned long long rays = 0;
+    uint32_t shadow_rays = 0;
+    if (i < cam.image_width && j < work.first_row + work.row_count)
+    {
+        const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + i;
+        double* const sum = sums + 3 * pixel;
+        double red = sum[0], green = sum[1], blue = sum[2];
+        for (int k = 0; k < work.pass_count; ++k)
+        {
+            const int pass = work.first_pass + k;
+            pcg32 g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
+            vec3<Real> sample;
+            if constexpr (Mode == mode_quality)
+            {
+                quality_path qp = start_quality(cam, sc.motion != 0, i, j, pass, g);
+                do
+                {
+                    ++rays;
+                } while (!trace_quality(qp, sc, cam, g, sample, stack, tl, shadow_rays));
+            }
+            else
+            {
+                path<Real> p = start_path(cam, Mode == mode_all_books && sc.motion != 0, i, j, pass, g);
+                do
+                {
+                    ++rays;
+                } while (!(Mode == mode_all_books ? trace_all_books(p, sc, cam, g, sample, stack, tl)
+                                                  : trace_first_book(p, sc, cam, g, sample, stack, tl)));
+            }
+            red += without_nan(sample.x);
+            green += without_nan(sample.y);
+            blue += without_nan(sample.z);
+        }
+        sum[0] = red;
+        sum[1] = green;
+        sum[2] = blue;
+    }
+    add_counts<false>(rays, shadow_rays, ray_count, nullptr, tl);
+}
+
 // The parity probe: closest_hit on a batch of given rays, with the traversal of the all-books kernel
 template <typename Real>
 __global__ void __launch_bounds__(256) closest_hit_kernel(const scene_view<Real> sc, const Real* origins,
@@ -585,6 +640,74 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
     }
     return hipSuccess;
 }
+
+namespace
+{
+template <typename Real>
+const void* plain_megakernel_for(int mode)
+{
+    if constexpr (sizeof(Real) == sizeof(float))
+    {
+        if (mode == dev::mode_quality)
+        {
+            return reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality>);
+        }
+    }
+    return mode == dev::mode_all_books ? reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_all_books>)
+                                       : reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_first_book>);
+}
+}
+
+template <typename Real>
+hipError_t launch_plain_megakernel(int mode, const dev::camera_rec<Real>& cam, const dev::scene_view<Real>& sc,
+                                   uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays,
+                                   hipStream_t stream)
+{
+    if ((mode == dev::mode_quality && sizeof(Real) != sizeof(float)) || work.pass_count < 1 || work.row_count < 1)
+    {
+        return hipErrorInvalidValue;
+    }
+    const dim3 block(plain_block_x, plain_block_y);
+    const dim3 grid(static_cast<unsigned>((cam.image_width + plain_block_x - 1) / plain_block_x),
+                    static_cast<unsigned>((work.row_count + plain_block_y - 1) / plain_block_y));
+    const size_t lds = stack_lds_bytes<Real>(sc.stack_depth, plain_block_x * plain_block_y);
+    dev::camera_rec<Real> c = cam;
+    dev::scene_view<Real> v = sc;
+    uint64_t s = seed;
+    dev::launch_work w = work;
+    void* args[] = {&c, &v, &s, &w, &sums, &rays};
+    return hipLaunchKernel(plain_megakernel_for<Real>(mode), grid, block, args, lds, stream);
+}
+
+template <typename Real>
+kernel_attributes plain_megakernel_attributes(int mode, int depth)
+{
+    hipFuncAttributes a{};
+    const void* f = plain_megakernel_for<Real>(mode);
+    kernel_attributes k{};
+    if (hipFuncGetAttributes(&a, f) == hipSuccess)
+    {
+        k.registers = a.numRegs;
+        k.local_bytes = static_cast<int>(a.localSizeBytes);
+    }
+    k.lds_bytes = static_cast<int>(stack_lds_bytes<Real>(depth, plain_block_x * plain_block_y));
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, plain_block_x * plain_block_y, k.lds_bytes) ==
+        hipSuccess)
+    {
+        k.blocks_per_cu = blocks;
+    }
+    return k;
+}
+
+template hipError_t launch_plain_megakernel<float>(int, const dev::camera_rec<float>&, const dev::scene_view<float>&,
+                                                   uint64_t, const dev::launch_work&, double*, unsigned long long*,
+                                                   hipStream_t);
+template hipError_t launch_plain_megakernel<double>(int, const dev::camera_rec<double>&,
+                                                    const dev::scene_view<double>&, uint64_t, const dev::launch_work&,
+                                                    double*, unsigned long long*, hipStream_t);
+template kernel_attributes plain_megakernel_attributes<float>(int, int);
+template kernel_attributes plain_megakernel_attributes<double>(int, int);
 
 template <typename Real>
 hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origins, const Real* directions,

@@ -372,8 +372,10 @@ __global__ void __launch_bounds__(256) wf_begin(wf_pool pool, int p)
 
 // The refilled slots start their items' paths and join the active list's last segment
 __global__ void __launch_bounds__(256) wf_refill(const camera_rec<float> cam, const scene_view<float> sc,
-                                                 const uint64_t seed, const launch_work work, wf_pool pool, int p)
+                                                 wf_pool pool, int p)
 {
+    const uint64_t seed = pool.launch->seed;
+    const launch_work work = pool.launch->work;
     __shared__ uint32_t prefix[wf_stripes + 1];
     load_prefix(prefix, pool.prefix + wf_prefix_free, wf_stripes + 1);
     const uint32_t refill = pool.counters[wf_counter_refill];
@@ -449,7 +451,7 @@ __device__ inline void after_shade(const wf_pool& pool, int p, uint32_t s, bool 
 }
 
 __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, const scene_view<float> sc,
-                                                const uint64_t seed, const launch_work work, wf_pool pool, int p)
+                                                wf_pool pool, int p)
 {
     __shared__ uint32_t prefix[wf_classes * wf_stripes + 1];
     load_prefix(prefix, pool.prefix + wf_prefix_class, wf_classes * wf_stripes + 1);
@@ -489,8 +491,7 @@ __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, con
 // The quality integrator's shade: the same queues in class order, each vertex's light sample and its shadow ray
 // traced here (an any-hit walk with this thread's LDS stack), in the megakernel's order of random numbers
 __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> cam, const scene_view<float> sc,
-                                                        const uint64_t seed, const launch_work work, wf_pool pool,
-                                                        int p)
+                                                        wf_pool pool, int p)
 {
     __shared__ uint32_t prefix[wf_classes * wf_stripes + 1];
     load_prefix(prefix, pool.prefix + wf_prefix_class, wf_classes * wf_stripes + 1);
@@ -641,24 +642,22 @@ hipError_t wf_launch_init(const dev::wf_pool& pool, uint32_t items, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
-                            const dev::launch_work& work, const dev::wf_pool& pool, int parity, bool quality,
-                            hipStream_t stream)
+hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc,
+                            const dev::wf_pool& pool, int parity, bool quality, hipStream_t stream)
 {
     const dim3 grid(wf_blocks(pool.capacity));
     const size_t lds = stack_lds_bytes<float>(sc.stack_depth, dev::wf_block);
     hipLaunchKernelGGL(dev::wf_begin, dim3(1), dim3(256), 0, stream, pool, parity);
-    hipLaunchKernelGGL(dev::wf_refill, grid, dim3(dev::wf_block), 0, stream, cam, sc, seed, work, pool, parity);
+    hipLaunchKernelGGL(dev::wf_refill, grid, dim3(dev::wf_block), 0, stream, cam, sc, pool, parity);
     hipLaunchKernelGGL(dev::wf_extend, grid, dim3(dev::wf_block), lds, stream, sc, pool, parity);
     hipLaunchKernelGGL(dev::wf_sort, dim3(1), dim3(64), 0, stream, pool, parity);
     if (quality)
     {
-        hipLaunchKernelGGL(dev::wf_shade_quality, grid, dim3(dev::wf_block), lds, stream, cam, sc, seed, work, pool,
-                           parity);
+        hipLaunchKernelGGL(dev::wf_shade_quality, grid, dim3(dev::wf_block), lds, stream, cam, sc, pool, parity);
     }
     else
     {
-        hipLaunchKernelGGL(dev::wf_shade, grid, dim3(dev::wf_block), 0, stream, cam, sc, seed, work, pool, parity);
+        hipLaunchKernelGGL(dev::wf_shade, grid, dim3(dev::wf_block), 0, stream, cam, sc, pool, parity);
     }
     return hipGetLastError();
 }

@@ -54,11 +54,20 @@ constexpr int wf_prefix_class = wf_stripes + 2;
 constexpr int wf_prefix_free = wf_prefix_class + wf_classes * wf_stripes + 1;
 constexpr int wf_prefix_size = wf_prefix_free + wf_stripes + 1;
 
+// What changes from one launch of the pool to the next, in device memory: the bounce kernels read it there, so that
+// the captured bounce graph stays valid for every launch of the pool and is captured once
+struct wf_launch
+{
+    uint64_t seed;
+    launch_work work;
+};
+
 // The slots' state, structure of arrays, 16-byte records. A slot renders work items: item k of a launch is pass
 // first_pass + k / pixels of the launch's pixel k % pixels (pixels in row order from its first row); its sample goes
 // to samples[k], and the launch ends by adding each pixel's samples to its sums in pass order.
 struct wf_pool
 {
+    const wf_launch* launch; // the current launch's seed and work
     float4* ray_o;   // origin, time
     float4* ray_d;   // direction, the primitive the ray leaves (int bits)
     float4* thr;     // throughput, rays left | inward << 30 | specular << 31 (int bits)
@@ -86,10 +95,9 @@ inline uint32_t wf_blocks(uint32_t capacity) { return (capacity + dev::wf_block 
 // A launch's start: `items` work items (passes x pixels of work), every slot free, no slot active
 hipError_t wf_launch_init(const dev::wf_pool& pool, uint32_t items, hipStream_t stream);
 // One bounce of parity `parity`: plan (refills, the active list's prefix sums), refill, extend, the queues' prefix
-// sums, shade
-hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,
-                            const dev::launch_work& work, const dev::wf_pool& pool, int parity, bool quality,
-                            hipStream_t stream);
+// sums, shade. The launch's seed and work are read from pool.launch.
+hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc,
+                            const dev::wf_pool& pool, int parity, bool quality, hipStream_t stream);
 // The drain: the slots of the parity-0 active list finish their paths, a thread each (mode: the scene's kernel_mode;
 // count: at least the list's length). Only when every work item has been handed out.
 hipError_t wf_launch_drain(int mode, const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,

@@ -173,7 +173,7 @@ public:
             throw std::invalid_argument("the wavefront integrator renders in float32 only");
         }
         current_ = {options.integrator == RT_INTEGRATOR_WAVEFRONT, options.megakernel_waves,
-                    options.block_size > 0 ? options.block_size : 64};
+                    options.block_size > 0 ? options.block_size : 64, options.integrator == RT_INTEGRATOR_PLAIN};
         if (options.integrator == RT_INTEGRATOR_AUTO && wavefront_supported<Real>())
         {
             // the autotuner's candidates, in the order they render the first full passes
@@ -269,7 +269,7 @@ public:
                 const int64_t left_in_round = round_size - work.first_pass;
                 int64_t count = seconds_per_pass_ > 0 ? static_cast<int64_t>(launch_seconds_ / seconds_per_pass_) : 1;
                 // and no more passes than the megakernel's staging holds (one kernel a launch)
-                const int64_t staged = current_.wavefront ? count : max_staged_passes();
+                const int64_t staged = current_.wavefront || current_.plain ? count : max_staged_passes();
                 count = std::max<int64_t>(1, std::min({count, staged, left_in_round, target_passes - pass}));
                 work.pass_count = tuning() ? static_cast<int32_t>(std::min<int64_t>(schedule_[tuned_].passes, std::min(
                                                  left_in_round, target_passes - pass)))
@@ -529,6 +529,15 @@ public:
             info.threads_per_block = dev::wf_block;
             std::snprintf(info.name, sizeof info.name, "wavefront");
         }
+        else if (current_.plain)
+        {
+            a = plain_megakernel_attributes<Real>(mode_, depth);
+            const char* name =
+                mode_ == dev::mode_quality ? "quality" : (mode_ == dev::mode_all_books ? "all books" : "first book");
+            info.threads_per_block = plain_block_x * plain_block_y;
+            std::snprintf(info.name, sizeof info.name, "plain %s%s %dx%d", name, sizeof(Real) == 8 ? " fp64" : "",
+                          plain_block_x, plain_block_y);
+        }
         else
         {
             a = megakernel_attributes<Real>(mode_, current_.block, depth, current_.waves);
@@ -723,7 +732,7 @@ public:
         {
             throw std::invalid_argument("a step adds whole passes to an image without a pass in progress");
         }
-        if (current_.wavefront || tuning())
+        if (current_.wavefront || current_.plain || tuning())
         {
             throw std::logic_error("steps render on the megakernel of a session that isn't tuning");
         }
@@ -797,6 +806,7 @@ private:
         bool wavefront = false;
         int waves = 0;
         int block = 128;
+        bool plain = false; // the reference's mapping: a thread per pixel, its passes in a loop
     };
     variant current_{};
     struct probe
@@ -959,6 +969,12 @@ private:
                 last_bounces_ += wavefront_->render(seed, part, sums_.data(), rays_.data());
             }
         }
+        else if (v.plain)
+        {
+            check(launch_plain_megakernel<Real>(mode_, cam_, dscene_.view(), seed, work, sums_.data(), rays_.data(),
+                                                stream()),
+                  "launching the kernel");
+        }
         else
         {
             ensure_staging();
@@ -1026,7 +1042,7 @@ void check_options(const rt_session_options& options)
     {
         throw std::invalid_argument("unknown precision");
     }
-    if (options.integrator < RT_INTEGRATOR_AUTO || options.integrator > RT_INTEGRATOR_WAVEFRONT)
+    if (options.integrator < RT_INTEGRATOR_AUTO || options.integrator > RT_INTEGRATOR_PLAIN)
     {
         throw std::invalid_argument("unknown integrator");
     }

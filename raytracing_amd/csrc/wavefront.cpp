@@ -5,27 +5,6 @@
 
 namespace rt
 {
-namespace
-{
-class graph_owner
-{
-public:
-    ~graph_owner()
-    {
-        if (exec)
-        {
-            (void)hipGraphExecDestroy(exec);
-        }
-        if (graph)
-        {
-            (void)hipGraphDestroy(graph);
-        }
-    }
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
-}
-
 template <typename Real>
 wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_rec<Real>& cam, int mode,
                            const rt_session_options& options, hipStream_t stream)
@@ -91,27 +70,55 @@ wavefront<Real>::wavefront(const dev::scene_view<Real>& sc, const dev::camera_re
         counters_ = device_array<uint32_t>(dev::wf_counters);
         prefix_ = device_array<uint32_t>(dev::wf_prefix_size);
         tallies_ = device_array<unsigned long long>(2 * dev::wf_stripes);
+        launch_ = device_array<dev::wf_launch>(1);
         check(hipMemsetAsync(counters_.data(), 0, counters_.bytes(), stream_), "clearing counters");
         check(hipMemsetAsync(tallies_.data(), 0, tallies_.bytes(), stream_), "clearing counters");
         check(hipHostMalloc(reinterpret_cast<void**>(&host_counters_), dev::wf_counters * sizeof(uint32_t),
                             hipHostMallocDefault),
               "allocating pinned memory");
-        pool_ = {ray_o_.data(), ray_d_.data(), thr_.data(), qst_.size() > 0 ? qst_.data() : nullptr, rng_.data(),
+        check(hipHostMalloc(reinterpret_cast<void**>(&host_launch_), sizeof(dev::wf_launch), hipHostMallocDefault),
+              "allocating pinned memory");
+        pool_ = {launch_.data(), ray_o_.data(), ray_d_.data(), thr_.data(), qst_.size() > 0 ? qst_.data() : nullptr, rng_.data(),
                  hit_.data(), item_.data(), samples_.data(), {active0_.data(), active1_.data()}, free_.data(),
                  queue_.data(), counters_.data(), prefix_.data(), tallies_.data(), capacity_, segment};
         bytes_ = ray_o_.bytes() * 3 + qst_.bytes() + rng_.bytes() + hit_.bytes() + item_.bytes() + samples_.bytes() +
                  active0_.bytes() * 2 + free_.bytes() + queue_.bytes() + counters_.bytes() + prefix_.bytes() +
-                 tallies_.bytes();
+                 tallies_.bytes() + launch_.bytes();
     }
 }
 
 template <typename Real>
 wavefront<Real>::~wavefront()
 {
+    (void)hipStreamSynchronize(stream_); // a launch that threw may still run the graph
+    if (graph_exec_)
+    {
+        (void)hipGraphExecDestroy(graph_exec_);
+    }
+    if (graph_)
+    {
+        (void)hipGraphDestroy(graph_);
+    }
     if (host_counters_)
     {
         (void)hipHostFree(host_counters_);
     }
+    if (host_launch_)
+    {
+        (void)hipHostFree(host_launch_);
+    }
+}
+
+// One replay's bounces: an even number (parity 0, 1, 0, 1...), so that each replay ends on parity 0
+template <typename Real>
+hipError_t wavefront<Real>::launch_bounces()
+{
+    hipError_t launched = hipSuccess;
+    for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
+    {
+        launched = wf_launch_bounce(cam_, sc_, pool_, b & 1, quality_, stream_);
+    }
+    return launched;
 }
 
 template <typename Real>
@@ -147,26 +154,23 @@ int64_t wavefront<Real>::render_part(uint64_t seed, const dev::launch_work& work
 {
     const uint32_t items =
         static_cast<uint32_t>(work.pass_count) * static_cast<uint32_t>(work.row_count) * static_cast<uint32_t>(cam_.image_width);
+    // the launch's seed and work go to device memory before its first kernel; the previous launch has ended (every
+    // render_part waits for its work), so the pinned record is free to rewrite
+    host_launch_->seed = seed;
+    host_launch_->work = work;
+    check(hipMemcpyAsync(launch_.data(), host_launch_, sizeof(dev::wf_launch), hipMemcpyHostToDevice, stream_),
+          "starting the wavefront's launch");
     check(wf_launch_init(pool_, items, stream_), "starting the wavefront's launch");
-    // the bounce loop: an even number of bounces (parity 0, 1, 0, 1...) at a time, captured once in a graph (or
-    // launched one by one), repeated until the items are all handed out and no slot is active
-    const auto bounces = [&] {
-        hipError_t launched = hipSuccess;
-        for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
-        {
-            launched = wf_launch_bounce(cam_, sc_, seed, work, pool_, b & 1, quality_, stream_);
-        }
-        return launched;
-    };
-    graph_owner g;
-    if (use_graph_)
+    // the bounce loop, captured at the pool's first launch (or launched kernel by kernel), replayed until the items
+    // are all handed out and no slot is active
+    if (use_graph_ && !graph_exec_)
     {
         check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capturing the bounce loop");
-        const hipError_t launched = bounces();
-        const hipError_t ended = hipStreamEndCapture(stream_, &g.graph);
+        const hipError_t launched = launch_bounces();
+        const hipError_t ended = hipStreamEndCapture(stream_, &graph_);
         check(launched, "launching the bounce loop's kernels");
         check(ended, "capturing the bounce loop");
-        check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "instantiating the bounce loop");
+        check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiating the bounce loop");
     }
     // every bounce either advances a path or hands out items to every free slot: a safe bound on the rounds
     const long long max_bounces =
@@ -180,11 +184,11 @@ int64_t wavefront<Real>::render_part(uint64_t seed, const dev::launch_work& work
         }
         if (use_graph_)
         {
-            check(hipGraphLaunch(g.exec, stream_), "launching the bounce loop");
+            check(hipGraphLaunch(graph_exec_, stream_), "launching the bounce loop");
         }
         else
         {
-            check(bounces(), "launching the bounce loop's kernels");
+            check(launch_bounces(), "launching the bounce loop's kernels");
         }
         done += bounces_per_graph_;
         check(hipMemcpyAsync(host_counters_, counters_.data(), dev::wf_counters * sizeof(uint32_t),

@@ -1,5 +1,6 @@
-// The wavefront integrator's host driver: the pool of path slots in device memory, and the bounce loop, captured in a
-// hipGraph of several bounces that is replayed until no slot is active (kernels in kernels/wavefront.hip)
+// The wavefront integrator's host driver: the pool of path slots in device memory, and the bounce loop, captured once
+// per pool in a hipGraph of several bounces that is replayed until no slot is active (kernels in
+// kernels/wavefront.hip)
 #pragma once
 
 #include <cstdint>
@@ -57,12 +58,19 @@ private:
     device_array<float4> samples_;
     device_array<uint32_t> active0_, active1_, free_, queue_, counters_, prefix_;
     device_array<unsigned long long> tallies_;
-    uint32_t* host_counters_ = nullptr; // pinned: the counters after each round
-    int max_passes_ = 1;                // passes one launch of the pool renders
+    device_array<dev::wf_launch> launch_;
+    uint32_t* host_counters_ = nullptr;    // pinned: the counters after each round
+    dev::wf_launch* host_launch_ = nullptr; // pinned: the next launch's seed and work, copied to launch_
+    int max_passes_ = 1;                   // passes one launch of the pool renders
     int64_t render_part(uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays);
+    hipError_t launch_bounces();
     dev::wf_pool pool_{};
     int bounces_per_graph_ = 8;
     bool use_graph_ = true;
+    // the bounce loop's graph: captured at the pool's first launch and replayed by every launch after it (the kernels
+    // read the launch's seed and work from launch_), destroyed with the pool
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
     std::size_t bytes_ = 0;
 };
 }

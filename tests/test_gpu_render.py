@@ -269,3 +269,29 @@ def test_megakernel_fp64_launch_splits(rt):
             b.render_to(p)
         sb = b.sums()
     assert np.array_equal(sa.view(np.uint64), sb.view(np.uint64))
+
+
+@pytest.mark.parametrize("name,sampling,precision", [("weekend", None, "float32"), ("final", None, "float32"),
+                                                     ("cornell", None, "float32"), ("final", None, "float64"),
+                                                     ("cornell-ggx", "mis", "float32"),
+                                                     ("triangles", "mis", "float32")])
+def test_plain_equals_megakernel(rt, name, sampling, precision):
+    """The plain megakernel (the reference's mapping: a thread per pixel of a 16x8 block, a launch's passes in its
+    loop, src/gpu/renderer.cu:340-397) and the persistent queue megakernel: the same sums bit for bit and the same ray
+    and shadow-ray counts, on an image side that isn't a multiple of the block, over launches of several passes and a
+    band of rows"""
+    scene = rt.build_scene(name, width=37, samples=9, strata=3, sampling=sampling, earth_width=512)
+    with rt.Session(scene, seed=5, integrator="megakernel", precision=precision) as a:
+        a.render_to(9)
+        sa = a.sums()
+        ca = a.counters()
+    with rt.Session(scene, seed=5, integrator="plain", precision=precision) as b:
+        assert b.kernel().name.decode().startswith("plain"), b.kernel().name
+        b.render_to(2)
+        b.render_to(4, 7)  # a band of the fifth pass's rows
+        b.render_to(9)
+        sb = b.sums()
+        cb = b.counters()
+    assert ca[0] == cb[0] and ca[1] == cb[1], f"rays {ca[:2]} vs {cb[:2]}"
+    mism = np.count_nonzero(sa.view(np.uint64) != sb.view(np.uint64))
+    assert mism == 0, f"{mism} of {sa.size} sums differ; max |diff| {np.abs(sa - sb).max()}"
