@@ -15,7 +15,6 @@
 
 #include "chunk_queue.hpp"
 #include "kernel_api.hpp"
-#include "path_tracer.hpp"
 #include "quality.hpp"
 
 namespace rt::dev

@@ -1,7 +1,7 @@
-// The path tracer's device functions, shared by every kernel (the megakernel, the wavefront stages, the parity probes):
-// random numbers, the BVH walk, the intersectors, the camera, the materials, the textures, the media and the lights
-// list. Each function follows the reference GPU renderer's semantics (cited per function); the kernels differ only in
-// how they schedule this work, so they compute the same numbers.
+// The path tracer's device functions that closest-hit parity rests on, shared by every kernel (the megakernel, the
+// wavefront stages, the parity probes): vectors and rays, the per-sample random numbers, the intersectors and the BVH
+// walk. Each follows the reference GPU renderer's semantics (cited per function); the kernels differ only in how they
+// schedule this work, so they compute the same numbers. The shading is in books.hpp and quality.hpp.
 //
 // Arithmetic: float (or double for the --fp64 path) as written, compiled with -ffp-contract=off. Multiply-adds are
 // fused only where the source writes fma(), in a fixed pattern the CPU twin (oracle/gpu_twin.cpp) restates:
@@ -101,21 +101,6 @@ __device__ inline vec3<Real> load3(const Real* p)
 {
     return {p[0], p[1], p[2]};
 }
-// Mirror reflection and refraction (src/gpu/vec3.cuh:170-186)
-template <typename Real>
-__device__ inline vec3<Real> reflect(vec3<Real> v, vec3<Real> n)
-{
-    return v - (Real(2) * dot(v, n)) * n;
-}
-template <typename Real>
-__device__ inline vec3<Real> refract(vec3<Real> uv, vec3<Real> n, Real etai_over_etat)
-{
-    const Real cos_theta = m_min(dot(-uv, n), Real(1));
-    const vec3<Real> perp = etai_over_etat * (uv + cos_theta * n);
-    const vec3<Real> parallel = (-m_sqrt(m_fabs(Real(1) - length_squared(perp)))) * n;
-    return perp + parallel;
-}
-
 template <typename Real>
 struct ray
 {
@@ -190,26 +175,6 @@ __device__ inline vec3<Real> random_unit_vector(pcg32& g)
     const Real r = Real(2) * m_sqrt(u * (Real(1) - u));
     return {r * c, r * s, Real(1) - Real(2) * u};
 }
-template <typename Real>
-__device__ inline vec3<Real> random_in_unit_disk(pcg32& g)
-{
-    const Real r = m_sqrt(random_real<Real>(g));
-    const Real theta_over_pi = Real(2) * random_real<Real>(g);
-    Real s, c;
-    m_sincospi(theta_over_pi, &s, &c);
-    return {r * c, r * s, Real(0)};
-}
-template <typename Real>
-__device__ inline vec3<Real> random_cosine_direction(pcg32& g)
-{
-    const Real r1 = random_real<Real>(g);
-    const Real r2 = random_real<Real>(g);
-    Real s, c;
-    m_sincospi(Real(2) * r1, &s, &c);
-    const Real sin_theta = m_sqrt(r2);
-    return {c * sin_theta, s * sin_theta, m_sqrt(Real(1) - r2)};
-}
-
 // --- the BVH walk (src/gpu/bvh.cuh:483-706) ------------------------------------------------------------------------------
 template <typename Real>
 __device__ inline Real safe_inverse(Real x)
@@ -733,718 +698,5 @@ __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Re
             return closest;
         }
     }
-}
-
-// --- sphere surface helpers (src/gpu/sphere.cuh:125-168) ---------------------------------------------------------------
-template <typename Real>
-__device__ inline void sphere_uv(vec3<Real> outward, const sphere_frame<Real>& f, Real& u, Real& v)
-{
-    const vec3<Real> n{f.cos_theta * outward.x - f.sin_theta * outward.z, outward.y,
-                       f.sin_theta * outward.x + f.cos_theta * outward.z};
-    const Real phi = m_atan2(-n.z, n.x) + pi_v<Real>;
-    const Real theta = m_acos(m_min(m_max(-n.y, Real(-1)), Real(1)));
-    u = phi / (Real(2) * pi_v<Real>);
-    v = theta / pi_v<Real>;
-}
-template <typename Real>
-__device__ inline vec3<Real> onto_sphere(vec3<Real> p, vec3<Real> outward, const sphere<Real>& s,
-                                         const sphere_frame<Real>& f)
-{
-    const Real p_p = dot(p, p);
-    const Real p_c = dot(p, s.center);
-    if (!(p_p + Real(2) * m_fabs(p_c) + m_fabs(f.k) < f.limit))
-    {
-        return p;
-    }
-    const Real fv = p_p - Real(2) * p_c + f.k;
-    return p - (fv / (Real(2) * s.radius)) * outward;
-}
-
-// --- the camera (src/gpu/camera.cuh:33-103) ----------------------------------------------------------------------------
-template <typename Real>
-struct path
-{
-    ray<Real> r;
-    vec3<Real> throughput;
-    int rays_left;
-    int from;
-    bool inward;
-};
-
-template <typename Real>
-__device__ inline ray<Real> camera_ray(const camera_rec<Real>& cam, int i, int j, int s_i, int s_j, pcg32& g)
-{
-    const Real dx = (Real(s_i) + random_real<Real>(g)) * cam.recip_sqrt_spp - Real(0.5);
-    const Real dy = (Real(s_j) + random_real<Real>(g)) * cam.recip_sqrt_spp - Real(0.5);
-    const vec3<Real> to_sample = load3(cam.center_to_pixel00) + (Real(i) + dx) * load3(cam.pixel_delta_u) +
-                                 (Real(j) + dy) * load3(cam.pixel_delta_v);
-    const vec3<Real> center = load3(cam.center);
-    if (!cam.defocus)
-    {
-        return {center, to_sample, Real(0)};
-    }
-    const vec3<Real> p = random_in_unit_disk<Real>(g);
-    const vec3<Real> lens = p.x * load3(cam.defocus_disk_u) + p.y * load3(cam.defocus_disk_v);
-    return {center + lens, to_sample - lens, Real(0)};
-}
-
-// The path of pass `pass` of pixel (i, j): cell (pass % k, pass / k) of its k×k grid, and a random time when something
-// in the scene moves (src/gpu/camera.cuh:53-60,444-457)
-template <typename Real>
-__device__ inline path<Real> start_path(const camera_rec<Real>& cam, bool motion, int i, int j, int pass, pcg32& g)
-{
-    path<Real> p;
-    p.r = camera_ray(cam, i, j, pass % cam.sqrt_spp, pass / cam.sqrt_spp, g);
-    if (motion)
-    {
-        p.r.time = random_real<Real>(g);
-    }
-    p.throughput = {Real(1), Real(1), Real(1)};
-    p.rays_left = cam.max_depth;
-    p.from = -1;
-    p.inward = false;
-    return p;
-}
-
-template <typename Real>
-__device__ inline vec3<Real> sky(const camera_rec<Real>& cam, vec3<Real> direction)
-{
-    const Real a = Real(0.5) * (unit_vector(direction).y + Real(1));
-    return (Real(1) - a) * load3(cam.sky_bottom) + a * load3(cam.sky_top);
-}
-
-// --- materials (src/gpu/material.cuh:55-126) ----------------------------------------------------------------------------
-// The orthonormal basis of Duff et al. (JCGT 2017) around a unit normal
-template <typename Real>
-__device__ inline vec3<Real> around(vec3<Real> n, vec3<Real> v)
-{
-    const Real sign = m_copysign(Real(1), n.z);
-    const Real a = Real(-1) / (sign + n.z);
-    const Real b = n.x * n.y * a;
-    const vec3<Real> x_axis{Real(1) + sign * n.x * n.x * a, sign * b, -sign * n.x};
-    const vec3<Real> y_axis{b, sign + n.y * n.y * a, -n.y};
-    return v.x * x_axis + v.y * y_axis + v.z * n;
-}
-template <typename Real>
-__device__ inline Real reflectance(Real cosine, Real ri)
-{
-    Real r0 = (Real(1) - ri) / (Real(1) + ri);
-    r0 = r0 * r0;
-    const Real x = Real(1) - cosine;
-    const Real x2 = x * x;
-    return r0 + (Real(1) - r0) * (x2 * x2 * x);
-}
-
-template <typename Real>
-struct hit_record
-{
-    vec3<Real> p;
-    vec3<Real> normal; // facing against the ray
-    bool front_face;
-};
-
-// scatter for lambertian (the first book's kernel), metal and dielectric: false when the light is absorbed
-template <typename Real>
-__device__ inline bool scatter(const material_rec<Real>& m, vec3<Real> direction, const hit_record<Real>& rec, pcg32& g,
-                               vec3<Real>& scattered, vec3<Real>& throughput)
-{
-    const vec3<Real> albedo = load3(m.albedo);
-    if (m.kind == RT_MATERIAL_LAMBERTIAN)
-    {
-        scattered = around(rec.normal, random_cosine_direction<Real>(g));
-        if (!(dot(rec.normal, scattered) > Real(0)))
-        {
-            return false;
-        }
-        throughput = throughput * albedo;
-        return true;
-    }
-    if (m.kind == RT_MATERIAL_METAL)
-    {
-        scattered = unit_vector(reflect(direction, rec.normal)) + m.fuzz * random_unit_vector<Real>(g);
-        if (dot(scattered, rec.normal) <= Real(0))
-        {
-            return false;
-        }
-        throughput = throughput * albedo;
-        return true;
-    }
-    // dielectric
-    const Real ri = rec.front_face ? Real(1) / m.refraction_index : m.refraction_index;
-    const vec3<Real> ud = unit_vector(direction);
-    const Real cos_theta = m_min(dot(-ud, rec.normal), Real(1));
-    const Real sin_theta = m_sqrt(Real(1) - cos_theta * cos_theta);
-    const bool cannot_refract = ri * sin_theta > Real(1);
-    scattered = cannot_refract || reflectance(cos_theta, ri) > random_real<Real>(g) ? reflect(ud, rec.normal)
-                                                                                      : refract(ud, rec.normal, ri);
-    return true;
-}
-
-// --- textures (src/gpu/texture.cuh:69-111, src/gpu/perlin.cuh:38-106) --------------------------------------------------
-template <typename Real, typename Tally>
-__device__ inline Real perlin_noise(const noise_rec<Real>& t, vec3<Real> p, Tally& tl)
-{
-    tl.add(tally_perlin, 8);
-    const Real fx = m_floor(p.x), fy = m_floor(p.y), fz = m_floor(p.z);
-    const int i = static_cast<int>(fx), j = static_cast<int>(fy), k = static_cast<int>(fz);
-    const Real u = p.x - fx, v = p.y - fy, w = p.z - fz;
-    const Real uu = u * u * (Real(3) - Real(2) * u);
-    const Real vv = v * v * (Real(3) - Real(2) * v);
-    const Real ww = w * w * (Real(3) - Real(2) * w);
-    const int x0 = t.perm_x[i & 255], x1 = t.perm_x[(i + 1) & 255];
-    const int y0 = t.perm_y[j & 255], y1 = t.perm_y[(j + 1) & 255];
-    const int z0 = t.perm_z[k & 255], z1 = t.perm_z[(k + 1) & 255];
-    Real accum = 0;
-#pragma unroll
-    for (int di = 0; di < 2; ++di)
-    {
-        const Real wx = di == 0 ? Real(1) - uu : uu;
-        const int px = di == 0 ? x0 : x1;
-#pragma unroll
-        for (int dj = 0; dj < 2; ++dj)
-        {
-            const Real wy = dj == 0 ? Real(1) - vv : vv;
-            const int py = dj == 0 ? y0 : y1;
-#pragma unroll
-            for (int dk = 0; dk < 2; ++dk)
-            {
-                const Real wz = dk == 0 ? Real(1) - ww : ww;
-                const int pz = dk == 0 ? z0 : z1;
-                const Real* g = t.gradients[px ^ py ^ pz];
-                const vec3<Real> offset{u - Real(di), v - Real(dj), w - Real(dk)};
-                accum += wx * wy * wz * dot(vec3<Real>{g[0], g[1], g[2]}, offset);
-            }
-        }
-    }
-    return accum;
-}
-template <typename Real, typename Tally>
-__device__ inline Real turbulence(const noise_rec<Real>& t, vec3<Real> p, int depth, Tally& tl)
-{
-    Real accum = 0;
-    Real weight = 1;
-    for (int layer = 0; layer < depth; ++layer)
-    {
-        accum += weight * perlin_noise(t, p, tl);
-        weight *= Real(0.5);
-        p = Real(2) * p;
-    }
-    return m_fabs(accum);
-}
-template <typename Real, typename Tally>
-__device__ inline vec3<Real> texture_value(const scene_view<Real>& sc, int index, Real u, Real v, vec3<Real> p,
-                                           Tally& tl)
-{
-    texture_rec<Real> t = sc.textures[index];
-    tl.add(tally_texture);
-    while (t.kind == RT_TEXTURE_CHECKER)
-    {
-        tl.add(tally_texture);
-        const int x = static_cast<int>(m_floor(t.inv_scale * p.x));
-        const int y = static_cast<int>(m_floor(t.inv_scale * p.y));
-        const int z = static_cast<int>(m_floor(t.inv_scale * p.z));
-        t = sc.textures[(x + y + z) % 2 == 0 ? t.even : t.odd];
-    }
-    if (t.kind == RT_TEXTURE_IMAGE)
-    {
-        const image_info im = sc.images[t.table];
-        const Real cu = m_min(m_max(u, Real(0)), Real(1));
-        const Real fv = Real(1) - m_min(m_max(v, Real(0)), Real(1));
-        const int column = static_cast<int>(cu * static_cast<Real>(im.width));
-        const int row = static_cast<int>(fv * static_cast<Real>(im.height));
-        const int ci = column < im.width - 1 ? column : im.width - 1;
-        const int rj = row < im.height - 1 ? row : im.height - 1;
-        const texel px = sc.texels[im.offset + rj * im.width + ci];
-        tl.add(tally_texel);
-        return {static_cast<Real>(px.r), static_cast<Real>(px.g), static_cast<Real>(px.b)};
-    }
-    if (t.kind == RT_TEXTURE_NOISE)
-    {
-        const noise_rec<Real>& marble = sc.noises[t.table];
-        const Real gray = Real(0.5) * (Real(1) + m_sin(marble.scale * p.z + Real(10) * turbulence(marble, p, 7, tl)));
-        return {gray, gray, gray};
-    }
-    return load3(t.rgb);
-}
-
-// --- media (src/gpu/constant_medium.cuh:48-125) ---------------------------------------------------------------------------
-template <typename Real, typename Tally>
-__device__ inline bool boundary_crossings(const medium_rec<Real>& m, const quad_rec<Real>* quads, const ray<Real>& r,
-                                          Real inverse_a, Real& t_in, Real& t_out, Tally& tl)
-{
-    if (m.quad_count == 0)
-    {
-        const sphere<Real> s{{m_fma(r.time, m.displacement[0], m.center[0]), m_fma(r.time, m.displacement[1], m.center[1]),
-                              m_fma(r.time, m.displacement[2], m.center[2])},
-                             m.radius};
-        return sphere_roots(s, r, inverse_a, t_in, t_out) && t_out - t_in > Real(0.0001);
-    }
-    t_in = inf<Real>;
-    t_out = -inf<Real>;
-    for (int i = m.first_quad; i < m.first_quad + m.quad_count; ++i)
-    {
-        Real t;
-        bool front;
-        tl.add(tally_boundary_quad);
-        if (hit_quad(quads[i], r, -inf<Real>, inf<Real>, t, front))
-        {
-            t_in = m_min(t_in, t);
-            t_out = m_max(t_out, t);
-        }
-    }
-    return t_out - t_in >= Real(0.0001);
-}
-template <typename Real, typename Tally>
-__device__ inline bool medium_collision(const medium_rec<Real>& m, const quad_rec<Real>* quads, const ray<Real>& r,
-                                        Real inverse_a, vec3<Real> inv, vec3<Real> os, Real t_min, Real t_max, pcg32& g,
-                                        Real& t, Tally& tl)
-{
-    if (box_entry(m.box_min, m.box_max, inv, os, t_min, t_max) == inf<Real>)
-    {
-        return false;
-    }
-    Real line_in, line_out;
-    if (!boundary_crossings(m, quads, r, inverse_a, line_in, line_out, tl))
-    {
-        return false;
-    }
-    Real t_in = m_max(line_in, t_min);
-    const Real t_out = m_min(line_out, t_max);
-    if (t_in >= t_out)
-    {
-        return false;
-    }
-    t_in = m_max(t_in, Real(0));
-    const Real length = m_sqrt(length_squared(r.direction));
-    const Real inside = (t_out - t_in) * length;
-    const Real hit_distance = m.neg_inv_density * m_log(random_real<Real>(g));
-    if (hit_distance > inside)
-    {
-        return false;
-    }
-    t = t_in + hit_distance / length;
-    return true;
-}
-
-// The nearest surface, or a particle of smoke in front of it: the media are tested after the surfaces, in their list's
-// order, each against the closest event so far (src/gpu/camera.cuh:259-284)
-template <typename Real, typename Tally>
-__device__ inline hit<Real> media_events(const scene_view<Real>& sc, const ray<Real>& r, hit<Real> closest, pcg32& g,
-                                         Tally& tl)
-{
-    if (sc.medium_count == 0)
-    {
-        return closest;
-    }
-    const Real t_min = Real(0.001);
-    const Real inverse_a = Real(1) / length_squared(r.direction);
-    const vec3<Real> inv{safe_inverse(r.direction.x), safe_inverse(r.direction.y), safe_inverse(r.direction.z)};
-    const vec3<Real> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
-    for (int k = 0; k < sc.medium_count; ++k)
-    {
-        Real t;
-        tl.add(tally_medium);
-        if (medium_collision(sc.media[k], sc.boundary_quads, r, inverse_a, inv, os, t_min, closest.t, g, t, tl))
-        {
-            closest = {sc.media_base + k, t, true};
-        }
-    }
-    return closest;
-}
-template <typename Real, typename Tally, typename Stack>
-__device__ inline hit<Real> closest_event(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
-                                          pcg32& g, const Stack& stack, Tally& tl)
-{
-    return media_events(sc, r, closest_hit<Real, true>(sc, r, from, inward, stack, tl), g, tl);
-}
-
-// --- the surface at a hit (src/gpu/camera.cuh:155-253) -------------------------------------------------------------------
-template <typename Real>
-struct surface
-{
-    hit_record<Real> rec;
-    vec3<Real> outward;
-    vec3<Real> texture_point;
-    Real u, v;
-    int material;
-    int texture_index;
-    int from;
-};
-
-template <typename Real, typename Tally>
-__device__ inline surface<Real> surface_at(const scene_view<Real>& sc, const ray<Real>& r, const hit<Real>& h,
-                                           Tally& tl)
-{
-    tl.add(tally_hit);
-    surface<Real> s;
-    s.u = Real(0);
-    s.v = Real(0);
-    s.outward = {Real(0), Real(0), Real(0)};
-    const bool on_quad = h.index >= sc.quad_base && h.index < sc.triangle_base;
-    s.rec.p = on_quad ? r.at_unfused(h.t) : r.at(h.t);
-    s.texture_point = s.rec.p;
-    s.from = h.index;
-    if (h.index >= sc.media_base)
-    {
-        tl.add(tally_medium);
-        s.rec.normal = {Real(1), Real(0), Real(0)};
-        s.rec.front_face = true;
-        s.material = sc.media[h.index - sc.media_base].material;
-        s.texture_index = sc.materials[s.material].texture;
-        s.from = -1;
-        return s;
-    }
-    s.rec.front_face = h.entering;
-    s.material = sc.prim_materials[h.index];
-    const uint32_t flags = sc.materials[s.material].flags;
-    s.texture_index = sc.materials[s.material].texture;
-    if (h.index >= sc.triangle_base)
-    {
-        const triangle_rec<Real> tri = sc.triangles[h.index - sc.triangle_base];
-        tl.add(tally_triangle);
-        s.outward = load3(tri.normal);
-        s.rec.normal = s.rec.front_face ? s.outward : -s.outward;
-        if (flags & needs_uv)
-        {
-            Real t, bary[2];
-            bool front;
-            hit_triangle(tri, r, -inf<Real>, inf<Real>, t, front, bary);
-            s.u = bary[0];
-            s.v = bary[1];
-        }
-        return s;
-    }
-    if (h.index >= sc.quad_base)
-    {
-        const quad_rec<Real> face = sc.quads[h.index - sc.quad_base];
-        tl.add(tally_quad);
-        s.outward = load3(face.normal);
-        s.rec.normal = s.rec.front_face ? s.outward : -s.outward;
-        if (flags & needs_uv)
-        {
-            quad_uv(face, r, h.t, s.u, s.v);
-        }
-        return s;
-    }
-    const sphere<Real> ball = h.index < sc.moving_base ? load_sphere(sc.spheres, h.index)
-                                                       : moving_at(sc.moving_spheres[h.index - sc.moving_base], r.time);
-    tl.add(h.index < sc.moving_base ? tally_sphere : tally_moving);
-    s.outward = (Real(1) / ball.radius) * (s.rec.p - ball.center);
-    s.rec.normal = s.rec.front_face ? s.outward : -s.outward;
-    if (flags & (needs_uv | needs_exact_point))
-    {
-        const sphere_frame<Real> frame = sc.sphere_frames[h.index];
-        tl.add(tally_frame);
-        if (flags & needs_uv)
-        {
-            sphere_uv(s.outward, frame, s.u, s.v);
-        }
-        if (flags & needs_exact_point)
-        {
-            s.texture_point = onto_sphere(s.rec.p, s.outward, ball, frame);
-        }
-    }
-    return s;
-}
-
-// --- the lights list (src/gpu/lights.cuh:52-226, src/gpu/pdf.cuh:13-26) ---------------------------------------------------
-template <typename Real>
-constexpr Real uniform_sphere_density = static_cast<Real>(1.0 / (4.0 * 3.14159265358979323846));
-template <typename Real>
-constexpr Real inverse_pi = static_cast<Real>(1.0 / 3.14159265358979323846);
-
-template <typename Real>
-__device__ inline Real cosine_density(vec3<Real> n, vec3<Real> d)
-{
-    const Real cosine = dot(n, d) * m_rsqrt(length_squared(d));
-    return cosine > Real(0) ? cosine * inverse_pi<Real> : Real(0);
-}
-template <typename Real>
-__device__ inline Real quad_light_pdf(const light_rec<Real>& l, vec3<Real> origin, vec3<Real> direction)
-{
-    Real t;
-    bool front;
-    if (!hit_quad(l.quad, ray<Real>{origin, direction, Real(0)}, Real(0.001), inf<Real>, t, front))
-    {
-        return Real(0);
-    }
-    const Real len2 = length_squared(direction);
-    const Real distance_squared = t * t * len2;
-    const Real cosine = m_fabs(dot(direction, load3(l.quad.normal))) * m_rsqrt(len2);
-    return distance_squared / (cosine * l.area);
-}
-template <typename Real>
-__device__ inline vec3<Real> quad_light_random(const light_rec<Real>& l, vec3<Real> origin, pcg32& g)
-{
-    const Real a = random_real<Real>(g);
-    const Real b = random_real<Real>(g);
-    return unit_vector((load3(l.quad.q) - origin) + a * load3(l.u) + b * load3(l.v));
-}
-template <typename Real>
-__device__ inline Real one_minus_cos_theta_max(Real radius, Real distance_squared)
-{
-    const Real s2 = radius * radius / distance_squared;
-    return s2 / (Real(1) + m_sqrt(Real(1) - s2));
-}
-template <typename Real>
-__device__ inline Real sphere_light_distance_squared(vec3<Real> c)
-{
-    return m_fma(c.z, c.z, m_fma(c.x, c.x, c.y * c.y));
-}
-template <typename Real>
-__device__ inline Real sphere_light_pdf(const light_rec<Real>& l, vec3<Real> origin, vec3<Real> direction, bool own)
-{
-    const vec3<Real> to_center = load3(l.center) - origin;
-    const Real d2 = sphere_light_distance_squared(to_center);
-    if (d2 < l.radius * l.radius)
-    {
-        return uniform_sphere_density<Real>;
-    }
-    if (own && !(dot(direction, to_center) > Real(0)))
-    {
-        return Real(0);
-    }
-    Real t;
-    bool entering;
-    if (!hit_sphere(sphere<Real>{load3(l.center), l.radius}, ray<Real>{origin, direction, Real(0)},
-                    Real(1) / length_squared(direction), Real(0.001), inf<Real>, own, t, entering))
-    {
-        return Real(0);
-    }
-    return Real(1) / (Real(2) * pi_v<Real> * one_minus_cos_theta_max(l.radius, d2));
-}
-template <typename Real>
-__device__ inline vec3<Real> sphere_light_random(const light_rec<Real>& l, vec3<Real> origin, pcg32& g)
-{
-    const vec3<Real> to_center = load3(l.center) - origin;
-    const Real d2 = sphere_light_distance_squared(to_center);
-    if (d2 < l.radius * l.radius)
-    {
-        return random_unit_vector<Real>(g);
-    }
-    const Real omc = one_minus_cos_theta_max(l.radius, d2);
-    const Real r1 = random_real<Real>(g);
-    const Real r2 = random_real<Real>(g);
-    Real s, c;
-    m_sincospi(Real(2) * r1, &s, &c);
-    const Real omz = r2 * omc;
-    const Real sin_theta = m_sqrt(omz * (Real(2) - omz));
-    return around(m_rsqrt(d2) * to_center, vec3<Real>{c * sin_theta, s * sin_theta, Real(1) - omz});
-}
-template <typename Real, typename Tally>
-__device__ inline Real lights_pdf_value(const scene_view<Real>& sc, vec3<Real> origin, vec3<Real> direction, int from,
-                                        Tally& tl)
-{
-    tl.add(tally_light, static_cast<uint32_t>(sc.light_count));
-    Real sum = 0;
-    for (int k = 0; k < sc.light_count; ++k)
-    {
-        const light_rec<Real>& l = sc.lights[k];
-        const bool own = from >= 0 && l.twin == from;
-        if (l.kind == RT_LIGHT_QUAD)
-        {
-            sum += own ? Real(0) : l.weight * quad_light_pdf(l, origin, direction);
-        }
-        else
-        {
-            sum += l.weight * sphere_light_pdf(l, origin, direction, own);
-        }
-    }
-    return sum;
-}
-template <typename Real, typename Tally>
-__device__ inline vec3<Real> lights_random(const scene_view<Real>& sc, vec3<Real> origin, pcg32& g, Tally& tl)
-{
-    tl.add(tally_light);
-    const Real xi = random_real<Real>(g);
-    int k = 0;
-    while (k < sc.light_count - 1 && !(xi < sc.lights[k].cdf))
-    {
-        ++k;
-    }
-    const light_rec<Real>& l = sc.lights[k];
-    return l.kind == RT_LIGHT_QUAD ? quad_light_random(l, origin, g) : sphere_light_random(l, origin, g);
-}
-
-// --- one ray of a path (src/gpu/camera.cuh:109-151, 304-437) --------------------------------------------------------------
-enum class bounce
-{
-    none,
-    own_density,
-    weighted,
-};
-
-template <typename Real, typename Tally>
-__device__ inline bounce diffuse_bounce(const scene_view<Real>& sc, bool matte, const hit_record<Real>& rec, int from,
-                                        pcg32& g, vec3<Real>& direction, Real& weight, Tally& tl)
-{
-    int strategy = sc.sampling;
-    if (!matte && strategy != RT_SAMPLING_LIGHTS)
-    {
-        strategy = RT_SAMPLING_MATERIAL;
-    }
-    if (strategy == RT_SAMPLING_MATERIAL)
-    {
-        if (matte)
-        {
-            direction = around(rec.normal, random_cosine_direction<Real>(g));
-            return dot(rec.normal, direction) > Real(0) ? bounce::own_density : bounce::none;
-        }
-        direction = random_unit_vector<Real>(g);
-        return bounce::own_density;
-    }
-    bool toward_lights = strategy == RT_SAMPLING_LIGHTS;
-    if (strategy == RT_SAMPLING_MIXTURE)
-    {
-        toward_lights = random_real<Real>(g) < Real(0.5);
-    }
-    if (toward_lights)
-    {
-        direction = lights_random(sc, rec.p, g, tl);
-    }
-    else if (strategy == RT_SAMPLING_UNIFORM)
-    {
-        direction = random_unit_vector<Real>(g);
-    }
-    else
-    {
-        direction = around(rec.normal, random_cosine_direction<Real>(g));
-    }
-    const Real p_scatter = matte ? cosine_density(rec.normal, direction) : uniform_sphere_density<Real>;
-    Real p = uniform_sphere_density<Real>;
-    if (strategy != RT_SAMPLING_UNIFORM)
-    {
-        const Real p_lights = lights_pdf_value(sc, rec.p, direction, from, tl);
-        p = strategy == RT_SAMPLING_LIGHTS ? p_lights : Real(0.5) * p_lights + Real(0.5) * p_scatter;
-    }
-    if (!(p > Real(0)) || !(p_scatter > Real(0)))
-    {
-        return bounce::none;
-    }
-    weight = p_scatter / p;
-    return bounce::weighted;
-}
-
-// One ray of a first-book path: spheres that stand still, lambertian of one color, metal, glass, and the sky.
-// True when the sample is complete (its color in `sample`).
-template <typename Real, typename Tally>
-__device__ inline bool first_book_after_walk(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                             pcg32& g, vec3<Real>& sample, const hit<Real>& h, Tally& tl)
-{
-    if (h.index < 0)
-    {
-        sample = p.throughput * sky(cam, p.r.direction);
-        return true;
-    }
-    const sphere<Real> s = load_sphere(sc.spheres, h.index);
-    tl.add(tally_hit);
-    hit_record<Real> rec;
-    rec.p = p.r.at(h.t);
-    const vec3<Real> outward = (Real(1) / s.radius) * (rec.p - s.center);
-    rec.front_face = h.entering;
-    rec.normal = rec.front_face ? outward : -outward;
-    const material_rec<Real> m = sc.materials[sc.prim_materials[h.index]];
-    vec3<Real> direction;
-    if (!scatter(m, p.r.direction, rec, g, direction, p.throughput))
-    {
-        sample = {Real(0), Real(0), Real(0)};
-        return true;
-    }
-    if (--p.rays_left == 0)
-    {
-        sample = {Real(0), Real(0), Real(0)};
-        return true;
-    }
-    p.r = {rec.p, direction, p.r.time};
-    p.from = h.index;
-    p.inward = dot(direction, outward) < Real(0);
-    return false;
-}
-template <typename Real, typename Tally, typename Stack>
-__device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                        pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
-{
-    return first_book_after_walk(p, sc, cam, g, sample, closest_hit<Real, false>(sc, p.r, p.from, p.inward, stack, tl),
-                                 tl);
-}
-
-// What a hit does to the path once it is known (the part of trace_ray after closest_event)
-template <typename Real, typename Tally>
-__device__ inline bool shade(path<Real>& p, const scene_view<Real>& sc, const hit<Real>& h, pcg32& g,
-                             vec3<Real>& sample, Tally& tl)
-{
-    const surface<Real> s = surface_at(sc, p.r, h, tl);
-    const material_rec<Real> m = sc.materials[s.material];
-    const bool colored = m.kind == RT_MATERIAL_LAMBERTIAN || m.kind == RT_MATERIAL_ISOTROPIC ||
-                         (m.kind == RT_MATERIAL_DIFFUSE_LIGHT && s.rec.front_face);
-    vec3<Real> c{Real(0), Real(0), Real(0)};
-    if (colored)
-    {
-        c = s.texture_index < 0 ? load3(m.albedo) : texture_value(sc, s.texture_index, s.u, s.v, s.texture_point, tl);
-    }
-    vec3<Real> direction;
-    if (m.kind == RT_MATERIAL_DIFFUSE_LIGHT)
-    {
-        sample = p.throughput * c;
-        return true;
-    }
-    if (m.kind == RT_MATERIAL_LAMBERTIAN || m.kind == RT_MATERIAL_ISOTROPIC)
-    {
-        Real weight;
-        const bounce next =
-            diffuse_bounce(sc, m.kind == RT_MATERIAL_LAMBERTIAN, s.rec, s.from, g, direction, weight, tl);
-        if (next == bounce::none)
-        {
-            sample = {Real(0), Real(0), Real(0)};
-            return true;
-        }
-        p.throughput = p.throughput * c;
-        if (next == bounce::weighted)
-        {
-            p.throughput = weight * p.throughput;
-        }
-    }
-    else if (!scatter(m, p.r.direction, s.rec, g, direction, p.throughput))
-    {
-        sample = {Real(0), Real(0), Real(0)};
-        return true;
-    }
-    if (--p.rays_left == 0)
-    {
-        sample = {Real(0), Real(0), Real(0)};
-        return true;
-    }
-    p.r = {s.rec.p, direction, p.r.time};
-    p.from = s.from;
-    p.inward = dot(direction, s.outward) < Real(0);
-    return false;
-}
-
-// One ray of a path with every feature of the three books (src/gpu/camera.cuh:364-437)
-template <typename Real, typename Tally>
-__device__ inline bool all_books_after_walk(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                            pcg32& g, vec3<Real>& sample, const hit<Real>& surface_hit, Tally& tl)
-{
-    const hit<Real> h = media_events(sc, p.r, surface_hit, g, tl);
-    if (h.index < 0)
-    {
-        sample = p.throughput * sky(cam, p.r.direction);
-        return true;
-    }
-    return shade(p, sc, h, g, sample, tl);
-}
-template <typename Real, typename Tally, typename Stack>
-__device__ inline bool trace_all_books(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                       pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
-{
-    return all_books_after_walk(p, sc, cam, g, sample, closest_hit<Real, true>(sc, p.r, p.from, p.inward, stack, tl),
-                                tl);
-}
-
-// A NaN channel of a sample counts as 0 (src/gpu/renderer.cu:311-316)
-template <typename Real>
-__device__ inline double without_nan(Real x)
-{
-    return static_cast<double>(x == x ? x : Real(0));
 }
 }

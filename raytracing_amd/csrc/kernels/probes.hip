@@ -11,7 +11,7 @@
 #include <cstdint>
 
 #include "kernel_api.hpp"
-#include "path_tracer.hpp"
+#include "books.hpp"
 
 namespace rt::dev
 {
@@ -194,9 +194,11 @@ __global__ void __launch_bounds__(256) sample_numbers_kernel(uint64_t seed, cons
         pcg32 g = sample_generator(seed, pixel, pass);
         const vec3<float> u = random_unit_vector<float>(g);
         g = sample_generator(seed, pixel, pass);
-        const vec3<float> d = random_in_unit_disk<float>(g);
+        vec3<float> d{0.0f, 0.0f, 0.0f};
+        const float a = random_real<float>(g);
+        concentric_disk(a, random_real<float>(g), d.x, d.y);
         g = sample_generator(seed, pixel, pass);
-        const vec3<float> c = random_cosine_direction<float>(g);
+        const vec3<float> c = cosine_about(vec3<float>{0.0f, 0.0f, 1.0f}, g);
         float* out = directions + 9 * k;
         out[0] = u.x;
         out[1] = u.y;

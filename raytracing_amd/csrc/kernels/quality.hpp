@@ -1,602 +1,642 @@
-// The quality integrator's device functions (float): light samples at every bounce (next-event estimation) weighed
-// against the material's own samples by multiple importance sampling, GGX rough metals and glossy coats, the lamps of
-// the world as a light set, the environment map with its alias table, and shadow rays (any-hit traversal plus the
-// media's transmittance). The reference's semantics are cited per function: src/gpu/quality_path.cuh, bsdf.cuh,
-// ggx.cuh, mis.cuh, lamps.cuh, light_sampler.cuh, environment.cuh, background.cuh, shadow.cuh.
-//
-// Under the material combination (-p material) the integrator draws exactly the books' numbers in the books' order,
-// so its images equal trace_all_books's; under nee and mis the expected image is the same, with less noise.
+// The quality integrator's device functions (float): next-event estimation at every vertex that isn't a mirror's,
+// weighed against the scattering's own samples by multiple importance sampling; GGX rough conductors (with the
+// energy lost to single scattering restored) and glossy coats over a matte base; the world's lamps and the
+// environment map (alias-table importance sampling) as the light set; shadow rays by an any-hit walk and the media's
+// transmittance. Written for this renderer from the published mathematics:
+//   - GGX's normal distribution and the height-correlated Smith masking-shadowing (Walter et al. 2007, Heitz 2014),
+//     evaluated in the vertex's tangent frame, where sin^2 theta_h = h_x^2 + h_y^2 holds its precision at small alpha;
+//   - sampling of the visible normals by spherical caps (Dupuy and Benyoub 2023), in the hemisphere configuration of
+//     the stretched view direction, in the same frame;
+//   - the multiple-scattering compensation 1 + F0 (1 - E) / E (Turquin 2019), E from this repository's table;
+//   - the glossy coat's coupled diffuse base (Ashikhmin and Shirley 2000);
+//   - Veach's power heuristic (1995) for the two strategies' weights.
+// The semantics an image depends on follow the reference (src/gpu/quality_path.cuh, bsdf.cuh, lamps.cuh,
+// light_sampler.cuh, environment.cuh, background.cuh, shadow.cuh): which vertices take a light sample, the light
+// set's uniform choice, which emitter hits count under each combination, the map's cells and densities, the camera's
+// filtered view of the map.
 #pragma once
 
+#include "books.hpp"
 #include "ggx_energy_table.hpp"
-#include "path_tracer.hpp"
 
 namespace rt::dev
 {
-// --- multiple importance sampling (src/gpu/mis.cuh:30-62) -----------------------------------------------------------
-// Veach's power heuristic, self^2 / (self^2 + other^2), as 1 / (1 + r^2) with r = other / self: no density is squared
-__device__ inline float power_weight(float self, float other)
+__device__ inline float max3(vec3<float> c) { return fmaxf(fmaxf(c.x, c.y), c.z); }
+
+// --- combining the two strategies -------------------------------------------------------------------------------------
+// The power heuristic's weight of a strategy of density `mine` against one of density `theirs`: mine^2 / (mine^2 +
+// theirs^2), formed as 1 / (1 + (theirs / mine)^2) so that no density is squared (densities of distant lamps
+// overflow); 0 for a strategy that couldn't have drawn the direction, and 0 where the ratio is NaN
+__device__ inline float power_heuristic(float mine, float theirs)
 {
-    if (!(self > 0.0f))
+    if (!(mine > 0.0f))
     {
         return 0.0f;
     }
-    const float r = other / self;
-    const float w = 1.0f / fmaf(r, r, 1.0f);
+    const float ratio = theirs / mine;
+    const float w = 1.0f / fmaf(ratio, ratio, 1.0f);
     return w == w ? w : 0.0f;
 }
-__device__ inline float light_weight(int combination, float p_light, float p_bsdf)
+// The weights under the scene's combination (src/gpu/mis.cuh:30-62): -p material counts only the scattering's
+// samples of emitters, -p nee only the light samples (a scattered ray that meets a light the light set could have
+// drawn counts 0), -p mis both, by the power heuristic. Densities of 0 mean "that strategy couldn't": a delta lobe's
+// ray (bsdf density 0) keeps all of an emitter it meets.
+struct combination
 {
-    return combination == RT_COMBINE_POWER_HEURISTIC ? power_weight(p_light, p_bsdf) : 1.0f;
-}
-__device__ inline float bsdf_weight(int combination, float p_bsdf, float p_light)
-{
-    if (combination == RT_COMBINE_BSDF_ONLY || !(p_bsdf > 0.0f) || !(p_light > 0.0f))
+    int mode;
+    __device__ float light_sample(float p_light, float p_bsdf) const
     {
-        return 1.0f;
+        return mode == RT_COMBINE_POWER_HEURISTIC ? power_heuristic(p_light, p_bsdf) : 1.0f;
     }
-    return combination == RT_COMBINE_POWER_HEURISTIC ? power_weight(p_bsdf, p_light) : 0.0f;
-}
+    __device__ float scattered_hit(float p_bsdf, float p_light) const
+    {
+        if (mode == RT_COMBINE_BSDF_ONLY || !(p_bsdf > 0.0f) || !(p_light > 0.0f))
+        {
+            return 1.0f;
+        }
+        return mode == RT_COMBINE_POWER_HEURISTIC ? power_heuristic(p_bsdf, p_light) : 0.0f;
+    }
+};
 
-// --- GGX (src/gpu/ggx.cuh: Walter 2007, Heitz 2014/2018, Dupuy-Benyoub 2023, Turquin 2019) ---------------------------
-constexpr float min_ggx_alpha = 1e-3f;
+// --- GGX in the tangent frame (z: the normal) --------------------------------------------------------------------------
+constexpr float min_ggx_alpha = 1e-3f; // below: the conductor is a perfect mirror
 
-__device__ inline vec3<float> cross(vec3<float> a, vec3<float> b)
+// D(h) = a^2 / (pi (a^2 h_z^2 + h_x^2 + h_y^2)^2) for a unit h with h_z > 0
+__device__ inline float ggx_density(vec3<float> h, float a2)
 {
-    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-// A(c) = sqrt(c^2 (1 - alpha^2) + alpha^2): Smith's Lambda for GGX is (A(c)/c - 1)/2
-__device__ inline float smith_a(float c, float alpha2) { return sqrtf(fmaf(c * c, 1.0f - alpha2, alpha2)); }
-// D(h) = alpha^2 / (pi (alpha^2 c_h^2 + sin^2 theta_h)^2), sin^2 from |n x h|^2
-__device__ inline float ggx_d(vec3<float> n, vec3<float> h, float alpha2)
-{
-    const float c = dot(n, h);
-    if (!(c > 0.0f))
+    if (!(h.z > 0.0f))
     {
         return 0.0f;
     }
-    const vec3<float> s = cross(n, h);
-    const float d = fmaf(alpha2 * c, c, dot(s, s));
-    return alpha2 * inverse_pi<float> / (d * d);
+    const float q = fmaf(a2 * h.z, h.z, fmaf(h.x, h.x, h.y * h.y));
+    return a2 * one_over_pi<float> / (q * q);
 }
-__device__ inline float ggx_g1(float c, float alpha2) { return 2.0f * c / (c + smith_a(c, alpha2)); }
-__device__ inline float ggx_g2(float cv, float cl, float alpha2)
+// sqrt(a^2 + (1 - a^2) c^2) = c (1 + 2 Lambda(c)): Smith's masking term of GGX at cos = c
+__device__ inline float smith_root(float c, float a2) { return sqrtf(fmaf(1.0f - a2, c * c, a2)); }
+// Schlick's Fresnel per channel: F0 + (1 - F0) (1 - cos)^5
+__device__ inline vec3<float> fresnel_schlick(vec3<float> f0, float cos_d)
 {
-    return 2.0f * cv * cl / (cl * smith_a(cv, alpha2) + cv * smith_a(cl, alpha2));
-}
-// Schlick's F = F0 + (1 - F0)(1 - cos)^5 per channel
-__device__ inline vec3<float> schlick(vec3<float> f0, float cosine)
-{
-    const float x = fmaxf(0.0f, 1.0f - cosine);
+    const float x = fmaxf(0.0f, 1.0f - cos_d);
     const float x2 = x * x;
-    const float x5 = x2 * x2 * x;
-    return {fmaf(1.0f - f0.x, x5, f0.x), fmaf(1.0f - f0.y, x5, f0.y), fmaf(1.0f - f0.z, x5, f0.z)};
+    const float k = x2 * x2 * x;
+    return {fmaf(1.0f - f0.x, k, f0.x), fmaf(1.0f - f0.y, k, f0.y), fmaf(1.0f - f0.z, k, f0.z)};
 }
-// f(v, l)·c_l in the height-correlated "V form", and the visible-normals density of l: D(h) / (2 (c_v + A_v))
-__device__ inline vec3<float> ggx_eval(vec3<float> n, vec3<float> v, vec3<float> l, float alpha, vec3<float> f0,
-                                       float& pdf)
+
+// A GGX reflection of (local) v into l: f(v, l) cos_l = F D G2 / (4 cos_v) with the height-correlated
+// G2 = 2 cos_v cos_l / (cos_l A(v) + cos_v A(l)), and the visible-normal density of l, G1(v) D / (4 cos_v) =
+// D / (2 (cos_v + A(v)))
+struct ggx_eval_result
 {
-    pdf = 0.0f;
-    const float cv = dot(n, v);
-    const float cl = dot(n, l);
-    const vec3<float> sum = v + l;
-    const float len2 = length_squared(sum);
-    if (!(cv > 0.0f) || !(cl > 0.0f) || !(len2 > 0.0f))
+    vec3<float> value; // f cos_l
+    float pdf;
+};
+__device__ inline ggx_eval_result ggx_reflection(vec3<float> v, vec3<float> l, float alpha, vec3<float> f0)
+{
+    ggx_eval_result out{{0.0f, 0.0f, 0.0f}, 0.0f};
+    const vec3<float> half = v + l;
+    const float len2 = length_squared(half);
+    if (!(v.z > 0.0f) || !(l.z > 0.0f) || !(len2 > 0.0f))
     {
-        return {0.0f, 0.0f, 0.0f};
+        return out;
     }
-    const vec3<float> h = rsqrtf(len2) * sum;
-    const float alpha2 = alpha * alpha;
-    const float d = ggx_d(n, h, alpha2);
-    const float av = smith_a(cv, alpha2);
-    const float al = smith_a(cl, alpha2);
-    pdf = d / (2.0f * (cv + av));
-    return (d * cl / (2.0f * (cl * av + cv * al))) * schlick(f0, dot(v, h));
+    const vec3<float> h = rsqrtf(len2) * half;
+    const float a2 = alpha * alpha;
+    const float d = ggx_density(h, a2);
+    const float av = smith_root(v.z, a2);
+    const float al = smith_root(l.z, a2);
+    out.pdf = d / (2.0f * (v.z + av));
+    out.value = (d * l.z / (2.0f * fmaf(l.z, av, v.z * al))) * fresnel_schlick(f0, dot(v, h));
+    return out;
 }
-// The visible normals by spherical caps: two numbers (azimuth, then height), whatever the outcome
-__device__ inline bool ggx_sample(vec3<float> n, vec3<float> v, float alpha, vec3<float> f0, pcg32& g,
-                                  vec3<float>& l, vec3<float>& weight, float& pdf)
+// A visible normal of (local) v by spherical caps (Dupuy and Benyoub 2023): v stretched to the hemisphere
+// configuration, a point uniform on the cap of the unit sphere above -v_h.z's plane, added to v_h, then unstretched
+__device__ inline vec3<float> visible_normal(vec3<float> v, float alpha, float u1, float u2)
+{
+    const vec3<float> vh = unit_vector(vec3<float>{alpha * v.x, alpha * v.y, v.z});
+    const float z = fmaf(1.0f - u2, 1.0f + vh.z, -vh.z);
+    const float sin_theta = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float s, c;
+    sincospif(2.0f * u1, &s, &c);
+    const vec3<float> m{sin_theta * c + vh.x, sin_theta * s + vh.y, z + vh.z};
+    return vec3<float>{alpha * m.x, alpha * m.y, fmaxf(0.0f, m.z)};
+}
+// One GGX draw from (local) v: the direction l, its weight f cos_l / pdf = F G2 / G1(v), its density; false when the
+// reflected direction goes below the surface (two numbers are drawn either way)
+__device__ inline bool ggx_draw(vec3<float> v, float alpha, vec3<float> f0, pcg32& g, vec3<float>& l,
+                                vec3<float>& weight, float& pdf)
 {
     const float u1 = random_real<float>(g);
     const float u2 = random_real<float>(g);
-    const float cv = dot(n, v);
-    if (!(cv > 0.0f))
+    if (!(v.z > 0.0f))
     {
-        l = {0.0f, 0.0f, 0.0f};
         return false;
     }
-    // the view squeezed into the hemispheres' space: its tangential part times alpha
-    const vec3<float> vs = unit_vector(alpha * (v - cv * n) + cv * n);
-    const float zs = dot(n, vs);
-    // uniform in the cap of heights (-z_s, 1]
-    const float z = fmaf(1.0f - u2, 1.0f + zs, -zs);
-    const float sin_theta = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    float sin_phi, cos_phi;
-    sincospif(2.0f * u1, &sin_phi, &cos_phi);
-    const vec3<float> c = around(n, vec3<float>{sin_theta * cos_phi, sin_theta * sin_phi, z});
-    const vec3<float> hs = c + vs;
-    const float hn = dot(n, hs);
-    const vec3<float> m = alpha * (hs - hn * n) + fmaxf(0.0f, hn) * n;
+    const vec3<float> m = visible_normal(v, alpha, u1, u2);
     const float m2 = length_squared(m);
     if (!(m2 > 0.0f))
     {
-        l = {0.0f, 0.0f, 0.0f};
         return false;
     }
     const vec3<float> h = rsqrtf(m2) * m;
     const float vh = dot(v, h);
     l = (2.0f * vh) * h - v;
-    const float cl = dot(n, l);
-    if (!(cl > 0.0f) || !(vh > 0.0f))
+    if (!(l.z > 0.0f) || !(vh > 0.0f))
     {
         return false;
     }
-    const float alpha2 = alpha * alpha;
-    const float av = smith_a(cv, alpha2);
-    const float al = smith_a(cl, alpha2);
-    weight = (cl * (cv + av) / (cl * av + cv * al)) * schlick(f0, vh);
-    pdf = ggx_d(n, h, alpha2) / (2.0f * (cv + av));
+    const float a2 = alpha * alpha;
+    const float av = smith_root(v.z, a2);
+    const float al = smith_root(l.z, a2);
+    weight = (l.z * (v.z + av) / fmaf(l.z, av, v.z * al)) * fresnel_schlick(f0, vh);
+    pdf = ggx_density(h, a2) / (2.0f * (v.z + av));
     return true;
 }
-// E_ss(mu, r) of a white conductor, bilinearly from the table (nodes at sqrt(mu) = i/31, r = j/31)
-__device__ inline float ggx_energy(float cos_v, float roughness)
+
+// E(mu, r): the energy a white GGX conductor of roughness r reflects from cos_v = mu in one scattering, bilinear in the
+// table's (sqrt(mu), r) grid of 32 x 32 nodes
+__device__ inline float single_scatter_energy(float cos_v, float roughness)
 {
-    constexpr int last = ggx_energy_size - 1;
-    const float x = sqrtf(fminf(fmaxf(cos_v, 0.0f), 1.0f)) * static_cast<float>(last);
-    const float y = fminf(fmaxf(roughness, 0.0f), 1.0f) * static_cast<float>(last);
-    const int i = static_cast<int>(x) < last - 1 ? static_cast<int>(x) : last - 1;
-    const int j = static_cast<int>(y) < last - 1 ? static_cast<int>(y) : last - 1;
+    constexpr int n = ggx_energy_size;
+    const float x = sqrtf(fminf(fmaxf(cos_v, 0.0f), 1.0f)) * static_cast<float>(n - 1);
+    const float y = fminf(fmaxf(roughness, 0.0f), 1.0f) * static_cast<float>(n - 1);
+    const int i = min(static_cast<int>(x), n - 2);
+    const int j = min(static_cast<int>(y), n - 2);
     const float fx = x - static_cast<float>(i);
     const float fy = y - static_cast<float>(j);
-    const float* row = ggx_energy_table + j * ggx_energy_size + i;
-    const float below = (1.0f - fx) * row[0] + fx * row[1];
-    const float above = (1.0f - fx) * row[ggx_energy_size] + fx * row[ggx_energy_size + 1];
-    return (1.0f - fy) * below + fy * above;
+    const float* e = ggx_energy_table + j * n + i;
+    const float lower = e[0] + fx * (e[1] - e[0]);
+    const float upper = e[n] + fx * (e[n + 1] - e[n]);
+    return lower + fy * (upper - lower);
 }
-// Turquin's multiple-scattering factor 1 + F0 (1 - E)/E
-__device__ inline vec3<float> ggx_compensation(vec3<float> f0, float cos_v, float roughness)
+// Turquin's factor for the light single scattering loses: 1 + F0 (1 - E) / E per channel
+__device__ inline vec3<float> multiple_scatter_factor(vec3<float> f0, float cos_v, float roughness)
 {
-    const float e = fminf(ggx_energy(cos_v, roughness), 1.0f);
+    const float e = fminf(single_scatter_energy(cos_v, roughness), 1.0f);
     const float lost = (1.0f - e) / e;
     return {fmaf(f0.x, lost, 1.0f), fmaf(f0.y, lost, 1.0f), fmaf(f0.z, lost, 1.0f)};
 }
 
-constexpr float gloss_coat_f0 = 0.04f;
-constexpr float gloss_diffuse_scale = static_cast<float>(28.0 / (23.0 * 3.14159265358979323846) * (1.0 - 0.04));
-__device__ inline float gloss_crossing(float c)
+// The glossy coat: a GGX dielectric layer of F0 = 0.04 over a matte base of color rho, with Ashikhmin and Shirley's
+// coupled base (28 / (23 pi)) rho (1 - F0) (1 - (1 - cos_l / 2)^5) (1 - (1 - cos_v / 2)^5); sampled half the time by
+// the coat's visible normals and half by the cosine, so its density is the two's mean
+constexpr float coat_f0 = 0.04f;
+constexpr float coat_base_scale = static_cast<float>(28.0 / (23.0 * 3.14159265358979323846) * (1.0 - 0.04));
+__device__ inline float coat_transmission(float c)
 {
     const float x = 1.0f - 0.5f * c;
     const float x2 = x * x;
     return 1.0f - x2 * x2 * x;
 }
-// The coat (GGX, F0 0.04) plus Ashikhmin-Shirley's coupled diffuse base; the density is the half-half mixture's
-__device__ inline vec3<float> gloss_eval(vec3<float> n, vec3<float> v, vec3<float> l, float alpha, vec3<float> rho,
-                                         float& pdf)
+__device__ inline ggx_eval_result glossy_coat(vec3<float> v, vec3<float> l, float alpha, vec3<float> rho)
 {
-    pdf = 0.0f;
-    const float cv = dot(n, v);
-    const float cl = dot(n, l);
-    if (!(cv > 0.0f) || !(cl > 0.0f))
+    if (!(v.z > 0.0f) || !(l.z > 0.0f))
     {
-        return {0.0f, 0.0f, 0.0f};
+        return {{0.0f, 0.0f, 0.0f}, 0.0f};
     }
-    float coat_pdf;
-    const vec3<float> coat = ggx_eval(n, v, l, alpha, vec3<float>{gloss_coat_f0, gloss_coat_f0, gloss_coat_f0}, coat_pdf);
-    pdf = 0.5f * (coat_pdf + cl * inverse_pi<float>);
-    const float base = gloss_diffuse_scale * (gloss_crossing(cl) * gloss_crossing(cv)) * cl;
-    return coat + base * rho;
-}
-__device__ inline bool gloss_sample(vec3<float> n, vec3<float> v, float alpha, vec3<float> rho, pcg32& g,
-                                    vec3<float>& l, vec3<float>& weight, float& pdf)
-{
-    if (random_real<float>(g) < 0.5f)
-    {
-        vec3<float> coat_weight;
-        float coat_pdf;
-        if (!ggx_sample(n, v, alpha, vec3<float>{gloss_coat_f0, gloss_coat_f0, gloss_coat_f0}, g, l, coat_weight,
-                        coat_pdf))
-        {
-            return false;
-        }
-    }
-    else
-    {
-        l = around(n, random_cosine_direction<float>(g));
-        if (!(dot(n, l) > 0.0f))
-        {
-            return false;
-        }
-    }
-    const vec3<float> f = gloss_eval(n, v, l, alpha, rho, pdf);
-    if (!(pdf > 0.0f))
-    {
-        return false;
-    }
-    weight = (1.0f / pdf) * f;
-    return true;
+    ggx_eval_result coat = ggx_reflection(v, l, alpha, {coat_f0, coat_f0, coat_f0});
+    const float base = coat_base_scale * (coat_transmission(l.z) * coat_transmission(v.z)) * l.z;
+    coat.value = coat.value + base * rho;
+    coat.pdf = 0.5f * (coat.pdf + l.z * one_over_pi<float>);
+    return coat;
 }
 
-// --- the BSDF lobes (src/gpu/bsdf.cuh:27-201) --------------------------------------------------------------------------
-enum class lobe : int
+// --- the vertex's scattering ------------------------------------------------------------------------------------------
+enum class scatter_kind : int
 {
-    lambertian,
-    isotropic,
-    conductor,
-    gloss,
-    smooth_conductor,
-    books_delta,
+    lambert,   // Lambert's cosine lobe (the books' matte)
+    phase,     // the isotropic phase function (smoke)
+    conductor, // GGX conductor, alpha >= min_ggx_alpha
+    coat,      // glossy coat over a matte base
+    mirror,    // a conductor too smooth for GGX: Schlick's Fresnel times a perfect mirror
+    books,     // the books' metal and glass (delta lobes)
 };
-struct bsdf
+struct scatterer
 {
-    lobe kind;
-    vec3<float> albedo;
-    float alpha;
-    vec3<float> n;
-    vec3<float> wo;
-    float roughness;
+    scatter_kind kind;
+    vec3<float> color;   // albedo, F0 (conductors) or the base color (coat)
+    frame<float> local;  // the tangent frame around the normal facing the viewer (GGX lobes)
+    vec3<float> v;       // toward the viewer, in `local` (GGX lobes)
+    float alpha;         // GGX: r^2
+    float roughness;     // GGX: r, the energy table's column
 };
-struct bsdf_sample
+__device__ inline bool is_delta(scatter_kind k) { return k == scatter_kind::mirror || k == scatter_kind::books; }
+
+// The vertex's scattering from its material and GGX side entry (src/gpu/bsdf.cuh:58-110): a conductor's F0 is the
+// material's albedo, a coat's base is the material's color
+__device__ inline scatterer make_scatterer(const material_rec<float>& m, const microfacet_rec* side, vec3<float> color,
+                                           const vertex<float>& vx, vec3<float> ray_direction)
+{
+    scatterer s{};
+    s.color = color;
+    const int side_kind = side ? side->kind : RT_MICROFACET_NONE;
+    if (side_kind == RT_MICROFACET_CONDUCTOR || side_kind == RT_MICROFACET_GLOSS)
+    {
+        s.alpha = side->alpha;
+        s.roughness = side->roughness;
+        s.local = frame_around(vx.n);
+        s.v = s.local.to_local(-unit_vector(ray_direction));
+        if (side_kind == RT_MICROFACET_GLOSS)
+        {
+            s.kind = scatter_kind::coat;
+        }
+        else
+        {
+            s.kind = s.alpha >= min_ggx_alpha ? scatter_kind::conductor : scatter_kind::mirror;
+            s.color = load3(m.albedo);
+        }
+        return s;
+    }
+    s.kind = m.kind == RT_MATERIAL_LAMBERTIAN   ? scatter_kind::lambert
+             : m.kind == RT_MATERIAL_ISOTROPIC ? scatter_kind::phase
+                                               : scatter_kind::books;
+    return s;
+}
+
+// f cos at world direction wi, and the density the scattering would draw it with (0 for delta lobes)
+__device__ inline ggx_eval_result scatter_value(const scatterer& s, const vertex<float>& vx, vec3<float> wi)
+{
+    switch (s.kind)
+    {
+    case scatter_kind::lambert:
+    {
+        const float pdf = cosine_density(vx.n, wi);
+        return {pdf * s.color, pdf};
+    }
+    case scatter_kind::phase:
+        return {one_over_four_pi<float> * s.color, one_over_four_pi<float>};
+    case scatter_kind::conductor:
+    {
+        ggx_eval_result r = ggx_reflection(s.v, s.local.to_local(wi), s.alpha, s.color);
+        r.value = multiple_scatter_factor(s.color, s.v.z, s.roughness) * r.value;
+        return r;
+    }
+    case scatter_kind::coat:
+        return glossy_coat(s.v, s.local.to_local(wi), s.alpha, s.color);
+    default:
+        return {{0.0f, 0.0f, 0.0f}, 0.0f};
+    }
+}
+
+struct scatter_draw
 {
     vec3<float> wi;
-    vec3<float> weight;
-    float pdf;
-    bool delta;
+    vec3<float> weight; // f cos / pdf, or the delta lobe's factor
+    float pdf;          // 0 for a delta lobe
 };
-__device__ inline bsdf make_bsdf(const material_rec<float>& m, const microfacet_rec* side, vec3<float> c,
-                                 const hit_record<float>& rec, vec3<float> ray_direction)
+// A direction drawn from the scattering; false when the path is absorbed
+__device__ inline bool scatter_sample(const scatterer& s, const material_rec<float>& m, vec3<float> ray_direction,
+                                      const vertex<float>& vx, pcg32& g, scatter_draw& out)
 {
-    bsdf b{};
-    b.n = rec.normal;
-    b.albedo = c;
-    if (side && side->kind == RT_MICROFACET_CONDUCTOR)
+    out.pdf = 0.0f;
+    switch (s.kind)
     {
-        b.wo = -unit_vector(ray_direction);
-        b.kind = side->alpha >= min_ggx_alpha ? lobe::conductor : lobe::smooth_conductor;
-        b.albedo = load3(m.albedo);
-        b.alpha = side->alpha;
-        b.roughness = side->roughness;
-        return b;
-    }
-    if (side && side->kind == RT_MICROFACET_GLOSS)
-    {
-        b.wo = -unit_vector(ray_direction);
-        b.kind = lobe::gloss;
-        b.alpha = side->alpha;
-        return b;
-    }
-    b.kind = m.kind == RT_MATERIAL_LAMBERTIAN ? lobe::lambertian
-             : m.kind == RT_MATERIAL_ISOTROPIC ? lobe::isotropic
-                                               : lobe::books_delta;
-    return b;
-}
-__device__ inline bool is_delta(const bsdf& b) { return b.kind >= lobe::smooth_conductor; }
-
-__device__ inline bool sample_bsdf(const bsdf& b, const material_rec<float>& m, vec3<float> direction,
-                                   const hit_record<float>& rec, pcg32& g, bsdf_sample& out)
-{
-    switch (b.kind)
-    {
-    case lobe::lambertian:
-        out.wi = around(b.n, random_cosine_direction<float>(g));
-        if (!(dot(b.n, out.wi) > 0.0f))
-        {
-            return false;
-        }
-        out.weight = b.albedo;
-        out.pdf = cosine_density(b.n, out.wi);
-        out.delta = false;
-        return true;
-    case lobe::isotropic:
+    case scatter_kind::lambert:
+        out.wi = cosine_about(vx.n, g);
+        out.weight = s.color;
+        out.pdf = cosine_density(vx.n, out.wi);
+        return dot(vx.n, out.wi) > 0.0f;
+    case scatter_kind::phase:
         out.wi = random_unit_vector<float>(g);
-        out.weight = b.albedo;
-        out.pdf = uniform_sphere_density<float>;
-        out.delta = false;
+        out.weight = s.color;
+        out.pdf = one_over_four_pi<float>;
         return true;
-    case lobe::books_delta:
+    case scatter_kind::books:
+        if (m.kind == RT_MATERIAL_METAL)
+        {
+            out.weight = load3(m.albedo);
+            return fuzzed_mirror(ray_direction, vx.n, m.fuzz, g, out.wi);
+        }
+        out.weight = {1.0f, 1.0f, 1.0f};
+        out.wi = dielectric_direction(ray_direction, vx.n, vx.front, m.refraction_index, g);
+        return true;
+    case scatter_kind::mirror:
     {
-        vec3<float> weight{1.0f, 1.0f, 1.0f};
-        if (!scatter(m, direction, rec, g, out.wi, weight))
+        if (!(s.v.z > 0.0f))
         {
             return false;
         }
-        out.weight = weight;
-        out.pdf = 0.0f;
-        out.delta = true;
+        out.wi = s.local.to_world({-s.v.x, -s.v.y, s.v.z});
+        out.weight = fresnel_schlick(s.color, s.v.z);
         return true;
     }
-    case lobe::conductor:
-        out.delta = false;
-        if (!ggx_sample(b.n, b.wo, b.alpha, b.albedo, g, out.wi, out.weight, out.pdf))
+    case scatter_kind::conductor:
+    {
+        vec3<float> l;
+        if (!ggx_draw(s.v, s.alpha, s.color, g, l, out.weight, out.pdf))
         {
             return false;
         }
-        out.weight = out.weight * ggx_compensation(b.albedo, dot(b.n, b.wo), b.roughness);
+        out.wi = s.local.to_world(l);
+        out.weight = multiple_scatter_factor(s.color, s.v.z, s.roughness) * out.weight;
         return true;
-    case lobe::smooth_conductor:
-        if (!(dot(b.n, b.wo) > 0.0f))
+    }
+    case scatter_kind::coat:
+    {
+        vec3<float> l;
+        if (random_real<float>(g) < 0.5f)
+        {
+            vec3<float> coat_weight;
+            float coat_pdf;
+            if (!ggx_draw(s.v, s.alpha, {coat_f0, coat_f0, coat_f0}, g, l, coat_weight, coat_pdf))
+            {
+                return false;
+            }
+        }
+        else
+        {
+            l = s.local.to_local(cosine_about(vx.n, g));
+            if (!(l.z > 0.0f))
+            {
+                return false;
+            }
+        }
+        const ggx_eval_result f = glossy_coat(s.v, l, s.alpha, s.color);
+        if (!(f.pdf > 0.0f))
         {
             return false;
         }
-        out.wi = unit_vector(reflect(-b.wo, b.n));
-        out.weight = schlick(b.albedo, dot(b.n, b.wo));
-        out.pdf = 0.0f;
-        out.delta = true;
+        out.wi = s.local.to_world(l);
+        out.weight = (1.0f / f.pdf) * f.value;
+        out.pdf = f.pdf;
         return true;
-    case lobe::gloss:
-        out.delta = false;
-        return gloss_sample(b.n, b.wo, b.alpha, b.albedo, g, out.wi, out.weight, out.pdf);
+    }
     }
     return false;
 }
-__device__ inline vec3<float> eval_bsdf(const bsdf& b, vec3<float> wi, float& pdf)
-{
-    switch (b.kind)
-    {
-    case lobe::lambertian:
-        pdf = cosine_density(b.n, wi);
-        return pdf * b.albedo;
-    case lobe::isotropic:
-        pdf = uniform_sphere_density<float>;
-        return uniform_sphere_density<float> * b.albedo;
-    case lobe::conductor:
-    {
-        const vec3<float> single = ggx_eval(b.n, b.wo, wi, b.alpha, b.albedo, pdf);
-        return ggx_compensation(b.albedo, dot(b.n, b.wo), b.roughness) * single;
-    }
-    case lobe::gloss:
-        return gloss_eval(b.n, b.wo, wi, b.alpha, b.albedo, pdf);
-    default:
-        pdf = 0.0f;
-        return {0.0f, 0.0f, 0.0f};
-    }
-}
 
-// --- lamps (src/gpu/lamps.cuh:55-146) ------------------------------------------------------------------------------------
-struct lamp_sample
+// --- the light set: lamps and the environment map ------------------------------------------------------------------
+// A lamp (a glowing quad or sphere of the world): points uniform on a quad's area, density s^2 / (A cos theta_l) per
+// solid angle; directions uniform in the cone of a sphere seen from outside, 1 / (2 pi (1 - cos theta_max)). Exactly
+// two numbers per draw (src/gpu/lamps.cuh:55-146).
+__device__ inline float quad_lamp_density(float d2, float cos_l, float area) { return d2 / (cos_l * area); }
+__device__ inline float sphere_lamp_density(float radius, float d2)
+{
+    return 0.5f * one_over_pi<float> / cone_drop(radius, d2);
+}
+struct light_draw
 {
     vec3<float> wi;
-    float distance;
-    float pdf;
+    float distance; // to the light's point (infinity for the map)
+    vec3<float> radiance;
+    float pdf;      // per solid angle, the light's choice included
+    int skip;       // the lamp's own primitive, which its shadow ray doesn't count (-1: none)
 };
-__device__ inline float quad_lamp_density(float distance_squared, float cos_light, float area)
-{
-    return distance_squared / (cos_light * area);
-}
-__device__ inline float sphere_lamp_density(float radius, float distance_squared)
-{
-    return 1.0f / (2.0f * pi_v<float> * one_minus_cos_theta_max(radius, distance_squared));
-}
-// Exactly two numbers in every call
-__device__ inline bool sample_lamp(const emitter_rec& e, vec3<float> x, pcg32& g, lamp_sample& out)
+__device__ inline bool draw_lamp(const emitter_rec& e, vec3<float> x, pcg32& g, light_draw& out)
 {
     const float a = random_real<float>(g);
     const float b = random_real<float>(g);
-    const vec3<float> corner = load3(e.corner);
+    const vec3<float> anchor = load3(e.corner);
     if (e.kind == RT_LIGHT_QUAD)
     {
-        const vec3<float> to = (corner - x) + a * load3(e.u) + b * load3(e.v);
+        const vec3<float> to = (anchor - x) + a * load3(e.u) + b * load3(e.v);
         const float d2 = length_squared(to);
         const float inv = rsqrtf(d2);
         out.wi = inv * to;
         out.distance = d2 * inv;
-        const float cos_light = -dot(out.wi, load3(e.normal));
-        if (!(cos_light > 0.0f))
-        {
-            return false;
-        }
-        out.pdf = quad_lamp_density(d2, cos_light, e.size);
-        return true;
+        const float cos_l = -dot(out.wi, load3(e.normal));
+        out.pdf = quad_lamp_density(d2, cos_l, e.size);
+        return cos_l > 0.0f;
     }
-    const vec3<float> to_center = corner - x;
-    const float d2 = sphere_light_distance_squared(to_center);
+    const vec3<float> to_centre = anchor - x;
+    const float d2 = squared_distance(to_centre);
     const float r2 = e.size * e.size;
     if (!(d2 > r2))
     {
         return false;
     }
-    const float omc = one_minus_cos_theta_max(e.size, d2);
-    float sin_phi, cos_phi;
-    sincospif(2.0f * a, &sin_phi, &cos_phi);
-    const float omz = b * omc;
-    const float sin2 = omz * (2.0f - omz);
-    const float sin_theta = sqrtf(sin2);
-    const float cos_theta = 1.0f - omz;
     const float inv = rsqrtf(d2);
-    out.wi = around(inv * to_center, vec3<float>{cos_phi * sin_theta, sin_phi * sin_theta, cos_theta});
-    const float distance = d2 * inv;
-    out.distance = distance * cos_theta - sqrtf(fmaxf(0.0f, fmaf(-d2, sin2, r2)));
+    // 1 - cos theta uniform in [0, 1 - cos theta_max]: sin^2 and cos from it without cancellation
+    const float drop = b * cone_drop(e.size, d2);
+    const float sin2 = drop * (2.0f - drop);
+    const float cos_t = 1.0f - drop;
+    const float sin_t = sqrtf(sin2);
+    float s, c;
+    sincospif(2.0f * a, &s, &c);
+    out.wi = frame_around(inv * to_centre).to_world({sin_t * c, sin_t * s, cos_t});
+    // the near crossing of the sphere along wi: |c - x| cos - sqrt(r^2 - |c - x|^2 sin^2)
+    out.distance = d2 * inv * cos_t - sqrtf(fmaxf(0.0f, fmaf(-d2, sin2, r2)));
     out.pdf = sphere_lamp_density(e.size, d2);
     return true;
 }
-__device__ inline float lamp_pdf(const emitter_rec& e, vec3<float> x, vec3<float> unit_dir, float distance)
+// The density with which the lamp would have been drawn toward unit_dir from x, its point `distance` away
+__device__ inline float lamp_density(const emitter_rec& e, vec3<float> x, vec3<float> unit_dir, float distance)
 {
     if (e.kind == RT_LIGHT_QUAD)
     {
-        const float cos_light = -dot(unit_dir, load3(e.normal));
-        return cos_light > 0.0f ? quad_lamp_density(distance * distance, cos_light, e.size) : 0.0f;
+        const float cos_l = -dot(unit_dir, load3(e.normal));
+        return cos_l > 0.0f ? quad_lamp_density(distance * distance, cos_l, e.size) : 0.0f;
     }
-    const float d2 = sphere_light_distance_squared(load3(e.corner) - x);
+    const float d2 = squared_distance(load3(e.corner) - x);
     return d2 > e.size * e.size ? sphere_lamp_density(e.size, d2) : 0.0f;
 }
 
-// --- the environment map (src/gpu/environment.cuh, src/gpu/background.cuh) --------------------------------------------
-__device__ inline vec3<float> to_map(const quality_view& e, vec3<float> d)
-{
-    return {e.env_cos_yaw * d.x + e.env_sin_yaw * d.z, d.y, -e.env_sin_yaw * d.x + e.env_cos_yaw * d.z};
-}
-__device__ inline vec3<float> from_map(const quality_view& e, vec3<float> d)
-{
-    return {e.env_cos_yaw * d.x - e.env_sin_yaw * d.z, d.y, e.env_sin_yaw * d.x + e.env_cos_yaw * d.z};
-}
-// u = 0.5 + atan2(m_z, m_x)/(2 pi), t = atan2(sqrt(m_x^2 + m_z^2), m_y)/pi: +y up, seen from inside
-__device__ inline void map_coordinates(vec3<float> m, float& u, float& t)
-{
-    const float phi = atan2f(m.z, m.x);
-    const float theta = atan2f(sqrtf(m.x * m.x + m.z * m.z), m.y);
-    u = 0.5f + phi * (0.5f * inverse_pi<float>);
-    t = theta * inverse_pi<float>;
-}
-__device__ inline int clamp_index(int k, int n) { return k < 0 ? 0 : (k < n ? k : n - 1); }
-// The one map from a direction to its cell: the radiance and the density of a direction come from the same texel
-__device__ inline int env_cell(const quality_view& e, vec3<float> world_dir)
+// The environment map: equirectangular, +y up, turned by the yaw; u = 1/2 + atan2(z, x) / (2 pi) across, t = polar
+// angle / pi down. Its cells are exact in solid angle (rows bounded by cosines), and each texel holds its cell's
+// radiance and density per steradian. A direction's radiance and density are read from the same texel, whichever way
+// the direction was found (src/gpu/environment.cuh:59-135).
+struct map_point
 {
     float u, t;
-    map_coordinates(to_map(e, world_dir), u, t);
-    const int i = clamp_index(static_cast<int>(u * static_cast<float>(e.env_width)), e.env_width);
-    const int j = clamp_index(static_cast<int>(t * static_cast<float>(e.env_height)), e.env_height);
-    return j * e.env_width + i;
-}
-// A direction at fractions (s_theta in cos theta, s_phi in phi) of cell (i, j): uniform over the cell's solid angle
-__device__ inline vec3<float> cell_direction(const quality_view& e, int i, int j, float s_theta, float s_phi)
+};
+__device__ inline vec3<float> world_to_map(const quality_view& q, vec3<float> d)
 {
-    const float top = e.env_row_cos[j];
-    const float bottom = e.env_row_cos[j + 1];
-    const float cos_theta = top - s_theta * (top - bottom);
-    const float sin_theta = sqrtf(fmaxf(0.0f, (1.0f - cos_theta) * (1.0f + cos_theta)));
-    float sin_phi, cos_phi;
-    sincospif(2.0f * (static_cast<float>(i) + s_phi) / static_cast<float>(e.env_width) - 1.0f, &sin_phi, &cos_phi);
-    return from_map(e, vec3<float>{sin_theta * cos_phi, cos_theta, sin_theta * sin_phi});
+    return {q.env_cos_yaw * d.x + q.env_sin_yaw * d.z, d.y, q.env_cos_yaw * d.z - q.env_sin_yaw * d.x};
 }
-__device__ inline vec3<float> env_radiance(const quality_view& e, vec3<float> dir, float& pdf)
+__device__ inline vec3<float> map_to_world(const quality_view& q, vec3<float> d)
 {
-    const env_texel t = e.env_texels[env_cell(e, dir)];
-    pdf = t.pdf;
-    return e.env_scale * vec3<float>{t.r, t.g, t.b};
+    return {q.env_cos_yaw * d.x - q.env_sin_yaw * d.z, d.y, q.env_sin_yaw * d.x + q.env_cos_yaw * d.z};
 }
-__device__ inline vec3<float> env_lerp(vec3<float> a, vec3<float> b, float s) { return a + s * (b - a); }
-// Bilinear between the four nearest texel centres (columns wrap, rows clamp): what camera rays and rays after a delta
-// vertex see
-__device__ inline vec3<float> env_radiance_filtered(const quality_view& e, vec3<float> dir)
+__device__ inline map_point map_of(const quality_view& q, vec3<float> world_dir)
 {
-    float u, t;
-    map_coordinates(to_map(e, dir), u, t);
-    const float x = u * static_cast<float>(e.env_width) - 0.5f;
-    const float y = t * static_cast<float>(e.env_height) - 0.5f;
-    const float x0 = floorf(x);
-    const float y0 = floorf(y);
-    const float fx = x - x0;
-    const float fy = y - y0;
-    const int i = static_cast<int>(x0);
-    const int i0 = i < 0 ? i + e.env_width : i;
-    const int i1 = i0 + 1 < e.env_width ? i0 + 1 : 0;
-    const int j = static_cast<int>(y0);
-    const int j0 = clamp_index(j, e.env_height) * e.env_width;
-    const int j1 = clamp_index(j + 1, e.env_height) * e.env_width;
-    const auto col = [&](int k) {
-        const env_texel t = e.env_texels[k];
-        return vec3<float>{t.r, t.g, t.b};
+    const vec3<float> m = world_to_map(q, world_dir);
+    return {0.5f + atan2f(m.z, m.x) * (0.5f * one_over_pi<float>),
+            atan2f(sqrtf(m.x * m.x + m.z * m.z), m.y) * one_over_pi<float>};
+}
+__device__ inline int texel_index(const quality_view& q, vec3<float> world_dir)
+{
+    const map_point mp = map_of(q, world_dir);
+    const int i = min(max(static_cast<int>(mp.u * static_cast<float>(q.env_width)), 0), q.env_width - 1);
+    const int j = min(max(static_cast<int>(mp.t * static_cast<float>(q.env_height)), 0), q.env_height - 1);
+    return j * q.env_width + i;
+}
+__device__ inline vec3<float> texel_radiance(const quality_view& q, int k, float& density)
+{
+    const env_texel x = q.env_texels[k];
+    density = x.pdf;
+    return q.env_scale * vec3<float>{x.r, x.g, x.b};
+}
+// The map between its four nearest texel centres (columns wrap around, rows stop at the poles): what the camera and
+// rays after a mirror see, so that the map looks smooth where it is magnified
+__device__ inline vec3<float> map_bilinear(const quality_view& q, vec3<float> dir)
+{
+    const map_point mp = map_of(q, dir);
+    const float x = mp.u * static_cast<float>(q.env_width) - 0.5f;
+    const float y = mp.t * static_cast<float>(q.env_height) - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float sx = x - fx0, sy = y - fy0;
+    const int w = q.env_width;
+    const int c0 = static_cast<int>(fx0) < 0 ? static_cast<int>(fx0) + w : static_cast<int>(fx0);
+    const int c1 = c0 + 1 < w ? c0 + 1 : 0;
+    const int r0 = min(max(static_cast<int>(fy0), 0), q.env_height - 1) * w;
+    const int r1 = min(max(static_cast<int>(fy0) + 1, 0), q.env_height - 1) * w;
+    const auto rgb = [&](int k) {
+        const env_texel e = q.env_texels[k];
+        return vec3<float>{e.r, e.g, e.b};
     };
-    const vec3<float> top = env_lerp(col(j0 + i0), col(j0 + i1), fx);
-    const vec3<float> bottom = env_lerp(col(j1 + i0), col(j1 + i1), fx);
-    return e.env_scale * env_lerp(top, bottom, fy);
+    const vec3<float> upper = lerp3(rgb(r0 + c0), rgb(r0 + c1), sx);
+    const vec3<float> lower = lerp3(rgb(r1 + c0), rgb(r1 + c1), sx);
+    return q.env_scale * lerp3(upper, lower, sy);
 }
-__device__ inline bool environment_sampled(const quality_view& e) { return e.env_present && e.env_bins != nullptr; }
-__device__ inline vec3<float> background(const quality_view& q, const camera_rec<float>& cam, vec3<float> dir,
-                                         bool filtered, float& pdf)
+__device__ inline bool map_sampled(const quality_view& q) { return q.env_present && q.env_bins != nullptr; }
+// What a ray that leaves the scene sees: the books' sky without a map; with one, the bilinear map (a camera ray or a
+// ray after a mirror) or its texel and the texel's density (any other ray, whose weight may need it)
+__device__ inline vec3<float> escaped(const quality_view& q, const camera_rec<float>& cam, vec3<float> dir,
+                                      bool smooth, float& density)
 {
-    pdf = 0.0f;
+    density = 0.0f;
     if (!q.env_present)
     {
         return sky(cam, dir);
     }
-    if (filtered)
+    if (smooth)
     {
-        return env_radiance_filtered(q, dir);
+        return map_bilinear(q, dir);
     }
-    float texel_pdf;
-    const vec3<float> le = env_radiance(q, dir, texel_pdf);
-    pdf = environment_sampled(q) ? texel_pdf : 0.0f;
+    float texel_density;
+    const vec3<float> le = texel_radiance(q, texel_index(q, dir), texel_density);
+    density = map_sampled(q) ? texel_density : 0.0f;
     return le;
 }
-// Four numbers: the bin (top bits of one output), its coin, then cos theta and phi inside the cell
-__device__ inline bool sample_environment(const quality_view& e, pcg32& g, vec3<float>& wi, vec3<float>& le, float& pdf)
+// A direction of the map drawn in proportion to radiance x solid angle: the alias table's bin from a word's top bits,
+// its coin, then a point uniform in the chosen cell's solid angle (cos theta and phi uniform within the cell). Four
+// numbers.
+__device__ inline bool draw_map(const quality_view& q, pcg32& g, light_draw& out)
 {
-    if (!environment_sampled(e))
-    {
-        return false;
-    }
-    const int bin = static_cast<int>(g.next() >> (32 - e.env_log2_bins));
-    const alias_bin b = e.env_bins[bin];
-    const int cell = random_real<float>(g) < b.q ? bin : b.alias;
-    const int j = cell / e.env_width;
-    const int i = cell - j * e.env_width;
-    const float s_theta = random_real<float>(g);
-    const float s_phi = random_real<float>(g);
-    wi = cell_direction(e, i, j, s_theta, s_phi);
-    le = env_radiance(e, wi, pdf);
-    return pdf > 0.0f;
+    const int bin = static_cast<int>(g.next() >> (32 - q.env_log2_bins));
+    const alias_bin entry = q.env_bins[bin];
+    const int cell = random_real<float>(g) < entry.q ? bin : entry.alias;
+    const float in_theta = random_real<float>(g);
+    const float in_phi = random_real<float>(g);
+    const int row = cell / q.env_width;
+    const int column = cell - row * q.env_width;
+    const float cos_top = q.env_row_cos[row];
+    const float cos_theta = cos_top - in_theta * (cos_top - q.env_row_cos[row + 1]);
+    const float sin_theta = sqrtf(fmaxf(0.0f, (1.0f - cos_theta) * (1.0f + cos_theta)));
+    float s, c;
+    sincospif(2.0f * (static_cast<float>(column) + in_phi) / static_cast<float>(q.env_width) - 1.0f, &s, &c);
+    out.wi = map_to_world(q, vec3<float>{sin_theta * c, cos_theta, sin_theta * s});
+    float density;
+    out.radiance = texel_radiance(q, texel_index(q, out.wi), density);
+    out.pdf = density;
+    out.distance = inf<float>;
+    out.skip = -1;
+    return density > 0.0f;
 }
 
-// --- the light sampler (src/gpu/light_sampler.cuh:40-102) ------------------------------------------------------------
-struct light_sample
-{
-    vec3<float> wi;
-    float distance;
-    vec3<float> le;
-    float pdf;
-    int skip;
-};
-__device__ inline int light_count(const quality_view& q)
-{
-    return q.lamp_count + (environment_sampled(q) ? 1 : 0);
-}
-// One number picks the light uniformly (lamps first, the map last); then 2 more for a lamp, 4 for the map
+// The light set: the lamps then the map (when it is sampled), one chosen uniformly by one number (src/gpu/
+// light_sampler.cuh:40-102)
+__device__ inline int light_set_size(const quality_view& q) { return q.lamp_count + (map_sampled(q) ? 1 : 0); }
 template <typename Tally>
-__device__ inline bool sample_light(const quality_view& q, vec3<float> x, pcg32& g, light_sample& out, Tally& tl)
+__device__ inline bool draw_light(const quality_view& q, vec3<float> x, pcg32& g, light_draw& out, Tally& tl)
 {
-    const int n = light_count(q);
-    if (n == 0)
-    {
-        return false;
-    }
-    const float xi = random_real<float>(g);
-    int k = static_cast<int>(xi * static_cast<float>(n));
-    k = k < n ? k : n - 1;
+    const int n = light_set_size(q);
+    const int k = min(static_cast<int>(random_real<float>(g) * static_cast<float>(n)), n - 1);
     if (k < q.lamp_count)
     {
         const emitter_rec e = q.lamps[k];
         tl.add(tally_lamp);
-        lamp_sample s;
-        if (!sample_lamp(e, x, g, s))
+        if (!draw_lamp(e, x, g, out))
         {
             return false;
         }
-        out.wi = s.wi;
-        out.distance = s.distance;
-        out.le = load3(e.radiance);
-        out.pdf = q.light_pmf * s.pdf;
+        out.radiance = load3(e.radiance);
         out.skip = e.primitive;
-        return true;
     }
-    float pdf;
-    tl.add(tally_alias_bin);
-    tl.add(tally_env_texel);
-    if (!sample_environment(q, g, out.wi, out.le, pdf))
+    else
     {
-        return false;
+        tl.add(tally_alias_bin);
+        tl.add(tally_env_texel);
+        if (!draw_map(q, g, out))
+        {
+            return false;
+        }
     }
-    out.distance = inf<float>;
-    out.pdf = q.light_pmf * pdf;
-    out.skip = -1;
+    out.pdf *= q.light_pmf;
     return true;
 }
-__device__ inline float lamp_hit_pdf(const quality_view& q, int id, vec3<float> x, vec3<float> unit_dir, float distance)
+// The density with which the light set would have drawn the lamp primitive `id` in unit_dir from x
+__device__ inline float lamp_hit_density(const quality_view& q, int id, vec3<float> x, vec3<float> unit_dir,
+                                         float distance)
 {
-    if (q.lamp_count == 0)
-    {
-        return 0.0f;
-    }
-    const int k = q.lamp_of[id];
-    return k >= 0 ? q.light_pmf * lamp_pdf(q.lamps[k], x, unit_dir, distance) : 0.0f;
+    const int k = q.lamp_count > 0 ? q.lamp_of[id] : -1;
+    return k >= 0 ? q.light_pmf * lamp_density(q.lamps[k], x, unit_dir, distance) : 0.0f;
 }
 
-// --- shadow rays (src/gpu/shadow.cuh:26-165) -------------------------------------------------------------------------
-// Anything in (0.001, t_max) but the primitive `skip`? closest_hit's walk with a fixed t_max, ending at the first hit
+// --- shadow rays -------------------------------------------------------------------------------------------------------
+// Is any primitive but `skip` hit in (0.001, t_max)? The four-wide walk of closest_hit, ending at the first hit, with
+// the leaving primitive's self-intersection rule: a sphere it leaves inward can still be hit from inside (its far
+// root), any other primitive it leaves cannot. Intervals open at both ends (src/gpu/shadow.cuh:26-120).
+template <typename Tally>
+__device__ inline bool leaf_blocks(const scene_view<float>& sc, const ray<float>& r, int32_t ref, float inverse_a,
+                                   float t_max, int from, bool inward, int skip, Tally& tl)
+{
+    constexpr float t_min = 0.001f;
+    const int kind = leaf_kind(ref);
+    const int first = leaf_first(ref);
+    const int end = first + leaf_count(ref);
+    float t;
+    bool side;
+    for (int i = first; i < end; ++i)
+    {
+        if (kind == prim_quad)
+        {
+            const int id = sc.quad_base + i;
+            if (id == from || id == skip)
+            {
+                continue;
+            }
+            tl.add(tally_quad);
+            if (hit_quad(sc.quads[i], r, t_min, t_max, t, side))
+            {
+                return true;
+            }
+        }
+        else if (kind == prim_triangle)
+        {
+            const int id = sc.triangle_base + i;
+            if (id == from || id == skip)
+            {
+                continue;
+            }
+            tl.add(tally_triangle);
+            if (hit_triangle(sc.triangles[i], r, t_min, t_max, t, side))
+            {
+                return true;
+            }
+        }
+        else
+        {
+            const bool moving = kind == prim_moving_sphere;
+            const int id = moving ? sc.moving_base + i : i;
+            const bool own = id == from;
+            if ((own && !inward) || id == skip)
+            {
+                continue;
+            }
+            tl.add(moving ? tally_moving : tally_sphere);
+            const sphere<float> s = moving ? moving_at(sc.moving_spheres[i], r.time) : load_sphere(sc.spheres, i);
+            if (hit_sphere(s, r, inverse_a, t_min, t_max, own, t, side))
+            {
+                return true;
+            }
+        }
+    }
+    return false;
+}
 template <typename Tally, typename Stack>
 __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r, float t_max, int from, bool inward,
                                 int skip, const Stack& stack, Tally& tl)
 {
-    const float t_min = 0.001f;
     if (sc.stack_depth == 0)
     {
         return false;
@@ -605,79 +645,23 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
     const vec3<float> inv{safe_inverse(r.direction.x), safe_inverse(r.direction.y), safe_inverse(r.direction.z)};
     const vec3<float> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
     const slab_order so = slab_order_of(inv);
-    int size = 0;
+    int depth = 0;
     int32_t ref = sc.root_ref;
     while (true)
     {
         if (ref < 0)
         {
-            const int kind = leaf_kind(ref);
-            const int first = leaf_first(ref);
-            const int end = first + leaf_count(ref);
-            if (kind == prim_quad)
+            if (leaf_blocks(sc, r, ref, inverse_a, t_max, from, inward, skip, tl))
             {
-                for (int i = first; i < end; ++i)
-                {
-                    const int id = sc.quad_base + i;
-                    if (id == from || id == skip)
-                    {
-                        continue;
-                    }
-                    tl.add(tally_quad);
-                    float t;
-                    bool front;
-                    if (hit_quad(sc.quads[i], r, t_min, t_max, t, front))
-                    {
-                        return true;
-                    }
-                }
-            }
-            else if (kind == prim_triangle)
-            {
-                for (int i = first; i < end; ++i)
-                {
-                    const int id = sc.triangle_base + i;
-                    if (id == from || id == skip)
-                    {
-                        continue;
-                    }
-                    tl.add(tally_triangle);
-                    float t;
-                    bool front;
-                    if (hit_triangle(sc.triangles[i], r, t_min, t_max, t, front))
-                    {
-                        return true;
-                    }
-                }
-            }
-            else
-            {
-                const bool moving = kind == prim_moving_sphere;
-                for (int i = first; i < end; ++i)
-                {
-                    const int id = moving ? sc.moving_base + i : i;
-                    const bool own = id == from;
-                    if ((own && !inward) || id == skip)
-                    {
-                        continue;
-                    }
-                    tl.add(moving ? tally_moving : tally_sphere);
-                    const sphere<float> s = moving ? moving_at(sc.moving_spheres[i], r.time) : load_sphere(sc.spheres, i);
-                    float t;
-                    bool entering;
-                    if (hit_sphere(s, r, inverse_a, t_min, t_max, own, t, entering))
-                    {
-                        return true;
-                    }
-                }
+                return true;
             }
         }
         else
         {
-            // any entered child will do: the first one entered is next, the others go on the stack
+            // every entered child must be looked at, in any order: the first goes next, the rest on the stack
             tl.add(tally_node);
             float t[wide_children];
-            child_entries(sc, ref, so, inv, os, t_min, t_max, t);
+            child_entries(sc, ref, so, inv, os, 0.001f, t_max, t);
             const int4 refs = child_refs(sc, ref);
             const int32_t child[wide_children] = {refs.x, refs.y, refs.z, refs.w};
             int32_t next = no_child;
@@ -686,14 +670,11 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
             {
                 if (t[c] < inf<float>)
                 {
-                    if (next == no_child)
+                    if (next != no_child)
                     {
-                        next = child[c];
+                        stack.store(depth++, next, 0.001f);
                     }
-                    else
-                    {
-                        stack.store(size++, child[c], t_min);
-                    }
+                    next = child[c];
                 }
             }
             if (next != no_child)
@@ -702,16 +683,15 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
                 continue;
             }
         }
-        if (size == 0)
+        if (depth == 0)
         {
             return false;
         }
-        --size;
-        ref = stack.ref_at(size);
+        ref = stack.ref_at(--depth);
     }
 }
-// The chance the segment crosses every medium without a collision: e^(-sum rho L), the expected value of the media's
-// free-flight test over the same segments (no random number drawn)
+// The chance that the segment (0.001, t_max) crosses the media without a collision: e^(-sum rho_k L_k) over the
+// lengths L_k inside each medium (the expected value of their free-flight tests; no number drawn)
 template <typename Tally>
 __device__ inline float transmittance(const scene_view<float>& sc, const ray<float>& r, float t_max, Tally& tl)
 {
@@ -719,69 +699,64 @@ __device__ inline float transmittance(const scene_view<float>& sc, const ray<flo
     {
         return 1.0f;
     }
-    const float t_min = 0.001f;
     const float inverse_a = 1.0f / length_squared(r.direction);
     const vec3<float> inv{safe_inverse(r.direction.x), safe_inverse(r.direction.y), safe_inverse(r.direction.z)};
     const vec3<float> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
-    const float length = sqrtf(length_squared(r.direction));
-    float exponent = 0.0f;
+    const float speed = sqrtf(length_squared(r.direction));
+    float optical_depth = 0.0f;
     for (int k = 0; k < sc.medium_count; ++k)
     {
-        const medium_rec<float>& m = sc.media[k];
         tl.add(tally_medium);
-        if (box_entry(m.box_min, m.box_max, inv, os, t_min, t_max) == inf<float>)
+        const medium_rec<float>& m = sc.media[k];
+        float a, b;
+        if (smoke_span(m, sc.boundary_quads, r, inverse_a, inv, os, 0.001f, t_max, a, b, tl))
         {
-            continue;
+            optical_depth -= (b - a) * speed / m.neg_inv_density;
         }
-        float line_in, line_out;
-        if (!boundary_crossings(m, sc.boundary_quads, r, inverse_a, line_in, line_out, tl))
-        {
-            continue;
-        }
-        float t_in = fmaxf(line_in, t_min);
-        const float t_out = fminf(line_out, t_max);
-        if (t_in >= t_out)
-        {
-            continue;
-        }
-        t_in = fmaxf(t_in, 0.0f);
-        exponent += (t_out - t_in) * length / m.neg_inv_density;
     }
-    return expf(exponent);
+    return expf(-optical_depth);
 }
 
-// --- the quality path (src/gpu/quality_path.cuh:27-188) ---------------------------------------------------------------
+// --- one step of a quality path ----------------------------------------------------------------------------------------
 struct quality_path
 {
     path<float> base;
-    vec3<float> radiance;
-    float bsdf_pdf;
-    bool specular;
+    vec3<float> radiance; // the light gathered so far
+    float bsdf_pdf;       // the last scattering's density, if that vertex took a light sample (else 0)
+    bool specular;        // the last vertex was a delta lobe's (or the camera): the map is seen bilinear
 };
 
-__device__ inline float largest_channel(vec3<float> c) { return fmaxf(fmaxf(c.x, c.y), c.z); }
-
-template <typename Tally>
-__device__ inline bool light_contribution(const scene_view<float>& sc, const bsdf& b, vec3<float> throughput,
-                                          vec3<float> x, pcg32& g, light_sample& ls, vec3<float>& contribution,
-                                          Tally& tl)
+__device__ inline quality_path start_quality(const camera_rec<float>& cam, bool motion, int i, int j, int pass,
+                                             pcg32& g)
 {
-    if (!sample_light(sc.quality, x, g, ls, tl))
+    return {start_path(cam, motion, i, j, pass, g), {0.0f, 0.0f, 0.0f}, 0.0f, true};
+}
+
+// The light sample's contribution at a vertex before its shadow ray: throughput f Le w / pdf, or false when it adds
+// nothing (a density of 0, no reflection toward the light, or a sum that isn't finite and positive)
+template <typename Tally>
+__device__ inline bool light_term(const scene_view<float>& sc, const scatterer& s, const vertex<float>& vx,
+                                  vec3<float> throughput, pcg32& g, light_draw& ld, vec3<float>& term, Tally& tl)
+{
+    if (!draw_light(sc.quality, vx.p, g, ld, tl))
     {
         return false;
     }
-    float p_bsdf;
-    const vec3<float> f = eval_bsdf(b, ls.wi, p_bsdf);
-    if (!(ls.pdf > 0.0f) || !(largest_channel(f) > 0.0f))
+    const ggx_eval_result f = scatter_value(s, vx, ld.wi);
+    if (!(ld.pdf > 0.0f) || !(max3(f.value) > 0.0f))
     {
         return false;
     }
-    contribution = (light_weight(sc.quality.combination, ls.pdf, p_bsdf) / ls.pdf) * (throughput * (f * ls.le));
-    const float sum = contribution.x + contribution.y + contribution.z;
+    const float w = combination{sc.quality.combination}.light_sample(ld.pdf, f.pdf) / ld.pdf;
+    term = w * (throughput * (f.value * ld.radiance));
+    const float sum = term.x + term.y + term.z;
     return sum > 0.0f && sum < inf<float>;
 }
 
-// What a hit (or a miss) does to a quality path: the part of trace_quality_ray after the closest event
+// What an event (or a miss) does to a quality path (src/gpu/quality_path.cuh:61-188): an escaped ray or an emitter
+// adds its light, weighed against the light set's chance to have drawn it; any other vertex takes a light sample
+// (when the path may still trace a ray, the lobe isn't a delta and a light exists), traces its shadow ray, then
+// scatters
 template <typename Tally, typename Stack>
 __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
                                      const hit<float>& h, pcg32& g, vec3<float>& sample, const Stack& stack,
@@ -789,59 +764,57 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
 {
     path<float>& p = q.base;
     const quality_view& qv = sc.quality;
+    const combination mix{qv.combination};
     if (h.index < 0)
     {
-        float map_pdf;
         if (qv.env_present)
         {
             tl.add(tally_env_texel, q.specular ? 4u : 1u);
         }
-        const vec3<float> le = background(qv, cam, p.r.direction, q.specular, map_pdf);
-        const float w =
-            q.bsdf_pdf > 0.0f ? bsdf_weight(qv.combination, q.bsdf_pdf, qv.light_pmf * map_pdf) : 1.0f;
+        float map_density;
+        const vec3<float> le = escaped(qv, cam, p.r.direction, q.specular, map_density);
+        const float w = q.bsdf_pdf > 0.0f ? mix.scattered_hit(q.bsdf_pdf, qv.light_pmf * map_density) : 1.0f;
         sample = q.radiance + w * (p.throughput * le);
         return true;
     }
-    const surface<float> s = surface_at(sc, p.r, h, tl);
-    const material_rec<float> m = sc.materials[s.material];
-    const bool colored = m.kind == RT_MATERIAL_LAMBERTIAN || m.kind == RT_MATERIAL_ISOTROPIC ||
-                         (m.kind == RT_MATERIAL_DIFFUSE_LIGHT && s.rec.front_face);
-    vec3<float> c{0.0f, 0.0f, 0.0f};
-    if (colored)
+    const vertex<float> vx = make_vertex(sc, p.r, h, tl);
+    const material_rec<float> m = sc.materials[vx.material];
+    // the color is read once: matte, smoke and a lamp's front face have one (a coat's base is the matte color)
+    const bool lamp = m.kind == RT_MATERIAL_DIFFUSE_LIGHT;
+    const bool colored = m.kind == RT_MATERIAL_LAMBERTIAN || m.kind == RT_MATERIAL_ISOTROPIC || (lamp && vx.front);
+    const vec3<float> color = colored ? material_color(sc, m, vx, tl) : vec3<float>{0.0f, 0.0f, 0.0f};
+    if (lamp)
     {
-        c = s.texture_index < 0 ? load3(m.albedo) : texture_value(sc, s.texture_index, s.u, s.v, s.texture_point, tl);
-    }
-    if (m.kind == RT_MATERIAL_DIFFUSE_LIGHT)
-    {
+        const vec3<float> le = color;
         float w = 1.0f;
         if (q.bsdf_pdf > 0.0f)
         {
-            const float distance = h.t * sqrtf(length_squared(p.r.direction));
             tl.add(tally_lamp);
-            w = bsdf_weight(qv.combination, q.bsdf_pdf,
-                            lamp_hit_pdf(qv, h.index, p.r.origin, unit_vector(p.r.direction), distance));
+            const float distance = h.t * sqrtf(length_squared(p.r.direction));
+            w = mix.scattered_hit(q.bsdf_pdf,
+                                  lamp_hit_density(qv, h.index, p.r.origin, unit_vector(p.r.direction), distance));
         }
-        sample = q.radiance + w * (p.throughput * c);
+        sample = q.radiance + w * (p.throughput * le);
         return true;
     }
-    const microfacet_rec* side = qv.microfacets ? &qv.microfacets[s.material] : nullptr;
-    const bsdf b = make_bsdf(m, side, c, s.rec, p.r.direction);
-    const bool nee = qv.combination != RT_COMBINE_BSDF_ONLY && p.rays_left > 1 && !is_delta(b) &&
-                     light_count(qv) > 0 && largest_channel(p.throughput) > 0.0f;
-    light_sample ls;
-    vec3<float> contribution{0.0f, 0.0f, 0.0f};
-    const bool lit = nee && light_contribution(sc, b, p.throughput, s.rec.p, g, ls, contribution, tl);
-    bsdf_sample next;
-    const bool scattered = sample_bsdf(b, m, p.r.direction, s.rec, g, next);
+    const scatterer s = make_scatterer(m, qv.microfacets ? &qv.microfacets[vx.material] : nullptr, color, vx,
+                                       p.r.direction);
+    const bool nee = qv.combination != RT_COMBINE_BSDF_ONLY && p.rays_left > 1 && !is_delta(s.kind) &&
+                     light_set_size(qv) > 0 && max3(p.throughput) > 0.0f;
+    light_draw ld;
+    vec3<float> term;
+    const bool lit = nee && light_term(sc, s, vx, p.throughput, g, ld, term, tl);
+    scatter_draw next;
+    const bool scattered = scatter_sample(s, m, p.r.direction, vx, g, next);
     if (lit)
     {
         ++shadow_rays;
-        const ray<float> shadow{s.rec.p, ls.wi, p.r.time};
-        const bool inward = dot(ls.wi, s.outward) < 0.0f;
-        const float visible = occluded(sc, shadow, ls.distance, s.from, inward, ls.skip, stack, tl)
-                                  ? 0.0f
-                                  : transmittance(sc, shadow, ls.distance, tl);
-        q.radiance = q.radiance + visible * contribution;
+        const ray<float> shadow{vx.p, ld.wi, p.r.time};
+        const bool inward = dot(ld.wi, vx.outward) < 0.0f;
+        if (!occluded(sc, shadow, ld.distance, vx.from, inward, ld.skip, stack, tl))
+        {
+            q.radiance = q.radiance + transmittance(sc, shadow, ld.distance, tl) * term;
+        }
     }
     if (!scattered)
     {
@@ -850,21 +823,15 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
     }
     p.throughput = p.throughput * next.weight;
     q.bsdf_pdf = nee ? next.pdf : 0.0f;
-    q.specular = next.delta;
-    if (qv.combination != RT_COMBINE_BSDF_ONLY && p.throughput.x == 0.0f && p.throughput.y == 0.0f &&
-        p.throughput.z == 0.0f)
+    q.specular = is_delta(s.kind);
+    if ((qv.combination != RT_COMBINE_BSDF_ONLY && !(max3(p.throughput) > 0.0f)) || --p.rays_left == 0)
     {
         sample = q.radiance;
         return true;
     }
-    if (--p.rays_left == 0)
-    {
-        sample = q.radiance;
-        return true;
-    }
-    p.r = {s.rec.p, next.wi, p.r.time};
-    p.from = s.from;
-    p.inward = dot(next.wi, s.outward) < 0.0f;
+    p.r = {vx.p, next.wi, p.r.time};
+    p.from = vx.from;
+    p.inward = dot(next.wi, vx.outward) < 0.0f;
     return false;
 }
 
@@ -875,16 +842,5 @@ __device__ inline bool trace_quality(quality_path& q, const scene_view<float>& s
 {
     const hit<float> h = closest_event(sc, q.base.r, q.base.from, q.base.inward, g, stack, tl);
     return shade_quality(q, sc, cam, h, g, sample, stack, tl, shadow_rays);
-}
-
-__device__ inline quality_path start_quality(const camera_rec<float>& cam, bool motion, int i, int j, int pass,
-                                             pcg32& g)
-{
-    quality_path q;
-    q.base = start_path(cam, motion, i, j, pass, g);
-    q.radiance = {0.0f, 0.0f, 0.0f};
-    q.bsdf_pdf = 0.0f;
-    q.specular = true;
-    return q;
 }
 }

@@ -21,7 +21,6 @@
 // captured in a hipGraph. When the items are handed out and few slots are left, the drain kernel finishes them a
 // thread each.
 #include "kernel_api.hpp"
-#include "path_tracer.hpp"
 #include "quality.hpp"
 #include "wavefront_api.hpp"
 

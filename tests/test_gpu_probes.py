@@ -6,6 +6,8 @@
   the hand-built tie cases' answers; the product's own walk gives the twin's scene-order answers on the same cases.
 - The per-sample generator (rt_sample_numbers): raw words, float and double draws equal the twin's for seeds of
   several rounds (seed + r 2^32), pixels and passes; the draws' bit layout is the reference's (random.cuh:76-90).
+- The device's direction samplers (the shading's own): a uniform unit vector, the concentric map's uniform disk
+  point and the cosine lobe about a normal, against their distributions' closed-form moments and CDFs.
 """
 import zlib
 
@@ -97,3 +99,34 @@ def test_sample_numbers_bit_exact(rt):
     # neighbouring pixels and passes get unrelated streams
     w, _, _, _ = rt.sample_numbers(9, np.arange(64, dtype=np.uint64), np.zeros(64, np.uint64), 1)
     assert len(np.unique(w)) == 64
+
+
+def test_direction_samplers_distributions(rt):
+    """rt_sample_numbers' directions over 2^18 items: the uniform unit vector (|v| = 1, E[z] = 0, E[z^2] = 1/3), the
+    uniform disk point (r <= 1, z = 0, E[x] = 0, E[r^2] = 1/2, P(r^2 <= t) = t) and the cosine lobe about +z (|v| = 1,
+    z > 0, E[z] = 2/3, E[z^2] = 1/2, P(z >= t) = 1 - t^2). A sampler with a wrong density (a uniform hemisphere for
+    the cosine lobe, r uniform for the disk) misses these by hundreds of standard errors; each check allows 5."""
+    n = 1 << 18
+    _, _, _, dirs = rt.sample_numbers(77, np.arange(n, dtype=np.uint64), np.full(n, 3, np.uint64), 1)
+    unit, disk, cosine = dirs[:, 0].astype(np.float64), dirs[:, 1].astype(np.float64), dirs[:, 2].astype(np.float64)
+
+    def close(values, expect, label):
+        se = values.std() / np.sqrt(values.size)
+        assert abs(values.mean() - expect) < 5 * se + 1e-9, f"{label}: {values.mean():.6f} vs {expect:.6f} (se {se:.2e})"
+
+    assert np.allclose(np.linalg.norm(unit, axis=1), 1.0, atol=1e-5)
+    close(unit[:, 2], 0.0, "uniform E[z]")
+    close(unit[:, 2] ** 2, 1 / 3, "uniform E[z^2]")
+    close(unit[:, 0] * unit[:, 1], 0.0, "uniform E[xy]")
+    r2 = disk[:, 0] ** 2 + disk[:, 1] ** 2
+    assert np.all(r2 <= 1.0 + 1e-6) and np.all(disk[:, 2] == 0)
+    close(disk[:, 0], 0.0, "disk E[x]")
+    close(r2, 0.5, "disk E[r^2]")
+    for t in (0.1, 0.25, 0.5, 0.9):
+        close((r2 <= t).astype(np.float64), t, f"disk P(r^2 <= {t})")
+    assert np.allclose(np.linalg.norm(cosine, axis=1), 1.0, atol=1e-5) and np.all(cosine[:, 2] > 0)
+    close(cosine[:, 2], 2 / 3, "cosine E[z]")
+    close(cosine[:, 2] ** 2, 0.5, "cosine E[z^2]")
+    close(cosine[:, 0], 0.0, "cosine E[x]")
+    for t in (0.2, 0.5, 0.8, 0.95):
+        close((cosine[:, 2] >= t).astype(np.float64), 1 - t * t, f"cosine P(z >= {t})")
