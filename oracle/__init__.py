@@ -43,7 +43,7 @@ def _load() -> C.CDLL:
     P = C.POINTER
     lib.oracle_last_error.restype = C.c_char_p
     lib.oracle_render.restype = C.c_int
-    lib.oracle_render.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
+    lib.oracle_render.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
                                   C.c_uint32, C.c_int, C.c_int, P(C.c_double), C.c_int64, P(C.c_int32),
                                   P(C.c_uint64)]
     lib.oracle_flatten.restype = C.c_int64
@@ -96,19 +96,22 @@ def _ptr(a, t):
 
 
 def render(name: str, width: int = 0, spp: int = 0, seed: int = 0, sampling: int = -1, max_depth: int = 0,
-           threads: int = 0, rows: tuple | None = None, earth_width: int = 0, earth_seed: int = 0):
+           threads: int = 0, rows: tuple | None = None, earth_width: int = 0, earth_seed: int = 0, height: int = 0):
     """The reference's CPU render (double, mt19937_64 per row): (image (h, w, 3), rays). rows=(begin, end) renders
-    only those rows, each exactly as in the whole image."""
+    only those rows, each exactly as in the whole image. height > 0: that many rows, the view widened or narrowed
+    (the reference's --height)."""
     L = lib()
     dims = np.zeros(3, np.int32)
-    if L.oracle_render(name.encode(), width, spp, sampling, max_depth, seed, threads, earth_width, earth_seed, 0, 0,
+    if L.oracle_render(name.encode(), width, height, spp, sampling, max_depth, seed, threads, earth_width, earth_seed,
+                       0, 0,
                        None, 0, _ptr(dims, C.c_int32), None):
         raise RuntimeError(L.oracle_last_error().decode())
     w, h = int(dims[0]), int(dims[1])
     r0, r1 = rows if rows else (0, h)
     out = np.zeros((r1 - r0, w, 3), np.float64)
     rays = C.c_uint64()
-    if L.oracle_render(name.encode(), width, spp, sampling, max_depth, seed, threads, earth_width, earth_seed, r0, r1,
+    if L.oracle_render(name.encode(), width, height, spp, sampling, max_depth, seed, threads, earth_width, earth_seed,
+                       r0, r1,
                        _ptr(out, C.c_double), out.size, _ptr(dims, C.c_int32), C.byref(rays)):
         raise RuntimeError(L.oracle_last_error().decode())
     return out, rays.value
@@ -118,7 +121,7 @@ def frame(name: str, width: int = 0, spp: int = 0):
     """(width, height, sqrt_spp) of a scene with the overrides"""
     L = lib()
     dims = np.zeros(3, np.int32)
-    if L.oracle_render(name.encode(), width, spp, -1, 0, 0, 1, 0, 0, 0, 0, None, 0, _ptr(dims, C.c_int32), None):
+    if L.oracle_render(name.encode(), width, 0, spp, -1, 0, 0, 1, 0, 0, 0, 0, None, 0, _ptr(dims, C.c_int32), None):
         raise RuntimeError(L.oracle_last_error().decode())
     return tuple(int(x) for x in dims)
 

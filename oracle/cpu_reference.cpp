@@ -199,7 +199,8 @@ const char* oracle_last_error() { return last_error.c_str(); }
 
 // Renders rows [row_begin, row_end) of a scene (row_end <= 0: to the last row) on `threads` threads (0: all cores).
 // dims receives width, height, sqrt_spp. Returns 0, or -1 with oracle_last_error().
-int oracle_render(const char* name, int width, int spp, int sampling, int max_depth, std::uint32_t seed, int threads,
+int oracle_render(const char* name, int width, int height, int spp, int sampling, int max_depth, std::uint32_t seed,
+                  int threads,
                   int earth_width, std::uint32_t earth_seed, int row_begin, int row_end, double* out,
                   std::int64_t capacity, std::int32_t* dims, std::uint64_t* rays)
 {
@@ -213,6 +214,11 @@ int oracle_render(const char* name, int width, int spp, int sampling, int max_de
         if (width > 0)
         {
             s.camera.image_width = width;
+        }
+        if (height > 0)
+        {
+            // the reference's --height: the aspect ratio that gives the width that many rows (view_options.ixx:101)
+            s.camera.aspect_ratio = static_cast<double>(s.camera.image_width) / height;
         }
         if (spp > 0)
         {
@@ -463,7 +469,7 @@ int main(int argc, char** argv)
             name = a;
     }
     std::int32_t dims[3];
-    if (oracle_render(name.c_str(), width, spp, sampling, 0, seed, threads, 0, 0, 0, 0, nullptr, 0, dims, nullptr))
+    if (oracle_render(name.c_str(), width, 0, spp, sampling, 0, seed, threads, 0, 0, 0, 0, nullptr, 0, dims, nullptr))
     {
         std::fprintf(stderr, "Error: %s\n", oracle_last_error());
         return 2;
@@ -471,7 +477,7 @@ int main(int argc, char** argv)
     std::vector<double> pixels(3ull * dims[0] * dims[1]);
     std::uint64_t rays = 0;
     const auto start = std::chrono::steady_clock::now();
-    oracle_render(name.c_str(), width, spp, sampling, 0, seed, threads, 0, 0, 0, 0, pixels.data(),
+    oracle_render(name.c_str(), width, 0, spp, sampling, 0, seed, threads, 0, 0, 0, 0, pixels.data(),
                   static_cast<std::int64_t>(pixels.size()), dims, &rays);
     const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     double mean = 0;

@@ -18,16 +18,18 @@ from tests.rays import scene_rays
 pytestmark = pytest.mark.gpu
 
 
-def _compare_with_rerun(name, width, spp, sampling=None, precision="float32", integrator="auto", depth=0):
+def _compare_with_rerun(name, width, spp, sampling=None, precision="float32", integrator="auto", depth=0,
+                        earth_width=512):
     import raytracing_amd as rt
 
     samp = -1 if sampling is None else rt.SAMPLING[sampling]
     verdicts = []
     for cpu_seed, gpu_seed in ((0, 1), (2, 3)):
-        scene = rt.build_scene(name, width=width, samples=spp, sampling=sampling, max_depth=depth, earth_width=512)
+        scene = rt.build_scene(name, width=width, samples=spp, sampling=sampling, max_depth=depth,
+                               earth_width=earth_width)
         gpu = rt.render(scene, seed=gpu_seed, precision=precision, integrator=integrator).pixels
         cpu, _ = oracle.render(name, width=width, spp=spp, seed=cpu_seed, sampling=samp, max_depth=depth,
-                               earth_width=512)
+                               earth_width=earth_width)
         v = image_compare.compare(cpu, gpu)
         verdicts.append(v)
         print(f"{name} {width}w {spp}spp {precision} {integrator}: CPU {cpu.mean():.6f} GPU {gpu.mean():.6f}: "
@@ -98,6 +100,12 @@ def test_weekend_config0_vs_cpu(rt):
 ])
 def test_scene_vs_cpu(rt, name, width, spp, sampling):
     _compare_with_rerun(name, width, spp, sampling)
+
+
+def test_final_benchmark_texture_vs_cpu(rt):
+    """The final scene with the benchmark's own 2048x1024 noise texture (the other tests use a 512-wide one), at a
+    reduced resolution: ImageCompare against the CPU"""
+    _compare_with_rerun("final", 160, 64, earth_width=2048)
 
 
 def test_fp64_vs_cpu(rt):

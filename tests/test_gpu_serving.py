@@ -135,3 +135,35 @@ def test_milestones_are_powers_of_two(rt):
         seen.append(server.status(rid).milestone)
     assert seen == [1, 2, 4, 8, 16, 32, 40]
     server.close()
+
+
+def test_benchmark_mix_same_alone_and_in_mix(rt):
+    """Requests drawn from the serving benchmark's own mix (its four scenes resident, each scene twice, each request
+    one of the four resolutions 256x256 to 1920x1080, a log-uniform spp in [16, 1024] and its own seed; seeded, with
+    the paths of one request capped at 2^28 so the test stays short): every result in the mix equals its alone-session
+    bytes"""
+    scenes = ["weekend", "glass", "final", "triangles"]
+    sizes = [(256, 256), (512, 512), (1080, 1080), (1920, 1080)]
+    rng = np.random.default_rng(44)
+    picks = []
+    for k in range(8):
+        name, (w, h) = scenes[k % 4], sizes[rng.integers(4)]  # every scene twice
+        spp = int(round(np.exp(rng.uniform(np.log(16), np.log(1024)))))
+        spp = max(1, min(spp, (1 << 28) // (w * h)))
+        picks.append((name, w, h, spp, 500 + k))
+    print(picks)
+    server = rt.Server(max_active=4, passes_per_step=8)
+    resident = {name: server.add_scene(rt.build_scene(name, width=256, height=256, samples=1)) for name in scenes}
+    views = {(name, w, h): rt.build_scene(name, width=w, height=h, samples=1) for name, w, h, _, _ in picks}
+    ids = [server.submit_view(resident[name], views[(name, w, h)].desc.camera, seed, spp)
+           for name, w, h, spp, seed in picks]
+    _run(server, ids)
+    for rid, (name, w, h, spp, seed) in zip(ids, picks):
+        st = server.status(rid)
+        assert st.state == 2 and st.passes_done == spp, (name, w, h, spp, st.state, st.passes_done)
+        with rt.Session(views[(name, w, h)], seed=seed) as s:
+            s.render_to(spp)
+            alone = s.pixels()
+        mixed = server.pixels(rid)
+        assert np.array_equal(alone.view(np.uint64), mixed.view(np.uint64)), (name, w, h, spp)
+    server.close()
