@@ -3,8 +3,15 @@
 // prints, and the image as PFM (linear floats) or PNG (the books' bytes).
 //
 //   rt_gpu [scene] [-w N] [--height N] [-s N] [--strata K] [-p uniform|material|lights|mixture|nee|mis]
-//          [--seed N] [--fp64] [--integrator auto|megakernel|wavefront] [--launch-ms N] [-o FILE.pfm|FILE.png]...
+//          [--seed N] [--fp64] [--integrator auto|megakernel|wavefront|plain] [--launch-ms N] [-o FILE.pfm|FILE.png]...
+//          [--tonemap books|none|agx|neutral] [--exposure EV]
+// Exit code: 0 rendered and saved, 2 an error (an option out of range or malformed, a scene the library refuses, a
+// file that couldn't be written).
+#include <cerrno>
 #include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -13,6 +20,7 @@
 #include <vector>
 
 #include "../../../include/raytracing_amd.h"
+#include "pfm.hpp"
 
 namespace
 {
@@ -103,33 +111,36 @@ bool save_png(const std::string& path, int w, int h, const std::vector<uint8_t>&
     return static_cast<bool>(f);
 }
 
-// PFM: "PF", the size, -1.0 (little-endian), then rows from the bottom (src/image.ixx:150-187)
-bool save_pfm(const std::string& path, int w, int h, const std::vector<double>& pixels)
-{
-    std::ofstream f(path, std::ios::binary);
-    if (!f)
-    {
-        return false;
-    }
-    f << "PF\n" << w << " " << h << "\n-1.0\n";
-    for (int y = h - 1; y >= 0; --y)
-    {
-        for (int x = 0; x < w; ++x)
-        {
-            for (int c = 0; c < 3; ++c)
-            {
-                const float v = static_cast<float>(pixels[3 * (static_cast<std::size_t>(y) * w + x) + c]);
-                f.write(reinterpret_cast<const char*>(&v), 4);
-            }
-        }
-    }
-    return static_cast<bool>(f);
-}
-
 int fail(const char* what)
 {
     std::printf("Error: %s: %s\n", what, rt_last_error());
     return 2;
+}
+
+// A whole number of an option, or the exit with a message (src/main_gpu.cpp refuses malformed numbers alike)
+long long whole_number(const std::string& option, const std::string& text, long long lo, long long hi)
+{
+    char* end = nullptr;
+    errno = 0;
+    const long long v = std::strtoll(text.c_str(), &end, 10);
+    if (text.empty() || *end != '\0' || errno != 0 || v < lo || v > hi)
+    {
+        std::printf("Error: %s: expected an integer from %lld to %lld, not '%s'\n", option.c_str(), lo, hi,
+                    text.c_str());
+        std::exit(2);
+    }
+    return v;
+}
+double real_number(const std::string& option, const std::string& text)
+{
+    char* end = nullptr;
+    const double v = std::strtod(text.c_str(), &end);
+    if (text.empty() || *end != '\0' || !std::isfinite(v))
+    {
+        std::printf("Error: %s: expected a number, not '%s'\n", option.c_str(), text.c_str());
+        std::exit(2);
+    }
+    return v;
 }
 }
 
@@ -156,7 +167,7 @@ int main(int argc, char** argv)
         if (a == "-h" || a == "--help")
         {
             std::printf("Usage: rt_gpu [scene] [-w N] [--height N] [-s N] [--strata K] [-p STRATEGY] [--seed N]\n"
-                        "              [--fp64] [--integrator auto|megakernel|wavefront] [--launch-ms N] [-o FILE]\n"
+                        "              [--fp64] [--integrator auto|megakernel|wavefront|plain] [--launch-ms N] [-o FILE]\n"
                         "              [--tonemap books|none|agx|neutral] [--exposure EV]\n"
                         "Scenes: %s\n"
                         "Or a glTF 2.0 model, FILE.glb or FILE.gltf, rendered in a studio\n",
@@ -164,25 +175,25 @@ int main(int argc, char** argv)
             return 0;
         }
         if (a == "-w" || a == "--width")
-            ov.image_width = std::stoi(value());
+            ov.image_width = static_cast<int32_t>(whole_number(a, value(), 1, 65536));
         else if (a == "--height")
-            ov.image_height = std::stoi(value());
+            ov.image_height = static_cast<int32_t>(whole_number(a, value(), 1, 65536));
         else if (a == "-s" || a == "--samples")
-            ov.samples_per_pixel = std::stoi(value());
+            ov.samples_per_pixel = static_cast<int32_t>(whole_number(a, value(), 1, 1 << 30));
         else if (a == "--strata")
-            ov.strata = std::stoi(value());
+            ov.strata = static_cast<int32_t>(whole_number(a, value(), 1, 46340));
         else if (a == "--depth")
-            ov.max_depth = std::stoi(value());
+            ov.max_depth = static_cast<int32_t>(whole_number(a, value(), 1, 1 << 20));
         else if (a == "--seed")
-            so.seed = std::stoull(value());
+            so.seed = static_cast<uint64_t>(whole_number(a, value(), 0, INT64_MAX));
         else if (a == "--fp64")
             so.precision = RT_FLOAT64;
         else if (a == "--launch-ms")
-            so.launch_seconds = std::stoi(value()) / 1000.0;
+            so.launch_seconds = static_cast<double>(whole_number(a, value(), 1, 500)) / 1000.0;
         else if (a == "-o" || a == "--output")
             outputs.push_back(value());
         else if (a == "--exposure")
-            exposure = std::stod(value());
+            exposure = real_number(a, value());
         else if (a == "--tonemap")
         {
             const std::string v = value();
@@ -204,9 +215,20 @@ int main(int argc, char** argv)
         else if (a == "--integrator")
         {
             const std::string v = value();
-            so.integrator = v == "megakernel" ? RT_INTEGRATOR_MEGAKERNEL
-                            : v == "wavefront" ? RT_INTEGRATOR_WAVEFRONT
-                                               : RT_INTEGRATOR_AUTO;
+            const char* names[] = {"auto", "megakernel", "wavefront", "plain"};
+            so.integrator = -1;
+            for (int k = 0; k < 4; ++k)
+            {
+                if (v == names[k])
+                {
+                    so.integrator = k;
+                }
+            }
+            if (so.integrator < 0)
+            {
+                std::printf("Error: unknown integrator '%s'\n", v.c_str());
+                return 2;
+            }
         }
         else if (a == "-p" || a == "--sampling")
         {
@@ -295,12 +317,13 @@ int main(int argc, char** argv)
                 static_cast<double>(rays) / paths, static_cast<double>(rays) / kernel_seconds / 1e9);
     std::printf("Paths: %.1f M per second\nMean radiance: %.6f\n", paths / kernel_seconds / 1e6, mean);
     std::vector<uint8_t> rgba(4ull * w * hgt);
+    bool saved = true;
     for (const std::string& out : outputs)
     {
         bool ok = false;
         if (out.size() > 4 && (out.substr(out.size() - 4) == ".pfm" || out.substr(out.size() - 4) == ".PFM"))
         {
-            ok = save_pfm(out, w, hgt, pixels);
+            ok = rt::cli::save_pfm(out, w, hgt, pixels);
         }
         else
         {
@@ -308,8 +331,9 @@ int main(int argc, char** argv)
                  save_png(out, w, hgt, rgba);
         }
         std::printf("%s: %s\n", ok ? "Saved" : "Error: could not save", out.c_str());
+        saved = saved && ok;
     }
     rt_session_destroy(session);
     rt_scene_free(h);
-    return 0;
+    return saved ? 0 : 2;
 }

@@ -29,6 +29,8 @@
 #include <string>
 #include <vector>
 
+#include "pfm.hpp"
+
 namespace
 {
 constexpr double rounding_margin = 1e-6;                     // of a pixel's level: float rounding, not noise
@@ -36,60 +38,8 @@ constexpr double sign_threshold = 1000 * rounding_margin;    // the sign test co
 constexpr double false_alarm = 1.0 / 400;                    // each check's false-alarm rate
 constexpr double infinity = std::numeric_limits<double>::infinity();
 
-struct image
-{
-    int width = 0;
-    int height = 0;
-    std::vector<double> rgb; // rows from the top
-};
-
-// PF, the width, the height, a scale whose sign is the byte order (negative: little-endian), rows from the bottom
-bool load_pfm(const std::string& path, image& out, std::string& error)
-{
-    std::ifstream f(path, std::ios::binary);
-    if (!f)
-    {
-        error = "can't open " + path;
-        return false;
-    }
-    std::string magic;
-    double scale = 0;
-    f >> magic >> out.width >> out.height >> scale;
-    if (!f || magic != "PF" || out.width <= 0 || out.height <= 0 || out.width > 1 << 16 || out.height > 1 << 16 ||
-        scale == 0)
-    {
-        error = path + ": not a color PFM file";
-        return false;
-    }
-    f.get(); // the single white-space character after the scale
-    const std::size_t n = static_cast<std::size_t>(out.width) * static_cast<std::size_t>(out.height) * 3;
-    std::vector<uint32_t> words(n);
-    f.read(reinterpret_cast<char*>(words.data()), static_cast<std::streamsize>(n * 4));
-    if (static_cast<std::size_t>(f.gcount()) != n * 4)
-    {
-        error = path + ": the file is shorter than its size says";
-        return false;
-    }
-    const bool little = scale < 0;
-    out.rgb.resize(n);
-    const std::size_t row = static_cast<std::size_t>(out.width) * 3;
-    for (int y = 0; y < out.height; ++y)
-    {
-        const std::size_t from = static_cast<std::size_t>(out.height - 1 - y) * row;
-        for (std::size_t k = 0; k < row; ++k)
-        {
-            uint32_t w = words[from + k];
-            if (!little)
-            {
-                w = (w >> 24) | ((w >> 8) & 0xff00u) | ((w << 8) & 0xff0000u) | (w << 24);
-            }
-            float v;
-            std::memcpy(&v, &w, 4);
-            out.rgb[static_cast<std::size_t>(y) * row + k] = v;
-        }
-    }
-    return true;
-}
+using image = rt::cli::pfm_image;
+using rt::cli::load_pfm;
 
 // Welford's running mean and sum of squared deviations (one pass, no cancellation when the spread is tiny)
 struct running

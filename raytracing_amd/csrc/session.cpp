@@ -98,10 +98,18 @@ std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t 
     {
         throw std::invalid_argument("unknown precision");
     }
-    if (precision != RT_FLOAT32 && (scene.mesh_count != 0 || scene.microfacet_count != 0 ||
-                                    scene.environment != nullptr || scene.combination != RT_COMBINE_BSDF_ONLY))
+    const bool quality = scene.mesh_count != 0 || scene.microfacet_count != 0 || scene.environment != nullptr ||
+                         scene.combination != RT_COMBINE_BSDF_ONLY;
+    if (precision != RT_FLOAT32 && quality)
     {
         throw std::invalid_argument("triangles, GGX, environment maps and light samples render in float32 only");
+    }
+    // the quality integrator draws its own light samples: the lights list's strategies are the books' (the reference's
+    // quality kernel refuses them too, tests/routing_tests.cu:53-67)
+    if (quality && scene.camera.sampling != RT_SAMPLING_MATERIAL)
+    {
+        throw std::invalid_argument("triangles, GGX, environment maps and light samples take -p material, nee or mis, "
+                                    "not the lights list's uniform, lights or mixture");
     }
     return std::make_shared<const shared_scene>(scene, precision, tree);
 }

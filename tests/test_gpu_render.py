@@ -166,6 +166,24 @@ def test_session_errors(rt):
     s.close()
 
 
+def test_quality_scene_refusals(rt):
+    """What the quality integrator refuses (tests/routing_tests.cu:53-67): float64, and the lights list's strategies
+    (uniform, lights, mixture) on a scene that needs it (GGX here); the books' kernels take both"""
+    ggx = rt.build_scene("cornell-ggx", width=32, samples=4)
+    with pytest.raises(ValueError, match="float32 only"):
+        rt.Session(ggx, precision="float64")
+    ggx.desc.camera.sampling = rt.SAMPLING["uniform"]
+    with pytest.raises(ValueError, match="lights list"):
+        rt.Session(ggx)
+    for strategy in ("lights", "mixture"):  # without a lights list the scene itself is refused first
+        ggx.desc.camera.sampling = rt.SAMPLING[strategy]
+        with pytest.raises(RuntimeError, match="need a lights list"):
+            rt.Session(ggx)
+    books = rt.build_scene("cornell", width=32, samples=4, sampling="mixture")
+    with rt.Session(books, precision="float64") as s:
+        s.render_to(1)
+
+
 def test_display_bytes(rt):
     scene = rt.build_scene("weekend", width=80, samples=4)
     with rt.Session(scene) as s:
