@@ -291,8 +291,10 @@ def main():
                          "--integrator megakernel")
     ap.add_argument("--block-size", type=int, default=0, choices=[0, 64, 128, 192, 256],
                     help="the megakernel's threads per block (0: the tuned default), with --integrator megakernel")
-    ap.add_argument("--tree", default="default", choices=["default", "median", "sah"],
-                    help="the BVH's splits: the library's default, the reference's object median, or the SAH")
+    ap.add_argument("--tree", default="default", choices=["default", "median", "sah", "two_level"],
+                    help="the BVH's splits: the library's default (SAH, one level), the reference's object median, "
+                         "the SAH, or the SAH in two levels (meshes placed several times walked through their "
+                         "instances)")
     ap.add_argument("--serving", action="store_true", help="config 4: the serving loop under Poisson request streams")
     ap.add_argument("--rates", default="1,4,16,64")
     ap.add_argument("--requests", type=int, default=16, help="requests per rate (serving)")

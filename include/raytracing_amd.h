@@ -347,15 +347,21 @@ typedef struct rt_session_options
     int32_t tree;           /* rt_tree: how the session's BVH is split */
 } rt_session_options;
 
-/* The BVH's splits. Both trees hold every primitive and find the same closest hit; they differ in how many boxes a
- * ray tests on the way (of two primitives hit at exactly the same distance, the one of lower index in the scene is
- * kept with either). */
+/* The BVH's splits. Every tree holds every primitive and finds the same closest hit on scenes without instanced
+ * meshes (of two primitives hit at exactly the same distance, the one of lower index in the scene is kept with
+ * either); an instanced mesh's hits are found in its own coordinates, so their distances may differ from the baked
+ * tree's in the last bits and their ids are the instance's own (each instance a range of ids). */
 enum rt_tree
 {
-    RT_TREE_DEFAULT = 0, /* the surface-area heuristic */
-    RT_TREE_MEDIAN = 1,  /* the reference's object-median splits on the longest axis, with its leaf sizes
-                            (src/gpu/bvh.cuh:382-444); scenes with triangles take the surface-area tree */
-    RT_TREE_SAH = 2,     /* binned surface-area-heuristic splits, leaves of one kind and at most 8 primitives */
+    RT_TREE_DEFAULT = 0,   /* binned surface-area-heuristic splits, leaves of one kind and at most 8 primitives, in
+                              one level: every instance's triangles baked into world coordinates */
+    RT_TREE_MEDIAN = 1,    /* the reference's object-median splits on the longest axis, with its leaf sizes
+                              (src/gpu/bvh.cuh:382-444); scenes with triangles take the default tree */
+    RT_TREE_SAH = 2,       /* the default tree */
+    RT_TREE_TWO_LEVEL = 3, /* the default's splits in two levels: a mesh placed by several instances gets a tree of
+                              its own, once, which each instance's leaf enters with the ray moved into the mesh's
+                              coordinates (the reference's mesh_set instances, src/gpu/renderer.h:305-376); it holds
+                              each such mesh once, and the kernels that can enter an instance are kernels of their own */
 };
 /* RT_INTEGRATOR_AUTO (float32): the session's autotuner renders its first full passes with each candidate (the
  * megakernel at each register budget, one wave a block, and the wavefront), a few passes each; the three fastest render
@@ -503,17 +509,17 @@ int rt_environment_tables(const rt_environment_desc* environment, float* texels,
  * [0, passes) of the view's first round, into scratch sums (the view's image and counters don't change). events[k] counts
  * reads of one kind of record (in order: wide BVH nodes, spheres, moving spheres, quads, media, media boundary quads,
  * hits, sphere frames, textures, texels, Perlin gradients, the books' lights, triangles, lamps of the light set,
- * environment texels, alias bins) and event_bytes[k] is that record's size; bytes_per_path is their weighted sum per
- * path, plus the double sums' read and write of each pixel once per launch. This is the algorithmic traffic the
- * roofline is measured with. No counterpart in the reference. */
-#define RT_TRAFFIC_EVENTS 16
+ * environment texels, alias bins, mesh instances) and event_bytes[k] is that record's size; bytes_per_path is their
+ * weighted sum per path, plus the double sums' read and write of each pixel once per launch. This is the algorithmic
+ * traffic the roofline is measured with. No counterpart in the reference. */
+#define RT_TRAFFIC_EVENTS 17
 typedef struct rt_traffic
 {
     uint64_t paths;
     uint64_t rays;
     uint64_t shadow_rays;
-    uint64_t events[16];
-    double event_bytes[16];
+    uint64_t events[17];
+    double event_bytes[17];
     double bytes_per_path;
 } rt_traffic;
 

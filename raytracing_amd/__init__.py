@@ -152,12 +152,12 @@ class RenderStats(C.Structure):
 
 
 class Traffic(C.Structure):
-    _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("events", C.c_uint64 * 16),
-                ("event_bytes", C.c_double * 16), ("bytes_per_path", C.c_double)]
+    _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("events", C.c_uint64 * 17),
+                ("event_bytes", C.c_double * 17), ("bytes_per_path", C.c_double)]
 
 
 TRAFFIC_EVENTS = ["node", "sphere", "moving_sphere", "quad", "medium", "boundary_quad", "hit", "frame", "texture",
-                  "texel", "perlin", "light", "triangle", "lamp", "env_texel", "alias_bin"]
+                  "texel", "perlin", "light", "triangle", "lamp", "env_texel", "alias_bin", "instance"]
 
 
 class ServerOptions(C.Structure):
@@ -242,7 +242,7 @@ RT_OK, RT_ERROR_RUNTIME, RT_ERROR_INVALID_ARGUMENT, RT_ERROR_LOGIC = 0, 1, 2, 3
 SAMPLING = {"uniform": 0, "material": 1, "lights": 2, "mixture": 3, "nee": 4, "mis": 5}
 MATERIAL_KINDS = {"lambertian": 0, "metal": 1, "dielectric": 2, "diffuse_light": 3, "isotropic": 4}
 INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2, "plain": 3}
-TREES = {"default": 0, "median": 1, "sah": 2}
+TREES = {"default": 0, "median": 1, "sah": 2, "two_level": 3}
 TONE_MAPS = {"books": 0, "none": 1, "agx": 2, "neutral": 3}
 
 

@@ -64,7 +64,7 @@ template <typename Real>
 class builder
 {
 public:
-    builder(const rt_scene& scene, int max_leaf, bool sah) : max_leaf_(max_leaf), sah_(sah)
+    builder(const rt_scene& scene, int max_leaf, bool sah, bool two_level) : max_leaf_(max_leaf), sah_(sah)
     {
         const auto n_spheres = static_cast<std::size_t>(scene.sphere_count);
         for (std::size_t i = 0; i < n_spheres; ++i)
@@ -116,26 +116,76 @@ public:
             quads_.push_back(make_quad_rec<Real>(scene.quads[i]));
         }
         out_.triangle_base = static_cast<int32_t>(inputs_.size());
-        for (const world_triangle& w : world_triangles(scene))
+        for (const world_triangle& w : world_triangles(scene, two_level))
         {
             inputs_.push_back({dev::prim_triangle, static_cast<int>(triangles_.size()), -1, w.material, {},
                                triangle_box(w)});
             triangles_.push_back(make_triangle_rec(w));
         }
+        out_.instanced_base = static_cast<int32_t>(inputs_.size());
+        // meshes placed more than once: a tree of their own, once, and one leaf per instance in the scene's tree
+        const std::vector<bool> instanced = instanced_meshes(scene, two_level);
+        meshes_.resize(static_cast<std::size_t>(scene.mesh_count));
+        for (int64_t m = 0; m < scene.mesh_count; ++m)
+        {
+            if (instanced[static_cast<std::size_t>(m)])
+            {
+                meshes_[static_cast<std::size_t>(m)].tree = builder(scene.meshes[m]).build();
+            }
+        }
+        for (int64_t k = 0; k < scene.instance_count; ++k)
+        {
+            const rt_mesh_instance_desc& d = scene.instances[k];
+            if (!instanced[static_cast<std::size_t>(d.mesh)])
+            {
+                continue;
+            }
+            const bvh_arrays<Real>& tree = meshes_[static_cast<std::size_t>(d.mesh)].tree;
+            if (tree.nodes.empty())
+            {
+                continue; // a mesh without triangles
+            }
+            inputs_.push_back({dev::prim_instance, static_cast<int>(instance_descs_.size()), static_cast<int>(k),
+                               scene.meshes[d.mesh].material, {}, instance_box(tree, d.object_to_world)});
+            instance_descs_.push_back(d);
+        }
         out_.sphere_ids.resize(n_spheres);
         out_.quad_ids.resize(static_cast<std::size_t>(scene.quad_count));
     }
 
+    // A mesh's own tree: its triangles in its own coordinates, in a surface-area tree
+    explicit builder(const rt_mesh_desc& mesh) : max_leaf_(sah_leaf), sah_(true)
+    {
+        for (int64_t k = 0; k < mesh.triangle_count; ++k)
+        {
+            world_triangle w{};
+            w.material = mesh.material;
+            for (int c = 0; c < 3; ++c)
+            {
+                const float* p = mesh.positions + 3 * static_cast<std::size_t>(mesh.indices[3 * k + c]);
+                for (int a = 0; a < 3; ++a)
+                {
+                    w.p[c][a] = p[a];
+                }
+            }
+            inputs_.push_back({dev::prim_triangle, static_cast<int>(triangles_.size()), -1, w.material, {},
+                               triangle_box(w)});
+            triangles_.push_back(make_triangle_rec(w));
+        }
+        out_.instanced_base = static_cast<int32_t>(inputs_.size());
+    }
+
     bvh_arrays<Real> build() &&
     {
-        out_.prim_materials.resize(inputs_.size());
-        out_.prim_ranks.resize(inputs_.size());
+        out_.prim_materials.resize(static_cast<std::size_t>(out_.instanced_base));
+        out_.prim_ranks.resize(static_cast<std::size_t>(out_.instanced_base));
         out_.sphere_frames.resize(static_cast<std::size_t>(out_.quad_base));
         std::vector<int> order(inputs_.size());
         std::iota(order.begin(), order.end(), 0);
+        const bool meshes = !triangles_.empty() || !instance_descs_.empty();
         if (!order.empty())
         {
-            if (triangles_.empty() && !sah_)
+            if (!meshes && !sah_)
             {
                 add_subtree(order, 1);
             }
@@ -153,7 +203,9 @@ public:
                 add_subtree_sah(order, 1);
             }
             pack_wide();
+            append_meshes();
         }
+        out_.media_base = out_.instanced_base + next_instanced_id_;
         return std::move(out_);
     }
 
@@ -170,6 +222,16 @@ private:
     int max_leaf_;
     bool sah_;
     std::vector<input> inputs_;
+    struct mesh_tree
+    {
+        bvh_arrays<Real> tree; // empty unless the mesh is instanced
+        int32_t root_ref = -1; // in the scene's node array, once appended
+        int32_t first_triangle = 0;
+    };
+    std::vector<mesh_tree> meshes_;
+    std::vector<rt_mesh_instance_desc> instance_descs_; // of the instanced meshes, by scene order
+    std::vector<int32_t> instance_mesh_;                // of out_.instances, by leaf order
+    int32_t next_instanced_id_ = 0;
     std::vector<dev::sphere_rec<Real>> spheres_;
     std::vector<dev::moving_sphere_rec<Real>> moving_;
     std::vector<dev::quad_rec<Real>> quads_;
@@ -182,7 +244,67 @@ private:
         return kind == dev::prim_sphere          ? static_cast<int>(out_.spheres.size())
                : kind == dev::prim_moving_sphere ? static_cast<int>(out_.moving_spheres.size())
                : kind == dev::prim_quad          ? static_cast<int>(out_.quads.size())
+               : kind == dev::prim_instance      ? static_cast<int>(out_.instances.size())
                                                  : static_cast<int>(out_.triangles.size());
+    }
+
+    // An instance's box in the world: the mesh tree's root box (its coordinates) through the instance's transform,
+    // every corner, padded as a triangle's
+    static box_d instance_box(const bvh_arrays<Real>& tree, const double* m)
+    {
+        box_d box;
+        const ref_node<Real>& root = tree.nodes.front();
+        double largest = 0;
+        for (int corner = 0; corner < 8; ++corner)
+        {
+            const double x = corner & 1 ? root.box_max[0] : root.box_min[0];
+            const double y = corner & 2 ? root.box_max[1] : root.box_min[1];
+            const double z = corner & 4 ? root.box_max[2] : root.box_min[2];
+            for (int a = 0; a < 3; ++a)
+            {
+                const double w = m[4 * a] * x + m[4 * a + 1] * y + m[4 * a + 2] * z + m[4 * a + 3];
+                box.min[a] = std::min(box.min[a], w);
+                box.max[a] = std::max(box.max[a], w);
+                largest = std::max(largest, std::fabs(w));
+            }
+        }
+        const double margin = 1e-6 * (1 + largest);
+        for (int a = 0; a < 3; ++a)
+        {
+            box.min[a] -= margin;
+            box.max[a] += margin;
+        }
+        return box;
+    }
+
+    // The world-to-object transform of an object-to-world one (row-major 3x4), inverted in double by cofactors
+    static void invert_affine(const double* m, Real* out)
+    {
+        const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+        const double co[9] = {e * i - f * h, c * h - b * i, b * f - c * e, f * g - d * i, a * i - c * g,
+                              c * d - a * f, d * h - e * g, b * g - a * h, a * e - b * d};
+        const double det = a * co[0] + b * co[3] + c * co[6];
+        double scale = 0;
+        for (int k = 0; k < 9; ++k)
+        {
+            scale = std::max(scale, std::fabs(m[k + k / 3]));
+        }
+        if (!(std::fabs(det) > 1e-12 * scale * scale * scale))
+        {
+            throw std::invalid_argument("an instance's transform is singular");
+        }
+        const double t[3] = {m[3], m[7], m[11]};
+        for (int r = 0; r < 3; ++r)
+        {
+            double shift = 0;
+            for (int k = 0; k < 3; ++k)
+            {
+                const double v = co[3 * r + k] / det;
+                out[4 * r + k] = static_cast<Real>(v);
+                shift -= v * t[k];
+            }
+            out[4 * r + 3] = static_cast<Real>(shift);
+        }
     }
 
     // A triangle's box in double, padded as a quad's (a flat box would let rounding in the slab test lose its hits)
@@ -232,6 +354,24 @@ private:
     void add_to_leaf(const input& in)
     {
         int id = 0;
+        if (in.kind == dev::prim_instance)
+        {
+            const rt_mesh_instance_desc* d = &instance_descs_[static_cast<std::size_t>(in.index)];
+            dev::instance_rec<Real> r{};
+            invert_affine(d->object_to_world, r.to_object);
+            r.id_base = out_.instanced_base + next_instanced_id_;
+            r.material = in.material;
+            const mesh_tree& mt = meshes_[static_cast<std::size_t>(d->mesh)];
+            if (static_cast<int64_t>(next_instanced_id_) + static_cast<int64_t>(mt.tree.triangles.size()) +
+                    out_.instanced_base > INT32_MAX / 2)
+            {
+                throw std::runtime_error("too many instanced triangles for 32-bit primitive ids");
+            }
+            next_instanced_id_ += static_cast<int32_t>(mt.tree.triangles.size());
+            out_.instances.push_back(r);
+            instance_mesh_.push_back(d->mesh);
+            return;
+        }
         switch (in.kind)
         {
         case dev::prim_sphere:
@@ -358,7 +498,9 @@ private:
         }
         out_.nodes.push_back(node);
         const std::size_t n = indices.size();
-        if (n == 1 || (one_kind && n <= 2))
+        // an instance is a leaf of its own (the walk enters one mesh tree at a time)
+        const bool instances = one_kind && inputs_[indices[0]].kind == dev::prim_instance;
+        if (n == 1 || (one_kind && !instances && n <= 2))
         {
             return leaf(index, indices);
         }
@@ -419,7 +561,7 @@ private:
         const double parent = half_area(box);
         const double leaf_cost = static_cast<double>(n);
         const double split_cost = parent > 0 ? 1.0 + best_cost / parent : std::numeric_limits<double>::infinity();
-        if (n <= static_cast<std::size_t>(sah_leaf) && one_kind && (best_axis < 0 || leaf_cost <= split_cost))
+        if (n <= static_cast<std::size_t>(sah_leaf) && one_kind && !instances && (best_axis < 0 || leaf_cost <= split_cost))
         {
             return leaf(index, indices);
         }
@@ -427,7 +569,7 @@ private:
         if (n <= static_cast<std::size_t>(sah_leaf) && !one_kind && (best_axis < 0 || leaf_cost <= split_cost))
         {
             // few primitives of several kinds: one subtree per kind, as the reference's builder splits a small leaf
-            int kind = 3;
+            int kind = dev::prim_instance;
             for (const int i : indices)
             {
                 kind = std::min(kind, inputs_[i].kind);
@@ -535,6 +677,10 @@ private:
         const auto me = static_cast<int32_t>(out_.wide.size());
         out_.wide.emplace_back();
         out_.stack_entries = std::max(out_.stack_entries, pushed + count - 1);
+        for (int c = 0; c < count; ++c)
+        {
+            entering_instance(kids[c], pushed + count - 1);
+        }
         int32_t refs[dev::wide_children];
         for (int c = 0; c < dev::wide_children; ++c)
         {
@@ -559,7 +705,73 @@ private:
     {
         out_.wide.clear();
         out_.stack_entries = 1;
+        entering_instance(0, 0);
         out_.root_ref = out_.nodes[0].count != 0 ? leaf_ref_of(0) : collapse(0, 0);
+    }
+
+    // A walk that reaches an instance's leaf with `pushed` entries on its stack walks the mesh's tree on top of them
+    int32_t instance_stack_ = 0;
+    void entering_instance(int node, int pushed)
+    {
+        const ref_node<Real>& n = out_.nodes[static_cast<std::size_t>(node)];
+        if (n.count != 0 && (n.count >> 8) == dev::prim_instance)
+        {
+            const int32_t mesh = instance_mesh_[static_cast<std::size_t>(n.index)];
+            instance_stack_ =
+                std::max(instance_stack_, pushed + meshes_[static_cast<std::size_t>(mesh)].tree.stack_entries);
+        }
+    }
+
+    // The instanced meshes' trees after the scene's: their wide nodes and triangles appended to the scene's arrays
+    // (inner references moved by the node offset, leaves by the triangle offset), each instance pointed at its mesh's
+    // root. The stack's worst case: the scene tree's, or inside an instance what was on the stack at its leaf plus its
+    // mesh tree's worst (entering_instance).
+    static int32_t moved(int32_t ref, int32_t node_offset, int32_t triangle_offset)
+    {
+        if (ref == dev::no_child)
+        {
+            return ref;
+        }
+        if (ref >= 0)
+        {
+            return ref + node_offset;
+        }
+        const int32_t first = dev::leaf_first(ref) + triangle_offset;
+        if (first + dev::leaf_count(ref) - 1 > dev::max_leaf_first)
+        {
+            throw std::runtime_error("an instanced mesh's leaf beyond the packed reference's range");
+        }
+        return dev::make_leaf_ref(dev::leaf_kind(ref), dev::leaf_count(ref), first);
+    }
+    void append_meshes()
+    {
+        for (mesh_tree& mt : meshes_)
+        {
+            if (mt.tree.triangles.empty())
+            {
+                continue;
+            }
+            const auto node_offset = static_cast<int32_t>(out_.wide.size());
+            const auto triangle_offset = static_cast<int32_t>(out_.triangles.size());
+            for (dev::wide_node<Real> w : mt.tree.wide)
+            {
+                for (int32_t& ref : w.ref)
+                {
+                    ref = moved(ref, node_offset, triangle_offset);
+                }
+                out_.wide.push_back(w);
+            }
+            mt.root_ref = moved(mt.tree.root_ref, node_offset, triangle_offset);
+            mt.first_triangle = triangle_offset;
+            out_.triangles.insert(out_.triangles.end(), mt.tree.triangles.begin(), mt.tree.triangles.end());
+        }
+        for (std::size_t k = 0; k < out_.instances.size(); ++k)
+        {
+            const mesh_tree& mt = meshes_[static_cast<std::size_t>(instance_mesh_[k])];
+            out_.instances[k].root_ref = mt.root_ref;
+            out_.instances[k].first_triangle = mt.first_triangle;
+        }
+        out_.stack_entries = std::max(out_.stack_entries, instance_stack_);
     }
 };
 }
@@ -633,8 +845,28 @@ dev::quad_rec<Real> make_quad_rec(const rt_quad_desc& d)
 
 int leaf_size_for(std::size_t shapes) { return shapes <= 8 ? 8 : 1; }
 
-std::vector<world_triangle> world_triangles(const rt_scene& scene)
+std::vector<bool> instanced_meshes(const rt_scene& scene, bool two_level)
 {
+    std::vector<int> uses(static_cast<std::size_t>(std::max<int64_t>(scene.mesh_count, 0)), 0);
+    for (int64_t k = 0; k < scene.instance_count; ++k)
+    {
+        const int32_t m = scene.instances[k].mesh;
+        if (m >= 0 && m < scene.mesh_count)
+        {
+            ++uses[static_cast<std::size_t>(m)];
+        }
+    }
+    std::vector<bool> out(uses.size());
+    for (std::size_t m = 0; m < uses.size(); ++m)
+    {
+        out[m] = two_level && uses[m] > 1;
+    }
+    return out;
+}
+
+std::vector<world_triangle> world_triangles(const rt_scene& scene, bool two_level)
+{
+    const std::vector<bool> instanced = instanced_meshes(scene, two_level);
     std::vector<world_triangle> out;
     const auto place = [&](const rt_mesh_desc& m, const double* transform) {
         for (int64_t k = 0; k < m.triangle_count; ++k)
@@ -664,7 +896,10 @@ std::vector<world_triangle> world_triangles(const rt_scene& scene)
     }
     for (int64_t i = 0; i < scene.instance_count; ++i)
     {
-        place(scene.meshes[scene.instances[i].mesh], scene.instances[i].object_to_world);
+        if (!instanced[static_cast<std::size_t>(scene.instances[i].mesh)])
+        {
+            place(scene.meshes[scene.instances[i].mesh], scene.instances[i].object_to_world);
+        }
     }
     return out;
 }
@@ -672,11 +907,11 @@ std::vector<world_triangle> world_triangles(const rt_scene& scene)
 template <typename Real>
 bvh_arrays<Real> build_bvh(const rt_scene& scene, int max_leaf_size, int32_t tree)
 {
-    if (tree < RT_TREE_DEFAULT || tree > RT_TREE_SAH)
+    if (tree < RT_TREE_DEFAULT || tree > RT_TREE_TWO_LEVEL)
     {
         throw std::invalid_argument("unknown tree");
     }
-    return builder<Real>(scene, max_leaf_size, tree != RT_TREE_MEDIAN).build();
+    return builder<Real>(scene, max_leaf_size, tree != RT_TREE_MEDIAN, tree == RT_TREE_TWO_LEVEL).build();
 }
 
 template bvh_arrays<float> build_bvh<float>(const rt_scene&, int, int32_t);

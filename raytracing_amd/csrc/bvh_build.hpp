@@ -33,13 +33,17 @@ struct bvh_arrays
     std::vector<dev::sphere_rec<Real>> spheres;
     std::vector<dev::moving_sphere_rec<Real>> moving_spheres;
     std::vector<dev::quad_rec<Real>> quads;
-    std::vector<dev::triangle_rec<Real>> triangles;
-    std::vector<int32_t> prim_materials;
-    std::vector<int32_t> prim_ranks; // by global id: the primitive's index in the scene (spheres, quads, triangles)
+    std::vector<dev::triangle_rec<Real>> triangles;   // world triangles first, then each instanced mesh's own
+    std::vector<dev::instance_rec<Real>> instances;   // the instances of meshes placed more than once, by id_base
+    std::vector<int32_t> prim_materials; // by global id below instanced_base
+    std::vector<int32_t> prim_ranks; // by global id below instanced_base: the primitive's index in the scene (spheres,
+                                     // quads, triangles); an instanced triangle's rank is its id
     std::vector<dev::sphere_frame<Real>> sphere_frames;
     int32_t moving_base = 0;
     int32_t quad_base = 0;
     int32_t triangle_base = 0;
+    int32_t instanced_base = 0; // the first id of an instanced mesh's triangle
+    int32_t media_base = 0;     // past the last surface id
     std::vector<int32_t> sphere_ids; // the global id of each sphere of the scene, by its index there
     std::vector<int32_t> quad_ids;
 };
@@ -50,20 +54,26 @@ int leaf_size_for(std::size_t shapes);
 
 // The tree over every sphere, quad and triangle of a scene (tree: an rt_tree). RT_TREE_MEDIAN on a scene without
 // triangles gives the reference's tree (object-median splits, leaves of max_leaf_size); otherwise, and for every scene
-// with triangles (meshes baked into world coordinates through their instances), a surface-area-heuristic tree of
-// small leaves: the reference's object-median split on millions of triangles would make a tree several times slower
-// to walk.
+// with triangles, a surface-area-heuristic tree of small leaves: the reference's object-median split on millions of
+// triangles would make a tree several times slower to walk. RT_TREE_TWO_LEVEL: two levels where meshes are placed more
+// than once: each such mesh gets a tree of its own over its triangles in its own coordinates, once, and each of its
+// instances is a leaf of the scene's tree (a mesh placed once, or a scene without instances, is baked into world
+// coordinates); otherwise every instance is baked.
 template <typename Real>
 bvh_arrays<Real> build_bvh(const rt_scene& scene, int max_leaf_size, int32_t tree);
 
-// The world triangles of a scene's meshes, through their instances (or each mesh once, as it is, when the scene has
-// no instances), with their materials: what build_bvh puts in the tree, in that order before it sorts them
+// Which meshes are placed by more than one instance: those are walked through their instances, the others baked
+std::vector<bool> instanced_meshes(const rt_scene& scene, bool two_level = true);
+
+// The world triangles of a scene's meshes that aren't instanced (through their instance, or each mesh once, as it
+// is, when the scene has no instances), with their materials: what build_bvh puts in the tree in world coordinates, in
+// that order before it sorts them (two_level false: every mesh through every instance)
 struct world_triangle
 {
     double p[3][3];
     int32_t material;
 };
-std::vector<world_triangle> world_triangles(const rt_scene& scene);
+std::vector<world_triangle> world_triangles(const rt_scene& scene, bool two_level = true);
 
 // The quad's record from its description, in double, rounded once (src/gpu/quad.cuh:33-43)
 template <typename Real>

@@ -561,8 +561,9 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
     boundary_quads_ = device_array<dev::quad_rec<Real>>(boundary);
     lights_ = device_array<dev::light_rec<Real>>(lights);
     triangles_ = device_array<dev::triangle_rec<Real>>(tree_.triangles);
+    instances_ = device_array<dev::instance_rec<Real>>(tree_.instances);
     bytes_ = bytes_of(nodes_) + bytes_of(spheres_) + bytes_of(moving_) + bytes_of(quads_) + bytes_of(prim_materials_) +
-             bytes_of(prim_ranks_) +
+             bytes_of(prim_ranks_) + bytes_of(instances_) +
              bytes_of(frames_) + bytes_of(materials_) + bytes_of(textures_) + bytes_of(images_) + bytes_of(texels_) +
              bytes_of(noises_) + bytes_of(media_) + bytes_of(boundary_quads_) + bytes_of(lights_) +
              bytes_of(triangles_);
@@ -613,10 +614,13 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
     v.moving_spheres = moving_.data();
     v.quads = quads_.data();
     v.triangles = triangles_.data();
+    v.instances = instances_.data();
+    v.instance_count = static_cast<int32_t>(tree_.instances.size());
     v.moving_base = tree_.moving_base;
     v.quad_base = tree_.quad_base;
     v.triangle_base = tree_.triangle_base;
-    v.media_base = static_cast<int32_t>(tree_.prim_materials.size());
+    v.instanced_base = tree_.instanced_base;
+    v.media_base = tree_.media_base;
     v.prim_materials = prim_materials_.data();
     v.prim_ranks = prim_ranks_.data();
     v.sphere_frames = frames_.data();

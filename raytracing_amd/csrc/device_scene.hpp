@@ -102,6 +102,7 @@ private:
     device_array<dev::sphere_rec<Real>> spheres_;
     device_array<dev::moving_sphere_rec<Real>> moving_;
     device_array<dev::quad_rec<Real>> quads_;
+    device_array<dev::instance_rec<Real>> instances_;
     device_array<int32_t> prim_materials_;
     device_array<int32_t> prim_ranks_;
     device_array<dev::sphere_frame<Real>> frames_;

@@ -10,7 +10,9 @@
 //     test is the reference's, and the closest hit is the same primitive at the same distance (of two primitives hit
 //     at exactly the same distance, the one of lower index in the scene, whatever the walk's order).
 //   - primitives keep the reference's global ids: spheres that stand still [0, moving_base), moving spheres
-//     [moving_base, quad_base), quads [quad_base, ...), and the media after the surfaces.
+//     [moving_base, quad_base), quads [quad_base, triangle_base), triangles in world coordinates [triangle_base,
+//     instanced_base), the triangles of meshes placed by several instances [instanced_base, media_base) (each
+//     instance its own range), and the media after the surfaces.
 #pragma once
 
 #include <cstddef>
@@ -20,16 +22,18 @@
 
 namespace rt::dev
 {
-// The kinds of primitives in a leaf
+// The kinds of primitives in a leaf. An instance leaf holds one instance of a mesh: the walk goes on into the mesh's
+// own tree (its triangles in the mesh's coordinates) with the ray moved there.
 enum prim_kind : int32_t
 {
     prim_sphere = 0,
     prim_moving_sphere = 1,
     prim_quad = 2,
     prim_triangle = 3,
+    prim_instance = 4,
 };
 
-// A leaf reference: bit 31 set, the kind in bits 28-29, the count minus 1 in bits 24-27 (at most 16 primitives), and
+// A leaf reference: bit 31 set, the kind in bits 28-30, the count minus 1 in bits 24-27 (at most 16 primitives), and
 // the first primitive's index in its kind's array in bits 0-23
 constexpr int32_t leaf_first_bits = 24;
 constexpr int32_t max_leaf_first = (1 << leaf_first_bits) - 1;
@@ -38,7 +42,7 @@ __host__ __device__ constexpr int32_t make_leaf_ref(int32_t kind, int32_t count,
     return static_cast<int32_t>(0x80000000u | (static_cast<uint32_t>(kind) << 28) |
                                 (static_cast<uint32_t>(count - 1) << 24) | static_cast<uint32_t>(first));
 }
-__host__ __device__ constexpr int32_t leaf_kind(int32_t ref) { return (ref >> 28) & 3; }
+__host__ __device__ constexpr int32_t leaf_kind(int32_t ref) { return (ref >> 28) & 7; }
 __host__ __device__ constexpr int32_t leaf_count(int32_t ref) { return ((ref >> 24) & 15) + 1; }
 __host__ __device__ constexpr int32_t leaf_first(int32_t ref) { return ref & max_leaf_first; }
 
@@ -99,6 +103,23 @@ struct alignas(16) triangle_rec
     Real normal[3];
 };
 static_assert(sizeof(triangle_rec<float>) == 48);
+
+// An instance of a mesh (two-level BVH, the reference's mesh_set instances, src/gpu/renderer.h:305-376): the world to
+// object transform (row-major 3x4: x' = A x + b), the root of the mesh's own tree (in the same node array), where the
+// mesh's triangles start in the triangle array, the global id of its first triangle (an instance's triangle k has the
+// id id_base + k: every instance has ids of its own, for the self-intersection rule and the ties), and its material.
+// A ray keeps its parameter t in the mesh's coordinates (its direction is moved by A, not normalized), so distances
+// found there compare with the world's directly.
+template <typename Real>
+struct alignas(16) instance_rec
+{
+    Real to_object[12];
+    int32_t root_ref;
+    int32_t first_triangle;
+    int32_t id_base;
+    int32_t material;
+};
+static_assert(sizeof(instance_rec<float>) == 64);
 
 // What only a sphere hit reads besides its material (src/gpu/sphere.cuh:36-46)
 template <typename Real>
@@ -290,9 +311,12 @@ struct scene_view
     const moving_sphere_rec<Real>* moving_spheres;
     const quad_rec<Real>* quads;
     const triangle_rec<Real>* triangles;
+    const instance_rec<Real>* instances; // by id_base, increasing
+    int32_t instance_count;
     int32_t moving_base;
     int32_t quad_base;
     int32_t triangle_base;
+    int32_t instanced_base; // the first instanced triangle's id: prim_materials and prim_ranks end here
     int32_t media_base;
     const int32_t* prim_materials;
     const int32_t* prim_ranks; // by global id: the primitive's index in the scene, which breaks exact ties

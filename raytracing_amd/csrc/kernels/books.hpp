@@ -263,9 +263,59 @@ __device__ inline void triangle_vertex(const scene_view<Real>& sc, const ray<Rea
     }
 }
 
-// The vertex of any event: a surface's, or a particle of smoke, which has no surface (any normal will do: the phase
-// function ignores it) and whose next ray leaves no primitive
+// The instance an instanced triangle's id belongs to: the last whose id_base is no larger (instances are sorted by it)
+template <typename Real>
+__device__ inline int instance_of(const scene_view<Real>& sc, int id)
+{
+    int lo = 0, hi = sc.instance_count - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sc.instances[mid].id_base <= id)
+        {
+            lo = mid;
+        }
+        else
+        {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+// A surface's material: from the table, or an instanced triangle's from its instance
+template <typename Real>
+__device__ inline int material_of(const scene_view<Real>& sc, int id)
+{
+    return id < sc.instanced_base ? sc.prim_materials[id] : sc.instances[instance_of(sc, id)].material;
+}
+// An instanced triangle: its mesh's record, its normal carried to the world by the inverse transpose of the
+// instance's object-to-world map (the transpose of its world-to-object one, A), its barycentrics from the same test in
+// the mesh's coordinates
 template <typename Real, typename Tally>
+__device__ inline void instanced_vertex(const scene_view<Real>& sc, const instance_rec<Real>& in, const ray<Real>& r,
+                                        const hit<Real>& h, uint32_t flags, vertex<Real>& vx, Tally& tl)
+{
+    const triangle_rec<Real> tri = sc.triangles[in.first_triangle + (h.index - in.id_base)];
+    tl.add(tally_triangle);
+    const Real* a = in.to_object;
+    const vec3<Real> n = load3(tri.normal);
+    vx.outward = unit_vector(vec3<Real>{m_fma(a[0], n.x, m_fma(a[4], n.y, a[8] * n.z)),
+                                        m_fma(a[1], n.x, m_fma(a[5], n.y, a[9] * n.z)),
+                                        m_fma(a[2], n.x, m_fma(a[6], n.y, a[10] * n.z))});
+    if (flags & needs_uv)
+    {
+        Real t, bary[2];
+        bool front;
+        hit_triangle(tri, into_instance(a, r), -inf<Real>, inf<Real>, t, front, bary);
+        vx.u = bary[0];
+        vx.v = bary[1];
+    }
+}
+
+// The vertex of any event: a surface's, or a particle of smoke, which has no surface (any normal will do: the phase
+// function ignores it) and whose next ray leaves no primitive. Instances: the scene's tree has two levels (a walk that
+// can enter an instance found h)
+template <bool Instances, typename Real, typename Tally>
 __device__ inline vertex<Real> make_vertex(const scene_view<Real>& sc, const ray<Real>& r, const hit<Real>& h, Tally& tl)
 {
     tl.add(tally_hit);
@@ -289,9 +339,18 @@ __device__ inline vertex<Real> make_vertex(const scene_view<Real>& sc, const ray
         vx.from = -1;
         return vx;
     }
-    vx.material = sc.prim_materials[h.index];
     vx.from = h.index;
     vx.front = h.entering;
+    if (Instances && h.index >= sc.instanced_base)
+    {
+        tl.add(tally_instance);
+        const instance_rec<Real> in = sc.instances[instance_of(sc, h.index)];
+        vx.material = in.material;
+        instanced_vertex(sc, in, r, h, sc.materials[vx.material].flags, vx, tl);
+        vx.n = vx.front ? vx.outward : -vx.outward;
+        return vx;
+    }
+    vx.material = sc.prim_materials[h.index];
     const uint32_t flags = sc.materials[vx.material].flags;
     if (h.index >= sc.triangle_base)
     {
@@ -500,11 +559,11 @@ __device__ inline hit<Real> media_events(const scene_view<Real>& sc, const ray<R
     }
     return closest;
 }
-template <typename Real, typename Tally, typename Stack>
+template <typename Real, int Kinds = walk_instances, typename Tally, typename Stack>
 __device__ inline hit<Real> closest_event(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
                                           pcg32& g, const Stack& stack, Tally& tl)
 {
-    return media_events(sc, r, closest_hit<Real, true>(sc, r, from, inward, stack, tl), g, tl);
+    return media_events(sc, r, closest_hit<Real, Kinds>(sc, r, from, inward, stack, tl), g, tl);
 }
 
 // --- the books' lobes ---------------------------------------------------------------------------------------------------
@@ -747,7 +806,7 @@ template <typename Real, typename Tally>
 __device__ inline bool shade(path<Real>& p, const scene_view<Real>& sc, const hit<Real>& h, pcg32& g,
                              vec3<Real>& sample, Tally& tl)
 {
-    const vertex<Real> vx = make_vertex(sc, p.r, h, tl);
+    const vertex<Real> vx = make_vertex<false>(sc, p.r, h, tl);
     const material_rec<Real> m = sc.materials[vx.material];
     const vec3<Real> black{Real(0), Real(0), Real(0)};
     const books_lobe lobe = lobe_of(m.kind);
@@ -801,7 +860,7 @@ template <typename Real, typename Tally, typename Stack>
 __device__ inline bool trace_all_books(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
                                        pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
 {
-    return all_books_after_walk(p, sc, cam, g, sample, closest_hit<Real, true>(sc, p.r, p.from, p.inward, stack, tl),
+    return all_books_after_walk(p, sc, cam, g, sample, closest_hit<Real, walk_books>(sc, p.r, p.from, p.inward, stack, tl),
                                 tl);
 }
 
@@ -811,7 +870,7 @@ template <typename Real, typename Tally, typename Stack>
 __device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
                                         pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
 {
-    const hit<Real> h = closest_hit<Real, false>(sc, p.r, p.from, p.inward, stack, tl);
+    const hit<Real> h = closest_hit<Real, walk_spheres>(sc, p.r, p.from, p.inward, stack, tl);
     if (h.index < 0)
     {
         return end_path(p, sky(cam, p.r.direction), sample);

@@ -176,6 +176,30 @@ __global__ void __launch_bounds__(256) add_staged(const staged_sample<Real>* __r
 // The queue's head, cleared by a kernel before each launch (no memset node where a caller captures the launches)
 __global__ void clear_queue(uint32_t* next_chunk) { *next_chunk = 0; }
 
+// What a persistent megakernel reads at every ray: the camera, the scene, the launch's queue and seed. They are the
+// kernel's first argument, so they lie at the start of the kernarg segment.
+template <typename Real>
+struct megakernel_args
+{
+    camera_rec<Real> cam;
+    scene_view<Real> sc;
+    chunk_queue q;
+    uint64_t seed;
+};
+
+// The kernel's first argument, read again from the kernarg segment where this is called (constant memory: scalar
+// loads through the scalar cache). The empty asm hides the pointer's value, so the compiler can't hoist the reads to
+// the kernel's entry: without it the camera's and the scene's uniform values were all loaded there and held for the
+// whole kernel, more than the scalar registers hold, and the spilled ones were read back from vector registers' lanes
+// (v_readlane) inside the walk's loop. Read per ray, each value is live only where it is used.
+template <typename Args>
+__device__ inline const Args& first_kernel_argument()
+{
+    auto at = __builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(at));
+    return *(const Args*)at;
+}
+
 // Waves: the waves per SIMD the register allocation must leave room for (0: the compiler's choice); a tighter budget
 // trades spills for occupancy, and the session's autotuner picks per scene. The grid is as many waves as fit the chip
 // at once; each lane follows its path ray by ray (src/gpu/renderer.cu:340-397) and claims its next item from its
@@ -184,11 +208,10 @@ __global__ void clear_queue(uint32_t* next_chunk) { *next_chunk = 0; }
 // stays live across the shading code, which costs registers.)
 template <typename Real, bool AllBooks, bool Count, int Waves>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
-megakernel(const camera_rec<Real> cam, const scene_view<Real> sc, const uint64_t seed, const chunk_queue q,
-           staged_sample<Real>* __restrict__ staging, unsigned long long* __restrict__ ray_count,
-           unsigned long long* __restrict__ tallies)
+megakernel(const megakernel_args<Real> args, staged_sample<Real>* __restrict__ staging,
+           unsigned long long* __restrict__ ray_count, unsigned long long* __restrict__ tallies)
 {
-    const lds_stack stack = block_stack(sc.stack_depth);
+    const lds_stack stack = block_stack(args.sc.stack_depth);
     tally<Count> tl;
     wave_queue w = wave_queue::start();
     unsigned long long rays = 0;
@@ -197,13 +220,17 @@ megakernel(const camera_rec<Real> cam, const scene_view<Real> sc, const uint64_t
     pcg32 g{};
     while (true)
     {
+        const megakernel_args<Real>& a = first_kernel_argument<megakernel_args<Real>>();
+        const camera_rec<Real>& cam = a.cam;
+        const scene_view<Real>& sc = a.sc;
+        const chunk_queue& q = a.q;
         int si, sj, pass_local;
         if (w.claim(q, at < 0, si, sj, pass_local))
         {
             const int pass = q.work.first_pass + pass_local;
             const uint64_t pixel =
                 static_cast<uint64_t>(sj) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(si);
-            g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
+            g = sample_generator(a.seed, pixel, static_cast<uint64_t>(pass));
             p = start_path(cam, AllBooks && sc.motion != 0, si, sj, pass, g);
             at = staged_at(q, si, sj, pass_local);
         }
@@ -240,9 +267,8 @@ struct far_stack
 
 // The quality integrator's paths (src/gpu/quality.cu:53-109): the same loop over the queue's items, with light
 // samples and their shadow rays at every vertex that isn't a mirror's (quality.hpp)
-template <bool Count, typename Stack>
-__device__ inline void quality_paths(const camera_rec<float>& cam, const scene_view<float>& sc, uint64_t seed,
-                                     const chunk_queue& q, staged_sample<float>* __restrict__ staging,
+template <bool Count, bool Instances, typename Stack>
+__device__ inline void quality_paths(staged_sample<float>* __restrict__ staging,
                                      unsigned long long* __restrict__ ray_count,
                                      unsigned long long* __restrict__ tallies, const Stack& stack)
 {
@@ -255,13 +281,17 @@ __device__ inline void quality_paths(const camera_rec<float>& cam, const scene_v
     pcg32 g{};
     while (true)
     {
+        const megakernel_args<float>& a = first_kernel_argument<megakernel_args<float>>();
+        const camera_rec<float>& cam = a.cam;
+        const scene_view<float>& sc = a.sc;
+        const chunk_queue& q = a.q;
         int si, sj, pass_local;
         if (w.claim(q, at < 0, si, sj, pass_local))
         {
             const int pass = q.work.first_pass + pass_local;
             const uint64_t pixel =
                 static_cast<uint64_t>(sj) * static_cast<uint64_t>(cam.image_width) + static_cast<uint64_t>(si);
-            g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
+            g = sample_generator(a.seed, pixel, static_cast<uint64_t>(pass));
             qp = start_quality(cam, sc.motion != 0, si, sj, pass, g);
             at = staged_at(q, si, sj, pass_local);
         }
@@ -275,7 +305,7 @@ __device__ inline void quality_paths(const camera_rec<float>& cam, const scene_v
         }
         ++rays;
         vec3<float> sample;
-        if (trace_quality(qp, sc, cam, g, sample, stack, tl, shadow_rays))
+        if (trace_quality<Instances>(qp, sc, cam, g, sample, stack, tl, shadow_rays))
         {
             stage(staging + at, sample);
             at = -1;
@@ -286,21 +316,21 @@ __device__ inline void quality_paths(const camera_rec<float>& cam, const scene_v
 
 // Split: the stack's first far.near levels in LDS, the rest in HBM (a deep tree, such as a large mesh's, would
 // otherwise hold so much LDS per wave that fewer waves fit than the registers allow)
-template <bool Count, int Waves, bool Split>
+template <bool Count, int Waves, bool Split, bool Instances>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
-quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc, const uint64_t seed, const chunk_queue q,
-                   staged_sample<float>* __restrict__ staging, unsigned long long* __restrict__ ray_count,
-                   unsigned long long* __restrict__ tallies, const far_stack far)
+quality_megakernel(const megakernel_args<float> args, staged_sample<float>* __restrict__ staging,
+                   unsigned long long* __restrict__ ray_count, unsigned long long* __restrict__ tallies,
+                   const far_stack far)
 {
     if constexpr (Split)
     {
         const uint32_t column = blockIdx.x * blockDim.x + threadIdx.x;
         const split_stack stack{block_stack(far.near), far.near, far.refs + column, far.ts + column, far.stride};
-        quality_paths<Count>(cam, sc, seed, q, staging, ray_count, tallies, stack);
+        quality_paths<Count, Instances>(staging, ray_count, tallies, stack);
     }
     else
     {
-        quality_paths<Count>(cam, sc, seed, q, staging, ray_count, tallies, block_stack(sc.stack_depth));
+        quality_paths<Count, Instances>(staging, ray_count, tallies, block_stack(args.sc.stack_depth));
     }
 }
 
@@ -309,27 +339,38 @@ quality_megakernel(const camera_rec<float> cam, const scene_view<float> sc, cons
 // double sums as it ends. No queue and no staging: a wave waits for its longest pixel, and a launch for its costliest
 // blocks. It is the baseline the batched integrators are measured against; its sums are the others' bytes (the same
 // samples added in the same order).
-template <typename Real, int Mode>
-__global__ void __launch_bounds__(128) plain_megakernel(const camera_rec<Real> cam, const scene_view<Real> sc,
-                                                        const uint64_t seed, const launch_work work,
-                                                        double* __restrict__ sums,
+template <typename Real>
+struct plain_args
+{
+    camera_rec<Real> cam;
+    scene_view<Real> sc;
+    launch_work work;
+    uint64_t seed;
+};
+template <typename Real, int Mode, bool Instances>
+__global__ void __launch_bounds__(128) plain_megakernel(const plain_args<Real> args, double* __restrict__ sums,
                                                         unsigned long long* __restrict__ ray_count)
 {
-    const lds_stack stack = block_stack(sc.stack_depth);
+    const lds_stack stack = block_stack(args.sc.stack_depth);
     tally<false> tl;
+    const launch_work& work = args.work;
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int j = work.first_row + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
     unsigned long long rays = 0;
     uint32_t shadow_rays = 0;
-    if (i < cam.image_width && j < work.first_row + work.row_count)
+    if (i < args.cam.image_width && j < work.first_row + work.row_count)
     {
-        const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(cam.image_width) + i;
+        const uint64_t pixel = static_cast<uint64_t>(j) * static_cast<uint64_t>(args.cam.image_width) + i;
         double* const sum = sums + 3 * pixel;
         double red = sum[0], green = sum[1], blue = sum[2];
         for (int k = 0; k < work.pass_count; ++k)
         {
-            const int pass = work.first_pass + k;
-            pcg32 g = sample_generator(seed, pixel, static_cast<uint64_t>(pass));
+            // (the camera and the scene read again per pass, as the queue megakernels read them per ray)
+            const plain_args<Real>& a = first_kernel_argument<plain_args<Real>>();
+            const camera_rec<Real>& cam = a.cam;
+            const scene_view<Real>& sc = a.sc;
+            const int pass = a.work.first_pass + k;
+            pcg32 g = sample_generator(a.seed, pixel, static_cast<uint64_t>(pass));
             vec3<Real> sample;
             if constexpr (Mode == mode_quality)
             {
@@ -337,7 +378,7 @@ __global__ void __launch_bounds__(128) plain_megakernel(const camera_rec<Real> c
                 do
                 {
                     ++rays;
-                } while (!trace_quality(qp, sc, cam, g, sample, stack, tl, shadow_rays));
+                } while (!trace_quality<Instances>(qp, sc, cam, g, sample, stack, tl, shadow_rays));
             }
             else
             {
@@ -376,7 +417,7 @@ __global__ void __launch_bounds__(256) closest_hit_kernel(const scene_view<Real>
                       {directions[3 * k], directions[3 * k + 1], directions[3 * k + 2]},
                       times[k]};
     tally<false> tl;
-    const hit<Real> h = closest_hit<Real, true>(sc, r, from[k], inward[k] != 0, stack, tl);
+    const hit<Real> h = closest_hit<Real, walk_instances>(sc, r, from[k], inward[k] != 0, stack, tl);
     index[k] = h.index;
     t[k] = h.t;
     entering[k] = h.entering ? 1 : 0;
@@ -457,14 +498,19 @@ void* static_scratch(size_t* bytes)
 namespace
 {
 template <typename Real, bool Count, int Waves>
-const void* megakernel_for_waves(int mode, bool split)
+const void* megakernel_for_waves(int mode, bool split, bool instanced)
 {
     if constexpr (sizeof(Real) == sizeof(float))
     {
+        if (mode == dev::mode_quality && instanced)
+        {
+            return split ? reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, true, true>)
+                         : reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, false, true>);
+        }
         if (mode == dev::mode_quality)
         {
-            return split ? reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, true>)
-                         : reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, false>);
+            return split ? reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, true, false>)
+                         : reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, false, false>);
         }
     }
     return mode == dev::mode_all_books ? reinterpret_cast<const void*>(&dev::megakernel<Real, true, Count, Waves>)
@@ -474,24 +520,24 @@ const void* megakernel_for_waves(int mode, bool split)
 // The instance for a mode and an occupancy target: the counting instances and the FP64 ones at the compiler's choice.
 // split: the quality megakernel with its stack's deeper levels in HBM
 template <typename Real, bool Count>
-const void* megakernel_for(int mode, int waves, bool split = false)
+const void* megakernel_for(int mode, int waves, bool split, bool instanced)
 {
     if constexpr (!Count && sizeof(Real) == sizeof(float))
     {
         if (waves == 5)
         {
-            return megakernel_for_waves<Real, Count, 5>(mode, split);
+            return megakernel_for_waves<Real, Count, 5>(mode, split, instanced);
         }
         if (waves == 6)
         {
-            return megakernel_for_waves<Real, Count, 6>(mode, split);
+            return megakernel_for_waves<Real, Count, 6>(mode, split, instanced);
         }
         if (waves == 7)
         {
-            return megakernel_for_waves<Real, Count, 7>(mode, split);
+            return megakernel_for_waves<Real, Count, 7>(mode, split, instanced);
         }
     }
-    return megakernel_for_waves<Real, Count, 0>(mode, split);
+    return megakernel_for_waves<Real, Count, 0>(mode, split, instanced);
 }
 
 // The stack levels a split quality megakernel keeps in LDS: as many as let the waves its registers allow share a CU's
@@ -558,7 +604,9 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
         return hipErrorInvalidValue;
     }
     size_t lds = stack_lds_bytes<Real>(sc.stack_depth, block_threads);
-    const void* f = tallies ? megakernel_for<Real, true>(mode, 0) : megakernel_for<Real, false>(mode, waves);
+    const bool instanced = sc.instance_count > 0;
+    const void* f = tallies ? megakernel_for<Real, true>(mode, 0, false, instanced)
+                            : megakernel_for<Real, false>(mode, waves, false, instanced);
     // as many blocks as fit the chip at once
     int device = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&device);
@@ -579,7 +627,7 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
     dev::far_stack far{sc.stack_depth, nullptr, nullptr, 0};
     if (!tallies && mode == dev::mode_quality && far_buffer != nullptr && sc.stack_depth > 0)
     {
-        const void* split = megakernel_for<Real, false>(mode, waves, true);
+        const void* split = megakernel_for<Real, false>(mode, waves, true, instanced);
         const int near = near_levels(split, sc.stack_depth, block_threads, device);
         const size_t split_lds = stack_lds_bytes<Real>(near, block_threads);
         int split_per_cu = 0;
@@ -618,11 +666,9 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
             1, std::min<uint32_t>(static_cast<uint32_t>(std::max(1, per_cu * cus / grid_ways)),
                                   (q.chunks + waves_per_block - 1) / waves_per_block));
         hipLaunchKernelGGL(dev::clear_queue, dim3(1), dim3(1), 0, stream, queue);
-        dev::camera_rec<Real> c = cam;
-        dev::scene_view<Real> v = sc;
-        uint64_t s = seed;
-        void* args[] = {&c, &v, &s, &q, &staged, &rays, &tallies, &far};
-        void* book_args[] = {&c, &v, &s, &q, &staged, &rays, &tallies}; // the books' kernels take no far_stack
+        dev::megakernel_args<Real> a{cam, sc, q, seed};
+        void* args[] = {&a, &staged, &rays, &tallies, &far};
+        void* book_args[] = {&a, &staged, &rays, &tallies}; // the books' kernels take no far_stack
         e = hipLaunchKernel(f, dim3(blocks), dim3(block_threads), mode == dev::mode_quality ? args : book_args, lds,
                             stream);
         if (e != hipSuccess)
@@ -643,17 +689,19 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
 namespace
 {
 template <typename Real>
-const void* plain_megakernel_for(int mode)
+const void* plain_megakernel_for(int mode, bool instanced)
 {
     if constexpr (sizeof(Real) == sizeof(float))
     {
         if (mode == dev::mode_quality)
         {
-            return reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality>);
+            return instanced ? reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, true>)
+                             : reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, false>);
         }
     }
-    return mode == dev::mode_all_books ? reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_all_books>)
-                                       : reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_first_book>);
+    return mode == dev::mode_all_books
+               ? reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_all_books, false>)
+               : reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_first_book, false>);
 }
 }
 
@@ -670,19 +718,16 @@ hipError_t launch_plain_megakernel(int mode, const dev::camera_rec<Real>& cam, c
     const dim3 grid(static_cast<unsigned>((cam.image_width + plain_block_x - 1) / plain_block_x),
                     static_cast<unsigned>((work.row_count + plain_block_y - 1) / plain_block_y));
     const size_t lds = stack_lds_bytes<Real>(sc.stack_depth, plain_block_x * plain_block_y);
-    dev::camera_rec<Real> c = cam;
-    dev::scene_view<Real> v = sc;
-    uint64_t s = seed;
-    dev::launch_work w = work;
-    void* args[] = {&c, &v, &s, &w, &sums, &rays};
-    return hipLaunchKernel(plain_megakernel_for<Real>(mode), grid, block, args, lds, stream);
+    dev::plain_args<Real> a{cam, sc, work, seed};
+    void* args[] = {&a, &sums, &rays};
+    return hipLaunchKernel(plain_megakernel_for<Real>(mode, sc.instance_count > 0), grid, block, args, lds, stream);
 }
 
 template <typename Real>
-kernel_attributes plain_megakernel_attributes(int mode, int depth)
+kernel_attributes plain_megakernel_attributes(int mode, int depth, bool instanced)
 {
     hipFuncAttributes a{};
-    const void* f = plain_megakernel_for<Real>(mode);
+    const void* f = plain_megakernel_for<Real>(mode, instanced);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)
     {
@@ -705,8 +750,8 @@ template hipError_t launch_plain_megakernel<float>(int, const dev::camera_rec<fl
 template hipError_t launch_plain_megakernel<double>(int, const dev::camera_rec<double>&,
                                                     const dev::scene_view<double>&, uint64_t, const dev::launch_work&,
                                                     double*, unsigned long long*, hipStream_t);
-template kernel_attributes plain_megakernel_attributes<float>(int, int);
-template kernel_attributes plain_megakernel_attributes<double>(int, int);
+template kernel_attributes plain_megakernel_attributes<float>(int, int, bool);
+template kernel_attributes plain_megakernel_attributes<double>(int, int, bool);
 
 template <typename Real>
 hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origins, const Real* directions,
@@ -730,10 +775,10 @@ hipError_t launch_display(const double* sums, int64_t pixels, double inverse_pas
 }
 
 template <typename Real>
-kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves)
+kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves, bool instanced)
 {
     hipFuncAttributes a{};
-    const void* f = megakernel_for<Real, false>(mode, waves);
+    const void* f = megakernel_for<Real, false>(mode, waves, false, instanced);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)
     {
@@ -760,8 +805,8 @@ template hipError_t launch_megakernel<double>(int, const dev::camera_rec<double>
 template hipError_t launch_closest_hit<float>(const dev::scene_view<float>&, const float*, const float*, const float*,
                                               const int32_t*, const uint8_t*, int64_t, int32_t*, float*, uint8_t*,
                                               hipStream_t);
-template kernel_attributes megakernel_attributes<float>(int, int, int, int);
-template kernel_attributes megakernel_attributes<double>(int, int, int, int);
+template kernel_attributes megakernel_attributes<float>(int, int, int, int, bool);
+template kernel_attributes megakernel_attributes<double>(int, int, int, int, bool);
 template size_t stack_lds_bytes<float>(int, int);
 template size_t stack_lds_bytes<double>(int, int);
 }

@@ -365,6 +365,7 @@ enum tally_event
     tally_lamp,           // a lamp of the quality integrator's light set
     tally_env_texel,      // an environment map texel (radiance and density)
     tally_alias_bin,      // a bin of the environment map's alias table
+    tally_instance,       // an instance record (the ray moved into its mesh)
     tally_events
 };
 template <bool On>
@@ -467,25 +468,49 @@ __device__ inline lds_stack stack_in(unsigned char* lds, int entries, int thread
 // are read only on an exact tie.
 // (A first candidate at an infinite t, which an overflowing quad or triangle distance could give while nothing is hit
 // yet, is no hit: the ranks are read only once a closest hit exists.)
-template <typename Real>
+// (An instanced mesh's triangles rank by their ids, which follow every other surface's rank: only a walk that can
+// enter an instance meets them.)
+// What a walk may meet: the first book's trees hold spheres that stand still only; the books' add moving spheres and
+// quads; the quality integrator's add triangles, and in a two-level tree instances of meshes (a walk that can enter
+// an instance carries the world ray beside the moved one: a kernel of its own, so that the others don't hold it)
+enum walk_kinds : int
+{
+    walk_spheres = 0,
+    walk_books = 1,
+    walk_all = 2,
+    walk_instances = 3,
+};
+template <int Kinds, typename Real>
+__device__ inline int32_t rank_of(const scene_view<Real>& sc, int id)
+{
+    if constexpr (Kinds == walk_instances)
+    {
+        return id < sc.instanced_base ? sc.prim_ranks[id] : id;
+    }
+    return sc.prim_ranks[id];
+}
+template <int Kinds, typename Real>
 __device__ inline bool better(const scene_view<Real>& sc, int id, Real t, const hit<Real>& closest)
 {
-    return t < closest.t || (closest.index >= 0 && sc.prim_ranks[id] < sc.prim_ranks[closest.index]);
+    return t < closest.t || (closest.index >= 0 && rank_of<Kinds>(sc, id) < rank_of<Kinds>(sc, closest.index));
 }
 
-// The primitives of a leaf against the ray, the closest hit so far updated (src/gpu/bvh.cuh:612-680)
-template <typename Real, bool AllKinds, typename Tally>
+// The primitives of a leaf against the ray, the closest hit so far updated (src/gpu/bvh.cuh:612-680). A triangle's id
+// is its index in the triangle array plus triangle_shift (the world's triangles: triangle_base; an instance's: its
+// id_base less where its mesh's triangles start).
+template <typename Real, int Kinds, typename Tally>
 __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Real>& r, int32_t ref, int from,
-                                       bool inward, Real inverse_a, Real t_min, hit<Real>& closest, Tally& tl)
+                                       bool inward, Real inverse_a, Real t_min, int triangle_shift, hit<Real>& closest,
+                                       Tally& tl)
 {
     const int kind = leaf_kind(ref);
     const int first = leaf_first(ref);
     const int end = first + leaf_count(ref);
-    if (AllKinds && kind == prim_triangle)
+    if (Kinds >= walk_all && kind == prim_triangle)
     {
         for (int i = first; i < end; ++i)
         {
-            const int id = sc.triangle_base + i;
+            const int id = triangle_shift + i;
             if (id == from)
             {
                 continue; // a ray never meets the flat face it leaves
@@ -494,13 +519,13 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
             tl.add(tally_triangle);
             Real t;
             bool front;
-            if (hit_triangle(tri, r, t_min, closest.t, t, front) && better(sc, id, t, closest))
+            if (hit_triangle(tri, r, t_min, closest.t, t, front) && better<Kinds>(sc, id, t, closest))
             {
                 closest = {id, t, front};
             }
         }
     }
-    else if (AllKinds && kind == prim_quad)
+    else if (Kinds >= walk_books && kind == prim_quad)
     {
         for (int i = first; i < end; ++i)
         {
@@ -513,7 +538,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
             tl.add(tally_quad);
             Real t;
             bool front;
-            if (hit_quad(face, r, t_min, closest.t, t, front) && better(sc, id, t, closest))
+            if (hit_quad(face, r, t_min, closest.t, t, front) && better<Kinds>(sc, id, t, closest))
             {
                 closest = {id, t, front};
             }
@@ -521,7 +546,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
     }
     else
     {
-        const bool moving = AllKinds && kind == prim_moving_sphere;
+        const bool moving = Kinds >= walk_books && kind == prim_moving_sphere;
         for (int i = first; i < end; ++i)
         {
             const int id = moving ? sc.moving_base + i : i;
@@ -534,7 +559,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
             tl.add(moving ? tally_moving : tally_sphere);
             Real t;
             bool entering;
-            if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering, true) && better(sc, id, t, closest))
+            if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering, true) && better<Kinds>(sc, id, t, closest))
             {
                 closest = {id, t, entering};
             }
@@ -645,12 +670,12 @@ __device__ inline bool descend(const scene_view<Real>& sc, int32_t& ref, const s
     return false;
 }
 
-// The next stack entry the ray may enter no farther than the closest hit so far (at exactly that distance it may hold
-// a tie), or false when there is none
+// The next stack entry above `floor` the ray may enter no farther than the closest hit so far (at exactly that distance
+// it may hold a tie), or false when there is none
 template <typename Real, typename Stack>
-__device__ inline bool pop(const Stack& stack, int& size, Real t_max, int32_t& ref)
+__device__ inline bool pop(const Stack& stack, int& size, Real t_max, int32_t& ref, int floor = 0)
 {
-    while (size > 0)
+    while (size > floor)
     {
         --size;
         if (stack.distance_at(size) <= t_max)
@@ -662,12 +687,43 @@ __device__ inline bool pop(const Stack& stack, int& size, Real t_max, int32_t& r
     return false;
 }
 
+// The ray as the walk sees it, in the world's coordinates or moved into an instance's mesh, with what the box and
+// sphere tests take from it
+template <typename Real>
+struct walk_ray
+{
+    ray<Real> r;
+    Real inverse_a;
+    vec3<Real> inv, os;
+    slab_order so;
+    __device__ void set(const ray<Real>& to)
+    {
+        r = to;
+        inverse_a = Real(1) / length_squared(r.direction);
+        inv = {safe_inverse(r.direction.x), safe_inverse(r.direction.y), safe_inverse(r.direction.z)};
+        os = {r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
+        so = slab_order_of(inv);
+    }
+};
+
+// A world ray in an instance's mesh coordinates: x' = A x + b, the direction by A alone, so that a point at t along
+// the world ray is the point at t along the moved one
+template <typename Real>
+__device__ inline ray<Real> into_instance(const Real* m, const ray<Real>& r)
+{
+    const auto row = [&](int k, vec3<Real> v) { return m_fma(m[4 * k], v.x, m_fma(m[4 * k + 1], v.y, m[4 * k + 2] * v.z)); };
+    return {{row(0, r.origin) + m[3], row(1, r.origin) + m[7], row(2, r.origin) + m[11]},
+            {row(0, r.direction), row(1, r.direction), row(2, r.direction)},
+            r.time};
+}
+
 // The closest primitive a ray hits at t in (0.001, ∞), with the self-intersection rule for the primitive it leaves
-// (src/gpu/bvh.cuh:576-706). AllKinds false: a tree of spheres that stand still only (the first book's scenes). Nodes
-// are visited near first and leaves tested in the reference's order; the loop takes one node or one leaf per
-// iteration (on MI355X this measured faster than Aila and Laine's while-while form, which walks inner nodes in an
-// inner loop until every lane of the wave has a leaf).
-template <typename Real, bool AllKinds, typename Tally, typename Stack>
+// (src/gpu/bvh.cuh:576-706). Kinds: what the scene's tree may hold (walk_kinds). Nodes are visited near first and
+// leaves tested in the reference's order; the loop takes one node or one leaf per iteration (on MI355X this measured
+// faster than Aila and Laine's while-while form, which walks inner nodes in an inner loop until every lane of the wave
+// has a leaf). An instance leaf moves the ray into its mesh's coordinates and goes on into its mesh's tree; the stack's
+// entries below `floor` are the scene tree's, and when the mesh tree's are used up the ray is moved back.
+template <typename Real, int Kinds, typename Tally, typename Stack>
 __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
                                         const Stack& stack, Tally& tl)
 {
@@ -677,23 +733,55 @@ __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Re
     {
         return closest;
     }
-    const Real inverse_a = Real(1) / length_squared(r.direction);
-    const vec3<Real> inv{safe_inverse(r.direction.x), safe_inverse(r.direction.y), safe_inverse(r.direction.z)};
-    const vec3<Real> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
-    const slab_order so = slab_order_of(inv);
+    walk_ray<Real> w;
+    w.set(r);
     int size = 0;
+    int floor = 0;                       // in an instance: the scene tree's part of the stack
+    int triangle_shift = sc.triangle_base; // in an instance: its triangles' id_base less its first triangle
     int32_t ref = sc.root_ref;
     while (true)
     {
         if (ref < 0)
         {
-            closest_in_leaf<Real, AllKinds>(sc, r, ref, from, inward, inverse_a, t_min, closest, tl);
+            if constexpr (Kinds == walk_instances)
+            {
+                if (leaf_kind(ref) == prim_instance)
+                {
+                    const instance_rec<Real>& in = sc.instances[leaf_first(ref)];
+                    tl.add(tally_instance);
+                    w.set(into_instance(in.to_object, r));
+                    floor = size;
+                    triangle_shift = in.id_base - in.first_triangle;
+                    ref = in.root_ref;
+                    continue;
+                }
+            }
+            closest_in_leaf<Real, Kinds>(sc, w.r, ref, from, inward, w.inverse_a, t_min, triangle_shift, closest, tl);
         }
-        else if (descend(sc, ref, so, inv, os, t_min, closest.t, stack, size, tl))
+        else if (descend(sc, ref, w.so, w.inv, w.os, t_min, closest.t, stack, size, tl))
         {
             continue;
         }
-        if (!pop(stack, size, closest.t, ref))
+        if (pop(stack, size, closest.t, ref, floor))
+        {
+            continue;
+        }
+        if constexpr (Kinds == walk_instances)
+        {
+            if (floor == 0)
+            {
+                return closest; // (an instance entered on an empty stack leaves nothing of the scene tree to walk)
+            }
+            // the mesh tree is done: back to the world's coordinates and the scene tree's entries
+            w.set(r);
+            floor = 0;
+            triangle_shift = sc.triangle_base;
+            if (!pop(stack, size, closest.t, ref))
+            {
+                return closest;
+            }
+        }
+        else
         {
             return closest;
         }

@@ -565,10 +565,11 @@ __device__ inline bool draw_light(const quality_view& q, vec3<float> x, pcg32& g
     return true;
 }
 // The density with which the light set would have drawn the lamp primitive `id` in unit_dir from x
-__device__ inline float lamp_hit_density(const quality_view& q, int id, vec3<float> x, vec3<float> unit_dir,
+__device__ inline float lamp_hit_density(const scene_view<float>& sc, int id, vec3<float> x, vec3<float> unit_dir,
                                          float distance)
 {
-    const int k = q.lamp_count > 0 ? q.lamp_of[id] : -1;
+    const quality_view& q = sc.quality;
+    const int k = q.lamp_count > 0 && id < sc.instanced_base ? q.lamp_of[id] : -1; // (an instanced mesh lights nothing)
     return k >= 0 ? q.light_pmf * lamp_density(q.lamps[k], x, unit_dir, distance) : 0.0f;
 }
 
@@ -578,7 +579,7 @@ __device__ inline float lamp_hit_density(const quality_view& q, int id, vec3<flo
 // root), any other primitive it leaves cannot. Intervals open at both ends (src/gpu/shadow.cuh:26-120).
 template <typename Tally>
 __device__ inline bool leaf_blocks(const scene_view<float>& sc, const ray<float>& r, int32_t ref, float inverse_a,
-                                   float t_max, int from, bool inward, int skip, Tally& tl)
+                                   float t_max, int from, bool inward, int skip, int triangle_shift, Tally& tl)
 {
     constexpr float t_min = 0.001f;
     const int kind = leaf_kind(ref);
@@ -603,7 +604,7 @@ __device__ inline bool leaf_blocks(const scene_view<float>& sc, const ray<float>
         }
         else if (kind == prim_triangle)
         {
-            const int id = sc.triangle_base + i;
+            const int id = triangle_shift + i;
             if (id == from || id == skip)
             {
                 continue;
@@ -633,7 +634,7 @@ __device__ inline bool leaf_blocks(const scene_view<float>& sc, const ray<float>
     }
     return false;
 }
-template <typename Tally, typename Stack>
+template <bool Instances, typename Tally, typename Stack>
 __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r, float t_max, int from, bool inward,
                                 int skip, const Stack& stack, Tally& tl)
 {
@@ -641,17 +642,31 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
     {
         return false;
     }
-    const float inverse_a = 1.0f / length_squared(r.direction);
-    const vec3<float> inv{safe_inverse(r.direction.x), safe_inverse(r.direction.y), safe_inverse(r.direction.z)};
-    const vec3<float> os{r.origin.x * inv.x, r.origin.y * inv.y, r.origin.z * inv.z};
-    const slab_order so = slab_order_of(inv);
+    walk_ray<float> w;
+    w.set(r);
     int depth = 0;
+    int floor = 0; // in an instance's mesh: the scene tree's part of the stack
+    int triangle_shift = sc.triangle_base;
     int32_t ref = sc.root_ref;
     while (true)
     {
         if (ref < 0)
         {
-            if (leaf_blocks(sc, r, ref, inverse_a, t_max, from, inward, skip, tl))
+            if constexpr (Instances)
+            {
+                if (leaf_kind(ref) == prim_instance)
+                {
+                    // into the instance's mesh, as closest_hit goes
+                    const instance_rec<float>& in = sc.instances[leaf_first(ref)];
+                    tl.add(tally_instance);
+                    w.set(into_instance(in.to_object, r));
+                    floor = depth;
+                    triangle_shift = in.id_base - in.first_triangle;
+                    ref = in.root_ref;
+                    continue;
+                }
+            }
+            if (leaf_blocks(sc, w.r, ref, w.inverse_a, t_max, from, inward, skip, triangle_shift, tl))
             {
                 return true;
             }
@@ -661,7 +676,7 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
             // every entered child must be looked at, in any order: the first goes next, the rest on the stack
             tl.add(tally_node);
             float t[wide_children];
-            child_entries(sc, ref, so, inv, os, 0.001f, t_max, t);
+            child_entries(sc, ref, w.so, w.inv, w.os, 0.001f, t_max, t);
             const int4 refs = child_refs(sc, ref);
             const int32_t child[wide_children] = {refs.x, refs.y, refs.z, refs.w};
             int32_t next = no_child;
@@ -681,6 +696,16 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
             {
                 ref = next;
                 continue;
+            }
+        }
+        if constexpr (Instances)
+        {
+            if (depth == floor && floor > 0)
+            {
+                // the mesh is done: back to the world's coordinates and the scene tree's entries
+                w.set(r);
+                floor = 0;
+                triangle_shift = sc.triangle_base;
             }
         }
         if (depth == 0)
@@ -757,7 +782,7 @@ __device__ inline bool light_term(const scene_view<float>& sc, const scatterer& 
 // adds its light, weighed against the light set's chance to have drawn it; any other vertex takes a light sample
 // (when the path may still trace a ray, the lobe isn't a delta and a light exists), traces its shadow ray, then
 // scatters
-template <typename Tally, typename Stack>
+template <bool Instances, typename Tally, typename Stack>
 __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
                                      const hit<float>& h, pcg32& g, vec3<float>& sample, const Stack& stack,
                                      Tally& tl, uint32_t& shadow_rays)
@@ -777,7 +802,7 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
         sample = q.radiance + w * (p.throughput * le);
         return true;
     }
-    const vertex<float> vx = make_vertex(sc, p.r, h, tl);
+    const vertex<float> vx = make_vertex<Instances>(sc, p.r, h, tl);
     const material_rec<float> m = sc.materials[vx.material];
     // the color is read once: matte, smoke and a lamp's front face have one (a coat's base is the matte color)
     const bool lamp = m.kind == RT_MATERIAL_DIFFUSE_LIGHT;
@@ -792,7 +817,7 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
             tl.add(tally_lamp);
             const float distance = h.t * sqrtf(length_squared(p.r.direction));
             w = mix.scattered_hit(q.bsdf_pdf,
-                                  lamp_hit_density(qv, h.index, p.r.origin, unit_vector(p.r.direction), distance));
+                                  lamp_hit_density(sc, h.index, p.r.origin, unit_vector(p.r.direction), distance));
         }
         sample = q.radiance + w * (p.throughput * le);
         return true;
@@ -811,7 +836,7 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
         ++shadow_rays;
         const ray<float> shadow{vx.p, ld.wi, p.r.time};
         const bool inward = dot(ld.wi, vx.outward) < 0.0f;
-        if (!occluded(sc, shadow, ld.distance, vx.from, inward, ld.skip, stack, tl))
+        if (!occluded<Instances>(sc, shadow, ld.distance, vx.from, inward, ld.skip, stack, tl))
         {
             q.radiance = q.radiance + transmittance(sc, shadow, ld.distance, tl) * term;
         }
@@ -835,12 +860,14 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
     return false;
 }
 
-template <typename Tally, typename Stack>
+// Instances: the scene's tree has instance leaves (a two-level tree); a kernel of its own
+template <bool Instances, typename Tally, typename Stack>
 __device__ inline bool trace_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
                                      pcg32& g, vec3<float>& sample, const Stack& stack, Tally& tl,
                                      uint32_t& shadow_rays)
 {
-    const hit<float> h = closest_event(sc, q.base.r, q.base.from, q.base.inward, g, stack, tl);
-    return shade_quality(q, sc, cam, h, g, sample, stack, tl, shadow_rays);
+    const hit<float> h = closest_event<float, Instances ? walk_instances : walk_all>(sc, q.base.r, q.base.from,
+                                                                                   q.base.inward, g, stack, tl);
+    return shade_quality<Instances>(q, sc, cam, h, g, sample, stack, tl, shadow_rays);
 }
 }

@@ -232,7 +232,7 @@ __device__ inline int hit_class(const scene_view<float>& sc, int index)
     {
         return wf_class_miss;
     }
-    const int m = index >= sc.media_base ? sc.media[index - sc.media_base].material : sc.prim_materials[index];
+    const int m = index >= sc.media_base ? sc.media[index - sc.media_base].material : material_of(sc, index);
     if (sc.quality.microfacets)
     {
         const int side = sc.quality.microfacets[m].kind;
@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> 
         const int4 hv = pool.hit[s];
         const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
         tally<false> tl;
-        done = shade_quality(q, sc, cam, h, sp.g, sample, stack, tl, shadow_rays);
+        done = shade_quality<true>(q, sc, cam, h, sp.g, sample, stack, tl, shadow_rays);
         if (!done)
         {
             sp.p = q.base;
@@ -565,7 +565,7 @@ __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, con
             bool done;
             if constexpr (Mode == mode_quality)
             {
-                done = trace_quality(q, sc, cam, sp.g, sample, stack, tl, shadow_rays);
+                done = trace_quality<true>(q, sc, cam, sp.g, sample, stack, tl, shadow_rays);
             }
             else if constexpr (Mode == mode_all_books)
             {

@@ -94,10 +94,17 @@ int64_t server::submit_view(int64_t scene_id, const rt_camera_desc& camera, uint
     r->camera = camera;
     r->seed = seed;
     r->passes = passes;
-    if (view_device_bytes(*f->second.scene, camera, options_.passes_per_step) + f->second.bytes >
-        options_.memory_budget_bytes)
+    // what the resident scenes hold stays held while requests may use them: a request that doesn't fit beside them
+    // could never be admitted
+    std::size_t resident = 0;
+    for (const auto& [id, sc] : scenes_)
     {
-        throw std::invalid_argument("the request needs more device memory than the server's whole budget");
+        resident += sc.bytes;
+    }
+    if (view_device_bytes(*f->second.scene, camera, options_.passes_per_step) + resident > options_.memory_budget_bytes)
+    {
+        throw std::invalid_argument("the request needs more device memory than the server's budget leaves beside "
+                                    "the resident scenes");
     }
     r->st.id = r->id;
     r->st.state = 0;

@@ -539,7 +539,7 @@ public:
         }
         else if (current_.plain)
         {
-            a = plain_megakernel_attributes<Real>(mode_, depth);
+            a = plain_megakernel_attributes<Real>(mode_, depth, dscene_.view().instance_count > 0);
             const char* name =
                 mode_ == dev::mode_quality ? "quality" : (mode_ == dev::mode_all_books ? "all books" : "first book");
             info.threads_per_block = plain_block_x * plain_block_y;
@@ -548,7 +548,8 @@ public:
         }
         else
         {
-            a = megakernel_attributes<Real>(mode_, current_.block, depth, current_.waves);
+            a = megakernel_attributes<Real>(mode_, current_.block, depth, current_.waves,
+                                            dscene_.view().instance_count > 0);
             const char* name =
                 mode_ == dev::mode_quality ? "quality" : (mode_ == dev::mode_all_books ? "all books" : "first book");
             info.threads_per_block = current_.block;
@@ -616,6 +617,10 @@ public:
             if (n <= 0)
             {
                 return;
+            }
+            if (dscene_.view().triangle_base != dscene_.view().media_base)
+            {
+                throw std::invalid_argument("the reference-walk probe walks trees of spheres and quads only");
             }
             const auto& nodes = dscene_.tree().nodes;
             static_assert(sizeof(ref_node<float>) == 32, "the probe reads the reference's 32-byte node");
@@ -686,7 +691,8 @@ public:
             sizeof(dev::quad_rec<Real>), sizeof(dev::medium_rec<Real>), sizeof(dev::quad_rec<Real>),
             4.0 + sizeof(dev::material_rec<Real>), sizeof(dev::sphere_frame<Real>), sizeof(dev::texture_rec<Real>),
             sizeof(dev::texel), 4.0 * sizeof(Real) + 3.0, sizeof(dev::light_rec<Real>),
-            sizeof(dev::triangle_rec<Real>), sizeof(dev::emitter_rec), sizeof(dev::env_texel), sizeof(dev::alias_bin)};
+            sizeof(dev::triangle_rec<Real>), sizeof(dev::emitter_rec), sizeof(dev::env_texel), sizeof(dev::alias_bin),
+            sizeof(dev::instance_rec<Real>)};
         double bytes = 0;
         for (int e = 0; e < RT_TRAFFIC_EVENTS; ++e)
         {
@@ -1062,7 +1068,7 @@ void check_options(const rt_session_options& options)
     {
         throw std::invalid_argument("the block size must be 64, 128, 192 or 256 threads");
     }
-    if (options.tree < RT_TREE_DEFAULT || options.tree > RT_TREE_SAH)
+    if (options.tree < RT_TREE_DEFAULT || options.tree > RT_TREE_TWO_LEVEL)
     {
         throw std::invalid_argument("unknown tree");
     }

@@ -112,7 +112,7 @@ def test_surface_area_tree_holds_every_primitive(rt, name):
 def test_unknown_tree_is_refused(rt):
     s = rt.build_scene("weekend", width=32, samples=1)
     with pytest.raises(ValueError, match="tree"):
-        rt.Session(s, tree=3)
+        rt.Session(s, tree=4)
     with pytest.raises(ValueError, match="tree"):
         rt.render(s, tree=-1)
     with pytest.raises(ValueError, match="tree"):

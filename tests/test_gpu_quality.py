@@ -234,3 +234,48 @@ def test_split_stack_equals_wavefront(rt, waves):
         assert m.counters()[:2] == counts
     mism = np.count_nonzero(got.view(np.uint64) != reference.view(np.uint64))
     assert mism == 0, f"{mism} of {got.size} sums differ"
+
+
+def test_instanced_meshes_find_the_baked_hits(rt):
+    """Config 3's two blobs, placed 32 times each, walked through their instances (each mesh's tree once, in its own
+    coordinates: RT_TREE_TWO_LEVEL) against every instance baked into world coordinates (the default tree): the same
+    rays hit and miss alike, at the same distances to float rounding, on the same sides; and renders of both agree by
+    ImageCompare"""
+    scene = rt.build_scene("triangles", width=96, samples=16, sampling="mis")
+    cam = scene.desc.camera
+    rng = np.random.default_rng(123)
+    n = 30000
+    center = np.array([cam.center.x, cam.center.y, cam.center.z])
+    p00 = np.array([cam.pixel00_loc.x, cam.pixel00_loc.y, cam.pixel00_loc.z])
+    du = np.array([cam.pixel_delta_u.x, cam.pixel_delta_u.y, cam.pixel_delta_u.z])
+    dv = np.array([cam.pixel_delta_v.x, cam.pixel_delta_v.y, cam.pixel_delta_v.z])
+    targets = p00 + rng.uniform(0, cam.image_width, n)[:, None] * du + rng.uniform(0, cam.image_height, n)[:, None] * dv
+    o = np.repeat(center[None], n, axis=0).astype(np.float32)
+    d = (targets - center).astype(np.float32)
+    t0 = np.zeros(n, np.float32)
+    none = np.full(n, -1, np.int32)
+    out = np.zeros(n, np.uint8)
+    hits = {}
+    for tree in ("two_level", "default"):
+        with rt.Session(scene, integrator="megakernel", tree=tree) as s:
+            hits[tree] = s.closest_hit(o, d, t0, none, out)
+    (ia, ta, ea), (ib, tb, eb) = hits["two_level"], hits["default"]
+    both = (ia >= 0) & (ib >= 0)
+    assert both.mean() > 0.5
+    assert np.count_nonzero((ia >= 0) != (ib >= 0)) <= 3
+    rel = np.abs(ta[both] - tb[both]) / tb[both]
+    assert np.quantile(rel, 0.999) < 1e-5 and rel.max() < 1e-3, f"t differs: max {rel.max()}"
+    assert np.count_nonzero(ea[both] != eb[both]) <= 3
+    print(f"{both.sum()} rays hit both; {np.count_nonzero(ia[both] != ib[both])} by other ids (instanced triangles)")
+    verdicts = []
+    for seeds in ((41, 42), (43, 44)):
+        imgs = []
+        for tree, seed in zip(("two_level", "default"), seeds):
+            with rt.Session(scene, seed=seed, integrator="megakernel", tree=tree) as s:
+                s.render_to(16)
+                imgs.append(s.pixels())
+        v = _consistent(imgs[0], imgs[1], "triangles two-level vs baked")
+        verdicts.append(v)
+        if v.consistent:
+            return
+    raise AssertionError("two-level and baked trees differ beyond noise twice: " + " | ".join(v.summary() for v in verdicts))

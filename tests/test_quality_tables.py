@@ -167,12 +167,24 @@ def test_triangle_scene_tree(rt):
     d = s.desc
     tris = sum(d.meshes[d.instances[k].mesh].triangle_count for k in range(d.instance_count))
     assert tris == 20 * 4 ** 8 + 64 * 20 * 4 ** 5
-    boxes, ints, _, _, depth = rt.scene_bvh(s)
+    # one level (the default): every instance's triangles baked into the scene's tree
+    boxes, ints, _, _, depth = rt.scene_bvh(s, "default")
     leaf = ints[:, 1] != 0
     counts = ints[leaf, 1] & 255
     kinds = ints[leaf, 1] >> 8
     assert counts[kinds == 3].sum() == tris and counts[kinds == 2].sum() == 1
     assert counts.max() <= 8 and depth <= 64
+    # two levels: the icosphere, placed once, baked; the two blobs, placed 32 times each, one leaf per instance
+    # (kind 4, one instance a leaf)
+    uses = np.bincount([d.instances[k].mesh for k in range(d.instance_count)], minlength=d.mesh_count)
+    baked = sum(d.meshes[m].triangle_count for m in range(d.mesh_count) if uses[m] == 1)
+    assert baked == 20 * 4 ** 8 and sorted(uses) == [1, 32, 32]
+    b2, i2, _, _, depth2 = rt.scene_bvh(s, "two_level")
+    leaf2 = i2[:, 1] != 0
+    counts2, kinds2 = i2[leaf2, 1] & 255, i2[leaf2, 1] >> 8
+    assert counts2[kinds2 == 3].sum() == baked and counts2[kinds2 == 2].sum() == 1
+    assert np.all(counts2[kinds2 == 4] == 1) and np.count_nonzero(kinds2 == 4) == 64
+    assert len(b2) < len(boxes) and depth2 <= depth
     # every inner node's box holds both children's
     inner = np.nonzero(~leaf)[0]
     first, second = inner + 1, ints[inner, 0]
