@@ -259,32 +259,29 @@ __device__ inline sphere<Real> moving_at(const moving_sphere_rec<Real>& m, Real 
             m.radius};
 }
 
-// The ray against a quad, closed [t_min, t_max] (src/gpu/quad.cuh:56-84)
+// The ray against a quad, closed [t_min, t_max] (src/gpu/quad.cuh:56-84). Every test is evaluated and the answer
+// taken at the end (as in hit_triangle): the lanes of a wave meet different primitives, so early exits saved little
+// arithmetic and cost a mask save and restore each. t and front_face are written on a hit only.
 template <typename Real>
 __device__ inline bool hit_quad(const quad_rec<Real>& s, const ray<Real>& r, Real t_min, Real t_max, Real& t,
                                 bool& front_face)
 {
     const vec3<Real> normal = load3(s.normal);
     const Real denominator = dot(normal, r.direction);
-    if (!(m_fabs(denominator) >= Real(1e-8)))
-    {
-        return false;
-    }
     const vec3<Real> otq = load3(s.q) - r.origin;
-    t = dot(normal, otq) / denominator;
-    if (!(t_min <= t && t <= t_max))
-    {
-        return false;
-    }
-    const vec3<Real> p{m_fma(t, r.direction.x, -otq.x), m_fma(t, r.direction.y, -otq.y), m_fma(t, r.direction.z, -otq.z)};
+    const Real at = dot(normal, otq) / denominator;
+    const vec3<Real> p{m_fma(at, r.direction.x, -otq.x), m_fma(at, r.direction.y, -otq.y),
+                       m_fma(at, r.direction.z, -otq.z)};
     const Real alpha = dot(p, load3(s.alpha_axis));
     const Real beta = dot(p, load3(s.beta_axis));
-    if (!(Real(0) <= alpha && alpha <= Real(1) && Real(0) <= beta && beta <= Real(1)))
+    const bool met = (m_fabs(denominator) >= Real(1e-8)) & (t_min <= at) & (at <= t_max) & (Real(0) <= alpha) &
+                     (alpha <= Real(1)) & (Real(0) <= beta) & (beta <= Real(1));
+    if (met)
     {
-        return false;
+        t = at;
+        front_face = denominator < Real(0);
     }
-    front_face = denominator < Real(0);
-    return true;
+    return met;
 }
 template <typename Real>
 __device__ inline void quad_uv(const quad_rec<Real>& s, const ray<Real>& r, Real t, Real& u, Real& v)
@@ -306,43 +303,36 @@ __device__ inline bool hit_triangle(const triangle_rec<Real>& tri, const ray<Rea
     const vec3<Real> d = r.direction;
     const vec3<Real> pv{d.y * e2.z - d.z * e2.y, d.z * e2.x - d.x * e2.z, d.x * e2.y - d.y * e2.x};
     const Real det = dot(e1, pv);
-    if (!(m_fabs(det) > Real(1e-30)))
-    {
-        return false;
-    }
     const Real inv = Real(1) / det;
     const vec3<Real> s = r.origin - load3(tri.v0);
     const Real u = dot(s, pv) * inv;
-    if (!(u >= Real(0) && u <= Real(1)))
-    {
-        return false;
-    }
     const vec3<Real> qv{s.y * e1.z - s.z * e1.y, s.z * e1.x - s.x * e1.z, s.x * e1.y - s.y * e1.x};
     const Real v = dot(d, qv) * inv;
-    if (!(v >= Real(0) && u + v <= Real(1)))
+    const Real at = dot(e2, qv) * inv;
+    // every test evaluated, the answer taken at the end (no early exits: see hit_quad)
+    const bool met = (m_fabs(det) > Real(1e-30)) & (u >= Real(0)) & (u <= Real(1)) & (v >= Real(0)) &
+                     (u + v <= Real(1)) & (t_min <= at) & (at <= t_max);
+    if (met)
     {
-        return false;
+        t = at;
+        front_face = det > Real(0);
+        if (bary)
+        {
+            bary[0] = u;
+            bary[1] = v;
+        }
     }
-    t = dot(e2, qv) * inv;
-    if (!(t_min <= t && t <= t_max))
-    {
-        return false;
-    }
-    front_face = det > Real(0);
-    if (bary)
-    {
-        bary[0] = u;
-        bary[1] = v;
-    }
-    return true;
+    return met;
 }
 
+// (entering as a 32-bit value: a bool carried through the walk's loops is a lane mask, merged with three scalar
+// instructions at every join of the leaf loops' branches; a 32-bit value is a vector register and takes a select)
 template <typename Real>
 struct hit
 {
     int index;
     Real t;
-    bool entering;
+    uint32_t entering;
 };
 
 // What a path reads from the scene, counted per event for the algorithmic-bytes measurement (an instantiation of the
@@ -492,7 +482,12 @@ __device__ inline int32_t rank_of(const scene_view<Real>& sc, int id)
 template <int Kinds, typename Real>
 __device__ inline bool better(const scene_view<Real>& sc, int id, Real t, const hit<Real>& closest)
 {
-    return t < closest.t || (closest.index >= 0 && rank_of<Kinds>(sc, id) < rank_of<Kinds>(sc, closest.index));
+    bool take = t < closest.t;
+    if (!take && closest.index >= 0) // (one branch for the rare tie)
+    {
+        take = rank_of<Kinds>(sc, id) < rank_of<Kinds>(sc, closest.index);
+    }
+    return take;
 }
 
 // The primitives of a leaf against the ray, the closest hit so far updated (src/gpu/bvh.cuh:612-680). A triangle's id
