@@ -213,14 +213,11 @@ __device__ inline bool sphere_roots(const sphere<Real>& s, const ray<Real>& r, R
     const Real h = dot(r.direction, oc) * inverse_a;
     const vec3<Real> l{m_fma(-h, r.direction.x, oc.x), m_fma(-h, r.direction.y, oc.y), m_fma(-h, r.direction.z, oc.z)};
     const Real disc = m_fma(-l.x, l.x, m_fma(-l.y, l.y, m_fma(-l.z, l.z, s.radius * s.radius)));
-    if (disc < Real(0))
-    {
-        return false;
-    }
-    const Real half_chord = m_sqrt(disc * inverse_a);
+    // (the roots computed on every lane, a miss's from 0: no branch, the answer taken by the caller)
+    const Real half_chord = m_sqrt(m_max(disc, Real(0)) * inverse_a);
     near = h - half_chord;
     far = h + half_chord;
-    return true;
+    return disc >= Real(0);
 }
 
 // The nearest root in (t_min, t_max), the far one only for a ray leaving this sphere inward (src/gpu/sphere.cuh:99-117)
@@ -231,17 +228,16 @@ __device__ inline bool hit_sphere(const sphere<Real>& s, const ray<Real>& r, Rea
                                   bool far_only, Real& t, bool& entering, bool closed = false)
 {
     Real near, far;
-    if (!sphere_roots(s, r, inverse_a, near, far))
+    const bool real = sphere_roots(s, r, inverse_a, near, far);
+    const bool from_outside = !far_only && near > t_min;
+    const Real at = from_outside ? near : far;
+    const bool met = real & (t_min < at) & ((at < t_max) | (closed & (at == t_max)));
+    if (met)
     {
-        return false;
+        t = at;
+        entering = from_outside;
     }
-    t = near;
-    entering = !far_only && t > t_min;
-    if (!entering)
-    {
-        t = far;
-    }
-    return t_min < t && (t < t_max || (closed && t == t_max));
+    return met;
 }
 
 template <typename Real>
