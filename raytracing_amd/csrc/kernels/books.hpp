@@ -864,13 +864,15 @@ __device__ inline bool trace_all_books(path<Real>& p, const scene_view<Real>& sc
                                 tl);
 }
 
-// One ray of a first-book path (spheres that stand still, solid matte, mirrors, glass, the sky): the same lobes with
-// only what those scenes need, so the kernel carries no texture, smoke or light code
-template <typename Real, typename Tally, typename Stack>
-__device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
-                                        pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
+// What the hit h does to a first-book path (spheres that stand still, solid matte, mirrors, glass, the sky): the same
+// lobes with only what those scenes need, so the kernel carries no texture, smoke or light code. The matte lobe is the
+// first book's scatter, which always goes on (src/gpu/material.cuh: lambertian), where the third book's strategies
+// end a path whose direction falls below the surface (diffuse_direction): the two differ on a rare draw, so every
+// integrator shades a first-book scene with this function.
+template <typename Real, typename Tally>
+__device__ inline bool first_book_after_walk(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
+                                             pcg32& g, vec3<Real>& sample, const hit<Real>& h, Tally& tl)
 {
-    const hit<Real> h = closest_hit<Real, walk_spheres>(sc, p.r, p.from, p.inward, stack, tl);
     if (h.index < 0)
     {
         return end_path(p, sky(cam, p.r.direction), sample);
@@ -903,6 +905,13 @@ __device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& s
         break;
     }
     return continue_path(p, vx, d, sample);
+}
+template <typename Real, typename Tally, typename Stack>
+__device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
+                                        pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
+{
+    return first_book_after_walk(p, sc, cam, g, sample,
+                                 closest_hit<Real, walk_spheres>(sc, p.r, p.from, p.inward, stack, tl), tl);
 }
 
 // A NaN channel of a sample counts as 0 (src/gpu/renderer.cu:311-316)

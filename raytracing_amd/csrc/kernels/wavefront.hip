@@ -449,6 +449,8 @@ __device__ inline void after_shade(const wf_pool& pool, int p, uint32_t s, bool 
     block_append<1>(valid && done ? 0 : -1, s, pool.counters + wf_counter_free, pool.free, pool.segment);
 }
 
+// FirstBook: the scene is a first book's (first_book_after_walk, the megakernel's shading of such scenes)
+template <bool FirstBook>
 __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, const scene_view<float> sc,
                                                 wf_pool pool, int p)
 {
@@ -470,7 +472,11 @@ __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, con
         const int4 hv = pool.hit[s];
         const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
         tally<false> tl;
-        if (h.index < 0)
+        if constexpr (FirstBook)
+        {
+            done = first_book_after_walk(sp.p, sc, cam, sp.g, sample, h, tl);
+        }
+        else if (h.index < 0)
         {
             sample = sp.p.throughput * sky(cam, sp.p.r.direction);
             done = true;
@@ -642,7 +648,7 @@ hipError_t wf_launch_init(const dev::wf_pool& pool, uint32_t items, hipStream_t 
 }
 
 hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc,
-                            const dev::wf_pool& pool, int parity, bool quality, hipStream_t stream)
+                            const dev::wf_pool& pool, int parity, int mode, hipStream_t stream)
 {
     const dim3 grid(wf_blocks(pool.capacity));
     const size_t lds = stack_lds_bytes<float>(sc.stack_depth, dev::wf_block);
@@ -650,13 +656,17 @@ hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_
     hipLaunchKernelGGL(dev::wf_refill, grid, dim3(dev::wf_block), 0, stream, cam, sc, pool, parity);
     hipLaunchKernelGGL(dev::wf_extend, grid, dim3(dev::wf_block), lds, stream, sc, pool, parity);
     hipLaunchKernelGGL(dev::wf_sort, dim3(1), dim3(64), 0, stream, pool, parity);
-    if (quality)
+    if (mode == dev::mode_quality)
     {
         hipLaunchKernelGGL(dev::wf_shade_quality, grid, dim3(dev::wf_block), lds, stream, cam, sc, pool, parity);
     }
+    else if (mode == dev::mode_first_book)
+    {
+        hipLaunchKernelGGL(dev::wf_shade<true>, grid, dim3(dev::wf_block), 0, stream, cam, sc, pool, parity);
+    }
     else
     {
-        hipLaunchKernelGGL(dev::wf_shade, grid, dim3(dev::wf_block), 0, stream, cam, sc, pool, parity);
+        hipLaunchKernelGGL(dev::wf_shade<false>, grid, dim3(dev::wf_block), 0, stream, cam, sc, pool, parity);
     }
     return hipGetLastError();
 }
@@ -699,7 +709,7 @@ kernel_attributes wf_attributes(int depth, int which)
 {
     hipFuncAttributes a{};
     const void* f = which == 0   ? reinterpret_cast<const void*>(&dev::wf_extend)
-                    : which == 1 ? reinterpret_cast<const void*>(&dev::wf_shade)
+                    : which == 1 ? reinterpret_cast<const void*>(&dev::wf_shade<false>)
                                  : reinterpret_cast<const void*>(&dev::wf_shade_quality);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)

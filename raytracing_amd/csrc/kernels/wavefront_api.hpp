@@ -95,9 +95,10 @@ inline uint32_t wf_blocks(uint32_t capacity) { return (capacity + dev::wf_block 
 // A launch's start: `items` work items (passes x pixels of work), every slot free, no slot active
 hipError_t wf_launch_init(const dev::wf_pool& pool, uint32_t items, hipStream_t stream);
 // One bounce of parity `parity`: plan (refills, the active list's prefix sums), refill, extend, the queues' prefix
-// sums, shade. The launch's seed and work are read from pool.launch.
+// sums, shade (mode: the scene's kernel_mode, whose shading the megakernel's equals). The launch's seed and work are
+// read from pool.launch.
 hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc,
-                            const dev::wf_pool& pool, int parity, bool quality, hipStream_t stream);
+                            const dev::wf_pool& pool, int parity, int mode, hipStream_t stream);
 // The drain: the slots of the parity-0 active list finish their paths, a thread each (mode: the scene's kernel_mode;
 // count: at least the list's length). Only when every work item has been handed out.
 hipError_t wf_launch_drain(int mode, const dev::camera_rec<float>& cam, const dev::scene_view<float>& sc, uint64_t seed,

@@ -116,7 +116,7 @@ hipError_t wavefront<Real>::launch_bounces()
     hipError_t launched = hipSuccess;
     for (int b = 0; b < bounces_per_graph_ && launched == hipSuccess; ++b)
     {
-        launched = wf_launch_bounce(cam_, sc_, pool_, b & 1, quality_, stream_);
+        launched = wf_launch_bounce(cam_, sc_, pool_, b & 1, mode_, stream_);
     }
     return launched;
 }

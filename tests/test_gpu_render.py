@@ -230,6 +230,22 @@ def test_wavefront_equals_megakernel(rt, name):
     assert mism == 0, f"{mism} of {sa.size} sums differ; max |diff| {np.abs(sa - sb).max()}"
 
 
+@pytest.mark.parametrize("name,width,spp,seed", [("weekend", 1080, 47, 500), ("glass", 600, 64, 7)])
+def test_wavefront_equals_megakernel_at_size(rt, name, width, spp, seed):
+    """Many paths (weekend: 55 M, where the two once parted on 5 pixels: a first-book scene's matte lobe, which always
+    goes on, shaded by the wavefront with the third book's rule, which ends a path drawn below the surface); the same
+    sums bit for bit"""
+    scene = rt.build_scene(name, width=width, samples=1)
+    with rt.Session(scene, seed=seed, integrator="megakernel") as a:
+        a.render_to(spp)
+        sa = a.sums()
+    with rt.Session(scene, seed=seed, integrator="wavefront") as b:
+        b.render_to(spp)
+        sb = b.sums()
+    mism = np.count_nonzero(np.any(sa.view(np.uint64) != sb.view(np.uint64), axis=-1))
+    assert mism == 0, f"{mism} pixels differ"
+
+
 @pytest.mark.parametrize("name", ["weekend", "final", "cornell"])
 @pytest.mark.parametrize("drain,graph", [(-1, 0), (1 << 30, 0), (0, -1), (5000, 2)])
 def test_wavefront_drain_and_graph_equal(rt, name, drain, graph):
