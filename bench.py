@@ -381,8 +381,10 @@ def main():
         cpu = cpu_baseline(name, width, 64 if name == "final" else 16)
         print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
-    hbm = measured_traffic(args.scene, traffic_key(kernel))
-    issue = measured_issue(args.scene, traffic_key(kernel))
+    # (the profiles' lines name the two-level tree's workload apart: its kernels and traffic are its own)
+    profiled = args.scene + ("-two-level" if args.tree == "two_level" else "")
+    hbm = measured_traffic(profiled, traffic_key(kernel))
+    issue = measured_issue(profiled, traffic_key(kernel))
     plain = others.get("plain")
     line = {
         "metric": "Mpaths/s (1080^2 final scene)" if args.scene == "final" else f"Mpaths/s ({args.scene})",
