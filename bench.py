@@ -295,6 +295,9 @@ def main():
                     help="the BVH's splits: the library's default (SAH, one level), the reference's object median, "
                          "the SAH, or the SAH in two levels (meshes placed several times walked through their "
                          "instances)")
+    ap.add_argument("--nodes", default="auto", choices=["auto", "float", "packed"],
+                    help="the BVH's inner nodes: the library's choice (packed for the quality integrator's trees "
+                         "beyond 16 MiB of float nodes), 128-byte float nodes, or 64-byte packed nodes")
     ap.add_argument("--serving", action="store_true", help="config 4: the serving loop under Poisson request streams")
     ap.add_argument("--rates", default="1,4,16,64")
     ap.add_argument("--requests", type=int, default=16, help="requests per rate (serving)")
@@ -346,6 +349,7 @@ def main():
                                 ("wavefront_slots", args.wavefront_slots),
                                 ("megakernel_waves", args.megakernel_waves), ("block_size", args.block_size)) if v}
     tuning["tree"] = args.tree
+    tuning["nodes"] = args.nodes
     value, ms, avg_launch, launches_per_step, session = measure(rt, scene, args.integrator, args.steps, args.warmup,
                                                                 **tuning)
     kernel = session.kernel().name.decode()
@@ -371,7 +375,7 @@ def main():
             if other == args.integrator:
                 others[other] = round(value, 2)
                 continue
-            ov, _, _, _, os_ = measure(rt, scene, other, 1, 1, tree=args.tree)
+            ov, _, _, _, os_ = measure(rt, scene, other, 1, 1, tree=args.tree, nodes=args.nodes)
             os_.close()
             others[other] = round(ov, 2)
             print(f"{other}: {ov:.1f} Mpaths/s", file=sys.stderr, flush=True)
@@ -399,7 +403,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": description, "integrator": args.integrator, "tree": args.tree, "kernel": kernel,
+        "config": {"workload": description, "integrator": args.integrator, "tree": args.tree, "nodes": args.nodes,
+                   "kernel": kernel,
                    "paths_per_step": scene.width * scene.height * scene.passes,
                    "rays_per_path": round(traffic.rays / traffic.paths, 3),
                    "shadow_rays_per_path": round(traffic.shadow_rays / traffic.paths, 3),

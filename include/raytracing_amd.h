@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 1
+#define RT_ABI_VERSION 2
 
 enum rt_status
 {
@@ -345,7 +345,22 @@ typedef struct rt_session_options
     int32_t megakernel_waves;/* megakernel: the waves per SIMD its registers are budgeted for: 0 (the compiler's
                                choice), 5, 6 or 7 (float only) */
     int32_t tree;           /* rt_tree: how the session's BVH is split */
+    int32_t nodes;          /* rt_nodes: how its inner nodes lie in memory */
 } rt_session_options;
+
+/* How the BVH's inner nodes lie in memory. Both hold the same tree and give the same closest hits (bit for bit): a
+ * packed box holds its float box, so the walk enters every child it entered before. */
+enum rt_nodes
+{
+    RT_NODES_AUTO = 0,   /* packed for the quality integrator's one-level trees whose float nodes pass 16 MiB (half
+                            the L2 caches' 32 MiB), float otherwise and for the plain megakernel (RT_INTEGRATOR_PLAIN
+                            renders faster on float nodes) */
+    RT_NODES_FLOAT = 1,  /* 128 bytes a node: four children's boxes as floats */
+    RT_NODES_PACKED = 2, /* 64 bytes a node: each child's planes as bytes on a grid of power-of-two steps over the
+                            node's box, rounded outward (no counterpart in the reference, whose nodes are the binary
+                            tree's 32-byte records, src/gpu/bvh.cuh:29-38). Float32 scenes that render on the quality
+                            integrator with a one-level tree; otherwise RT_ERROR_INVALID_ARGUMENT */
+};
 
 /* The BVH's splits. Every tree holds every primitive and finds the same closest hit on scenes without instanced
  * meshes (of two primitives hit at exactly the same distance, the one of lower index in the scene is kept with
@@ -491,6 +506,14 @@ int rt_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* pas
  * learn the counts. */
 int rt_scene_bvh(const rt_scene* scene, int32_t tree, float* node_boxes, int32_t* node_ints, int64_t* node_count,
                  int32_t* sphere_ids, int32_t* quad_ids, int32_t* depth);
+
+/* The four-wide tree the kernels walk for a scene in float32 with the splits of `tree` (an rt_tree), on the host (no
+ * GPU needed): per inner node its 128-byte record of float planes (lo[3][4] then hi[3][4] as floats, then the four
+ * children's references as int32: a node's index, a leaf reference with bit 31 set, or 0x7fffffff for no child) and
+ * the same node packed into 64 bytes as RT_NODES_PACKED lays it out (origin[3], step[3] as floats, lo[3][4] and hi[3][4]
+ * as bytes, the four references). Call with NULL buffers to learn the count. No counterpart in the reference (its
+ * nodes are the binary tree's, rt_scene_bvh); for tests of the packing. */
+int rt_scene_wide_nodes(const rt_scene* scene, int32_t tree, void* float_nodes, void* packed_nodes, int64_t* count);
 
 /* The quality integrator's light set as the kernels read it, on the host (build_lamps, src/gpu/lamps.cuh:159 and
  * src/gpu/lamp_table.cu:60-119): the scene's glowing quads and still spheres, 80 bytes each (20 floats: corner[3],

@@ -128,7 +128,7 @@ class SessionOptions(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("precision", C.c_int32), ("integrator", C.c_int32),
                 ("launch_seconds", C.c_double), ("block_size", C.c_int32), ("wavefront_slots", C.c_int32),
                 ("wavefront_graph", C.c_int32), ("wavefront_drain", C.c_int32), ("megakernel_waves", C.c_int32),
-                ("tree", C.c_int32)]
+                ("tree", C.c_int32), ("nodes", C.c_int32)]
 
 
 class RenderReport(C.Structure):
@@ -213,6 +213,7 @@ _SIGNATURES = {
     "rt_sample_numbers": (C.c_int, [C.c_uint64, _P(C.c_uint64), _P(C.c_uint64), C.c_int64, C.c_int32,
                                     _P(C.c_uint32), _P(C.c_float), _P(C.c_double), _P(C.c_float)]),
     "rt_session_traffic": (C.c_int, [C.c_void_p, C.c_int64, _P(Traffic)]),
+    "rt_scene_wide_nodes": (C.c_int, [_P(Scene), C.c_int32, C.c_void_p, C.c_void_p, _P(C.c_int64)]),
     "rt_scene_bvh": (C.c_int, [_P(Scene), C.c_int32, _P(C.c_float), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32),
                                _P(C.c_int32), _P(C.c_int32)]),
     "rt_scene_lamps": (C.c_int, [_P(Scene), _P(C.c_float), _P(C.c_int64)]),
@@ -243,6 +244,7 @@ SAMPLING = {"uniform": 0, "material": 1, "lights": 2, "mixture": 3, "nee": 4, "m
 MATERIAL_KINDS = {"lambertian": 0, "metal": 1, "dielectric": 2, "diffuse_light": 3, "isotropic": 4}
 INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2, "plain": 3}
 TREES = {"default": 0, "median": 1, "sah": 2, "two_level": 3}
+NODES = {"auto": 0, "float": 1, "packed": 2}
 TONE_MAPS = {"books": 0, "none": 1, "agx": 2, "neutral": 3}
 
 
@@ -306,10 +308,11 @@ def build_scene(name: str, **overrides) -> BuiltScene:
 
 def _session_options(seed=0, precision="float32", integrator="auto", launch_seconds=0.0, block_size=0,
                      wavefront_slots=0, wavefront_graph=0, wavefront_drain=0, megakernel_waves=0,
-                     tree="default") -> SessionOptions:
+                     tree="default", nodes="auto") -> SessionOptions:
     return SessionOptions(seed, 1 if precision in ("float64", "fp64") else 0, INTEGRATORS[integrator],
                           launch_seconds, block_size, wavefront_slots, wavefront_graph, wavefront_drain,
-                          megakernel_waves, TREES[tree] if isinstance(tree, str) else int(tree))
+                          megakernel_waves, TREES[tree] if isinstance(tree, str) else int(tree),
+                          NODES[nodes] if isinstance(nodes, str) else int(nodes))
 
 
 @dataclass
@@ -478,6 +481,26 @@ def scene_bvh(scene: BuiltScene, tree: str = "median"):
                              sphere_ids.ctypes.data_as(_P(C.c_int32)), quad_ids.ctypes.data_as(_P(C.c_int32)),
                              C.byref(depth)))
     return boxes, ints, sphere_ids, quad_ids, depth.value
+
+
+WIDE_NODE = np.dtype([("lo", np.float32, (3, 4)), ("hi", np.float32, (3, 4)), ("ref", np.int32, 4),
+                      ("unused", np.int32, 4)])
+PACKED_NODE = np.dtype([("origin", np.float32, 3), ("step", np.float32, 3), ("lo", np.uint8, (3, 4)),
+                        ("hi", np.uint8, (3, 4)), ("ref", np.int32, 4)])
+
+
+def scene_wide_nodes(scene: BuiltScene, tree: str = "default"):
+    """The four-wide tree the kernels walk (float32, host only): its 128-byte float nodes and the same nodes packed
+    into 64 bytes as RT_NODES_PACKED lays them out, as numpy structured arrays (WIDE_NODE, PACKED_NODE)."""
+    count = C.c_int64()
+    kind = TREES[tree] if isinstance(tree, str) else int(tree)
+    _check(_lib.rt_scene_wide_nodes(C.byref(scene.desc), kind, None, None, C.byref(count)))
+    wide = np.zeros(count.value, WIDE_NODE)
+    packed = np.zeros(count.value, PACKED_NODE)
+    assert WIDE_NODE.itemsize == 128 and PACKED_NODE.itemsize == 64
+    _check(_lib.rt_scene_wide_nodes(C.byref(scene.desc), kind, wide.ctypes.data, packed.ctypes.data,
+                                    C.byref(count)))
+    return wide, packed
 
 
 LAMP_FIELDS = 20  # floats per lamp record: corner[3], size, u[3], kind, v[3], primitive, normal[3], 0, radiance[3], 0

@@ -845,6 +845,94 @@ dev::quad_rec<Real> make_quad_rec(const rt_quad_desc& d)
 
 int leaf_size_for(std::size_t shapes) { return shapes <= 8 ? 8 : 1; }
 
+std::vector<dev::packed_node> pack_nodes(const std::vector<dev::wide_node<float>>& wide)
+{
+    std::vector<dev::packed_node> out(wide.size());
+    for (std::size_t n = 0; n < wide.size(); ++n)
+    {
+        const dev::wide_node<float>& w = wide[n];
+        dev::packed_node& p = out[n];
+        for (int a = 0; a < 3; ++a)
+        {
+            // the node's box on this axis: the union of its children's
+            double lo = std::numeric_limits<double>::infinity();
+            double hi = -lo;
+            for (int c = 0; c < dev::wide_children; ++c)
+            {
+                if (w.ref[c] != dev::no_child)
+                {
+                    lo = std::min(lo, static_cast<double>(w.lo[a][c]));
+                    hi = std::max(hi, static_cast<double>(w.hi[a][c]));
+                }
+            }
+            if (!(lo <= hi) || !std::isfinite(lo) || !std::isfinite(hi))
+            {
+                throw std::runtime_error("packing the BVH: a node without a finite box");
+            }
+            // the step: the smallest power of two that spans the box in at most 254 steps (one more for the corner's
+            // rounding down), and no finer than the last place of the box's largest coordinate over 2^22: then every
+            // grid point corner + q step (q <= 255) is a whole multiple of the step below 2^24 of them, a float, and
+            // the kernels' fma(q, step, corner) gives it exactly
+            const double magnitude = std::max(std::fabs(lo), std::fabs(hi));
+            int k = magnitude > 0 ? std::ilogb(magnitude) - 22 : -126;
+            if (hi > lo)
+            {
+                k = std::max(k, std::ilogb((hi - lo) / 254.0));
+                while (std::ldexp(254.0, k) < hi - lo)
+                {
+                    ++k;
+                }
+            }
+            k = std::max(k, -126);
+            const double step = std::ldexp(1.0, k);
+            // the corner: the multiple of the step at or below the box
+            const double corner = std::floor(lo / step) * step;
+            if (static_cast<double>(static_cast<float>(corner)) != corner ||
+                std::fabs(corner / step) + 255 >= 16777216.0)
+            {
+                throw std::runtime_error("packing the BVH: a node's grid isn't made of floats");
+            }
+            p.origin[a] = static_cast<float>(corner);
+            p.step[a] = static_cast<float>(step);
+            for (int c = 0; c < dev::wide_children; ++c)
+            {
+                if (w.ref[c] == dev::no_child)
+                {
+                    p.lo[a][c] = 255;
+                    p.hi[a][c] = 0;
+                    continue;
+                }
+                // each plane rounded outward to the grid: the nearest grid points at or beyond it (the difference in
+                // double may round when a plane is far finer than the step, as a box's rounded-out zero is: the
+                // grid point is then moved on until it holds the plane, compared exactly)
+                const double lo_c = w.lo[a][c];
+                const double hi_c = w.hi[a][c];
+                double q_lo = std::floor((lo_c - corner) / step);
+                double q_hi = std::ceil((hi_c - corner) / step);
+                while (corner + q_lo * step > lo_c)
+                {
+                    q_lo -= 1;
+                }
+                while (corner + q_hi * step < hi_c)
+                {
+                    q_hi += 1;
+                }
+                if (q_lo < 0 || q_hi > 255 || q_lo > q_hi)
+                {
+                    throw std::runtime_error("packing the BVH: a child's box leaves its node's grid");
+                }
+                p.lo[a][c] = static_cast<uint8_t>(q_lo);
+                p.hi[a][c] = static_cast<uint8_t>(q_hi);
+            }
+        }
+        for (int c = 0; c < dev::wide_children; ++c)
+        {
+            p.ref[c] = w.ref[c];
+        }
+    }
+    return out;
+}
+
 std::vector<bool> instanced_meshes(const rt_scene& scene, bool two_level)
 {
     std::vector<int> uses(static_cast<std::size_t>(std::max<int64_t>(scene.mesh_count, 0)), 0);

@@ -48,6 +48,11 @@ struct bvh_arrays
     std::vector<int32_t> quad_ids;
 };
 
+// The four-wide tree's inner nodes packed (dev::packed_node), node for node: each child's planes on its node's grid, the
+// lower planes rounded down and the upper ones up with a whole step to spare, so that every packed box holds the float
+// box it replaces. Throws if a node's coordinates leave no such grid (never for finite boxes).
+std::vector<dev::packed_node> pack_nodes(const std::vector<dev::wide_node<float>>& wide);
+
 // The leaf size the reference uses: one primitive per leaf, or one leaf of up to 8 for a scene of at most 8 shapes
 // (src/gpu/renderer.cu:75-82)
 int leaf_size_for(std::size_t shapes);

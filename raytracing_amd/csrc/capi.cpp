@@ -373,6 +373,27 @@ int rt_scene_bvh(const rt_scene* scene, int32_t kind, float* node_boxes, int32_t
     });
 }
 
+int rt_scene_wide_nodes(const rt_scene* scene, int32_t kind, void* float_nodes, void* packed_nodes, int64_t* count)
+{
+    return guarded([&] {
+        need(scene, "scene");
+        need(count, "count");
+        rt::validate_scene(*scene);
+        const auto tree = rt::build_bvh<float>(
+            *scene, rt::leaf_size_for(static_cast<size_t>(scene->sphere_count + scene->quad_count)), kind);
+        *count = static_cast<int64_t>(tree.wide.size());
+        if (float_nodes)
+        {
+            std::memcpy(float_nodes, tree.wide.data(), tree.wide.size() * sizeof(rt::dev::wide_node<float>));
+        }
+        if (packed_nodes)
+        {
+            const std::vector<rt::dev::packed_node> packed = rt::pack_nodes(tree.wide);
+            std::memcpy(packed_nodes, packed.data(), packed.size() * sizeof(rt::dev::packed_node));
+        }
+    });
+}
+
 int rt_scene_lamps(const rt_scene* scene, float* lamps, int64_t* count)
 {
     return guarded([&] {

@@ -376,7 +376,7 @@ dev::camera_rec<Real> to_camera(const rt_camera_desc& d)
 }
 
 template <typename Real>
-device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
+device_scene<Real>::device_scene(const rt_scene& s, int32_t tree, int32_t nodes)
     : tree_(build_bvh<Real>(s, leaf_size_for(static_cast<std::size_t>(s.sphere_count + s.quad_count)), tree))
 {
     // materials: a texture of one color becomes the material's albedo (src/gpu/renderer.cu:548-601)
@@ -607,7 +607,29 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree)
         bytes_ += bytes_of(lamps_) + bytes_of(lamp_of_) + bytes_of(microfacets_) + bytes_of(env_texels_) +
                   bytes_of(env_bins_) + bytes_of(env_row_cos_);
     }
+    // packed nodes for the quality integrator's one-level trees larger than the L2 caches (or when asked): half the
+    // bytes a walk reads per node, and fewer registers to hold a node's planes
+    if (nodes != RT_NODES_AUTO && nodes != RT_NODES_FLOAT && nodes != RT_NODES_PACKED)
+    {
+        throw std::invalid_argument("unknown node layout");
+    }
+    // (a tree whose root is a leaf has no inner node to pack: its walks read none)
+    const bool packable = sizeof(Real) == sizeof(float) && quality_ && tree_.instances.empty();
+    if (nodes == RT_NODES_PACKED && !packable)
+    {
+        throw std::invalid_argument("packed nodes are for float32 scenes on the quality integrator with a one-level tree");
+    }
+    constexpr std::size_t packed_above = std::size_t{16} << 20;
+    if (packable && (nodes == RT_NODES_PACKED || (nodes == RT_NODES_AUTO && nodes_.bytes() > packed_above)))
+    {
+        if constexpr (sizeof(Real) == sizeof(float))
+        {
+            packed_ = device_array<dev::packed_node>(pack_nodes(tree_.wide));
+            bytes_ += bytes_of(packed_);
+        }
+    }
     v.nodes = nodes_.data();
+    v.packed_nodes = packed_.size() > 0 ? packed_.data() : nullptr;
     v.root_ref = tree_.root_ref;
     v.stack_depth = tree_.nodes.empty() ? 0 : tree_.stack_entries;
     v.spheres = spheres_.data();

@@ -89,7 +89,8 @@ template <typename Real>
 class device_scene
 {
 public:
-    device_scene(const rt_scene& scene, int32_t tree); // tree: an rt_tree
+    // tree: an rt_tree; nodes: an rt_nodes (where the quality integrator's kernels walk packed nodes)
+    device_scene(const rt_scene& scene, int32_t tree, int32_t nodes = RT_NODES_AUTO);
     const dev::scene_view<Real>& view() const { return view_; }
     const bvh_arrays<Real>& tree() const { return tree_; }
     std::size_t bytes() const { return bytes_; }
@@ -99,6 +100,7 @@ public:
 private:
     bvh_arrays<Real> tree_;
     device_array<dev::wide_node<Real>> nodes_;
+    device_array<dev::packed_node> packed_; // the same tree packed, for the quality integrator's walks (or empty)
     device_array<dev::sphere_rec<Real>> spheres_;
     device_array<dev::moving_sphere_rec<Real>> moving_;
     device_array<dev::quad_rec<Real>> quads_;

@@ -66,6 +66,25 @@ static_assert(offsetof(wide_node<float>, hi) == 12 * sizeof(float) && offsetof(w
 static_assert(offsetof(wide_node<double>, hi) == 12 * sizeof(double) &&
               offsetof(wide_node<double>, ref) == 24 * sizeof(double));
 
+// The same inner node in 64 bytes, for trees larger than the caches (a packed tree): each child's box on a grid of
+// 8-bit steps over the node's own box, one power-of-two step per axis, the grid's corner a multiple of the step, so
+// that every plane origin + q x step is a float. The host rounds every lower plane down and every upper one up: the
+// packed box holds the float one, and since the slab test is the same multiply-add on either box's planes and
+// monotonic in them, the walk enters every child it entered before, at no farther a distance (the closest hit is the
+// same primitive at the same distance: the primitives' tests don't change). A missing child has no_child as its
+// reference and lo 255, hi 0. Half the bytes of wide_node<float>, read as four dwordx4 loads from one 64-byte
+// half-line; a plane is unpacked by a v_cvt_f32_ubyte and an exact multiply-add.
+struct alignas(64) packed_node
+{
+    float origin[3];            // bytes 0-11
+    float step[3];              // bytes 12-23: powers of two
+    uint8_t lo[3][wide_children]; // bytes 24-35: lo[axis][child]
+    uint8_t hi[3][wide_children]; // bytes 36-47
+    int32_t ref[wide_children]; // bytes 48-63
+};
+static_assert(sizeof(packed_node) == 64 && offsetof(packed_node, lo) == 24 && offsetof(packed_node, hi) == 36 &&
+              offsetof(packed_node, ref) == 48);
+
 template <typename Real>
 struct alignas(16) sphere_rec
 {
@@ -305,6 +324,7 @@ struct scene_view
 {
     quality_view quality;
     const wide_node<Real>* nodes;
+    const packed_node* packed_nodes; // the same tree packed (null unless the scene's kernels walk it: walk_packed)
     int32_t root_ref;  // the root: an inner node's index (0), a leaf reference, or -1 with no primitive at all
     int32_t stack_depth; // the most entries the traversal stack can hold (at least 1), or 0 with no primitive at all
     const sphere_rec<Real>* spheres;
@@ -334,6 +354,25 @@ struct scene_view
     int32_t sampling; // rt_sampling_strategy
     int32_t motion;   // some sphere or medium boundary moves: camera rays draw a time
 };
+
+// What a walk may meet: the first book's trees hold spheres that stand still only; the books' add moving spheres and
+// quads; the quality integrator's add triangles, and in a two-level tree instances of meshes (a walk that can enter
+// an instance carries the world ray beside the moved one: a kernel of its own, so that the others don't hold it); a
+// packed tree (packed_node) is walked by kernels of its own too
+enum walk_kinds : int
+{
+    walk_spheres = 0,
+    walk_books = 1,
+    walk_all = 2,
+    walk_instances = 3,
+    walk_packed = 4, // walk_all's kinds in a packed tree (float only)
+};
+// The walk a scene's quality kernels (and the parity probe) take: into instances, over packed nodes, or plain
+template <typename Real>
+__host__ __device__ inline int tree_walk(const scene_view<Real>& sc)
+{
+    return sc.instance_count > 0 ? walk_instances : sc.packed_nodes != nullptr ? walk_packed : walk_all;
+}
 
 // Which integrator a scene's kernels run (src/gpu/renderer.cu:1011-1023, src/gpu/quality.cu:53-109): the first book's
 // (spheres that stand still, no texture, the material strategy), the three books', or the quality integrator's (light

@@ -55,7 +55,7 @@ hipError_t launch_plain_megakernel(int mode, const dev::camera_rec<Real>& cam, c
                                    uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays,
                                    hipStream_t stream);
 template <typename Real>
-kernel_attributes plain_megakernel_attributes(int mode, int depth, bool instanced = false);
+kernel_attributes plain_megakernel_attributes(int mode, int depth, int walk = dev::walk_all);
 
 template <typename Real>
 hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origins, const Real* directions,
@@ -83,7 +83,8 @@ hipError_t launch_display(const double* sums, int64_t pixels, double inverse_pas
 
 hipError_t launch_copy(const void* in, void* out, size_t bytes, hipStream_t stream);
 
-// instanced: the kernels for a two-level tree (their walks can enter instances)
+// walk: the quality kernels' walk of the scene's tree (dev::tree_walk: a two-level tree's walks can enter instances, a
+// packed tree's read packed nodes)
 template <typename Real>
-kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves = 0, bool instanced = false);
+kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves = 0, int walk = dev::walk_all);
 }

@@ -266,8 +266,10 @@ struct far_stack
 };
 
 // The quality integrator's paths (src/gpu/quality.cu:53-109): the same loop over the queue's items, with light
-// samples and their shadow rays at every vertex that isn't a mirror's (quality.hpp)
-template <bool Count, bool Instances, typename Stack>
+// samples and their shadow rays at every vertex that isn't a mirror's (quality.hpp). One ray per iteration: a path's
+// shadow ray is its own iteration, walked by the same code as the paths' next rays (step_quality), so a wave's lanes
+// walk together whichever kind of ray each traces, and the shading's values aren't held across a second walk.
+template <bool Count, int Walk, typename Stack>
 __device__ inline void quality_paths(staged_sample<float>* __restrict__ staging,
                                      unsigned long long* __restrict__ ray_count,
                                      unsigned long long* __restrict__ tallies, const Stack& stack)
@@ -303,9 +305,8 @@ __device__ inline void quality_paths(staged_sample<float>* __restrict__ staging,
             }
             continue;
         }
-        ++rays;
         vec3<float> sample;
-        if (trace_quality<Instances>(qp, sc, cam, g, sample, stack, tl, shadow_rays))
+        if (step_quality<Walk>(qp, sc, cam, g, sample, stack, tl, rays, shadow_rays))
         {
             stage(staging + at, sample);
             at = -1;
@@ -316,7 +317,8 @@ __device__ inline void quality_paths(staged_sample<float>* __restrict__ staging,
 
 // Split: the stack's first far.near levels in LDS, the rest in HBM (a deep tree, such as a large mesh's, would
 // otherwise hold so much LDS per wave that fewer waves fit than the registers allow)
-template <bool Count, int Waves, bool Split, bool Instances>
+// Walk: the tree's walk (walk_all, walk_instances or walk_packed)
+template <bool Count, int Waves, bool Split, int Walk>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
 quality_megakernel(const megakernel_args<float> args, staged_sample<float>* __restrict__ staging,
                    unsigned long long* __restrict__ ray_count, unsigned long long* __restrict__ tallies,
@@ -326,11 +328,11 @@ quality_megakernel(const megakernel_args<float> args, staged_sample<float>* __re
     {
         const uint32_t column = blockIdx.x * blockDim.x + threadIdx.x;
         const split_stack stack{block_stack(far.near), far.near, far.refs + column, far.ts + column, far.stride};
-        quality_paths<Count, Instances>(staging, ray_count, tallies, stack);
+        quality_paths<Count, Walk>(staging, ray_count, tallies, stack);
     }
     else
     {
-        quality_paths<Count, Instances>(staging, ray_count, tallies, block_stack(args.sc.stack_depth));
+        quality_paths<Count, Walk>(staging, ray_count, tallies, block_stack(args.sc.stack_depth));
     }
 }
 
@@ -347,7 +349,7 @@ struct plain_args
     launch_work work;
     uint64_t seed;
 };
-template <typename Real, int Mode, bool Instances>
+template <typename Real, int Mode, int Walk>
 __global__ void __launch_bounds__(128) plain_megakernel(const plain_args<Real> args, double* __restrict__ sums,
                                                         unsigned long long* __restrict__ ray_count)
 {
@@ -378,7 +380,7 @@ __global__ void __launch_bounds__(128) plain_megakernel(const plain_args<Real> a
                 do
                 {
                     ++rays;
-                } while (!trace_quality<Instances>(qp, sc, cam, g, sample, stack, tl, shadow_rays));
+                } while (!trace_quality<Walk>(qp, sc, cam, g, sample, stack, tl, shadow_rays));
             }
             else
             {
@@ -400,8 +402,9 @@ __global__ void __launch_bounds__(128) plain_megakernel(const plain_args<Real> a
     add_counts<false>(rays, shadow_rays, ray_count, nullptr, tl);
 }
 
-// The parity probe: closest_hit on a batch of given rays, with the traversal of the all-books kernel
-template <typename Real>
+// The parity probe: closest_hit on a batch of given rays, with the walk of the scene's kernels (Kinds: walk_instances,
+// which walks any one-level or two-level tree, or walk_packed for a packed tree)
+template <typename Real, int Kinds>
 __global__ void __launch_bounds__(256) closest_hit_kernel(const scene_view<Real> sc, const Real* origins,
                                                           const Real* directions, const Real* times,
                                                           const int32_t* from, const uint8_t* inward, int64_t n,
@@ -417,7 +420,7 @@ __global__ void __launch_bounds__(256) closest_hit_kernel(const scene_view<Real>
                       {directions[3 * k], directions[3 * k + 1], directions[3 * k + 2]},
                       times[k]};
     tally<false> tl;
-    const hit<Real> h = closest_hit<Real, walk_instances>(sc, r, from[k], inward[k] != 0, stack, tl);
+    const hit<Real> h = closest_hit<Real, Kinds>(sc, r, from[k], inward[k] != 0, stack, tl);
     index[k] = h.index;
     t[k] = h.t;
     entering[k] = h.entering ? 1 : 0;
@@ -497,20 +500,24 @@ void* static_scratch(size_t* bytes)
 
 namespace
 {
+template <bool Count, int Waves, int Walk>
+const void* quality_megakernel_for(bool split)
+{
+    return split ? reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, true, Walk>)
+                 : reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, false, Walk>);
+}
+
+// walk: the quality kernels' walk of the scene's tree (dev::tree_walk)
 template <typename Real, bool Count, int Waves>
-const void* megakernel_for_waves(int mode, bool split, bool instanced)
+const void* megakernel_for_waves(int mode, bool split, int walk)
 {
     if constexpr (sizeof(Real) == sizeof(float))
     {
-        if (mode == dev::mode_quality && instanced)
-        {
-            return split ? reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, true, true>)
-                         : reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, false, true>);
-        }
         if (mode == dev::mode_quality)
         {
-            return split ? reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, true, false>)
-                         : reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, false, false>);
+            return walk == dev::walk_instances ? quality_megakernel_for<Count, Waves, dev::walk_instances>(split)
+                   : walk == dev::walk_packed  ? quality_megakernel_for<Count, Waves, dev::walk_packed>(split)
+                                               : quality_megakernel_for<Count, Waves, dev::walk_all>(split);
         }
     }
     return mode == dev::mode_all_books ? reinterpret_cast<const void*>(&dev::megakernel<Real, true, Count, Waves>)
@@ -520,24 +527,24 @@ const void* megakernel_for_waves(int mode, bool split, bool instanced)
 // The instance for a mode and an occupancy target: the counting instances and the FP64 ones at the compiler's choice.
 // split: the quality megakernel with its stack's deeper levels in HBM
 template <typename Real, bool Count>
-const void* megakernel_for(int mode, int waves, bool split, bool instanced)
+const void* megakernel_for(int mode, int waves, bool split, int walk)
 {
     if constexpr (!Count && sizeof(Real) == sizeof(float))
     {
         if (waves == 5)
         {
-            return megakernel_for_waves<Real, Count, 5>(mode, split, instanced);
+            return megakernel_for_waves<Real, Count, 5>(mode, split, walk);
         }
         if (waves == 6)
         {
-            return megakernel_for_waves<Real, Count, 6>(mode, split, instanced);
+            return megakernel_for_waves<Real, Count, 6>(mode, split, walk);
         }
         if (waves == 7)
         {
-            return megakernel_for_waves<Real, Count, 7>(mode, split, instanced);
+            return megakernel_for_waves<Real, Count, 7>(mode, split, walk);
         }
     }
-    return megakernel_for_waves<Real, Count, 0>(mode, split, instanced);
+    return megakernel_for_waves<Real, Count, 0>(mode, split, walk);
 }
 
 // The stack levels a split quality megakernel keeps in LDS: as many as let the waves its registers allow share a CU's
@@ -604,9 +611,9 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
         return hipErrorInvalidValue;
     }
     size_t lds = stack_lds_bytes<Real>(sc.stack_depth, block_threads);
-    const bool instanced = sc.instance_count > 0;
-    const void* f = tallies ? megakernel_for<Real, true>(mode, 0, false, instanced)
-                            : megakernel_for<Real, false>(mode, waves, false, instanced);
+    const int walk = dev::tree_walk(sc);
+    const void* f = tallies ? megakernel_for<Real, true>(mode, 0, false, walk)
+                            : megakernel_for<Real, false>(mode, waves, false, walk);
     // as many blocks as fit the chip at once
     int device = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&device);
@@ -627,7 +634,7 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
     dev::far_stack far{sc.stack_depth, nullptr, nullptr, 0};
     if (!tallies && mode == dev::mode_quality && far_buffer != nullptr && sc.stack_depth > 0)
     {
-        const void* split = megakernel_for<Real, false>(mode, waves, true, instanced);
+        const void* split = megakernel_for<Real, false>(mode, waves, true, walk);
         const int near = near_levels(split, sc.stack_depth, block_threads, device);
         const size_t split_lds = stack_lds_bytes<Real>(near, block_threads);
         int split_per_cu = 0;
@@ -689,19 +696,22 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
 namespace
 {
 template <typename Real>
-const void* plain_megakernel_for(int mode, bool instanced)
+const void* plain_megakernel_for(int mode, int walk)
 {
     if constexpr (sizeof(Real) == sizeof(float))
     {
         if (mode == dev::mode_quality)
         {
-            return instanced ? reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, true>)
-                             : reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, false>);
+            return walk == dev::walk_instances
+                       ? reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, dev::walk_instances>)
+                   : walk == dev::walk_packed
+                       ? reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, dev::walk_packed>)
+                       : reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, dev::walk_all>);
         }
     }
     return mode == dev::mode_all_books
-               ? reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_all_books, false>)
-               : reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_first_book, false>);
+               ? reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_all_books, dev::walk_books>)
+               : reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_first_book, dev::walk_spheres>);
 }
 }
 
@@ -720,14 +730,14 @@ hipError_t launch_plain_megakernel(int mode, const dev::camera_rec<Real>& cam, c
     const size_t lds = stack_lds_bytes<Real>(sc.stack_depth, plain_block_x * plain_block_y);
     dev::plain_args<Real> a{cam, sc, work, seed};
     void* args[] = {&a, &sums, &rays};
-    return hipLaunchKernel(plain_megakernel_for<Real>(mode, sc.instance_count > 0), grid, block, args, lds, stream);
+    return hipLaunchKernel(plain_megakernel_for<Real>(mode, dev::tree_walk(sc)), grid, block, args, lds, stream);
 }
 
 template <typename Real>
-kernel_attributes plain_megakernel_attributes(int mode, int depth, bool instanced)
+kernel_attributes plain_megakernel_attributes(int mode, int depth, int walk)
 {
     hipFuncAttributes a{};
-    const void* f = plain_megakernel_for<Real>(mode, instanced);
+    const void* f = plain_megakernel_for<Real>(mode, walk);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)
     {
@@ -750,8 +760,8 @@ template hipError_t launch_plain_megakernel<float>(int, const dev::camera_rec<fl
 template hipError_t launch_plain_megakernel<double>(int, const dev::camera_rec<double>&,
                                                     const dev::scene_view<double>&, uint64_t, const dev::launch_work&,
                                                     double*, unsigned long long*, hipStream_t);
-template kernel_attributes plain_megakernel_attributes<float>(int, int, bool);
-template kernel_attributes plain_megakernel_attributes<double>(int, int, bool);
+template kernel_attributes plain_megakernel_attributes<float>(int, int, int);
+template kernel_attributes plain_megakernel_attributes<double>(int, int, int);
 
 template <typename Real>
 hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origins, const Real* directions,
@@ -761,8 +771,17 @@ hipError_t launch_closest_hit(const dev::scene_view<Real>& sc, const Real* origi
     const int threads = 256;
     const dim3 grid(static_cast<unsigned>((n + threads - 1) / threads));
     const size_t lds = stack_lds_bytes<Real>(sc.stack_depth, threads);
-    hipLaunchKernelGGL((dev::closest_hit_kernel<Real>), grid, dim3(threads), lds, stream, sc, origins, directions,
-                       times, from, inward, n, index, t, entering);
+    if constexpr (sizeof(Real) == sizeof(float))
+    {
+        if (dev::tree_walk(sc) == dev::walk_packed)
+        {
+            hipLaunchKernelGGL((dev::closest_hit_kernel<float, dev::walk_packed>), grid, dim3(threads), lds, stream, sc,
+                               origins, directions, times, from, inward, n, index, t, entering);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((dev::closest_hit_kernel<Real, dev::walk_instances>), grid, dim3(threads), lds, stream, sc,
+                       origins, directions, times, from, inward, n, index, t, entering);
     return hipGetLastError();
 }
 
@@ -775,10 +794,10 @@ hipError_t launch_display(const double* sums, int64_t pixels, double inverse_pas
 }
 
 template <typename Real>
-kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves, bool instanced)
+kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves, int walk)
 {
     hipFuncAttributes a{};
-    const void* f = megakernel_for<Real, false>(mode, waves, false, instanced);
+    const void* f = megakernel_for<Real, false>(mode, waves, false, walk);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)
     {
@@ -805,8 +824,8 @@ template hipError_t launch_megakernel<double>(int, const dev::camera_rec<double>
 template hipError_t launch_closest_hit<float>(const dev::scene_view<float>&, const float*, const float*, const float*,
                                               const int32_t*, const uint8_t*, int64_t, int32_t*, float*, uint8_t*,
                                               hipStream_t);
-template kernel_attributes megakernel_attributes<float>(int, int, int, int, bool);
-template kernel_attributes megakernel_attributes<double>(int, int, int, int, bool);
+template kernel_attributes megakernel_attributes<float>(int, int, int, int, int);
+template kernel_attributes megakernel_attributes<double>(int, int, int, int, int);
 template size_t stack_lds_bytes<float>(int, int);
 template size_t stack_lds_bytes<double>(int, int);
 }

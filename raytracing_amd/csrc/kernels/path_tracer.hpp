@@ -456,16 +456,6 @@ __device__ inline lds_stack stack_in(unsigned char* lds, int entries, int thread
 // yet, is no hit: the ranks are read only once a closest hit exists.)
 // (An instanced mesh's triangles rank by their ids, which follow every other surface's rank: only a walk that can
 // enter an instance meets them.)
-// What a walk may meet: the first book's trees hold spheres that stand still only; the books' add moving spheres and
-// quads; the quality integrator's add triangles, and in a two-level tree instances of meshes (a walk that can enter
-// an instance carries the world ray beside the moved one: a kernel of its own, so that the others don't hold it)
-enum walk_kinds : int
-{
-    walk_spheres = 0,
-    walk_books = 1,
-    walk_all = 2,
-    walk_instances = 3,
-};
 template <int Kinds, typename Real>
 __device__ inline int32_t rank_of(const scene_view<Real>& sc, int id)
 {
@@ -489,11 +479,16 @@ __device__ inline bool better(const scene_view<Real>& sc, int id, Real t, const 
 // The primitives of a leaf against the ray, the closest hit so far updated (src/gpu/bvh.cuh:612-680). A triangle's id
 // is its index in the triangle array plus triangle_shift (the world's triangles: triangle_base; an instance's: its
 // id_base less where its mesh's triangles start).
-template <typename Real, int Kinds, typename Tally>
+// Shadows: the walk may also answer a shadow ray's question (any_hit, a per-lane flag): is any primitive but `skip` hit
+// in (t_min, t_max)? Then the first hit found is the answer, and spheres are tested on the open interval as the
+// reference's shadow rays test them (src/gpu/shadow.cuh:26-120); quads and triangles keep their closed test.
+template <typename Real, int Kinds, bool Shadows = false, typename Tally>
 __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Real>& r, int32_t ref, int from,
                                        bool inward, Real inverse_a, Real t_min, int triangle_shift, hit<Real>& closest,
-                                       Tally& tl)
+                                       Tally& tl, int skip = -1, bool any_hit = false)
 {
+    // (a shadow ray takes any hit; a closest-hit walk's ties go by the scene's order)
+    const auto take = [&](int id, Real t) { return (Shadows && any_hit) || better<Kinds>(sc, id, t, closest); };
     const int kind = leaf_kind(ref);
     const int first = leaf_first(ref);
     const int end = first + leaf_count(ref);
@@ -502,7 +497,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
         for (int i = first; i < end; ++i)
         {
             const int id = triangle_shift + i;
-            if (id == from)
+            if (id == from || (Shadows && id == skip))
             {
                 continue; // a ray never meets the flat face it leaves
             }
@@ -510,7 +505,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
             tl.add(tally_triangle);
             Real t;
             bool front;
-            if (hit_triangle(tri, r, t_min, closest.t, t, front) && better<Kinds>(sc, id, t, closest))
+            if (hit_triangle(tri, r, t_min, closest.t, t, front) && take(id, t))
             {
                 closest = {id, t, front};
             }
@@ -521,7 +516,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
         for (int i = first; i < end; ++i)
         {
             const int id = sc.quad_base + i;
-            if (id == from)
+            if (id == from || (Shadows && id == skip))
             {
                 continue;
             }
@@ -529,7 +524,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
             tl.add(tally_quad);
             Real t;
             bool front;
-            if (hit_quad(face, r, t_min, closest.t, t, front) && better<Kinds>(sc, id, t, closest))
+            if (hit_quad(face, r, t_min, closest.t, t, front) && take(id, t))
             {
                 closest = {id, t, front};
             }
@@ -542,7 +537,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
         {
             const int id = moving ? sc.moving_base + i : i;
             const bool own = id == from;
-            if (own && !inward)
+            if ((own && !inward) || (Shadows && id == skip))
             {
                 continue;
             }
@@ -550,7 +545,7 @@ __device__ inline void closest_in_leaf(const scene_view<Real>& sc, const ray<Rea
             tl.add(moving ? tally_moving : tally_sphere);
             Real t;
             bool entering;
-            if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering, true) && better<Kinds>(sc, id, t, closest))
+            if (hit_sphere(s, r, inverse_a, t_min, closest.t, own, t, entering, !(Shadows && any_hit)) && take(id, t))
             {
                 closest = {id, t, entering};
             }
@@ -601,6 +596,46 @@ __device__ inline void child_entries(const scene_view<Real>& sc, int32_t ref, co
     }
 }
 
+// Where the ray enters each child of a packed node (packed_node), or infinity, and the children's references: the four
+// 16-byte words of the node's 64 bytes loaded at once, each plane unpacked from its byte to its float, exactly (corner
+// + q x step is a float: a multiple of the power-of-two step), then the slab test of child_entries on those planes: the
+// same multiply-add, monotonic in the plane, so a packed box's lower planes give distances no larger than its float
+// box's and its upper planes no smaller, and the walk enters every child it enters on float nodes, at no farther a
+// distance. The ray's slab order picks the near planes (the lower ones where the inverse direction is positive). A
+// missing child (no_child) is never entered.
+__device__ inline void packed_entries(const scene_view<float>& sc, int32_t ref, vec3<float> inv, vec3<float> os,
+                                      float t_min, float t_max, float (&t)[wide_children], int4& refs)
+{
+    const uint4* const node = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(sc.packed_nodes) +
+                                                             static_cast<uint32_t>(ref) * uint32_t{sizeof(packed_node)});
+    const uint4 w0 = node[0]; // corner x, y, z; step x
+    const uint4 w1 = node[1]; // step y, z; lo x; lo y
+    const uint4 w2 = node[2]; // lo z; hi x, y, z
+    const uint4 w3 = node[3]; // the references
+    refs = make_int4(static_cast<int>(w3.x), static_cast<int>(w3.y), static_cast<int>(w3.z), static_cast<int>(w3.w));
+    const float cx = __uint_as_float(w0.x), cy = __uint_as_float(w0.y), cz = __uint_as_float(w0.z);
+    const float sx = __uint_as_float(w0.w), sy = __uint_as_float(w1.x), sz = __uint_as_float(w1.y);
+    const bool rx = signbit(inv.x), ry = signbit(inv.y), rz = signbit(inv.z);
+    const uint32_t nx = rx ? w2.y : w1.z, fx = rx ? w1.z : w2.y;
+    const uint32_t ny = ry ? w2.z : w1.w, fy = ry ? w1.w : w2.z;
+    const uint32_t nz = rz ? w2.w : w2.x, fz = rz ? w2.x : w2.w;
+    // a plane's distance: its float from its byte (v_cvt_f32_ubyte, then an exact fma), and the slab's multiply-add
+    const auto at = [](uint32_t word, int c, float step, float corner, float inv_a, float os_a) {
+        const float plane = fmaf(static_cast<float>((word >> (8 * c)) & 0xffu), step, corner);
+        return fmaf(plane, inv_a, -os_a);
+    };
+    const int32_t child[wide_children] = {refs.x, refs.y, refs.z, refs.w};
+#pragma unroll
+    for (int c = 0; c < wide_children; ++c)
+    {
+        const float enter = fmaxf(fmaxf(at(nx, c, sx, cx, inv.x, os.x), at(ny, c, sy, cy, inv.y, os.y)),
+                                  fmaxf(at(nz, c, sz, cz, inv.z, os.z), t_min));
+        const float exit = fminf(fminf(at(fx, c, sx, cx, inv.x, os.x), at(fy, c, sy, cy, inv.y, os.y)),
+                                 fminf(at(fz, c, sz, cz, inv.z, os.z), t_max));
+        t[c] = ((enter <= exit) & (child[c] != no_child)) ? enter : inf<float>;
+    }
+}
+
 // A wide node's four child references (read as the planes are: the node array's base plus a 32-bit offset)
 template <typename Real>
 __device__ inline int4 child_refs(const scene_view<Real>& sc, int32_t ref)
@@ -623,18 +658,35 @@ __device__ inline void order(Real& ta, int32_t& ra, Real& tb, int32_t& rb)
     ra = r;
 }
 
+// An inner node's four children: where the ray enters each (or infinity) and their references, from the tree's own
+// node records (Kinds: walk_packed reads packed_node, the others wide_node)
+template <int Kinds, typename Real>
+__device__ inline void node_children(const scene_view<Real>& sc, int32_t ref, const slab_order& so, vec3<Real> inv,
+                                     vec3<Real> os, Real t_min, Real t_max, Real (&t)[wide_children], int4& refs)
+{
+    if constexpr (Kinds == walk_packed && sizeof(Real) == sizeof(float))
+    {
+        packed_entries(sc, ref, inv, os, t_min, t_max, t, refs);
+    }
+    else
+    {
+        child_entries(sc, ref, so, inv, os, t_min, t_max, t);
+        refs = child_refs(sc, ref);
+    }
+}
+
 // An inner node: its four children's boxes tested and sorted by entry distance (a five-exchange network); the entered
 // ones but the nearest go on the stack, farthest first, and the nearest is the next reference (true), or none is
 // entered (false)
-template <typename Real, typename Tally, typename Stack>
+template <int Kinds, typename Real, typename Tally, typename Stack>
 __device__ inline bool descend(const scene_view<Real>& sc, int32_t& ref, const slab_order& so, vec3<Real> inv,
                                vec3<Real> os, Real t_min, Real t_max, const Stack& stack, int& size, Tally& tl)
 {
     tl.add(tally_node);
     Real t[wide_children];
-    child_entries(sc, ref, so, inv, os, t_min, t_max, t);
+    int4 refs;
+    node_children<Kinds>(sc, ref, so, inv, os, t_min, t_max, t, refs);
     Real t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
-    const int4 refs = child_refs(sc, ref);
     int32_t r0 = refs.x, r1 = refs.y, r2 = refs.z, r3 = refs.w;
     order(t0, r0, t1, r1);
     order(t2, r2, t3, r3);
@@ -714,12 +766,19 @@ __device__ inline ray<Real> into_instance(const Real* m, const ray<Real>& r)
 // faster than Aila and Laine's while-while form, which walks inner nodes in an inner loop until every lane of the wave
 // has a leaf). An instance leaf moves the ray into its mesh's coordinates and goes on into its mesh's tree; the stack's
 // entries below `floor` are the scene tree's, and when the mesh tree's are used up the ray is moved back.
-template <typename Real, int Kinds, typename Tally, typename Stack>
+//
+// Shadows: the same walk answers a shadow ray's question for the lanes whose any_hit is set (closest_in_leaf): the
+// closest distance starts at t_max, primitive `skip` is passed over and the walk ends at the first hit. Its answer is
+// the any-hit walk's (quality.hpp, occluded): the same boxes are entered (children beyond t_max never are, and t_max
+// doesn't shrink), only in another order. So the lanes of a wave walk their shadow rays and their paths' next rays in
+// one loop.
+template <typename Real, int Kinds, typename Tally, typename Stack, bool Shadows = false>
 __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Real>& r, int from, bool inward,
-                                        const Stack& stack, Tally& tl)
+                                        const Stack& stack, Tally& tl, Real t_max = inf<Real>, int skip = -1,
+                                        bool any_hit = false)
 {
     const Real t_min = Real(0.001);
-    hit<Real> closest{-1, inf<Real>, false};
+    hit<Real> closest{-1, Shadows ? t_max : inf<Real>, false};
     if (sc.stack_depth == 0)
     {
         return closest;
@@ -747,9 +806,14 @@ __device__ inline hit<Real> closest_hit(const scene_view<Real>& sc, const ray<Re
                     continue;
                 }
             }
-            closest_in_leaf<Real, Kinds>(sc, w.r, ref, from, inward, w.inverse_a, t_min, triangle_shift, closest, tl);
+            closest_in_leaf<Real, Kinds, Shadows>(sc, w.r, ref, from, inward, w.inverse_a, t_min, triangle_shift, closest,
+                                                  tl, skip, any_hit);
+            if (Shadows && any_hit && closest.index >= 0)
+            {
+                return closest;
+            }
         }
-        else if (descend(sc, ref, w.so, w.inv, w.os, t_min, closest.t, stack, size, tl))
+        else if (descend<Kinds>(sc, ref, w.so, w.inv, w.os, t_min, closest.t, stack, size, tl))
         {
             continue;
         }

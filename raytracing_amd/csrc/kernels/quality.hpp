@@ -634,7 +634,7 @@ __device__ inline bool leaf_blocks(const scene_view<float>& sc, const ray<float>
     }
     return false;
 }
-template <bool Instances, typename Tally, typename Stack>
+template <int Walk, typename Tally, typename Stack>
 __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r, float t_max, int from, bool inward,
                                 int skip, const Stack& stack, Tally& tl)
 {
@@ -652,7 +652,7 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
     {
         if (ref < 0)
         {
-            if constexpr (Instances)
+            if constexpr (Walk == walk_instances)
             {
                 if (leaf_kind(ref) == prim_instance)
                 {
@@ -676,8 +676,8 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
             // every entered child must be looked at, in any order: the first goes next, the rest on the stack
             tl.add(tally_node);
             float t[wide_children];
-            child_entries(sc, ref, w.so, w.inv, w.os, 0.001f, t_max, t);
-            const int4 refs = child_refs(sc, ref);
+            int4 refs;
+            node_children<Walk>(sc, ref, w.so, w.inv, w.os, 0.001f, t_max, t, refs);
             const int32_t child[wide_children] = {refs.x, refs.y, refs.z, refs.w};
             int32_t next = no_child;
 #pragma unroll
@@ -698,7 +698,7 @@ __device__ inline bool occluded(const scene_view<float>& sc, const ray<float>& r
                 continue;
             }
         }
-        if constexpr (Instances)
+        if constexpr (Walk == walk_instances)
         {
             if (depth == floor && floor > 0)
             {
@@ -743,18 +743,36 @@ __device__ inline float transmittance(const scene_view<float>& sc, const ray<flo
 }
 
 // --- one step of a quality path ----------------------------------------------------------------------------------------
+// A light sample's shadow ray waiting to be traced as the path's next ray (the queue megakernel's paths): from the
+// vertex (the path's ray's origin, leaving the same primitive) toward `direction`, up to `distance`, the sampled lamp
+// passed over; `term` is added to the radiance when nothing blocks it
+struct pending_shadow
+{
+    vec3<float> direction;
+    vec3<float> term;
+    float distance;
+    int skip;
+    bool inward;
+    bool waiting;
+};
+
 struct quality_path
 {
     path<float> base;
     vec3<float> radiance; // the light gathered so far
     float bsdf_pdf;       // the last scattering's density, if that vertex took a light sample (else 0)
     bool specular;        // the last vertex was a delta lobe's (or the camera): the map is seen bilinear
+    bool ended;           // the path ended at the vertex whose shadow ray waits
+    pending_shadow shadow;
 };
 
 __device__ inline quality_path start_quality(const camera_rec<float>& cam, bool motion, int i, int j, int pass,
                                              pcg32& g)
 {
-    return {start_path(cam, motion, i, j, pass, g), {0.0f, 0.0f, 0.0f}, 0.0f, true};
+    quality_path q{};
+    q.base = start_path(cam, motion, i, j, pass, g);
+    q.specular = true;
+    return q;
 }
 
 // The light sample's contribution at a vertex before its shadow ray: throughput f Le w / pdf, or false when it adds
@@ -782,7 +800,12 @@ __device__ inline bool light_term(const scene_view<float>& sc, const scatterer& 
 // adds its light, weighed against the light set's chance to have drawn it; any other vertex takes a light sample
 // (when the path may still trace a ray, the lobe isn't a delta and a light exists), traces its shadow ray, then
 // scatters
-template <bool Instances, typename Tally, typename Stack>
+//
+// Deferred: the shadow ray isn't traced here but left waiting in the path (q.shadow) for the caller to trace as its next
+// ray (step_quality); a path that ends at this vertex then ends once that ray is traced (q.ended). The random numbers
+// are drawn in the same order and the light added in the same order either way: the shadow ray draws none, and its
+// light is added before anything the next vertex adds.
+template <int Walk, bool Deferred = false, typename Tally, typename Stack>
 __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
                                      const hit<float>& h, pcg32& g, vec3<float>& sample, const Stack& stack,
                                      Tally& tl, uint32_t& shadow_rays)
@@ -802,7 +825,7 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
         sample = q.radiance + w * (p.throughput * le);
         return true;
     }
-    const vertex<float> vx = make_vertex<Instances>(sc, p.r, h, tl);
+    const vertex<float> vx = make_vertex<Walk == walk_instances>(sc, p.r, h, tl);
     const material_rec<float> m = sc.materials[vx.material];
     // the color is read once: matte, smoke and a lamp's front face have one (a coat's base is the matte color)
     const bool lamp = m.kind == RT_MATERIAL_DIFFUSE_LIGHT;
@@ -834,40 +857,91 @@ __device__ inline bool shade_quality(quality_path& q, const scene_view<float>& s
     if (lit)
     {
         ++shadow_rays;
-        const ray<float> shadow{vx.p, ld.wi, p.r.time};
         const bool inward = dot(ld.wi, vx.outward) < 0.0f;
-        if (!occluded<Instances>(sc, shadow, ld.distance, vx.from, inward, ld.skip, stack, tl))
+        if constexpr (Deferred)
         {
-            q.radiance = q.radiance + transmittance(sc, shadow, ld.distance, tl) * term;
+            q.shadow = {ld.wi, term, ld.distance, ld.skip, inward, true};
+        }
+        else
+        {
+            const ray<float> shadow{vx.p, ld.wi, p.r.time};
+            if (!occluded<Walk>(sc, shadow, ld.distance, vx.from, inward, ld.skip, stack, tl))
+            {
+                q.radiance = q.radiance + transmittance(sc, shadow, ld.distance, tl) * term;
+            }
         }
     }
-    if (!scattered)
+    // (a waiting shadow ray leaves the vertex as the next ray would: its origin and primitive go into the path's ray)
+    bool ends = !scattered;
+    if (scattered)
+    {
+        p.throughput = p.throughput * next.weight;
+        q.bsdf_pdf = nee ? next.pdf : 0.0f;
+        q.specular = is_delta(s.kind);
+        ends = (qv.combination != RT_COMBINE_BSDF_ONLY && !(max3(p.throughput) > 0.0f)) || --p.rays_left == 0;
+    }
+    if (ends && !(Deferred && lit))
     {
         sample = q.radiance;
         return true;
     }
-    p.throughput = p.throughput * next.weight;
-    q.bsdf_pdf = nee ? next.pdf : 0.0f;
-    q.specular = is_delta(s.kind);
-    if ((qv.combination != RT_COMBINE_BSDF_ONLY && !(max3(p.throughput) > 0.0f)) || --p.rays_left == 0)
-    {
-        sample = q.radiance;
-        return true;
-    }
-    p.r = {vx.p, next.wi, p.r.time};
+    p.r = {vx.p, scattered ? next.wi : vx.n, p.r.time}; // (the direction of a path that ended isn't read)
     p.from = vx.from;
-    p.inward = dot(next.wi, vx.outward) < 0.0f;
+    p.inward = scattered && dot(next.wi, vx.outward) < 0.0f;
+    q.ended = ends;
     return false;
 }
 
-// Instances: the scene's tree has instance leaves (a two-level tree); a kernel of its own
-template <bool Instances, typename Tally, typename Stack>
+// One ray of a quality path: its waiting shadow ray, if it has one, else its next ray, both walked by the same code
+// (closest_hit's shadow mode), so that the lanes of a wave that trace shadow rays and those that trace their paths'
+// next rays walk the tree together. The light the shadow ray carries is added when nothing blocks it; a path whose
+// last vertex left a shadow ray ends after it. The same numbers as trace_quality's loop, in the same order (shade_quality,
+// Deferred). True when the path's sample is done.
+template <int Walk, typename Tally, typename Stack>
+__device__ inline bool step_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
+                                    pcg32& g, vec3<float>& sample, const Stack& stack, Tally& tl,
+                                    unsigned long long& rays, uint32_t& shadow_rays)
+{
+    pending_shadow& sh = q.shadow;
+    const bool shadow = sh.waiting;
+    // (each choice made on values, component by component: a choice between two members' places would keep the path
+    // in scratch memory rather than in registers)
+    const vec3<float> d = q.base.r.direction;
+    const vec3<float> ds = sh.direction;
+    const ray<float> r{q.base.r.origin, {shadow ? ds.x : d.x, shadow ? ds.y : d.y, shadow ? ds.z : d.z},
+                       q.base.r.time};
+    const bool inward = shadow ? static_cast<bool>(sh.inward) : static_cast<bool>(q.base.inward);
+    const hit<float> h = closest_hit<float, Walk, Tally, Stack, true>(
+        sc, r, q.base.from, inward, stack, tl, shadow ? static_cast<float>(sh.distance) : inf<float>,
+        shadow ? static_cast<int>(sh.skip) : -1, shadow);
+    if (shadow)
+    {
+        sh.waiting = false;
+        if (h.index < 0)
+        {
+            q.radiance = q.radiance + transmittance(sc, r, sh.distance, tl) * sh.term;
+        }
+        if (q.ended)
+        {
+            sample = q.radiance;
+            return true;
+        }
+        return false;
+    }
+    ++rays;
+    return shade_quality<Walk, true>(q, sc, cam, media_events(sc, q.base.r, h, g, tl), g, sample, stack, tl,
+                                          shadow_rays);
+}
+
+// Walk: the tree's walk (walk_all; walk_instances: a two-level tree, whose walk can enter instances; walk_packed: a
+// packed tree), a kernel of its own each
+template <int Walk, typename Tally, typename Stack>
 __device__ inline bool trace_quality(quality_path& q, const scene_view<float>& sc, const camera_rec<float>& cam,
                                      pcg32& g, vec3<float>& sample, const Stack& stack, Tally& tl,
                                      uint32_t& shadow_rays)
 {
-    const hit<float> h = closest_event<float, Instances ? walk_instances : walk_all>(sc, q.base.r, q.base.from,
+    const hit<float> h = closest_event<float, Walk>(sc, q.base.r, q.base.from,
                                                                                    q.base.inward, g, stack, tl);
-    return shade_quality<Instances>(q, sc, cam, h, g, sample, stack, tl, shadow_rays);
+    return shade_quality<Walk>(q, sc, cam, h, g, sample, stack, tl, shadow_rays);
 }
 }

@@ -398,6 +398,8 @@ __global__ void __launch_bounds__(64) wf_sort(wf_pool pool, int p)
                 wf_classes * wf_stripes);
 }
 
+// Walk: walk_instances (any one-level or two-level tree of float nodes) or walk_packed (a packed tree)
+template <int Walk>
 __global__ void __launch_bounds__(256) wf_extend(const scene_view<float> sc, wf_pool pool, int p)
 {
     __shared__ uint32_t prefix[wf_stripes + 2];
@@ -416,7 +418,7 @@ __global__ void __launch_bounds__(256) wf_extend(const scene_view<float> sc, wf_
         s = active_slot(pool, p, prefix, g);
         slot_path sp = load_path(pool, s);
         tally<false> tl;
-        const hit<float> h = closest_event(sc, sp.p.r, sp.p.from, sp.p.inward, sp.g, stack, tl);
+        const hit<float> h = closest_event<float, Walk>(sc, sp.p.r, sp.p.from, sp.p.inward, sp.g, stack, tl);
         pool.hit[s] = make_int4(h.index, __float_as_int(h.t), h.entering ? 1 : 0, 0);
         if (sc.medium_count > 0)
         {
@@ -495,6 +497,7 @@ __global__ void __launch_bounds__(256) wf_shade(const camera_rec<float> cam, con
 
 // The quality integrator's shade: the same queues in class order, each vertex's light sample and its shadow ray
 // traced here (an any-hit walk with this thread's LDS stack), in the megakernel's order of random numbers
+template <int Walk>
 __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> cam, const scene_view<float> sc,
                                                         wf_pool pool, int p)
 {
@@ -524,7 +527,7 @@ __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> 
         const int4 hv = pool.hit[s];
         const hit<float> h{hv.x, __int_as_float(hv.y), hv.z != 0};
         tally<false> tl;
-        done = shade_quality<true>(q, sc, cam, h, sp.g, sample, stack, tl, shadow_rays);
+        done = shade_quality<Walk>(q, sc, cam, h, sp.g, sample, stack, tl, shadow_rays);
         if (!done)
         {
             sp.p = q.base;
@@ -539,7 +542,7 @@ __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> 
 
 // The drain: once every item is handed out and few slots are left active, each of them (a thread each) traces its
 // path to the end, as the megakernel does: the bounces of a nearly empty pool would cost a launch each for little work
-template <int Mode>
+template <int Mode, int Walk>
 __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, const scene_view<float> sc,
                                                 const uint64_t seed, const launch_work work, wf_pool pool)
 {
@@ -571,7 +574,7 @@ __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, con
             bool done;
             if constexpr (Mode == mode_quality)
             {
-                done = trace_quality<true>(q, sc, cam, sp.g, sample, stack, tl, shadow_rays);
+                done = trace_quality<Walk>(q, sc, cam, sp.g, sample, stack, tl, shadow_rays);
             }
             else if constexpr (Mode == mode_all_books)
             {
@@ -654,11 +657,26 @@ hipError_t wf_launch_bounce(const dev::camera_rec<float>& cam, const dev::scene_
     const size_t lds = stack_lds_bytes<float>(sc.stack_depth, dev::wf_block);
     hipLaunchKernelGGL(dev::wf_begin, dim3(1), dim3(256), 0, stream, pool, parity);
     hipLaunchKernelGGL(dev::wf_refill, grid, dim3(dev::wf_block), 0, stream, cam, sc, pool, parity);
-    hipLaunchKernelGGL(dev::wf_extend, grid, dim3(dev::wf_block), lds, stream, sc, pool, parity);
-    hipLaunchKernelGGL(dev::wf_sort, dim3(1), dim3(64), 0, stream, pool, parity);
-    if (mode == dev::mode_quality)
+    const bool packed = dev::tree_walk(sc) == dev::walk_packed;
+    if (packed)
     {
-        hipLaunchKernelGGL(dev::wf_shade_quality, grid, dim3(dev::wf_block), lds, stream, cam, sc, pool, parity);
+        hipLaunchKernelGGL(dev::wf_extend<dev::walk_packed>, grid, dim3(dev::wf_block), lds, stream, sc, pool, parity);
+    }
+    else
+    {
+        hipLaunchKernelGGL(dev::wf_extend<dev::walk_instances>, grid, dim3(dev::wf_block), lds, stream, sc, pool,
+                           parity);
+    }
+    hipLaunchKernelGGL(dev::wf_sort, dim3(1), dim3(64), 0, stream, pool, parity);
+    if (mode == dev::mode_quality && packed)
+    {
+        hipLaunchKernelGGL(dev::wf_shade_quality<dev::walk_packed>, grid, dim3(dev::wf_block), lds, stream, cam, sc,
+                           pool, parity);
+    }
+    else if (mode == dev::mode_quality)
+    {
+        hipLaunchKernelGGL(dev::wf_shade_quality<dev::walk_instances>, grid, dim3(dev::wf_block), lds, stream, cam, sc,
+                           pool, parity);
     }
     else if (mode == dev::mode_first_book)
     {
@@ -678,20 +696,25 @@ hipError_t wf_launch_drain(int mode, const dev::camera_rec<float>& cam, const de
     const int threads = 128;
     const dim3 grid((count + threads - 1) / threads);
     const size_t lds = stack_lds_bytes<float>(sc.stack_depth, threads);
-    if (mode == dev::mode_quality)
+    if (mode == dev::mode_quality && dev::tree_walk(sc) == dev::walk_packed)
     {
-        hipLaunchKernelGGL((dev::wf_drain<dev::mode_quality>), grid, dim3(threads), lds, stream, cam, sc, seed, work,
-                           pool);
+        hipLaunchKernelGGL((dev::wf_drain<dev::mode_quality, dev::walk_packed>), grid, dim3(threads), lds, stream, cam,
+                           sc, seed, work, pool);
+    }
+    else if (mode == dev::mode_quality)
+    {
+        hipLaunchKernelGGL((dev::wf_drain<dev::mode_quality, dev::walk_instances>), grid, dim3(threads), lds, stream,
+                           cam, sc, seed, work, pool);
     }
     else if (mode == dev::mode_all_books)
     {
-        hipLaunchKernelGGL((dev::wf_drain<dev::mode_all_books>), grid, dim3(threads), lds, stream, cam, sc, seed, work,
-                           pool);
+        hipLaunchKernelGGL((dev::wf_drain<dev::mode_all_books, dev::walk_instances>), grid, dim3(threads), lds, stream,
+                           cam, sc, seed, work, pool);
     }
     else
     {
-        hipLaunchKernelGGL((dev::wf_drain<dev::mode_first_book>), grid, dim3(threads), lds, stream, cam, sc, seed,
-                           work, pool);
+        hipLaunchKernelGGL((dev::wf_drain<dev::mode_first_book, dev::walk_instances>), grid, dim3(threads), lds, stream,
+                           cam, sc, seed, work, pool);
     }
     return hipGetLastError();
 }
@@ -708,9 +731,9 @@ hipError_t wf_launch_finish(const dev::camera_rec<float>& cam, const dev::launch
 kernel_attributes wf_attributes(int depth, int which)
 {
     hipFuncAttributes a{};
-    const void* f = which == 0   ? reinterpret_cast<const void*>(&dev::wf_extend)
+    const void* f = which == 0   ? reinterpret_cast<const void*>(&dev::wf_extend<dev::walk_instances>)
                     : which == 1 ? reinterpret_cast<const void*>(&dev::wf_shade<false>)
-                                 : reinterpret_cast<const void*>(&dev::wf_shade_quality);
+                                 : reinterpret_cast<const void*>(&dev::wf_shade_quality<dev::walk_instances>);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)
     {

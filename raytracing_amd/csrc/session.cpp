@@ -61,16 +61,17 @@ private:
 class shared_scene
 {
 public:
-    shared_scene(const rt_scene& scene, int32_t precision, int32_t tree) : tables(scene_tables::copy_of(scene))
+    shared_scene(const rt_scene& scene, int32_t precision, int32_t tree, int32_t nodes)
+        : tables(scene_tables::copy_of(scene))
     {
         view = tables.view();
         if (precision == RT_FLOAT64)
         {
-            f64 = std::make_unique<device_scene<double>>(view, tree);
+            f64 = std::make_unique<device_scene<double>>(view, tree, nodes);
         }
         else
         {
-            f32 = std::make_unique<device_scene<float>>(view, tree);
+            f32 = std::make_unique<device_scene<float>>(view, tree, nodes);
         }
     }
     template <typename Real>
@@ -91,7 +92,7 @@ public:
     std::unique_ptr<device_scene<double>> f64;
 };
 
-std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision, int32_t tree)
+std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision, int32_t tree, int32_t nodes)
 {
     validate_scene(scene);
     if (precision != RT_FLOAT32 && precision != RT_FLOAT64)
@@ -111,7 +112,7 @@ std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t 
         throw std::invalid_argument("triangles, GGX, environment maps and light samples take -p material, nee or mis, "
                                     "not the lights list's uniform, lights or mixture");
     }
-    return std::make_shared<const shared_scene>(scene, precision, tree);
+    return std::make_shared<const shared_scene>(scene, precision, tree, nodes);
 }
 
 std::size_t device_bytes(const shared_scene& scene) { return scene.f32 ? scene.f32->bytes() : scene.f64->bytes(); }
@@ -539,7 +540,7 @@ public:
         }
         else if (current_.plain)
         {
-            a = plain_megakernel_attributes<Real>(mode_, depth, dscene_.view().instance_count > 0);
+            a = plain_megakernel_attributes<Real>(mode_, depth, dev::tree_walk(dscene_.view()));
             const char* name =
                 mode_ == dev::mode_quality ? "quality" : (mode_ == dev::mode_all_books ? "all books" : "first book");
             info.threads_per_block = plain_block_x * plain_block_y;
@@ -549,7 +550,7 @@ public:
         else
         {
             a = megakernel_attributes<Real>(mode_, current_.block, depth, current_.waves,
-                                            dscene_.view().instance_count > 0);
+                                            dev::tree_walk(dscene_.view()));
             const char* name =
                 mode_ == dev::mode_quality ? "quality" : (mode_ == dev::mode_all_books ? "all books" : "first book");
             info.threads_per_block = current_.block;
@@ -1072,6 +1073,10 @@ void check_options(const rt_session_options& options)
     {
         throw std::invalid_argument("unknown tree");
     }
+    if (options.nodes < RT_NODES_AUTO || options.nodes > RT_NODES_PACKED)
+    {
+        throw std::invalid_argument("unknown node layout");
+    }
 }
 }
 
@@ -1079,7 +1084,11 @@ std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_op
 {
     validate_scene(scene);
     check_options(options);
-    return make_session(upload_scene(scene, options.precision, options.tree), scene.camera, options);
+    // (the plain megakernel renders config 3 faster on float nodes, 565 against 506 Mpaths/s: its fewer waves per SIMD
+    // hide less of the packed planes' unpacking; the queue megakernel, 1163 against 1038, and the wavefront take them)
+    const int32_t nodes =
+        options.nodes == RT_NODES_AUTO && options.integrator == RT_INTEGRATOR_PLAIN ? RT_NODES_FLOAT : options.nodes;
+    return make_session(upload_scene(scene, options.precision, options.tree, nodes), scene.camera, options);
 }
 
 std::unique_ptr<session> make_session(std::shared_ptr<const shared_scene> scene, const rt_camera_desc& camera,

@@ -61,10 +61,10 @@ public:
 
 // A scene resident on the device: its tables and its arrays (tree, records, textures) in one precision, shared by
 // every session rendered from it (the server keeps scenes resident while requests of any resolution come and go);
-// tree is an rt_tree
+// tree is an rt_tree, nodes an rt_nodes
 class shared_scene;
 std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision,
-                                                 int32_t tree = RT_TREE_DEFAULT);
+                                                 int32_t tree = RT_TREE_DEFAULT, int32_t nodes = RT_NODES_AUTO);
 std::size_t device_bytes(const shared_scene& scene);
 const rt_scene& scene_of(const shared_scene& scene);
 
