@@ -17,7 +17,9 @@ them; every launch is waited for). Also reported:
   - the plain megakernel (the reference's mapping: a thread per pixel of a 16x8 block, all of a launch's passes in
     its loop), the wavefront and the persistent queue megakernel on the same workload, and the measured integrator's
     ratio to the plain one. All trace the same paths with the same random numbers (their images are the same bytes),
-    so at equal spp they are at equal RMSE, and the ratio of their Mpaths/s is the ratio at equal RMSE.
+    so at equal spp they are at equal RMSE, and the ratio of their Mpaths/s is the ratio at equal RMSE;
+  - config3: the same comparison on the triangle scene (1920x1080, 256 spp, -p mis): the autotuned integrator against
+    the plain megakernel, one timed render each (--no-config3 skips it).
 
 Other workloads (--scene): weekend (400x225, 64 spp, depth 50), glass (600x600, 1024 spp, depth 50, -p mixture),
 glass-mis (the same with -p mis), triangles (1920x1080, 256 spp, depth 16, -p mis).
@@ -281,6 +283,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-megakernel", action="store_true",
                     help="skip the comparison renders (the plain megakernel, the wavefront, the queue megakernel)")
+    ap.add_argument("--no-config3", action="store_true",
+                    help="skip config 3's comparison (the triangle scene: the autotuned integrator and the plain "
+                         "megakernel) that the final scene's line carries")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel over this many bytes (the HBM counters' calibration)")
     ap.add_argument("--wavefront-graph", type=int, default=0, help="bounces per hipGraph (-1: none, every kernel launched on its own)")
@@ -380,6 +385,21 @@ def main():
             others[other] = round(ov, 2)
             print(f"{other}: {ov:.1f} Mpaths/s", file=sys.stderr, flush=True)
 
+    # config 3 beside the headline: the batched integrator the session's autotuner picks against the plain megakernel
+    # on the triangle scene at its full size (1920x1080, 256 spp), one timed render each
+    config3 = None
+    if args.scene == "final" and not args.no_config3:
+        c3_name, c3_width, c3_spp, c3_sampling, c3_description = WORKLOADS["triangles"]
+        c3 = rt.build_scene(c3_name, width=c3_width, samples=c3_spp, sampling=c3_sampling)
+        config3 = {"workload": c3_description}
+        for c3_integrator in ("auto", "plain"):
+            v3, ms3, _, _, s3 = measure(rt, c3, c3_integrator, 1, 1)
+            k3 = s3.kernel().name.decode()
+            s3.close()
+            config3[c3_integrator] = {"mpaths_s": round(v3, 2), "ms_per_step": round(ms3, 2), "kernel": k3}
+            print(f"config 3 {c3_integrator}: {v3:.1f} Mpaths/s, kernel {k3}", file=sys.stderr, flush=True)
+        config3["vs_plain_megakernel"] = round(config3["auto"]["mpaths_s"] / config3["plain"]["mpaths_s"], 3)
+
     cpu = None
     if not args.no_cpu and name != "triangles" and sampling is None:
         cpu = cpu_baseline(name, width, 64 if name == "final" else 16)
@@ -424,6 +444,8 @@ def main():
                      "paths_per_launch": round(paths_per_launch)},
         "cpu_baseline": cpu,
     }
+    if config3 is not None:
+        line["config3"] = config3
     print(json.dumps(line), flush=True)
 
 

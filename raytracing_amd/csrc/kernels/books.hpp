@@ -417,6 +417,8 @@ __device__ inline Real turbulence(const noise_rec<Real>& tb, vec3<Real> p, int o
     }
     return m_fabs(sum);
 }
+// The marble's octaves of turbulence (the second book's noise_texture, scale z + 10 turb(p, 7))
+constexpr int marble_octaves = 7;
 // An image's nearest texel to (u, v), v up, both clamped to [0, 1]
 template <typename Real>
 __device__ inline vec3<Real> image_color(const scene_view<Real>& sc, int table, Real u, Real v)
@@ -431,8 +433,10 @@ __device__ inline vec3<Real> image_color(const scene_view<Real>& sc, int table, 
 }
 // A texture's color at the vertex: a checker picks one of its two textures by the parity of the point's cell (and may
 // nest), an image by (u, v), the marble by 1/2 (1 + sin(scale z + 10 turbulence(p))) in gray
+// (turb: the marble's turbulence at the vertex when the wave computed it beforehand, wave_turbulence; else null)
 template <typename Real, typename Tally>
-__device__ inline vec3<Real> texture_color(const scene_view<Real>& sc, int index, const vertex<Real>& vx, Tally& tl)
+__device__ inline vec3<Real> texture_color(const scene_view<Real>& sc, int index, const vertex<Real>& vx, Tally& tl,
+                                           const Real* turb = nullptr)
 {
     texture_rec<Real> tex = sc.textures[index];
     tl.add(tally_texture);
@@ -451,7 +455,8 @@ __device__ inline vec3<Real> texture_color(const scene_view<Real>& sc, int index
     case RT_TEXTURE_NOISE:
     {
         const noise_rec<Real>& marble = sc.noises[tex.table];
-        const Real phase = marble.scale * vx.tex_p.z + Real(10) * turbulence(marble, vx.tex_p, 7, tl);
+        const Real phase =
+            marble.scale * vx.tex_p.z + Real(10) * (turb ? *turb : turbulence(marble, vx.tex_p, marble_octaves, tl));
         const Real gray = Real(0.5) * (Real(1) + m_sin(phase));
         return {gray, gray, gray};
     }
@@ -462,10 +467,88 @@ __device__ inline vec3<Real> texture_color(const scene_view<Real>& sc, int index
 // A material's color at the vertex: its albedo, or its texture's color
 template <typename Real, typename Tally>
 __device__ inline vec3<Real> material_color(const scene_view<Real>& sc, const material_rec<Real>& m,
-                                            const vertex<Real>& vx, Tally& tl)
+                                            const vertex<Real>& vx, Tally& tl, const Real* turb = nullptr)
 {
     const int index = m.texture; // (named first: `m.texture <` would parse as a template's arguments)
-    return index < 0 ? load3(m.albedo) : texture_color(sc, index, vx, tl);
+    return index < 0 ? load3(m.albedo) : texture_color(sc, index, vx, tl, turb);
+}
+
+// The noise table whose marble gives a material's color at the vertex (the texture chain followed as texture_color
+// follows it), or -1 when its color comes from anything else
+template <typename Real>
+__device__ inline int marble_table(const scene_view<Real>& sc, const material_rec<Real>& m, const vertex<Real>& vx)
+{
+    const int index = m.texture;
+    if (index < 0)
+    {
+        return -1;
+    }
+    texture_rec<Real> tex = sc.textures[index];
+    while (tex.kind == RT_TEXTURE_CHECKER)
+    {
+        const int cell = static_cast<int>(m_floor(tex.inv_scale * vx.tex_p.x)) +
+                         static_cast<int>(m_floor(tex.inv_scale * vx.tex_p.y)) +
+                         static_cast<int>(m_floor(tex.inv_scale * vx.tex_p.z));
+        tex = sc.textures[cell % 2 == 0 ? tex.even : tex.odd];
+    }
+    return tex.kind == RT_TEXTURE_NOISE ? tex.table : -1;
+}
+
+// The marble's turbulence for every lane of a wave that asks for one (table >= 0: its noise table, at its point p),
+// computed by the whole wave together, one asking lane after the other: lane 8 o + k takes corner k of octave o (56
+// lanes for the 7 octaves), the interpolations along x, y and z pair lanes by exchanges, and the octaves are summed in
+// order. Every value is formed by the same operations on the same operands as turbulence()'s loop (a point of octave
+// o is 2^o p exactly, as repeated doubling gives it), so the result is the same bits; a marble met by one lane of a
+// wave costs the wave one lane's corner instead of the 56 corners one after the other. Call with every lane of the
+// wave active. Float only.
+__device__ inline float wave_turbulence(const scene_view<float>& sc, int table, vec3<float> p)
+{
+    uint64_t ask = __ballot(table >= 0);
+    float mine = 0.0f;
+    const int lane = static_cast<int>(__lane_id());
+    const int octave = lane >> 3;
+    const int a = (lane >> 2) & 1, b = (lane >> 1) & 1, c = lane & 1; // the corner, at index 4a + 2b + c
+    while (ask != 0)
+    {
+        const int who = __builtin_ctzll(ask);
+        ask &= ask - 1;
+        // the asking lane's table and point, the same in every lane
+        const int t = __builtin_amdgcn_readlane(table, who);
+        const vec3<float> q{__int_as_float(__builtin_amdgcn_readlane(__float_as_int(p.x), who)),
+                            __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p.y), who)),
+                            __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p.z), who))};
+        const noise_rec<float>& tb = sc.noises[t];
+        // this lane's octave's point and its lattice cell, and its corner's contribution (gradient_noise's terms)
+        const float scale = static_cast<float>(1 << (octave < marble_octaves ? octave : 0));
+        const vec3<float> po{scale * q.x, scale * q.y, scale * q.z};
+        const float cx = floorf(po.x), cy = floorf(po.y), cz = floorf(po.z);
+        const vec3<float> f{po.x - cx, po.y - cy, po.z - cz};
+        const vec3<float> smooth{f.x * f.x * (3.0f - 2.0f * f.x), f.y * f.y * (3.0f - 2.0f * f.y),
+                                 f.z * f.z * (3.0f - 2.0f * f.z)};
+        const int ix = static_cast<int>(cx), iy = static_cast<int>(cy), iz = static_cast<int>(cz);
+        const int h = tb.perm_x[(ix + a) & 255] ^ tb.perm_y[(iy + b) & 255] ^ tb.perm_z[(iz + c) & 255];
+        const float* grad = tb.gradients[h];
+        const float g = dot(vec3<float>{grad[0], grad[1], grad[2]},
+                            vec3<float>{f.x - static_cast<float>(a), f.y - static_cast<float>(b), f.z - static_cast<float>(c)});
+        // along x: corner (0, b, c) with (1, b, c); along y: (0, c) with (1, c); along z
+        const float gx = __shfl_xor(g, 4, 64);
+        const float x = g + smooth.x * (gx - g);
+        const float xy = __shfl_xor(x, 2, 64);
+        const float y = x + smooth.y * (xy - x);
+        const float yz = __shfl_xor(y, 1, 64);
+        const float noise = y + smooth.z * (yz - y);
+        // the octaves in order (lane 8 o holds octave o's noise)
+        float sum = 0.0f;
+        float amplitude = 1.0f;
+#pragma unroll
+        for (int o = 0; o < marble_octaves; ++o)
+        {
+            sum += amplitude * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(noise), 8 * o));
+            amplitude *= 0.5f;
+        }
+        mine = lane == who ? fabsf(sum) : mine;
+    }
+    return mine;
 }
 
 // --- smoke: constant-density media ------------------------------------------------------------------------------------
@@ -801,18 +884,23 @@ __device__ inline bool end_path(const path<Real>& p, vec3<Real> light, vec3<Real
     return true;
 }
 
-// What the event h does to the path (true: the sample is complete, its light in `sample`)
-template <typename Real, typename Tally>
-__device__ inline bool shade(path<Real>& p, const scene_view<Real>& sc, const hit<Real>& h, pcg32& g,
-                             vec3<Real>& sample, Tally& tl)
+// Whether a vertex's lobe reads the material's color (a lamp's back face is black)
+__device__ inline bool reads_color(books_lobe lobe, bool front)
 {
-    const vertex<Real> vx = make_vertex<false>(sc, p.r, h, tl);
-    const material_rec<Real> m = sc.materials[vx.material];
+    return lobe == books_lobe::matte || lobe == books_lobe::smoke || (lobe == books_lobe::emitter && front);
+}
+
+// What an event does to the path at its vertex vx of material m (true: the sample is complete, its light in
+// `sample`); turb: the vertex's marble turbulence when the wave computed it beforehand (wave_turbulence), else null
+template <typename Real, typename Tally>
+__device__ inline bool shade_at(path<Real>& p, const scene_view<Real>& sc, const vertex<Real>& vx,
+                                const material_rec<Real>& m, pcg32& g, vec3<Real>& sample, Tally& tl,
+                                const Real* turb = nullptr)
+{
     const vec3<Real> black{Real(0), Real(0), Real(0)};
     const books_lobe lobe = lobe_of(m.kind);
-    // the color is read once, for the lobes that have one (a lamp's back face is black)
-    const bool colored = lobe == books_lobe::matte || lobe == books_lobe::smoke || (lobe == books_lobe::emitter && vx.front);
-    const vec3<Real> color = colored ? material_color(sc, m, vx, tl) : black;
+    // the color is read once, for the lobes that have one
+    const vec3<Real> color = reads_color(lobe, vx.front) ? material_color(sc, m, vx, tl, turb) : black;
     vec3<Real> d;
     switch (lobe)
     {
@@ -842,6 +930,15 @@ __device__ inline bool shade(path<Real>& p, const scene_view<Real>& sc, const hi
     }
     return continue_path(p, vx, d, sample);
 }
+// What the event h does to the path (true: the sample is complete, its light in `sample`)
+template <typename Real, typename Tally>
+__device__ inline bool shade(path<Real>& p, const scene_view<Real>& sc, const hit<Real>& h, pcg32& g,
+                             vec3<Real>& sample, Tally& tl)
+{
+    const vertex<Real> vx = make_vertex<false>(sc, p.r, h, tl);
+    const material_rec<Real> m = sc.materials[vx.material];
+    return shade_at(p, sc, vx, m, g, sample, tl);
+}
 
 // One ray of a path with every feature of the three books: the walk, the media, then the event (src/gpu/camera.cuh:
 // 364-437); a ray that meets nothing sees the sky
@@ -862,6 +959,44 @@ __device__ inline bool trace_all_books(path<Real>& p, const scene_view<Real>& sc
 {
     return all_books_after_walk(p, sc, cam, g, sample, closest_hit<Real, walk_books>(sc, p.r, p.from, p.inward, stack, tl),
                                 tl);
+}
+
+// trace_all_books for a lane of a wave whose every lane calls it together (busy: the lane has a path; the others keep
+// the wave whole): the walk and the media, then the vertex and its material, then the marble turbulence of every lane
+// that needs one, by the whole wave (wave_turbulence), then the shading. The same numbers as trace_all_books, in the
+// same order. Float only.
+template <typename Tally, typename Stack>
+__device__ inline bool trace_all_books_wave(path<float>& p, const scene_view<float>& sc, const camera_rec<float>& cam,
+                                            pcg32& g, vec3<float>& sample, const Stack& stack, Tally& tl, bool busy)
+{
+    hit<float> h{-1, inf<float>, 0};
+    if (busy)
+    {
+        h = media_events(sc, p.r, closest_hit<float, walk_books>(sc, p.r, p.from, p.inward, stack, tl), g, tl);
+    }
+    vertex<float> vx{};
+    material_rec<float> m{};
+    int table = -1;
+    if (busy && h.index >= 0)
+    {
+        vx = make_vertex<false>(sc, p.r, h, tl);
+        m = sc.materials[vx.material];
+        table = reads_color(lobe_of(m.kind), vx.front) ? marble_table(sc, m, vx) : -1;
+    }
+    const float turb = wave_turbulence(sc, table, vx.tex_p);
+    if (table >= 0)
+    {
+        tl.add(tally_perlin, 8 * marble_octaves);
+    }
+    if (!busy)
+    {
+        return false;
+    }
+    if (h.index < 0)
+    {
+        return end_path(p, sky(cam, p.r.direction), sample);
+    }
+    return shade_at(p, sc, vx, m, g, sample, tl, &turb);
 }
 
 // What the hit h does to a first-book path (spheres that stand still, solid matte, mirrors, glass, the sky): the same

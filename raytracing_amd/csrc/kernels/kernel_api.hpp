@@ -84,7 +84,8 @@ hipError_t launch_display(const double* sums, int64_t pixels, double inverse_pas
 hipError_t launch_copy(const void* in, void* out, size_t bytes, hipStream_t stream);
 
 // walk: the quality kernels' walk of the scene's tree (dev::tree_walk: a two-level tree's walks can enter instances, a
-// packed tree's read packed nodes)
+// packed tree's read packed nodes); marble: the scene has a noise texture (the three books' kernel for it)
 template <typename Real>
-kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves = 0, int walk = dev::walk_all);
+kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves = 0, int walk = dev::walk_all,
+                                        bool marble = false);
 }

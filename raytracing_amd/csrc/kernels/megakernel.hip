@@ -206,7 +206,9 @@ __device__ inline const Args& first_kernel_argument()
 // wave's place in the queue as the path ends (wave_queue). (Walking the tree in bursts of a few steps per iteration,
 // so that a lane whose walk is over shades while a long walk goes on, measured slower here: the walk's state then
 // stays live across the shading code, which costs registers.)
-template <typename Real, bool AllBooks, bool Count, int Waves>
+// Marble: the scene has a marble (a noise texture); the three books' float kernel then keeps its waves whole until
+// every lane is done, so that its lanes evaluate the marble's turbulence together (wave_turbulence needs every lane)
+template <typename Real, bool AllBooks, bool Count, int Waves, bool Marble = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
 megakernel(const megakernel_args<Real> args, staged_sample<Real>* __restrict__ staging,
            unsigned long long* __restrict__ ray_count, unsigned long long* __restrict__ tallies)
@@ -218,6 +220,7 @@ megakernel(const megakernel_args<Real> args, staged_sample<Real>* __restrict__ s
     int32_t at = -1; // where this lane's path's sample goes, or -1 between items
     path<Real> p{};
     pcg32 g{};
+    constexpr bool whole = Marble && AllBooks && sizeof(Real) == sizeof(float);
     while (true)
     {
         const megakernel_args<Real>& a = first_kernel_argument<megakernel_args<Real>>();
@@ -234,7 +237,8 @@ megakernel(const megakernel_args<Real> args, staged_sample<Real>* __restrict__ s
             p = start_path(cam, AllBooks && sc.motion != 0, si, sj, pass, g);
             at = staged_at(q, si, sj, pass_local);
         }
-        if (at < 0)
+        const bool busy = at >= 0;
+        if (whole ? __ballot(busy) == 0 : !busy)
         {
             if (w.drained)
             {
@@ -242,11 +246,19 @@ megakernel(const megakernel_args<Real> args, staged_sample<Real>* __restrict__ s
             }
             continue; // the wave's chunk ran out for this lane in this round; it claims again
         }
-        ++rays;
+        rays += busy ? 1 : 0;
         vec3<Real> sample;
-        const bool done = AllBooks ? trace_all_books(p, sc, cam, g, sample, stack, tl)
-                                   : trace_first_book(p, sc, cam, g, sample, stack, tl);
-        if (done)
+        bool done;
+        if constexpr (whole)
+        {
+            done = trace_all_books_wave(p, sc, cam, g, sample, stack, tl, busy);
+        }
+        else
+        {
+            done = AllBooks ? trace_all_books(p, sc, cam, g, sample, stack, tl)
+                            : trace_first_book(p, sc, cam, g, sample, stack, tl);
+        }
+        if (busy && done)
         {
             stage(staging + at, sample);
             at = -1;
@@ -509,7 +521,7 @@ const void* quality_megakernel_for(bool split)
 
 // walk: the quality kernels' walk of the scene's tree (dev::tree_walk)
 template <typename Real, bool Count, int Waves>
-const void* megakernel_for_waves(int mode, bool split, int walk)
+const void* megakernel_for_waves(int mode, bool split, int walk, bool marble)
 {
     if constexpr (sizeof(Real) == sizeof(float))
     {
@@ -520,6 +532,13 @@ const void* megakernel_for_waves(int mode, bool split, int walk)
                                                : quality_megakernel_for<Count, Waves, dev::walk_all>(split);
         }
     }
+    if constexpr (sizeof(Real) == sizeof(float))
+    {
+        if (mode == dev::mode_all_books && marble)
+        {
+            return reinterpret_cast<const void*>(&dev::megakernel<Real, true, Count, Waves, true>);
+        }
+    }
     return mode == dev::mode_all_books ? reinterpret_cast<const void*>(&dev::megakernel<Real, true, Count, Waves>)
                                        : reinterpret_cast<const void*>(&dev::megakernel<Real, false, Count, Waves>);
 }
@@ -527,24 +546,25 @@ const void* megakernel_for_waves(int mode, bool split, int walk)
 // The instance for a mode and an occupancy target: the counting instances and the FP64 ones at the compiler's choice.
 // split: the quality megakernel with its stack's deeper levels in HBM
 template <typename Real, bool Count>
-const void* megakernel_for(int mode, int waves, bool split, int walk)
+// marble: the scene has a noise texture (the three books' kernel that keeps its waves whole)
+const void* megakernel_for(int mode, int waves, bool split, int walk, bool marble)
 {
     if constexpr (!Count && sizeof(Real) == sizeof(float))
     {
         if (waves == 5)
         {
-            return megakernel_for_waves<Real, Count, 5>(mode, split, walk);
+            return megakernel_for_waves<Real, Count, 5>(mode, split, walk, marble);
         }
         if (waves == 6)
         {
-            return megakernel_for_waves<Real, Count, 6>(mode, split, walk);
+            return megakernel_for_waves<Real, Count, 6>(mode, split, walk, marble);
         }
         if (waves == 7)
         {
-            return megakernel_for_waves<Real, Count, 7>(mode, split, walk);
+            return megakernel_for_waves<Real, Count, 7>(mode, split, walk, marble);
         }
     }
-    return megakernel_for_waves<Real, Count, 0>(mode, split, walk);
+    return megakernel_for_waves<Real, Count, 0>(mode, split, walk, marble);
 }
 
 // The stack levels a split quality megakernel keeps in LDS: as many as let the waves its registers allow share a CU's
@@ -612,8 +632,9 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
     }
     size_t lds = stack_lds_bytes<Real>(sc.stack_depth, block_threads);
     const int walk = dev::tree_walk(sc);
-    const void* f = tallies ? megakernel_for<Real, true>(mode, 0, false, walk)
-                            : megakernel_for<Real, false>(mode, waves, false, walk);
+    const bool marble = sc.noises != nullptr;
+    const void* f = tallies ? megakernel_for<Real, true>(mode, 0, false, walk, marble)
+                            : megakernel_for<Real, false>(mode, waves, false, walk, marble);
     // as many blocks as fit the chip at once
     int device = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&device);
@@ -634,7 +655,7 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
     dev::far_stack far{sc.stack_depth, nullptr, nullptr, 0};
     if (!tallies && mode == dev::mode_quality && far_buffer != nullptr && sc.stack_depth > 0)
     {
-        const void* split = megakernel_for<Real, false>(mode, waves, true, walk);
+        const void* split = megakernel_for<Real, false>(mode, waves, true, walk, marble);
         const int near = near_levels(split, sc.stack_depth, block_threads, device);
         const size_t split_lds = stack_lds_bytes<Real>(near, block_threads);
         int split_per_cu = 0;
@@ -794,10 +815,10 @@ hipError_t launch_display(const double* sums, int64_t pixels, double inverse_pas
 }
 
 template <typename Real>
-kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves, int walk)
+kernel_attributes megakernel_attributes(int mode, int block_threads, int depth, int waves, int walk, bool marble)
 {
     hipFuncAttributes a{};
-    const void* f = megakernel_for<Real, false>(mode, waves, false, walk);
+    const void* f = megakernel_for<Real, false>(mode, waves, false, walk, marble);
     kernel_attributes k{};
     if (hipFuncGetAttributes(&a, f) == hipSuccess)
     {
@@ -824,8 +845,8 @@ template hipError_t launch_megakernel<double>(int, const dev::camera_rec<double>
 template hipError_t launch_closest_hit<float>(const dev::scene_view<float>&, const float*, const float*, const float*,
                                               const int32_t*, const uint8_t*, int64_t, int32_t*, float*, uint8_t*,
                                               hipStream_t);
-template kernel_attributes megakernel_attributes<float>(int, int, int, int, int);
-template kernel_attributes megakernel_attributes<double>(int, int, int, int, int);
+template kernel_attributes megakernel_attributes<float>(int, int, int, int, int, bool);
+template kernel_attributes megakernel_attributes<double>(int, int, int, int, int, bool);
 template size_t stack_lds_bytes<float>(int, int);
 template size_t stack_lds_bytes<double>(int, int);
 }

@@ -550,7 +550,7 @@ public:
         else
         {
             a = megakernel_attributes<Real>(mode_, current_.block, depth, current_.waves,
-                                            dev::tree_walk(dscene_.view()));
+                                            dev::tree_walk(dscene_.view()), dscene_.view().noises != nullptr);
             const char* name =
                 mode_ == dev::mode_quality ? "quality" : (mode_ == dev::mode_all_books ? "all books" : "first book");
             info.threads_per_block = current_.block;
