@@ -367,8 +367,12 @@ def main():
     paths_per_launch = scene.width * scene.height * scene.passes / launches_per_step
     achieved = bytes_per_path * paths_per_launch / avg_launch / 1e9
     events = {n: int(traffic.events[i]) for i, n in enumerate(rt.TRAFFIC_EVENTS) if traffic.events[i]}
+    phase_cycles = sum(traffic.phase_lane_cycles)
+    phases = {n: round(traffic.phase_lane_cycles[i] / phase_cycles, 4) for i, n in enumerate(rt.TRAFFIC_PHASES)} \
+        if phase_cycles else None
     print(f"traffic: {bytes_per_path:.0f} B/path, {traffic.rays / traffic.paths:.2f} rays/path, "
-          f"{traffic.shadow_rays / traffic.paths:.2f} shadow rays/path, {events}", file=sys.stderr, flush=True)
+          f"{traffic.shadow_rays / traffic.paths:.2f} shadow rays/path, {events}; lane-time by phase {phases}",
+          file=sys.stderr, flush=True)
     session.close()
 
     # the other integrators on the same workload: the plain megakernel (the reference's mapping, the baseline), the
@@ -440,6 +444,8 @@ def main():
                      # measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE) over the live launch time
                      "hbm_frac": None if hbm is None else round(hbm / avg_launch / 1e9 / HBM_PEAK_GBS, 4),
                      "issue": issue,
+                     # the counting instance's lanes' time by phase (queue, walk, media, shade)
+                     "lane_time_by_phase": phases,
                      "bytes_per_path": round(bytes_per_path, 1), "avg_launch_ms": round(1000 * avg_launch, 3),
                      "paths_per_launch": round(paths_per_launch)},
         "cpu_baseline": cpu,

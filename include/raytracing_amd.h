@@ -536,6 +536,7 @@ int rt_environment_tables(const rt_environment_desc* environment, float* texels,
  * weighted sum per path, plus the double sums' read and write of each pixel once per launch. This is the algorithmic
  * traffic the roofline is measured with. No counterpart in the reference. */
 #define RT_TRAFFIC_EVENTS 17
+#define RT_TRAFFIC_PHASES 4
 typedef struct rt_traffic
 {
     uint64_t paths;
@@ -544,6 +545,10 @@ typedef struct rt_traffic
     uint64_t events[17];
     double event_bytes[17];
     double bytes_per_path;
+    /* the same kernel's lanes' shader-clock cycles by phase of their paths' iterations (in order: the queue, with a
+     * lane's wait for an item; the BVH walks; the media; the shading), summed over lanes: where the issue time goes,
+     * a lane that idles while others trace counting the wait in the phase it goes on with */
+    uint64_t phase_lane_cycles[4];
 } rt_traffic;
 
 int rt_session_traffic(rt_session* session, int64_t passes, rt_traffic* traffic);

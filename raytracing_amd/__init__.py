@@ -153,9 +153,11 @@ class RenderStats(C.Structure):
 
 class Traffic(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("events", C.c_uint64 * 17),
-                ("event_bytes", C.c_double * 17), ("bytes_per_path", C.c_double)]
+                ("event_bytes", C.c_double * 17), ("bytes_per_path", C.c_double),
+                ("phase_lane_cycles", C.c_uint64 * 4)]
 
 
+TRAFFIC_PHASES = ["queue", "walk", "media", "shade"]
 TRAFFIC_EVENTS = ["node", "sphere", "moving_sphere", "quad", "medium", "boundary_quad", "hit", "frame", "texture",
                   "texel", "perlin", "light", "triangle", "lamp", "env_texel", "alias_bin", "instance"]
 

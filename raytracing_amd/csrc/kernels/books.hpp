@@ -947,18 +947,18 @@ __device__ inline bool all_books_after_walk(path<Real>& p, const scene_view<Real
                                             pcg32& g, vec3<Real>& sample, const hit<Real>& surface_hit, Tally& tl)
 {
     const hit<Real> h = media_events(sc, p.r, surface_hit, g, tl);
-    if (h.index < 0)
-    {
-        return end_path(p, sky(cam, p.r.direction), sample);
-    }
-    return shade(p, sc, h, g, sample, tl);
+    tl.lap(phase_media);
+    const bool done = h.index < 0 ? end_path(p, sky(cam, p.r.direction), sample) : shade(p, sc, h, g, sample, tl);
+    tl.lap(phase_shade);
+    return done;
 }
 template <typename Real, typename Tally, typename Stack>
 __device__ inline bool trace_all_books(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
                                        pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
 {
-    return all_books_after_walk(p, sc, cam, g, sample, closest_hit<Real, walk_books>(sc, p.r, p.from, p.inward, stack, tl),
-                                tl);
+    const hit<Real> h = closest_hit<Real, walk_books>(sc, p.r, p.from, p.inward, stack, tl);
+    tl.lap(phase_walk);
+    return all_books_after_walk(p, sc, cam, g, sample, h, tl);
 }
 
 // trace_all_books for a lane of a wave whose every lane calls it together (busy: the lane has a path; the others keep
@@ -972,7 +972,10 @@ __device__ inline bool trace_all_books_wave(path<float>& p, const scene_view<flo
     hit<float> h{-1, inf<float>, 0};
     if (busy)
     {
-        h = media_events(sc, p.r, closest_hit<float, walk_books>(sc, p.r, p.from, p.inward, stack, tl), g, tl);
+        h = closest_hit<float, walk_books>(sc, p.r, p.from, p.inward, stack, tl);
+        tl.lap(phase_walk);
+        h = media_events(sc, p.r, h, g, tl);
+        tl.lap(phase_media);
     }
     vertex<float> vx{};
     material_rec<float> m{};
@@ -992,11 +995,10 @@ __device__ inline bool trace_all_books_wave(path<float>& p, const scene_view<flo
     {
         return false;
     }
-    if (h.index < 0)
-    {
-        return end_path(p, sky(cam, p.r.direction), sample);
-    }
-    return shade_at(p, sc, vx, m, g, sample, tl, &turb);
+    const bool done = h.index < 0 ? end_path(p, sky(cam, p.r.direction), sample)
+                                  : shade_at(p, sc, vx, m, g, sample, tl, &turb);
+    tl.lap(phase_shade);
+    return done;
 }
 
 // What the hit h does to a first-book path (spheres that stand still, solid matte, mirrors, glass, the sky): the same
@@ -1045,8 +1047,11 @@ template <typename Real, typename Tally, typename Stack>
 __device__ inline bool trace_first_book(path<Real>& p, const scene_view<Real>& sc, const camera_rec<Real>& cam,
                                         pcg32& g, vec3<Real>& sample, const Stack& stack, Tally& tl)
 {
-    return first_book_after_walk(p, sc, cam, g, sample,
-                                 closest_hit<Real, walk_spheres>(sc, p.r, p.from, p.inward, stack, tl), tl);
+    const hit<Real> h = closest_hit<Real, walk_spheres>(sc, p.r, p.from, p.inward, stack, tl);
+    tl.lap(phase_walk);
+    const bool done = first_book_after_walk(p, sc, cam, g, sample, h, tl);
+    tl.lap(phase_shade);
+    return done;
 }
 
 // A NaN channel of a sample counts as 0 (src/gpu/renderer.cu:311-316)

@@ -34,7 +34,8 @@ __device__ inline lds_stack block_stack(int entries)
     return stack_in(lds, entries, blockDim.x * blockDim.y, threadIdx.y * blockDim.x + threadIdx.x);
 }
 
-static_assert(tally_events == RT_TRAFFIC_EVENTS, "the traffic report lists every tally event");
+static_assert(tally_events == RT_TRAFFIC_EVENTS && tally_phases == RT_TRAFFIC_PHASES,
+              "the traffic report lists every tally event and phase");
 
 // The wave's counts: the rays (and shadow rays) summed across the wave, then one atomic per wave
 template <bool Count, typename Tally>
@@ -61,6 +62,15 @@ __device__ inline void add_counts(unsigned long long rays, unsigned long long sh
             if (lane0)
             {
                 atomicAdd(tallies + e, total);
+            }
+        }
+        // each phase's lane-cycles, after the events
+        for (int k = 0; k < tally_phases; ++k)
+        {
+            const unsigned long long total = wave_sum(static_cast<unsigned long long>(tl.cycles[k]));
+            if (lane0)
+            {
+                atomicAdd(tallies + tally_events + k, total);
             }
         }
     }
@@ -246,6 +256,7 @@ megakernel(const megakernel_args<Real> args, staged_sample<Real>* __restrict__ s
             }
             continue; // the wave's chunk ran out for this lane in this round; it claims again
         }
+        tl.lap(phase_queue);
         rays += busy ? 1 : 0;
         vec3<Real> sample;
         bool done;
@@ -317,6 +328,7 @@ __device__ inline void quality_paths(staged_sample<float>* __restrict__ staging,
             }
             continue;
         }
+        tl.lap(phase_queue);
         vec3<float> sample;
         if (step_quality<Walk>(qp, sc, cam, g, sample, stack, tl, rays, shadow_rays))
         {

@@ -354,16 +354,38 @@ enum tally_event
     tally_instance,       // an instance record (the ray moved into its mesh)
     tally_events
 };
+// The phases of a path's iteration, timed by the counting instance (lap): the queue (claiming an item, starting its
+// path, staging its sample, and a lane's wait while others trace), the walk (closest hits and shadow rays' any-hit
+// walks), the media (free flights, transmittance), the shading (vertices, materials, scattering, light samples)
+enum tally_phase
+{
+    phase_queue = 0,
+    phase_walk,
+    phase_media,
+    phase_shade,
+    tally_phases
+};
 template <bool On>
 struct tally
 {
     __device__ void add(tally_event, uint32_t = 1) {}
+    __device__ void lap(tally_phase) {}
 };
 template <>
 struct tally<true>
 {
     uint32_t n[tally_events] = {};
+    uint64_t cycles[tally_phases] = {};
+    uint64_t last = __builtin_readcyclecounter();
     __device__ void add(tally_event e, uint32_t k = 1) { n[e] += k; }
+    // the shader clock's cycles since the last lap, to `phase` (per lane: a lane that waits while others trace counts
+    // the wait in the phase of its next lap)
+    __device__ void lap(tally_phase phase)
+    {
+        const uint64_t now = __builtin_readcyclecounter();
+        cycles[phase] += now - last;
+        last = now;
+    }
 };
 
 // A distance on the traversal stack in 16 bits: the upper half of a float no larger than t (t > 0, so dropping the low

@@ -914,6 +914,7 @@ __device__ inline bool step_quality(quality_path& q, const scene_view<float>& sc
     const hit<float> h = closest_hit<float, Walk, Tally, Stack, true>(
         sc, r, q.base.from, inward, stack, tl, shadow ? static_cast<float>(sh.distance) : inf<float>,
         shadow ? static_cast<int>(sh.skip) : -1, shadow);
+    tl.lap(phase_walk);
     if (shadow)
     {
         sh.waiting = false;
@@ -921,6 +922,7 @@ __device__ inline bool step_quality(quality_path& q, const scene_view<float>& sc
         {
             q.radiance = q.radiance + transmittance(sc, r, sh.distance, tl) * sh.term;
         }
+        tl.lap(phase_media);
         if (q.ended)
         {
             sample = q.radiance;
@@ -929,8 +931,11 @@ __device__ inline bool step_quality(quality_path& q, const scene_view<float>& sc
         return false;
     }
     ++rays;
-    return shade_quality<Walk, true>(q, sc, cam, media_events(sc, q.base.r, h, g, tl), g, sample, stack, tl,
-                                          shadow_rays);
+    const hit<float> event = media_events(sc, q.base.r, h, g, tl);
+    tl.lap(phase_media);
+    const bool done = shade_quality<Walk, true>(q, sc, cam, event, g, sample, stack, tl, shadow_rays);
+    tl.lap(phase_shade);
+    return done;
 }
 
 // Walk: the tree's walk (walk_all; walk_instances: a two-level tree, whose walk can enter instances; walk_packed: a

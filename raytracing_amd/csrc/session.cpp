@@ -670,8 +670,8 @@ public:
             throw std::invalid_argument("traffic is counted over 1 to sqrt_spp^2 passes of the first round");
         }
         device_array<double> scratch(values());
-        // [rays, shadow rays, the tally events]
-        device_array<unsigned long long> counts(2 + RT_TRAFFIC_EVENTS);
+        // [rays, shadow rays, the tally events, the phases' lane-cycles]
+        device_array<unsigned long long> counts(2 + RT_TRAFFIC_EVENTS + RT_TRAFFIC_PHASES);
         check(hipMemsetAsync(scratch.data(), 0, scratch.bytes(), stream()), "clearing scratch");
         check(hipMemsetAsync(counts.data(), 0, counts.bytes(), stream()), "clearing counters");
         const dev::launch_work work{0, static_cast<int32_t>(passes), 0, camera_.image_height};
@@ -681,14 +681,17 @@ public:
                                       stream(), counts.data() + 2),
               "launching the counting kernel");
         check(hipStreamSynchronize(stream()), "counting traffic");
-        unsigned long long host[2 + RT_TRAFFIC_EVENTS] = {};
+        unsigned long long host[2 + RT_TRAFFIC_EVENTS + RT_TRAFFIC_PHASES] = {};
         check(hipMemcpy(host, counts.data(), sizeof host, hipMemcpyDeviceToHost), "reading counters");
         rt_traffic t{};
         t.paths = static_cast<uint64_t>(passes) * static_cast<uint64_t>(values() / 3);
         t.rays = host[0];
         t.shadow_rays = host[1];
+        // (a packed tree's walks read 64-byte nodes)
+        const double node_bytes = dscene_.view().packed_nodes != nullptr ? sizeof(dev::packed_node)
+                                                                         : sizeof(dev::wide_node<Real>);
         const double sizes[RT_TRAFFIC_EVENTS] = {
-            sizeof(dev::wide_node<Real>), sizeof(dev::sphere_rec<Real>), sizeof(dev::moving_sphere_rec<Real>),
+            node_bytes, sizeof(dev::sphere_rec<Real>), sizeof(dev::moving_sphere_rec<Real>),
             sizeof(dev::quad_rec<Real>), sizeof(dev::medium_rec<Real>), sizeof(dev::quad_rec<Real>),
             4.0 + sizeof(dev::material_rec<Real>), sizeof(dev::sphere_frame<Real>), sizeof(dev::texture_rec<Real>),
             sizeof(dev::texel), 4.0 * sizeof(Real) + 3.0, sizeof(dev::light_rec<Real>),
@@ -700,6 +703,10 @@ public:
             t.events[e] = host[2 + e];
             t.event_bytes[e] = sizes[e];
             bytes += sizes[e] * static_cast<double>(host[2 + e]);
+        }
+        for (int k = 0; k < RT_TRAFFIC_PHASES; ++k)
+        {
+            t.phase_lane_cycles[k] = host[2 + RT_TRAFFIC_EVENTS + k];
         }
         // the pixel's double sums, read and written once per launch of `passes` passes
         bytes += 48.0 * static_cast<double>(values() / 3);
