@@ -236,24 +236,23 @@ def serving(rt, rates, per_rate, seed, tau):
 
 
 def rmse_protocol(rt, workload, width, oracle_spp):
-    """RMSE of a workload's FP32 render against an FP64 render of `oracle_spp` passes (the oracle), at a reduced width
-    (the same scene, camera and depth; fewer pixels so that the oracle's 64k passes take a minute), for the megakernel
-    and the wavefront at the workload's spp, and the spp each needs to reach the megakernel's RMSE. The two
-    integrators trace the same paths, so their RMSEs are equal at equal spp. The quality integrator has no FP64 form:
-    for the triangle scene the oracle is FP32 at the same pass count, and says so."""
+    """RMSE of a workload's FP32 render against an FP64 render of `oracle_spp` passes (the oracle: the FP64 megakernel,
+    RT_FLOAT64_ALL, which renders the quality integrator's scenes in float64 too), at a reduced width (the same scene,
+    camera and depth; fewer pixels so that the oracle's 64k passes take a minute), for the megakernel and the
+    wavefront at the workload's spp, and the spp each needs to reach the megakernel's RMSE. The two integrators trace
+    the same paths, so their RMSEs are equal at equal spp."""
     import numpy as np
 
     name, _, spp, sampling, description = WORKLOADS[workload]
     oracle_scene = rt.build_scene(name, width=width, samples=oracle_spp, strata=1, sampling=sampling)
-    fp64 = name != "triangles" and sampling is None
     t0 = time.perf_counter()
-    with rt.Session(oracle_scene, seed=12345, precision="float64" if fp64 else "float32",
-                    integrator="megakernel", launch_seconds=0.2) as s:
+    with rt.Session(oracle_scene, seed=12345, precision="float64_all", integrator="megakernel",
+                    launch_seconds=0.2) as s:
         s.render_to(oracle_spp)
         oracle = s.pixels()
     oracle_seconds = time.perf_counter() - t0
     out = {"workload": description, "width": oracle.shape[1], "height": oracle.shape[0],
-           "oracle": f"{'FP64' if fp64 else 'FP32'} megakernel, {oracle_spp} passes, {oracle_seconds:.1f} s",
+           "oracle": f"FP64 megakernel, {oracle_spp} passes, {oracle_seconds:.1f} s",
            "oracle_mean": float(oracle.mean())}
     scene = rt.build_scene(name, width=width, samples=spp, sampling=sampling)
     for integrator in ("megakernel", "wavefront"):

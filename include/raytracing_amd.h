@@ -317,7 +317,10 @@ const char* rt_scene_names(void);
 enum rt_precision
 {
     RT_FLOAT32 = 0,
-    RT_FLOAT64 = 1,
+    RT_FLOAT64 = 1,     /* the books' scenes (the quality integrator is float32 only, as the reference's; a scene that
+                           needs it is RT_ERROR_INVALID_ARGUMENT) */
+    RT_FLOAT64_ALL = 2, /* float64 for every scene, the quality integrator's too, on the megakernels: no counterpart in
+                           the reference, for the FP64 oracle that renders' RMSE is measured against */
 };
 
 enum rt_integrator

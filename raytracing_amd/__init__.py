@@ -245,6 +245,7 @@ RT_OK, RT_ERROR_RUNTIME, RT_ERROR_INVALID_ARGUMENT, RT_ERROR_LOGIC = 0, 1, 2, 3
 SAMPLING = {"uniform": 0, "material": 1, "lights": 2, "mixture": 3, "nee": 4, "mis": 5}
 MATERIAL_KINDS = {"lambertian": 0, "metal": 1, "dielectric": 2, "diffuse_light": 3, "isotropic": 4}
 INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2, "plain": 3}
+PRECISIONS = {"float32": 0, "fp32": 0, "float64": 1, "fp64": 1, "float64_all": 2}
 TREES = {"default": 0, "median": 1, "sah": 2, "two_level": 3}
 NODES = {"auto": 0, "float": 1, "packed": 2}
 TONE_MAPS = {"books": 0, "none": 1, "agx": 2, "neutral": 3}
@@ -311,7 +312,7 @@ def build_scene(name: str, **overrides) -> BuiltScene:
 def _session_options(seed=0, precision="float32", integrator="auto", launch_seconds=0.0, block_size=0,
                      wavefront_slots=0, wavefront_graph=0, wavefront_drain=0, megakernel_waves=0,
                      tree="default", nodes="auto") -> SessionOptions:
-    return SessionOptions(seed, 1 if precision in ("float64", "fp64") else 0, INTEGRATORS[integrator],
+    return SessionOptions(seed, PRECISIONS[precision], INTEGRATORS[integrator],
                           launch_seconds, block_size, wavefront_slots, wavefront_graph, wavefront_drain,
                           megakernel_waves, TREES[tree] if isinstance(tree, str) else int(tree),
                           NODES[nodes] if isinstance(nodes, str) else int(nodes))

@@ -570,9 +570,9 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree, int32_t nodes)
 
     dev::scene_view<Real>& v = view_;
     quality_ = needs_quality(s);
-    if (quality_ && sizeof(Real) == sizeof(float))
+    if (quality_)
     {
-        // the quality integrator's tables (src/gpu/quality.cu:112-170)
+        // the quality integrator's tables (src/gpu/quality.cu:112-170; float records, read as Real by the FP64 kernels)
         const lamp_set set = build_lamp_set(s, tree_.sphere_ids, tree_.quad_ids, tree_.prim_materials.size());
         lamps_ = device_array<dev::emitter_rec>(set.lamps);
         lamp_of_ = device_array<int32_t>(set.lamp_of);

@@ -292,8 +292,8 @@ struct far_stack
 // samples and their shadow rays at every vertex that isn't a mirror's (quality.hpp). One ray per iteration: a path's
 // shadow ray is its own iteration, walked by the same code as the paths' next rays (step_quality), so a wave's lanes
 // walk together whichever kind of ray each traces, and the shading's values aren't held across a second walk.
-template <bool Count, int Walk, typename Stack>
-__device__ inline void quality_paths(staged_sample<float>* __restrict__ staging,
+template <typename Real, bool Count, int Walk, typename Stack>
+__device__ inline void quality_paths(staged_sample<Real>* __restrict__ staging,
                                      unsigned long long* __restrict__ ray_count,
                                      unsigned long long* __restrict__ tallies, const Stack& stack)
 {
@@ -302,13 +302,13 @@ __device__ inline void quality_paths(staged_sample<float>* __restrict__ staging,
     unsigned long long rays = 0;
     uint32_t shadow_rays = 0;
     int32_t at = -1;
-    quality_path qp{};
+    quality_path<Real> qp{};
     pcg32 g{};
     while (true)
     {
-        const megakernel_args<float>& a = first_kernel_argument<megakernel_args<float>>();
-        const camera_rec<float>& cam = a.cam;
-        const scene_view<float>& sc = a.sc;
+        const megakernel_args<Real>& a = first_kernel_argument<megakernel_args<Real>>();
+        const camera_rec<Real>& cam = a.cam;
+        const scene_view<Real>& sc = a.sc;
         const chunk_queue& q = a.q;
         int si, sj, pass_local;
         if (w.claim(q, at < 0, si, sj, pass_local))
@@ -329,7 +329,7 @@ __device__ inline void quality_paths(staged_sample<float>* __restrict__ staging,
             continue;
         }
         tl.lap(phase_queue);
-        vec3<float> sample;
+        vec3<Real> sample;
         if (step_quality<Walk>(qp, sc, cam, g, sample, stack, tl, rays, shadow_rays))
         {
             stage(staging + at, sample);
@@ -341,10 +341,11 @@ __device__ inline void quality_paths(staged_sample<float>* __restrict__ staging,
 
 // Split: the stack's first far.near levels in LDS, the rest in HBM (a deep tree, such as a large mesh's, would
 // otherwise hold so much LDS per wave that fewer waves fit than the registers allow)
-// Walk: the tree's walk (walk_all, walk_instances or walk_packed)
-template <bool Count, int Waves, bool Split, int Walk>
+// Walk: the tree's walk (walk_all, walk_instances or walk_packed). Real: float, or double for the FP64 oracle (at the
+// compiler's register budget)
+template <typename Real, bool Count, int Waves, bool Split, int Walk>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Waves > 0 ? Waves : 1, 8)))
-quality_megakernel(const megakernel_args<float> args, staged_sample<float>* __restrict__ staging,
+quality_megakernel(const megakernel_args<Real> args, staged_sample<Real>* __restrict__ staging,
                    unsigned long long* __restrict__ ray_count, unsigned long long* __restrict__ tallies,
                    const far_stack far)
 {
@@ -352,11 +353,11 @@ quality_megakernel(const megakernel_args<float> args, staged_sample<float>* __re
     {
         const uint32_t column = blockIdx.x * blockDim.x + threadIdx.x;
         const split_stack stack{block_stack(far.near), far.near, far.refs + column, far.ts + column, far.stride};
-        quality_paths<Count, Walk>(staging, ray_count, tallies, stack);
+        quality_paths<Real, Count, Walk>(staging, ray_count, tallies, stack);
     }
     else
     {
-        quality_paths<Count, Walk>(staging, ray_count, tallies, block_stack(args.sc.stack_depth));
+        quality_paths<Real, Count, Walk>(staging, ray_count, tallies, block_stack(args.sc.stack_depth));
     }
 }
 
@@ -400,7 +401,7 @@ __global__ void __launch_bounds__(128) plain_megakernel(const plain_args<Real> a
             vec3<Real> sample;
             if constexpr (Mode == mode_quality)
             {
-                quality_path qp = start_quality(cam, sc.motion != 0, i, j, pass, g);
+                quality_path<Real> qp = start_quality(cam, sc.motion != 0, i, j, pass, g);
                 do
                 {
                     ++rays;
@@ -524,24 +525,30 @@ void* static_scratch(size_t* bytes)
 
 namespace
 {
-template <bool Count, int Waves, int Walk>
+template <typename Real, bool Count, int Waves, int Walk>
 const void* quality_megakernel_for(bool split)
 {
-    return split ? reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, true, Walk>)
-                 : reinterpret_cast<const void*>(&dev::quality_megakernel<Count, Waves, false, Walk>);
+    return split ? reinterpret_cast<const void*>(&dev::quality_megakernel<Real, Count, Waves, true, Walk>)
+                 : reinterpret_cast<const void*>(&dev::quality_megakernel<Real, Count, Waves, false, Walk>);
 }
 
 // walk: the quality kernels' walk of the scene's tree (dev::tree_walk)
 template <typename Real, bool Count, int Waves>
 const void* megakernel_for_waves(int mode, bool split, int walk, bool marble)
 {
-    if constexpr (sizeof(Real) == sizeof(float))
+    if (mode == dev::mode_quality)
     {
-        if (mode == dev::mode_quality)
+        if constexpr (sizeof(Real) == sizeof(float))
         {
-            return walk == dev::walk_instances ? quality_megakernel_for<Count, Waves, dev::walk_instances>(split)
-                   : walk == dev::walk_packed  ? quality_megakernel_for<Count, Waves, dev::walk_packed>(split)
-                                               : quality_megakernel_for<Count, Waves, dev::walk_all>(split);
+            return walk == dev::walk_instances ? quality_megakernel_for<Real, Count, Waves, dev::walk_instances>(split)
+                   : walk == dev::walk_packed  ? quality_megakernel_for<Real, Count, Waves, dev::walk_packed>(split)
+                                               : quality_megakernel_for<Real, Count, Waves, dev::walk_all>(split);
+        }
+        else
+        {
+            // (FP64: the compiler's budget, float nodes)
+            return walk == dev::walk_instances ? quality_megakernel_for<Real, Count, 0, dev::walk_instances>(split)
+                                               : quality_megakernel_for<Real, Count, 0, dev::walk_all>(split);
         }
     }
     if constexpr (sizeof(Real) == sizeof(float))
@@ -629,7 +636,7 @@ hipError_t launch_megakernel(int mode, const dev::camera_rec<Real>& cam, const d
                              hipStream_t stream, unsigned long long* tallies, int waves, void* far_buffer,
                              size_t far_bytes, int grid_ways)
 {
-    if ((mode == dev::mode_quality && sizeof(Real) != sizeof(float)) || block_threads % 64 != 0 || grid_ways < 1 ||
+    if (block_threads % 64 != 0 || grid_ways < 1 ||
         block_threads < 64 || block_threads > 256 || work.pass_count < 1 || work.row_count < 1 ||
         staging_samples > static_cast<size_t>(INT32_MAX))
     {
@@ -731,16 +738,18 @@ namespace
 template <typename Real>
 const void* plain_megakernel_for(int mode, int walk)
 {
-    if constexpr (sizeof(Real) == sizeof(float))
+    if (mode == dev::mode_quality)
     {
-        if (mode == dev::mode_quality)
+        if constexpr (sizeof(Real) == sizeof(float))
         {
-            return walk == dev::walk_instances
-                       ? reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, dev::walk_instances>)
-                   : walk == dev::walk_packed
-                       ? reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, dev::walk_packed>)
-                       : reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, dev::walk_all>);
+            if (walk == dev::walk_packed)
+            {
+                return reinterpret_cast<const void*>(&dev::plain_megakernel<float, dev::mode_quality, dev::walk_packed>);
+            }
         }
+        return walk == dev::walk_instances
+                   ? reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_quality, dev::walk_instances>)
+                   : reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_quality, dev::walk_all>);
     }
     return mode == dev::mode_all_books
                ? reinterpret_cast<const void*>(&dev::plain_megakernel<Real, dev::mode_all_books, dev::walk_books>)
@@ -753,7 +762,7 @@ hipError_t launch_plain_megakernel(int mode, const dev::camera_rec<Real>& cam, c
                                    uint64_t seed, const dev::launch_work& work, double* sums, unsigned long long* rays,
                                    hipStream_t stream)
 {
-    if ((mode == dev::mode_quality && sizeof(Real) != sizeof(float)) || work.pass_count < 1 || work.row_count < 1)
+    if (work.pass_count < 1 || work.row_count < 1)
     {
         return hipErrorInvalidValue;
     }

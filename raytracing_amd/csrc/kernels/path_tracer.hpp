@@ -42,6 +42,8 @@ __device__ inline float m_sin(float x) { return sinf(x); }
 __device__ inline double m_sin(double x) { return sin(x); }
 __device__ inline float m_log(float x) { return logf(x); }
 __device__ inline double m_log(double x) { return log(x); }
+__device__ inline float m_exp(float x) { return expf(x); }
+__device__ inline double m_exp(double x) { return exp(x); }
 __device__ inline void m_sincospi(float x, float* s, float* c) { sincospif(x, s, c); }
 __device__ inline void m_sincospi(double x, double* s, double* c) { sincospi(x, s, c); }
 

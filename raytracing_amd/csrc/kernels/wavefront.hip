@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(256) wf_shade_quality(const camera_rec<float> 
         s = queued_slot(pool, prefix, g);
         slot_path sp = load_path(pool, s);
         const float4 st = pool.qst[s];
-        quality_path q;
+        quality_path<float> q;
         q.base = sp.p;
         q.radiance = {st.x, st.y, st.z};
         q.bsdf_pdf = st.w;
@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(128) wf_drain(const camera_rec<float> cam, con
         const uint32_t s = active_slot(pool, 0, prefix, k);
         slot_path sp = load_path(pool, s);
         tally<false> tl;
-        quality_path q;
+        quality_path<float> q;
         if constexpr (Mode == mode_quality)
         {
             const float4 st = pool.qst[s];

@@ -65,7 +65,7 @@ public:
         : tables(scene_tables::copy_of(scene))
     {
         view = tables.view();
-        if (precision == RT_FLOAT64)
+        if (precision != RT_FLOAT32)
         {
             f64 = std::make_unique<device_scene<double>>(view, tree, nodes);
         }
@@ -95,13 +95,15 @@ public:
 std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision, int32_t tree, int32_t nodes)
 {
     validate_scene(scene);
-    if (precision != RT_FLOAT32 && precision != RT_FLOAT64)
+    if (precision != RT_FLOAT32 && precision != RT_FLOAT64 && precision != RT_FLOAT64_ALL)
     {
         throw std::invalid_argument("unknown precision");
     }
     const bool quality = scene.mesh_count != 0 || scene.microfacet_count != 0 || scene.environment != nullptr ||
                          scene.combination != RT_COMBINE_BSDF_ONLY;
-    if (precision != RT_FLOAT32 && quality)
+    // (the reference's quality kernel is float32 only, src/gpu/quality.cu:53-109, tests/routing_tests.cu:53-67;
+    // RT_FLOAT64_ALL renders it in float64 for an oracle)
+    if (precision == RT_FLOAT64 && quality)
     {
         throw std::invalid_argument("triangles, GGX, environment maps and light samples render in float32 only");
     }
@@ -1060,7 +1062,7 @@ namespace
 {
 void check_options(const rt_session_options& options)
 {
-    if (options.precision != RT_FLOAT32 && options.precision != RT_FLOAT64)
+    if (options.precision != RT_FLOAT32 && options.precision != RT_FLOAT64 && options.precision != RT_FLOAT64_ALL)
     {
         throw std::invalid_argument("unknown precision");
     }
@@ -1102,11 +1104,11 @@ std::unique_ptr<session> make_session(std::shared_ptr<const shared_scene> scene,
                                       const rt_session_options& options)
 {
     check_options(options);
-    if ((options.precision == RT_FLOAT64) != (scene->f64 != nullptr))
+    if ((options.precision != RT_FLOAT32) != (scene->f64 != nullptr))
     {
         throw std::invalid_argument("the session's precision must be the resident scene's");
     }
-    if (options.precision == RT_FLOAT64)
+    if (options.precision != RT_FLOAT32)
     {
         return std::make_unique<session_impl<double>>(std::move(scene), camera, options);
     }

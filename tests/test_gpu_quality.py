@@ -279,3 +279,44 @@ def test_instanced_meshes_find_the_baked_hits(rt):
         if v.consistent:
             return
     raise AssertionError("two-level and baked trees differ beyond noise twice: " + " | ".join(v.summary() for v in verdicts))
+
+
+def test_fp64_quality_furnaces_exactly_one(rt):
+    """The quality integrator in float64 (RT_FLOAT64_ALL, the RMSE protocol's oracle): the furnaces' closed forms hold
+    exactly as in float32"""
+    for name, sampling in (("env-furnace", "material"), ("mesh-furnace", "material"), ("mesh-furnace", "mis")):
+        scene = rt.build_scene(name, width=160, samples=16, sampling=sampling)
+        img = rt.render(scene, seed=4, precision="float64_all").pixels
+        assert np.all(img == 1.0), f"{name} {sampling}: {np.count_nonzero(img != 1.0)} values differ from 1"
+
+
+@pytest.mark.parametrize("name,sampling,width,spp", [("triangles", "mis", 96, 64), ("cornell-ggx", "mis", 96, 256),
+                                                     ("sun-test", "nee", 96, 64), ("glass", "mis", 96, 256)])
+def test_fp64_quality_agrees_with_fp32(rt, name, sampling, width, spp):
+    """The float64 quality integrator draws other numbers (53-bit draws) in double arithmetic: its render of the same
+    scene must not differ from the float32 one by more than noise (ImageCompare, with the re-run rule)"""
+    scene = rt.build_scene(name, width=width, samples=spp, sampling=sampling)
+    verdicts = []
+    for seeds in ((21, 22), (23, 24)):
+        a = rt.render(scene, seed=seeds[0], precision="float32", integrator="megakernel").pixels
+        b = rt.render(scene, seed=seeds[1], precision="float64_all", integrator="megakernel").pixels
+        assert np.all(np.isfinite(b))
+        v = _consistent(a, b, f"{name} {sampling} fp32 vs fp64")
+        verdicts.append(v)
+        if v.consistent:
+            return
+    raise AssertionError(f"{name}: float32 and float64 differ beyond noise twice: " +
+                         " | ".join(v.summary() for v in verdicts))
+
+
+def test_fp64_refused_where_the_reference_refuses(rt):
+    """RT_FLOAT64 keeps the reference's refusal of the quality integrator (tests/routing_tests.cu:53-67); only the
+    explicit RT_FLOAT64_ALL renders it in float64; the wavefront is float32 only"""
+    ggx = rt.build_scene("cornell-ggx", width=32, samples=4)
+    with pytest.raises(ValueError, match="float32 only"):
+        rt.Session(ggx, precision="float64")
+    with rt.Session(ggx, precision="float64_all") as s:
+        s.render_to(2)
+        assert "fp64" in s.kernel().name.decode() or s.kernel().name
+    with pytest.raises(ValueError, match="float32 only"):
+        rt.Session(ggx, precision="float64_all", integrator="wavefront")
