@@ -8,7 +8,8 @@ them; every launch is waited for). Also reported:
     session's own stream, over the timed region), against 8 TB/s of HBM; `traffic` is the HBM bytes per launch that
     rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured for the same workload, when profiles/hbm_traffic.txt
     holds a line for it (scaled by the counters' reading for a copy kernel of known size);
-    `hbm_frac` is that measured traffic over the live launch time against 8 TB/s, and `issue` the kernel's active
+    `hbm_frac` is that measured traffic over the live launch time against 8 TB/s, `l2` the L2's bytes per launch and
+    GB/s from rocprofv3's TCC request counters (profiles/l2_traffic.txt), and `issue` the kernel's active
     lanes per VALU instruction, VALU busy share and waves per SIMD from rocprofv3's SQ counters
     (profiles/valu_issue.txt) for the same workload;
   - cpu_baseline: the CPU restatement of the reference renderer (double, mt19937_64 per row, the reference's
@@ -146,6 +147,19 @@ def measured_traffic(workload: str, key: str):
         parts = line.split()
         if len(parts) >= 4 and not line.startswith("#") and parts[0] == workload and parts[1] == key:
             return float(parts[2]) + float(parts[3])
+    return None
+
+
+def measured_l2(workload: str, key: str):
+    """The L2's bytes per launch (read requests x 128 B + write requests x 64 B, the copy kernel's sizes) and hit rate
+    for this workload and kernel, from the rocprofv3 TCC counter summary (profiles/l2_traffic.txt), or None"""
+    path = os.path.join(ROOT, "profiles", "l2_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 5 and not line.startswith("#") and parts[0] == workload and parts[1] == key:
+            return 128.0 * float(parts[2]) + 64.0 * float(parts[3]), float(parts[4])
     return None
 
 
@@ -412,6 +426,7 @@ def main():
     profiled = args.scene + ("-two-level" if args.tree == "two_level" else "")
     hbm = measured_traffic(profiled, traffic_key(kernel))
     issue = measured_issue(profiled, traffic_key(kernel))
+    l2 = measured_l2(profiled, traffic_key(kernel))
     plain = others.get("plain")
     line = {
         "metric": "Mpaths/s (1080^2 final scene)" if args.scene == "final" else f"Mpaths/s ({args.scene})",
@@ -443,6 +458,9 @@ def main():
                      # measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE) over the live launch time
                      "hbm_frac": None if hbm is None else round(hbm / avg_launch / 1e9 / HBM_PEAK_GBS, 4),
                      "issue": issue,
+                     # the L2's measured bytes per launch (requests x their sizes) over the live launch time
+                     "l2": None if l2 is None else {"bytes_per_launch": round(l2[0]),
+                                                    "GB/s": round(l2[0] / avg_launch / 1e9, 1), "hit_rate": l2[1]},
                      # the counting instance's lanes' time by phase (queue, walk, media, shade)
                      "lane_time_by_phase": phases,
                      "bytes_per_path": round(bytes_per_path, 1), "avg_launch_ms": round(1000 * avg_launch, 3),
