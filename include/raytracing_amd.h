@@ -484,6 +484,15 @@ int rt_session_closest_hit(rt_session* session, const float* origins, const floa
                            const int32_t* from, const uint8_t* inward, int64_t count, int32_t* index, float* t,
                            uint8_t* entering);
 
+/* Shadow rays (src/gpu/shadow.cuh:26-120, the reference's `occluded`): for each ray, whether any primitive but
+ * skip[k] (a global id, or -1) lies in (0.001, t_max[k]) along it, with the leaving primitive's rules (from, inward) of
+ * rt_session_closest_hit; spheres on the open interval, quads and triangles on the closed one. Both of the product's
+ * shadow walks answer: `blocked` the wavefront's (occluded), `walked` the quality megakernel's (its closest-hit walk
+ * in any-hit mode), 1 or 0 per ray; either may be NULL. Float32 sessions only. */
+int rt_session_occluded(rt_session* session, const float* origins, const float* directions, const float* times,
+                        const int32_t* from, const uint8_t* inward, const float* t_max, const int32_t* skip,
+                        int64_t count, uint8_t* blocked, uint8_t* walked);
+
 /* The reference's own traversal, for reference-exact closest hits (src/gpu/bvh.cuh:576-706): the session's binary tree
  * (with RT_TREE_MEDIAN, the reference's tree) walked near first in the reference's order (the first child when both
  * are entered at the same distance; the farther child pushed; a popped node visited only when entered before the

@@ -78,6 +78,11 @@ def _load() -> C.CDLL:
     lib.twin_brute_force.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
                                      P(C.c_float), P(C.c_int32), P(C.c_uint8), C.c_int64, P(C.c_int32),
                                      P(C.c_float), P(C.c_int32)]
+    lib.twin_custom_occluded.restype = C.c_int
+    lib.twin_custom_occluded.argtypes = [P(C.c_double), C.c_int64, P(C.c_double), C.c_int64, P(C.c_float),
+                                         P(C.c_float), P(C.c_float), P(C.c_int32), P(C.c_uint8), P(C.c_float),
+                                         P(C.c_int32), C.c_int64, P(C.c_uint8), P(C.c_uint8), P(C.c_uint8),
+                                         P(C.c_int32), P(C.c_int32)]
     return lib
 
 
@@ -248,6 +253,34 @@ def twin_custom_closest_hit(spheres, quads, origins, directions, walk_order: boo
                                  _ptr(ent, C.c_uint8), _ptr(sid, C.c_int32), _ptr(qid, C.c_int32)):
         raise RuntimeError(L.twin_last_error().decode())
     return ids, t, ent, sid, qid
+
+
+def twin_custom_occluded(spheres, quads, origins, directions, times, from_index, inward, t_max, skip_index):
+    """Shadow rays by brute force in a scene of spheres (x, y, z, r, dx, dy, dz rows; a zero displacement stands still)
+    and quads (q, u, v rows); from_index and skip_index name input primitives (spheres, then quads) or -1:
+    (blocked, band, near, sphere_ids, quad_ids), the ids in the twin tree's numbering"""
+    L = lib()
+    sp = np.ascontiguousarray(np.asarray(spheres, np.float64).reshape(-1, 7))
+    qd = np.ascontiguousarray(np.asarray(quads, np.float64).reshape(-1, 9))
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+    tm = np.ascontiguousarray(times, np.float32)
+    f = np.ascontiguousarray(from_index, np.int32)
+    inw = np.ascontiguousarray(inward, np.uint8)
+    tx = np.ascontiguousarray(t_max, np.float32)
+    sk = np.ascontiguousarray(skip_index, np.int32)
+    n = len(o)
+    blocked = np.zeros(n, np.uint8)
+    band = np.zeros(n, np.uint8)
+    near = np.zeros(n, np.uint8)
+    sid = np.zeros(len(sp), np.int32)
+    qid = np.zeros(len(qd), np.int32)
+    if L.twin_custom_occluded(_ptr(sp, C.c_double), len(sp), _ptr(qd, C.c_double), len(qd), _ptr(o, C.c_float),
+                              _ptr(d, C.c_float), _ptr(tm, C.c_float), _ptr(f, C.c_int32), _ptr(inw, C.c_uint8),
+                              _ptr(tx, C.c_float), _ptr(sk, C.c_int32), n, _ptr(blocked, C.c_uint8),
+                              _ptr(band, C.c_uint8), _ptr(near, C.c_uint8), _ptr(sid, C.c_int32), _ptr(qid, C.c_int32)):
+        raise RuntimeError(L.twin_last_error().decode())
+    return blocked, band, near, sid, qid
 
 
 def sample_doubles(seed: int, pixel: int, pass_: int, n: int) -> np.ndarray:

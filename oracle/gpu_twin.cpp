@@ -598,6 +598,123 @@ inline scene_tree make(const char* name, int width, int spp, int earth_width)
     st.t = builder(f.spheres, f.quads, leaf_size(f.spheres.size() + f.quads.size())).build();
     return st;
 }
+
+// --- shadow rays by brute force (the reference's any-hit check, tests/lamp_tests.cu:1618-1900) ---------------------
+// The question of src/gpu/shadow.cuh:26-120: anything but `skip` in (0.001, t_max)? Each primitive is tested over
+// (0.001, inf) with closest_hit's rules for the primitive the ray leaves; its distance is then compared with t_max,
+// open for a sphere and closed for a quad: neither test's choice of root or distance depends on t_max, so this is the
+// same answer as a test over (0.001, t_max) itself.
+struct shadow_answer
+{
+    bool blocked = false;
+    bool band = false; // a hit within 1e-4 t_max of t_max: the walk's box tests may round either way there
+    bool near = false; // a decision within a few float roundings of going the other way (near_decision)
+};
+
+inline float norm(const f3& v) { return std::sqrt(dot(v, v)); }
+
+// Whether primitive g's test for this ray sits within rounding of another outcome: a line that grazes a sphere, a
+// root or a quad distance at t_min, a quad's plane nearly parallel to the ray, a hit on a quad's edge. The zones are a
+// few float steps (2^-24 ~ 6e-8) of the magnitudes each test subtracts, with a margin: 1e-6 relative.
+inline bool near_decision(const sphere* s, const quad* q, const ray& r, float inverse_a, bool own, float t_max)
+{
+    constexpr float rel = 1e-6f;
+    constexpr float t_min = 0.001f;
+    if (q)
+    {
+        const float denom = dot(q->normal, r.direction);
+        if (std::fabs(denom) < 1e-6f)
+        {
+            return true;
+        }
+        const f3 otq = sub(q->q, r.origin);
+        const float t = dot(q->normal, otq) / denom;
+        const float size = norm(otq) + std::fabs(t);
+        if (std::fabs(t - t_min) <= rel * (1.0f + size) / std::fabs(denom))
+        {
+            return true;
+        }
+        if (!(t > 0.5f * t_min && t <= 1.0002f * t_max))
+        {
+            return false;
+        }
+        const f3 p{std::fmaf(t, r.direction.x, -otq.x), std::fmaf(t, r.direction.y, -otq.y),
+                   std::fmaf(t, r.direction.z, -otq.z)};
+        const float alpha = dot(p, q->alpha_axis);
+        const float beta = dot(p, q->beta_axis);
+        const float ea = rel * (1.0f + size) * norm(q->alpha_axis);
+        const float eb = rel * (1.0f + size) * norm(q->beta_axis);
+        const bool inside = alpha >= -ea && alpha <= 1.0f + ea && beta >= -eb && beta <= 1.0f + eb;
+        const bool edge = std::fabs(alpha) <= ea || std::fabs(alpha - 1.0f) <= ea || std::fabs(beta) <= eb ||
+                          std::fabs(beta - 1.0f) <= eb;
+        return inside && edge;
+    }
+    const f3 oc = sub(s->center, r.origin);
+    const float h = dot(r.direction, oc) * inverse_a;
+    if (!(h > -s->radius - 1.0f && h < t_max + s->radius + 1.0f))
+    {
+        return false; // the sphere lies well before the origin or beyond t_max
+    }
+    const f3 l{std::fmaf(-h, r.direction.x, oc.x), std::fmaf(-h, r.direction.y, oc.y),
+               std::fmaf(-h, r.direction.z, oc.z)};
+    const float r2 = s->radius * s->radius;
+    const float disc = r2 - dot(l, l);
+    const float size = norm(oc) + std::fabs(h);
+    if (std::fabs(disc) <= rel * (1.0f + size) * (1.0f + s->radius) + 10.0f * rel * r2)
+    {
+        return true; // grazing
+    }
+    if (disc < 0.0f)
+    {
+        return false;
+    }
+    const float half = std::sqrt(disc * inverse_a);
+    const float tol = rel * (1.0f + size);
+    return (!own && std::fabs(h - half - t_min) <= tol) || std::fabs(h + half - t_min) <= tol;
+}
+
+inline shadow_answer shadow_by_brute_force(const tree& tr, const ray& r, float t_max, int from, bool inward, int skip)
+{
+    shadow_answer a;
+    const float inverse_a = 1.0f / dot(r.direction, r.direction);
+    const auto blocks = [&](float t, bool closed) {
+        a.blocked = a.blocked || t < t_max || (closed && t == t_max);
+        a.band = a.band || (std::isfinite(t_max) && std::fabs(t - t_max) <= 1e-4f * t_max);
+    };
+    for (std::size_t i = 0; i < tr.spheres.size() + tr.moving.size(); ++i)
+    {
+        const int g = static_cast<int>(i);
+        const bool own = g == from;
+        if (g == skip || (own && !inward))
+        {
+            continue;
+        }
+        const sphere s = g < tr.moving_base ? tr.spheres[i] : tr.moving[i - tr.moving_base].at(r.time);
+        float t;
+        bool e;
+        if (hit_sphere(s, r, inverse_a, 0.001f, INFINITY, own, t, e))
+        {
+            blocks(t, false);
+        }
+        a.near = a.near || near_decision(&s, nullptr, r, inverse_a, own, t_max);
+    }
+    for (std::size_t i = 0; i < tr.quads.size(); ++i)
+    {
+        const int g = tr.quad_base + static_cast<int>(i);
+        if (g == skip || g == from)
+        {
+            continue;
+        }
+        float t;
+        bool f;
+        if (hit_quad(tr.quads[i], r, 0.001f, INFINITY, t, f))
+        {
+            blocks(t, true);
+        }
+        a.near = a.near || near_decision(nullptr, &tr.quads[i], r, inverse_a, false, t_max);
+    }
+    return a;
+}
 }
 
 using namespace twin;
@@ -878,6 +995,75 @@ int twin_brute_force(const char* name, int width, int spp, int earth_width, cons
             {
                 tied[k] = 0;
             }
+        }
+        return 0;
+    }
+    catch (const std::exception& e)
+    {
+        last_error = e.what();
+        return -1;
+    }
+}
+
+// Shadow rays in a scene of plain primitives by brute force: spheres as center x, y, z, radius and displacement
+// x, y, z (7 doubles each; a zero displacement stands still), quads as q, u, v (9 doubles each). Per ray: `from` and
+// `skip` as an input primitive's index (spheres, then quads) or -1, the ray's time, inward and t_max; out: blocked,
+// band and near (shadow_answer), 1 or 0 each. sphere_ids and quad_ids (may be null) receive each input primitive's
+// global id in the tree built for the scene with the reference's leaf sizes.
+int twin_custom_occluded(const double* spheres, std::int64_t sphere_count, const double* quads,
+                         std::int64_t quad_count, const float* origins, const float* directions, const float* times,
+                         const std::int32_t* from, const std::uint8_t* inward, const float* t_max,
+                         const std::int32_t* skip, std::int64_t n, std::uint8_t* blocked, std::uint8_t* band,
+                         std::uint8_t* near, std::int32_t* sphere_ids, std::int32_t* quad_ids)
+{
+    try
+    {
+        std::vector<oracle::flat_sphere> fs;
+        for (std::int64_t k = 0; k < sphere_count; ++k)
+        {
+            const double* v = spheres + 7 * k;
+            fs.push_back({{v[0], v[1], v[2]}, v[3], 0, {v[4], v[5], v[6]}});
+        }
+        std::vector<oracle::flat_quad> fq;
+        for (std::int64_t k = 0; k < quad_count; ++k)
+        {
+            const double* v = quads + 9 * k;
+            fq.push_back({{v[0], v[1], v[2]}, {v[3], v[4], v[5]}, {v[6], v[7], v[8]}, 0});
+        }
+        const tree tr = builder(fs, fq, leaf_size(fs.size() + fq.size())).build();
+        const auto id_of = [&](std::int32_t input) {
+            if (input < 0)
+            {
+                return -1;
+            }
+            if (input < sphere_count)
+            {
+                return tr.sphere_ids[static_cast<std::size_t>(input)];
+            }
+            if (input < sphere_count + quad_count)
+            {
+                return tr.quad_ids[static_cast<std::size_t>(input - sphere_count)];
+            }
+            throw std::out_of_range("a primitive index beyond the scene");
+        };
+        for (std::int64_t k = 0; k < n; ++k)
+        {
+            const ray r{{origins[3 * k], origins[3 * k + 1], origins[3 * k + 2]},
+                        {directions[3 * k], directions[3 * k + 1], directions[3 * k + 2]},
+                        times[k]};
+            const shadow_answer a = shadow_by_brute_force(tr, r, t_max[k], id_of(from[k]), inward[k] != 0,
+                                                          id_of(skip[k]));
+            blocked[k] = a.blocked ? 1 : 0;
+            band[k] = a.band ? 1 : 0;
+            near[k] = a.near ? 1 : 0;
+        }
+        if (sphere_ids)
+        {
+            std::copy(tr.sphere_ids.begin(), tr.sphere_ids.end(), sphere_ids);
+        }
+        if (quad_ids)
+        {
+            std::copy(tr.quad_ids.begin(), tr.quad_ids.end(), quad_ids);
         }
         return 0;
     }

@@ -209,6 +209,9 @@ _SIGNATURES = {
     "rt_render": (C.c_int, [_P(Scene), _P(RenderOptions), _P(C.c_double), C.c_size_t, _P(RenderStats)]),
     "rt_session_closest_hit": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_int32),
                                          _P(C.c_uint8), C.c_int64, _P(C.c_int32), _P(C.c_float), _P(C.c_uint8)]),
+    "rt_session_occluded": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_int32),
+                                      _P(C.c_uint8), _P(C.c_float), _P(C.c_int32), C.c_int64, _P(C.c_uint8),
+                                      _P(C.c_uint8)]),
     "rt_session_reference_hits": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float),
                                             _P(C.c_int32), _P(C.c_uint8), C.c_int64, _P(C.c_int32), _P(C.c_float),
                                             _P(C.c_uint8)]),
@@ -433,6 +436,26 @@ class Session:
     def reference_hits(self, origins, directions, times, from_ids, inward):
         """The reference's own walk of the session's binary tree, with its tie rule: (ids, t, entering)."""
         return self._hits(_lib.rt_session_reference_hits, origins, directions, times, from_ids, inward)
+
+    def occluded(self, origins, directions, times, from_ids, inward, t_max, skip):
+        """Shadow rays through both shadow walks: (blocked, walked), each 1 where something but `skip` lies in
+        (0.001, t_max)."""
+        o = np.ascontiguousarray(origins, dtype=np.float32)
+        d = np.ascontiguousarray(directions, dtype=np.float32)
+        tm = np.ascontiguousarray(times, dtype=np.float32)
+        f = np.ascontiguousarray(from_ids, dtype=np.int32)
+        inw = np.ascontiguousarray(inward, dtype=np.uint8)
+        tx = np.ascontiguousarray(t_max, dtype=np.float32)
+        sk = np.ascontiguousarray(skip, dtype=np.int32)
+        n = len(tm)
+        blocked = np.zeros(n, np.uint8)
+        walked = np.zeros(n, np.uint8)
+        _check(_lib.rt_session_occluded(self._h, o.ctypes.data_as(_P(C.c_float)), d.ctypes.data_as(_P(C.c_float)),
+                                        tm.ctypes.data_as(_P(C.c_float)), f.ctypes.data_as(_P(C.c_int32)),
+                                        inw.ctypes.data_as(_P(C.c_uint8)), tx.ctypes.data_as(_P(C.c_float)),
+                                        sk.ctypes.data_as(_P(C.c_int32)), n, blocked.ctypes.data_as(_P(C.c_uint8)),
+                                        walked.ctypes.data_as(_P(C.c_uint8))))
+        return blocked, walked
 
     def _hits(self, fn, origins, directions, times, from_ids, inward):
         o = np.ascontiguousarray(origins, dtype=np.float32)

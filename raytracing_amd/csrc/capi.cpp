@@ -301,6 +301,26 @@ int rt_session_closest_hit(rt_session* session, const float* origins, const floa
     });
 }
 
+int rt_session_occluded(rt_session* session, const float* origins, const float* directions, const float* times,
+                        const int32_t* from, const uint8_t* inward, const float* t_max, const int32_t* skip,
+                        int64_t count, uint8_t* blocked, uint8_t* walked)
+{
+    return guarded([&] {
+        need(session, "session");
+        if (count > 0)
+        {
+            need(origins, "origins");
+            need(directions, "directions");
+            need(times, "times");
+            need(from, "from");
+            need(inward, "inward");
+            need(t_max, "t_max");
+            need(skip, "skip");
+        }
+        session->impl->occluded(origins, directions, times, from, inward, t_max, skip, count, blocked, walked);
+    });
+}
+
 int rt_session_reference_hits(rt_session* session, const float* origins, const float* directions, const float* times,
                               const int32_t* from, const uint8_t* inward, int64_t count, int32_t* index, float* t,
                               uint8_t* entering)

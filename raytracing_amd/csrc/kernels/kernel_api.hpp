@@ -70,6 +70,10 @@ hipError_t launch_reference_walk(const dev::scene_view<float>& sc, const void* b
                                  const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index, float* t,
                                  uint8_t* entering, int32_t* overflow, hipStream_t stream);
 size_t binary_node_bytes();
+// Both shadow walks' answers for n rays (probes.hip): blocked from occluded, walked from closest_hit's any-hit mode
+hipError_t launch_occluded(const dev::scene_view<float>& sc, const float* origins, const float* directions,
+                           const float* times, const int32_t* from, const uint8_t* inward, const float* t_max,
+                           const int32_t* skip, int64_t n, uint8_t* blocked, uint8_t* walked, hipStream_t stream);
 // The per-sample generator's outputs for n (pixel, pass) items (probes.hip); any output may be null
 hipError_t launch_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passes, int64_t n, int draws,
                                  uint32_t* words, float* floats, double* doubles, float* directions,

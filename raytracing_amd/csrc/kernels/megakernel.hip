@@ -28,12 +28,6 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-__device__ inline lds_stack block_stack(int entries)
-{
-    extern __shared__ __align__(16) unsigned char lds[];
-    return stack_in(lds, entries, blockDim.x * blockDim.y, threadIdx.y * blockDim.x + threadIdx.x);
-}
-
 static_assert(tally_events == RT_TRAFFIC_EVENTS && tally_phases == RT_TRAFFIC_PHASES,
               "the traffic report lists every tally event and phase");
 

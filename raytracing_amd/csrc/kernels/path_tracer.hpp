@@ -470,6 +470,12 @@ __device__ inline lds_stack stack_in(unsigned char* lds, int entries, int thread
     return {reinterpret_cast<int32_t*>(lds) + tid,
             reinterpret_cast<uint16_t*>(lds + stack_ref_bytes(entries, threads)) + tid, threads};
 }
+// This thread's stack in the block's dynamic LDS (a launch gives the block stack_bytes(entries, threads))
+__device__ inline lds_stack block_stack(int entries)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    return stack_in(lds, entries, blockDim.x * blockDim.y, threadIdx.y * blockDim.x + threadIdx.x);
+}
 
 // Ties: a primitive hit at exactly the closest distance so far (every test accepts t = t_max in the closest-hit walk)
 // replaces the closest one when its index in the scene is lower. The reference keeps whichever its walk meets first

@@ -7,11 +7,14 @@
 //     reference's order with the reference's intervals, so that on tied rays too its ids are the reference's.
 //   - sample_numbers_kernel: the per-sample generator's raw words, its float and double draws, and the direction
 //     samplers built on them, for given (seed, pixel, pass).
+//   - occluded_kernel: a shadow ray's answer from both of the product's shadow walks, for given rays, lengths and
+//     skipped primitives.
 #include <algorithm>
 #include <cstdint>
 
 #include "kernel_api.hpp"
 #include "books.hpp"
+#include "quality.hpp"
 
 namespace rt::dev
 {
@@ -211,10 +214,64 @@ __global__ void __launch_bounds__(256) sample_numbers_kernel(uint64_t seed, cons
         out[8] = c.z;
     }
 }
+
+// Ray k's shadow question (src/gpu/shadow.cuh:26-120): is any primitive but skip[k] hit in (0.001, t_max[k]), with
+// the self-intersection rules of the primitive from[k] it leaves? Answered by both shadow walks of the product: the
+// wavefront's shading asks occluded (blocked), the quality megakernel walks its shadow rays as closest_hit in its
+// any-hit mode (walked). Walk: the scene's tree walk (walk_instances for any float tree, walk_packed for packed nodes).
+template <int Walk>
+__global__ void __launch_bounds__(256) occluded_kernel(const scene_view<float> sc, const float* __restrict__ origins,
+                                                       const float* __restrict__ directions,
+                                                       const float* __restrict__ times,
+                                                       const int32_t* __restrict__ from,
+                                                       const uint8_t* __restrict__ inward,
+                                                       const float* __restrict__ t_max,
+                                                       const int32_t* __restrict__ skip, int64_t n,
+                                                       uint8_t* __restrict__ blocked, uint8_t* __restrict__ walked)
+{
+    const lds_stack stack = block_stack(sc.stack_depth);
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+    {
+        return;
+    }
+    const ray<float> r{{origins[3 * k], origins[3 * k + 1], origins[3 * k + 2]},
+                       {directions[3 * k], directions[3 * k + 1], directions[3 * k + 2]},
+                       times[k]};
+    tally<false> tl;
+    blocked[k] = occluded<Walk>(sc, r, t_max[k], from[k], inward[k] != 0, skip[k], stack, tl) ? 1 : 0;
+    const hit<float> h = closest_hit<float, Walk, tally<false>, lds_stack, true>(sc, r, from[k], inward[k] != 0, stack,
+                                                                                 tl, t_max[k], skip[k], true);
+    walked[k] = h.index >= 0 ? 1 : 0;
+}
 }
 
 namespace rt
 {
+hipError_t launch_occluded(const dev::scene_view<float>& sc, const float* origins, const float* directions,
+                           const float* times, const int32_t* from, const uint8_t* inward, const float* t_max,
+                           const int32_t* skip, int64_t n, uint8_t* blocked, uint8_t* walked, hipStream_t stream)
+{
+    if (n <= 0)
+    {
+        return hipSuccess;
+    }
+    const int threads = 256;
+    const dim3 grid(static_cast<unsigned>((n + threads - 1) / threads));
+    const size_t lds = stack_lds_bytes<float>(sc.stack_depth, threads);
+    if (dev::tree_walk(sc) == dev::walk_packed)
+    {
+        hipLaunchKernelGGL((dev::occluded_kernel<dev::walk_packed>), grid, dim3(threads), lds, stream, sc, origins,
+                           directions, times, from, inward, t_max, skip, n, blocked, walked);
+    }
+    else
+    {
+        hipLaunchKernelGGL((dev::occluded_kernel<dev::walk_instances>), grid, dim3(threads), lds, stream, sc, origins,
+                           directions, times, from, inward, t_max, skip, n, blocked, walked);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_reference_walk(const dev::scene_view<float>& sc, const void* binary_nodes, int node_count,
                                  const float* origins, const float* directions, const float* times,
                                  const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index, float* t,

@@ -618,7 +618,8 @@ __device__ inline Real lamp_hit_density(const scene_view<Real>& sc, int id, vec3
 // --- shadow rays -------------------------------------------------------------------------------------------------------
 // Is any primitive but `skip` hit in (0.001, t_max)? The four-wide walk of closest_hit, ending at the first hit, with
 // the leaving primitive's self-intersection rule: a sphere it leaves inward can still be hit from inside (its far
-// root), any other primitive it leaves cannot. Intervals open at both ends (src/gpu/shadow.cuh:26-120).
+// root), any other primitive it leaves cannot. A sphere's interval is open at t_max, a quad's and a triangle's closed
+// (src/gpu/shadow.cuh:26-120).
 template <typename Tally, typename Real>
 __device__ inline bool leaf_blocks(const scene_view<Real>& sc, const ray<Real>& r, int32_t ref, Real inverse_a,
                                    Real t_max, int from, bool inward, int skip, int triangle_shift, Tally& tl)

@@ -608,6 +608,46 @@ public:
         }
     }
 
+    void occluded(const float* origins, const float* directions, const float* times, const int32_t* from,
+                  const uint8_t* inward, const float* t_max, const int32_t* skip, int64_t n, uint8_t* blocked,
+                  uint8_t* walked) override
+    {
+        if constexpr (sizeof(Real) != sizeof(float))
+        {
+            throw std::invalid_argument("the shadow-ray probe runs in float32 sessions only");
+        }
+        else
+        {
+            if (n <= 0)
+            {
+                return;
+            }
+            const std::size_t m = static_cast<std::size_t>(n);
+            device_array<float> o(3 * m), d(3 * m), tm(m), tx(m);
+            device_array<int32_t> f(m), sk(m);
+            device_array<uint8_t> in(m), b(m), w(m);
+            check(hipMemcpy(o.data(), origins, o.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(d.data(), directions, d.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(tm.data(), times, tm.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(f.data(), from, f.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(in.data(), inward, in.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(tx.data(), t_max, tx.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(sk.data(), skip, sk.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(launch_occluded(dscene_.view(), o.data(), d.data(), tm.data(), f.data(), in.data(), tx.data(),
+                                  sk.data(), n, b.data(), w.data(), stream()),
+                  "launching the shadow-ray probe");
+            check(hipStreamSynchronize(stream()), "tracing the probe rays");
+            if (blocked)
+            {
+                check(hipMemcpy(blocked, b.data(), b.bytes(), hipMemcpyDeviceToHost), "copying answers");
+            }
+            if (walked)
+            {
+                check(hipMemcpy(walked, w.data(), w.bytes(), hipMemcpyDeviceToHost), "copying answers");
+            }
+        }
+    }
+
     void reference_hits(const float* origins, const float* directions, const float* times, const int32_t* from,
                         const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) override
     {

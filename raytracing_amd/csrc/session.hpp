@@ -39,6 +39,11 @@ public:
     virtual rt_kernel_info kernel() const = 0;
     virtual void closest_hit(const float* origins, const float* directions, const float* times, const int32_t* from,
                              const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) = 0;
+    // A shadow ray's answer from both shadow walks (probes.hip): blocked by occluded, walked by closest_hit's any-hit
+    // mode; either output may be null
+    virtual void occluded(const float* origins, const float* directions, const float* times, const int32_t* from,
+                          const uint8_t* inward, const float* t_max, const int32_t* skip, int64_t n, uint8_t* blocked,
+                          uint8_t* walked) = 0;
     // The reference's own traversal of the session's binary tree, with its tie rule (probes.hip)
     virtual void reference_hits(const float* origins, const float* directions, const float* times, const int32_t* from,
                                 const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) = 0;

@@ -1,18 +1,23 @@
-"""A scene of given spheres and quads through the C ABI's rt_scene (for the tie tests' hand-built cases): every
-primitive a gray lambertian, a small camera looking down -z."""
+"""A scene of given spheres and quads through the C ABI's rt_scene (for the tie tests' hand-built cases and the shadow
+rays' brute-force check): every primitive a gray lambertian, a small camera looking down -z. Spheres are x, y, z, r
+rows, or x, y, z, r, dx, dy, dz rows for moving ones; combination: the scene's rt_light_combination (0, the books'
+material samples; 2, -p mis, renders it with the quality integrator)."""
 import ctypes as C
 
 import numpy as np
 
 
 class CustomScene:
-    def __init__(self, rt, spheres, quads):
+    def __init__(self, rt, spheres, quads, combination=0):
         self.rt = rt
-        sp = np.asarray(spheres, np.float64).reshape(-1, 4)
+        sp = np.asarray(spheres, np.float64)
+        sp = sp.reshape(-1, sp.shape[-1] if sp.size else 4)
+        if sp.shape[1] == 4:
+            sp = np.concatenate([sp, np.zeros((len(sp), 3))], axis=1)
         qd = np.asarray(quads, np.float64).reshape(-1, 9)
         self._spheres = (rt.SphereDesc * max(1, len(sp)))()
-        for k, (x, y, z, r) in enumerate(sp):
-            self._spheres[k] = rt.SphereDesc(rt.Vec3(x, y, z), r, rt.Vec3(0, 0, 0), 1.0, 0.0, 0, 0)
+        for k, (x, y, z, r, dx, dy, dz) in enumerate(sp):
+            self._spheres[k] = rt.SphereDesc(rt.Vec3(x, y, z), r, rt.Vec3(dx, dy, dz), 1.0, 0.0, 0, 0)
         self._quads = (rt.QuadDesc * max(1, len(qd)))()
         for k, v in enumerate(qd):
             self._quads[k] = rt.QuadDesc(rt.Vec3(*v[0:3]), rt.Vec3(*v[3:6]), rt.Vec3(*v[6:9]), 0, 0)
@@ -32,5 +37,6 @@ class CustomScene:
         d.materials, d.material_count = C.cast(self._materials, C.POINTER(rt.MaterialDesc)), 1
         d.textures, d.texture_count = C.cast(self._textures, C.POINTER(rt.TextureDesc)), 1
         d.camera = cam
+        d.combination = combination
         self.desc = d
         self.sphere_count, self.quad_count = len(sp), len(qd)

@@ -8,6 +8,12 @@
   several rounds (seed + r 2^32), pixels and passes; the draws' bit layout is the reference's (random.cuh:76-90).
 - The device's direction samplers (the shading's own): a uniform unit vector, the concentric map's uniform disk
   point and the cosine lobe about a normal, against their distributions' closed-form moments and CDFs.
+- Shadow rays (rt_session_occluded): both shadow walks, the wavefront's occluded and the quality megakernel's any-hit
+  closest_hit, against a loop over every primitive (oracle.twin_custom_occluded) on the reference's any-hit check's
+  scene and rays (tests/lamp_tests.cu:1618-1900): 200 still and moving spheres and parallelograms, rays leaving a
+  primitive or starting anywhere, finite and infinite t_max, a random or the closest primitive skipped; on the
+  surface-area tree, the reference's median tree and packed nodes. No ray may differ but where the oracle marks the
+  answer as within rounding of t_max (band) or of another decision (near), and those stay rare.
 """
 import zlib
 
@@ -17,6 +23,7 @@ import pytest
 import oracle
 from tests.custom_scene import CustomScene
 from tests.rays import scene_rays
+from tests.shadow_rays import t7_rays, t7_scene
 from tests.test_tie_rules import DOWN, FAR_QUADS, FAR_SPHERES, SQUARE
 
 pytestmark = pytest.mark.gpu
@@ -130,3 +137,84 @@ def test_direction_samplers_distributions(rt):
     close(cosine[:, 0], 0.0, "cosine E[x]")
     for t in (0.2, 0.5, 0.8, 0.95):
         close((cosine[:, 2] >= t).astype(np.float64), 1 - t * t, f"cosine P(z >= {t})")
+
+
+@pytest.mark.parametrize("tree,nodes,combination", [("default", "float", 0), ("median", "float", 0),
+                                                     ("default", "packed", 2)])
+def test_shadow_rays_against_brute_force(rt, tree, nodes, combination):
+    spheres, quads = t7_scene()
+    scene = CustomScene(rt, spheres, quads, combination=combination)
+    _, _, sid, qid, _ = rt.scene_bvh(scene, tree)
+    session_id = np.concatenate([sid, qid])            # input index -> the session tree's global id
+    input_index = np.full(session_id.max() + 1, -1, np.int32)
+    input_index[session_id] = np.arange(len(session_id), dtype=np.int32)
+    to_session = lambda idx: np.where(idx >= 0, session_id[np.maximum(idx, 0)], -1).astype(np.int32)
+    n = 400000
+    o, d, tm, fi, inw, tmax, ski, closest = t7_rays(spheres, quads, n, seed=7691 + len(tree) + 7 * combination)
+    with rt.Session(scene, integrator="megakernel", tree=tree, nodes=nodes) as s:
+        # the closest hit as the skipped primitive: the lamp a light sample aimed at
+        hid, _, _ = s.closest_hit(o, d, tm, to_session(fi), inw)
+        ski = np.where(closest, np.where(hid >= 0, input_index[np.maximum(hid, 0)], -1), ski).astype(np.int32)
+        blocked, walked = s.occluded(o, d, tm, to_session(fi), inw, tmax, to_session(ski))
+    want, band, near, _, _ = oracle.twin_custom_occluded(spheres, quads, o, d, tm, fi, inw, tmax, ski)
+    assert 0.2 < want.mean() < 0.8, "the ray set should hold blocked and open rays"
+    assert band.mean() < 0.01 and near.mean() < 0.01
+    for walk, got in (("occluded", blocked), ("any-hit closest_hit", walked)):
+        differ = got != want
+        hard = differ & (band == 0) & (near == 0)
+        print(f"{tree}/{nodes}, {walk}: {int(want.sum())} of {n} rays blocked; {int(band.sum())} in the t_max band "
+              f"({int((differ & (band == 1)).sum())} differ), {int((near & ~band.astype(bool)).sum())} near another "
+              f"decision ({int((differ & (near == 1) & (band == 0)).sum())} differ)")
+        first = np.nonzero(hard)[0][:5]
+        assert hard.sum() == 0, f"{walk}: {int(hard.sum())} rays differ from brute force, first {first}"
+        assert differ.sum() <= max(5, n // 10000), f"{walk}: too many rounding-zone differences"
+
+
+@pytest.mark.parametrize("tree,nodes", [("default", "float"), ("default", "packed"), ("two_level", "auto")])
+def test_triangle_shadow_rays_agree_with_closest_hits(rt, tree, nodes):
+    """Config 3's triangles (2.62 M, the blobs baked into world space or walked through their 64 instances): a shadow
+    ray is blocked exactly when the same tree's closest hit (itself checked against brute force and across trees in
+    test_gpu_quality.py) lies within t_max -- short of it by 0.1 % or beyond it by 0.1 %, from the camera and from the
+    hit points onward, leaving the triangle hit -- and both shadow walks agree when the hit triangle is skipped"""
+    scene = rt.build_scene("triangles", width=64, samples=1, sampling="mis")
+    cam = scene.desc.camera
+    rng = np.random.default_rng(2024)
+    n = 60000
+    center = np.array([cam.center.x, cam.center.y, cam.center.z])
+    p00 = np.array([cam.pixel00_loc.x, cam.pixel00_loc.y, cam.pixel00_loc.z])
+    du = np.array([cam.pixel_delta_u.x, cam.pixel_delta_u.y, cam.pixel_delta_u.z])
+    dv = np.array([cam.pixel_delta_v.x, cam.pixel_delta_v.y, cam.pixel_delta_v.z])
+    targets = p00 + rng.uniform(0, cam.image_width, n)[:, None] * du + rng.uniform(0, cam.image_height, n)[:, None] * dv
+    o = np.repeat(center[None], n, axis=0).astype(np.float32)
+    d = (targets - center).astype(np.float32)
+    none = np.full(n, -1, np.int32)
+    with rt.Session(scene, integrator="megakernel", tree=tree, nodes=nodes) as s:
+        ids, t, _ = s.closest_hit(o, d, np.zeros(n, np.float32), none, np.zeros(n, np.uint8))
+        hit = ids >= 0
+        assert hit.mean() > 0.5
+        # onward from the hit points, leaving the primitive hit
+        bo = (o[hit].astype(np.float64) + t[hit, None].astype(np.float64) * d[hit]).astype(np.float32)
+        bd = rng.normal(size=(len(bo), 3)).astype(np.float32)
+        O = np.concatenate([o, bo])
+        D = np.concatenate([d, bd])
+        F = np.concatenate([none, ids[hit]]).astype(np.int32)
+        m = len(O)
+        tm = np.zeros(m, np.float32)
+        inw = np.zeros(m, np.uint8)
+        ids2, t2, _ = s.closest_hit(O, D, tm, F, inw)
+        met = ids2 >= 0
+        assert 0.2 < met.mean() < 0.95
+        noskip = np.full(m, -1, np.int32)
+        for factor in (0.999, 1.001):
+            t_max = np.where(met, t2 * np.float32(factor), rng.uniform(1, 100, m)).astype(np.float32)
+            want = met & (factor > 1)
+            blocked, walked = s.occluded(O, D, tm, F, inw, t_max, noskip)
+            for walk, got in (("occluded", blocked), ("any-hit closest_hit", walked)):
+                bad = np.nonzero(got.astype(bool) != want)[0]
+                assert len(bad) == 0, f"{walk}, t_max = {factor} x the closest hit: {len(bad)} of {m} differ, first {bad[:5]}"
+        # the hit triangle skipped: whatever lies behind it within 2x its distance
+        t_max = np.where(met, 2 * t2, np.inf).astype(np.float32)
+        blocked, walked = s.occluded(O, D, tm, F, inw, t_max, np.where(met, ids2, -1).astype(np.int32))
+        assert np.array_equal(blocked, walked)
+        assert 0 < blocked.sum() < met.sum()
+        print(f"{tree}/{nodes}: {m} rays, {int(met.sum())} hit; behind the hit within 2x: {int(blocked.sum())}")
