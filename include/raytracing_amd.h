@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 2
+#define RT_ABI_VERSION 3
 
 enum rt_status
 {
@@ -415,8 +415,15 @@ int rt_session_create(const rt_scene* scene, const rt_session_options* options, 
 void rt_session_destroy(rt_session* session);
 /* gpu::session::set_camera: a new view, starting from no samples */
 int rt_session_set_camera(rt_session* session, const rt_camera_desc* camera);
-/* gpu::session::render_to: render until the image holds `passes` complete passes (rows must be 0) */
+/* gpu::session::render_to: render until the image holds `passes` complete passes and `rows` rows of the next */
 int rt_session_render_to(rt_session* session, int64_t passes, int32_t rows, rt_render_report* report);
+/* Progress (gpu::render_options::progress, src/gpu/renderer.h:423-427; the on_pass of session::render_to): called on
+ * the calling thread after each launch that completes passes (one launch may complete several), with the passes done
+ * so far and the render's target. It must not call into the library for the same session. */
+typedef void (*rt_progress_fn)(int64_t done, int64_t total, void* user);
+/* rt_session_render_to with `progress` (may be NULL) called as passes complete, `user` passed through */
+int rt_session_render_to_progress(rt_session* session, int64_t passes, int32_t rows, rt_render_report* report,
+                                  rt_progress_fn progress, void* user);
 /* rt_session_render_to without device allocation. No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, serving loops that must not call hipMalloc). The session's
  * whole passes up to `passes` render on the megakernel (at the session's register budget, or the compiler's when the
@@ -462,6 +469,8 @@ typedef struct rt_render_options
 {
     rt_session_options session;
     int64_t passes;
+    rt_progress_fn progress; /* may be NULL: called as passes complete (rt_progress_fn) */
+    void* progress_user;
 } rt_render_options;
 
 typedef struct rt_render_stats

@@ -142,8 +142,12 @@ class KernelInfo(C.Structure):
                 ("compute_units", C.c_int32)]
 
 
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_int64, C.c_int64, C.c_void_p)
+
+
 class RenderOptions(C.Structure):
-    _fields_ = [("session", SessionOptions), ("passes", C.c_int64)]
+    _fields_ = [("session", SessionOptions), ("passes", C.c_int64), ("progress", PROGRESS_FN),
+                ("progress_user", C.c_void_p)]
 
 
 class RenderStats(C.Structure):
@@ -197,6 +201,8 @@ _SIGNATURES = {
     "rt_session_destroy": (None, [C.c_void_p]),
     "rt_session_set_camera": (C.c_int, [C.c_void_p, _P(CameraDesc)]),
     "rt_session_render_to": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _P(RenderReport)]),
+    "rt_session_render_to_progress": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _P(RenderReport), PROGRESS_FN,
+                                                C.c_void_p]),
     "rt_session_render_to_static": (C.c_int, [C.c_void_p, C.c_int64, _P(RenderReport)]),
     "rt_session_render_for": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, _P(RenderReport)]),
     "rt_session_state": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int32)]),
@@ -331,11 +337,20 @@ class RenderResult:
     shadow_rays: int
 
 
-def render(scene: BuiltScene, passes: int = 0, **options) -> RenderResult:
-    """gpu::render: a session rendered to ``passes`` (0: the scene's own samples)."""
+def _progress_fn(progress):
+    """A C progress callback calling progress(done, total), or a null one"""
+    if progress is None:
+        return PROGRESS_FN()
+    return PROGRESS_FN(lambda done, total, _user: progress(done, total))
+
+
+def render(scene: BuiltScene, passes: int = 0, progress=None, **options) -> RenderResult:
+    """gpu::render: a session rendered to ``passes`` (0: the scene's own samples); progress(done, total), if given, is
+    called as passes complete."""
     desc = scene.desc
     out = np.zeros((desc.camera.image_height, desc.camera.image_width, 3), dtype=np.float64)
-    ro = RenderOptions(_session_options(**options), passes if passes else scene.passes)
+    callback = _progress_fn(progress)  # (kept alive through the call)
+    ro = RenderOptions(_session_options(**options), passes if passes else scene.passes, callback, None)
     stats = RenderStats()
     _check(_lib.rt_render(C.byref(desc), C.byref(ro), out.ctypes.data_as(_P(C.c_double)), out.size, C.byref(stats)))
     return RenderResult(out, stats.seconds, stats.kernel_seconds, stats.rays, stats.shadow_rays)
@@ -370,9 +385,15 @@ class Session:
         _check(_lib.rt_session_set_camera(self._h, C.byref(camera)))
         self.camera = CameraDesc.from_buffer_copy(camera)
 
-    def render_to(self, passes: int, rows: int = 0) -> RenderReport:
+    def render_to(self, passes: int, rows: int = 0, progress=None) -> RenderReport:
+        """Renders until the image holds ``passes`` passes and ``rows`` rows of the next; progress(done, total), if
+        given, is called as passes complete."""
         r = RenderReport()
-        _check(_lib.rt_session_render_to(self._h, passes, rows, C.byref(r)))
+        if progress is None:
+            _check(_lib.rt_session_render_to(self._h, passes, rows, C.byref(r)))
+        else:
+            callback = _progress_fn(progress)
+            _check(_lib.rt_session_render_to_progress(self._h, passes, rows, C.byref(r), callback, None))
         return r
 
     def render_to_static(self, passes: int) -> RenderReport:

@@ -1,6 +1,7 @@
 // The C ABI (include/raytracing_amd.h): exceptions become status codes and a thread-local message
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -154,6 +155,24 @@ int rt_session_render_to(rt_session* session, int64_t passes, int32_t rows, rt_r
     });
 }
 
+int rt_session_render_to_progress(rt_session* session, int64_t passes, int32_t rows, rt_render_report* report,
+                                  rt_progress_fn progress, void* user)
+{
+    return guarded([&] {
+        need(session, "session");
+        std::function<void(int64_t)> on_pass;
+        if (progress)
+        {
+            on_pass = [&](int64_t done) { progress(done, passes, user); };
+        }
+        const rt_render_report r = session->impl->render_to(passes, rows, on_pass);
+        if (report)
+        {
+            *report = r;
+        }
+    });
+}
+
 int rt_session_render_to_static(rt_session* session, int64_t passes, rt_render_report* report)
 {
     return guarded([&] {
@@ -270,7 +289,12 @@ int rt_render(const rt_scene* scene, const rt_render_options* options, double* p
         auto s = rt::make_session(*scene, o.session);
         const int64_t passes =
             o.passes > 0 ? o.passes : static_cast<int64_t>(scene->camera.sqrt_spp) * scene->camera.sqrt_spp;
-        s->render_to(passes, 0);
+        std::function<void(int64_t)> on_pass;
+        if (o.progress)
+        {
+            on_pass = [&](int64_t done) { o.progress(done, passes, o.progress_user); };
+        }
+        s->render_to(passes, 0, on_pass);
         s->pixels(pixels, count);
         if (stats)
         {

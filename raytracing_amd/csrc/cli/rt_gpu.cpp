@@ -287,8 +287,21 @@ int main(int argc, char** argv)
     const int w = s->camera.image_width, hgt = s->camera.image_height;
     const int64_t passes = rt_scene_passes(h);
     std::printf("Rendering %dx%d, rays per pixel: %lld\n", w, hgt, static_cast<long long>(passes));
+    // the passes left, counted down as launches complete them, each count as wide as the first so that a shorter one
+    // overwrites it whole; the line is erased when the render ends (src/main_gpu.cpp:516-550)
+    int digits = static_cast<int>(std::to_string(passes).size());
+    const rt_progress_fn show_progress = [](int64_t done, int64_t total, void* user) {
+        if (done < total)
+        {
+            std::printf("\rPasses remaining: %*lld", *static_cast<int*>(user), static_cast<long long>(total - done));
+            std::fflush(stdout);
+        }
+    };
+    show_progress(0, passes, &digits);
     rt_render_report report{};
-    if (rt_session_render_to(session, passes, 0, &report) != RT_OK)
+    const int rendered = rt_session_render_to_progress(session, passes, 0, &report, show_progress, &digits);
+    std::printf("\r%30s\r", "");
+    if (rendered != RT_OK)
     {
         return fail("rendering");
     }

@@ -249,7 +249,8 @@ public:
 
     accumulation state() const override { return state_; }
 
-    rt_render_report render_to(int64_t target_passes, int32_t target_rows) override
+    rt_render_report render_to(int64_t target_passes, int32_t target_rows,
+                               const std::function<void(int64_t done)>& on_pass = {}) override
     {
         settle();
         require_view();
@@ -296,6 +297,7 @@ public:
             report.bounces += last_bounces_;
             report.gpu_seconds += seconds;
             report.longest_launch_seconds = std::max(report.longest_launch_seconds, seconds);
+            const int64_t passes_before = state_.passes;
             if (whole)
             {
                 seconds_per_pass_ = seconds / work.pass_count;
@@ -320,6 +322,10 @@ public:
             else
             {
                 state_.rows = work.first_row + work.row_count;
+            }
+            if (on_pass && state_.passes > passes_before)
+            {
+                on_pass(state_.passes); // (launch() has waited for the launch)
             }
         }
         report.passes = state_.passes;

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -24,7 +25,9 @@ class session
 public:
     virtual ~session() = default;
     virtual void set_camera(const rt_camera_desc& camera) = 0;
-    virtual rt_render_report render_to(int64_t passes, int32_t rows) = 0;
+    // on_pass (may be empty): called on the host after each launch that completes passes, with the passes done
+    virtual rt_render_report render_to(int64_t passes, int32_t rows,
+                                       const std::function<void(int64_t done)>& on_pass = {}) = 0;
     virtual rt_render_report render_for(double seconds, int64_t pass_limit) = 0;
     // render_to's whole passes on the megakernel with its staging and queue head in the library's static scratch:
     // allocates no device memory (the caller under graph capture or in a loop that can't allocate)

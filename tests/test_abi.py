@@ -29,7 +29,7 @@ def test_every_declared_symbol_is_exported(rt):
     lib = rt.library()
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.rt_abi_version() == 2
+    assert lib.rt_abi_version() == 3
 
 
 def test_scene_names(rt):

@@ -41,6 +41,8 @@ def test_final_pfm_against_cpu(tmp_path):
         out = tmp_path / f"gpu{gpu_seed}.pfm"
         code, text = run(RT_GPU, "final", "-w", 120, "-s", 64, "--seed", gpu_seed, "-o", out)
         assert code == 0 and "Saved" in text
+        # the passes left, counted down (src/main_gpu.cpp:516-550; the text mode reads each "\r" as a line end)
+        assert "Passes remaining: 64" in text
         cpu, _ = oracle.render("final", width=120, spp=64, seed=cpu_seed)
         ref = tmp_path / f"cpu{cpu_seed}.pfm"
         image_compare.write_pfm(str(ref), cpu)

@@ -133,6 +133,30 @@ def test_same_seed_same_bytes(rt):
         assert not np.array_equal(sa, d.sums())
 
 
+def test_progress_callbacks(rt):
+    """render_options::progress and session::render_to's on_pass (src/gpu/renderer.h:423-427): called after launches
+    that complete passes, with the passes done (increasing, the last the target) and the total; the image is the one
+    rendered without them"""
+    scene = rt.build_scene("final", width=96, samples=16, earth_width=512)
+    calls = []
+    # (the megakernel alone: the autotuner's first launches may take all 16 passes at once)
+    r = rt.render(scene, seed=5, integrator="megakernel", launch_seconds=0.0001,
+                  progress=lambda done, total: calls.append((done, total)))
+    assert len(calls) >= 2 and calls[-1] == (16, 16)
+    assert all(total == 16 for _, total in calls)
+    assert all(a[0] < b[0] for a, b in zip(calls, calls[1:]))
+    plain = rt.render(scene, seed=5)
+    assert np.array_equal(r.pixels.view(np.uint64), plain.pixels.view(np.uint64))
+    with rt.Session(scene, seed=5, integrator="megakernel", launch_seconds=0.0001) as s:
+        seen = []
+        s.render_to(7, 40, progress=lambda done, total: seen.append((done, total)))  # pass 7's band of rows
+        assert seen and seen[-1] == (7, 7)
+        seen.clear()
+        s.render_to(16, progress=lambda done, total: seen.append((done, total)))
+        assert seen[0][0] == 8 and seen[-1] == (16, 16), seen  # the band's pass completes first
+        assert np.array_equal(s.pixels().view(np.uint64), plain.pixels.view(np.uint64))
+
+
 def test_rounds_of_strata(rt):
     """More passes than one grid: round r draws seed + r * 2^32 (src/gpu/renderer.h:564-584)"""
     scene = rt.build_scene("weekend", width=64, samples=50, strata=4)
