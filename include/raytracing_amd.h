@@ -521,6 +521,24 @@ int rt_session_reference_hits(rt_session* session, const float* origins, const f
 int rt_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passes, int64_t count, int32_t draws,
                       uint32_t* words, float* floats, double* doubles, float* directions);
 
+/* The quality path's scattering lobes (src/gpu/bsdf.cuh:58-201, src/gpu/ggx.cuh:59-324), which the reference's tests
+ * drive through test kernels (tests/bsdf_tests.cu, tests/ggx_tests.cu, tests/gloss_tests.cu) */
+enum rt_lobe
+{
+    RT_LOBE_LAMBERT = 0,   /* Lambert's cosine lobe of albedo `color` */
+    RT_LOBE_PHASE = 1,     /* the isotropic phase function */
+    RT_LOBE_CONDUCTOR = 2, /* GGX conductor of F0 `color` and alpha (= roughness^2; below 1e-3 a Fresnel mirror),
+                              with the multiple-scattering compensation of the energy table at `roughness` */
+    RT_LOBE_GLOSS = 3,     /* glossy coat (GGX dielectric of F0 0.04, alpha) over a matte base of color `color` */
+};
+/* On device 0, for `count` cases in the tangent frame (the normal is +z; v toward the viewer, l toward the light, 3
+ * floats each): value (3 per case) = f(v, l) cos_l and density = the density the lobe draws l with from v (per solid
+ * angle; 0 for a mirror); and a direction drawn from v by the per-sample generator of (seed, pixel = case, pass = 0):
+ * drawn (3 per case), its weight f cos / density (3) and drawn_density, all zero where the draw ends the path. */
+int rt_scatter_probe(int32_t lobe, float alpha, float roughness, const float color[3], int64_t count, const float* v,
+                     const float* l, uint64_t seed, float* value, float* density, float* drawn, float* weight,
+                     float* drawn_density);
+
 /* The flat BVH this library builds for a scene in float32 with the splits of `tree` (an rt_tree), on the host (no GPU
  * needed), in the reference's depth-first node layout (src/gpu/bvh.cuh:29-38,382-435): per node box_min[3], box_max[3]
  * as floats, and index, count as ints; and the global ids of the scene's spheres and quads. Call with NULL buffers to

@@ -223,6 +223,9 @@ _SIGNATURES = {
                                             _P(C.c_uint8)]),
     "rt_sample_numbers": (C.c_int, [C.c_uint64, _P(C.c_uint64), _P(C.c_uint64), C.c_int64, C.c_int32,
                                     _P(C.c_uint32), _P(C.c_float), _P(C.c_double), _P(C.c_float)]),
+    "rt_scatter_probe": (C.c_int, [C.c_int32, C.c_float, C.c_float, _P(C.c_float), C.c_int64, _P(C.c_float),
+                                   _P(C.c_float), C.c_uint64, _P(C.c_float), _P(C.c_float), _P(C.c_float),
+                                   _P(C.c_float), _P(C.c_float)]),
     "rt_session_traffic": (C.c_int, [C.c_void_p, C.c_int64, _P(Traffic)]),
     "rt_scene_wide_nodes": (C.c_int, [_P(Scene), C.c_int32, C.c_void_p, C.c_void_p, _P(C.c_int64)]),
     "rt_scene_bvh": (C.c_int, [_P(Scene), C.c_int32, _P(C.c_float), _P(C.c_int32), _P(C.c_int64), _P(C.c_int32),
@@ -510,6 +513,29 @@ def sample_numbers(seed: int, pixels, passes, draws: int):
                                   draws, words.ctypes.data_as(_P(C.c_uint32)), floats.ctypes.data_as(_P(C.c_float)),
                                   doubles.ctypes.data_as(_P(C.c_double)), dirs.ctypes.data_as(_P(C.c_float))))
     return words, floats, doubles, dirs
+
+
+LOBES = {"lambert": 0, "phase": 1, "conductor": 2, "gloss": 3}
+
+
+def scatter_probe(lobe, v, l, color=(1.0, 1.0, 1.0), alpha: float = 0.0, roughness: float = 0.0, seed: int = 0):
+    """The quality path's scattering lobe on given cases in the tangent frame (normal +z), on device 0: a dict of
+    value (f cos_l, n x 3) and density of l seen from v, and drawn, weight (f cos / density) and drawn_density of a
+    direction drawn from v by the generator of (seed, case, 0)"""
+    v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
+    l = np.ascontiguousarray(l, dtype=np.float32).reshape(-1, 3)
+    if len(v) != len(l):
+        raise ValueError("as many directions l as v")
+    n = len(v)
+    c = (C.c_float * 3)(*color)
+    out = {"value": np.zeros((n, 3), np.float32), "density": np.zeros(n, np.float32),
+           "drawn": np.zeros((n, 3), np.float32), "weight": np.zeros((n, 3), np.float32),
+           "drawn_density": np.zeros(n, np.float32)}
+    f = lambda a: a.ctypes.data_as(_P(C.c_float))
+    _check(_lib.rt_scatter_probe(LOBES[lobe] if isinstance(lobe, str) else int(lobe), alpha, roughness, c, n, f(v),
+                                 f(l), seed, f(out["value"]), f(out["density"]), f(out["drawn"]), f(out["weight"]),
+                                 f(out["drawn_density"])))
+    return out
 
 
 def scene_bvh(scene: BuiltScene, tree: str = "median"):

@@ -372,6 +372,26 @@ int rt_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* pas
     return guarded([&] { rt::sample_numbers(seed, pixels, passes, count, draws, words, floats, doubles, directions); });
 }
 
+int rt_scatter_probe(int32_t lobe, float alpha, float roughness, const float color[3], int64_t count, const float* v,
+                     const float* l, uint64_t seed, float* value, float* density, float* drawn, float* weight,
+                     float* drawn_density)
+{
+    return guarded([&] {
+        if (count > 0)
+        {
+            need(v, "v");
+            need(l, "l");
+            need(value, "value");
+            need(density, "density");
+            need(drawn, "drawn");
+            need(weight, "weight");
+            need(drawn_density, "drawn_density");
+        }
+        rt::scatter_probe(lobe, alpha, roughness, color, count, v, l, seed, value, density, drawn, weight,
+                          drawn_density);
+    });
+}
+
 int rt_session_traffic(rt_session* session, int64_t passes, rt_traffic* traffic)
 {
     return guarded([&] {

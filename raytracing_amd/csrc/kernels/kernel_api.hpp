@@ -70,6 +70,10 @@ hipError_t launch_reference_walk(const dev::scene_view<float>& sc, const void* b
                                  const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index, float* t,
                                  uint8_t* entering, int32_t* overflow, hipStream_t stream);
 size_t binary_node_bytes();
+// The scattering lobe `lobe` (rt_lobe) on n cases in the tangent frame (probes.hip, scatter_kernel)
+hipError_t launch_scatter_probe(int lobe, float alpha, float roughness, const float color[3], int64_t n,
+                                const float* v, const float* l, uint64_t seed, float* value, float* density,
+                                float* drawn, float* weight, float* drawn_density, hipStream_t stream);
 // Both shadow walks' answers for n rays (probes.hip): blocked from occluded, walked from closest_hit's any-hit mode
 hipError_t launch_occluded(const dev::scene_view<float>& sc, const float* origins, const float* directions,
                            const float* times, const int32_t* from, const uint8_t* inward, const float* t_max,

@@ -9,6 +9,8 @@
 //     samplers built on them, for given (seed, pixel, pass).
 //   - occluded_kernel: a shadow ray's answer from both of the product's shadow walks, for given rays, lengths and
 //     skipped primitives.
+//   - scatter_kernel: the quality path's scattering lobes on their own, in the tangent frame: the value and density of
+//     given directions, and directions drawn from them.
 #include <algorithm>
 #include <cstdint>
 
@@ -244,10 +246,73 @@ __global__ void __launch_bounds__(256) occluded_kernel(const scene_view<float> s
                                                                                  tl, t_max[k], skip[k], true);
     walked[k] = h.index >= 0 ? 1 : 0;
 }
+
+// Case k of one lobe (rt_lobe) in the tangent frame (the normal +z): f cos and the density of the direction l[k] seen
+// from v[k] (scatter_value), and a direction drawn from v[k] with the generator of (seed, k, 0) (scatter_sample): the
+// direction, its weight f cos / density and its density, or zeros where the draw ends the path
+__global__ void __launch_bounds__(256) scatter_kernel(int lobe, float alpha, float roughness, vec3<float> color,
+                                                      int64_t n, const float* __restrict__ v,
+                                                      const float* __restrict__ l, uint64_t seed,
+                                                      float* __restrict__ value, float* __restrict__ density,
+                                                      float* __restrict__ drawn, float* __restrict__ weight,
+                                                      float* __restrict__ drawn_density)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+    {
+        return;
+    }
+    scatterer<float> s{};
+    s.kind = lobe == RT_LOBE_LAMBERT     ? scatter_kind::lambert
+             : lobe == RT_LOBE_PHASE     ? scatter_kind::phase
+             : lobe == RT_LOBE_CONDUCTOR ? (alpha >= min_ggx_alpha ? scatter_kind::conductor : scatter_kind::mirror)
+                                         : scatter_kind::coat;
+    s.color = color;
+    s.local = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+    s.v = {v[3 * k], v[3 * k + 1], v[3 * k + 2]};
+    s.alpha = alpha;
+    s.roughness = roughness;
+    vertex<float> vx{};
+    vx.n = {0.0f, 0.0f, 1.0f};
+    vx.outward = vx.n;
+    vx.front = true;
+    const ggx_eval_result<float> e = scatter_value(s, vx, vec3<float>{l[3 * k], l[3 * k + 1], l[3 * k + 2]});
+    value[3 * k] = e.value.x;
+    value[3 * k + 1] = e.value.y;
+    value[3 * k + 2] = e.value.z;
+    density[k] = e.pdf;
+    pcg32 g = sample_generator(seed, static_cast<uint64_t>(k), 0);
+    const material_rec<float> m{};
+    scatter_draw<float> d{};
+    const bool ok = scatter_sample(s, m, -s.v, vx, g, d);
+    drawn[3 * k] = ok ? d.wi.x : 0.0f;
+    drawn[3 * k + 1] = ok ? d.wi.y : 0.0f;
+    drawn[3 * k + 2] = ok ? d.wi.z : 0.0f;
+    weight[3 * k] = ok ? d.weight.x : 0.0f;
+    weight[3 * k + 1] = ok ? d.weight.y : 0.0f;
+    weight[3 * k + 2] = ok ? d.weight.z : 0.0f;
+    drawn_density[k] = ok ? d.pdf : 0.0f;
+}
 }
 
 namespace rt
 {
+hipError_t launch_scatter_probe(int lobe, float alpha, float roughness, const float color[3], int64_t n,
+                                const float* v, const float* l, uint64_t seed, float* value, float* density,
+                                float* drawn, float* weight, float* drawn_density, hipStream_t stream)
+{
+    if (n <= 0)
+    {
+        return hipSuccess;
+    }
+    const int threads = 256;
+    const dim3 grid(static_cast<unsigned>((n + threads - 1) / threads));
+    hipLaunchKernelGGL(dev::scatter_kernel, grid, dim3(threads), 0, stream, lobe, alpha, roughness,
+                       dev::vec3<float>{color[0], color[1], color[2]}, n, v, l, seed, value, density, drawn, weight,
+                       drawn_density);
+    return hipGetLastError();
+}
+
 hipError_t launch_occluded(const dev::scene_view<float>& sc, const float* origins, const float* directions,
                            const float* times, const int32_t* from, const uint8_t* inward, const float* t_max,
                            const int32_t* skip, int64_t n, uint8_t* blocked, uint8_t* walked, hipStream_t stream)

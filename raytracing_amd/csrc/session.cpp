@@ -1104,6 +1104,37 @@ void sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passe
     }
 }
 
+void scatter_probe(int32_t lobe, float alpha, float roughness, const float color[3], int64_t n, const float* v,
+                   const float* l, uint64_t seed, float* value, float* density, float* drawn, float* weight,
+                   float* drawn_density)
+{
+    if (lobe < RT_LOBE_LAMBERT || lobe > RT_LOBE_GLOSS)
+    {
+        throw std::invalid_argument("unknown scattering lobe");
+    }
+    if (n < 0 || !color || !(alpha >= 0.0f) || !(roughness >= 0.0f) || roughness > 1.0f)
+    {
+        throw std::invalid_argument("0 or more cases, a color, alpha >= 0 and a roughness in [0, 1]");
+    }
+    if (n == 0)
+    {
+        return;
+    }
+    const std::size_t m = static_cast<std::size_t>(n);
+    device_array<float> dv(3 * m), dl(3 * m), val(3 * m), den(m), dr(3 * m), wt(3 * m), dd(m);
+    check(hipMemcpy(dv.data(), v, dv.bytes(), hipMemcpyHostToDevice), "copying directions");
+    check(hipMemcpy(dl.data(), l, dl.bytes(), hipMemcpyHostToDevice), "copying directions");
+    check(launch_scatter_probe(lobe, alpha, roughness, color, n, dv.data(), dl.data(), seed, val.data(), den.data(),
+                               dr.data(), wt.data(), dd.data(), nullptr),
+          "launching the scattering probe");
+    check(hipDeviceSynchronize(), "scattering");
+    check(hipMemcpy(value, val.data(), val.bytes(), hipMemcpyDeviceToHost), "copying values");
+    check(hipMemcpy(density, den.data(), den.bytes(), hipMemcpyDeviceToHost), "copying densities");
+    check(hipMemcpy(drawn, dr.data(), dr.bytes(), hipMemcpyDeviceToHost), "copying draws");
+    check(hipMemcpy(weight, wt.data(), wt.bytes(), hipMemcpyDeviceToHost), "copying weights");
+    check(hipMemcpy(drawn_density, dd.data(), dd.bytes(), hipMemcpyDeviceToHost), "copying densities");
+}
+
 namespace
 {
 void check_options(const rt_session_options& options)
