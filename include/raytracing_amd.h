@@ -502,6 +502,16 @@ int rt_session_occluded(rt_session* session, const float* origins, const float* 
                         const int32_t* from, const uint8_t* inward, const float* t_max, const int32_t* skip,
                         int64_t count, uint8_t* blocked, uint8_t* walked);
 
+/* Light samples (src/gpu/light_sampler.cuh:40-102, src/gpu/lamps.cuh:41-146, src/gpu/environment.cuh:59-135): from
+ * each of `count` points (3 floats each), one draw of the session's light set with the per-sample generator of
+ * (seed, pixel = the point's index, pass = 0): the direction wi (3 floats), the distance to the lamp's point
+ * (infinity for the map), its radiance (3), its density per solid angle with the light's choice included, and the lamp
+ * primitive aimed at (skip: a global id, -1 for the map, -2 where the draw failed and the other outputs are 0);
+ * hit_density: the density the light set gives the same direction when a scattered ray finds that light (what MIS
+ * weighs a BSDF sample by). Quality scenes (light samples, a map or GGX) in float32 sessions only. */
+int rt_session_light_samples(rt_session* session, const float* points, int64_t count, uint64_t seed, float* wi,
+                             float* distance, float* radiance, float* density, int32_t* skip, float* hit_density);
+
 /* The reference's own traversal, for reference-exact closest hits (src/gpu/bvh.cuh:576-706): the session's binary tree
  * (with RT_TREE_MEDIAN, the reference's tree) walked near first in the reference's order (the first child when both
  * are entered at the same distance; the farther child pushed; a popped node visited only when entered before the

@@ -218,6 +218,9 @@ _SIGNATURES = {
     "rt_session_occluded": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_int32),
                                       _P(C.c_uint8), _P(C.c_float), _P(C.c_int32), C.c_int64, _P(C.c_uint8),
                                       _P(C.c_uint8)]),
+    "rt_session_light_samples": (C.c_int, [C.c_void_p, _P(C.c_float), C.c_int64, C.c_uint64, _P(C.c_float),
+                                           _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_int32),
+                                           _P(C.c_float)]),
     "rt_session_reference_hits": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float),
                                             _P(C.c_int32), _P(C.c_uint8), C.c_int64, _P(C.c_int32), _P(C.c_float),
                                             _P(C.c_uint8)]),
@@ -460,6 +463,20 @@ class Session:
     def reference_hits(self, origins, directions, times, from_ids, inward):
         """The reference's own walk of the session's binary tree, with its tie rule: (ids, t, entering)."""
         return self._hits(_lib.rt_session_reference_hits, origins, directions, times, from_ids, inward)
+
+    def light_samples(self, points, seed: int = 0) -> dict:
+        """One light-set draw from each point: wi, distance, radiance, density, skip (-1 the map, -2 failed) and
+        hit_density (the density a scattered ray finding the same light is given)"""
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        n = len(p)
+        out = {"wi": np.zeros((n, 3), np.float32), "distance": np.zeros(n, np.float32),
+               "radiance": np.zeros((n, 3), np.float32), "density": np.zeros(n, np.float32),
+               "skip": np.zeros(n, np.int32), "hit_density": np.zeros(n, np.float32)}
+        f = lambda a: a.ctypes.data_as(_P(C.c_float))
+        _check(_lib.rt_session_light_samples(self._h, f(p), n, seed, f(out["wi"]), f(out["distance"]),
+                                             f(out["radiance"]), f(out["density"]),
+                                             out["skip"].ctypes.data_as(_P(C.c_int32)), f(out["hit_density"])))
+        return out
 
     def occluded(self, origins, directions, times, from_ids, inward, t_max, skip):
         """Shadow rays through both shadow walks: (blocked, walked), each 1 where something but `skip` lies in

@@ -345,6 +345,25 @@ int rt_session_occluded(rt_session* session, const float* origins, const float* 
     });
 }
 
+int rt_session_light_samples(rt_session* session, const float* points, int64_t count, uint64_t seed, float* wi,
+                             float* distance, float* radiance, float* density, int32_t* skip, float* hit_density)
+{
+    return guarded([&] {
+        need(session, "session");
+        if (count > 0)
+        {
+            need(points, "points");
+            need(wi, "wi");
+            need(distance, "distance");
+            need(radiance, "radiance");
+            need(density, "density");
+            need(skip, "skip");
+            need(hit_density, "hit_density");
+        }
+        session->impl->light_samples(points, count, seed, wi, distance, radiance, density, skip, hit_density);
+    });
+}
+
 int rt_session_reference_hits(rt_session* session, const float* origins, const float* directions, const float* times,
                               const int32_t* from, const uint8_t* inward, int64_t count, int32_t* index, float* t,
                               uint8_t* entering)

@@ -70,6 +70,10 @@ hipError_t launch_reference_walk(const dev::scene_view<float>& sc, const void* b
                                  const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index, float* t,
                                  uint8_t* entering, int32_t* overflow, hipStream_t stream);
 size_t binary_node_bytes();
+// The quality path's light samples from n points (probes.hip, light_kernel)
+hipError_t launch_light_probe(const dev::scene_view<float>& sc, const float* points, int64_t n, uint64_t seed,
+                              float* wi, float* distance, float* radiance, float* density, int32_t* skip,
+                              float* hit_density, hipStream_t stream);
 // The scattering lobe `lobe` (rt_lobe) on n cases in the tangent frame (probes.hip, scatter_kernel)
 hipError_t launch_scatter_probe(int lobe, float alpha, float roughness, const float color[3], int64_t n,
                                 const float* v, const float* l, uint64_t seed, float* value, float* density,

@@ -11,6 +11,8 @@
 //     skipped primitives.
 //   - scatter_kernel: the quality path's scattering lobes on their own, in the tangent frame: the value and density of
 //     given directions, and directions drawn from them.
+//   - light_kernel: the quality path's light samples from given points (the light set's choice, a lamp's point or the
+//     map's direction), with the density a scattered ray finding the same light would be given.
 #include <algorithm>
 #include <cstdint>
 
@@ -293,10 +295,68 @@ __global__ void __launch_bounds__(256) scatter_kernel(int lobe, float alpha, flo
     weight[3 * k + 2] = ok ? d.weight.z : 0.0f;
     drawn_density[k] = ok ? d.pdf : 0.0f;
 }
+
+// Point k's light sample with the generator of (seed, k, 0) (draw_light): its direction, distance, radiance and density
+// (the light's choice included), and the lamp primitive it aims at (-1: the map; -2: the draw failed, the outputs
+// zero); hit_density: the density the light set gives that direction when a scattered ray finds the light there
+// (lamp_hit_density for a lamp, the texel's density for the map, times the choice's chance)
+__global__ void __launch_bounds__(256) light_kernel(const scene_view<float> sc, const float* __restrict__ points,
+                                                    int64_t n, uint64_t seed, float* __restrict__ wi,
+                                                    float* __restrict__ distance, float* __restrict__ radiance,
+                                                    float* __restrict__ density, int32_t* __restrict__ skip,
+                                                    float* __restrict__ hit_density)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+    {
+        return;
+    }
+    const vec3<float> x{points[3 * k], points[3 * k + 1], points[3 * k + 2]};
+    pcg32 g = sample_generator(seed, static_cast<uint64_t>(k), 0);
+    light_draw<float> d{};
+    tally<false> tl;
+    const bool ok = draw_light(sc.quality, x, g, d, tl);
+    float hit = 0.0f;
+    if (ok && d.skip >= 0)
+    {
+        hit = lamp_hit_density(sc, d.skip, x, d.wi, d.distance);
+    }
+    else if (ok)
+    {
+        float texel_density;
+        texel_radiance(sc.quality, texel_index(sc.quality, d.wi), texel_density);
+        hit = sc.quality.light_pmf * texel_density;
+    }
+    wi[3 * k] = ok ? d.wi.x : 0.0f;
+    wi[3 * k + 1] = ok ? d.wi.y : 0.0f;
+    wi[3 * k + 2] = ok ? d.wi.z : 0.0f;
+    distance[k] = ok ? d.distance : 0.0f;
+    radiance[3 * k] = ok ? d.radiance.x : 0.0f;
+    radiance[3 * k + 1] = ok ? d.radiance.y : 0.0f;
+    radiance[3 * k + 2] = ok ? d.radiance.z : 0.0f;
+    density[k] = ok ? d.pdf : 0.0f;
+    skip[k] = ok ? d.skip : -2;
+    hit_density[k] = hit;
+}
 }
 
 namespace rt
 {
+hipError_t launch_light_probe(const dev::scene_view<float>& sc, const float* points, int64_t n, uint64_t seed,
+                              float* wi, float* distance, float* radiance, float* density, int32_t* skip,
+                              float* hit_density, hipStream_t stream)
+{
+    if (n <= 0)
+    {
+        return hipSuccess;
+    }
+    const int threads = 256;
+    const dim3 grid(static_cast<unsigned>((n + threads - 1) / threads));
+    hipLaunchKernelGGL(dev::light_kernel, grid, dim3(threads), 0, stream, sc, points, n, seed, wi, distance, radiance,
+                       density, skip, hit_density);
+    return hipGetLastError();
+}
+
 hipError_t launch_scatter_probe(int lobe, float alpha, float roughness, const float color[3], int64_t n,
                                 const float* v, const float* l, uint64_t seed, float* value, float* density,
                                 float* drawn, float* weight, float* drawn_density, hipStream_t stream)

@@ -654,6 +654,45 @@ public:
         }
     }
 
+    void light_samples(const float* points, int64_t n, uint64_t seed, float* wi, float* distance, float* radiance,
+                       float* density, int32_t* skip, float* hit_density) override
+    {
+        if constexpr (sizeof(Real) != sizeof(float))
+        {
+            throw std::invalid_argument("the light-sample probe runs in float32 sessions only");
+        }
+        else
+        {
+            if (!dscene_.quality())
+            {
+                throw std::invalid_argument("the light-sample probe needs a scene with light samples (-p nee or mis, "
+                                            "an environment map or GGX)");
+            }
+            if (!(dscene_.view().quality.light_pmf > 0.0f))
+            {
+                throw std::invalid_argument("the scene's light set is empty");
+            }
+            if (n <= 0)
+            {
+                return;
+            }
+            const std::size_t m = static_cast<std::size_t>(n);
+            device_array<float> p(3 * m), w(3 * m), dist(m), rad(3 * m), den(m), hit(m);
+            device_array<int32_t> sk(m);
+            check(hipMemcpy(p.data(), points, p.bytes(), hipMemcpyHostToDevice), "copying points");
+            check(launch_light_probe(dscene_.view(), p.data(), n, seed, w.data(), dist.data(), rad.data(), den.data(),
+                                     sk.data(), hit.data(), stream()),
+                  "launching the light-sample probe");
+            check(hipStreamSynchronize(stream()), "drawing light samples");
+            check(hipMemcpy(wi, w.data(), w.bytes(), hipMemcpyDeviceToHost), "copying samples");
+            check(hipMemcpy(distance, dist.data(), dist.bytes(), hipMemcpyDeviceToHost), "copying samples");
+            check(hipMemcpy(radiance, rad.data(), rad.bytes(), hipMemcpyDeviceToHost), "copying samples");
+            check(hipMemcpy(density, den.data(), den.bytes(), hipMemcpyDeviceToHost), "copying samples");
+            check(hipMemcpy(skip, sk.data(), sk.bytes(), hipMemcpyDeviceToHost), "copying samples");
+            check(hipMemcpy(hit_density, hit.data(), hit.bytes(), hipMemcpyDeviceToHost), "copying samples");
+        }
+    }
+
     void reference_hits(const float* origins, const float* directions, const float* times, const int32_t* from,
                         const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) override
     {

@@ -47,6 +47,9 @@ public:
     virtual void occluded(const float* origins, const float* directions, const float* times, const int32_t* from,
                           const uint8_t* inward, const float* t_max, const int32_t* skip, int64_t n, uint8_t* blocked,
                           uint8_t* walked) = 0;
+    // The light set's samples from given points (probes.hip, light_kernel); quality scenes in float32
+    virtual void light_samples(const float* points, int64_t n, uint64_t seed, float* wi, float* distance,
+                               float* radiance, float* density, int32_t* skip, float* hit_density) = 0;
     // The reference's own traversal of the session's binary tree, with its tie rule (probes.hip)
     virtual void reference_hits(const float* origins, const float* directions, const float* times, const int32_t* from,
                                 const uint8_t* inward, int64_t n, int32_t* index, float* t, uint8_t* entering) = 0;
