@@ -502,6 +502,16 @@ int rt_session_occluded(rt_session* session, const float* origins, const float* 
                         const int32_t* from, const uint8_t* inward, const float* t_max, const int32_t* skip,
                         int64_t count, uint8_t* blocked, uint8_t* walked);
 
+/* The media along given rays (src/gpu/camera.cuh:259-284, src/gpu/constant_medium.cuh:14-125, src/gpu/shadow.cuh:
+ * 122-165): for each ray (origins and directions, 3 floats each, times) whose surfaces end at t_surface, the media's
+ * event as the paths find it -- each medium's chord within (0.001, t_surface) and a free flight through it with the
+ * per-sample generator of (seed, pixel = the ray's index, pass = 0), the nearest collision kept -- as the medium's
+ * index among the scene's media (-1: none) and t (infinity: none); and the transmittance of (0.001, t_surface), the
+ * weight of a shadow ray through the same media. Float32 sessions only. */
+int rt_session_media_events(rt_session* session, const float* origins, const float* directions, const float* times,
+                            const float* t_surface, int64_t count, uint64_t seed, int32_t* medium, float* t,
+                            float* transmitted);
+
 /* Light samples (src/gpu/light_sampler.cuh:40-102, src/gpu/lamps.cuh:41-146, src/gpu/environment.cuh:59-135): from
  * each of `count` points (3 floats each), one draw of the session's light set with the per-sample generator of
  * (seed, pixel = the point's index, pass = 0): the direction wi (3 floats), the distance to the lamp's point

@@ -218,6 +218,8 @@ _SIGNATURES = {
     "rt_session_occluded": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_int32),
                                       _P(C.c_uint8), _P(C.c_float), _P(C.c_int32), C.c_int64, _P(C.c_uint8),
                                       _P(C.c_uint8)]),
+    "rt_session_media_events": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_float),
+                                          C.c_int64, C.c_uint64, _P(C.c_int32), _P(C.c_float), _P(C.c_float)]),
     "rt_session_light_samples": (C.c_int, [C.c_void_p, _P(C.c_float), C.c_int64, C.c_uint64, _P(C.c_float),
                                            _P(C.c_float), _P(C.c_float), _P(C.c_float), _P(C.c_int32),
                                            _P(C.c_float)]),
@@ -463,6 +465,21 @@ class Session:
     def reference_hits(self, origins, directions, times, from_ids, inward):
         """The reference's own walk of the session's binary tree, with its tie rule: (ids, t, entering)."""
         return self._hits(_lib.rt_session_reference_hits, origins, directions, times, from_ids, inward)
+
+    def media_events(self, origins, directions, times, t_surface, seed: int = 0):
+        """The media's event along each ray up to t_surface: (medium index or -1, t or inf, transmittance)"""
+        o = np.ascontiguousarray(origins, dtype=np.float32)
+        d = np.ascontiguousarray(directions, dtype=np.float32)
+        tm = np.ascontiguousarray(times, dtype=np.float32)
+        ts = np.ascontiguousarray(t_surface, dtype=np.float32)
+        n = len(tm)
+        medium = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        tr = np.zeros(n, np.float32)
+        f = lambda a: a.ctypes.data_as(_P(C.c_float))
+        _check(_lib.rt_session_media_events(self._h, f(o), f(d), f(tm), f(ts), n, seed,
+                                            medium.ctypes.data_as(_P(C.c_int32)), f(t), f(tr)))
+        return medium, t, tr
 
     def light_samples(self, points, seed: int = 0) -> dict:
         """One light-set draw from each point: wi, distance, radiance, density, skip (-1 the map, -2 failed) and

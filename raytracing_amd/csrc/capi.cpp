@@ -345,6 +345,26 @@ int rt_session_occluded(rt_session* session, const float* origins, const float* 
     });
 }
 
+int rt_session_media_events(rt_session* session, const float* origins, const float* directions, const float* times,
+                            const float* t_surface, int64_t count, uint64_t seed, int32_t* medium, float* t,
+                            float* transmitted)
+{
+    return guarded([&] {
+        need(session, "session");
+        if (count > 0)
+        {
+            need(origins, "origins");
+            need(directions, "directions");
+            need(times, "times");
+            need(t_surface, "t_surface");
+            need(medium, "medium");
+            need(t, "t");
+            need(transmitted, "transmitted");
+        }
+        session->impl->media_events(origins, directions, times, t_surface, count, seed, medium, t, transmitted);
+    });
+}
+
 int rt_session_light_samples(rt_session* session, const float* points, int64_t count, uint64_t seed, float* wi,
                              float* distance, float* radiance, float* density, int32_t* skip, float* hit_density)
 {

@@ -70,6 +70,10 @@ hipError_t launch_reference_walk(const dev::scene_view<float>& sc, const void* b
                                  const int32_t* from, const uint8_t* inward, int64_t n, int32_t* index, float* t,
                                  uint8_t* entering, int32_t* overflow, hipStream_t stream);
 size_t binary_node_bytes();
+// The media's events and transmittance along n rays (probes.hip, media_kernel)
+hipError_t launch_media_probe(const dev::scene_view<float>& sc, const float* origins, const float* directions,
+                              const float* times, const float* t_surface, int64_t n, uint64_t seed, int32_t* medium,
+                              float* t, float* transmitted, hipStream_t stream);
 // The quality path's light samples from n points (probes.hip, light_kernel)
 hipError_t launch_light_probe(const dev::scene_view<float>& sc, const float* points, int64_t n, uint64_t seed,
                               float* wi, float* distance, float* radiance, float* density, int32_t* skip,

@@ -11,6 +11,8 @@
 //     skipped primitives.
 //   - scatter_kernel: the quality path's scattering lobes on their own, in the tangent frame: the value and density of
 //     given directions, and directions drawn from them.
+//   - media_kernel: the media's events along given rays (a free flight through each medium's chord) and the
+//     transmittance a shadow ray is weighted by.
 //   - light_kernel: the quality path's light samples from given points (the light set's choice, a lamp's point or the
 //     map's direction), with the density a scattered ray finding the same light would be given.
 #include <algorithm>
@@ -338,10 +340,51 @@ __global__ void __launch_bounds__(256) light_kernel(const scene_view<float> sc, 
     skip[k] = ok ? d.skip : -2;
     hit_density[k] = hit;
 }
+
+// Ray k against the scene's media up to t_surface[k] (the surfaces' closest hit): the media's event (media_events,
+// with the generator of (seed, k, 0)): the medium's index among the scene's media and t, or -1 and infinity; and the
+// transmittance of (0.001, t_surface[k]) (transmittance)
+__global__ void __launch_bounds__(256) media_kernel(const scene_view<float> sc, const float* __restrict__ origins,
+                                                    const float* __restrict__ directions,
+                                                    const float* __restrict__ times,
+                                                    const float* __restrict__ t_surface, int64_t n, uint64_t seed,
+                                                    int32_t* __restrict__ medium, float* __restrict__ t,
+                                                    float* __restrict__ transmitted)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+    {
+        return;
+    }
+    const ray<float> r{{origins[3 * k], origins[3 * k + 1], origins[3 * k + 2]},
+                       {directions[3 * k], directions[3 * k + 1], directions[3 * k + 2]},
+                       times[k]};
+    pcg32 g = sample_generator(seed, static_cast<uint64_t>(k), 0);
+    tally<false> tl;
+    const hit<float> h = media_events(sc, r, hit<float>{-1, t_surface[k], false}, g, tl);
+    medium[k] = h.index >= 0 ? h.index - sc.media_base : -1;
+    t[k] = h.index >= 0 ? h.t : inf<float>;
+    transmitted[k] = transmittance(sc, r, t_surface[k], tl);
+}
 }
 
 namespace rt
 {
+hipError_t launch_media_probe(const dev::scene_view<float>& sc, const float* origins, const float* directions,
+                              const float* times, const float* t_surface, int64_t n, uint64_t seed, int32_t* medium,
+                              float* t, float* transmitted, hipStream_t stream)
+{
+    if (n <= 0)
+    {
+        return hipSuccess;
+    }
+    const int threads = 256;
+    const dim3 grid(static_cast<unsigned>((n + threads - 1) / threads));
+    hipLaunchKernelGGL(dev::media_kernel, grid, dim3(threads), 0, stream, sc, origins, directions, times, t_surface, n,
+                       seed, medium, t, transmitted);
+    return hipGetLastError();
+}
+
 hipError_t launch_light_probe(const dev::scene_view<float>& sc, const float* points, int64_t n, uint64_t seed,
                               float* wi, float* distance, float* radiance, float* density, int32_t* skip,
                               float* hit_density, hipStream_t stream)

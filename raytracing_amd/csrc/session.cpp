@@ -654,6 +654,36 @@ public:
         }
     }
 
+    void media_events(const float* origins, const float* directions, const float* times, const float* t_surface,
+                      int64_t n, uint64_t seed, int32_t* medium, float* t, float* transmitted) override
+    {
+        if constexpr (sizeof(Real) != sizeof(float))
+        {
+            throw std::invalid_argument("the media probe runs in float32 sessions only");
+        }
+        else
+        {
+            if (n <= 0)
+            {
+                return;
+            }
+            const std::size_t m = static_cast<std::size_t>(n);
+            device_array<float> o(3 * m), d(3 * m), tm(m), ts(m), tt(m), tr(m);
+            device_array<int32_t> md(m);
+            check(hipMemcpy(o.data(), origins, o.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(d.data(), directions, d.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(tm.data(), times, tm.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(hipMemcpy(ts.data(), t_surface, ts.bytes(), hipMemcpyHostToDevice), "copying rays");
+            check(launch_media_probe(dscene_.view(), o.data(), d.data(), tm.data(), ts.data(), n, seed, md.data(),
+                                     tt.data(), tr.data(), stream()),
+                  "launching the media probe");
+            check(hipStreamSynchronize(stream()), "tracing the media");
+            check(hipMemcpy(medium, md.data(), md.bytes(), hipMemcpyDeviceToHost), "copying events");
+            check(hipMemcpy(t, tt.data(), tt.bytes(), hipMemcpyDeviceToHost), "copying events");
+            check(hipMemcpy(transmitted, tr.data(), tr.bytes(), hipMemcpyDeviceToHost), "copying events");
+        }
+    }
+
     void light_samples(const float* points, int64_t n, uint64_t seed, float* wi, float* distance, float* radiance,
                        float* density, int32_t* skip, float* hit_density) override
     {

@@ -47,6 +47,10 @@ public:
     virtual void occluded(const float* origins, const float* directions, const float* times, const int32_t* from,
                           const uint8_t* inward, const float* t_max, const int32_t* skip, int64_t n, uint8_t* blocked,
                           uint8_t* walked) = 0;
+    // The media's events and transmittance along given rays (probes.hip, media_kernel); float32
+    virtual void media_events(const float* origins, const float* directions, const float* times,
+                              const float* t_surface, int64_t n, uint64_t seed, int32_t* medium, float* t,
+                              float* transmitted) = 0;
     // The light set's samples from given points (probes.hip, light_kernel); quality scenes in float32
     virtual void light_samples(const float* points, int64_t n, uint64_t seed, float* wi, float* distance,
                                float* radiance, float* density, int32_t* skip, float* hit_density) = 0;
