@@ -349,7 +349,21 @@ typedef struct rt_session_options
                                choice), 5, 6 or 7 (float only) */
     int32_t tree;           /* rt_tree: how the session's BVH is split */
     int32_t nodes;          /* rt_nodes: how its inner nodes lie in memory */
+    int32_t kernel;         /* rt_kernel: which shading renders the scene */
 } rt_session_options;
+
+/* Which shading renders a scene (gpu::session_options all_books_kernel and quality_kernel, src/gpu/renderer.h:418-454,
+ * src/gpu/renderer.cu:1180-1199): by default the quality integrator for scenes that need it (light samples, GGX, an
+ * environment map, triangles), the first book's for scenes of still spheres and the books' basic materials, and the
+ * three books' for the rest. The others are checks, and measures of what generality costs. */
+enum rt_kernel
+{
+    RT_KERNEL_AUTO = 0,
+    RT_KERNEL_ALL_BOOKS = 1, /* the three books' shading also for the first book's scenes (the same image, bit for
+                                bit); refused for scenes that need the quality integrator */
+    RT_KERNEL_QUALITY = 2,   /* the quality integrator also for the books' scenes under -p material (the same image
+                                within its noise); float32 (or RT_FLOAT64_ALL) only */
+};
 
 /* How the BVH's inner nodes lie in memory. Both hold the same tree and give the same closest hits (bit for bit): a
  * packed box holds its float box, so the walk enters every child it entered before. */

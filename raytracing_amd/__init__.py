@@ -128,7 +128,7 @@ class SessionOptions(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("precision", C.c_int32), ("integrator", C.c_int32),
                 ("launch_seconds", C.c_double), ("block_size", C.c_int32), ("wavefront_slots", C.c_int32),
                 ("wavefront_graph", C.c_int32), ("wavefront_drain", C.c_int32), ("megakernel_waves", C.c_int32),
-                ("tree", C.c_int32), ("nodes", C.c_int32)]
+                ("tree", C.c_int32), ("nodes", C.c_int32), ("kernel", C.c_int32)]
 
 
 class RenderReport(C.Structure):
@@ -265,6 +265,7 @@ INTEGRATORS = {"auto": 0, "megakernel": 1, "wavefront": 2, "plain": 3}
 PRECISIONS = {"float32": 0, "fp32": 0, "float64": 1, "fp64": 1, "float64_all": 2}
 TREES = {"default": 0, "median": 1, "sah": 2, "two_level": 3}
 NODES = {"auto": 0, "float": 1, "packed": 2}
+KERNELS = {"auto": 0, "all_books": 1, "quality": 2}
 TONE_MAPS = {"books": 0, "none": 1, "agx": 2, "neutral": 3}
 
 
@@ -328,11 +329,12 @@ def build_scene(name: str, **overrides) -> BuiltScene:
 
 def _session_options(seed=0, precision="float32", integrator="auto", launch_seconds=0.0, block_size=0,
                      wavefront_slots=0, wavefront_graph=0, wavefront_drain=0, megakernel_waves=0,
-                     tree="default", nodes="auto") -> SessionOptions:
+                     tree="default", nodes="auto", kernel="auto") -> SessionOptions:
     return SessionOptions(seed, PRECISIONS[precision], INTEGRATORS[integrator],
                           launch_seconds, block_size, wavefront_slots, wavefront_graph, wavefront_drain,
                           megakernel_waves, TREES[tree] if isinstance(tree, str) else int(tree),
-                          NODES[nodes] if isinstance(nodes, str) else int(nodes))
+                          NODES[nodes] if isinstance(nodes, str) else int(nodes),
+                          KERNELS[kernel] if isinstance(kernel, str) else int(kernel))
 
 
 @dataclass

@@ -4,7 +4,7 @@
 //
 //   rt_gpu [scene] [-w N] [--height N] [-s N] [--strata K] [-p uniform|material|lights|mixture|nee|mis]
 //          [--seed N] [--fp64] [--integrator auto|megakernel|wavefront|plain] [--launch-ms N] [-o FILE.pfm|FILE.png]...
-//          [--tonemap books|none|agx|neutral] [--exposure EV]
+//          [--tonemap books|none|agx|neutral] [--exposure EV] [--all-books-kernel | --quality-kernel]
 // Exit code: 0 rendered and saved, 2 an error (an option out of range or malformed, a scene the library refuses, a
 // file that couldn't be written).
 #include <cerrno>
@@ -153,6 +153,8 @@ int main(int argc, char** argv)
     std::vector<std::string> outputs;
     int32_t tone_map = RT_TONE_BOOKS;
     double exposure = 0;
+    bool all_books_kernel = false;
+    bool quality_kernel = false;
     for (int i = 1; i < argc; ++i)
     {
         const std::string a = argv[i];
@@ -168,6 +170,7 @@ int main(int argc, char** argv)
         {
             std::printf("Usage: rt_gpu [scene] [-w N] [--height N] [-s N] [--strata K] [-p STRATEGY] [--seed N]\n"
                         "              [--fp64] [--integrator auto|megakernel|wavefront|plain] [--launch-ms N] [-o FILE]\n"
+                        "              [--all-books-kernel | --quality-kernel]\n"
                         "              [--tonemap books|none|agx|neutral] [--exposure EV]\n"
                         "Scenes: %s\n"
                         "Or a glTF 2.0 model, FILE.glb or FILE.gltf, rendered in a studio\n",
@@ -188,6 +191,10 @@ int main(int argc, char** argv)
             so.seed = static_cast<uint64_t>(whole_number(a, value(), 0, INT64_MAX));
         else if (a == "--fp64")
             so.precision = RT_FLOAT64;
+        else if (a == "--all-books-kernel")
+            all_books_kernel = true;
+        else if (a == "--quality-kernel")
+            quality_kernel = true;
         else if (a == "--launch-ms")
             so.launch_seconds = static_cast<double>(whole_number(a, value(), 1, 500)) / 1000.0;
         else if (a == "-o" || a == "--output")
@@ -256,6 +263,14 @@ int main(int argc, char** argv)
             return 2;
         }
     }
+    // the kernel asked for (src/main_gpu.cpp:395-400)
+    if (all_books_kernel && quality_kernel)
+    {
+        std::printf("Error: --all-books-kernel picks the books' kernel for all three books; --quality-kernel needs the "
+                    "quality kernel\n");
+        return 2;
+    }
+    so.kernel = all_books_kernel ? RT_KERNEL_ALL_BOOKS : quality_kernel ? RT_KERNEL_QUALITY : RT_KERNEL_AUTO;
     if (outputs.empty())
     {
         outputs.push_back(scene + "-gpu.png");

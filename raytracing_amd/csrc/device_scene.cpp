@@ -376,7 +376,7 @@ dev::camera_rec<Real> to_camera(const rt_camera_desc& d)
 }
 
 template <typename Real>
-device_scene<Real>::device_scene(const rt_scene& s, int32_t tree, int32_t nodes)
+device_scene<Real>::device_scene(const rt_scene& s, int32_t tree, int32_t nodes, bool quality_kernel)
     : tree_(build_bvh<Real>(s, leaf_size_for(static_cast<std::size_t>(s.sphere_count + s.quad_count)), tree))
 {
     // materials: a texture of one color becomes the material's albedo (src/gpu/renderer.cu:548-601)
@@ -569,7 +569,7 @@ device_scene<Real>::device_scene(const rt_scene& s, int32_t tree, int32_t nodes)
              bytes_of(triangles_);
 
     dev::scene_view<Real>& v = view_;
-    quality_ = needs_quality(s);
+    quality_ = quality_kernel || needs_quality(s);
     if (quality_)
     {
         // the quality integrator's tables (src/gpu/quality.cu:112-170; float records, read as Real by the FP64 kernels)

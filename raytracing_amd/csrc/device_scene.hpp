@@ -90,7 +90,8 @@ class device_scene
 {
 public:
     // tree: an rt_tree; nodes: an rt_nodes (where the quality integrator's kernels walk packed nodes)
-    device_scene(const rt_scene& scene, int32_t tree, int32_t nodes = RT_NODES_AUTO);
+    // quality_kernel: build the quality integrator's tables even where the scene doesn't need them (RT_KERNEL_QUALITY)
+    device_scene(const rt_scene& scene, int32_t tree, int32_t nodes = RT_NODES_AUTO, bool quality_kernel = false);
     const dev::scene_view<Real>& view() const { return view_; }
     const bvh_arrays<Real>& tree() const { return tree_; }
     std::size_t bytes() const { return bytes_; }

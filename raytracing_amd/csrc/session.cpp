@@ -61,17 +61,17 @@ private:
 class shared_scene
 {
 public:
-    shared_scene(const rt_scene& scene, int32_t precision, int32_t tree, int32_t nodes)
+    shared_scene(const rt_scene& scene, int32_t precision, int32_t tree, int32_t nodes, bool quality_kernel)
         : tables(scene_tables::copy_of(scene))
     {
         view = tables.view();
         if (precision != RT_FLOAT32)
         {
-            f64 = std::make_unique<device_scene<double>>(view, tree, nodes);
+            f64 = std::make_unique<device_scene<double>>(view, tree, nodes, quality_kernel);
         }
         else
         {
-            f32 = std::make_unique<device_scene<float>>(view, tree, nodes);
+            f32 = std::make_unique<device_scene<float>>(view, tree, nodes, quality_kernel);
         }
     }
     template <typename Real>
@@ -92,15 +92,30 @@ public:
     std::unique_ptr<device_scene<double>> f64;
 };
 
-std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision, int32_t tree, int32_t nodes)
+std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision, int32_t tree, int32_t nodes,
+                                                 int32_t kernel)
 {
     validate_scene(scene);
     if (precision != RT_FLOAT32 && precision != RT_FLOAT64 && precision != RT_FLOAT64_ALL)
     {
         throw std::invalid_argument("unknown precision");
     }
-    const bool quality = scene.mesh_count != 0 || scene.microfacet_count != 0 || scene.environment != nullptr ||
-                         scene.combination != RT_COMBINE_BSDF_ONLY;
+    if (kernel < RT_KERNEL_AUTO || kernel > RT_KERNEL_QUALITY)
+    {
+        throw std::invalid_argument("unknown kernel");
+    }
+    const bool needed = scene.mesh_count != 0 || scene.microfacet_count != 0 || scene.environment != nullptr ||
+                        scene.combination != RT_COMBINE_BSDF_ONLY;
+    const bool quality = needed || kernel == RT_KERNEL_QUALITY;
+    // the reference's refusals of the kernels asked for (src/gpu/renderer.cu:2070-2087)
+    if (kernel == RT_KERNEL_ALL_BOOKS && needed)
+    {
+        throw std::invalid_argument("the quality kernel and the kernel for all three books can't both render");
+    }
+    if (kernel == RT_KERNEL_QUALITY && precision == RT_FLOAT64)
+    {
+        throw std::invalid_argument("the quality kernel computes in float only");
+    }
     // (the reference's quality kernel is float32 only, src/gpu/quality.cu:53-109, tests/routing_tests.cu:53-67;
     // RT_FLOAT64_ALL renders it in float64 for an oracle)
     if (precision == RT_FLOAT64 && quality)
@@ -114,7 +129,7 @@ std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t 
         throw std::invalid_argument("triangles, GGX, environment maps and light samples take -p material, nee or mis, "
                                     "not the lights list's uniform, lights or mixture");
     }
-    return std::make_shared<const shared_scene>(scene, precision, tree, nodes);
+    return std::make_shared<const shared_scene>(scene, precision, tree, nodes, quality);
 }
 
 std::size_t device_bytes(const shared_scene& scene) { return scene.f32 ? scene.f32->bytes() : scene.f64->bytes(); }
@@ -171,9 +186,17 @@ public:
         : shared_(std::move(scene)), scene_(shared_->tables), options_(options), dscene_(shared_->device<Real>())
     {
         view_scene_ = shared_->view;
-        mode_ = dscene_.quality()                 ? dev::mode_quality
-                : first_book_scene(view_scene_) ? dev::mode_first_book
-                                                : dev::mode_all_books;
+        if (options.kernel == RT_KERNEL_QUALITY && !dscene_.quality())
+        {
+            throw std::invalid_argument("the resident scene has no quality tables for the quality kernel");
+        }
+        if (options.kernel == RT_KERNEL_ALL_BOOKS && dscene_.quality())
+        {
+            throw std::invalid_argument("the quality kernel and the kernel for all three books can't both render");
+        }
+        mode_ = dscene_.quality()                                                         ? dev::mode_quality
+                : options.kernel != RT_KERNEL_ALL_BOOKS && first_book_scene(view_scene_) ? dev::mode_first_book
+                                                                                          : dev::mode_all_books;
         launch_seconds_ = options.launch_seconds > 0 ? options.launch_seconds : 0.05;
         if (!megakernel_waves_supported(options.megakernel_waves))
         {
@@ -1232,6 +1255,10 @@ void check_options(const rt_session_options& options)
     {
         throw std::invalid_argument("unknown node layout");
     }
+    if (options.kernel < RT_KERNEL_AUTO || options.kernel > RT_KERNEL_QUALITY)
+    {
+        throw std::invalid_argument("unknown kernel");
+    }
 }
 }
 
@@ -1243,7 +1270,8 @@ std::unique_ptr<session> make_session(const rt_scene& scene, const rt_session_op
     // hide less of the packed planes' unpacking; the queue megakernel, 1163 against 1038, and the wavefront take them)
     const int32_t nodes =
         options.nodes == RT_NODES_AUTO && options.integrator == RT_INTEGRATOR_PLAIN ? RT_NODES_FLOAT : options.nodes;
-    return make_session(upload_scene(scene, options.precision, options.tree, nodes), scene.camera, options);
+    return make_session(upload_scene(scene, options.precision, options.tree, nodes, options.kernel), scene.camera,
+                        options);
 }
 
 std::unique_ptr<session> make_session(std::shared_ptr<const shared_scene> scene, const rt_camera_desc& camera,

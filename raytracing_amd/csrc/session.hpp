@@ -79,7 +79,8 @@ public:
 // tree is an rt_tree, nodes an rt_nodes
 class shared_scene;
 std::shared_ptr<const shared_scene> upload_scene(const rt_scene& scene, int32_t precision,
-                                                 int32_t tree = RT_TREE_DEFAULT, int32_t nodes = RT_NODES_AUTO);
+                                                 int32_t tree = RT_TREE_DEFAULT, int32_t nodes = RT_NODES_AUTO,
+                                                 int32_t kernel = RT_KERNEL_AUTO);
 std::size_t device_bytes(const shared_scene& scene);
 const rt_scene& scene_of(const shared_scene& scene);
 

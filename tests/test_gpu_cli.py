@@ -105,12 +105,26 @@ def test_png_is_the_display_bytes(rt, tmp_path):
     (["weekend", "-s", "0"], "expected an integer"),
     (["weekend", "--exposure", "nan"], "expected a number"),
     (["weekend", "--wavefront"], "unknown option"),
+    (["weekend", "--all-books-kernel", "--quality-kernel"], "--quality-kernel needs the quality kernel"),
+    (["cornell", "-p", "mis", "--all-books-kernel"], "can't both render"),
+    (["weekend", "--quality-kernel", "--fp64"], "float only"),
 ])
 def test_refusals(tmp_path, args, message):
     out = tmp_path / "x.pfm"
     code, text = run(RT_GPU, *args, "-o", out)
     assert code == 2 and message in text, text
     assert not out.exists()
+
+
+def test_all_books_kernel_same_image(tmp_path):
+    """--all-books-kernel renders a first book's scene to the same image, bit for bit (src/main_gpu.cpp:78-80)"""
+    a, b = tmp_path / "a.pfm", tmp_path / "b.pfm"
+    code, text = run(RT_GPU, "weekend", "-w", 64, "-s", 16, "-p", "material", "-o", a)
+    assert code == 0 and "Kernel: first book" in text
+    code, text = run(RT_GPU, "weekend", "-w", 64, "-s", 16, "-p", "material", "--all-books-kernel", "-o", b)
+    assert code == 0 and "Kernel: all books" in text
+    code, text = run(STATS, "same", a, b)
+    assert code == 0, text
 
 
 def test_plain_integrator_same_image(tmp_path):

@@ -361,3 +361,48 @@ def test_plain_equals_megakernel(rt, name, sampling, precision):
     assert ca[0] == cb[0] and ca[1] == cb[1], f"rays {ca[:2]} vs {cb[:2]}"
     mism = np.count_nonzero(sa.view(np.uint64) != sb.view(np.uint64))
     assert mism == 0, f"{mism} of {sa.size} sums differ; max |diff| {np.abs(sa - sb).max()}"
+
+
+def test_all_books_kernel_on_first_book_scenes(rt):
+    """RT_KERNEL_ALL_BOOKS (the reference's all_books_kernel, src/gpu/renderer.h:428-431): the three books' shading
+    renders the first book's scenes (still spheres of the basic materials under -p material) to the same image, bit for
+    bit, with the cost of its generality"""
+    for name, width, spp in (("weekend", 96, 16), ("furnace", 64, 16)):
+        scene = rt.build_scene(name, width=width, samples=spp, sampling="material")
+        sums = {}
+        for kernel, expect in (("auto", "first book"), ("all_books", "all books")):
+            with rt.Session(scene, seed=7, integrator="megakernel", kernel=kernel) as s:
+                s.render_to(spp)
+                sums[kernel] = s.sums()
+                assert s.kernel().name.decode().startswith(expect), s.kernel().name
+        differ = np.count_nonzero(sums["auto"].view(np.uint64) != sums["all_books"].view(np.uint64))
+        assert differ == 0, f"{name}: {differ} of {sums['auto'].size} sums differ"
+
+
+def test_quality_kernel_on_books_scenes(rt):
+    """RT_KERNEL_QUALITY (the reference's quality_kernel): the quality integrator renders the books' scenes under
+    -p material to the books' image within its noise (ImageCompare, with the reference's re-run rule)"""
+    for name, width, spp in (("cornell", 96, 64), ("final", 96, 64), ("smoke", 96, 64), ("weekend", 96, 32)):
+        scene = rt.build_scene(name, width=width, samples=spp, earth_width=512, sampling="material")
+        verdicts = []
+        for seeds in ((1, 2), (3, 4)):
+            imgs = []
+            for kernel, seed in zip(("auto", "quality"), seeds):
+                with rt.Session(scene, seed=seed, integrator="megakernel", kernel=kernel) as s:
+                    s.render_to(spp)
+                    imgs.append(s.pixels())
+                    assert ("quality" in s.kernel().name.decode()) == (kernel == "quality")
+            v = image_compare.compare(imgs[0], imgs[1])
+            verdicts.append(v)
+            if v.consistent:
+                break
+        assert verdicts[-1].consistent, f"{name}: " + " | ".join(v.summary() for v in verdicts)
+
+
+def test_kernel_refusals(rt):
+    with pytest.raises(ValueError, match="can't both render"):
+        rt.Session(rt.build_scene("cornell", width=32, samples=4, sampling="mis"), kernel="all_books")
+    with pytest.raises(ValueError, match="float only"):
+        rt.Session(rt.build_scene("weekend", width=32, samples=4), kernel="quality", precision="float64")
+    with pytest.raises(ValueError):
+        rt.Session(rt.build_scene("cornell", width=32, samples=4, sampling="mixture"), kernel="quality")
