@@ -429,6 +429,11 @@ int rt_session_create(const rt_scene* scene, const rt_session_options* options, 
 void rt_session_destroy(rt_session* session);
 /* gpu::session::set_camera: a new view, starting from no samples */
 int rt_session_set_camera(rt_session* session, const rt_camera_desc* camera);
+/* gpu::session::camera and has_view (src/gpu/renderer.h:595-604): the last view set successfully; whether the
+ * session has a view (a set_camera whose image the GPU couldn't hold leaves none, and the renders and their reports
+ * refuse with RT_ERROR_LOGIC until another succeeds) */
+int rt_session_camera(const rt_session* session, rt_camera_desc* camera);
+int rt_session_has_view(const rt_session* session, int32_t* has_view);
 /* gpu::session::render_to: render until the image holds `passes` complete passes and `rows` rows of the next */
 int rt_session_render_to(rt_session* session, int64_t passes, int32_t rows, rt_render_report* report);
 /* Progress (gpu::render_options::progress, src/gpu/renderer.h:423-427; the on_pass of session::render_to): called on

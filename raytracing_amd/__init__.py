@@ -201,6 +201,8 @@ _SIGNATURES = {
     "rt_session_destroy": (None, [C.c_void_p]),
     "rt_session_set_camera": (C.c_int, [C.c_void_p, _P(CameraDesc)]),
     "rt_session_render_to": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _P(RenderReport)]),
+    "rt_session_camera": (C.c_int, [C.c_void_p, _P(CameraDesc)]),
+    "rt_session_has_view": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
     "rt_session_render_to_progress": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _P(RenderReport), PROGRESS_FN,
                                                 C.c_void_p]),
     "rt_session_render_to_static": (C.c_int, [C.c_void_p, C.c_int64, _P(RenderReport)]),
@@ -394,6 +396,17 @@ class Session:
     def set_camera(self, camera: CameraDesc) -> None:
         _check(_lib.rt_session_set_camera(self._h, C.byref(camera)))
         self.camera = CameraDesc.from_buffer_copy(camera)
+
+    def view(self) -> CameraDesc:
+        """The last view set successfully (session::camera)"""
+        c = CameraDesc()
+        _check(_lib.rt_session_camera(self._h, C.byref(c)))
+        return c
+
+    def has_view(self) -> bool:
+        v = C.c_int32()
+        _check(_lib.rt_session_has_view(self._h, C.byref(v)))
+        return bool(v.value)
 
     def render_to(self, passes: int, rows: int = 0, progress=None) -> RenderReport:
         """Renders until the image holds ``passes`` passes and ``rows`` rows of the next; progress(done, total), if

@@ -143,6 +143,24 @@ int rt_session_set_camera(rt_session* session, const rt_camera_desc* camera)
     });
 }
 
+int rt_session_camera(const rt_session* session, rt_camera_desc* camera)
+{
+    return guarded([&] {
+        need(session, "session");
+        need(camera, "camera");
+        *camera = session->impl->camera();
+    });
+}
+
+int rt_session_has_view(const rt_session* session, int32_t* has_view)
+{
+    return guarded([&] {
+        need(session, "session");
+        need(has_view, "has_view");
+        *has_view = session->impl->has_view() ? 1 : 0;
+    });
+}
+
 int rt_session_render_to(rt_session* session, int64_t passes, int32_t rows, rt_render_report* report)
 {
     return guarded([&] {

@@ -271,6 +271,8 @@ public:
     }
 
     accumulation state() const override { return state_; }
+    rt_camera_desc camera() const override { return viewed_; }
+    bool has_view() const override { return has_view_; }
 
     rt_render_report render_to(int64_t target_passes, int32_t target_rows,
                                const std::function<void(int64_t done)>& on_pass = {}) override
@@ -1068,6 +1070,7 @@ private:
     device_array<uint8_t> rgba_;
     device_array<ref_node<float>> binary_nodes_; // the reference-walk probe's copy of the binary tree
     bool has_view_ = false;
+    rt_camera_desc viewed_{}; // the last view set successfully
     accumulation state_{};
     double kernel_seconds_ = 0;
     double seconds_per_pass_ = 0;
@@ -1108,6 +1111,7 @@ private:
         kernel_seconds_ = 0;
         seconds_per_pass_ = 0;
         has_view_ = true;
+        viewed_ = camera;
     }
 
     // One launch of the view's work, timed on the GPU

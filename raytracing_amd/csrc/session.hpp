@@ -33,6 +33,10 @@ public:
     // allocates no device memory (the caller under graph capture or in a loop that can't allocate)
     virtual rt_render_report render_to_static(int64_t passes) = 0;
     virtual accumulation state() const = 0;
+    // The last view set successfully, and whether the session has one (a set_camera whose image the GPU couldn't
+    // hold leaves none until another succeeds)
+    virtual rt_camera_desc camera() const = 0;
+    virtual bool has_view() const = 0;
     virtual void pixels(double* out, std::size_t count) = 0;
     virtual void sums(double* out, std::size_t count) = 0;
     virtual void display(uint8_t* out, std::size_t count) = 0;

@@ -187,6 +187,19 @@ def test_session_errors(rt):
     cam.sampling = rt.SAMPLING["material"]
     with pytest.raises(ValueError):
         s.set_camera(cam)
+    # session::camera and has_view: a refused camera keeps the old view; a new one starts from no samples
+    assert s.has_view() and bytes(s.view()) == bytes(scene.desc.camera)
+    wider = rt.CameraDesc.from_buffer_copy(scene.desc.camera)
+    wider.image_width, wider.image_height = 48, 40
+    s.set_camera(wider)
+    assert s.has_view() and s.view().image_width == 48 and s.state() == (0, 0)
+    bad = rt.CameraDesc.from_buffer_copy(wider)
+    bad.image_width = 0
+    with pytest.raises(RuntimeError, match="at least one pixel"):  # (the reference's std::runtime_error)
+        s.set_camera(bad)
+    assert s.has_view() and bytes(s.view()) == bytes(wider)
+    s.render_to(1)
+    assert s.pixels().shape == (40, 48, 3)
     s.close()
 
 
