@@ -350,7 +350,16 @@ typedef struct rt_session_options
     int32_t tree;           /* rt_tree: how the session's BVH is split */
     int32_t nodes;          /* rt_nodes: how its inner nodes lie in memory */
     int32_t kernel;         /* rt_kernel: which shading renders the scene */
+    int32_t wait;           /* rt_wait: how the host waits for the session's launches */
 } rt_session_options;
+
+/* How the host waits for the GPU (gpu::wait_mode, src/gpu/renderer.h:505-516,536-540) */
+enum rt_wait
+{
+    RT_WAIT_SPIN = 0,  /* asks the GPU again and again: reacts at once, keeps a core busy */
+    RT_WAIT_SLEEP = 1, /* sleeps until the GPU signals; a session asks for it only after rt_query_device_wait set the
+                          device's waits to sleep (RT_ERROR_LOGIC otherwise) */
+};
 
 /* Which shading renders a scene (gpu::session_options all_books_kernel and quality_kernel, src/gpu/renderer.h:418-454,
  * src/gpu/renderer.cu:1180-1199): by default the quality integrator for scenes that need it (light samples, GGX, an
@@ -707,6 +716,9 @@ typedef struct rt_device_info
 
 /* gpu::query_device (src/gpu/renderer.h:516): describes device 0 and starts the runtime on it */
 int rt_query_device(rt_device_info* info);
+/* gpu::query_device(wait): the same, and sets how the device's own waits (synchronous copies, whole-device waits)
+ * wait for the rest of the process (an rt_wait); call it before anything else starts the runtime */
+int rt_query_device_wait(rt_device_info* info, int32_t wait);
 
 /* A device-to-device copy of `bytes` bytes with a plain dwordx4 copy kernel, `repeats` times, and its average kernel
  * seconds: the calibration the HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled by. No counterpart in the reference. */

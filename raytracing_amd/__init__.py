@@ -128,7 +128,7 @@ class SessionOptions(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("precision", C.c_int32), ("integrator", C.c_int32),
                 ("launch_seconds", C.c_double), ("block_size", C.c_int32), ("wavefront_slots", C.c_int32),
                 ("wavefront_graph", C.c_int32), ("wavefront_drain", C.c_int32), ("megakernel_waves", C.c_int32),
-                ("tree", C.c_int32), ("nodes", C.c_int32), ("kernel", C.c_int32)]
+                ("tree", C.c_int32), ("nodes", C.c_int32), ("kernel", C.c_int32), ("wait", C.c_int32)]
 
 
 class RenderReport(C.Structure):
@@ -253,6 +253,7 @@ _SIGNATURES = {
     "rt_server_pixels": (C.c_int, [C.c_void_p, C.c_int64, _P(C.c_double), C.c_size_t]),
     "rt_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
     "rt_query_device": (C.c_int, [_P(DeviceInfo)]),
+    "rt_query_device_wait": (C.c_int, [_P(DeviceInfo), C.c_int32]),
     "rt_copy_probe": (C.c_int, [C.c_uint64, C.c_int32, _P(C.c_double)]),
 }
 for _name, (_res, _args) in _SIGNATURES.items():
@@ -268,6 +269,7 @@ PRECISIONS = {"float32": 0, "fp32": 0, "float64": 1, "fp64": 1, "float64_all": 2
 TREES = {"default": 0, "median": 1, "sah": 2, "two_level": 3}
 NODES = {"auto": 0, "float": 1, "packed": 2}
 KERNELS = {"auto": 0, "all_books": 1, "quality": 2}
+WAITS = {"spin": 0, "sleep": 1}
 TONE_MAPS = {"books": 0, "none": 1, "agx": 2, "neutral": 3}
 
 
@@ -331,12 +333,13 @@ def build_scene(name: str, **overrides) -> BuiltScene:
 
 def _session_options(seed=0, precision="float32", integrator="auto", launch_seconds=0.0, block_size=0,
                      wavefront_slots=0, wavefront_graph=0, wavefront_drain=0, megakernel_waves=0,
-                     tree="default", nodes="auto", kernel="auto") -> SessionOptions:
+                     tree="default", nodes="auto", kernel="auto", wait="spin") -> SessionOptions:
     return SessionOptions(seed, PRECISIONS[precision], INTEGRATORS[integrator],
                           launch_seconds, block_size, wavefront_slots, wavefront_graph, wavefront_drain,
                           megakernel_waves, TREES[tree] if isinstance(tree, str) else int(tree),
                           NODES[nodes] if isinstance(nodes, str) else int(nodes),
-                          KERNELS[kernel] if isinstance(kernel, str) else int(kernel))
+                          KERNELS[kernel] if isinstance(kernel, str) else int(kernel),
+                          WAITS[wait] if isinstance(wait, str) else int(wait))
 
 
 @dataclass
@@ -666,9 +669,13 @@ def environment_tables(rgb: np.ndarray, yaw_degrees: float = 0.0, scale: float =
     return EnvironmentTables(texels, row_cos, q[:bins], alias[:bins], log2.value, total.value)
 
 
-def query_device() -> DeviceInfo:
+def query_device(wait=None) -> DeviceInfo:
+    """Describes device 0 and starts the runtime; wait ("spin" or "sleep") also sets how the device's own waits wait"""
     info = DeviceInfo()
-    _check(_lib.rt_query_device(C.byref(info)))
+    if wait is None:
+        _check(_lib.rt_query_device(C.byref(info)))
+    else:
+        _check(_lib.rt_query_device_wait(C.byref(info), WAITS[wait] if isinstance(wait, str) else int(wait)))
     return info
 
 

@@ -671,6 +671,22 @@ int rt_server_release(rt_server* server, int64_t id)
     });
 }
 
+int rt_query_device_wait(rt_device_info* info, int32_t wait)
+{
+    return guarded([&] {
+        if (wait != RT_WAIT_SPIN && wait != RT_WAIT_SLEEP)
+        {
+            throw std::invalid_argument("unknown wait mode");
+        }
+        rt::check(hipSetDeviceFlags(wait == RT_WAIT_SLEEP ? hipDeviceScheduleBlockingSync : hipDeviceScheduleSpin),
+                  "setting how the device waits");
+        if (const int status = rt_query_device(info); status != RT_OK)
+        {
+            throw std::runtime_error(rt_last_error());
+        }
+    });
+}
+
 int rt_query_device(rt_device_info* info)
 {
     return guarded([&] {

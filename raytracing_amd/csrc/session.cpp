@@ -24,7 +24,11 @@ constexpr uint64_t round_seed_step = uint64_t{1} << 32;
 class hip_event
 {
 public:
-    hip_event() { check(hipEventCreate(&e_), "creating a HIP event"); }
+    // blocking: a wait for the event sleeps until the GPU signals instead of spinning (RT_WAIT_SLEEP)
+    explicit hip_event(bool blocking = false)
+    {
+        check(hipEventCreateWithFlags(&e_, blocking ? hipEventBlockingSync : hipEventDefault), "creating a HIP event");
+    }
     ~hip_event() { (void)hipEventDestroy(e_); }
     hip_event(const hip_event&) = delete;
     hip_event& operator=(const hip_event&) = delete;
@@ -183,7 +187,8 @@ class session_impl final : public session
 public:
     session_impl(std::shared_ptr<const shared_scene> scene, const rt_camera_desc& camera,
                  const rt_session_options& options)
-        : shared_(std::move(scene)), scene_(shared_->tables), options_(options), dscene_(shared_->device<Real>())
+        : shared_(std::move(scene)), scene_(shared_->tables), options_(options), dscene_(shared_->device<Real>()),
+          start_(options.wait == RT_WAIT_SLEEP), stop_(options.wait == RT_WAIT_SLEEP)
     {
         view_scene_ = shared_->view;
         if (options.kernel == RT_KERNEL_QUALITY && !dscene_.quality())
@@ -1262,6 +1267,22 @@ void check_options(const rt_session_options& options)
     if (options.kernel < RT_KERNEL_AUTO || options.kernel > RT_KERNEL_QUALITY)
     {
         throw std::invalid_argument("unknown kernel");
+    }
+    if (options.wait != RT_WAIT_SPIN && options.wait != RT_WAIT_SLEEP)
+    {
+        throw std::invalid_argument("unknown wait mode");
+    }
+    if (options.wait == RT_WAIT_SLEEP)
+    {
+        // an event that sleeps needs the device's waits to sleep too, else its copies still spin (as the reference
+        // requires, src/gpu/renderer.h:536-540)
+        unsigned flags = 0;
+        check(hipGetDeviceFlags(&flags), "reading how the device waits");
+        if ((flags & hipDeviceScheduleMask) != hipDeviceScheduleBlockingSync)
+        {
+            throw std::logic_error("a session that sleeps needs the device set to sleep first "
+                                   "(rt_query_device_wait with RT_WAIT_SLEEP)");
+        }
     }
 }
 }

@@ -6,6 +6,7 @@
   (scripts/verify/compare.ps1: a second, independent pair of seeds decides when the first pair fails);
 - byte reproducibility: a fixed seed gives the same sums whatever the launch grouping or session history.
 """
+import os
 import zlib
 
 import numpy as np
@@ -419,3 +420,36 @@ def test_kernel_refusals(rt):
         rt.Session(rt.build_scene("weekend", width=32, samples=4), kernel="quality", precision="float64")
     with pytest.raises(ValueError):
         rt.Session(rt.build_scene("cornell", width=32, samples=4, sampling="mixture"), kernel="quality")
+
+
+SLEEPING_SESSION = """
+import numpy as np, sys
+sys.path.insert(0, {root!r})
+import raytracing_amd as rt
+info = rt.query_device(wait="sleep")
+scene = rt.build_scene("weekend", width=64, samples=16)
+with rt.Session(scene, seed=3, integrator="megakernel", wait="sleep") as s:
+    s.render_to(16)
+    a = s.sums()
+with rt.Session(scene, seed=3, integrator="megakernel") as s:
+    s.render_to(16)
+    b = s.sums()
+assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+print("sleeping session ok", info.compute_units)
+"""
+
+
+def test_wait_modes(rt):
+    """session_options::wait and query_device(wait) (src/gpu/renderer.h:505-540): a session that sleeps renders the
+    same sums as one that spins, once the device's waits sleep too (set before the runtime starts: in a process of its
+    own), and is refused with a logic error before that"""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", SLEEPING_SESSION.format(root=root)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "sleeping session ok" in r.stdout, r.stdout + r.stderr
+    with pytest.raises(rt.RenderLogicError, match="sleep"):
+        rt.Session(rt.build_scene("weekend", width=32, samples=4), wait="sleep")
+    with pytest.raises(ValueError):
+        rt.Session(rt.build_scene("weekend", width=32, samples=4), wait=5)
