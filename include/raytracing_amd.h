@@ -499,6 +499,11 @@ typedef struct rt_render_options
     int64_t passes;
     rt_progress_fn progress; /* may be NULL: called as passes complete (rt_progress_fn) */
     void* progress_user;
+    /* render_options::display: a tone map other than RT_TONE_BOOKS fills rgba (W*H*4 bytes) with the display's bytes
+     * after the last pass, as rt_session_display_toned; RT_TONE_BOOKS (0) leaves rgba alone (may be NULL) */
+    int32_t tone_map;
+    double exposure;
+    uint8_t* rgba;
 } rt_render_options;
 
 typedef struct rt_render_stats
@@ -507,6 +512,7 @@ typedef struct rt_render_stats
     double kernel_seconds;
     uint64_t rays;
     uint64_t shadow_rays;
+    rt_kernel_info kernel; /* the kernel that rendered (render_result::kernel) */
 } rt_render_stats;
 
 int rt_render(const rt_scene* scene, const rt_render_options* options, double* pixels, size_t count,

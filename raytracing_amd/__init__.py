@@ -147,12 +147,13 @@ PROGRESS_FN = C.CFUNCTYPE(None, C.c_int64, C.c_int64, C.c_void_p)
 
 class RenderOptions(C.Structure):
     _fields_ = [("session", SessionOptions), ("passes", C.c_int64), ("progress", PROGRESS_FN),
-                ("progress_user", C.c_void_p)]
+                ("progress_user", C.c_void_p), ("tone_map", C.c_int32), ("exposure", C.c_double),
+                ("rgba", C.POINTER(C.c_uint8))]
 
 
 class RenderStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("kernel_seconds", C.c_double), ("rays", C.c_uint64),
-                ("shadow_rays", C.c_uint64)]
+                ("shadow_rays", C.c_uint64), ("kernel", KernelInfo)]
 
 
 class Traffic(C.Structure):
@@ -350,6 +351,8 @@ class RenderResult:
     kernel_seconds: float
     rays: int
     shadow_rays: int
+    kernel: str = ""             # the kernel that rendered
+    rgba: np.ndarray = None      # (height, width, 4) display bytes, for a tone map other than the books'
 
 
 def _progress_fn(progress):
@@ -359,16 +362,22 @@ def _progress_fn(progress):
     return PROGRESS_FN(lambda done, total, _user: progress(done, total))
 
 
-def render(scene: BuiltScene, passes: int = 0, progress=None, **options) -> RenderResult:
+def render(scene: BuiltScene, passes: int = 0, progress=None, tone_map: str = "books", exposure: float = 0.0,
+           **options) -> RenderResult:
     """gpu::render: a session rendered to ``passes`` (0: the scene's own samples); progress(done, total), if given, is
-    called as passes complete."""
+    called as passes complete; a tone map other than the books' also gives the display's bytes (render_result::rgba)."""
     desc = scene.desc
-    out = np.zeros((desc.camera.image_height, desc.camera.image_width, 3), dtype=np.float64)
+    h, w = desc.camera.image_height, desc.camera.image_width
+    out = np.zeros((h, w, 3), dtype=np.float64)
+    tm = TONE_MAPS[tone_map] if isinstance(tone_map, str) else int(tone_map)
+    rgba = np.zeros((h, w, 4), np.uint8) if tm != 0 else None
     callback = _progress_fn(progress)  # (kept alive through the call)
-    ro = RenderOptions(_session_options(**options), passes if passes else scene.passes, callback, None)
+    ro = RenderOptions(_session_options(**options), passes if passes else scene.passes, callback, None, tm, exposure,
+                       rgba.ctypes.data_as(_P(C.c_uint8)) if rgba is not None else None)
     stats = RenderStats()
     _check(_lib.rt_render(C.byref(desc), C.byref(ro), out.ctypes.data_as(_P(C.c_double)), out.size, C.byref(stats)))
-    return RenderResult(out, stats.seconds, stats.kernel_seconds, stats.rays, stats.shadow_rays)
+    return RenderResult(out, stats.seconds, stats.kernel_seconds, stats.rays, stats.shadow_rays,
+                        stats.kernel.name.decode(), rgba)
 
 
 class Session:

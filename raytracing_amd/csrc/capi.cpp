@@ -312,12 +312,24 @@ int rt_render(const rt_scene* scene, const rt_render_options* options, double* p
         {
             on_pass = [&](int64_t done) { o.progress(done, passes, o.progress_user); };
         }
+        if (o.tone_map != RT_TONE_BOOKS && !o.rgba)
+        {
+            throw std::invalid_argument("a tone map other than the books' needs the rgba buffer it fills");
+        }
         s->render_to(passes, 0, on_pass);
         s->pixels(pixels, count);
+        // the render's time, before the display's bytes (as the reference times it)
+        const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        if (o.tone_map != RT_TONE_BOOKS)
+        {
+            s->display_toned(o.tone_map, o.exposure, o.rgba,
+                             4ull * static_cast<size_t>(scene->camera.image_width) * scene->camera.image_height);
+        }
         if (stats)
         {
             s->counters(&stats->rays, &stats->shadow_rays, &stats->kernel_seconds);
-            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+            stats->kernel = s->kernel();
+            stats->seconds = seconds;
         }
     });
 }

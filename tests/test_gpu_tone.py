@@ -88,3 +88,19 @@ def test_books_and_refusals(rt):
         s.render_to(1)
         with pytest.raises(ValueError, match="float32"):
             s.display("agx", 0.0)
+
+
+def test_render_display_option(rt):
+    """render_options::display (src/gpu/renderer.h:444-448): a tone map other than the books' fills the result's rgba
+    with the display's bytes after the last pass, the same bytes a session's display gives; the pixels don't depend on
+    it; the result names the kernel that rendered"""
+    scene = rt.build_scene("cornell", width=64, samples=16)
+    plain = rt.render(scene, seed=2, integrator="megakernel")
+    assert plain.rgba is None and plain.kernel.startswith("all books")
+    toned = rt.render(scene, seed=2, integrator="megakernel", tone_map="agx", exposure=0.5)
+    assert np.array_equal(toned.pixels.view(np.uint64), plain.pixels.view(np.uint64))
+    with rt.Session(scene, seed=2, integrator="megakernel") as s:
+        s.render_to(scene.passes)
+        want = s.display("agx", 0.5)
+    assert np.array_equal(toned.rgba, want)
+    assert np.all(toned.rgba[..., 3] == 255)
