@@ -584,6 +584,9 @@ enum rt_lobe
     RT_LOBE_CONDUCTOR = 2, /* GGX conductor of F0 `color` and alpha (= roughness^2; below 1e-3 a Fresnel mirror),
                               with the multiple-scattering compensation of the energy table at `roughness` */
     RT_LOBE_GLOSS = 3,     /* glossy coat (GGX dielectric of F0 0.04, alpha) over a matte base of color `color` */
+    RT_LOBE_METAL = 4,     /* the books' fuzzed metal: albedo `color`, fuzz `roughness` (a delta lobe: density 0) */
+    RT_LOBE_DIELECTRIC_ENTERING = 5, /* the books' glass, from outside: refraction index `alpha` */
+    RT_LOBE_DIELECTRIC_LEAVING = 6,  /* the same, from inside (v on the inner side) */
 };
 /* On device 0, for `count` cases in the tangent frame (the normal is +z; v toward the viewer, l toward the light, 3
  * floats each): value (3 per case) = f(v, l) cos_l and density = the density the lobe draws l with from v (per solid

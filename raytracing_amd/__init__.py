@@ -576,7 +576,7 @@ def sample_numbers(seed: int, pixels, passes, draws: int):
     return words, floats, doubles, dirs
 
 
-LOBES = {"lambert": 0, "phase": 1, "conductor": 2, "gloss": 3}
+LOBES = {"lambert": 0, "phase": 1, "conductor": 2, "gloss": 3, "metal": 4, "glass_entering": 5, "glass_leaving": 6}
 
 
 def scatter_probe(lobe, v, l, color=(1.0, 1.0, 1.0), alpha: float = 0.0, roughness: float = 0.0, seed: int = 0):

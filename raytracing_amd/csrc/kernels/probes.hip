@@ -267,9 +267,11 @@ __global__ void __launch_bounds__(256) scatter_kernel(int lobe, float alpha, flo
         return;
     }
     scatterer<float> s{};
+    const bool books = lobe >= RT_LOBE_METAL;
     s.kind = lobe == RT_LOBE_LAMBERT     ? scatter_kind::lambert
              : lobe == RT_LOBE_PHASE     ? scatter_kind::phase
              : lobe == RT_LOBE_CONDUCTOR ? (alpha >= min_ggx_alpha ? scatter_kind::conductor : scatter_kind::mirror)
+             : books                     ? scatter_kind::books
                                          : scatter_kind::coat;
     s.color = color;
     s.local = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
@@ -277,16 +279,23 @@ __global__ void __launch_bounds__(256) scatter_kernel(int lobe, float alpha, flo
     s.alpha = alpha;
     s.roughness = roughness;
     vertex<float> vx{};
-    vx.n = {0.0f, 0.0f, 1.0f};
-    vx.outward = vx.n;
-    vx.front = true;
+    vx.n = {0.0f, 0.0f, 1.0f}; // facing the viewer
+    vx.outward = lobe == RT_LOBE_DIELECTRIC_LEAVING ? vec3<float>{0.0f, 0.0f, -1.0f} : vx.n;
+    vx.front = lobe != RT_LOBE_DIELECTRIC_LEAVING;
+    // the books' delta lobes read their material: a metal's albedo and fuzz, a dielectric's index
+    material_rec<float> m{};
+    m.kind = lobe == RT_LOBE_METAL ? RT_MATERIAL_METAL : RT_MATERIAL_DIELECTRIC;
+    m.albedo[0] = color.x;
+    m.albedo[1] = color.y;
+    m.albedo[2] = color.z;
+    m.fuzz = roughness;
+    m.refraction_index = alpha;
     const ggx_eval_result<float> e = scatter_value(s, vx, vec3<float>{l[3 * k], l[3 * k + 1], l[3 * k + 2]});
     value[3 * k] = e.value.x;
     value[3 * k + 1] = e.value.y;
     value[3 * k + 2] = e.value.z;
     density[k] = e.pdf;
     pcg32 g = sample_generator(seed, static_cast<uint64_t>(k), 0);
-    const material_rec<float> m{};
     scatter_draw<float> d{};
     const bool ok = scatter_sample(s, m, -s.v, vx, g, d);
     drawn[3 * k] = ok ? d.wi.x : 0.0f;

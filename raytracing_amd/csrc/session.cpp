@@ -1209,11 +1209,12 @@ void scatter_probe(int32_t lobe, float alpha, float roughness, const float color
                    const float* l, uint64_t seed, float* value, float* density, float* drawn, float* weight,
                    float* drawn_density)
 {
-    if (lobe < RT_LOBE_LAMBERT || lobe > RT_LOBE_GLOSS)
+    if (lobe < RT_LOBE_LAMBERT || lobe > RT_LOBE_DIELECTRIC_LEAVING)
     {
         throw std::invalid_argument("unknown scattering lobe");
     }
-    if (n < 0 || !color || !(alpha >= 0.0f) || !(roughness >= 0.0f) || roughness > 1.0f)
+    if (n < 0 || !color || !(alpha >= 0.0f) || !(roughness >= 0.0f) || roughness > 1.0f ||
+        (lobe >= RT_LOBE_DIELECTRIC_ENTERING && !(alpha > 0.0f)))
     {
         throw std::invalid_argument("0 or more cases, a color, alpha >= 0 and a roughness in [0, 1]");
     }

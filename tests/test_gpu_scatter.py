@@ -17,7 +17,11 @@ code:
   follow the density (a chi-square test over cells of the hemisphere, expected counts by quadrature of the float64
   density); the lobes are reciprocal; a white conductor's compensated energy is 1 within the table's interpolation,
   and its single-scattering energy (the draws' weights over the compensation) is the table's E;
-- below alpha 1e-3 the conductor is a mirror: the drawn direction is v reflected, weighted by Schlick's F, density 0.
+- below alpha 1e-3 the conductor is a mirror: the drawn direction is v reflected, weighted by Schlick's F, density 0;
+- the books' delta lobes: a fuzzed metal's direction is the unit mirror direction plus fuzz times a point uniform on
+  the unit sphere, absorbed below the surface, weighted by the albedo; glass reflects with Schlick's probability
+  (binomial test per angle), always beyond the critical angle, and otherwise refracts by Snell's law in the plane of
+  incidence (the reference's bsdf draws and distributions).
 """
 import json
 import math
@@ -243,3 +247,69 @@ def test_probe_refusals(rt):
         rt.scatter_probe(7, v, v)
     with pytest.raises(ValueError):
         rt.scatter_probe("conductor", v, v, roughness=2.0)
+
+
+@pytest.mark.parametrize("fuzz", [0.0, 0.3, 0.9])
+def test_books_metal(rt, fuzz):
+    """The books' fuzzed metal (src/gpu/material.cuh: metal): the mirror direction, unit, plus fuzz times a point
+    uniform on the unit sphere, absorbed when that leaves below the surface; the weight is the albedo"""
+    rng = np.random.default_rng(31)
+    n = 200000
+    v = _hemisphere(rng, n, 0.05)
+    v[: n // 2] = np.array([0.0, 0.0, 1.0])  # half at normal incidence, for the sphere point's distribution
+    albedo = [0.8, 0.6, 0.2]
+    d = rt.scatter_probe("metal", v, v, color=albedo, roughness=fuzz, seed=8)
+    ok = np.linalg.norm(d["drawn"], axis=1) > 0
+    mirror = v * np.array([-1, -1, 1])
+    u = (d["drawn"][ok].astype(np.float64) - mirror[ok]) / max(fuzz, 1e-30)
+    assert np.all(d["drawn_density"] == 0)
+    assert np.all(d["weight"][ok] == np.float32(albedo))
+    if fuzz == 0:
+        assert ok.all() and np.allclose(d["drawn"], mirror, atol=1e-6)
+        return
+    assert np.allclose(np.linalg.norm(u, axis=1), 1, atol=2e-5 / fuzz)
+    assert np.all(d["drawn"][ok][:, 2] > 0)
+    # absorbed exactly where the fuzzed direction would go below the surface: never at normal incidence for fuzz < 1
+    normal = ok[: n // 2]
+    assert normal.all() if fuzz < 1 else normal.mean() > 0.99
+    uz = u[: normal.sum(), 2]
+    assert stats_p(np.clip(uz, -1, 1), "uniform", (-1, 2)) > 1e-4, "the sphere point's z is uniform in [-1, 1]"
+
+
+def stats_p(x, dist, args):
+    from scipy import stats
+    return stats.kstest(x, dist, args=args).pvalue
+
+
+@pytest.mark.parametrize("side,ior", [("glass_entering", 1.5), ("glass_leaving", 1.5), ("glass_entering", 1.33)])
+def test_books_glass(rt, side, ior):
+    """The books' dielectric: the mirror direction with Schlick's probability (r0 = ((1 - eta) / (1 + eta))^2, eta the
+    ratio of indices across the interface), always beyond the critical angle, else Snell's refraction: in the plane of
+    incidence, sin theta_t = eta sin theta_i, into the other side; weight 1"""
+    n_views, per = 12, 40000
+    eta = 1 / ior if side == "glass_entering" else ior
+    for k in range(n_views):
+        cos_i = 0.05 + 0.95 * k / (n_views - 1)
+        v1 = np.array([math.sqrt(1 - cos_i ** 2), 0.0, cos_i], np.float32).astype(np.float64)
+        v = np.tile(v1, (per, 1))
+        d = rt.scatter_probe(side, v, v, alpha=ior, seed=100 + k)
+        w = d["drawn"].astype(np.float64)
+        assert np.all(d["weight"] == 1)
+        reflected = w[:, 2] > 0
+        mirror = v1 * np.array([-1, -1, 1])
+        assert np.allclose(w[reflected], mirror, atol=1e-6)
+        ci = v1[2]
+        tir = 1 - eta ** 2 * (1 - ci ** 2) < 0
+        if tir:
+            assert reflected.all()
+            continue
+        wt = w[~reflected]
+        wt /= np.linalg.norm(wt, axis=1, keepdims=True)
+        assert np.allclose(wt[:, 1], 0, atol=1e-6)                   # the plane of incidence
+        assert np.allclose(np.hypot(wt[:, 0], wt[:, 1]), eta * math.sqrt(1 - ci ** 2), atol=2e-6)  # Snell
+        assert np.all(wt[:, 0] < 0 + 1e-7) or ci > 0.999               # onward, across the normal's plane
+        r0 = ((1 - eta) / (1 + eta)) ** 2
+        schlick = r0 + (1 - r0) * (1 - ci) ** 5
+        from scipy import stats
+        p = stats.binomtest(int(reflected.sum()), per, schlick).pvalue
+        assert p > 1e-4, f"cos {ci:.3f}: {reflected.mean():.4f} reflected, Schlick {schlick:.4f}"
