@@ -429,8 +429,16 @@ public:
             throw std::invalid_argument("a time from 0 to 3600 seconds and a pass limit of 0 or more");
         }
         rt_render_report report{};
+        // a full launch: the session's planned length, or the whole time when that is less
+        const double full = std::min(seconds, launch_seconds_);
         while (state_.passes < pass_limit && (report.gpu_seconds < seconds || state_.passes == 0))
         {
+            // after a first launch, end early rather than make one shorter than a quarter of a full one
+            // (src/gpu/renderer.h:606-612)
+            if (report.launches > 0 && state_.passes > 0 && seconds - report.gpu_seconds < 0.25 * full)
+            {
+                break;
+            }
             const int64_t remaining = pass_limit - state_.passes;
             int64_t step = seconds_per_pass_ > 0
                                ? static_cast<int64_t>(std::max(1.0, (seconds - report.gpu_seconds) / seconds_per_pass_))

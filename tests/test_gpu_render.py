@@ -453,3 +453,26 @@ def test_wait_modes(rt):
         rt.Session(rt.build_scene("weekend", width=32, samples=4), wait="sleep")
     with pytest.raises(ValueError):
         rt.Session(rt.build_scene("weekend", width=32, samples=4), wait=5)
+
+
+def test_render_for(rt):
+    """session::render_for (src/gpu/renderer.h:606-614): about `seconds` of GPU time, never past pass_limit, at least
+    one complete pass, nothing once the limit is reached, no launch shorter than a quarter of a full one after the
+    first; and the launch planner keeps launches near their planned length (src/gpu/renderer.cu:201-308)"""
+    scene = rt.build_scene("final", width=240, samples=4096, earth_width=512)
+    with rt.Session(scene, seed=1, integrator="megakernel", launch_seconds=0.02) as s:
+        r = s.render_for(0.0, 1000)
+        assert r.passes >= 1 and r.rows == 0, "at least one complete pass"
+        before = s.state()[0]
+        r = s.render_for(0.25, 10 ** 6)
+        assert r.passes > before
+        assert 0.25 * 0.75 <= r.gpu_seconds <= 0.25 + 2 * r.longest_launch_seconds, (r.gpu_seconds, r.launches)
+        assert r.longest_launch_seconds < 4 * 0.02, f"a launch of {r.longest_launch_seconds:.3f} s planned at 0.02"
+        r = s.render_for(1.0, s.state()[0])
+        assert r.launches == 0, "a pass limit the image has reached does nothing"
+        limit = s.state()[0] + 3
+        r = s.render_for(10.0, limit)
+        assert r.passes == limit and s.state() == (limit, 0), "never past the pass limit"
+        for seconds, limit in ((-1.0, 10), (4000.0, 10), (1.0, -1)):
+            with pytest.raises(ValueError):
+                s.render_for(seconds, limit)
