@@ -575,6 +575,13 @@ int rt_session_reference_hits(rt_session* session, const float* origins, const f
 int rt_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passes, int64_t count, int32_t draws,
                       uint32_t* words, float* floats, double* doubles, float* directions);
 
+/* The display's tone maps on their own (src/gpu/tonemap.cu; the reference's tone tests drive encode and quantize
+ * through test kernels, tests/tone_tests.cu:387-400): `count` linear colours (3 floats each) times 2^exposure through
+ * RT_TONE_NONE, _AGX or _NEUTRAL, each channel quantized to a byte with dither[k] in [-0.5, 0.5) (NULL: none), into
+ * out (3 bytes per colour). On device 0. */
+int rt_tone_probe(int32_t tone_map, double exposure, const float* colours, const float* dither, int64_t count,
+                  uint8_t* out);
+
 /* The quality path's scattering lobes (src/gpu/bsdf.cuh:58-201, src/gpu/ggx.cuh:59-324), which the reference's tests
  * drive through test kernels (tests/bsdf_tests.cu, tests/ggx_tests.cu, tests/gloss_tests.cu) */
 enum rt_lobe

@@ -231,6 +231,7 @@ _SIGNATURES = {
                                             _P(C.c_uint8)]),
     "rt_sample_numbers": (C.c_int, [C.c_uint64, _P(C.c_uint64), _P(C.c_uint64), C.c_int64, C.c_int32,
                                     _P(C.c_uint32), _P(C.c_float), _P(C.c_double), _P(C.c_float)]),
+    "rt_tone_probe": (C.c_int, [C.c_int32, C.c_double, _P(C.c_float), _P(C.c_float), C.c_int64, _P(C.c_uint8)]),
     "rt_scatter_probe": (C.c_int, [C.c_int32, C.c_float, C.c_float, _P(C.c_float), C.c_int64, _P(C.c_float),
                                    _P(C.c_float), C.c_uint64, _P(C.c_float), _P(C.c_float), _P(C.c_float),
                                    _P(C.c_float), _P(C.c_float)]),
@@ -574,6 +575,18 @@ def sample_numbers(seed: int, pixels, passes, draws: int):
                                   draws, words.ctypes.data_as(_P(C.c_uint32)), floats.ctypes.data_as(_P(C.c_float)),
                                   doubles.ctypes.data_as(_P(C.c_double)), dirs.ctypes.data_as(_P(C.c_float))))
     return words, floats, doubles, dirs
+
+
+def tone_probe(tone_map, colours, exposure: float = 0.0, dither=None) -> np.ndarray:
+    """Linear colours (n x 3) through a tone map ("none", "agx", "neutral") at an exposure, quantized with a dither
+    per colour (None: none), on device 0: n x 3 bytes"""
+    c = np.ascontiguousarray(colours, dtype=np.float32).reshape(-1, 3)
+    d = None if dither is None else np.ascontiguousarray(dither, dtype=np.float32)
+    out = np.zeros((len(c), 3), np.uint8)
+    _check(_lib.rt_tone_probe(TONE_MAPS[tone_map] if isinstance(tone_map, str) else int(tone_map), exposure,
+                              c.ctypes.data_as(_P(C.c_float)), d.ctypes.data_as(_P(C.c_float)) if d is not None else None,
+                              len(c), out.ctypes.data_as(_P(C.c_uint8))))
+    return out
 
 
 LOBES = {"lambert": 0, "phase": 1, "conductor": 2, "gloss": 3, "metal": 4, "glass_entering": 5, "glass_leaving": 6}

@@ -441,6 +441,19 @@ int rt_sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* pas
     return guarded([&] { rt::sample_numbers(seed, pixels, passes, count, draws, words, floats, doubles, directions); });
 }
 
+int rt_tone_probe(int32_t tone_map, double exposure, const float* colours, const float* dither, int64_t count,
+                  uint8_t* out)
+{
+    return guarded([&] {
+        if (count > 0)
+        {
+            need(colours, "colours");
+            need(out, "out");
+        }
+        rt::tone_probe(tone_map, exposure, colours, dither, count, out);
+    });
+}
+
 int rt_scatter_probe(int32_t lobe, float alpha, float roughness, const float color[3], int64_t count, const float* v,
                      const float* l, uint64_t seed, float* value, float* density, float* drawn, float* weight,
                      float* drawn_density)

@@ -96,6 +96,9 @@ hipError_t launch_tone_map(const double* sums, int width, int64_t pixels, double
                            float exposure_scale, uint8_t* rgba, hipStream_t stream);
 hipError_t launch_display(const double* sums, int64_t pixels, double inverse_passes, uint8_t* rgba,
                           hipStream_t stream);
+// n linear colours through the tone map `op` and quantized with the given dither (tone.hip, tone_probe_kernel)
+hipError_t launch_tone_probe(const float* colours, const float* dither, int64_t n, int op, float exposure_scale,
+                             uint8_t* out, hipStream_t stream);
 
 hipError_t launch_copy(const void* in, void* out, size_t bytes, hipStream_t stream);
 

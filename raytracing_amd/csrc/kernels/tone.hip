@@ -178,10 +178,44 @@ __global__ void __launch_bounds__(256) tone_map_kernel(const double* __restrict_
     out[2] = quantize(v.b, dither);
     out[3] = 255;
 }
+
+// The parity probe: colour k (linear, times the exposure's scale) through the tone map `op` and quantized with
+// dither[k] (0 where dither is null), as the display kernel does for a pixel
+__global__ void __launch_bounds__(256) tone_probe_kernel(const float* __restrict__ colours,
+                                                         const float* __restrict__ dither, int64_t n, int op,
+                                                         float exposure_scale, uint8_t* __restrict__ out)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+    {
+        return;
+    }
+    const rgb light{colours[3 * k] * exposure_scale, colours[3 * k + 1] * exposure_scale,
+                    colours[3 * k + 2] * exposure_scale};
+    const rgb v = encode(light, op);
+    const float d = dither ? dither[k] : 0.0f;
+    out[3 * k] = quantize(v.r, d);
+    out[3 * k + 1] = quantize(v.g, d);
+    out[3 * k + 2] = quantize(v.b, d);
+}
 }
 
 namespace rt
 {
+hipError_t launch_tone_probe(const float* colours, const float* dither, int64_t n, int op, float exposure_scale,
+                             uint8_t* out, hipStream_t stream)
+{
+    if (n <= 0)
+    {
+        return hipSuccess;
+    }
+    const int threads = 256;
+    const dim3 grid(static_cast<unsigned>((n + threads - 1) / threads));
+    hipLaunchKernelGGL(dev::tone_probe_kernel, grid, dim3(threads), 0, stream, colours, dither, n, op, exposure_scale,
+                       out);
+    return hipGetLastError();
+}
+
 hipError_t launch_tone_map(const double* sums, int width, int64_t pixels, double inverse_passes, int op,
                            float exposure_scale, uint8_t* rgba, hipStream_t stream)
 {

@@ -1213,6 +1213,36 @@ void sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passe
     }
 }
 
+void tone_probe(int32_t tone_map, double exposure, const float* colours, const float* dither, int64_t n,
+                uint8_t* out)
+{
+    if (tone_map < RT_TONE_BOOKS || tone_map > RT_TONE_NEUTRAL || tone_map == RT_TONE_BOOKS)
+    {
+        throw std::invalid_argument("the tone probe takes RT_TONE_NONE, RT_TONE_AGX or RT_TONE_NEUTRAL");
+    }
+    if (!std::isfinite(exposure) || std::fabs(exposure) > 20 || n < 0)
+    {
+        throw std::invalid_argument("an exposure within +-20 stops and 0 or more colours");
+    }
+    if (n == 0)
+    {
+        return;
+    }
+    const std::size_t m = static_cast<std::size_t>(n);
+    device_array<float> c(3 * m), d(dither ? m : 0);
+    device_array<uint8_t> o(3 * m);
+    check(hipMemcpy(c.data(), colours, c.bytes(), hipMemcpyHostToDevice), "copying colours");
+    if (dither)
+    {
+        check(hipMemcpy(d.data(), dither, d.bytes(), hipMemcpyHostToDevice), "copying dither");
+    }
+    check(launch_tone_probe(c.data(), dither ? d.data() : nullptr, n, tone_map,
+                            static_cast<float>(std::exp2(exposure)), o.data(), nullptr),
+          "launching the tone probe");
+    check(hipDeviceSynchronize(), "tone mapping");
+    check(hipMemcpy(out, o.data(), o.bytes(), hipMemcpyDeviceToHost), "copying bytes");
+}
+
 void scatter_probe(int32_t lobe, float alpha, float roughness, const float color[3], int64_t n, const float* v,
                    const float* l, uint64_t seed, float* value, float* density, float* drawn, float* weight,
                    float* drawn_density)

@@ -94,6 +94,9 @@ std::size_t view_device_bytes(const shared_scene& scene, const rt_camera_desc& c
 
 // The per-sample generator's outputs on device 0 (the numbers the kernels draw): n items of (pixel, pass) for one
 // seed; any output may be null (probes.hip, sample_numbers_kernel)
+// n linear colours through a tone map at an exposure, quantized with a dither each (or none), on device 0
+void tone_probe(int32_t tone_map, double exposure, const float* colours, const float* dither, int64_t n,
+                uint8_t* out);
 // The quality path's scattering lobe `lobe` (rt_lobe) on n cases, on device 0 (probes.hip, scatter_kernel)
 void scatter_probe(int32_t lobe, float alpha, float roughness, const float color[3], int64_t n, const float* v,
                    const float* l, uint64_t seed, float* value, float* density, float* drawn, float* weight,

@@ -104,3 +104,43 @@ def test_render_display_option(rt):
         want = s.display("agx", 0.5)
     assert np.array_equal(toned.rgba, want)
     assert np.all(toned.rgba[..., 3] == 255)
+
+
+# The reference's grey bytes (tests/tone_tests.cu:1523-1550, the tone maps' published greys): a grey's byte with no
+# dither; "none" exact, AgX and PBR Neutral within one level (their channels may round apart)
+REFERENCE_GREYS = {
+    "none": (0, [(0.001, 3), (0.01, 25), (0.05, 63), (0.18, 118), (0.5, 188)]),
+    "agx": (1, [(0.001, 4), (0.01, 25), (0.05, 72), (0.18, 127), (0.5, 173), (1.0, 201), (2.0, 223), (4.0, 239),
+                (16.0, 255), (100.0, 255)]),
+    "neutral": (1, [(0.18, 105), (1.0, 240), (4.0, 253)]),
+}
+
+
+@pytest.mark.parametrize("op", ["none", "agx", "neutral"])
+def test_reference_grey_bytes(rt, op):
+    allowed, greys = REFERENCE_GREYS[op]
+    v = np.array([[g, g, g] for g, _ in greys], np.float32)
+    got = rt.tone_probe(op, v).astype(int)
+    want = np.array([b for _, b in greys])[:, None]
+    assert np.all(np.abs(got - want) <= allowed), f"{op}: {got.tolist()} against {want.ravel().tolist()}"
+
+
+def test_quantize_with_dither(rt):
+    """round(255 v + dither) with the dither in [-0.5, 0.5), black and white never dithered (src/gpu/tonemap.cu):
+    the sRGB OETF's bytes in float64, but for values within rounding of a step"""
+    rng = np.random.default_rng(12)
+    n = 200000
+    lin = rng.uniform(0, 1.2, (n, 3)).astype(np.float32)
+    lin[:100] = 0.0
+    lin[100:200] = 1.0
+    dither = rng.uniform(-0.5, 0.5, n).astype(np.float32)
+    got = rt.tone_probe("none", lin, dither=dither).astype(int)
+    x = np.clip(lin.astype(np.float64), 0, 1)
+    e = np.where(x <= 0.0031308, 12.92 * x, 1.055 * x ** (1 / 2.4) - 0.055)
+    raw = 255 * e + 0.5 + dither[:, None]
+    want = np.clip(np.floor(raw), 0, 255)
+    want = np.where(x <= 0, 0, np.where(x >= 1, 255, want))
+    near = np.abs(raw - np.round(raw)) < 1e-3
+    assert np.all(got[:100] == 0) and np.all(got[100:200] == 255)
+    assert np.all((got == want) | near), f"{np.count_nonzero((got != want) & ~near)} bytes off"
+    assert np.count_nonzero(got != want) < 1e-3 * got.size
