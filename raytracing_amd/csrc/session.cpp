@@ -1216,7 +1216,7 @@ void sample_numbers(uint64_t seed, const uint64_t* pixels, const uint64_t* passe
 void tone_probe(int32_t tone_map, double exposure, const float* colours, const float* dither, int64_t n,
                 uint8_t* out)
 {
-    if (tone_map < RT_TONE_BOOKS || tone_map > RT_TONE_NEUTRAL || tone_map == RT_TONE_BOOKS)
+    if (tone_map != RT_TONE_NONE && tone_map != RT_TONE_AGX && tone_map != RT_TONE_NEUTRAL)
     {
         throw std::invalid_argument("the tone probe takes RT_TONE_NONE, RT_TONE_AGX or RT_TONE_NEUTRAL");
     }

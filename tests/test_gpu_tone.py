@@ -144,3 +144,11 @@ def test_quantize_with_dither(rt):
     assert np.all(got[:100] == 0) and np.all(got[100:200] == 255)
     assert np.all((got == want) | near), f"{np.count_nonzero((got != want) & ~near)} bytes off"
     assert np.count_nonzero(got != want) < 1e-3 * got.size
+
+
+def test_tone_probe_refusals(rt):
+    grey = [[0.5, 0.5, 0.5]]
+    with pytest.raises(ValueError):
+        rt.tone_probe("books", grey)  # the books' bytes are the display kernel's, not a tone map's
+    with pytest.raises(ValueError):
+        rt.tone_probe("agx", grey, exposure=float("nan"))
