@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 3
+#define RT_ABI_VERSION 4
 
 enum rt_status
 {
@@ -665,6 +665,10 @@ typedef struct rt_server_options
     uint64_t memory_budget_bytes; /* device memory the server may hold for requests (0: 16 GiB) */
     int32_t max_active;           /* requests rendering at once (0: 8) */
     int32_t passes_per_step;      /* passes a request gets per scheduling step (0: 4) */
+    int64_t paths_per_step;       /* a request's step is raised to at least this many paths: a view of P pixels gets
+                                     max(passes_per_step, ceil(paths_per_step / P)) passes a step, at most 1024, with
+                                     staging for that many (0: passes_per_step alone). Small views then make few,
+                                     large launches instead of many launch-bound ones. */
 } rt_server_options;
 
 typedef struct rt_request_status
@@ -694,9 +698,9 @@ int rt_server_submit_view(rt_server* server, int64_t scene_id, const rt_camera_d
 /* Queues a request with a scene of its own (copied and uploaded for it, freed when it is released) */
 int rt_server_submit(rt_server* server, const rt_scene* scene, uint64_t seed, int64_t passes, int64_t* id);
 /* One scheduling step (continuous batching): admits queued requests as memory and slots free up, enqueues the next step
- * of every active request that has none in flight (up to its next power-of-two milestone, at most passes_per_step
- * passes, on one of the server's few streams, so the steps of different requests run side by side on the device),
- * then returns as soon as one step in flight has ended, with the number of requests still active in *active */
+ * of every active request that has none in flight (up to its next power-of-two milestone, at most its step's passes:
+ * passes_per_step, raised for small views by paths_per_step; on one of the server's few streams, so the steps of
+ * different requests run side by side on the device), then returns as soon as one step in flight has ended, with the number of requests still active in *active */
 int rt_server_step(rt_server* server, int32_t* active);
 
 /* The server's device memory: its budget; what it has reserved (resident scenes, and for each active request what its

@@ -168,7 +168,8 @@ TRAFFIC_EVENTS = ["node", "sphere", "moving_sphere", "quad", "medium", "boundary
 
 
 class ServerOptions(C.Structure):
-    _fields_ = [("memory_budget_bytes", C.c_uint64), ("max_active", C.c_int32), ("passes_per_step", C.c_int32)]
+    _fields_ = [("memory_budget_bytes", C.c_uint64), ("max_active", C.c_int32), ("passes_per_step", C.c_int32),
+                ("paths_per_step", C.c_int64)]
 
 
 class RequestStatus(C.Structure):
@@ -711,9 +712,10 @@ def copy_probe(nbytes: int, repeats: int = 5) -> float:
 class Server:
     """Progressive renders of mixed requests on one device, admitted by a memory budget (continuous batching)."""
 
-    def __init__(self, memory_budget_bytes=0, max_active=0, passes_per_step=0):
+    def __init__(self, memory_budget_bytes=0, max_active=0, passes_per_step=0, paths_per_step=0):
         h = C.c_void_p()
-        _check(_lib.rt_server_create(C.byref(ServerOptions(memory_budget_bytes, max_active, passes_per_step)),
+        _check(_lib.rt_server_create(C.byref(ServerOptions(memory_budget_bytes, max_active, passes_per_step,
+                                                           paths_per_step)),
                                      C.byref(h)))
         self._h = h
         self._scenes = {}

@@ -10,6 +10,11 @@
 
 namespace rt
 {
+namespace
+{
+constexpr int64_t max_step_passes = 1024; // a step's passes, and so its staging's (the megakernel stages 1024 at most)
+}
+
 server::server(const rt_server_options& options) : options_(options)
 {
     if (options_.memory_budget_bytes == 0)
@@ -24,12 +29,25 @@ server::server(const rt_server_options& options) : options_(options)
     {
         options_.passes_per_step = 4;
     }
+    if (options_.passes_per_step > max_step_passes || options_.paths_per_step < 0)
+    {
+        throw std::invalid_argument("passes_per_step at most 1024 and paths_per_step 0 or more");
+    }
     // no more streams than the hardware queues a process gets by default (4): more would only contend for them
     const int streams = std::min(4, options_.max_active);
     for (int k = 0; k < streams; ++k)
     {
         streams_.emplace_back();
     }
+}
+
+// A step of a view: passes_per_step, or more for a small view, so that each step traces at least paths_per_step paths
+// (a step of a 256x256 view at 4 passes is 0.26 M paths: a few launches' fixed costs take most of its time)
+int server::step_passes(const rt_camera_desc& camera) const
+{
+    const int64_t pixels = static_cast<int64_t>(camera.image_width) * camera.image_height;
+    const int64_t wanted = options_.paths_per_step > 0 ? (options_.paths_per_step + pixels - 1) / pixels : 0;
+    return static_cast<int>(std::clamp<int64_t>(wanted, options_.passes_per_step, max_step_passes));
 }
 
 double server::now() const
@@ -94,6 +112,7 @@ int64_t server::submit_view(int64_t scene_id, const rt_camera_desc& camera, uint
     r->camera = camera;
     r->seed = seed;
     r->passes = passes;
+    r->step_passes = step_passes(camera);
     // what the resident scenes hold stays held while requests may use them: a request that doesn't fit beside them
     // could never be admitted
     std::size_t resident = 0;
@@ -101,7 +120,7 @@ int64_t server::submit_view(int64_t scene_id, const rt_camera_desc& camera, uint
     {
         resident += sc.bytes;
     }
-    if (view_device_bytes(*f->second.scene, camera, options_.passes_per_step) + resident > options_.memory_budget_bytes)
+    if (view_device_bytes(*f->second.scene, camera, r->step_passes) + resident > options_.memory_budget_bytes)
     {
         throw std::invalid_argument("the request needs more device memory than the server's budget leaves beside "
                                     "the resident scenes");
@@ -146,7 +165,7 @@ void server::admit()
     while (!queue_.empty() && active < options_.max_active)
     {
         request& r = *requests_.at(queue_.front());
-        const std::size_t bytes = view_device_bytes(*scenes_.at(r.scene_id).scene, r.camera, options_.passes_per_step);
+        const std::size_t bytes = view_device_bytes(*scenes_.at(r.scene_id).scene, r.camera, r.step_passes);
         if (used_bytes_ + bytes > options_.memory_budget_bytes)
         {
             break;
@@ -163,7 +182,7 @@ void server::admit()
             // a stream of the server's few, shared in turn; staging for one step's passes at most
             r.stream = next_stream_;
             next_stream_ = (next_stream_ + 1) % streams_.size();
-            r.work->serve_on(streams_[r.stream].get(), options_.passes_per_step);
+            r.work->serve_on(streams_[r.stream].get(), r.step_passes);
             r.bytes = bytes;
             used_bytes_ += bytes;
             r.st.state = 1;
@@ -201,7 +220,7 @@ int server::step()
 {
     admit();
     // Every active request without a step in flight gets its next one, enqueued on its stream without waiting: up to
-    // the next power-of-two milestone, at most passes_per_step passes. The steps of requests on different streams run
+    // the next power-of-two milestone, at most its step's passes. The steps of requests on different streams run
     // side by side on the device, each persistent grid on a share of the chip.
     std::vector<request*> active;
     for (auto& [id, rp] : requests_)
@@ -224,7 +243,7 @@ int server::step()
         {
             next_milestone *= 2;
         }
-        const int64_t target = std::min({r->passes, done + options_.passes_per_step, std::max(next_milestone, done + 1)});
+        const int64_t target = std::min({r->passes, done + r->step_passes, std::max(next_milestone, done + 1)});
         try
         {
             r->work->enqueue_to(target, std::max(1, ways));

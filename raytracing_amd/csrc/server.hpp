@@ -38,6 +38,7 @@ public:
     void release(int64_t id);
 
 private:
+    int step_passes(const rt_camera_desc& camera) const;
     struct resident
     {
         std::shared_ptr<const shared_scene> scene;
@@ -53,6 +54,7 @@ private:
         rt_camera_desc camera{};
         uint64_t seed = 0;
         int64_t passes = 0;
+        int step_passes = 0; // the passes of one of its steps, and its staging's
         std::size_t bytes = 0;
         std::unique_ptr<session> work;
         std::size_t stream = 0; // of the server's streams

@@ -29,7 +29,7 @@ def test_every_declared_symbol_is_exported(rt):
     lib = rt.library()
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.rt_abi_version() == 3
+    assert lib.rt_abi_version() == 4
 
 
 def test_scene_names(rt):
@@ -165,6 +165,11 @@ def test_errors_before_the_gpu(rt):
         rt.Session(s, launch_seconds=2.0)
     with pytest.raises(ValueError):
         rt.Session(s, block_size=100)
+    # a server's step sizes are checked before it makes its streams
+    with pytest.raises(ValueError, match="paths_per_step"):
+        rt.Server(paths_per_step=-1)
+    with pytest.raises(ValueError, match="passes_per_step"):
+        rt.Server(passes_per_step=1025)
 
 
 def test_convexity_of_media_boundaries(rt):

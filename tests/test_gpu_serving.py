@@ -137,11 +137,34 @@ def test_milestones_are_powers_of_two(rt):
     server.close()
 
 
-def test_benchmark_mix_same_alone_and_in_mix(rt):
+def test_steps_raised_by_paths(rt):
+    """A small view's steps are raised to paths_per_step paths (ceil(paths / pixels) passes), still stopping at every
+    power-of-two milestone"""
+    view = _view(rt, "glass", 64, 64)
+    server = rt.Server(passes_per_step=4, paths_per_step=64 * 64 * 16 - 100)  # ceil: 16 passes a step
+    rid = server.submit(view, 3, 100)
+    seen = []
+    while server.status(rid).state < 2:
+        server.step()
+        seen.append(server.status(rid).passes_done)
+    assert seen == [1, 2, 4, 8, 16, 32, 48, 64, 80, 96, 100]
+    server.close()
+    with rt.Session(view, seed=3) as s:
+        s.render_to(100)
+        alone = s.pixels()
+    server = rt.Server(paths_per_step=64 * 64 * 16)
+    rid = server.submit(view, 3, 100)
+    _run(server, [rid])
+    assert np.array_equal(alone.view(np.uint64), server.pixels(rid).view(np.uint64))
+    server.close()
+
+
+@pytest.mark.parametrize("paths_per_step", [0, 1 << 22])
+def test_benchmark_mix_same_alone_and_in_mix(rt, paths_per_step):
     """Requests drawn from the serving benchmark's own mix (its four scenes resident, each scene twice, each request
     one of the four resolutions 256x256 to 1920x1080, a log-uniform spp in [16, 1024] and its own seed; seeded, with
     the paths of one request capped at 2^28 so the test stays short): every result in the mix equals its alone-session
-    bytes"""
+    bytes, with steps of a fixed pass count and with steps raised to 4 M paths for small views"""
     scenes = ["weekend", "glass", "final", "triangles"]
     sizes = [(256, 256), (512, 512), (1080, 1080), (1920, 1080)]
     rng = np.random.default_rng(44)
@@ -152,7 +175,7 @@ def test_benchmark_mix_same_alone_and_in_mix(rt):
         spp = max(1, min(spp, (1 << 28) // (w * h)))
         picks.append((name, w, h, spp, 500 + k))
     print(picks)
-    server = rt.Server(max_active=4, passes_per_step=8)
+    server = rt.Server(max_active=4, passes_per_step=8, paths_per_step=paths_per_step)
     resident = {name: server.add_scene(rt.build_scene(name, width=256, height=256, samples=1)) for name in scenes}
     views = {(name, w, h): rt.build_scene(name, width=w, height=h, samples=1) for name, w, h, _, _ in picks}
     ids = [server.submit_view(resident[name], views[(name, w, h)].desc.camera, seed, spp)
