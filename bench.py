@@ -167,7 +167,7 @@ SERVING_SCENES = ["weekend", "glass", "final", "triangles"]
 SERVING_RESOLUTIONS = [(256, 256), (512, 512), (1080, 1080), (1920, 1080)]
 
 
-def serving(rt, rates, per_rate, seed, tau, paths_per_step):
+def serving(rt, rates, per_rate, seed, tau, paths_per_step, max_active=8):
     """Config 4: a seeded Poisson stream of requests at each rate; each picks a scene of configs 0-3, a resolution and
     a log-uniform spp in [16, 1024], and is rendered progressively (power-of-two milestones) by the server, with the
     four scenes resident. Reports p50/p99 of the time from submission to the first milestone whose estimated relative
@@ -193,7 +193,8 @@ def serving(rt, rates, per_rate, seed, tau, paths_per_step):
         var = 16 * np.mean((imgs[0] - imgs[1]) ** 2) / 2
         sigma[name] = float(np.sqrt(var) / max(np.mean(imgs[0] + imgs[1]) / 2, 1e-9))
         print(f"serving pilot {name}: relative sigma per pass {sigma[name]:.3f}", file=sys.stderr, flush=True)
-    server = rt.Server(memory_budget_bytes=64 << 30, max_active=8, passes_per_step=8, paths_per_step=paths_per_step)
+    server = rt.Server(memory_budget_bytes=64 << 30, max_active=max_active, passes_per_step=8,
+                       paths_per_step=paths_per_step)
     for name in SERVING_SCENES:
         scenes[name] = server.add_scene(rt.build_scene(name, width=256, height=256, samples=1))
     results = {}
@@ -320,6 +321,7 @@ def main():
     ap.add_argument("--rates", default="1,4,16,64")
     ap.add_argument("--requests", type=int, default=16, help="requests per rate (serving)")
     ap.add_argument("--tau", type=float, default=0.05, help="target relative RMSE (serving)")
+    ap.add_argument("--max-active", type=int, default=8, help="serving: requests rendering at once")
     ap.add_argument("--paths-per-step", type=int, default=1 << 24,
                     help="serving: a small view's steps raised to this many paths (0: 8 passes a step alone)")
     ap.add_argument("--rmse", default="", help="the RMSE protocol for a workload (final, weekend, glass, glass-mis, "
@@ -331,13 +333,14 @@ def main():
         import raytracing_amd as rt
 
         rows, sigma = serving(rt, [float(r) for r in args.rates.split(",")], args.requests, 4, args.tau,
-                              args.paths_per_step)
+                              args.paths_per_step, args.max_active)
         print(json.dumps({"metric": "serving p99 time to target noise (config 4)", "value": rows[-1]["p99_time_to_target_s"],
                           "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
                           "config": {"workload": f"Poisson streams, {args.requests} requests per rate, scenes "
                                                  f"{SERVING_SCENES}, resolutions {SERVING_RESOLUTIONS}, spp "
                                                  f"log-uniform [16, 1024], target relative RMSE {args.tau}",
                                      "passes_per_step": 8, "paths_per_step": args.paths_per_step,
+                                     "max_active": args.max_active,
                                      "sigma_per_pass": sigma},
                           "rates": rows}), flush=True)
         return
